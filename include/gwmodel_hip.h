@@ -1,0 +1,145 @@
+/*
+ * gwmodel_hip.h -- C ABI of the MI355X (gfx950) GWR back-end.
+ *
+ * This is the boundary the HIP back-end sits behind.  Each entry point
+ * replaces one of libgwmodel's CUDA back-end member functions, which the
+ * reference dispatches through member-function pointers selected by
+ * setParallelType (src/gwmodelpp/GWRBasic.cpp:904-975,
+ * src/gwmodelpp/GWRMultiscale.cpp:1396-1474).  Plain C: opaque handle,
+ * caller-owned column-major FP64 buffers (copied in), integer status codes,
+ * no exceptions across the ABI and no device reset on error (the reference's
+ * checkCudaErrors resets the device, include/gwmodelpp/utils/CudaUtils.h:53-66).
+ *
+ * Matrices are column-major FP64 with n rows, exactly arma::mat's memory.
+ * Kernel ids: BandwidthWeight::KernelFunctionType
+ *             (include/gwmodelpp/spatialweight/BandwidthWeight.h:46-53).
+ * Criterion ids: GWRBasic::BandwidthSelectionCriterionType (GWRBasic.h:41-45).
+ */
+#ifndef GWMODEL_HIP_H
+#define GWMODEL_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GWM_HIP_ABI_VERSION 1
+
+typedef struct gwm_hip_ctx gwm_hip_ctx;
+
+/* status codes */
+enum {
+    GWM_OK = 0,
+    GWM_ERR_NOT_SPD = 1,          /* a local design is not SPD (inv_sympd failure) */
+    GWM_ERR_CANCELLED = 2,        /* the stop callback asked to stop (Status::Terminated) */
+    GWM_ERR_HIP = 3,              /* HIP runtime error; see gwm_hip_last_error */
+    GWM_ERR_OOM = 4,              /* device allocation failed */
+    GWM_ERR_INVALID = 5,          /* invalid argument or state */
+    GWM_ERR_NO_DEVICE = 6,        /* no usable gfx950 device */
+    GWM_ERR_INVALID_INITIAL = 7   /* golden section: both initial values invalid */
+};
+
+/* kernels: BandwidthWeight::KernelFunctionType */
+enum { GWM_GAUSSIAN = 0, GWM_EXPONENTIAL = 1, GWM_BISQUARE = 2, GWM_TRICUBE = 3, GWM_BOXCAR = 4 };
+/* distances: CRSDistance(false) / CRSDistance(true) */
+enum { GWM_EUCLIDEAN = 0, GWM_GREAT_CIRCLE = 1 };
+/* criteria: GWRBasic::BandwidthSelectionCriterionType */
+enum { GWM_CRIT_AIC = 0, GWM_CRIT_CV = 1 };
+/* arithmetic of the sweep: FP64 everywhere, or FP32 distances/weights with an
+ * FP16 hi+lo split contraction on the matrix cores (FP32 accumulate, FP64
+ * partial sums, FP64 solves). */
+enum { GWM_FP64 = 0, GWM_MIXED = 1 };
+
+/* Number of visible devices (hipGetDeviceCount). */
+int gwm_hip_device_count(int* count);
+
+/* Context: one device, one stream, device-resident problem data.
+ * Replaces GWRBasic::fit's cublasCreate + SpatialWeight::prepareCuda(gpuId)
+ * (src/gwmodelpp/GWRBasic.cpp:41-47) and IParallelCudaEnabled::setGPUId. */
+int gwm_hip_create(int device, gwm_hip_ctx** out);
+void gwm_hip_destroy(gwm_hip_ctx* ctx);
+const char* gwm_hip_last_error(const gwm_hip_ctx* ctx);
+
+/* Cancellation: polled between sweeps (ITelegram::stop, include/gwmodelpp/Logger.h:76-81). */
+int gwm_hip_set_stop_callback(gwm_hip_ctx* ctx, int (*stop)(void* user), void* user);
+
+/* Problem data (copied to the device).  coords n x 2 (x,y or lon,lat in
+ * degrees); X n x k; y n.  Replaces the H2D uploads of
+ * CRSDistance::prepareCuda (src/gwmodelpp/spatialweight/CRSDistance.cpp:158-168)
+ * and of fitCuda / bandwidthSizeCriterion*Cuda (GWRBasic.cpp:585-600, 709-720). */
+int gwm_hip_set_data(gwm_hip_ctx* ctx, int n, int k, const double* coords, const double* X, const double* y,
+                     int distance);
+
+/* Criterion at one bandwidth: replaces GWRBasic::bandwidthSizeCriterionCVCuda /
+ * bandwidthSizeCriterionAICCuda (src/gwmodelpp/GWRBasic.cpp:709-836) and their
+ * Serial/OpenMP twins (:233-307, :440-535).  *value = DBL_MAX with status
+ * GWM_ERR_NOT_SPD when a local design is not SPD or the value is not finite. */
+int gwm_hip_criterion(gwm_hip_ctx* ctx, int kernel, int adaptive, double bw, int criterion, int precision,
+                      double* value);
+
+/* Fit at one bandwidth: replaces GWRBasic::fitCuda (src/gwmodelpp/GWRBasic.cpp:585-663)
+ * / fitOmp (:386-438).  Outputs (nullable): betas n x k, betasSE_raw n x k (row
+ * sums of C_i.^2 before the sigma scaling of fit()), shat[2] = {tr S, tr S^T S},
+ * qDiag n, S n x n (only when S != NULL; the reference stores S when n < 8192,
+ * GWRBasic.h:708).  Returns GWM_ERR_NOT_SPD like fitOmp's rethrow. */
+int gwm_hip_fit(gwm_hip_ctx* ctx, int kernel, int adaptive, double bw, int precision, double* betas,
+                double* betasSE_raw, double* shat, double* qDiag, double* S);
+
+/* Golden-section bandwidth selection on device-resident data: replaces
+ * BandwidthSelector::optimize (src/gwmodelpp/BandwidthSelector.cpp:6-81)
+ * driving getCriterion.  Bounds follow GWRBasic::fit (GWRBasic.cpp:76-79):
+ * lower = has_lower ? lower : (adaptive ? 20 : 0), upper = has_upper ? upper :
+ * (adaptive ? n : maxDistance).  trace_* receive the criterion list sorted by
+ * bandwidth (BandwidthSelector::bandwidthCriterion, :83-95). */
+int gwm_hip_select_bandwidth(gwm_hip_ctx* ctx, int kernel, int adaptive, int criterion, int precision, int has_lower,
+                             double lower, int has_upper, double upper, double* bw_out, double* trace_bw,
+                             double* trace_value, int trace_cap, int* trace_len);
+
+/* GWRBasic::fit() end to end (src/gwmodelpp/GWRBasic.cpp:36-103): optional
+ * bandwidth selection, fit, CalcDiagnostic (:21-34) and the SE scaling.
+ * diag[7] = RSS, AIC, AICc, ENP, EDF, R2, R2adj (RegressionDiagnostic order). */
+int gwm_hip_gwr_run(gwm_hip_ctx* ctx, int kernel, int adaptive, double bw, int select_bw, int criterion,
+                    int precision, int has_lower, double lower, int has_upper, double upper, double* betas,
+                    double* betasSE, double* shat, double* qDiag, double* S, double* diag, double* bw_out,
+                    double* trace_bw, double* trace_value, int trace_cap, int* trace_len);
+
+/* CRSDistance::maxDistance (src/gwmodelpp/spatialweight/CRSDistance.cpp:133-143). */
+int gwm_hip_max_distance(gwm_hip_ctx* ctx, double* out);
+/* CRSDistance::distance(focus) (CRSDistance.cpp:122-131): n distances. */
+int gwm_hip_distance_row(gwm_hip_ctx* ctx, int focus, double* out);
+/* Adaptive bandwidth resolved per focus point (BandwidthWeight.cpp:40-55): n values. */
+int gwm_hip_adaptive_bandwidths(gwm_hip_ctx* ctx, double bw, double* out);
+
+/* Multiscale single-column kernels on the context's coordinates:
+ * bandwidthSizeCriterionVar{CV,AIC}Cuda (src/gwmodelpp/GWRMultiscale.cpp:1246-1392)
+ * and fitVarCuda (:1062-1140).  x, y: n-vectors (host). */
+int gwm_hip_var_criterion(gwm_hip_ctx* ctx, const double* x, const double* y, int kernel, int adaptive, double bw,
+                          int criterion, int precision, double* value);
+int gwm_hip_var_fit(gwm_hip_ctx* ctx, const double* x, const double* y, int kernel, int adaptive, double bw,
+                    int precision, double* beta);
+
+/* GWRMultiscale::fit() end to end (src/gwmodelpp/GWRMultiscale.cpp:68-197, backfitting
+ * :208-317) without the hat matrix (setHasHatMatrix(false)).  Per-variable arrays
+ * have k entries; init_type 0 Null / 1 Initial / 2 Specified; approach uses the
+ * GWM_CRIT_* ids; crit_type 0 CVR / 1 dCVR.  Outputs: betas n x k, bws_out k,
+ * diag[7] (AIC and the hat-based terms follow the reference with tr S = 0),
+ * rss_out, iterations_out. */
+int gwm_hip_mgwr_run(gwm_hip_ctx* ctx, const int* kernels, const int* adaptive, const double* bws,
+                     const int* init_type, const int* approach, const int* centered, const double* bw_threshold,
+                     int has_intercept, int crit_type, double crit_threshold, int max_iter, int retry_times,
+                     int adaptive_lower, int has_lower, double lower, int has_upper, double upper, int precision,
+                     double* betas, double* bws_out, double* diag, double* rss_out, int* iterations_out);
+
+/* Timing of the contraction kernels with HIP events on the context stream
+ * (enabled by gwm_hip_set_timing).  name: "contract" (all sweeps),
+ * "contract_fp64", "contract_mixed", "select", "solve". */
+int gwm_hip_set_timing(gwm_hip_ctx* ctx, int enabled);
+int gwm_hip_kernel_time(gwm_hip_ctx* ctx, const char* name, long long* launches, double* total_ms);
+int gwm_hip_reset_timers(gwm_hip_ctx* ctx);
+/* Local solves (focus x bandwidth pairs) evaluated since the last reset. */
+int gwm_hip_solve_count(gwm_hip_ctx* ctx, long long* solves);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GWMODEL_HIP_H */

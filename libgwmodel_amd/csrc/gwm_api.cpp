@@ -1,0 +1,893 @@
+// Host side of the C ABI declared in include/gwmodel_hip.h.
+//
+// One context = one device, one HIP stream and the device-resident problem
+// (coordinates, design, product table Z, workspaces).  Everything n x n stays
+// implicit: sweeps stream Z through the fused contraction kernel, so device
+// memory is O(n * k^2) per problem.  All HIP failures are reported as status
+// codes; the device is never reset.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/gwmodel_hip.h"
+#include "gwm_contract.h"
+
+using namespace gwm;
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+
+int nta_of(int k) { return (k * (k + 1) / 2 + k + 15) / 16; }
+int ntb_of(int k) { return (k * (k + 1) / 2 + 15) / 16; }
+
+struct KernelStat {
+    long long launches = 0;
+    double ms = 0.0;
+};
+
+}  // namespace
+
+struct gwm_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int (*stop)(void*) = nullptr;
+    void* stop_user = nullptr;
+
+    // problem
+    int n = 0, k = 0, dist = 0;
+    int nta = 0, ntb = 0, m = 0;
+    int n_pad = 0, rows_pad = 0;
+    std::vector<double> hcoords, hX, hy;
+
+    // device buffers
+    double *px = nullptr, *py = nullptr, *hx = nullptr, *hy_ = nullptr, *hz = nullptr, *sl = nullptr;
+    double *X = nullptr, *y = nullptr, *Z = nullptr, *G = nullptr;
+    double* part = nullptr;
+    size_t part_bytes = 0;
+    double *bwi = nullptr, *res2 = nullptr, *sii = nullptr, *ssi = nullptr;
+    double *betas = nullptr, *se = nullptr, *gvec = nullptr, *qd = nullptr, *red = nullptr, *blockmax = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vZ = nullptr;
+    double* S = nullptr;
+    int* fail = nullptr;
+
+    // timing
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::map<std::string, KernelStat> stats;
+    long long solves = 0;
+
+    Points points() const {
+        Points P;
+        P.px = px;
+        P.py = py;
+        P.hx = hx;
+        P.hy = hy_;
+        P.hz = hz;
+        P.sl = sl;
+        P.n = n;
+        return P;
+    }
+};
+
+namespace {
+
+#define GWM_HIP_CHECK(ctx, expr)                                                         \
+    do {                                                                                 \
+        hipError_t e__ = (expr);                                                         \
+        if (e__ != hipSuccess) {                                                         \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e__);             \
+            return e__ == hipErrorOutOfMemory ? GWM_ERR_OOM : GWM_ERR_HIP;               \
+        }                                                                                \
+    } while (0)
+
+template <class T>
+void dfree(T*& p) {
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+void free_problem(gwm_hip_ctx* c) {
+    dfree(c->px); dfree(c->py); dfree(c->hx); dfree(c->hy_); dfree(c->hz); dfree(c->sl);
+    dfree(c->X); dfree(c->y); dfree(c->Z); dfree(c->G); dfree(c->part);
+    dfree(c->bwi); dfree(c->res2); dfree(c->sii); dfree(c->ssi); dfree(c->betas); dfree(c->se);
+    dfree(c->gvec); dfree(c->qd); dfree(c->red); dfree(c->blockmax); dfree(c->vx); dfree(c->vy);
+    dfree(c->vZ); dfree(c->S); dfree(c->fail);
+    c->part_bytes = 0;
+    c->n = 0;
+}
+
+template <class T>
+int dalloc(gwm_hip_ctx* c, T** p, size_t count) {
+    GWM_HIP_CHECK(c, hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+    return GWM_OK;
+}
+
+// Segmentation of the data range: enough workgroups to fill 256 CUs several
+// times over, segment count a multiple of 8 (one slice of Z per XCD).
+void plan_segments(int n, int tile_focus, int& nseg, int& seg_len) {
+    int tiles = (n + tile_focus - 1) / tile_focus;
+    int want = (2048 + tiles - 1) / tiles;
+    want = std::max(1, want);
+    if (want > 1) want = ((want + 7) / 8) * 8;
+    int max_seg = (n + 63) / 64;
+    want = std::min(want, max_seg);
+    seg_len = ((n + want - 1) / want + 63) / 64 * 64;
+    nseg = (n + seg_len - 1) / seg_len;
+}
+
+int ensure_part(gwm_hip_ctx* c, size_t bytes) {
+    if (bytes <= c->part_bytes) return GWM_OK;
+    dfree(c->part);
+    c->part_bytes = 0;
+    GWM_HIP_CHECK(c, hipMalloc((void**)&c->part, bytes));
+    c->part_bytes = bytes;
+    return GWM_OK;
+}
+
+bool stopped(gwm_hip_ctx* c) { return c->stop && c->stop(c->stop_user); }
+
+void timer_begin(gwm_hip_ctx* c) {
+    if (c->timing) hipEventRecord(c->ev0, c->stream);
+}
+int timer_end(gwm_hip_ctx* c, const char* name) {
+    if (!c->timing) return GWM_OK;
+    GWM_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
+    GWM_HIP_CHECK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    GWM_HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    auto& s = c->stats[name];
+    s.launches += 1;
+    s.ms += ms;
+    return GWM_OK;
+}
+
+// Resolve the bandwidth: adaptive -> per-focus thresholds in c->bwi.
+int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw_pt) {
+    *bw_pt = nullptr;
+    if (!adaptive) return GWM_OK;
+    if (!(bw >= 1.0)) {
+        c->err = "adaptive bandwidth must be >= 1 neighbour";
+        return GWM_ERR_INVALID;
+    }
+    timer_begin(c);
+    GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, c->bwi, c->stream));
+    int rc = timer_end(c, "select");
+    if (rc) return rc;
+    *bw_pt = c->bwi;
+    return GWM_OK;
+}
+
+// One sweep of the fused contraction over the context's design.
+int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const double* Z, const double* bw_pt, double bw, int kernel,
+          int precision, int* PT_out, int* nseg_out) {
+    int tile_focus = contract_tile_focus(nta, ntb);
+    int nseg, seg_len;
+    plan_segments(c->n, tile_focus, nseg, seg_len);
+    const int PT = (nta + ntb) * 16;
+    int rc = ensure_part(c, (size_t)nseg * PT * c->n_pad * sizeof(double));
+    if (rc) return rc;
+    if (precision != GWM_FP64) {
+        c->err = "mixed precision is not available in this build";
+        return GWM_ERR_INVALID;
+    }
+    ContractArgs a;
+    a.P = c->points();
+    a.Z = Z;
+    a.bw_pt = bw_pt;
+    a.bw = bw;
+    a.kernel = kernel;
+    a.n = c->n;
+    a.n_pad = c->n_pad;
+    a.seg_len = seg_len;
+    a.nseg = nseg;
+    a.out = c->part;
+    timer_begin(c);
+    GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, a, c->stream));
+    rc = timer_end(c, "contract_fp64");
+    if (rc) return rc;
+    if (c->timing) {
+        auto& s = c->stats["contract"];
+        s.launches += 1;
+        s.ms += 0.0;   // the per-precision entry carries the time
+    }
+    *PT_out = PT;
+    *nseg_out = nseg;
+    return GWM_OK;
+}
+
+SolveArgs solve_args(gwm_hip_ctx* c, int PT, int nseg, int colB, const double* bw_pt, double bw, int kernel) {
+    SolveArgs s;
+    s.part = c->part;
+    s.nseg = nseg;
+    s.PT = PT;
+    s.colB = colB;
+    s.n = c->n;
+    s.n_pad = c->n_pad;
+    s.X = c->X;
+    s.y = c->y;
+    s.bw_pt = bw_pt;
+    s.bw = bw;
+    s.kernel = kernel;
+    s.res2 = c->res2;
+    s.sii = c->sii;
+    s.ssi = c->ssi;
+    s.betas = c->betas;
+    s.se_raw = c->se;
+    s.G = c->G;
+    s.ldg = c->ntb * 16;
+    s.gvec = c->gvec;
+    s.fail = c->fail;
+    return s;
+}
+
+double aicc(double rss, double n, double trS) {
+    return n * std::log(rss / n) + n * std::log(2 * kPi) + n * ((n + trS) / (n - 2 - trS));
+}
+
+int check_ready(gwm_hip_ctx* c) {
+    if (!c) return GWM_ERR_INVALID;
+    if (c->n <= 0) {
+        c->err = "no data: call gwm_hip_set_data first";
+        return GWM_ERR_INVALID;
+    }
+    return GWM_OK;
+}
+
+int check_kernel(gwm_hip_ctx* c, int kernel, int precision) {
+    if (kernel < 0 || kernel > 4) {
+        c->err = "unknown kernel";
+        return GWM_ERR_INVALID;
+    }
+    if (precision != GWM_FP64 && precision != GWM_MIXED) {
+        c->err = "unknown precision";
+        return GWM_ERR_INVALID;
+    }
+    return GWM_OK;
+}
+
+// Criterion of the full design: CV (GWRBasic.cpp:440-485) or AICc (:487-535).
+int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit, int precision, double* value) {
+    *value = DBL_MAX;
+    int rc = check_kernel(c, kernel, precision);
+    if (rc) return rc;
+    if (stopped(c)) return GWM_ERR_CANCELLED;
+    const double* bw_pt;
+    if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
+    GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
+    int PT, nseg;
+    const int mode = crit == GWM_CRIT_CV ? kModeCV : kModePlain;
+    if ((rc = sweep(c, mode, c->nta, 0, c->Z, bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
+    SolveArgs s = solve_args(c, PT, nseg, 0, bw_pt, bw, kernel);
+    timer_begin(c);
+    GWM_HIP_CHECK(c, launch_solve(c->k, crit == GWM_CRIT_CV ? kSolveCV : kSolveAIC, s, c->stream));
+    if ((rc = timer_end(c, "solve"))) return rc;
+    GWM_HIP_CHECK(c, launch_reduce(c->res2, crit == GWM_CRIT_CV ? nullptr : c->sii, nullptr, c->n, c->red, c->stream));
+    double red[3];
+    int fail = 0;
+    GWM_HIP_CHECK(c, hipMemcpyAsync(red, c->red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->solves += c->n;
+    if (fail) return GWM_ERR_NOT_SPD;
+    double v = crit == GWM_CRIT_CV ? red[0] : aicc(red[0], (double)c->n, red[1]);
+    if (!std::isfinite(v)) return GWM_ERR_NOT_SPD;
+    *value = v;
+    return GWM_OK;
+}
+
+// Golden-section search, BandwidthSelector::optimize (BandwidthSelector.cpp:6-81).
+// f returns a status and the criterion (DBL_MAX when invalid).
+int golden(bool adaptBw, double lower, double upper, const std::function<int(double, double*)>& f, double* xopt_out,
+           std::map<double, double>& log) {
+    double xU = upper, xL = lower;
+    const double eps = 1e-4;
+    const double R = (std::sqrt(5.0) - 1) / 2;
+    int iter = 0;
+    double d = R * (xU - xL);
+    double x1 = adaptBw ? std::floor(xL + d) : (xL + d);
+    double x2 = adaptBw ? std::round(xU - d) : (xU - d);
+    double f1 = DBL_MAX, f2 = DBL_MAX;
+    auto eval = [&](double x, double* fx) -> int {
+        int rc = f(x, fx);
+        if (rc == GWM_ERR_NOT_SPD) {
+            *fx = DBL_MAX;
+            return GWM_OK;
+        }
+        return rc;
+    };
+    int rc;
+    if ((rc = eval(x1, &f1))) return rc;
+    if ((rc = eval(x2, &f2))) return rc;
+    if (f1 == DBL_MAX && f2 == DBL_MAX) return GWM_ERR_INVALID_INITIAL;
+    if (f1 < DBL_MAX) log[x1] = f1;
+    if (f2 < DBL_MAX) log[x2] = f2;
+    double d1 = f2 - f1;
+    double xopt = f1 < f2 ? x1 : x2;
+    const double ea = 100;
+    while ((std::fabs(d) > eps) && (std::fabs(d1) > eps) && iter < ea) {
+        d = R * d;
+        if (f1 < f2) {
+            xL = x2;
+            x2 = x1;
+            x1 = adaptBw ? std::round(xL + d) : (xL + d);
+            f2 = f1;
+            if ((rc = eval(x1, &f1))) return rc;
+            if (f1 < DBL_MAX) log[x1] = f1;
+        } else {
+            xU = x1;
+            x1 = x2;
+            x2 = adaptBw ? std::floor(xU - d) : (xU - d);
+            f1 = f2;
+            if ((rc = eval(x2, &f2))) return rc;
+            if (f2 < DBL_MAX) log[x2] = f2;
+        }
+        iter = iter + 1;
+        xopt = (f1 < f2) ? x1 : x2;
+        d1 = f2 - f1;
+    }
+    *xopt_out = xopt;
+    return GWM_OK;
+}
+
+void write_trace(const std::map<double, double>& log, double* tx, double* tf, int cap, int* len) {
+    int m = 0;
+    for (const auto& kv : log) {
+        if (m < cap) {
+            if (tx) tx[m] = kv.first;
+            if (tf) tf[m] = kv.second;
+        }
+        ++m;
+    }
+    if (len) *len = m;
+}
+
+int max_distance_impl(gwm_hip_ctx* c, double* out) {
+    int nblocks = (c->n + 255) / 256;
+    GWM_HIP_CHECK(c, launch_max_distance(c->dist, c->points(), c->blockmax, nblocks, c->stream));
+    std::vector<double> bm(nblocks);
+    GWM_HIP_CHECK(c, hipMemcpyAsync(bm.data(), c->blockmax, nblocks * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    double m = 0.0;
+    for (double v : bm) m = v > m ? v : m;
+    *out = m;
+    return GWM_OK;
+}
+
+// Upload the design and its product table Z = [x_a x_b (a <= b) | x_a y].
+int upload_design(gwm_hip_ctx* c, const double* X, const double* y) {
+    const int n = c->n, k = c->k, ldz = c->nta * 16;
+    std::vector<double> Z((size_t)c->rows_pad * ldz, 0.0);
+    for (int j = 0; j < n; ++j) {
+        double* z = Z.data() + (size_t)j * ldz;
+        int p = 0;
+        for (int a = 0; a < k; ++a)
+            for (int b = a; b < k; ++b) z[p++] = X[j + (size_t)a * n] * X[j + (size_t)b * n];
+        for (int a = 0; a < k; ++a) z[p++] = X[j + (size_t)a * n] * y[j];
+    }
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->X, X, (size_t)n * k * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->y, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->Z, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GWM_OK;
+}
+
+// Single-column criterion / fit (GWRMultiscale.cpp:420-482, 560-636).
+int var_impl(gwm_hip_ctx* c, const double* x, const double* y, int kernel, int adaptive, double bw, int crit,
+             int precision, double* value, double* beta) {
+    int rc = check_kernel(c, kernel, precision);
+    if (rc) return rc;
+    if (value) *value = DBL_MAX;
+    if (stopped(c)) return GWM_ERR_CANCELLED;
+    const int n = c->n;
+    std::vector<double> Z((size_t)c->rows_pad * 16, 0.0);
+    for (int j = 0; j < n; ++j) {
+        Z[(size_t)j * 16 + 0] = x[j] * x[j];
+        Z[(size_t)j * 16 + 1] = x[j] * y[j];
+    }
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->vZ, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->vx, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->vy, y, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const double* bw_pt;
+    if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
+    GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
+    int PT, nseg;
+    const bool cv = value && crit == GWM_CRIT_CV;
+    if ((rc = sweep(c, cv ? kModeCV : kModePlain, 1, 0, c->vZ, bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
+    const int smode = beta ? kSolveBeta : (cv ? kSolveCV : kSolveAIC);
+    GWM_HIP_CHECK(c, launch_solve_var(smode, c->part, nseg, PT, 0, n, c->n_pad, c->vx, c->vy, bw_pt, bw, kernel,
+                                      c->res2, c->sii, beta ? c->betas : nullptr, c->fail, c->stream));
+    int fail = 0;
+    if (value) {
+        GWM_HIP_CHECK(c, launch_reduce(c->res2, cv ? nullptr : c->sii, nullptr, n, c->red, c->stream));
+        double red[3];
+        GWM_HIP_CHECK(c, hipMemcpyAsync(red, c->red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
+        GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        c->solves += n;
+        if (fail) return GWM_ERR_NOT_SPD;
+        double v = cv ? red[0] : aicc(red[0], (double)n, red[1]);
+        if (!std::isfinite(v)) return GWM_ERR_NOT_SPD;
+        *value = v;
+        return GWM_OK;
+    }
+    GWM_HIP_CHECK(c, hipMemcpyAsync(beta, c->betas, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->solves += n;
+    return fail ? GWM_ERR_NOT_SPD : GWM_OK;
+}
+
+// Full-design fit returning betas only (fitAll without the hat matrix,
+// GWRMultiscale.cpp:394-417).
+int fit_betas_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int precision, double* betas) {
+    int rc;
+    const double* bw_pt;
+    if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
+    GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
+    int PT, nseg;
+    if ((rc = sweep(c, kModePlain, c->nta, 0, c->Z, bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
+    SolveArgs s = solve_args(c, PT, nseg, 0, bw_pt, bw, kernel);
+    GWM_HIP_CHECK(c, launch_solve(c->k, kSolveBeta, s, c->stream));
+    int fail = 0;
+    GWM_HIP_CHECK(c, hipMemcpyAsync(betas, c->betas, (size_t)c->n * c->k * sizeof(double), hipMemcpyDeviceToHost,
+                                    c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->solves += c->n;
+    return fail ? GWM_ERR_NOT_SPD : GWM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gwm_hip_device_count(int* count) {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (count) *count = e == hipSuccess ? c : 0;
+    return e == hipSuccess ? GWM_OK : GWM_ERR_NO_DEVICE;
+}
+
+int gwm_hip_create(int device, gwm_hip_ctx** out) {
+    if (!out) return GWM_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count)
+        return GWM_ERR_NO_DEVICE;
+    gwm_hip_ctx* c = new gwm_hip_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return GWM_ERR_HIP;
+    }
+    *out = c;
+    return GWM_OK;
+}
+
+void gwm_hip_destroy(gwm_hip_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_problem(c);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* gwm_hip_last_error(const gwm_hip_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gwm_hip_set_stop_callback(gwm_hip_ctx* c, int (*stop)(void*), void* user) {
+    if (!c) return GWM_ERR_INVALID;
+    c->stop = stop;
+    c->stop_user = user;
+    return GWM_OK;
+}
+
+int gwm_hip_set_data(gwm_hip_ctx* c, int n, int k, const double* coords, const double* X, const double* y,
+                     int distance) {
+    if (!c) return GWM_ERR_INVALID;
+    if (n < 2 || k < 1 || k > 16 || !coords || !X || !y || (distance != GWM_EUCLIDEAN && distance != GWM_GREAT_CIRCLE)) {
+        c->err = "invalid data: need n >= 2, 1 <= k <= 16, coords/X/y, a known distance";
+        return GWM_ERR_INVALID;
+    }
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    free_problem(c);
+    c->n = n;
+    c->k = k;
+    c->dist = distance;
+    c->nta = nta_of(k);
+    c->ntb = ntb_of(k);
+    c->m = k * (k + 1) / 2;
+    c->n_pad = (n + 127) / 128 * 128;
+    c->rows_pad = (n + 63) / 64 * 64;
+    c->hcoords.assign(coords, coords + 2 * (size_t)n);
+    c->hX.assign(X, X + (size_t)n * k);
+    c->hy.assign(y, y + n);
+    int rc;
+    const size_t N = n;
+    if ((rc = dalloc(c, &c->px, N)) || (rc = dalloc(c, &c->py, N)) || (rc = dalloc(c, &c->X, N * k)) ||
+        (rc = dalloc(c, &c->y, N)) || (rc = dalloc(c, &c->Z, (size_t)c->rows_pad * c->nta * 16)) ||
+        (rc = dalloc(c, &c->G, (size_t)c->rows_pad * c->ntb * 16)) || (rc = dalloc(c, &c->bwi, N)) ||
+        (rc = dalloc(c, &c->res2, N)) || (rc = dalloc(c, &c->sii, N)) || (rc = dalloc(c, &c->ssi, N)) ||
+        (rc = dalloc(c, &c->betas, N * k)) || (rc = dalloc(c, &c->se, N * k)) || (rc = dalloc(c, &c->gvec, N * k)) ||
+        (rc = dalloc(c, &c->qd, N)) || (rc = dalloc(c, &c->red, 4)) || (rc = dalloc(c, &c->blockmax, (N + 255) / 256)) ||
+        (rc = dalloc(c, &c->vx, N)) || (rc = dalloc(c, &c->vy, N)) || (rc = dalloc(c, &c->vZ, (size_t)c->rows_pad * 16)) ||
+        (rc = dalloc(c, &c->fail, 1))) {
+        free_problem(c);
+        return rc;
+    }
+    GWM_HIP_CHECK(c, hipMemsetAsync(c->G, 0, (size_t)c->rows_pad * c->ntb * 16 * sizeof(double), c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->px, coords, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->py, coords + N, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (distance == GWM_GREAT_CIRCLE) {
+        if ((rc = dalloc(c, &c->hx, N)) || (rc = dalloc(c, &c->hy_, N)) || (rc = dalloc(c, &c->hz, N)) ||
+            (rc = dalloc(c, &c->sl, N))) {
+            free_problem(c);
+            return rc;
+        }
+        std::vector<double> hx(N), hy(N), hz(N), sl(N);
+        const double de2ra = kPi / 180;
+        for (size_t j = 0; j < N; ++j) {
+            double lon = coords[j] * de2ra, lat = coords[N + j] * de2ra;
+            hx[j] = 0.5 * std::cos(lat) * std::cos(lon);
+            hy[j] = 0.5 * std::cos(lat) * std::sin(lon);
+            hz[j] = 0.5 * std::sin(lat);
+            sl[j] = std::sin(lat);
+        }
+        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hx, hx.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hy_, hy.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hz, hz.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        GWM_HIP_CHECK(c, hipMemcpyAsync(c->sl, sl.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    }
+    return upload_design(c, X, y);
+}
+
+int gwm_hip_criterion(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit, int precision, double* value) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!value) return GWM_ERR_INVALID;
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    return criterion_impl(c, kernel, adaptive, bw, crit, precision, value);
+}
+
+int gwm_hip_fit(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int precision, double* betas, double* se_raw,
+                double* shat, double* qDiag, double* S) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = check_kernel(c, kernel, precision))) return rc;
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    if (stopped(c)) return GWM_ERR_CANCELLED;
+    const double* bw_pt;
+    if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
+    GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
+    int PT, nseg;
+    if ((rc = sweep(c, kModeHat, c->nta, c->ntb, c->Z, bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
+    SolveArgs s = solve_args(c, PT, nseg, c->nta * 16, bw_pt, bw, kernel);
+    timer_begin(c);
+    GWM_HIP_CHECK(c, launch_solve(c->k, kSolveFit, s, c->stream));
+    if ((rc = timer_end(c, "solve"))) return rc;
+    int fail = 0;
+    GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->solves += c->n;
+    if (fail) {
+        c->err = "inv_sympd(): matrix is singular or not positive definite";
+        return GWM_ERR_NOT_SPD;
+    }
+    if (qDiag) {
+        // second sweep: M_j = sum_i w_ij^2 vech(g_i g_i^T), bandwidth of i
+        if ((rc = sweep(c, kModeQDiag, c->ntb, 0, c->G, bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
+        GWM_HIP_CHECK(c, launch_qdiag(c->k, c->part, nseg, PT, c->n, c->n_pad, c->X, c->sii, c->qd, c->stream));
+        GWM_HIP_CHECK(c, hipMemcpyAsync(qDiag, c->qd, c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    if (S) {
+        if (c->n > 65535) {
+            c->err = "the S matrix is only formed for n <= 65535";
+            return GWM_ERR_INVALID;
+        }
+        if (!c->S) GWM_HIP_CHECK(c, hipMalloc((void**)&c->S, (size_t)c->n * c->n * sizeof(double)));
+        GWM_HIP_CHECK(c, launch_smatrix(c->dist, c->k, c->points(), c->X, c->gvec, bw_pt, bw, kernel, c->S, c->stream));
+        GWM_HIP_CHECK(c, hipMemcpyAsync(S, c->S, (size_t)c->n * c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    GWM_HIP_CHECK(c, launch_reduce(c->sii, c->ssi, c->res2, c->n, c->red, c->stream));
+    double red[3];
+    GWM_HIP_CHECK(c, hipMemcpyAsync(red, c->red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
+    const size_t nk = (size_t)c->n * c->k;
+    if (betas) GWM_HIP_CHECK(c, hipMemcpyAsync(betas, c->betas, nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (se_raw) GWM_HIP_CHECK(c, hipMemcpyAsync(se_raw, c->se, nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (shat) {
+        shat[0] = red[0];
+        shat[1] = red[1];
+    }
+    return GWM_OK;
+}
+
+int gwm_hip_select_bandwidth(gwm_hip_ctx* c, int kernel, int adaptive, int crit, int precision, int has_lower,
+                             double lower, int has_upper, double upper, double* bw_out, double* tx, double* tf,
+                             int cap, int* len) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = check_kernel(c, kernel, precision))) return rc;
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    if (len) *len = 0;
+    double lo = has_lower ? lower : (adaptive ? 20 : 0.0);
+    double hi;
+    if (has_upper) hi = upper;
+    else if (adaptive) hi = (double)c->n;
+    else if ((rc = max_distance_impl(c, &hi))) return rc;
+    std::map<double, double> log;
+    double xopt = 0.0;
+    rc = golden(adaptive != 0, lo, hi,
+                [&](double x, double* fx) { return criterion_impl(c, kernel, adaptive, x, crit, precision, fx); },
+                &xopt, log);
+    write_trace(log, tx, tf, cap, len);
+    if (rc) return rc;
+    if (bw_out) *bw_out = xopt;
+    return GWM_OK;
+}
+
+int gwm_hip_gwr_run(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int select_bw, int crit, int precision,
+                    int has_lower, double lower, int has_upper, double upper, double* betas, double* betasSE,
+                    double* shat, double* qDiag, double* S, double* diag, double* bw_out, double* tx, double* tf,
+                    int cap, int* len) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!betas || !betasSE || !shat || !diag) return GWM_ERR_INVALID;
+    if (len) *len = 0;
+    if (select_bw) {
+        if ((rc = gwm_hip_select_bandwidth(c, kernel, adaptive, crit, precision, has_lower, lower, has_upper, upper,
+                                           &bw, tx, tf, cap, len)))
+            return rc;
+    }
+    if (bw_out) *bw_out = bw;
+    if ((rc = gwm_hip_fit(c, kernel, adaptive, bw, precision, betas, betasSE, shat, qDiag, S))) return rc;
+    // CalcDiagnostic (GWRBasic.cpp:21-34) and SE scaling (:100-103)
+    const int n_ = c->n, k = c->k;
+    const double* X = c->hX.data();
+    const double* y = c->hy.data();
+    double rss = 0.0;
+    for (int i = 0; i < n_; ++i) {
+        double f = 0.0;
+        for (int a = 0; a < k; ++a) f += betas[i + (size_t)a * n_] * X[i + (size_t)a * n_];
+        double r = y[i] - f;
+        rss += r * r;
+    }
+    const double n = (double)n_;
+    double ym = 0.0;
+    for (int i = 0; i < n_; ++i) ym += y[i];
+    ym /= n;
+    double yss = 0.0;
+    for (int i = 0; i < n_; ++i) yss += (y[i] - ym) * (y[i] - ym);
+    double AIC = n * std::log(rss / n) + n * std::log(2 * kPi) + n + shat[0];
+    double AICc = aicc(rss, n, shat[0]);
+    double edf = n - 2 * shat[0] + shat[1];
+    double enp = 2 * shat[0] - shat[1];
+    double r2 = 1 - rss / yss;
+    double r2adj = 1 - (1 - r2) * (n - 1) / (edf - 1);
+    diag[0] = rss; diag[1] = AIC; diag[2] = AICc; diag[3] = enp; diag[4] = edf; diag[5] = r2; diag[6] = r2adj;
+    double sigma = rss / (n - 2 * shat[0] + shat[1]);
+    for (size_t i = 0; i < (size_t)n_ * k; ++i) betasSE[i] = std::sqrt(sigma * betasSE[i]);
+    return GWM_OK;
+}
+
+int gwm_hip_max_distance(gwm_hip_ctx* c, double* out) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!out) return GWM_ERR_INVALID;
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    return max_distance_impl(c, out);
+}
+
+int gwm_hip_distance_row(gwm_hip_ctx* c, int focus, double* out) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!out || focus < 0 || focus >= c->n) return GWM_ERR_INVALID;
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    GWM_HIP_CHECK(c, launch_distance_row(c->dist, c->points(), focus, c->res2, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(out, c->res2, c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GWM_OK;
+}
+
+int gwm_hip_adaptive_bandwidths(gwm_hip_ctx* c, double bw, double* out) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!out) return GWM_ERR_INVALID;
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    const double* bw_pt;
+    if ((rc = resolve_bandwidth(c, 1, bw, &bw_pt))) return rc;
+    GWM_HIP_CHECK(c, hipMemcpyAsync(out, c->bwi, c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GWM_OK;
+}
+
+int gwm_hip_var_criterion(gwm_hip_ctx* c, const double* x, const double* y, int kernel, int adaptive, double bw,
+                          int crit, int precision, double* value) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!x || !y || !value) return GWM_ERR_INVALID;
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    return var_impl(c, x, y, kernel, adaptive, bw, crit, precision, value, nullptr);
+}
+
+int gwm_hip_var_fit(gwm_hip_ctx* c, const double* x, const double* y, int kernel, int adaptive, double bw,
+                    int precision, double* beta) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!x || !y || !beta) return GWM_ERR_INVALID;
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    return var_impl(c, x, y, kernel, adaptive, bw, GWM_CRIT_AIC, precision, nullptr, beta);
+}
+
+int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, const double* bws, const int* init_in,
+                     const int* approach, const int* centered, const double* bw_threshold, int has_intercept,
+                     int crit_type, double crit_threshold, int max_iter, int retry_times, int adaptive_lower,
+                     int has_lower, double lower, int has_upper, double upper, int precision, double* betas_out,
+                     double* bws_out, double* diag, double* rss_out, int* iterations_out) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!kernels || !adaptive || !bws || !init_in || !approach || !centered || !bw_threshold || !betas_out ||
+        !bws_out || !diag)
+        return GWM_ERR_INVALID;
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    const int n = c->n, k = c->k;
+    const std::vector<double> X0 = c->hX, y = c->hy;
+    // GWRMultiscale.cpp:88-99: centre predictors
+    std::vector<double> Xc = X0;
+    for (int i = has_intercept ? 1 : 0; i < k; ++i)
+        if (centered[i]) {
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s += Xc[j + (size_t)i * n];
+            double mu = s / n;
+            for (int j = 0; j < n; ++j) Xc[j + (size_t)i * n] -= mu;
+        }
+    if ((rc = upload_design(c, Xc.data(), y.data()))) return rc;
+    auto restore = [&]() { upload_design(c, X0.data(), y.data()); };
+    std::vector<int> init(init_in, init_in + k);
+    std::vector<double> bw(bws, bws + k), maxD(k, 0.0);
+    auto col = [&](int v) { return Xc.data() + (size_t)v * n; };
+    auto fail_out = [&](int code) {
+        restore();
+        return code;
+    };
+    // initial per-variable bandwidths (GWRMultiscale.cpp:104-135)
+    for (int i = 0; i < k; ++i) {
+        if (init[i] != 0) continue;
+        bool ad = adaptive[i] != 0;
+        if ((rc = max_distance_impl(c, &maxD[i]))) return fail_out(rc);
+        double lo = has_lower ? lower : (ad ? (double)adaptive_lower : maxD[i] / 5000.0);
+        double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
+        std::map<double, double> log;
+        const double* xi = col(i);
+        rc = golden(ad, lo, hi, [&](double x, double* fx) {
+            return var_impl(c, xi, y.data(), kernels[i], adaptive[i], x, approach[i], precision, fx, nullptr);
+        }, &bw[i], log);
+        if (rc) return fail_out(rc);
+    }
+    // initial all-variable bandwidth (GWRMultiscale.cpp:137-160)
+    double maxDist;
+    if ((rc = max_distance_impl(c, &maxDist))) return fail_out(rc);
+    bool ad0 = adaptive[0] != 0;
+    double lo0 = has_lower ? lower : (ad0 ? (double)adaptive_lower : maxDist / 5000.0);
+    double hi0 = has_upper ? upper : (ad0 ? (double)n : maxDist);
+    std::map<double, double> log0;
+    double initBw = 0.0;
+    rc = golden(ad0, lo0, hi0, [&](double x, double* fx) {
+        return criterion_impl(c, kernels[0], adaptive[0], x, approach[0], precision, fx);
+    }, &initBw, log0);
+    if (rc) return fail_out(rc);
+    // backfitting (GWRMultiscale.cpp:208-317)
+    std::vector<double> betas((size_t)n * k), resid(n), yi(n), colv(n);
+    if ((rc = fit_betas_impl(c, kernels[0], adaptive[0], initBw, precision, betas.data()))) return fail_out(rc);
+    auto update_resid = [&]() {
+        for (int j = 0; j < n; ++j) {
+            double f = 0.0;
+            for (int a = 0; a < k; ++a) f += betas[j + (size_t)a * n] * Xc[j + (size_t)a * n];
+            resid[j] = y[j] - f;
+        }
+    };
+    update_resid();
+    double RSS0 = 0.0;
+    for (int j = 0; j < n; ++j) RSS0 += resid[j] * resid[j];
+    double RSS1 = DBL_MAX, crit = DBL_MAX;
+    std::vector<int> bwChangeNo(k, 0);
+    int iteration = 1;
+    for (; iteration <= max_iter && crit > crit_threshold; ++iteration) {
+        if (stopped(c)) return fail_out(GWM_ERR_CANCELLED);
+        for (int i = 0; i < k; ++i) {
+            const double* xi = col(i);
+            for (int j = 0; j < n; ++j) yi[j] = resid[j] + betas[j + (size_t)i * n] * xi[j];
+            if (init[i] != 2) {
+                bool ad = adaptive[i] != 0;
+                double lo = has_lower ? lower : (ad ? (double)adaptive_lower : maxD[i] / 5000.0);
+                double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
+                std::map<double, double> log;
+                double bw1 = 0.0;
+                rc = golden(ad, lo, hi, [&](double x, double* fx) {
+                    return var_impl(c, xi, yi.data(), kernels[i], adaptive[i], x, approach[i], precision, fx, nullptr);
+                }, &bw1, log);
+                if (rc) return fail_out(rc);
+                if (std::fabs(bw1 - bw[i]) > bw_threshold[i]) bwChangeNo[i] = 0;
+                else {
+                    bwChangeNo[i] += 1;
+                    if (bwChangeNo[i] >= retry_times) init[i] = 2;
+                }
+                bw[i] = bw1;
+            }
+            if ((rc = var_impl(c, xi, yi.data(), kernels[i], adaptive[i], bw[i], GWM_CRIT_AIC, precision, nullptr,
+                               colv.data())))
+                return fail_out(rc);
+            std::copy(colv.begin(), colv.end(), betas.begin() + (size_t)i * n);
+            update_resid();
+        }
+        RSS1 = 0.0;
+        for (int j = 0; j < n; ++j) RSS1 += resid[j] * resid[j];
+        crit = crit_type == 0 ? std::fabs(RSS1 - RSS0) : std::sqrt(std::fabs(RSS1 - RSS0) / RSS1);
+        RSS0 = RSS1;
+    }
+    if (iterations_out) *iterations_out = iteration - 1;
+    // CalcDiagnostic (GWRMultiscale.cpp:41-66) with shat = {0, 0} (no hat matrix)
+    const double nd = n;
+    double ym = 0.0;
+    for (int j = 0; j < n; ++j) ym += y[j];
+    ym /= nd;
+    double TSS = 0.0;
+    for (int j = 0; j < n; ++j) TSS += (y[j] - ym) * (y[j] - ym);
+    double r2 = 1 - RSS0 / TSS;
+    double edf = nd;
+    diag[0] = RSS0;
+    diag[1] = 0.0;
+    diag[2] = nd * std::log(RSS0 / nd) + nd * std::log(2 * kPi) + nd * (nd / (nd - 2));
+    diag[3] = 0.0;
+    diag[4] = edf;
+    diag[5] = r2;
+    diag[6] = 1 - (1 - r2) * (nd - 1) / (edf - 1);
+    if (rss_out) *rss_out = RSS0;
+    std::copy(betas.begin(), betas.end(), betas_out);
+    std::copy(bw.begin(), bw.end(), bws_out);
+    restore();
+    return GWM_OK;
+}
+
+int gwm_hip_set_timing(gwm_hip_ctx* c, int enabled) {
+    if (!c) return GWM_ERR_INVALID;
+    c->timing = enabled != 0;
+    return GWM_OK;
+}
+
+int gwm_hip_kernel_time(gwm_hip_ctx* c, const char* name, long long* launches, double* total_ms) {
+    if (!c || !name) return GWM_ERR_INVALID;
+    auto it = c->stats.find(name);
+    if (launches) *launches = it == c->stats.end() ? 0 : it->second.launches;
+    if (total_ms) *total_ms = it == c->stats.end() ? 0.0 : it->second.ms;
+    return GWM_OK;
+}
+
+int gwm_hip_reset_timers(gwm_hip_ctx* c) {
+    if (!c) return GWM_ERR_INVALID;
+    c->stats.clear();
+    c->solves = 0;
+    return GWM_OK;
+}
+
+int gwm_hip_solve_count(gwm_hip_ctx* c, long long* solves) {
+    if (!c || !solves) return GWM_ERR_INVALID;
+    *solves = c->solves;
+    return GWM_OK;
+}
+
+}  // extern "C"

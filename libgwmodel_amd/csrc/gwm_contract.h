@@ -1,0 +1,104 @@
+// Launch interface of the device kernels (host side of the HIP library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "gwm_device.h"
+
+namespace gwm {
+
+// Contraction modes.
+//   kModeCV    : columns [pairs | x*y] with w, self weight zeroed (leave-one-out CV)
+//   kModeHat   : columns [pairs | x*y] with w and [pairs] with w^2 (X^T W^2 X)
+//   kModeQDiag : columns of G = vech(g g^T) with w^2, bandwidth of the streamed index
+//   kModePlain : columns [pairs | x*y] with w (AICc criterion, multiscale fits)
+enum ContractMode { kModeCV = 0, kModeHat = 1, kModeQDiag = 2, kModePlain = 3 };
+
+struct ContractArgs {
+    Points P;
+    const double* Z;       // row-major [rows >= ceil(n/64)*64][NTA*16], zero padded
+    const double* bw_pt;   // per-point bandwidth (adaptive), or nullptr for a fixed bw
+    double bw;
+    int kernel;
+    int n;
+    int n_pad;             // focus padding of the output, multiple of 128
+    int seg_len;           // data points per segment, multiple of 64
+    int nseg;
+    double* out;           // [nseg][(NTA+NTB)*16][n_pad]
+};
+
+// Mixed precision: Z is split into FP16 hi/lo planes, stored per 32-row
+// block in the MFMA B-operand order; accumulation FP32 in registers, flushed
+// to FP64 partial sums.
+struct ContractMixedArgs {
+    Points P;
+    const _Float16* Zh;    // [rows/32][NTA*16 cols][32 rows] hi part
+    const _Float16* Zl;    // same, lo part scaled by 2^11
+    const double* colscale;// per column 2^e scale applied before the split
+    const double* bw_pt;
+    double bw;
+    int kernel;
+    int n;
+    int n_pad;
+    int seg_len;           // multiple of 256
+    int nseg;
+    double* out;
+};
+
+int contract_tile_focus(int nta, int ntb);
+hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const ContractArgs& a, hipStream_t s);
+hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const ContractMixedArgs& a, hipStream_t s);
+
+// Adaptive bandwidth resolution (BandwidthWeight.cpp:40-55): per focus the
+// interpolated distance between the floor(bw)-th and floor(bw)+1-th nearest
+// neighbours, or (bw/n)*max distance when bw >= n.
+hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out, hipStream_t s);
+// Maximum pairwise distance (CRSDistance::maxDistance, CRSDistance.cpp:133-143).
+hipError_t launch_max_distance(int dist, const Points& P, double* block_max, int nblocks, hipStream_t s);
+// Distances from one focus to all points (test hook).
+hipError_t launch_distance_row(int dist, const Points& P, int focus, double* out, hipStream_t s);
+
+enum SolveMode { kSolveCV = 0, kSolveAIC = 1, kSolveFit = 2, kSolveBeta = 3 };
+
+struct SolveArgs {
+    const double* part;    // contraction partial sums [nseg][PT][n_pad]
+    int nseg;
+    int PT;                // total padded columns
+    int colB;              // first column of the w^2 block
+    int n;
+    int n_pad;
+    const double* X;       // n x k column-major
+    const double* y;
+    const double* bw_pt;   // per-focus bandwidth or nullptr
+    double bw;
+    int kernel;
+    // outputs (per focus)
+    double* res2;          // (y_i - x_i^T beta_i)^2
+    double* sii;           // S_ii
+    double* ssi;           // sum_j S_ij^2
+    double* betas;         // n x k column-major (fit)
+    double* se_raw;        // n x k column-major (fit)
+    double* G;             // row-major [n][ldg] vech(g g^T) (fit)
+    int ldg;
+    double* gvec;          // row-major [n][k] g = A^-1 x_i (fit)
+    int* fail;             // set to 1 on a non-SPD local design
+};
+
+hipError_t launch_solve(int k, int mode, const SolveArgs& a, hipStream_t s);
+
+// qDiag_j = 1 - 2 S_jj + x_j^T M_j x_j, M_j from the kModeQDiag contraction.
+hipError_t launch_qdiag(int k, const double* part, int nseg, int PT, int n, int n_pad, const double* X,
+                        const double* sii, double* qdiag, hipStream_t s);
+
+// S matrix rows (only for small n): S_ij = w_ij g_i^T x_j.
+hipError_t launch_smatrix(int dist, int k, const Points& P, const double* X, const double* gvec,
+                          const double* bw_pt, double bw, int kernel, double* S, hipStream_t s);
+
+// Deterministic sums of up to 3 vectors of length n into out[0..m).
+hipError_t launch_reduce(const double* a, const double* b, const double* c, int n, double* out, hipStream_t s);
+
+// Single-column local solves for the multiscale path (GWRMultiscale.cpp:420-482,
+// 560-636): part columns 0 = sum w x^2, 1 = sum w x y, colB = sum w^2 x^2.
+hipError_t launch_solve_var(int mode, const double* part, int nseg, int PT, int colB, int n, int n_pad,
+                            const double* x, const double* y, const double* bw_pt, double bw, int kernel,
+                            double* res2, double* sii, double* beta, int* fail, hipStream_t s);
+
+}  // namespace gwm

@@ -1,0 +1,239 @@
+// Fused producer + contraction: for every focus location i and every column
+// c of the per-point product table Z,
+//     acc[i][c] = sum_j w(d(i, j); b)^p * Z[j][c]
+// without ever materialising the n x n weight matrix.  This is the
+// per-location X^T W X / X^T W y (and X^T W^2 X) of the reference's local
+// solves (src/gwmodelpp/GWRBasic.cpp:204-207, 241-246, 278-282, 403-406;
+// src/gwmodelpp/GWRMultiscale.cpp:372-375, 433-436), recast as one dense
+// contraction W * Z.
+//
+// FP64 path: distances and weights in FP64 on the VALU, contraction on the
+// FP64 matrix cores (v_mfma_f64_16x16x4f64).
+//
+// Tiling (one workgroup = 4 waves of 64 lanes):
+//   - a wave owns R tiles of 16 focus points; each lane produces one weight
+//     per tile and MFMA step: focus = lane % 16, data = lane / 16 (4 per step);
+//   - the data range is cut into nseg segments; blockIdx % nseg = segment so
+//     that consecutive workgroups (which the dispatcher spreads over the 8
+//     XCDs round-robin) stream different slices of Z and each XCD's L2 keeps
+//     one slice hot;
+//   - Z rows and data coordinates of a 64-point chunk are staged in LDS and
+//     shared by the 4 waves.
+// Output: partial sums out[(seg * PT + c) * n_pad + i], reduced in a fixed
+// order by the solve kernels (deterministic, independent of scheduling).
+#include "gwm_contract.h"
+
+namespace gwm {
+
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kChunk = 64;
+
+template <int DIST>
+struct CoordStage;
+
+template <>
+struct CoordStage<kEuclidean> {
+    double x[kChunk], y[kChunk];
+};
+template <>
+struct CoordStage<kGreatCircle> {
+    double lon[kChunk], lat[kChunk], hx[kChunk], hy[kChunk], hz[kChunk], sl[kChunk];
+};
+
+template <int NTA, int DIST, int MODE>
+struct SharedTile {
+    double z[kChunk][NTA * 16];
+    CoordStage<DIST> c;
+    double bw[MODE == kModeQDiag ? kChunk : 1];
+};
+
+template <int DIST>
+struct FocusCoords {
+    double a, b, hx, hy, hz, sl;
+};
+
+template <int DIST>
+__device__ __forceinline__ FocusCoords<DIST> load_focus(const Points& P, int f) {
+    FocusCoords<DIST> fc;
+    fc.a = P.px[f];
+    fc.b = P.py[f];
+    if (DIST == kGreatCircle) {
+        fc.hx = P.hx[f];
+        fc.hy = P.hy[f];
+        fc.hz = P.hz[f];
+        fc.sl = P.sl[f];
+    }
+    return fc;
+}
+
+template <int DIST, int NTA, int MODE>
+__device__ __forceinline__ double pair_distance(const FocusCoords<DIST>& fc, const SharedTile<NTA, DIST, MODE>& sh,
+                                                int r) {
+    if constexpr (DIST == kEuclidean) {
+        return sqrt(d2_euclid(fc.a, fc.b, sh.c.x[r], sh.c.y[r]));
+    } else {
+        return gcdist_f64(fc.a, fc.b, fc.hx, fc.hy, fc.hz, fc.sl, sh.c.lon[r], sh.c.lat[r], sh.c.hx[r],
+                          sh.c.hy[r], sh.c.hz[r], sh.c.sl[r]);
+    }
+}
+
+template <int NTA, int NTB, int DIST, int MODE, int R>
+__global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs args) {
+    __shared__ SharedTile<NTA, DIST, MODE> sh;
+    constexpr int LDZ = NTA * 16;
+    constexpr int PT = (NTA + NTB) * 16;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int seg = blockIdx.x % args.nseg;
+    const int tile = blockIdx.x / args.nseg;
+    const int fbase = tile * (kWaves * 16 * R) + wave * 16 * R;
+    const int n = args.n;
+
+    // focus points owned by this lane (one per A tile)
+    FocusCoords<DIST> fc[R];
+    double fbw[R];
+    int fidx[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int f = fbase + 16 * r + (lane & 15);
+        fidx[r] = f;
+        int fl = f < n ? f : n - 1;
+        fc[r] = load_focus<DIST>(args.P, fl);
+        fbw[r] = (MODE != kModeQDiag && args.bw_pt) ? args.bw_pt[fl] : args.bw;
+    }
+
+    v4d accA[R][NTA];
+    v4d accB[R][NTB > 0 ? NTB : 1];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int t = 0; t < NTA; ++t) accA[r][t] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < (NTB > 0 ? NTB : 1); ++t) accB[r][t] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+
+    const int jbeg = seg * args.seg_len;
+    const int jend = min(n, jbeg + args.seg_len);
+    const int kk = lane >> 4;
+
+    for (int j0 = jbeg; j0 < jend; j0 += kChunk) {
+        __syncthreads();
+        // stage Z rows [j0, j0 + kChunk) (zero padded beyond n in memory)
+        {
+            const double* src = args.Z + (size_t)j0 * LDZ;
+            double* dst = &sh.z[0][0];
+            for (int e = threadIdx.x; e < kChunk * LDZ; e += 256) dst[e] = src[e];
+        }
+        if (threadIdx.x < kChunk) {
+            int j = j0 + threadIdx.x;
+            int jl = j < n ? j : n - 1;
+            if constexpr (DIST == kEuclidean) {
+                sh.c.x[threadIdx.x] = args.P.px[jl];
+                sh.c.y[threadIdx.x] = args.P.py[jl];
+            } else {
+                sh.c.lon[threadIdx.x] = args.P.px[jl];
+                sh.c.lat[threadIdx.x] = args.P.py[jl];
+                sh.c.hx[threadIdx.x] = args.P.hx[jl];
+                sh.c.hy[threadIdx.x] = args.P.hy[jl];
+                sh.c.hz[threadIdx.x] = args.P.hz[jl];
+                sh.c.sl[threadIdx.x] = args.P.sl[jl];
+            }
+            if constexpr (MODE == kModeQDiag) sh.bw[threadIdx.x] = args.bw_pt ? args.bw_pt[jl] : args.bw;
+        }
+        __syncthreads();
+        const int steps = min(kChunk, jend - j0 + 3) / 4;
+        for (int s = 0; s < steps; ++s) {
+            const int rr = 4 * s + kk;    // row of the chunk this lane weighs
+            const int j = j0 + rr;
+            double zb[NTA];
+#pragma unroll
+            for (int t = 0; t < NTA; ++t) zb[t] = sh.z[rr][16 * t + (lane & 15)];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double d = pair_distance<DIST, NTA, MODE>(fc[r], sh, rr);
+                double b = MODE == kModeQDiag ? sh.bw[rr] : fbw[r];
+                double w = kernel_f64(args.kernel, d, b);
+                if (j >= jend) w = 0.0;
+                if (MODE == kModeCV && j == fidx[r]) w = 0.0;
+                double wa = MODE == kModeQDiag ? w * w : w;
+#pragma unroll
+                for (int t = 0; t < NTA; ++t)
+                    accA[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(wa, zb[t], accA[r][t], 0, 0, 0);
+                if (NTB > 0) {
+                    double w2 = w * w;
+#pragma unroll
+                    for (int t = 0; t < NTB; ++t)
+                        accB[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(w2, zb[t], accB[r][t], 0, 0, 0);
+                }
+            }
+        }
+    }
+
+    // D layout of v_mfma_f64_16x16x4f64 (checked on gfx950): lane holds
+    // column (lane % 16), rows (lane / 16) + 4 q, q = 0..3.
+    const size_t np = (size_t)args.n_pad;
+    double* out = args.out + (size_t)seg * PT * np;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int f0 = fbase + 16 * r + kk;
+#pragma unroll
+        for (int t = 0; t < NTA; ++t) {
+            double* o = out + (size_t)(16 * t + (lane & 15)) * np + f0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o[4 * q] = accA[r][t][q];
+        }
+#pragma unroll
+        for (int t = 0; t < NTB; ++t) {
+            double* o = out + (size_t)(LDZ + 16 * t + (lane & 15)) * np + f0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o[4 * q] = accB[r][t][q];
+        }
+    }
+}
+
+template <int NTA, int NTB, int DIST, int MODE>
+hipError_t launch_f64_r(const ContractArgs& a, hipStream_t s) {
+    constexpr int R = (NTA + NTB) <= 8 ? 2 : 1;
+    int tiles = (a.n + kWaves * 16 * R - 1) / (kWaves * 16 * R);
+    hipLaunchKernelGGL((contract_f64_kernel<NTA, NTB, DIST, MODE, R>), dim3(tiles * a.nseg), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int DIST, int MODE>
+hipError_t dispatch_ab(int nta, int ntb, const ContractArgs& a, hipStream_t s) {
+#define GWM_CASE(A, B) \
+    if (nta == A && ntb == B) return launch_f64_r<A, B, DIST, MODE>(a, s);
+    if (MODE == kModeHat) {
+        GWM_CASE(1, 1) GWM_CASE(2, 1) GWM_CASE(2, 2) GWM_CASE(3, 2) GWM_CASE(3, 3) GWM_CASE(4, 3)
+        GWM_CASE(5, 4) GWM_CASE(5, 5) GWM_CASE(6, 5) GWM_CASE(7, 6) GWM_CASE(8, 7) GWM_CASE(9, 8)
+        GWM_CASE(10, 9)
+    } else {
+        GWM_CASE(1, 0) GWM_CASE(2, 0) GWM_CASE(3, 0) GWM_CASE(4, 0) GWM_CASE(5, 0) GWM_CASE(6, 0)
+        GWM_CASE(7, 0) GWM_CASE(8, 0) GWM_CASE(9, 0) GWM_CASE(10, 0)
+    }
+#undef GWM_CASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+int contract_tile_focus(int nta, int ntb) { return kWaves * 16 * (((nta + ntb) <= 8) ? 2 : 1); }
+
+hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const ContractArgs& a, hipStream_t s) {
+    if (dist == kEuclidean) {
+        if (mode == kModeCV) return dispatch_ab<kEuclidean, kModeCV>(nta, ntb, a, s);
+        if (mode == kModePlain) return dispatch_ab<kEuclidean, kModePlain>(nta, ntb, a, s);
+        if (mode == kModeHat) return dispatch_ab<kEuclidean, kModeHat>(nta, ntb, a, s);
+        return dispatch_ab<kEuclidean, kModeQDiag>(nta, ntb, a, s);
+    } else {
+        if (mode == kModeCV) return dispatch_ab<kGreatCircle, kModeCV>(nta, ntb, a, s);
+        if (mode == kModePlain) return dispatch_ab<kGreatCircle, kModePlain>(nta, ntb, a, s);
+        if (mode == kModeHat) return dispatch_ab<kGreatCircle, kModeHat>(nta, ntb, a, s);
+        return dispatch_ab<kGreatCircle, kModeQDiag>(nta, ntb, a, s);
+    }
+}
+
+}  // namespace gwm

@@ -1,0 +1,367 @@
+// Mixed-precision fused producer + contraction on the FP16 matrix cores.
+//
+// Same contraction as gwm_contract.hip (acc[i][c] = sum_j w_ij^p Z[j][c]),
+// with the arithmetic split as follows:
+//   - coordinate differences and the squared distance (Euclidean) or the
+//     half-chord S (great circle) in FP64 -- nearby points cancel there;
+//   - the transcendental tail of the distance and the kernel weight in FP32;
+//   - w and Z split into FP16 hi + lo parts, lo scaled by 2^11 so small
+//     values stay normal: w = wh + wl * 2^-11, z = zh + zl * 2^-11 (each part
+//     carries 11 significant bits, the pair ~22);
+//   - w z ~= wh zh + 2^-11 (wh zl + wl zh) on v_mfma_f32_16x16x32_f16 with
+//     FP32 accumulation (the dropped wl zl term is below 2^-24 relative);
+//   - every 64 data points the FP32 accumulators are flushed into FP64 ones,
+//     so FP32 rounding only ever acts on 64-term partial sums;
+//   - Z columns carry a power-of-two scale (exact) that keeps them inside the
+//     FP16 range; it is undone when the FP64 sums are written.
+// Tiling: a wave owns R tiles of 16 focus points; per MFMA step a lane forms 8
+// weights (focus = lane % 16, data = 8 * (lane / 16) + e).  Z planes of a
+// 64-point chunk are staged in LDS with a padded column stride (40 halfs) so
+// the 16-byte B-operand reads of a quarter-wave hit distinct banks.
+#include "gwm_contract.h"
+
+namespace gwm {
+
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kChunkM = 64;        // data points per chunk (2 MFMA steps)
+constexpr int kColStride = 40;     // LDS halfs per column per 32-row block
+constexpr float kLoScale = 2048.0f;
+
+template <int DIST>
+struct MCoords;
+template <>
+struct MCoords<kEuclidean> {
+    double x[kChunkM], y[kChunkM];
+};
+template <>
+struct MCoords<kGreatCircle> {
+    double hx[kChunkM], hy[kChunkM], hz[kChunkM];
+    float sl[kChunkM];
+};
+
+template <int NTA, int DIST, int MODE>
+struct MShared {
+    _Float16 zh[2][NTA * 16][kColStride];
+    _Float16 zl[2][NTA * 16][kColStride];
+    MCoords<DIST> c;
+    float bw[MODE == kModeQDiag ? kChunkM : 1];
+};
+
+template <int DIST>
+struct MFocus {
+    double a, b, c;
+    float sl;
+};
+
+template <int DIST>
+__device__ __forceinline__ MFocus<DIST> load_mfocus(const Points& P, int f) {
+    MFocus<DIST> fc;
+    if constexpr (DIST == kEuclidean) {
+        fc.a = P.px[f];
+        fc.b = P.py[f];
+        fc.c = 0.0;
+        fc.sl = 0.f;
+    } else {
+        fc.a = P.hx[f];
+        fc.b = P.hy[f];
+        fc.c = P.hz[f];
+        fc.sl = (float)P.sl[f];
+    }
+    return fc;
+}
+
+// Kernel weight from the squared distance (Euclidean) or from the distance.
+struct BwConst {
+    float bw, inv_bw, bw2, inv_bw2, gauss;   // gauss = -log2(e) / (2 bw^2)
+};
+
+__device__ __forceinline__ BwConst make_bw(float b) {
+    BwConst c;
+    c.bw = b;
+    c.inv_bw = 1.0f / b;
+    c.bw2 = b * b;
+    c.inv_bw2 = 1.0f / (b * b);
+    c.gauss = -1.4426950408889634f * 0.5f * c.inv_bw2;
+    return c;
+}
+
+__device__ __forceinline__ float weight_from_d2(int kernel, float d2, const BwConst& c) {
+    switch (kernel) {
+    case kGaussian: return __builtin_amdgcn_exp2f(d2 * c.gauss);
+    case kExponential: return __builtin_amdgcn_exp2f(-1.4426950408889634f * __builtin_sqrtf(d2) * c.inv_bw);
+    case kBisquare: {
+        float t = 1.0f - d2 * c.inv_bw2;
+        return d2 < c.bw2 ? t * t : 0.0f;
+    }
+    case kTricube: {
+        float q = __builtin_sqrtf(d2) * c.inv_bw;
+        float t = 1.0f - q * q * q;
+        return d2 < c.bw2 ? (t * t) * t : 0.0f;
+    }
+    default: return d2 < c.bw2 ? 1.0f : 0.0f;
+    }
+}
+
+__device__ __forceinline__ void split16(float w, _Float16& hi, _Float16& lo) {
+    hi = (_Float16)w;
+    lo = (_Float16)((w - (float)hi) * kLoScale);
+}
+
+template <int NTA, int NTB, int DIST, int MODE, int R>
+__global__ __launch_bounds__(256) void contract_mixed_kernel(ContractMixedArgs args) {
+    __shared__ MShared<NTA, DIST, MODE> sh;
+    constexpr int LDZ = NTA * 16;
+    constexpr int PT = (NTA + NTB) * 16;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int seg = blockIdx.x % args.nseg;
+    const int tile = blockIdx.x / args.nseg;
+    const int fbase = tile * (kWaves * 16 * R) + wave * 16 * R;
+    const int n = args.n;
+    const int kq = lane >> 4;       // which 8-row group of the 32-row step
+    const int col = lane & 15;
+
+    MFocus<DIST> fc[R];
+    BwConst fbw[R];
+    int fidx[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int f = fbase + 16 * r + col;
+        fidx[r] = f;
+        int fl = f < n ? f : n - 1;
+        fc[r] = load_mfocus<DIST>(args.P, fl);
+        double b = (MODE != kModeQDiag && args.bw_pt) ? args.bw_pt[fl] : args.bw;
+        if (DIST == kEuclidean) fbw[r] = make_bw((float)b);
+        else fbw[r] = make_bw((float)b);
+    }
+
+    v4f hh[R][NTA + NTB], xx[R][NTA + NTB];
+    v4d acc[R][NTA + NTB];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int t = 0; t < NTA + NTB; ++t) acc[r][t] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    const int jbeg = seg * args.seg_len;
+    const int jend = min(n, jbeg + args.seg_len);
+
+    for (int j0 = jbeg; j0 < jend; j0 += kChunkM) {
+        __syncthreads();
+        // stage the two 32-row blocks of both planes: global [blk][col][32]
+        {
+            const size_t base = (size_t)(j0 / 32) * LDZ * 32;
+            const uint4* gh = reinterpret_cast<const uint4*>(args.Zh + base);
+            const uint4* gl = reinterpret_cast<const uint4*>(args.Zl + base);
+            constexpr int vecs = 2 * LDZ * 4;   // 16-byte vectors per plane (8 halfs each)
+            for (int e = threadIdx.x; e < vecs; e += 256) {
+                const int blk = e / (LDZ * 4);
+                const int rem = e % (LDZ * 4);
+                const int c = rem >> 2, part = rem & 3;
+                *reinterpret_cast<uint4*>(&sh.zh[blk][c][part * 8]) = gh[e];
+                *reinterpret_cast<uint4*>(&sh.zl[blk][c][part * 8]) = gl[e];
+            }
+        }
+        if (threadIdx.x < kChunkM) {
+            int j = j0 + threadIdx.x;
+            int jl = j < n ? j : n - 1;
+            if constexpr (DIST == kEuclidean) {
+                sh.c.x[threadIdx.x] = args.P.px[jl];
+                sh.c.y[threadIdx.x] = args.P.py[jl];
+            } else {
+                sh.c.hx[threadIdx.x] = args.P.hx[jl];
+                sh.c.hy[threadIdx.x] = args.P.hy[jl];
+                sh.c.hz[threadIdx.x] = args.P.hz[jl];
+                sh.c.sl[threadIdx.x] = (float)args.P.sl[jl];
+            }
+            if constexpr (MODE == kModeQDiag) sh.bw[threadIdx.x] = (float)(args.bw_pt ? args.bw_pt[jl] : args.bw);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int t = 0; t < NTA + NTB; ++t) {
+                hh[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
+                xx[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
+            }
+        const int nsteps = (jend - j0) > 32 ? 2 : 1;
+        for (int s = 0; s < nsteps; ++s) {
+            // B operands: column col of each tile, rows 8*kq .. 8*kq+7 of block s
+            v8h bh[NTA], bl[NTA];
+#pragma unroll
+            for (int t = 0; t < NTA; ++t) {
+                bh[t] = *reinterpret_cast<const v8h*>(&sh.zh[s][16 * t + col][8 * kq]);
+                bl[t] = *reinterpret_cast<const v8h*>(&sh.zl[s][16 * t + col][8 * kq]);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                v8h ah, al, a2h, a2l;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int rr = 32 * s + 8 * kq + e;
+                    const int j = j0 + rr;
+                    float w;
+                    if constexpr (DIST == kEuclidean) {
+                        double dx = sh.c.x[rr] - fc[r].a, dy = sh.c.y[rr] - fc[r].b;
+                        float d2 = (float)(dx * dx + dy * dy);
+                        BwConst bc = MODE == kModeQDiag ? make_bw(sh.bw[rr]) : fbw[r];
+                        w = weight_from_d2(args.kernel, d2, bc);
+                    } else {
+                        double ex = sh.c.hx[rr] - fc[r].a, ey = sh.c.hy[rr] - fc[r].b, ez = sh.c.hz[rr] - fc[r].c;
+                        float S = (float)(ex * ex + ey * ey + ez * ez);
+                        float d = gcdist_tail_f32(S, fc[r].sl, sh.c.sl[rr]);
+                        BwConst bc = MODE == kModeQDiag ? make_bw(sh.bw[rr]) : fbw[r];
+                        w = weight_from_d2(args.kernel, d * d, bc);
+                    }
+                    if (j >= jend) w = 0.0f;
+                    if (MODE == kModeCV && j == fidx[r]) w = 0.0f;
+                    const float wa = MODE == kModeQDiag ? w * w : w;
+                    _Float16 h, l;
+                    split16(wa, h, l);
+                    ah[e] = h;
+                    al[e] = l;
+                    if (NTB > 0) {
+                        split16(w * w, h, l);
+                        a2h[e] = h;
+                        a2l[e] = l;
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < NTA; ++t) {
+                    hh[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[t], hh[r][t], 0, 0, 0);
+                    xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[t], xx[r][t], 0, 0, 0);
+                    xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], xx[r][t], 0, 0, 0);
+                }
+#pragma unroll
+                for (int t = 0; t < NTB; ++t) {
+                    hh[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bh[t], hh[r][NTA + t], 0, 0, 0);
+                    xx[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bl[t], xx[r][NTA + t], 0, 0, 0);
+                    xx[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bh[t], xx[r][NTA + t], 0, 0, 0);
+                }
+            }
+        }
+        // flush the 64-point partial sums into FP64
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int t = 0; t < NTA + NTB; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    acc[r][t][q] += (double)hh[r][t][q] + (double)xx[r][t][q] * (1.0 / 2048.0);
+    }
+
+    // D layout of v_mfma_f32_16x16x32_f16 (checked on gfx950): lane holds
+    // column (lane % 16), rows 4 * (lane / 16) + q.
+    const size_t np = (size_t)args.n_pad;
+    double* out = args.out + (size_t)seg * PT * np;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int f0 = fbase + 16 * r + 4 * kq;
+#pragma unroll
+        for (int t = 0; t < NTA + NTB; ++t) {
+            const int zc = (t < NTA ? 16 * t : 16 * (t - NTA)) + col;
+            const double unscale = args.colscale[zc];
+            double* o = out + (size_t)(16 * t + col) * np + f0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o[q] = acc[r][t][q] * unscale;
+        }
+    }
+}
+
+template <int NTA, int NTB, int DIST, int MODE>
+hipError_t launch_mixed_r(const ContractMixedArgs& a, hipStream_t s) {
+    constexpr int R = (NTA + NTB) <= 6 ? 2 : 1;
+    int tiles = (a.n + kWaves * 16 * R - 1) / (kWaves * 16 * R);
+    hipLaunchKernelGGL((contract_mixed_kernel<NTA, NTB, DIST, MODE, R>), dim3(tiles * a.nseg), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int DIST, int MODE>
+hipError_t dispatch_mixed(int nta, int ntb, const ContractMixedArgs& a, hipStream_t s) {
+#define GWM_CASE(A, B) \
+    if (nta == A && ntb == B) return launch_mixed_r<A, B, DIST, MODE>(a, s);
+    if (MODE == kModeHat) {
+        GWM_CASE(1, 1) GWM_CASE(2, 1) GWM_CASE(2, 2) GWM_CASE(3, 2) GWM_CASE(3, 3) GWM_CASE(4, 3)
+        GWM_CASE(5, 4) GWM_CASE(5, 5) GWM_CASE(6, 5) GWM_CASE(7, 6) GWM_CASE(8, 7) GWM_CASE(9, 8)
+        GWM_CASE(10, 9)
+    } else {
+        GWM_CASE(1, 0) GWM_CASE(2, 0) GWM_CASE(3, 0) GWM_CASE(4, 0) GWM_CASE(5, 0) GWM_CASE(6, 0)
+        GWM_CASE(7, 0) GWM_CASE(8, 0) GWM_CASE(9, 0) GWM_CASE(10, 0)
+    }
+#undef GWM_CASE
+    return hipErrorInvalidValue;
+}
+
+// Column maxima of |Z| (as FP64 bit patterns, monotone for non-negative values).
+__global__ void colmax_kernel(const double* Z, int rows, int ldz, unsigned long long* colmax) {
+    const int c = blockIdx.y;
+    unsigned long long m = 0ull;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < rows; j += gridDim.x * blockDim.x) {
+        double v = fabs(Z[(size_t)j * ldz + c]);
+        unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        m = b > m ? b : m;
+    }
+    __shared__ unsigned long long s[256];
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = s[threadIdx.x] > s[threadIdx.x + w] ? s[threadIdx.x] : s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(colmax + c, s[0]);
+}
+
+// Split Z (FP64, row-major [rows][ldz]) into the FP16 hi/lo planes laid out
+// [rows/32][ldz][32] and the per-column unscale factors.
+__global__ void split_kernel(const double* Z, int rows, int ldz, const unsigned long long* colmax, _Float16* Zh,
+                             _Float16* Zl, double* unscale) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)rows * ldz) return;
+    const int j = (int)(e / ldz), c = (int)(e % ldz);
+    const double mx = __longlong_as_double((long long)colmax[c]);
+    int ex = 0;
+    if (mx > 0.0) ex = (int)floor(log2(16384.0 / mx));
+    ex = ex > 60 ? 60 : (ex < -60 ? -60 : ex);
+    const double s = ldexp(1.0, ex);
+    const double v = Z[e] * s;
+    const _Float16 hi = (_Float16)(float)v;
+    const _Float16 lo = (_Float16)(float)((v - (double)(float)hi) * 2048.0);
+    const size_t o = ((size_t)(j / 32) * ldz + c) * 32 + (j % 32);
+    Zh[o] = hi;
+    Zl[o] = lo;
+    if (j == 0) unscale[c] = ldexp(1.0, -ex);
+}
+
+}  // namespace
+
+int contract_mixed_tile_focus(int nta, int ntb) { return kWaves * 16 * (((nta + ntb) <= 6) ? 2 : 1); }
+
+hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const ContractMixedArgs& a, hipStream_t s) {
+    if (dist == kEuclidean) {
+        if (mode == kModeCV) return dispatch_mixed<kEuclidean, kModeCV>(nta, ntb, a, s);
+        if (mode == kModePlain) return dispatch_mixed<kEuclidean, kModePlain>(nta, ntb, a, s);
+        if (mode == kModeHat) return dispatch_mixed<kEuclidean, kModeHat>(nta, ntb, a, s);
+        return dispatch_mixed<kEuclidean, kModeQDiag>(nta, ntb, a, s);
+    } else {
+        if (mode == kModeCV) return dispatch_mixed<kGreatCircle, kModeCV>(nta, ntb, a, s);
+        if (mode == kModePlain) return dispatch_mixed<kGreatCircle, kModePlain>(nta, ntb, a, s);
+        if (mode == kModeHat) return dispatch_mixed<kGreatCircle, kModeHat>(nta, ntb, a, s);
+        return dispatch_mixed<kGreatCircle, kModeQDiag>(nta, ntb, a, s);
+    }
+}
+
+hipError_t launch_split_fp16(const double* Z, int rows, int ldz, unsigned long long* colmax, _Float16* Zh,
+                             _Float16* Zl, double* unscale, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(colmax, 0, ldz * sizeof(unsigned long long), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(colmax_kernel, dim3(std::min(64, (rows + 255) / 256), ldz), dim3(256), 0, s, Z, rows, ldz,
+                       colmax);
+    size_t total = (size_t)rows * ldz;
+    hipLaunchKernelGGL(split_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, Z, rows, ldz, colmax, Zh,
+                       Zl, unscale);
+    return hipGetLastError();
+}
+
+}  // namespace gwm

@@ -1,0 +1,173 @@
+// Device-side building blocks shared by the GWR kernels (gfx950 / CDNA4).
+//
+// Every kernel here is compiled with -ffp-contract=off: a*b+c is never fused
+// unless written as fma(), so the Euclidean distance and the kernel weights
+// are evaluated with the same roundings as the reference's Armadillo
+// expressions (include/gwmodelpp/spatialweight/CRSDistance.h:95-99,
+// include/gwmodelpp/spatialweight/BandwidthWeight.h:68-121).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gwm {
+
+enum KernelId { kGaussian = 0, kExponential = 1, kBisquare = 2, kTricube = 3, kBoxcar = 4 };
+enum DistId { kEuclidean = 0, kGreatCircle = 1 };
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+
+// Point table on the device, structure of arrays.  Euclidean uses px/py.
+// Great-circle uses the raw lon/lat (degrees) for the coincident-point test
+// of CRSDistance.cpp:26-37, the half unit vector (hx, hy, hz) =
+// 0.5*(cos lat cos lon, cos lat sin lon, sin lat) and sl = sin lat.
+struct Points {
+    const double* px;   // x or lon (degrees)
+    const double* py;   // y or lat (degrees)
+    const double* hx;
+    const double* hy;
+    const double* hz;
+    const double* sl;
+    int n;
+};
+
+// ---------------------------------------------------------------- FP64
+
+// Squared Euclidean distance, evaluated as (dx*dx) + (dy*dy) without
+// contraction; sqrt of it is bit-identical to CRSDistance::EuclideanDistance.
+__device__ __forceinline__ double d2_euclid(double xf, double yf, double xj, double yj) {
+    double dx = xj - xf;
+    double dy = yj - yf;
+    double s = dx * dx;
+    return s + dy * dy;
+}
+
+// WGS-84 ellipsoid-corrected great-circle distance in km
+// (CRSDistance::SpGcdist, CRSDistance.cpp:15-69), restated through the
+// identity S = sin^2(G)cos^2(L) + cos^2(F)sin^2(L) = |u_i - u_j|^2 / 4 for the
+// unit vectors u, and sin F cos G = (sin lat1 + sin lat2)/2,
+// cos F sin G = (sin lat1 - sin lat2)/2.  w = atan(sqrt(S/C)) = asin(sqrt(S)).
+__device__ __forceinline__ double gcdist_f64(double lonf, double latf, double hxf, double hyf, double hzf,
+                                             double slf, double lonj, double latj, double hxj, double hyj,
+                                             double hzj, double slj) {
+    const double a = 6378.137;
+    const double f = 1.0 / 298.257223563;
+    if (fabs(latj - latf) < 1e-8) {
+        if (fabs(lonj - lonf) < 1e-8) return 0.0;
+        if (fabs((fabs(lonj) + fabs(lonf)) - 360.0) < 1e-8) return 0.0;
+    }
+    double ex = hxj - hxf, ey = hyj - hyf, ez = hzj - hzf;
+    double S = ex * ex;
+    S = S + ey * ey;
+    S = S + ez * ez;
+    if (S <= 0.0) return 0.0;
+    double C = 1.0 - S;
+    double w = asin(sqrt(S));
+    double R = sqrt(S * C) / w;
+    double D = 2 * w * a;
+    double H1 = (3 * R - 1) / (2 * C);
+    double H2 = (3 * R + 1) / (2 * S);
+    double sp = 0.5 * (slj + slf);
+    double sm = 0.5 * (slj - slf);
+    return D * (1 + f * H1 * (sp * sp) - f * H2 * (sm * sm));
+}
+
+// Kernel weights with the CPU boundary (d < bw): BandwidthWeight.h:68-121.
+__device__ __forceinline__ double kernel_f64(int kernel, double d, double bw) {
+    switch (kernel) {
+    case kGaussian: return exp((d * d) / ((-2.0) * (bw * bw)));
+    case kExponential: return exp(-d / bw);
+    case kBisquare: {
+        double t = 1.0 - (d * d) / (bw * bw);
+        return d < bw ? t * t : 0.0;
+    }
+    case kTricube: {
+        double t = 1.0 - (d * d * d) / (bw * bw * bw);
+        return d < bw ? (t * t) * t : 0.0;
+    }
+    default: return d < bw ? 1.0 : 0.0;
+    }
+}
+
+template <int DIST>
+__device__ __forceinline__ double distance_f64(const Points& P, int f, int j) {
+    if (DIST == kEuclidean) {
+        return sqrt(d2_euclid(P.px[f], P.py[f], P.px[j], P.py[j]));
+    } else {
+        return gcdist_f64(P.px[f], P.py[f], P.hx[f], P.hy[f], P.hz[f], P.sl[f], P.px[j], P.py[j], P.hx[j],
+                          P.hy[j], P.hz[j], P.sl[j]);
+    }
+}
+
+// Selection key: a monotone image of the distance.  Euclidean uses the
+// squared distance (sqrt is monotone, so order statistics commute with it);
+// great-circle uses the distance itself.
+template <int DIST>
+__device__ __forceinline__ double select_value(const Points& P, int f, int j) {
+    if (DIST == kEuclidean) return d2_euclid(P.px[f], P.py[f], P.px[j], P.py[j]);
+    return distance_f64<DIST>(P, f, j);
+}
+template <int DIST>
+__device__ __forceinline__ double select_to_distance(double v) {
+    return DIST == kEuclidean ? sqrt(v) : v;
+}
+
+// ---------------------------------------------------------------- FP32
+
+// FP32 pieces of the mixed-precision producer: the same restatement as
+// gcdist_f64 with hardware square root / reciprocal and an FP32 arcsine
+// (odd polynomial on [0, 0.5], reflected as pi/2 - 2 asin(sqrt((1-x)/2))).
+__device__ __forceinline__ float asin_f32(float x) {
+    // x in [0, 1]
+    bool big = x > 0.5f;
+    float z = big ? 0.5f * (1.0f - x) : x * x;
+    float s = big ? __builtin_sqrtf(z) : x;
+    // asin(s) = s + s*z*p(z) on |s| <= 0.5 with the published Cephes asinf
+    // coefficients (S. L. Moshier, Cephes Math Library), ~2.5e-7 relative
+    float p = fmaf(fmaf(fmaf(fmaf(4.2163199048e-2f, z, 2.4181311049e-2f), z, 4.5470025998e-2f), z,
+                        7.4953002686e-2f), z, 1.6666752422e-1f);
+    float r = fmaf(s * z, p, s);
+    return big ? 1.5707963267948966f - 2.0f * r : r;
+}
+
+// S = sin^2(sigma/2) is formed in FP64 by the caller (the chord difference
+// of nearby points cancels); the transcendental tail runs in FP32.
+__device__ __forceinline__ float gcdist_tail_f32(float S, float slf, float slj) {
+    const float a = 6378.137f;
+    const float f = (float)(1.0 / 298.257223563);
+    if (!(S > 0.0f)) return 0.0f;
+    float C = 1.0f - S;
+    float s = __builtin_sqrtf(S);
+    float w = asin_f32(s);
+    float R = s * __builtin_sqrtf(C) * __builtin_amdgcn_rcpf(w);
+    float H1 = (3.0f * R - 1.0f) * 0.5f * __builtin_amdgcn_rcpf(C);
+    float H2 = (3.0f * R + 1.0f) * 0.5f * __builtin_amdgcn_rcpf(S);
+    float sp = 0.5f * (slj + slf);
+    float sm = 0.5f * (slj - slf);
+    float corr = fmaf(f * H1, sp * sp, -(f * H2) * (sm * sm));
+    return 2.0f * a * w * (1.0f + corr);
+}
+
+__device__ __forceinline__ float kernel_f32(int kernel, float d, float bw, float inv_bw) {
+    switch (kernel) {
+    case kGaussian: {
+        float t = d * inv_bw;
+        return __expf(-0.5f * t * t);
+    }
+    case kExponential: return __expf(-d * inv_bw);
+    case kBisquare: {
+        float q = d * inv_bw;
+        float t = 1.0f - q * q;
+        return d < bw ? t * t : 0.0f;
+    }
+    case kTricube: {
+        float q = d * inv_bw;
+        float t = 1.0f - q * q * q;
+        return d < bw ? (t * t) * t : 0.0f;
+    }
+    default: return d < bw ? 1.0f : 0.0f;
+    }
+}
+
+}  // namespace gwm

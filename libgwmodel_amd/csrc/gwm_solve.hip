@@ -1,0 +1,337 @@
+// Batched local solves: one thread per focus location, FP64 throughout.
+//
+// The reference inverts each k x k normal matrix with arma::inv_sympd and
+// multiplies out beta_i = A_i^-1 b_i, C_i = A_i^-1 X^T W_i, s_i = x_i^T C_i
+// (src/gwmodelpp/GWRBasic.cpp:209-220, 249-252, 285-290).  Here every local
+// quantity comes from the Cholesky factor of A_i without forming C_i (k x n):
+//   beta_i   = A^-1 b          (+ one FP64 iterative-refinement step)
+//   S_ii     = w_ii x_i^T A^-1 x_i
+//   sum_j S_ij^2 = g^T B g,    g = A^-1 x_i, B = X^T W_i^2 X
+//   SE_raw_a = (A^-1 B A^-1)_aa = rowsum(C_i .^ 2)_a
+// A failed Cholesky pivot (not > 0, or NaN) is the inv_sympd failure: the
+// criterion becomes DBL_MAX and the fit throws (GWRBasic.cpp:222-226, 254-258).
+#include "gwm_contract.h"
+
+namespace gwm {
+
+namespace {
+
+template <int K>
+struct Sym {
+    static constexpr int M = K * (K + 1) / 2;
+    // Z column of the pair (a, b), a <= b: pairs enumerated row by row
+    __device__ static constexpr int zcol(int a, int b) { return a * K - (a * (a - 1)) / 2 + (b - a); }
+    // packed lower storage of a symmetric matrix, element (i, j), j <= i
+    __device__ static constexpr int low(int i, int j) { return i * (i + 1) / 2 + j; }
+};
+
+template <int K>
+__device__ __forceinline__ double sum_parts(const SolveArgs& a, int c, int i) {
+    const size_t np = a.n_pad;
+    double s = 0.0;
+    for (int g = 0; g < a.nseg; ++g) s += a.part[((size_t)g * a.PT + c) * np + i];
+    return s;
+}
+
+// Cholesky A = L L^T in packed lower storage.  Returns false on a pivot that
+// is not positive (dpotrf's failure condition).
+template <int K>
+__device__ __forceinline__ bool cholesky(const double* A, double* L) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        double s = A[Sym<K>::low(j, j)];
+#pragma unroll
+        for (int p = 0; p < j; ++p) s -= L[Sym<K>::low(j, p)] * L[Sym<K>::low(j, p)];
+        if (!(s > 0.0)) return false;
+        double ljj = sqrt(s);
+        L[Sym<K>::low(j, j)] = ljj;
+#pragma unroll
+        for (int i = j + 1; i < K; ++i) {
+            double t = A[Sym<K>::low(i, j)];
+#pragma unroll
+            for (int p = 0; p < j; ++p) t -= L[Sym<K>::low(i, p)] * L[Sym<K>::low(j, p)];
+            L[Sym<K>::low(i, j)] = t / ljj;
+        }
+    }
+    return true;
+}
+
+// x = A^-1 v from the factor: L y = v, L^T x = y.
+template <int K>
+__device__ __forceinline__ void chol_solve(const double* L, const double* v, double* x) {
+    double y[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        double s = v[i];
+#pragma unroll
+        for (int p = 0; p < i; ++p) s -= L[Sym<K>::low(i, p)] * y[p];
+        y[i] = s / L[Sym<K>::low(i, i)];
+    }
+#pragma unroll
+    for (int i = K - 1; i >= 0; --i) {
+        double s = y[i];
+#pragma unroll
+        for (int p = i + 1; p < K; ++p) s -= L[Sym<K>::low(p, i)] * x[p];
+        x[i] = s / L[Sym<K>::low(i, i)];
+    }
+}
+
+template <int K>
+__device__ __forceinline__ double quad_form(const double* Bp, const double* g) {
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+        s += Bp[Sym<K>::low(a, a)] * g[a] * g[a];
+#pragma unroll
+        for (int b = 0; b < a; ++b) s += 2.0 * Bp[Sym<K>::low(a, b)] * g[a] * g[b];
+    }
+    return s;
+}
+
+template <int K, int MODE>
+__global__ __launch_bounds__(128) void solve_kernel(SolveArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int n = a.n;
+    constexpr int M = Sym<K>::M;
+
+    double A[M], L[M], b[K], beta[K], xi[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r)
+#pragma unroll
+        for (int c = 0; c <= r; ++c) A[Sym<K>::low(r, c)] = sum_parts<K>(a, Sym<K>::zcol(c, r), i);
+#pragma unroll
+    for (int r = 0; r < K; ++r) b[r] = sum_parts<K>(a, M + r, i);
+#pragma unroll
+    for (int r = 0; r < K; ++r) xi[r] = a.X[i + (size_t)r * n];
+
+    if (!cholesky<K>(A, L)) {
+        *a.fail = 1;
+        a.res2[i] = NAN;
+        if (MODE == kSolveAIC || MODE == kSolveFit) a.sii[i] = NAN;
+        return;
+    }
+    chol_solve<K>(L, b, beta);
+    {
+        // one step of iterative refinement on the local normal equation
+        double r[K], dlt[K];
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+            double s = b[p];
+#pragma unroll
+            for (int c = 0; c < K; ++c) s -= A[p >= c ? Sym<K>::low(p, c) : Sym<K>::low(c, p)] * beta[c];
+            r[p] = s;
+        }
+        chol_solve<K>(L, r, dlt);
+#pragma unroll
+        for (int p = 0; p < K; ++p) beta[p] += dlt[p];
+    }
+    // residual: y_i - sum_a beta_a x_a (GWRBase::Fitted order)
+    double yhat = 0.0;
+#pragma unroll
+    for (int p = 0; p < K; ++p) yhat += beta[p] * xi[p];
+    double res = a.y[i] - yhat;
+    a.res2[i] = res * res;
+    if (MODE == kSolveCV) return;
+    if (MODE == kSolveBeta) {
+#pragma unroll
+        for (int p = 0; p < K; ++p) a.betas[i + (size_t)p * n] = beta[p];
+        return;
+    }
+
+    const double bi = a.bw_pt ? a.bw_pt[i] : a.bw;
+    const double wii = kernel_f64(a.kernel, 0.0, bi);
+    double g[K];
+    chol_solve<K>(L, xi, g);
+    double xg = 0.0;
+#pragma unroll
+    for (int p = 0; p < K; ++p) xg += xi[p] * g[p];
+    a.sii[i] = wii * xg;
+    if (MODE == kSolveAIC) return;
+
+    // B = X^T W^2 X from the w^2 block; reuse A's storage
+    double* B = A;
+#pragma unroll
+    for (int r = 0; r < K; ++r)
+#pragma unroll
+        for (int c = 0; c <= r; ++c) B[Sym<K>::low(r, c)] = sum_parts<K>(a, a.colB + Sym<K>::zcol(c, r), i);
+    a.ssi[i] = quad_form<K>(B, g);
+#pragma unroll
+    for (int p = 0; p < K; ++p) a.betas[i + (size_t)p * n] = beta[p];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        double e[K], col[K];
+#pragma unroll
+        for (int p = 0; p < K; ++p) e[p] = p == c ? 1.0 : 0.0;
+        chol_solve<K>(L, e, col);
+        a.se_raw[i + (size_t)c * n] = quad_form<K>(B, col);
+    }
+#pragma unroll
+    for (int p = 0; p < K; ++p) a.gvec[(size_t)i * K + p] = g[p];
+    double* Gi = a.G + (size_t)i * a.ldg;
+#pragma unroll
+    for (int p = 0; p < K; ++p)
+#pragma unroll
+        for (int q = p; q < K; ++q) Gi[Sym<K>::zcol(p, q)] = g[p] * g[q];
+}
+
+template <int K>
+__global__ __launch_bounds__(128) void qdiag_kernel(const double* part, int nseg, int PT, int n, int n_pad,
+                                                    const double* X, const double* sii, double* qdiag) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double x[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) x[p] = X[j + (size_t)p * n];
+    double q = 0.0;
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+        for (int b = a; b < K; ++b) {
+            const int c = Sym<K>::zcol(a, b);
+            double m = 0.0;
+            for (int g = 0; g < nseg; ++g) m += part[((size_t)g * PT + c) * n_pad + j];
+            q += (a == b ? 1.0 : 2.0) * m * x[a] * x[b];
+        }
+    qdiag[j] = 1.0 - 2.0 * sii[j] + q;
+}
+
+template <int DIST>
+__global__ void smatrix_kernel(Points P, int k, const double* X, const double* gvec, const double* bw_pt, double bw,
+                               int kernel, double* S) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    const int n = P.n;
+    if (j >= n) return;
+    double d = distance_f64<DIST>(P, i, j);
+    double w = kernel_f64(kernel, d, bw_pt ? bw_pt[i] : bw);
+    double s = 0.0;
+    for (int p = 0; p < k; ++p) s += gvec[(size_t)i * k + p] * X[j + (size_t)p * n];
+    S[i + (size_t)j * n] = w * s;
+}
+
+// Deterministic fixed-order sums (one workgroup).
+__global__ __launch_bounds__(1024) void reduce3_kernel(const double* a, const double* b, const double* c, int n,
+                                                       double* out) {
+    __shared__ double sa[1024], sb[1024], sc[1024];
+    double xa = 0.0, xb = 0.0, xc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        if (a) xa += a[i];
+        if (b) xb += b[i];
+        if (c) xc += c[i];
+    }
+    sa[threadIdx.x] = xa;
+    sb[threadIdx.x] = xb;
+    sc[threadIdx.x] = xc;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+            sc[threadIdx.x] += sc[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sa[0];
+        out[1] = sb[0];
+        out[2] = sc[0];
+    }
+}
+
+// Single-column solves (GWRMultiscale.cpp:420-482, 560-636).
+__global__ __launch_bounds__(256) void solve_var_kernel(int mode, const double* part, int nseg, int PT, int colB,
+                                                        int n, int n_pad, const double* x, const double* y,
+                                                        const double* bw_pt, double bw, int kernel, double* res2,
+                                                        double* sii, double* beta, int* fail) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = 0.0, b = 0.0;
+    for (int g = 0; g < nseg; ++g) {
+        a += part[((size_t)g * PT + 0) * n_pad + i];
+        b += part[((size_t)g * PT + 1) * n_pad + i];
+    }
+    (void)colB;
+    if (!(a > 0.0)) {
+        *fail = 1;
+        if (res2) res2[i] = NAN;
+        return;
+    }
+    const double inv = 1.0 / a;
+    const double bt = inv * b;
+    if (beta) beta[i] = bt;
+    if (res2) {
+        double r = y[i] - x[i] * bt;
+        res2[i] = r * r;
+    }
+    if (mode != kSolveCV && sii) {
+        const double bi = bw_pt ? bw_pt[i] : bw;
+        sii[i] = x[i] * (inv * (x[i] * kernel_f64(kernel, 0.0, bi)));
+    }
+}
+
+template <int K>
+hipError_t launch_solve_k(int mode, const SolveArgs& a, hipStream_t s) {
+    dim3 grid((a.n + 127) / 128);
+    if (mode == kSolveCV) hipLaunchKernelGGL((solve_kernel<K, kSolveCV>), grid, dim3(128), 0, s, a);
+    else if (mode == kSolveAIC) hipLaunchKernelGGL((solve_kernel<K, kSolveAIC>), grid, dim3(128), 0, s, a);
+    else if (mode == kSolveBeta) hipLaunchKernelGGL((solve_kernel<K, kSolveBeta>), grid, dim3(128), 0, s, a);
+    else hipLaunchKernelGGL((solve_kernel<K, kSolveFit>), grid, dim3(128), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int K>
+hipError_t launch_qdiag_k(const double* part, int nseg, int PT, int n, int n_pad, const double* X,
+                          const double* sii, double* q, hipStream_t s) {
+    hipLaunchKernelGGL(qdiag_kernel<K>, dim3((n + 127) / 128), dim3(128), 0, s, part, nseg, PT, n, n_pad, X, sii, q);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+#define GWM_K_SWITCH(k, CALL)                                                                          \
+    switch (k) {                                                                                       \
+    case 1: return CALL(1); case 2: return CALL(2); case 3: return CALL(3); case 4: return CALL(4);    \
+    case 5: return CALL(5); case 6: return CALL(6); case 7: return CALL(7); case 8: return CALL(8);    \
+    case 9: return CALL(9); case 10: return CALL(10); case 11: return CALL(11); case 12: return CALL(12); \
+    case 13: return CALL(13); case 14: return CALL(14); case 15: return CALL(15); case 16: return CALL(16); \
+    default: return hipErrorInvalidValue;                                                              \
+    }
+
+hipError_t launch_solve(int k, int mode, const SolveArgs& a, hipStream_t s) {
+#define GWM_SOLVE(K) launch_solve_k<K>(mode, a, s)
+    GWM_K_SWITCH(k, GWM_SOLVE)
+#undef GWM_SOLVE
+}
+
+hipError_t launch_qdiag(int k, const double* part, int nseg, int PT, int n, int n_pad, const double* X,
+                        const double* sii, double* qdiag, hipStream_t s) {
+#define GWM_QD(K) launch_qdiag_k<K>(part, nseg, PT, n, n_pad, X, sii, qdiag, s)
+    GWM_K_SWITCH(k, GWM_QD)
+#undef GWM_QD
+}
+
+hipError_t launch_smatrix(int dist, int k, const Points& P, const double* X, const double* gvec,
+                          const double* bw_pt, double bw, int kernel, double* S, hipStream_t s) {
+    dim3 grid((P.n + 255) / 256, P.n);
+    if (dist == kEuclidean)
+        hipLaunchKernelGGL(smatrix_kernel<kEuclidean>, grid, dim3(256), 0, s, P, k, X, gvec, bw_pt, bw, kernel, S);
+    else
+        hipLaunchKernelGGL(smatrix_kernel<kGreatCircle>, grid, dim3(256), 0, s, P, k, X, gvec, bw_pt, bw, kernel, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double* a, const double* b, const double* c, int n, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce3_kernel, dim3(1), dim3(1024), 0, s, a, b, c, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_solve_var(int mode, const double* part, int nseg, int PT, int colB, int n, int n_pad,
+                            const double* x, const double* y, const double* bw_pt, double bw, int kernel,
+                            double* res2, double* sii, double* beta, int* fail, hipStream_t s) {
+    hipLaunchKernelGGL(solve_var_kernel, dim3((n + 255) / 256), dim3(256), 0, s, mode, part, nseg, PT, colB, n, n_pad,
+                       x, y, bw_pt, bw, kernel, res2, sii, beta, fail);
+    return hipGetLastError();
+}
+
+}  // namespace gwm

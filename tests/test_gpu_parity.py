@@ -1,0 +1,240 @@
+"""GPU parity of the FP64 HIP path against the CPU oracle (calls go through the
+C ABI of include/gwmodel_hip.h).
+
+Tolerances (FP64 mode): bandwidths, adaptive thresholds, Euclidean distances
+and golden-section iterates bit-exact; criteria within 1e-11 relative; beta,
+SE, qDiag, S within 1e-9 relative; reference KAT diagnostics at the
+reference's own tolerances (1e-8 GWRBasic, 1e-6 GWRMultiscale).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from libgwmodel_amd import capi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(a, b, floor=1e-300):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+def close(a, b, rtol, atol=0.0):
+    a, b = np.asarray(a), np.asarray(b)
+    return bool(np.all(np.abs(a - b) <= rtol * np.abs(b) + atol))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    say("create context")
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+# ------------------------------------------------------------------ distances
+
+def test_euclidean_distances_bitexact(ctx, oracle, londonhp100):
+    say("case: euclidean distances londonhp100")
+    ctx.set_data(londonhp100["coords"], londonhp100["X"], londonhp100["y"])
+    for f in [0, 1, 57, 99]:
+        g = ctx.distance_row(f)
+        o = oracle.distance_vec(londonhp100["coords"], f)
+        assert np.array_equal(g, o)
+
+
+def test_great_circle_distances(ctx, oracle):
+    say("case: great-circle distances")
+    d = synth.config2(n=500)
+    ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
+    for f in [0, 3, 250, 499]:
+        g = ctx.distance_row(f)
+        o = oracle.distance_vec(d["coords"], f, geographic=True)
+        assert close(g, o, 1e-12, 1e-9)
+    m = ctx.max_distance()
+    assert m == pytest.approx(oracle.max_distance(d["coords"], True), rel=1e-12)
+
+
+def test_max_distance_euclidean_bitexact(ctx, oracle, londonhp100):
+    say("case: max distance")
+    ctx.set_data(londonhp100["coords"], londonhp100["X"], londonhp100["y"])
+    assert ctx.max_distance() == oracle.max_distance(londonhp100["coords"])
+
+
+# ------------------------------------------------------------------ adaptive kNN
+
+@pytest.mark.parametrize("n,seed", [(100, 0), (1000, 1), (3000, 2)])
+def test_adaptive_thresholds_bitexact(ctx, oracle, n, seed):
+    say("case: adaptive thresholds n=%d" % n)
+    rng = np.random.default_rng(seed)
+    coords = np.round(rng.uniform(0, 30, (n, 2)), 1)   # many exact ties
+    coords[5] = coords[6]                               # coincident points
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    y = rng.normal(size=n)
+    ctx.set_data(coords, X, y)
+    for bw in [1.0, 2.0, 20.0, 36.5, n - 1.0, n - 0.25, float(n), 1.5 * n]:
+        g = ctx.adaptive_bandwidths(bw)
+        o = np.array([oracle.adaptive_threshold(coords, bw, f) for f in range(0, n, max(1, n // 97))])
+        assert np.array_equal(g[::max(1, n // 97)], o), "bw=%r" % bw
+
+
+# ------------------------------------------------------------------ criteria and fit
+
+CASES = [
+    # kernel, adaptive, bandwidth
+    (capi.GAUSSIAN, False, 15.0),
+    (capi.GAUSSIAN, True, 60.0),
+    (capi.BISQUARE, True, 45.0),
+    (capi.BISQUARE, False, 30.0),
+    (capi.TRICUBE, True, 80.5),
+    (capi.EXPONENTIAL, False, 10.0),
+    (capi.BOXCAR, False, 35.0),
+]
+
+
+@pytest.mark.parametrize("kernel,adaptive,bw", CASES)
+def test_criteria_match_oracle(ctx, oracle, kernel, adaptive, bw):
+    say("case: criteria kernel=%d adaptive=%d bw=%g" % (kernel, adaptive, bw))
+    d = synth.config1(n=400)
+    ctx.set_data(d["coords"], d["X"], d["y"])
+    for crit in [capi.CRIT_CV, capi.CRIT_AIC]:
+        g = ctx.criterion(kernel, adaptive, bw, crit)
+        o = oracle.criterion(d["coords"], d["X"], d["y"], kernel, adaptive, bw, crit)
+        assert g == pytest.approx(o, rel=1e-11), (crit, g, o)
+
+
+@pytest.mark.parametrize("kernel,adaptive,bw", CASES)
+def test_fit_matches_oracle(ctx, oracle, kernel, adaptive, bw):
+    say("case: fit kernel=%d adaptive=%d bw=%g" % (kernel, adaptive, bw))
+    d = synth.config1(n=300)
+    ctx.set_data(d["coords"], d["X"], d["y"])
+    g = ctx.fit(kernel, adaptive, bw, want_S=True)
+    o = oracle.fit_core(d["coords"], d["X"], d["y"], kernel, adaptive, bw, want_S=True)
+    assert close(g["betas"], o["betas"], 1e-9, 1e-12)
+    assert close(g["betasSE_raw"], o["betasSE_raw"], 1e-9, 1e-15)
+    assert close(g["shat"], o["shat"], 1e-10)
+    assert close(g["qDiag"], o["qDiag"], 1e-9, 1e-12)
+    assert close(g["S"], o["S"], 1e-9, 1e-12)
+
+
+@pytest.mark.parametrize("k", [1, 2, 5, 8, 12, 16])
+def test_fit_all_k(ctx, oracle, k):
+    say("case: fit k=%d" % k)
+    rng = np.random.default_rng(100 + k)
+    n = 257
+    coords = rng.uniform(0, 50, (n, 2))
+    X = np.column_stack([np.ones(n)] + [rng.normal(size=n) for _ in range(k - 1)]) if k > 1 else np.ones((n, 1))
+    y = X @ rng.normal(size=k) + rng.normal(size=n)
+    ctx.set_data(coords, X, y)
+    g = ctx.fit(capi.GAUSSIAN, True, 120.0)
+    o = oracle.fit_core(coords, X, y, capi.GAUSSIAN, True, 120.0)
+    assert close(g["betas"], o["betas"], 1e-9, 1e-11)
+    assert close(g["shat"], o["shat"], 1e-10)
+    assert close(g["qDiag"], o["qDiag"], 1e-9, 1e-12)
+    cv = ctx.criterion(capi.BISQUARE, True, 90.0, capi.CRIT_CV)
+    assert cv == pytest.approx(oracle.criterion(coords, X, y, capi.BISQUARE, True, 90.0, capi.CRIT_CV), rel=1e-11)
+
+
+# ------------------------------------------------------------------ reference KATs through the C ABI
+
+def test_kat_adaptive_gaussian_36(ctx, londonhp100):
+    say("case: KAT adaptive gaussian 36")
+    ctx.set_data(londonhp100["coords"], londonhp100["X"], londonhp100["y"])
+    d = ctx.gwr_run(capi.GAUSSIAN, True, 36)["diagnostic"]
+    assert abs(d["AIC"] - 2436.60445730413) < 1e-8
+    assert abs(d["AICc"] - 2448.27206524754) < 1e-8
+    assert abs(d["RSquare"] - 0.708010632044736) < 1e-8
+    assert abs(d["RSquareAdjust"] - 0.674975341723766) < 1e-8
+
+
+def test_kat_fixed_gaussian_5000(ctx, londonhp100):
+    say("case: KAT fixed gaussian 5000")
+    ctx.set_data(londonhp100["coords"], londonhp100["X"], londonhp100["y"])
+    d = ctx.gwr_run(capi.GAUSSIAN, False, 5000)["diagnostic"]
+    assert abs(d["AIC"] - 2437.649574267587) < 1e-8
+    assert abs(d["AICc"] - 2447.676281164379) < 1e-8
+    assert abs(d["RSquare"] - 0.701466295457) < 1e-8
+    assert abs(d["RSquareAdjust"] - 0.673691594464) < 1e-8
+
+
+def test_kat_cv_golden_section(ctx, oracle, londonhp100):
+    say("case: KAT CV golden section 67")
+    ctx.set_data(londonhp100["coords"], londonhp100["X"], londonhp100["y"])
+    r = ctx.gwr_run(capi.GAUSSIAN, True, 0, select_bw=True, crit=capi.CRIT_CV)
+    assert r["bandwidth"] == 67
+    o = oracle.gwr_run(londonhp100["coords"], londonhp100["X"], londonhp100["y"], capi.GAUSSIAN, True, 0,
+                       select_bw=True, crit=oracle.CRIT_CV)
+    assert [x for x, _ in r["trace"]] == [x for x, _ in o["trace"]]
+    assert close([f for _, f in r["trace"]], [f for _, f in o["trace"]], 1e-11)
+
+
+def test_kat_mgwr_no_hat(ctx, londonhp100):
+    say("case: KAT MGWR adaptive CV no hat")
+    ctx.set_data(londonhp100["coords"], londonhp100["Xm"], londonhp100["y"])
+    r = ctx.mgwr_run([capi.BISQUARE] * 3, [1] * 3, [0] * 3, [0] * 3, [capi.CRIT_CV] * 3, [0, 1, 1], [1e-5] * 3)
+    assert list(r["bandwidths"]) == [35, 98, 98]
+    assert abs(r["diagnostic"]["RSquare"] - 0.757377391669) < 1e-6
+
+
+# ------------------------------------------------------------------ searches vs oracle
+
+def test_config1_cv_search_matches_oracle(ctx, oracle):
+    say("case: config1 n=600 bisquare adaptive CV search")
+    d = synth.config1(n=600)
+    ctx.set_data(d["coords"], d["X"], d["y"])
+    g = ctx.gwr_run(capi.BISQUARE, True, 0, select_bw=True, crit=capi.CRIT_CV, lower=20, upper=600)
+    o = oracle.gwr_run(d["coords"], d["X"], d["y"], oracle.BISQUARE, True, 0, select_bw=True, crit=oracle.CRIT_CV,
+                       lower=20, upper=600)
+    assert g["bandwidth"] == o["bandwidth"]
+    assert [x for x, _ in g["trace"]] == [x for x, _ in o["trace"]]
+    for key in ["AICc", "RSquare", "RSquareAdjust"]:
+        assert g["diagnostic"][key] == pytest.approx(o["diagnostic"][key], rel=1e-10)
+    assert close(g["betas"], o["betas"], 1e-9, 1e-12)
+    assert close(g["betasSE"], o["betasSE"], 1e-9, 1e-12)
+
+
+def test_config2_aicc_search_matches_oracle(ctx, oracle):
+    say("case: config2 n=800 great-circle gaussian AICc search")
+    d = synth.config2(n=800)
+    lo, hi = synth.CONFIG2_BW_KM
+    ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
+    g = ctx.gwr_run(capi.GAUSSIAN, False, 0, select_bw=True, crit=capi.CRIT_AIC, lower=lo, upper=hi)
+    o = oracle.gwr_run(d["coords"], d["X"], d["y"], oracle.GAUSSIAN, False, 0, select_bw=True, crit=oracle.CRIT_AIC,
+                       lower=lo, upper=hi, geographic=True)
+    assert g["bandwidth"] == o["bandwidth"]
+    assert [x for x, _ in g["trace"]] == [x for x, _ in o["trace"]]
+    assert close(g["betas"], o["betas"], 1e-9, 1e-11)
+    assert g["diagnostic"]["AICc"] == pytest.approx(o["diagnostic"]["AICc"], rel=1e-11)
+
+
+# ------------------------------------------------------------------ failures
+
+def test_not_spd_and_cancel(ctx):
+    say("case: not SPD / cancel")
+    rng = np.random.default_rng(9)
+    n = 200
+    coords = rng.uniform(0, 10, (n, 2))
+    x1 = rng.normal(size=n)
+    X = np.column_stack([np.ones(n), x1, x1])   # collinear -> singular local designs
+    y = rng.normal(size=n)
+    ctx.set_data(coords, X, y)
+    assert ctx.criterion(capi.GAUSSIAN, False, 3.0, capi.CRIT_CV) == capi.DBL_MAX
+    with pytest.raises(capi.GwmError) as e:
+        ctx.fit(capi.GAUSSIAN, False, 3.0)
+    assert e.value.status == capi.GWM_ERR_NOT_SPD
+    X2 = np.column_stack([np.ones(n), x1])
+    ctx.set_data(coords, X2, y)
+    ctx.set_stop_callback(lambda: True)
+    with pytest.raises(capi.GwmError) as e:
+        ctx.select_bandwidth(capi.GAUSSIAN, True, capi.CRIT_CV)
+    assert e.value.status == capi.GWM_ERR_CANCELLED
+    ctx.set_stop_callback(None)
+    bw, _ = ctx.select_bandwidth(capi.GAUSSIAN, True, capi.CRIT_CV)
+    assert 20 <= bw <= n
