@@ -129,9 +129,15 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* ctx, const int* kernels, const int* adaptive, 
                      int adaptive_lower, int has_lower, double lower, int has_upper, double upper, int precision,
                      double* betas, double* bws_out, double* diag, double* rss_out, int* iterations_out);
 
-/* Timing of the contraction kernels with HIP events on the context stream
- * (enabled by gwm_hip_set_timing).  name: "contract" (all sweeps),
- * "contract_fp64", "contract_mixed", "select", "solve". */
+/* Mixed precision: relative error bound used to certify the golden-section
+ * decisions taken on mixed-precision criteria (default 1e-6); a decision
+ * closer than the bound is re-taken on FP64 re-evaluations.  No counterpart in
+ * the reference, which has no mixed-precision path. */
+int gwm_hip_set_certify_tolerance(gwm_hip_ctx* ctx, double rel_tol);
+
+/* Timing of the device kernels with HIP events on the context stream
+ * (enabled by gwm_hip_set_timing).  name: "contract_<fp64|mixed>_<cv|plain|
+ * hat|qdiag>" for the sweeps, "select" (adaptive bandwidths), "solve". */
 int gwm_hip_set_timing(gwm_hip_ctx* ctx, int enabled);
 int gwm_hip_kernel_time(gwm_hip_ctx* ctx, const char* name, long long* launches, double* total_ms);
 int gwm_hip_reset_timers(gwm_hip_ctx* ctx);

@@ -17,6 +17,10 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 OBJ_DIR = os.path.join(OUT_DIR, "obj")
 LIB = os.path.join(OUT_DIR, "libgwmodel_hip.so")
+FACADE_LIB = os.path.join(OUT_DIR, "libgwmodelpp_hip.so")
+FACADE_SRC = os.path.join(HERE, "cpp", "gwr_facade.cpp")
+FACADE_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
+FACADE_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_facade")
 
 SOURCES = ["gwm_contract.hip", "gwm_contract_mixed.hip", "gwm_select.hip", "gwm_solve.hip", "gwm_api.cpp"]
 
@@ -55,9 +59,30 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
     os.replace(tmp, LIB)
+    build_facade()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, FACADE_LIB, FACADE_TEST_BIN)
     return LIB
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("%s\n%s" % (" ".join(cmd), r.stderr))
+
+
+def build_facade():
+    """The C++ host interface (include/gwmodelpp/gwmodel.h) is host-only C++
+    over the C ABI: g++, linked against libgwmodel_hip.so, plus its KAT
+    program tests/cpp/test_facade."""
+    inc = "-I" + os.path.join(ROOT, "include")
+    tmp = FACADE_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", inc, FACADE_SRC, "-o", tmp, "-L" + OUT_DIR,
+          "-lgwmodel_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FACADE_LIB)
+    os.makedirs(os.path.dirname(FACADE_TEST_BIN), exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", inc, FACADE_TEST_SRC, "-o", FACADE_TEST_BIN, "-L" + OUT_DIR,
+          "-lgwmodelpp_hip", "-lgwmodel_hip", "-Wl,-rpath,$ORIGIN/../../../libgwmodel_amd/lib"])
 
 
 if __name__ == "__main__":

@@ -51,6 +51,7 @@ SIGNATURES = {
     "gwm_hip_var_fit": (_i, [_vp, _dp, _dp, _i, _i, _d, _i, _dp]),
     "gwm_hip_mgwr_run": (_i, [_vp, _ip, _ip, _dp, _ip, _ip, _ip, _dp, _i, _i, _d, _i, _i, _i, _i, _d, _i, _d, _i,
                               _dp, _dp, _dp, _dp, _ip]),
+    "gwm_hip_set_certify_tolerance": (_i, [_vp, _d]),
     "gwm_hip_set_timing": (_i, [_vp, _i]),
     "gwm_hip_kernel_time": (_i, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong), _dp]),
     "gwm_hip_reset_timers": (_i, [_vp]),
@@ -250,6 +251,9 @@ class Context:
             _p(betas), _p(bws_out), _p(diag), _p(rss), _ipa(iters)))
         return dict(betas=betas, bandwidths=bws_out, rss=float(rss[0]), iterations=int(iters[0]),
                     diagnostic=dict(zip(["RSS", "AIC", "AICc", "ENP", "EDF", "RSquare", "RSquareAdjust"], diag)))
+
+    def set_certify_tolerance(self, rel):
+        self._check(self._lib.gwm_hip_set_certify_tolerance(self.h, float(rel)))
 
     def set_timing(self, on=True):
         self._check(self._lib.gwm_hip_set_timing(self.h, int(on)))

@@ -1,0 +1,153 @@
+// C++ host interface (include/gwmodelpp/gwmodel.h) over the C ABI of
+// include/gwmodel_hip.h.  Behaviour follows the reference's GWRBasic::fit
+// (src/gwmodelpp/GWRBasic.cpp:36-128) and GWRMultiscale::fit
+// (src/gwmodelpp/GWRMultiscale.cpp:68-197): bandwidth selection bounds,
+// criterion list, diagnostics, SE scaling, Status::Terminated with zero
+// matrices on cancellation, exceptions on non-SPD local designs and on
+// invalid golden-section starting values.
+#include "gwmodelpp/gwmodel.h"
+
+#include <cfloat>
+#include <string>
+
+#include "gwmodel_hip.h"
+
+namespace gwm {
+
+namespace {
+
+int stop_adapter(void* user) { return static_cast<ITelegram*>(user)->stop() ? 1 : 0; }
+
+[[noreturn]] void raise(gwm_hip_ctx* ctx, int rc) {
+    std::string msg = ctx ? gwm_hip_last_error(ctx) : "";
+    switch (rc) {
+    case GWM_ERR_NOT_SPD: throw std::runtime_error("inv_sympd(): matrix is singular or not positive definite");
+    case GWM_ERR_INVALID_INITIAL: throw std::runtime_error("Invalid initial values.");
+    case GWM_ERR_INVALID: throw std::invalid_argument(msg);
+    case GWM_ERR_OOM: throw std::bad_alloc();
+    case GWM_ERR_NO_DEVICE: throw std::runtime_error("no HIP device available");
+    default: throw std::runtime_error("HIP back-end error: " + msg);
+    }
+}
+
+int distance_id(const SpatialWeight& sw) {
+    if (!sw.isValid() || sw.distance()->type() != Distance::CRSDistance)
+        throw std::invalid_argument("the HIP back-end supports CRSDistance only");
+    return sw.distance<CRSDistance>()->geographic() ? GWM_GREAT_CIRCLE : GWM_EUCLIDEAN;
+}
+
+}  // namespace
+
+HipContext::HipContext(int device) {
+    int rc = gwm_hip_create(device, &mCtx);
+    if (rc != GWM_OK) raise(nullptr, rc);
+}
+
+HipContext::~HipContext() { gwm_hip_destroy(mCtx); }
+
+Mat GWRBasic::fit() {
+    mStatus = Status::Success;
+    const std::size_t n = mCoords.n_rows(), k = mX.n_cols();
+    if (mParallelType != ParallelType::HIP) throw std::logic_error("parallel type not available in this build");
+    if (mCoords.n_cols() != 2 || mX.n_rows() != n || mY.n_elem() != n)
+        throw std::invalid_argument("coords must be n x 2, X n x k, y n");
+    const int dist = distance_id(mSpatialWeight);
+    BandwidthWeight* bw = mSpatialWeight.weight<BandwidthWeight>();
+    HipContext ctx(mGpuId);
+    gwm_hip_set_stop_callback(ctx.get(), &stop_adapter, mTelegram);
+    int rc = gwm_hip_set_data(ctx.get(), (int)n, (int)k, mCoords.memptr(), mX.memptr(), mY.memptr(), dist);
+    if (rc) raise(ctx.get(), rc);
+    const int prec = mPrecision == Precision::Mixed ? GWM_MIXED : GWM_FP64;
+    const int crit = mBandwidthSelectionCriterion == CV ? GWM_CRIT_CV : GWM_CRIT_AIC;
+
+    mBetas = Mat(n, k);
+    mBetasSE = Mat(n, k);
+    mSHat = Vec(2, 1);
+    mQDiag = Vec(n, 1);
+    mS = isStoreS() ? Mat(n, n) : Mat(1, n);
+    double diag[7] = {0};
+    double bw_out = bw->bandwidth();
+    std::vector<double> tx(512), tf(512);
+    int tlen = 0;
+    rc = gwm_hip_gwr_run(ctx.get(), (int)bw->kernel(), bw->adaptive() ? 1 : 0, bw->bandwidth(),
+                         mIsAutoselectBandwidth ? 1 : 0, crit, prec, mGoldenLowerBounds.has_value(),
+                         mGoldenLowerBounds.value_or(0.0), mGoldenUpperBounds.has_value(),
+                         mGoldenUpperBounds.value_or(0.0), mBetas.memptr(), mBetasSE.memptr(), mSHat.memptr(),
+                         mQDiag.memptr(), isStoreS() ? mS.memptr() : nullptr, diag, &bw_out, tx.data(), tf.data(),
+                         (int)tx.size(), &tlen);
+    if (rc == GWM_ERR_CANCELLED) {
+        mStatus = Status::Terminated;
+        mBetas = Mat(n, k);
+        return mBetas;
+    }
+    if (rc) raise(ctx.get(), rc);
+    if (mIsAutoselectBandwidth) {
+        bw->setBandwidth(bw_out);
+        mBandwidthSelectionCriterionList.clear();
+        for (int i = 0; i < tlen && i < (int)tx.size(); ++i) mBandwidthSelectionCriterionList.emplace_back(tx[i], tf[i]);
+    }
+    mDiagnostic = RegressionDiagnostic{diag[0], diag[1], diag[2], diag[3], diag[4], diag[5], diag[6]};
+    return mBetas;
+}
+
+void GWRMultiscale::setBandwidthInitilize(const std::vector<BandwidthInitilizeType>& v) {
+    if (v.size() != mX.n_cols()) throw std::length_error("BandwidthInitilize size do not match indepvars");
+    mBandwidthInitilize = v;
+}
+
+void GWRMultiscale::setBandwidthSelectionApproach(const std::vector<BandwidthSelectionCriterionType>& v) {
+    if (v.size() != mX.n_cols()) throw std::length_error("bandwidthSelectionApproach size do not match indepvars");
+    mBandwidthSelectionApproach = v;
+}
+
+Mat GWRMultiscale::fit() {
+    mStatus = Status::Success;
+    const std::size_t n = mCoords.n_rows(), k = mX.n_cols();
+    if (mParallelType != ParallelType::HIP) throw std::logic_error("parallel type not available in this build");
+    if (mCoords.n_cols() != 2 || mX.n_rows() != n || mY.n_elem() != n)
+        throw std::invalid_argument("coords must be n x 2, X n x k, y n");
+    if (mSpatialWeights.size() != k || mBandwidthInitilize.size() != k || mBandwidthSelectionApproach.size() != k ||
+        mPreditorCentered.size() != k || mBandwidthSelectThreshold.size() != k)
+        throw std::invalid_argument("per-variable settings must have one entry per independent variable");
+    if (mHasHatMatrix)
+        throw std::logic_error("the HIP back-end runs GWRMultiscale without the hat matrix: call setHasHatMatrix(false)");
+    const int dist = distance_id(mSpatialWeights[0]);
+    std::vector<int> kernels(k), adaptive(k), init(k), approach(k), centered(k);
+    std::vector<double> bws(k);
+    for (std::size_t i = 0; i < k; ++i) {
+        if (distance_id(mSpatialWeights[i]) != dist) throw std::invalid_argument("all distances must be alike");
+        const BandwidthWeight* w = mSpatialWeights[i].weight<BandwidthWeight>();
+        kernels[i] = (int)w->kernel();
+        adaptive[i] = w->adaptive() ? 1 : 0;
+        bws[i] = w->bandwidth();
+        init[i] = (int)mBandwidthInitilize[i];
+        approach[i] = mBandwidthSelectionApproach[i] == CV ? GWM_CRIT_CV : GWM_CRIT_AIC;
+        centered[i] = mPreditorCentered[i] ? 1 : 0;
+    }
+    HipContext ctx(mGpuId);
+    gwm_hip_set_stop_callback(ctx.get(), &stop_adapter, mTelegram);
+    int rc = gwm_hip_set_data(ctx.get(), (int)n, (int)k, mCoords.memptr(), mX.memptr(), mY.memptr(), dist);
+    if (rc) raise(ctx.get(), rc);
+    mBetas = Mat(n, k);
+    std::vector<double> bws_out(k);
+    double diag[7] = {0}, rss = 0.0;
+    int iters = 0;
+    rc = gwm_hip_mgwr_run(ctx.get(), kernels.data(), adaptive.data(), bws.data(), init.data(), approach.data(),
+                          centered.data(), mBandwidthSelectThreshold.data(), mHasIntercept ? 1 : 0,
+                          mCriterionType == CVR ? 0 : 1, mCriterionThreshold, (int)mMaxIteration,
+                          (int)mBandwidthSelectRetryTimes, mAdaptiveLower, mGoldenLowerBounds.has_value(),
+                          mGoldenLowerBounds.value_or(0.0), mGoldenUpperBounds.has_value(),
+                          mGoldenUpperBounds.value_or(0.0), mPrecision == Precision::Mixed ? GWM_MIXED : GWM_FP64,
+                          mBetas.memptr(), bws_out.data(), diag, &rss, &iters);
+    if (rc == GWM_ERR_CANCELLED) {
+        mStatus = Status::Terminated;
+        mBetas = Mat(n, k);
+        return mBetas;
+    }
+    if (rc) raise(ctx.get(), rc);
+    for (std::size_t i = 0; i < k; ++i) mSpatialWeights[i].weight<BandwidthWeight>()->setBandwidth(bws_out[i]);
+    mDiagnostic = RegressionDiagnostic{diag[0], diag[1], diag[2], diag[3], diag[4], diag[5], diag[6]};
+    return mBetas;
+}
+
+}  // namespace gwm

@@ -58,6 +58,12 @@ struct gwm_hip_ctx {
     double *vx = nullptr, *vy = nullptr, *vZ = nullptr;
     double* S = nullptr;
     int* fail = nullptr;
+    // FP16 hi/lo planes for the mixed-precision sweeps (Z, G and the
+    // single-column table) with their per-column unscale factors
+    _Float16 *Zh = nullptr, *Zl = nullptr, *Gh = nullptr, *Gl = nullptr, *vZh = nullptr, *vZl = nullptr;
+    double *zscale = nullptr, *gscale = nullptr, *vscale = nullptr;
+    unsigned long long* colmax = nullptr;
+    double certify_rel = 1e-6;   // relative bound used to certify mixed comparisons
 
     // timing
     bool timing = false;
@@ -101,6 +107,8 @@ void free_problem(gwm_hip_ctx* c) {
     dfree(c->bwi); dfree(c->res2); dfree(c->sii); dfree(c->ssi); dfree(c->betas); dfree(c->se);
     dfree(c->gvec); dfree(c->qd); dfree(c->red); dfree(c->blockmax); dfree(c->vx); dfree(c->vy);
     dfree(c->vZ); dfree(c->S); dfree(c->fail);
+    dfree(c->Zh); dfree(c->Zl); dfree(c->Gh); dfree(c->Gl); dfree(c->vZh); dfree(c->vZl);
+    dfree(c->zscale); dfree(c->gscale); dfree(c->vscale); dfree(c->colmax);
     c->part_bytes = 0;
     c->n = 0;
 }
@@ -166,43 +174,75 @@ int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw
     return GWM_OK;
 }
 
-// One sweep of the fused contraction over the context's design.
-int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const double* Z, const double* bw_pt, double bw, int kernel,
+// A product table in both representations.
+struct ZPlanes {
+    const double* z64;
+    const _Float16* zh;
+    const _Float16* zl;
+    const double* scale;
+};
+
+const char* mode_name(int mode) {
+    switch (mode) {
+    case kModeCV: return "cv";
+    case kModeHat: return "hat";
+    case kModeQDiag: return "qdiag";
+    default: return "plain";
+    }
+}
+
+// One sweep of the fused contraction.
+int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const double* bw_pt, double bw, int kernel,
           int precision, int* PT_out, int* nseg_out) {
-    int tile_focus = contract_tile_focus(nta, ntb);
+    const bool mixed = precision == GWM_MIXED;
+    int tile_focus = mixed ? contract_mixed_tile_focus(nta, ntb) : contract_tile_focus(nta, ntb);
     int nseg, seg_len;
     plan_segments(c->n, tile_focus, nseg, seg_len);
     const int PT = (nta + ntb) * 16;
     int rc = ensure_part(c, (size_t)nseg * PT * c->n_pad * sizeof(double));
     if (rc) return rc;
-    if (precision != GWM_FP64) {
-        c->err = "mixed precision is not available in this build";
-        return GWM_ERR_INVALID;
-    }
-    ContractArgs a;
-    a.P = c->points();
-    a.Z = Z;
-    a.bw_pt = bw_pt;
-    a.bw = bw;
-    a.kernel = kernel;
-    a.n = c->n;
-    a.n_pad = c->n_pad;
-    a.seg_len = seg_len;
-    a.nseg = nseg;
-    a.out = c->part;
     timer_begin(c);
-    GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, a, c->stream));
-    rc = timer_end(c, "contract_fp64");
-    if (rc) return rc;
+    if (mixed) {
+        ContractMixedArgs a;
+        a.P = c->points();
+        a.Zh = zp.zh;
+        a.Zl = zp.zl;
+        a.colscale = zp.scale;
+        a.bw_pt = bw_pt;
+        a.bw = bw;
+        a.kernel = kernel;
+        a.n = c->n;
+        a.n_pad = c->n_pad;
+        a.seg_len = seg_len;
+        a.nseg = nseg;
+        a.out = c->part;
+        GWM_HIP_CHECK(c, launch_contract_mixed(c->dist, mode, nta, ntb, a, c->stream));
+    } else {
+        ContractArgs a;
+        a.P = c->points();
+        a.Z = zp.z64;
+        a.bw_pt = bw_pt;
+        a.bw = bw;
+        a.kernel = kernel;
+        a.n = c->n;
+        a.n_pad = c->n_pad;
+        a.seg_len = seg_len;
+        a.nseg = nseg;
+        a.out = c->part;
+        GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, a, c->stream));
+    }
     if (c->timing) {
-        auto& s = c->stats["contract"];
-        s.launches += 1;
-        s.ms += 0.0;   // the per-precision entry carries the time
+        std::string name = std::string("contract_") + (mixed ? "mixed_" : "fp64_") + mode_name(mode);
+        if ((rc = timer_end(c, name.c_str()))) return rc;
     }
     *PT_out = PT;
     *nseg_out = nseg;
     return GWM_OK;
 }
+
+ZPlanes design_planes(gwm_hip_ctx* c) { return ZPlanes{c->Z, c->Zh, c->Zl, c->zscale}; }
+ZPlanes g_planes(gwm_hip_ctx* c) { return ZPlanes{c->G, c->Gh, c->Gl, c->gscale}; }
+ZPlanes var_planes(gwm_hip_ctx* c) { return ZPlanes{c->vZ, c->vZh, c->vZl, c->vscale}; }
 
 SolveArgs solve_args(gwm_hip_ctx* c, int PT, int nseg, int colB, const double* bw_pt, double bw, int kernel) {
     SolveArgs s;
@@ -265,7 +305,7 @@ int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit
     GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
     int PT, nseg;
     const int mode = crit == GWM_CRIT_CV ? kModeCV : kModePlain;
-    if ((rc = sweep(c, mode, c->nta, 0, c->Z, bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
+    if ((rc = sweep(c, mode, c->nta, 0, design_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
     SolveArgs s = solve_args(c, PT, nseg, 0, bw_pt, bw, kernel);
     timer_begin(c);
     GWM_HIP_CHECK(c, launch_solve(c->k, crit == GWM_CRIT_CV ? kSolveCV : kSolveAIC, s, c->stream));
@@ -285,9 +325,19 @@ int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit
 }
 
 // Golden-section search, BandwidthSelector::optimize (BandwidthSelector.cpp:6-81).
-// f returns a status and the criterion (DBL_MAX when invalid).
-int golden(bool adaptBw, double lower, double upper, const std::function<int(double, double*)>& f, double* xopt_out,
-           std::map<double, double>& log) {
+// f(x, exact, &v) returns a status and the criterion (DBL_MAX when invalid);
+// exact = true asks for the FP64 evaluation.
+//
+// With certify set (mixed-precision sweeps), every decision the search takes
+// on criterion values -- the branch f1 < f2, the stopping test |f2 - f1| > eps
+// and the final choice -- is certified: when a value's mixed-precision error
+// bound (tol_rel * |f|) could flip the decision, both points are re-evaluated
+// in FP64 and the decision is taken on the FP64 values.  The iterates are
+// therefore those of the FP64 search.
+typedef std::function<int(double, bool, double*)> CritFn;
+
+int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certify, double tol_rel,
+           double* xopt_out, std::map<double, double>& log) {
     double xU = upper, xL = lower;
     const double eps = 1e-4;
     const double R = (std::sqrt(5.0) - 1) / 2;
@@ -296,20 +346,55 @@ int golden(bool adaptBw, double lower, double upper, const std::function<int(dou
     double x1 = adaptBw ? std::floor(xL + d) : (xL + d);
     double x2 = adaptBw ? std::round(xU - d) : (xU - d);
     double f1 = DBL_MAX, f2 = DBL_MAX;
-    auto eval = [&](double x, double* fx) -> int {
-        int rc = f(x, fx);
+    bool e1 = !certify, e2 = !certify;
+    // evaluate; a failed (DBL_MAX) mixed value is confirmed in FP64
+    auto eval = [&](double x, bool& exact, double* fx) -> int {
+        int rc = f(x, exact, fx);
         if (rc == GWM_ERR_NOT_SPD) {
             *fx = DBL_MAX;
-            return GWM_OK;
+            rc = GWM_OK;
+        }
+        if (rc == GWM_OK && !exact && *fx == DBL_MAX) {
+            exact = true;
+            rc = f(x, true, fx);
+            if (rc == GWM_ERR_NOT_SPD) {
+                *fx = DBL_MAX;
+                rc = GWM_OK;
+            }
         }
         return rc;
     };
+    auto record = [&](double x, double fx) {
+        if (fx < DBL_MAX) log[x] = fx;
+    };
+    auto bound = [&](double fx, bool exact) { return exact || fx == DBL_MAX ? 0.0 : tol_rel * std::fabs(fx); };
+    // upgrade both points to FP64 when a decision on (f1, f2) is not certain
+    auto certify_pair = [&]() -> int {
+        if (e1 && e2) return GWM_OK;
+        double m = bound(f1, e1) + bound(f2, e2);
+        double diff = std::fabs(f2 - f1);
+        bool sure = diff > m && std::fabs(diff - eps) > m;
+        if (sure) return GWM_OK;
+        int rc;
+        if (!e1) {
+            e1 = true;
+            if ((rc = eval(x1, e1, &f1))) return rc;
+            record(x1, f1);
+        }
+        if (!e2) {
+            e2 = true;
+            if ((rc = eval(x2, e2, &f2))) return rc;
+            record(x2, f2);
+        }
+        return GWM_OK;
+    };
     int rc;
-    if ((rc = eval(x1, &f1))) return rc;
-    if ((rc = eval(x2, &f2))) return rc;
+    if ((rc = eval(x1, e1, &f1))) return rc;
+    if ((rc = eval(x2, e2, &f2))) return rc;
     if (f1 == DBL_MAX && f2 == DBL_MAX) return GWM_ERR_INVALID_INITIAL;
-    if (f1 < DBL_MAX) log[x1] = f1;
-    if (f2 < DBL_MAX) log[x2] = f2;
+    record(x1, f1);
+    record(x2, f2);
+    if ((rc = certify_pair())) return rc;
     double d1 = f2 - f1;
     double xopt = f1 < f2 ? x1 : x2;
     const double ea = 100;
@@ -320,17 +405,22 @@ int golden(bool adaptBw, double lower, double upper, const std::function<int(dou
             x2 = x1;
             x1 = adaptBw ? std::round(xL + d) : (xL + d);
             f2 = f1;
-            if ((rc = eval(x1, &f1))) return rc;
-            if (f1 < DBL_MAX) log[x1] = f1;
+            e2 = e1;
+            e1 = !certify;
+            if ((rc = eval(x1, e1, &f1))) return rc;
+            record(x1, f1);
         } else {
             xU = x1;
             x1 = x2;
             x2 = adaptBw ? std::floor(xU - d) : (xU - d);
             f1 = f2;
-            if ((rc = eval(x2, &f2))) return rc;
-            if (f2 < DBL_MAX) log[x2] = f2;
+            e1 = e2;
+            e2 = !certify;
+            if ((rc = eval(x2, e2, &f2))) return rc;
+            record(x2, f2);
         }
         iter = iter + 1;
+        if ((rc = certify_pair())) return rc;
         xopt = (f1 < f2) ? x1 : x2;
         d1 = f2 - f1;
     }
@@ -376,6 +466,7 @@ int upload_design(gwm_hip_ctx* c, const double* X, const double* y) {
     GWM_HIP_CHECK(c, hipMemcpyAsync(c->X, X, (size_t)n * k * sizeof(double), hipMemcpyHostToDevice, c->stream));
     GWM_HIP_CHECK(c, hipMemcpyAsync(c->y, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     GWM_HIP_CHECK(c, hipMemcpyAsync(c->Z, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    GWM_HIP_CHECK(c, launch_split_fp16(c->Z, c->rows_pad, ldz, c->colmax, c->Zh, c->Zl, c->zscale, c->stream));
     GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return GWM_OK;
 }
@@ -396,12 +487,15 @@ int var_impl(gwm_hip_ctx* c, const double* x, const double* y, int kernel, int a
     GWM_HIP_CHECK(c, hipMemcpyAsync(c->vZ, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     GWM_HIP_CHECK(c, hipMemcpyAsync(c->vx, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     GWM_HIP_CHECK(c, hipMemcpyAsync(c->vy, y, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (precision == GWM_MIXED)
+        GWM_HIP_CHECK(c, launch_split_fp16(c->vZ, c->rows_pad, 16, c->colmax, c->vZh, c->vZl, c->vscale, c->stream));
     const double* bw_pt;
     if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
     GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
     int PT, nseg;
     const bool cv = value && crit == GWM_CRIT_CV;
-    if ((rc = sweep(c, cv ? kModeCV : kModePlain, 1, 0, c->vZ, bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
+    if ((rc = sweep(c, cv ? kModeCV : kModePlain, 1, 0, var_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg)))
+        return rc;
     const int smode = beta ? kSolveBeta : (cv ? kSolveCV : kSolveAIC);
     GWM_HIP_CHECK(c, launch_solve_var(smode, c->part, nseg, PT, 0, n, c->n_pad, c->vx, c->vy, bw_pt, bw, kernel,
                                       c->res2, c->sii, beta ? c->betas : nullptr, c->fail, c->stream));
@@ -434,7 +528,7 @@ int fit_betas_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int prec
     if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
     GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
     int PT, nseg;
-    if ((rc = sweep(c, kModePlain, c->nta, 0, c->Z, bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
+    if ((rc = sweep(c, kModePlain, c->nta, 0, design_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
     SolveArgs s = solve_args(c, PT, nseg, 0, bw_pt, bw, kernel);
     GWM_HIP_CHECK(c, launch_solve(c->k, kSolveBeta, s, c->stream));
     int fail = 0;
@@ -524,7 +618,13 @@ int gwm_hip_set_data(gwm_hip_ctx* c, int n, int k, const double* coords, const d
         (rc = dalloc(c, &c->betas, N * k)) || (rc = dalloc(c, &c->se, N * k)) || (rc = dalloc(c, &c->gvec, N * k)) ||
         (rc = dalloc(c, &c->qd, N)) || (rc = dalloc(c, &c->red, 4)) || (rc = dalloc(c, &c->blockmax, (N + 255) / 256)) ||
         (rc = dalloc(c, &c->vx, N)) || (rc = dalloc(c, &c->vy, N)) || (rc = dalloc(c, &c->vZ, (size_t)c->rows_pad * 16)) ||
-        (rc = dalloc(c, &c->fail, 1))) {
+        (rc = dalloc(c, &c->fail, 1)) || (rc = dalloc(c, &c->Zh, (size_t)c->rows_pad * c->nta * 16)) ||
+        (rc = dalloc(c, &c->Zl, (size_t)c->rows_pad * c->nta * 16)) ||
+        (rc = dalloc(c, &c->Gh, (size_t)c->rows_pad * c->ntb * 16)) ||
+        (rc = dalloc(c, &c->Gl, (size_t)c->rows_pad * c->ntb * 16)) ||
+        (rc = dalloc(c, &c->vZh, (size_t)c->rows_pad * 16)) || (rc = dalloc(c, &c->vZl, (size_t)c->rows_pad * 16)) ||
+        (rc = dalloc(c, &c->zscale, (size_t)c->nta * 16)) || (rc = dalloc(c, &c->gscale, (size_t)c->ntb * 16)) ||
+        (rc = dalloc(c, &c->vscale, 16)) || (rc = dalloc(c, &c->colmax, (size_t)c->nta * 16))) {
         free_problem(c);
         return rc;
     }
@@ -574,7 +674,8 @@ int gwm_hip_fit(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int precisi
     if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
     GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
     int PT, nseg;
-    if ((rc = sweep(c, kModeHat, c->nta, c->ntb, c->Z, bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
+    if ((rc = sweep(c, kModeHat, c->nta, c->ntb, design_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg)))
+        return rc;
     SolveArgs s = solve_args(c, PT, nseg, c->nta * 16, bw_pt, bw, kernel);
     timer_begin(c);
     GWM_HIP_CHECK(c, launch_solve(c->k, kSolveFit, s, c->stream));
@@ -589,7 +690,10 @@ int gwm_hip_fit(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int precisi
     }
     if (qDiag) {
         // second sweep: M_j = sum_i w_ij^2 vech(g_i g_i^T), bandwidth of i
-        if ((rc = sweep(c, kModeQDiag, c->ntb, 0, c->G, bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
+        if (precision == GWM_MIXED)
+            GWM_HIP_CHECK(c, launch_split_fp16(c->G, c->rows_pad, c->ntb * 16, c->colmax, c->Gh, c->Gl, c->gscale,
+                                               c->stream));
+        if ((rc = sweep(c, kModeQDiag, c->ntb, 0, g_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
         GWM_HIP_CHECK(c, launch_qdiag(c->k, c->part, nseg, PT, c->n, c->n_pad, c->X, c->sii, c->qd, c->stream));
         GWM_HIP_CHECK(c, hipMemcpyAsync(qDiag, c->qd, c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
@@ -632,8 +736,10 @@ int gwm_hip_select_bandwidth(gwm_hip_ctx* c, int kernel, int adaptive, int crit,
     std::map<double, double> log;
     double xopt = 0.0;
     rc = golden(adaptive != 0, lo, hi,
-                [&](double x, double* fx) { return criterion_impl(c, kernel, adaptive, x, crit, precision, fx); },
-                &xopt, log);
+                [&](double x, bool exact, double* fx) {
+                    return criterion_impl(c, kernel, adaptive, x, crit, exact ? GWM_FP64 : precision, fx);
+                },
+                precision == GWM_MIXED, c->certify_rel, &xopt, log);
     write_trace(log, tx, tf, cap, len);
     if (rc) return rc;
     if (bw_out) *bw_out = xopt;
@@ -773,9 +879,10 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
         std::map<double, double> log;
         const double* xi = col(i);
-        rc = golden(ad, lo, hi, [&](double x, double* fx) {
-            return var_impl(c, xi, y.data(), kernels[i], adaptive[i], x, approach[i], precision, fx, nullptr);
-        }, &bw[i], log);
+        rc = golden(ad, lo, hi, [&](double x, bool exact, double* fx) {
+            return var_impl(c, xi, y.data(), kernels[i], adaptive[i], x, approach[i], exact ? GWM_FP64 : precision, fx,
+                            nullptr);
+        }, precision == GWM_MIXED, c->certify_rel, &bw[i], log);
         if (rc) return fail_out(rc);
     }
     // initial all-variable bandwidth (GWRMultiscale.cpp:137-160)
@@ -786,9 +893,9 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
     double hi0 = has_upper ? upper : (ad0 ? (double)n : maxDist);
     std::map<double, double> log0;
     double initBw = 0.0;
-    rc = golden(ad0, lo0, hi0, [&](double x, double* fx) {
-        return criterion_impl(c, kernels[0], adaptive[0], x, approach[0], precision, fx);
-    }, &initBw, log0);
+    rc = golden(ad0, lo0, hi0, [&](double x, bool exact, double* fx) {
+        return criterion_impl(c, kernels[0], adaptive[0], x, approach[0], exact ? GWM_FP64 : precision, fx);
+    }, precision == GWM_MIXED, c->certify_rel, &initBw, log0);
     if (rc) return fail_out(rc);
     // backfitting (GWRMultiscale.cpp:208-317)
     std::vector<double> betas((size_t)n * k), resid(n), yi(n), colv(n);
@@ -817,9 +924,10 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
                 double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
                 std::map<double, double> log;
                 double bw1 = 0.0;
-                rc = golden(ad, lo, hi, [&](double x, double* fx) {
-                    return var_impl(c, xi, yi.data(), kernels[i], adaptive[i], x, approach[i], precision, fx, nullptr);
-                }, &bw1, log);
+                rc = golden(ad, lo, hi, [&](double x, bool exact, double* fx) {
+                    return var_impl(c, xi, yi.data(), kernels[i], adaptive[i], x, approach[i],
+                                    exact ? GWM_FP64 : precision, fx, nullptr);
+                }, precision == GWM_MIXED, c->certify_rel, &bw1, log);
                 if (rc) return fail_out(rc);
                 if (std::fabs(bw1 - bw[i]) > bw_threshold[i]) bwChangeNo[i] = 0;
                 else {
@@ -860,6 +968,12 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
     std::copy(betas.begin(), betas.end(), betas_out);
     std::copy(bw.begin(), bw.end(), bws_out);
     restore();
+    return GWM_OK;
+}
+
+int gwm_hip_set_certify_tolerance(gwm_hip_ctx* c, double rel) {
+    if (!c || !(rel >= 0.0)) return GWM_ERR_INVALID;
+    c->certify_rel = rel;
     return GWM_OK;
 }
 

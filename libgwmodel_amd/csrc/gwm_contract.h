@@ -44,6 +44,11 @@ struct ContractMixedArgs {
 };
 
 int contract_tile_focus(int nta, int ntb);
+int contract_mixed_tile_focus(int nta, int ntb);
+// FP16 hi/lo planes ([rows/32][ldz][32]) and per-column unscale factors of a
+// row-major FP64 table; colmax is ldz words of scratch.
+hipError_t launch_split_fp16(const double* Z, int rows, int ldz, unsigned long long* colmax, _Float16* Zh,
+                             _Float16* Zl, double* unscale, hipStream_t s);
 hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const ContractArgs& a, hipStream_t s);
 hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const ContractMixedArgs& a, hipStream_t s);
 

@@ -42,7 +42,7 @@ struct MCoords<kGreatCircle> {
 };
 
 template <int NTA, int DIST, int MODE>
-struct MShared {
+struct alignas(16) MShared {
     _Float16 zh[2][NTA * 16][kColStride];
     _Float16 zl[2][NTA * 16][kColStride];
     MCoords<DIST> c;

@@ -1,0 +1,174 @@
+// Known-answer tests of the C++ host interface on the device, restating the
+// reference's Catch2 sections for londonhp100 (test/testGWRBasic.cpp:40-137,
+// test/testGWRMultiscale.cpp:135-173) with ParallelType::HIP.
+// Usage: test_facade <dir with londonhp100coords.csv and londonhp100data.csv>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "gwmodelpp/gwmodel.h"
+
+using namespace gwm;
+
+static int failures = 0;
+#define REQUIRE(cond)                                                              \
+    do {                                                                           \
+        if (!(cond)) {                                                             \
+            std::printf("FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);          \
+            ++failures;                                                            \
+        }                                                                          \
+    } while (0)
+#define REQUIRE_NEAR(a, b, tol)                                                    \
+    do {                                                                           \
+        double a_ = (a), b_ = (b);                                                 \
+        if (!(std::fabs(a_ - b_) <= (tol))) {                                      \
+            std::printf("FAILED %s:%d: %s = %.15g, expected %.15g\n", __FILE__, __LINE__, #a, a_, b_); \
+            ++failures;                                                            \
+        }                                                                          \
+    } while (0)
+
+static Mat read_csv(const std::string& path) {
+    std::ifstream f(path);
+    std::string line;
+    std::getline(f, line);   // header
+    std::vector<std::vector<double>> rows;
+    while (std::getline(f, line)) {
+        if (line.empty()) continue;
+        std::stringstream ss(line);
+        std::string cell;
+        std::vector<double> r;
+        while (std::getline(ss, cell, ',')) r.push_back(std::stod(cell));
+        rows.push_back(r);
+    }
+    Mat m(rows.size(), rows.empty() ? 0 : rows[0].size());
+    for (std::size_t i = 0; i < rows.size(); ++i)
+        for (std::size_t j = 0; j < rows[i].size(); ++j) m(i, j) = rows[i][j];
+    return m;
+}
+
+class StopAfter : public Logger {
+public:
+    bool stop() override { return true; }
+};
+
+int main(int argc, char** argv) {
+    std::string dir = argc > 1 ? argv[1] : "tests/golden";
+    Mat coords = read_csv(dir + "/londonhp100coords.csv");
+    Mat data = read_csv(dir + "/londonhp100data.csv");
+    const std::size_t n = coords.n_rows();
+    Vec y = data.col(0);
+    Mat x(n, 4), xm(n, 3);
+    for (std::size_t i = 0; i < n; ++i) {
+        x(i, 0) = 1.0; x(i, 1) = data(i, 1); x(i, 2) = data(i, 2); x(i, 3) = data(i, 3);
+        xm(i, 0) = 1.0; xm(i, 1) = data(i, 1); xm(i, 2) = data(i, 3);
+    }
+
+    {   // adaptive bandwidth | no bandwidth optimization
+        CRSDistance distance(false);
+        BandwidthWeight bandwidth(36, true, BandwidthWeight::Gaussian);
+        SpatialWeight spatial(&bandwidth, &distance);
+        GWRBasic algorithm;
+        algorithm.setCoords(coords);
+        algorithm.setDependentVariable(y);
+        algorithm.setIndependentVariables(x);
+        algorithm.setSpatialWeight(spatial);
+        algorithm.setParallelType(ParallelType::HIP);
+        algorithm.setGPUId(0);
+        algorithm.setGroupSize(64);
+        algorithm.fit();
+        REQUIRE(algorithm.hasIntercept() == true);
+        RegressionDiagnostic d = algorithm.diagnostic();
+        REQUIRE_NEAR(d.AIC, 2436.60445730413, 1e-8);
+        REQUIRE_NEAR(d.AICc, 2448.27206524754, 1e-8);
+        REQUIRE_NEAR(d.RSquare, 0.708010632044736, 1e-8);
+        REQUIRE_NEAR(d.RSquareAdjust, 0.674975341723766, 1e-8);
+        REQUIRE(algorithm.s().n_rows() == n);   // n < 8192: S is stored
+    }
+    {   // fixed bandwidth | no bandwidth optimization
+        CRSDistance distance(false);
+        BandwidthWeight bandwidth(5000, false, BandwidthWeight::Gaussian);
+        SpatialWeight spatial(&bandwidth, &distance);
+        GWRBasic algorithm;
+        algorithm.setCoords(coords);
+        algorithm.setDependentVariable(y);
+        algorithm.setIndependentVariables(x);
+        algorithm.setSpatialWeight(spatial);
+        algorithm.setParallelType(ParallelType::HIP);
+        algorithm.fit();
+        RegressionDiagnostic d = algorithm.diagnostic();
+        REQUIRE_NEAR(d.AIC, 2437.649574267587, 1e-8);
+        REQUIRE_NEAR(d.AICc, 2447.676281164379, 1e-8);
+        REQUIRE_NEAR(d.RSquare, 0.701466295457, 1e-8);
+        REQUIRE_NEAR(d.RSquareAdjust, 0.673691594464, 1e-8);
+    }
+    for (int prec = 0; prec < 2; ++prec) {   // CV bandwidth optimization (FP64 and mixed)
+        CRSDistance distance(false);
+        BandwidthWeight bandwidth(0, true, BandwidthWeight::Gaussian);
+        SpatialWeight spatial(&bandwidth, &distance);
+        GWRBasic algorithm;
+        algorithm.setCoords(coords);
+        algorithm.setDependentVariable(y);
+        algorithm.setIndependentVariables(x);
+        algorithm.setSpatialWeight(spatial);
+        algorithm.setIsAutoselectBandwidth(true);
+        algorithm.setBandwidthSelectionCriterion(GWRBasic::BandwidthSelectionCriterionType::CV);
+        algorithm.setParallelType(ParallelType::HIP);
+        algorithm.setPrecision(prec ? Precision::Mixed : Precision::FP64);
+        algorithm.fit();
+        std::size_t bw = (std::size_t)algorithm.spatialWeight().weight<BandwidthWeight>()->bandwidth();
+        REQUIRE(bw == 67);
+        REQUIRE(!algorithm.bandwidthSelectionCriterionList().empty());
+    }
+    {   // cancellation: Status::Terminated and zero betas
+        CRSDistance distance(false);
+        BandwidthWeight bandwidth(0, true, BandwidthWeight::Gaussian);
+        SpatialWeight spatial(&bandwidth, &distance);
+        GWRBasic algorithm;
+        StopAfter telegram;
+        algorithm.setTelegram(&telegram);
+        algorithm.setCoords(coords);
+        algorithm.setDependentVariable(y);
+        algorithm.setIndependentVariables(x);
+        algorithm.setSpatialWeight(spatial);
+        algorithm.setIsAutoselectBandwidth(true);
+        algorithm.setParallelType(ParallelType::HIP);
+        algorithm.fit();
+        REQUIRE(algorithm.status() == Status::Terminated);
+    }
+    {   // MGWR: optim bw cv | null init bw | adaptive | bisquare | without hat matrix
+        std::vector<SpatialWeight> spatials;
+        std::vector<bool> centered;
+        std::vector<GWRMultiscale::BandwidthInitilizeType> init;
+        std::vector<GWRMultiscale::BandwidthSelectionCriterionType> approach;
+        for (int i = 0; i < 3; ++i) {
+            CRSDistance distance;
+            BandwidthWeight bandwidth(0, true, BandwidthWeight::Bisquare);
+            spatials.push_back(SpatialWeight(&bandwidth, &distance));
+            centered.push_back(i != 0);
+            init.push_back(GWRMultiscale::BandwidthInitilizeType::Null);
+            approach.push_back(GWRMultiscale::BandwidthSelectionCriterionType::CV);
+        }
+        GWRMultiscale algorithm;
+        algorithm.setCoords(coords);
+        algorithm.setDependentVariable(y);
+        algorithm.setIndependentVariables(xm);
+        algorithm.setSpatialWeights(spatials);
+        algorithm.setHasHatMatrix(false);
+        algorithm.setPreditorCentered(centered);
+        algorithm.setBandwidthInitilize(init);
+        algorithm.setBandwidthSelectionApproach(approach);
+        algorithm.setBandwidthSelectThreshold(std::vector<double>(3, 1e-5));
+        algorithm.setParallelType(ParallelType::HIP);
+        algorithm.fit();
+        const std::vector<SpatialWeight>& sw = algorithm.spatialWeights();
+        REQUIRE(sw[0].weight<BandwidthWeight>()->bandwidth() == 35);
+        REQUIRE(sw[1].weight<BandwidthWeight>()->bandwidth() == 98);
+        REQUIRE(sw[2].weight<BandwidthWeight>()->bandwidth() == 98);
+        REQUIRE_NEAR(algorithm.diagnostic().RSquare, 0.757377391669, 1e-6);
+    }
+    std::printf(failures ? "FACADE TESTS FAILED: %d\n" : "FACADE TESTS PASSED\n", failures);
+    return failures ? 1 : 0;
+}

@@ -1,0 +1,99 @@
+"""Mixed-precision path against the FP64 path (both through the C ABI).
+
+Stated bounds: beta within 1e-6 relative (column-scaled: max|db| / max|b| per
+coefficient) of the FP64 path; criteria within 1e-7 relative; with certified
+golden-section decisions the selected bandwidth and every iterate equal the
+FP64 search's.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from libgwmodel_amd import capi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def col_rel(b, ref):
+    scale = np.maximum(np.max(np.abs(ref), axis=0), 1e-300)
+    return float(np.max(np.max(np.abs(b - ref), axis=0) / scale))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+MIX_CASES = [
+    (capi.GAUSSIAN, False, 12.0),
+    (capi.GAUSSIAN, True, 150.0),
+    (capi.BISQUARE, True, 120.0),
+    (capi.TRICUBE, False, 25.0),
+    (capi.EXPONENTIAL, False, 8.0),
+    (capi.BOXCAR, False, 30.0),
+]
+
+
+@pytest.mark.parametrize("kernel,adaptive,bw", MIX_CASES)
+def test_mixed_criteria_and_fit_euclidean(ctx, kernel, adaptive, bw):
+    say("case: mixed euclidean kernel=%d adaptive=%d bw=%g" % (kernel, adaptive, bw))
+    d = synth.config1(n=3000)
+    ctx.set_data(d["coords"], d["X"], d["y"])
+    for crit in [capi.CRIT_CV, capi.CRIT_AIC]:
+        f64 = ctx.criterion(kernel, adaptive, bw, crit, capi.FP64)
+        mix = ctx.criterion(kernel, adaptive, bw, crit, capi.MIXED)
+        assert mix == pytest.approx(f64, rel=1e-7), (crit, mix, f64)
+    g64 = ctx.fit(kernel, adaptive, bw, capi.FP64)
+    gm = ctx.fit(kernel, adaptive, bw, capi.MIXED)
+    assert col_rel(gm["betas"], g64["betas"]) < 1e-6
+    assert np.allclose(gm["shat"], g64["shat"], rtol=1e-6)
+    assert np.allclose(gm["qDiag"], g64["qDiag"], rtol=1e-6, atol=1e-9)
+    assert col_rel(gm["betasSE_raw"], g64["betasSE_raw"]) < 1e-5
+
+
+def test_mixed_great_circle_config2(ctx):
+    say("case: mixed great-circle config2 n=6000")
+    d = synth.config2(n=6000)
+    ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
+    for bw in [60.0, 250.0, 1500.0]:
+        f64 = ctx.criterion(capi.GAUSSIAN, False, bw, capi.CRIT_AIC, capi.FP64)
+        mix = ctx.criterion(capi.GAUSSIAN, False, bw, capi.CRIT_AIC, capi.MIXED)
+        assert mix == pytest.approx(f64, rel=1e-7), (bw, mix, f64)
+        g64 = ctx.fit(capi.GAUSSIAN, False, bw, capi.FP64)
+        gm = ctx.fit(capi.GAUSSIAN, False, bw, capi.MIXED)
+        assert col_rel(gm["betas"], g64["betas"]) < 1e-6, bw
+
+
+def test_certified_search_matches_fp64(ctx):
+    say("case: certified mixed search config2 n=4000")
+    d = synth.config2(n=4000)
+    lo, hi = synth.CONFIG2_BW_KM
+    ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
+    b64, t64 = ctx.select_bandwidth(capi.GAUSSIAN, False, capi.CRIT_AIC, capi.FP64, lo, hi)
+    bm, tm = ctx.select_bandwidth(capi.GAUSSIAN, False, capi.CRIT_AIC, capi.MIXED, lo, hi)
+    assert bm == b64
+    assert [x for x, _ in tm] == [x for x, _ in t64]
+    say("case: certified mixed search config1 adaptive bisquare CV n=2000")
+    d = synth.config1(n=2000)
+    ctx.set_data(d["coords"], d["X"], d["y"])
+    b64, t64 = ctx.select_bandwidth(capi.BISQUARE, True, capi.CRIT_CV, capi.FP64, 20, 2000)
+    bm, tm = ctx.select_bandwidth(capi.BISQUARE, True, capi.CRIT_CV, capi.MIXED, 20, 2000)
+    assert bm == b64
+    assert [x for x, _ in tm] == [x for x, _ in t64]
+
+
+def test_mixed_mgwr_bandwidths_match(ctx, londonhp100):
+    say("case: mixed MGWR londonhp100")
+    ctx.set_data(londonhp100["coords"], londonhp100["Xm"], londonhp100["y"])
+    r = ctx.mgwr_run([capi.BISQUARE] * 3, [1] * 3, [0] * 3, [0] * 3, [capi.CRIT_CV] * 3, [0, 1, 1], [1e-5] * 3,
+                     precision=capi.MIXED)
+    assert list(r["bandwidths"]) == [35, 98, 98]
+    assert abs(r["diagnostic"]["RSquare"] - 0.757377391669) < 1e-6
