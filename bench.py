@@ -167,6 +167,9 @@ def main():
         "mixed_sweep_over_fp64_sweep": avg_ms64 / avg_ms if avg_ms > 0 else None,
         "bandwidth_identical": mixed["bandwidth"] == fp64["bandwidth"],
         "beta_rel_err_vs_fp64": beta_rel_err(mixed["betas"], fp64["betas"]),
+        "se_rel_err_vs_fp64": beta_rel_err(mixed["betasSE"], fp64["betasSE"]),
+        "aicc_rel_err_vs_fp64": abs(mixed["diagnostic"]["AICc"] - fp64["diagnostic"]["AICc"]) /
+        abs(fp64["diagnostic"]["AICc"]),
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["mixed"] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK["mixed"],
                      "traffic": traffic["bytes_per_launch"] if traffic else None,

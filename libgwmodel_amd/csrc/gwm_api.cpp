@@ -63,7 +63,7 @@ struct gwm_hip_ctx {
     _Float16 *Zh = nullptr, *Zl = nullptr, *Gh = nullptr, *Gl = nullptr, *vZh = nullptr, *vZl = nullptr;
     double *zscale = nullptr, *gscale = nullptr, *vscale = nullptr;
     unsigned long long* colmax = nullptr;
-    double certify_rel = 1e-6;   // relative bound used to certify mixed comparisons
+    double certify_rel = 1e-7;   // relative bound used to certify mixed comparisons
 
     // timing
     bool timing = false;
