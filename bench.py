@@ -77,12 +77,26 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=512, help="focus points of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweeps-only", type=int, default=0,
+                    help="profiling mode: run this many mixed AICc sweeps (and as many FP64 ones) and exit")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="profiling mode: one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+        src = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe}))
+        return
 
     from libgwmodel_amd import capi, synth
 
@@ -93,6 +107,14 @@ def main():
     k = d["X"].shape[1]
     ctx = capi.Context(0)
     ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
+
+    if args.sweeps_only:
+        for prec in (capi.MIXED, capi.FP64):
+            for _ in range(args.sweeps_only):
+                ctx.criterion(capi.GAUSSIAN, False, 100.0, capi.CRIT_AIC, prec)
+        print(json.dumps({"sweeps": args.sweeps_only, "n": n}))
+        ctx.close()
+        return
 
     # warmup (kernel loads, allocation of the workspaces)
     log("warmup x%d" % args.warmup)
