@@ -333,7 +333,10 @@ int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit
 // and the final choice -- is certified: when a value's mixed-precision error
 // bound (tol_rel * |f|) could flip the decision, both points are re-evaluated
 // in FP64 and the decision is taken on the FP64 values.  The iterates are
-// therefore those of the FP64 search.
+// therefore those of the FP64 search.  Once a decision has needed the FP64
+// values, the search has entered the region where the criterion varies less
+// than the mixed-precision error, and later candidates are evaluated in FP64
+// directly instead of twice.
 typedef std::function<int(double, bool, double*)> CritFn;
 
 int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certify, double tol_rel,
@@ -347,6 +350,7 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
     double x2 = adaptBw ? std::round(xU - d) : (xU - d);
     double f1 = DBL_MAX, f2 = DBL_MAX;
     bool e1 = !certify, e2 = !certify;
+    bool sticky = false;   // FP64 from here on
     // evaluate; a failed (DBL_MAX) mixed value is confirmed in FP64
     auto eval = [&](double x, bool& exact, double* fx) -> int {
         int rc = f(x, exact, fx);
@@ -375,6 +379,7 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
         double diff = std::fabs(f2 - f1);
         bool sure = diff > m && std::fabs(diff - eps) > m;
         if (sure) return GWM_OK;
+        sticky = true;
         int rc;
         if (!e1) {
             e1 = true;
@@ -406,7 +411,7 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
             x1 = adaptBw ? std::round(xL + d) : (xL + d);
             f2 = f1;
             e2 = e1;
-            e1 = !certify;
+            e1 = !certify || sticky;
             if ((rc = eval(x1, e1, &f1))) return rc;
             record(x1, f1);
         } else {
@@ -415,7 +420,7 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
             x2 = adaptBw ? std::floor(xU - d) : (xU - d);
             f1 = f2;
             e1 = e2;
-            e2 = !certify;
+            e2 = !certify || sticky;
             if ((rc = eval(x2, e2, &f2))) return rc;
             record(x2, f2);
         }

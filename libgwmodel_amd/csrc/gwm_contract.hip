@@ -46,7 +46,7 @@ template <int NTA, int DIST, int MODE>
 struct SharedTile {
     double z[kChunk][NTA * 16];
     CoordStage<DIST> c;
-    double bw[MODE == kModeQDiag ? kChunk : 1];
+    KConst64 kc[MODE == kModeQDiag ? kChunk : 1];
 };
 
 template <int DIST>
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs args) {
 
     // focus points owned by this lane (one per A tile)
     FocusCoords<DIST> fc[R];
-    double fbw[R];
+    KConst64 fbw[R];
     int fidx[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs args) {
         fidx[r] = f;
         int fl = f < n ? f : n - 1;
         fc[r] = load_focus<DIST>(args.P, fl);
-        fbw[r] = (MODE != kModeQDiag && args.bw_pt) ? args.bw_pt[fl] : args.bw;
+        fbw[r] = make_kconst64((MODE != kModeQDiag && args.bw_pt) ? args.bw_pt[fl] : args.bw);
     }
 
     v4d accA[R][NTA];
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs args) {
                 sh.c.hz[threadIdx.x] = args.P.hz[jl];
                 sh.c.sl[threadIdx.x] = args.P.sl[jl];
             }
-            if constexpr (MODE == kModeQDiag) sh.bw[threadIdx.x] = args.bw_pt ? args.bw_pt[jl] : args.bw;
+            if constexpr (MODE == kModeQDiag) sh.kc[threadIdx.x] = make_kconst64(args.bw_pt ? args.bw_pt[jl] : args.bw);
         }
         __syncthreads();
         const int steps = min(kChunk, jend - j0 + 3) / 4;
@@ -154,8 +154,7 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs args) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double d = pair_distance<DIST, NTA, MODE>(fc[r], sh, rr);
-                double b = MODE == kModeQDiag ? sh.bw[rr] : fbw[r];
-                double w = kernel_f64(args.kernel, d, b);
+                double w = kernel_fast_f64(args.kernel, d, MODE == kModeQDiag ? sh.kc[rr] : fbw[r]);
                 if (j >= jend) w = 0.0;
                 if (MODE == kModeCV && j == fidx[r]) w = 0.0;
                 double wa = MODE == kModeQDiag ? w * w : w;

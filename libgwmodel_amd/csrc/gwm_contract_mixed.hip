@@ -20,6 +20,10 @@
 // the 16-byte B-operand reads of a quarter-wave hit distinct banks.
 #include "gwm_contract.h"
 
+// Mixed precision tolerates (and profits from) fused multiply-adds: let the
+// compiler contract a*b+c throughout this translation unit.
+#pragma clang fp contract(fast)
+
 namespace gwm {
 
 namespace {
@@ -38,7 +42,7 @@ struct MCoords<kEuclidean> {
 template <>
 struct MCoords<kGreatCircle> {
     double hx[kChunkM], hy[kChunkM], hz[kChunkM];
-    float sl[kChunkM];
+    double sl[kChunkM];
 };
 
 template <int NTA, int DIST, int MODE>
@@ -51,8 +55,7 @@ struct alignas(16) MShared {
 
 template <int DIST>
 struct MFocus {
-    double a, b, c;
-    float sl;
+    double a, b, c, sl;
 };
 
 template <int DIST>
@@ -62,12 +65,12 @@ __device__ __forceinline__ MFocus<DIST> load_mfocus(const Points& P, int f) {
         fc.a = P.px[f];
         fc.b = P.py[f];
         fc.c = 0.0;
-        fc.sl = 0.f;
+        fc.sl = 0.0;
     } else {
         fc.a = P.hx[f];
         fc.b = P.hy[f];
         fc.c = P.hz[f];
-        fc.sl = (float)P.sl[f];
+        fc.sl = P.sl[f];
     }
     return fc;
 }
@@ -101,6 +104,72 @@ __device__ __forceinline__ float weight_from_d2(int kernel, float d2, const BwCo
         return d2 < c.bw2 ? (t * t) * t : 0.0f;
     }
     default: return d2 < c.bw2 ? 1.0f : 0.0f;
+    }
+}
+
+// Squared great-circle distance (km^2) from the half-chord S = sin^2(theta),
+// theta = half the central angle, for S <= 0.25 (central angle <= 60 deg):
+//   theta^2 = S (asin(s)/s)^2,  asin(s)/s = 1 + S p(S)   (Cephes asinf polynomial)
+//   R = sinc(2 theta) = sum_m (-1)^m (2 theta)^(2m) / (2m+1)!
+//   d = 2 a theta (1 + f/8 [(3R-1) sp^2 / C - (3R+1) sm^2 / S]),  C = 1 - S,
+// with sp = sin lat_i + sin lat_j, sm = sin lat_j - sin lat_i (both formed in
+// FP64 by the caller: sm^2/S cancels for nearby points); 1/C and 1/S come
+// from one reciprocal of S*C.  No square root, two transcendentals per pair
+// including the kernel's exponential.  Larger S falls back to gcdist_tail_f32.
+__device__ __forceinline__ float gc_d2_f32(float S, float sp, float sm, float slf, float slj) {
+    const float a = 6378.137f;
+    const float f8 = (float)(1.0 / 298.257223563 / 8.0);
+    if (S > 0.25f) {
+        float d = gcdist_tail_f32(S, slf, slj);
+        return d * d;
+    }
+    if (!(S > 1e-30f)) return 0.0f;
+    const float p = fmaf(fmaf(fmaf(fmaf(4.2163199048e-2f, S, 2.4181311049e-2f), S, 4.5470025998e-2f), S,
+                              7.4953002686e-2f), S, 1.6666752422e-1f);
+    const float q = fmaf(S, p, 1.0f);
+    const float th2 = S * q * q;
+    const float u = 4.0f * th2;
+    const float R = fmaf(fmaf(fmaf(fmaf(fmaf(u, -2.5052108385441718e-8f, 2.7557319223985893e-6f), u,
+                                        -1.9841269841269841e-4f), u, 8.3333333333333333e-3f), u,
+                              -1.6666666666666667e-1f), u, 1.0f);
+    const float C = 1.0f - S;
+    const float rsc = __builtin_amdgcn_rcpf(S * C);
+    const float corr = f8 * ((3.0f * R - 1.0f) * (S * rsc) * (sp * sp) - (3.0f * R + 1.0f) * (C * rsc) * (sm * sm));
+    const float g = 1.0f + corr;
+    return (4.0f * a * a) * th2 * (g * g);
+}
+
+// Kernel weights of 8 squared distances (the kernel switch is taken once).
+__device__ __forceinline__ void weights8_from_d2(int kernel, const float* d2, const BwConst& c, float* w) {
+    switch (kernel) {
+    case kGaussian:
+#pragma unroll
+        for (int e = 0; e < 8; ++e) w[e] = __builtin_amdgcn_exp2f(d2[e] * c.gauss);
+        break;
+    case kExponential:
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            w[e] = __builtin_amdgcn_exp2f(-1.4426950408889634f * __builtin_sqrtf(d2[e]) * c.inv_bw);
+        break;
+    case kBisquare:
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float t = 1.0f - d2[e] * c.inv_bw2;
+            w[e] = d2[e] < c.bw2 ? t * t : 0.0f;
+        }
+        break;
+    case kTricube:
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float q = __builtin_sqrtf(d2[e]) * c.inv_bw;
+            float t = 1.0f - q * q * q;
+            w[e] = d2[e] < c.bw2 ? (t * t) * t : 0.0f;
+        }
+        break;
+    default:
+#pragma unroll
+        for (int e = 0; e < 8; ++e) w[e] = d2[e] < c.bw2 ? 1.0f : 0.0f;
+        break;
     }
 }
 
@@ -174,7 +243,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(ContractMixedArgs a
                 sh.c.hx[threadIdx.x] = args.P.hx[jl];
                 sh.c.hy[threadIdx.x] = args.P.hy[jl];
                 sh.c.hz[threadIdx.x] = args.P.hz[jl];
-                sh.c.sl[threadIdx.x] = (float)args.P.sl[jl];
+                sh.c.sl[threadIdx.x] = args.P.sl[jl];
             }
             if constexpr (MODE == kModeQDiag) sh.bw[threadIdx.x] = (float)(args.bw_pt ? args.bw_pt[jl] : args.bw);
         }
@@ -198,32 +267,40 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(ContractMixedArgs a
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 v8h ah, al, a2h, a2l;
+                const int rb = 32 * s + 8 * kq;
+                float d2[8], w[8];
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    const int rr = 32 * s + 8 * kq + e;
-                    const int j = j0 + rr;
-                    float w;
+                    const int rr = rb + e;
                     if constexpr (DIST == kEuclidean) {
                         double dx = sh.c.x[rr] - fc[r].a, dy = sh.c.y[rr] - fc[r].b;
-                        float d2 = (float)(dx * dx + dy * dy);
-                        BwConst bc = MODE == kModeQDiag ? make_bw(sh.bw[rr]) : fbw[r];
-                        w = weight_from_d2(args.kernel, d2, bc);
+                        d2[e] = (float)(dx * dx + dy * dy);
                     } else {
                         double ex = sh.c.hx[rr] - fc[r].a, ey = sh.c.hy[rr] - fc[r].b, ez = sh.c.hz[rr] - fc[r].c;
-                        float S = (float)(ex * ex + ey * ey + ez * ez);
-                        float d = gcdist_tail_f32(S, fc[r].sl, sh.c.sl[rr]);
-                        BwConst bc = MODE == kModeQDiag ? make_bw(sh.bw[rr]) : fbw[r];
-                        w = weight_from_d2(args.kernel, d * d, bc);
+                        const double slj = sh.c.sl[rr];
+                        d2[e] = gc_d2_f32((float)(ex * ex + ey * ey + ez * ez), (float)(fc[r].sl + slj),
+                                          (float)(slj - fc[r].sl), (float)fc[r].sl, (float)slj);
                     }
-                    if (j >= jend) w = 0.0f;
-                    if (MODE == kModeCV && j == fidx[r]) w = 0.0f;
-                    const float wa = MODE == kModeQDiag ? w * w : w;
+                }
+                if constexpr (MODE == kModeQDiag) {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) w[e] = weight_from_d2(args.kernel, d2[e], make_bw(sh.bw[rb + e]));
+                } else {
+                    weights8_from_d2(args.kernel, d2, fbw[r], w);
+                }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int j = j0 + rb + e;
+                    float we = w[e];
+                    if (j >= jend) we = 0.0f;
+                    if (MODE == kModeCV && j == fidx[r]) we = 0.0f;
+                    const float wa = MODE == kModeQDiag ? we * we : we;
                     _Float16 h, l;
                     split16(wa, h, l);
                     ah[e] = h;
                     al[e] = l;
                     if (NTB > 0) {
-                        split16(w * w, h, l);
+                        split16(we * we, h, l);
                         a2h[e] = h;
                         a2l[e] = l;
                     }

@@ -44,10 +44,31 @@ __device__ __forceinline__ double d2_euclid(double xf, double yf, double xj, dou
 }
 
 // WGS-84 ellipsoid-corrected great-circle distance in km
-// (CRSDistance::SpGcdist, CRSDistance.cpp:15-69), restated through the
-// identity S = sin^2(G)cos^2(L) + cos^2(F)sin^2(L) = |u_i - u_j|^2 / 4 for the
-// unit vectors u, and sin F cos G = (sin lat1 + sin lat2)/2,
-// cos F sin G = (sin lat1 - sin lat2)/2.  w = atan(sqrt(S/C)) = asin(sqrt(S)).
+// (CRSDistance::SpGcdist, CRSDistance.cpp:15-69), restated through exact
+// identities so that one FP64 reciprocal replaces the reference's three
+// divisions and two of its square roots:
+//   S = sin^2(G)cos^2(L) + cos^2(F)sin^2(L) = |u_i - u_j|^2 / 4 (u: unit vectors),
+//   C = 1 - S,  w = atan(sqrt(S/C)) = asin(sqrt(S)),
+//   R = sqrt(S C) / w = sin(2w) / (2w)  (a series in (2w)^2 for S <= 1/2),
+//   sin F cos G = (sin lat1 + sin lat2)/2,  cos F sin G = (sin lat1 - sin lat2)/2,
+//   H1 / (2C) and H2 / (2S) from 1/(S C).
+// R only enters through the flattening term (f ~ 3.4e-3), so its truncation
+// (< 1e-13) stays below 1e-15 relative in the distance.
+__device__ __forceinline__ double sinc2w_f64(double u) {
+    // sum_{m=0}^{10} (-u)^m / (2m+1)!,  u = (2w)^2 <= pi^2/4
+    double r = -1.9572941063391263e-20;
+    r = fma(r, u, 8.2206352466243297e-18);
+    r = fma(r, u, -2.8114572543455206e-15);
+    r = fma(r, u, 7.6471637318198164e-13);
+    r = fma(r, u, -1.6059043836821613e-10);
+    r = fma(r, u, 2.5052108385441719e-08);
+    r = fma(r, u, -2.7557319223985891e-06);
+    r = fma(r, u, 1.9841269841269841e-04);
+    r = fma(r, u, -8.3333333333333333e-03);
+    r = fma(r, u, 1.6666666666666667e-01);
+    return fma(-r, u, 1.0);
+}
+
 __device__ __forceinline__ double gcdist_f64(double lonf, double latf, double hxf, double hyf, double hzf,
                                              double slf, double lonj, double latj, double hxj, double hyj,
                                              double hzj, double slj) {
@@ -64,12 +85,13 @@ __device__ __forceinline__ double gcdist_f64(double lonf, double latf, double hx
     if (S <= 0.0) return 0.0;
     double C = 1.0 - S;
     double w = asin(sqrt(S));
-    double R = sqrt(S * C) / w;
-    double D = 2 * w * a;
-    double H1 = (3 * R - 1) / (2 * C);
-    double H2 = (3 * R + 1) / (2 * S);
+    double R = S <= 0.5 ? sinc2w_f64(4.0 * w * w) : sqrt(S * C) / w;
+    double inv = 1.0 / (S * C);
+    double H1 = (3 * R - 1) * (0.5 * S * inv);
+    double H2 = (3 * R + 1) * (0.5 * C * inv);
     double sp = 0.5 * (slj + slf);
     double sm = 0.5 * (slj - slf);
+    double D = 2 * w * a;
     return D * (1 + f * H1 * (sp * sp) - f * H2 * (sm * sm));
 }
 
@@ -87,6 +109,39 @@ __device__ __forceinline__ double kernel_f64(int kernel, double d, double bw) {
         return d < bw ? (t * t) * t : 0.0;
     }
     default: return d < bw ? 1.0 : 0.0;
+    }
+}
+
+// Per-bandwidth constants so that the sweep weighs pairs without divisions
+// (multiplication by the reciprocal: within an ulp of the reference's quotient;
+// the boundary tests d < bw are unchanged).
+struct KConst64 {
+    double bw, gauss, inv_bw, inv_bw2, inv_bw3;   // gauss = 1 / (-2 bw^2)
+};
+
+__device__ __forceinline__ KConst64 make_kconst64(double bw) {
+    KConst64 k;
+    k.bw = bw;
+    k.gauss = 1.0 / ((-2.0) * (bw * bw));
+    k.inv_bw = 1.0 / bw;
+    k.inv_bw2 = 1.0 / (bw * bw);
+    k.inv_bw3 = 1.0 / (bw * bw * bw);
+    return k;
+}
+
+__device__ __forceinline__ double kernel_fast_f64(int kernel, double d, const KConst64& k) {
+    switch (kernel) {
+    case kGaussian: return exp((d * d) * k.gauss);
+    case kExponential: return exp(-d * k.inv_bw);
+    case kBisquare: {
+        double t = 1.0 - (d * d) * k.inv_bw2;
+        return d < k.bw ? t * t : 0.0;
+    }
+    case kTricube: {
+        double t = 1.0 - (d * d * d) * k.inv_bw3;
+        return d < k.bw ? (t * t) * t : 0.0;
+    }
+    default: return d < k.bw ? 1.0 : 0.0;
     }
 }
 
