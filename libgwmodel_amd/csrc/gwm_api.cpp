@@ -201,36 +201,29 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
     const int PT = (nta + ntb) * 16;
     int rc = ensure_part(c, (size_t)nseg * PT * c->n_pad * sizeof(double));
     if (rc) return rc;
+    SweepLaunch L;
+    SweepJob& j = L.job0;
+    j.P = c->points();
+    j.Z = zp.z64;
+    j.Zh = zp.zh;
+    j.Zl = zp.zl;
+    j.colscale = zp.scale;
+    j.bw_pt = bw_pt;
+    j.bw = bw;
+    j.kernel = kernel;
+    j.n = c->n;
+    j.n_pad = c->n_pad;
+    j.seg_len = seg_len;
+    j.nseg = nseg;
+    j.nta = nta;
+    j.ntb = ntb;
+    j.out = c->part;
+    L.jobs = nullptr;
+    L.items = nullptr;
+    L.nitems = 0;
     timer_begin(c);
-    if (mixed) {
-        ContractMixedArgs a;
-        a.P = c->points();
-        a.Zh = zp.zh;
-        a.Zl = zp.zl;
-        a.colscale = zp.scale;
-        a.bw_pt = bw_pt;
-        a.bw = bw;
-        a.kernel = kernel;
-        a.n = c->n;
-        a.n_pad = c->n_pad;
-        a.seg_len = seg_len;
-        a.nseg = nseg;
-        a.out = c->part;
-        GWM_HIP_CHECK(c, launch_contract_mixed(c->dist, mode, nta, ntb, a, c->stream));
-    } else {
-        ContractArgs a;
-        a.P = c->points();
-        a.Z = zp.z64;
-        a.bw_pt = bw_pt;
-        a.bw = bw;
-        a.kernel = kernel;
-        a.n = c->n;
-        a.n_pad = c->n_pad;
-        a.seg_len = seg_len;
-        a.nseg = nseg;
-        a.out = c->part;
-        GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, a, c->stream));
-    }
+    if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(c->dist, mode, nta, ntb, L, c->stream));
+    else GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, L, c->stream));
     if (c->timing) {
         std::string name = std::string("contract_") + (mixed ? "mixed_" : "fp64_") + mode_name(mode);
         if ((rc = timer_end(c, name.c_str()))) return rc;

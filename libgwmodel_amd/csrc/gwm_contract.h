@@ -12,35 +12,34 @@ namespace gwm {
 //   kModePlain : columns [pairs | x*y] with w (AICc criterion, multiscale fits)
 enum ContractMode { kModeCV = 0, kModeHat = 1, kModeQDiag = 2, kModePlain = 3 };
 
-struct ContractArgs {
+// One problem's sweep.  The FP64 kernel reads Z, the mixed kernel the FP16
+// planes Zh/Zl with their per-column unscale factors.  nta/ntb are the
+// problem's own tile counts (<= the kernel's compile-time maxima).
+struct SweepJob {
     Points P;
-    const double* Z;       // row-major [rows >= ceil(n/64)*64][NTA*16], zero padded
-    const double* bw_pt;   // per-point bandwidth (adaptive), or nullptr for a fixed bw
+    const double* Z;         // row-major [rows >= ceil(n/64)*64][nta*16], zero padded
+    const _Float16* Zh;      // [rows/32][nta*16][32] hi part
+    const _Float16* Zl;      // same, lo part scaled by 2^11
+    const double* colscale;  // per column unscale (power of two)
+    const double* bw_pt;     // per-point bandwidth (adaptive), or nullptr for a fixed bw
     double bw;
     int kernel;
     int n;
-    int n_pad;             // focus padding of the output, multiple of 128
-    int seg_len;           // data points per segment, multiple of 64
+    int n_pad;               // focus padding of the output, multiple of 128
+    int seg_len;             // data points per segment, multiple of 64
     int nseg;
-    double* out;           // [nseg][(NTA+NTB)*16][n_pad]
+    int nta, ntb;
+    double* out;             // [nseg][(nta+ntb)*16][n_pad]
 };
 
-// Mixed precision: Z is split into FP16 hi/lo planes, stored per 32-row
-// block in the MFMA B-operand order; accumulation FP32 in registers, flushed
-// to FP64 partial sums.
-struct ContractMixedArgs {
-    Points P;
-    const _Float16* Zh;    // [rows/32][NTA*16 cols][32 rows] hi part
-    const _Float16* Zl;    // same, lo part scaled by 2^11
-    const double* colscale;// per column 2^e scale applied before the split
-    const double* bw_pt;
-    double bw;
-    int kernel;
-    int n;
-    int n_pad;
-    int seg_len;           // multiple of 256
-    int nseg;
-    double* out;
+// Launch of one sweep: either a single job (items == nullptr; workgroup b
+// covers tile b / nseg and segment b % nseg) or a batch (items: triples
+// (job, tile, segment), one per workgroup, jobs[] in device memory).
+struct SweepLaunch {
+    SweepJob job0;
+    const SweepJob* jobs;
+    const int* items;
+    int nitems;
 };
 
 int contract_tile_focus(int nta, int ntb);
@@ -49,8 +48,12 @@ int contract_mixed_tile_focus(int nta, int ntb);
 // row-major FP64 table; colmax is ldz words of scratch.
 hipError_t launch_split_fp16(const double* Z, int rows, int ldz, unsigned long long* colmax, _Float16* Zh,
                              _Float16* Zl, double* unscale, hipStream_t s);
-hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const ContractArgs& a, hipStream_t s);
-hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const ContractMixedArgs& a, hipStream_t s);
+// nta/ntb select the compile-time tile counts (the batch maxima); grid size is
+// nitems (batch) or tiles * nseg (single job).
+hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s);
+hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s);
+// the R (focus tiles per wave) each kernel uses for a (nta, ntb) instance
+int contract_r(int nta, int ntb, int precision_mixed);
 
 // Adaptive bandwidth resolution (BandwidthWeight.cpp:40-55): per focus the
 // interpolated distance between the floor(bw)-th and floor(bw)+1-th nearest
@@ -88,6 +91,36 @@ struct SolveArgs {
 };
 
 hipError_t launch_solve(int k, int mode, const SolveArgs& a, hipStream_t s);
+// Batched solves of jobs with the same k: start[] (device, njobs+1 entries) is
+// the prefix sum of the jobs' n; total = start[njobs].
+hipError_t launch_solve_batch(int k, int mode, const SolveArgs* jobs, const int* start, int njobs, int total,
+                              hipStream_t s);
+
+struct QDiagJob {
+    const double* part;
+    int nseg, PT, n, n_pad;
+    const double* X;
+    const double* sii;
+    double* qdiag;
+};
+hipError_t launch_qdiag_batch(int k, const QDiagJob* jobs, const int* start, int njobs, int total, hipStream_t s);
+
+struct ReduceJob {
+    const double* a;
+    const double* b;
+    const double* c;
+    int n;
+    double* out;   // 3 sums
+};
+hipError_t launch_reduce_batch(const ReduceJob* jobs, int njobs, hipStream_t s);
+
+struct SelectJob {
+    Points P;
+    double bw;
+    double* out;
+};
+// Batched adaptive bandwidths: items are (job, focus) pairs, one workgroup each.
+hipError_t launch_adaptive_bw_batch(int dist, const SelectJob* jobs, const int* items, int nitems, hipStream_t s);
 
 // qDiag_j = 1 - 2 S_jj + x_j^T M_j x_j, M_j from the kModeQDiag contraction.
 hipError_t launch_qdiag(int k, const double* part, int nseg, int PT, int n, int n_pad, const double* X,

@@ -13,14 +13,18 @@
 // Tiling (one workgroup = 4 waves of 64 lanes):
 //   - a wave owns R tiles of 16 focus points; each lane produces one weight
 //     per tile and MFMA step: focus = lane % 16, data = lane / 16 (4 per step);
-//   - the data range is cut into nseg segments; blockIdx % nseg = segment so
-//     that consecutive workgroups (which the dispatcher spreads over the 8
-//     XCDs round-robin) stream different slices of Z and each XCD's L2 keeps
-//     one slice hot;
+//   - the data range is cut into nseg segments; in a single-problem launch
+//     blockIdx % nseg = segment, so consecutive workgroups (which the
+//     dispatcher spreads over the 8 XCDs round-robin) stream different slices
+//     of Z and each XCD's L2 keeps one slice hot;
 //   - Z rows and data coordinates of a 64-point chunk are staged in LDS and
 //     shared by the 4 waves.
-// Output: partial sums out[(seg * PT + c) * n_pad + i], reduced in a fixed
-// order by the solve kernels (deterministic, independent of scheduling).
+// A batched launch covers several problems: workgroup b takes the work item
+// (job, tile, segment) items[3b..3b+2]; each problem's own tile counts
+// (nta, ntb) may be below the kernel's compile-time maxima.
+// Output: partial sums out[(seg * PT + c) * n_pad + i], PT = (nta + ntb) * 16,
+// reduced in a fixed order by the solve kernels (deterministic, independent
+// of scheduling and of the other problems of a batch).
 #include "gwm_contract.h"
 
 namespace gwm {
@@ -80,17 +84,30 @@ __device__ __forceinline__ double pair_distance(const FocusCoords<DIST>& fc, con
 }
 
 template <int NTA, int NTB, int DIST, int MODE, int R>
-__global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs args) {
+__global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
     __shared__ SharedTile<NTA, DIST, MODE> sh;
-    constexpr int LDZ = NTA * 16;
-    constexpr int PT = (NTA + NTB) * 16;
+    int tile, seg;
+    const SweepJob* jp;
+    if (L.items) {
+        const int* it = L.items + 3 * blockIdx.x;
+        jp = L.jobs + it[0];
+        tile = it[1];
+        seg = it[2];
+    } else {
+        jp = &L.job0;
+        seg = blockIdx.x % L.job0.nseg;
+        tile = blockIdx.x / L.job0.nseg;
+    }
+    const SweepJob& job = *jp;
+    const int nta = NTA == 1 ? 1 : job.nta;     // the problem's own tile counts
+    const int ntb = NTB == 0 ? 0 : job.ntb;
+    const int ldz = nta * 16;
+    const int PT = (nta + ntb) * 16;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int seg = blockIdx.x % args.nseg;
-    const int tile = blockIdx.x / args.nseg;
     const int fbase = tile * (kWaves * 16 * R) + wave * 16 * R;
-    const int n = args.n;
+    const int n = job.n;
 
     // focus points owned by this lane (one per A tile)
     FocusCoords<DIST> fc[R];
@@ -101,8 +118,8 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs args) {
         int f = fbase + 16 * r + (lane & 15);
         fidx[r] = f;
         int fl = f < n ? f : n - 1;
-        fc[r] = load_focus<DIST>(args.P, fl);
-        fbw[r] = make_kconst64((MODE != kModeQDiag && args.bw_pt) ? args.bw_pt[fl] : args.bw);
+        fc[r] = load_focus<DIST>(job.P, fl);
+        fbw[r] = make_kconst64((MODE != kModeQDiag && job.bw_pt) ? job.bw_pt[fl] : job.bw);
     }
 
     v4d accA[R][NTA];
@@ -115,33 +132,32 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs args) {
         for (int t = 0; t < (NTB > 0 ? NTB : 1); ++t) accB[r][t] = v4d{0.0, 0.0, 0.0, 0.0};
     }
 
-    const int jbeg = seg * args.seg_len;
-    const int jend = min(n, jbeg + args.seg_len);
+    const int jbeg = seg * job.seg_len;
+    const int jend = min(n, jbeg + job.seg_len);
     const int kk = lane >> 4;
 
     for (int j0 = jbeg; j0 < jend; j0 += kChunk) {
         __syncthreads();
         // stage Z rows [j0, j0 + kChunk) (zero padded beyond n in memory)
         {
-            const double* src = args.Z + (size_t)j0 * LDZ;
-            double* dst = &sh.z[0][0];
-            for (int e = threadIdx.x; e < kChunk * LDZ; e += 256) dst[e] = src[e];
+            const double* src = job.Z + (size_t)j0 * ldz;
+            for (int e = threadIdx.x; e < kChunk * ldz; e += 256) sh.z[e / ldz][e % ldz] = src[e];
         }
         if (threadIdx.x < kChunk) {
             int j = j0 + threadIdx.x;
             int jl = j < n ? j : n - 1;
             if constexpr (DIST == kEuclidean) {
-                sh.c.x[threadIdx.x] = args.P.px[jl];
-                sh.c.y[threadIdx.x] = args.P.py[jl];
+                sh.c.x[threadIdx.x] = job.P.px[jl];
+                sh.c.y[threadIdx.x] = job.P.py[jl];
             } else {
-                sh.c.lon[threadIdx.x] = args.P.px[jl];
-                sh.c.lat[threadIdx.x] = args.P.py[jl];
-                sh.c.hx[threadIdx.x] = args.P.hx[jl];
-                sh.c.hy[threadIdx.x] = args.P.hy[jl];
-                sh.c.hz[threadIdx.x] = args.P.hz[jl];
-                sh.c.sl[threadIdx.x] = args.P.sl[jl];
+                sh.c.lon[threadIdx.x] = job.P.px[jl];
+                sh.c.lat[threadIdx.x] = job.P.py[jl];
+                sh.c.hx[threadIdx.x] = job.P.hx[jl];
+                sh.c.hy[threadIdx.x] = job.P.hy[jl];
+                sh.c.hz[threadIdx.x] = job.P.hz[jl];
+                sh.c.sl[threadIdx.x] = job.P.sl[jl];
             }
-            if constexpr (MODE == kModeQDiag) sh.kc[threadIdx.x] = make_kconst64(args.bw_pt ? args.bw_pt[jl] : args.bw);
+            if constexpr (MODE == kModeQDiag) sh.kc[threadIdx.x] = make_kconst64(job.bw_pt ? job.bw_pt[jl] : job.bw);
         }
         __syncthreads();
         const int steps = min(kChunk, jend - j0 + 3) / 4;
@@ -150,22 +166,23 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs args) {
             const int j = j0 + rr;
             double zb[NTA];
 #pragma unroll
-            for (int t = 0; t < NTA; ++t) zb[t] = sh.z[rr][16 * t + (lane & 15)];
+            for (int t = 0; t < NTA; ++t) zb[t] = t < nta ? sh.z[rr][16 * t + (lane & 15)] : 0.0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double d = pair_distance<DIST, NTA, MODE>(fc[r], sh, rr);
-                double w = kernel_fast_f64(args.kernel, d, MODE == kModeQDiag ? sh.kc[rr] : fbw[r]);
+                double w = kernel_fast_f64(job.kernel, d, MODE == kModeQDiag ? sh.kc[rr] : fbw[r]);
                 if (j >= jend) w = 0.0;
                 if (MODE == kModeCV && j == fidx[r]) w = 0.0;
                 double wa = MODE == kModeQDiag ? w * w : w;
 #pragma unroll
                 for (int t = 0; t < NTA; ++t)
-                    accA[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(wa, zb[t], accA[r][t], 0, 0, 0);
+                    if (t < nta) accA[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(wa, zb[t], accA[r][t], 0, 0, 0);
                 if (NTB > 0) {
                     double w2 = w * w;
 #pragma unroll
                     for (int t = 0; t < NTB; ++t)
-                        accB[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(w2, zb[t], accB[r][t], 0, 0, 0);
+                        if (t < ntb)
+                            accB[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(w2, zb[t], accB[r][t], 0, 0, 0);
                 }
             }
         }
@@ -173,36 +190,50 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs args) {
 
     // D layout of v_mfma_f64_16x16x4f64 (checked on gfx950): lane holds
     // column (lane % 16), rows (lane / 16) + 4 q, q = 0..3.
-    const size_t np = (size_t)args.n_pad;
-    double* out = args.out + (size_t)seg * PT * np;
+    const size_t np = (size_t)job.n_pad;
+    double* out = job.out + (size_t)seg * PT * np;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int f0 = fbase + 16 * r + kk;
 #pragma unroll
         for (int t = 0; t < NTA; ++t) {
+            if (t >= nta) continue;
             double* o = out + (size_t)(16 * t + (lane & 15)) * np + f0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) o[4 * q] = accA[r][t][q];
         }
 #pragma unroll
         for (int t = 0; t < NTB; ++t) {
-            double* o = out + (size_t)(LDZ + 16 * t + (lane & 15)) * np + f0;
+            if (t >= ntb) continue;
+            double* o = out + (size_t)(ldz + 16 * t + (lane & 15)) * np + f0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) o[4 * q] = accB[r][t][q];
         }
     }
 }
 
+template <int NTA, int NTB>
+constexpr int r_of() {
+    return (NTA + NTB) <= 8 ? 2 : 1;
+}
+
 template <int NTA, int NTB, int DIST, int MODE>
-hipError_t launch_f64_r(const ContractArgs& a, hipStream_t s) {
-    constexpr int R = (NTA + NTB) <= 8 ? 2 : 1;
-    int tiles = (a.n + kWaves * 16 * R - 1) / (kWaves * 16 * R);
-    hipLaunchKernelGGL((contract_f64_kernel<NTA, NTB, DIST, MODE, R>), dim3(tiles * a.nseg), dim3(256), 0, s, a);
+hipError_t launch_f64_r(const SweepLaunch& a, hipStream_t s) {
+    constexpr int R = r_of<NTA, NTB>();
+    int grid;
+    if (a.items) {
+        grid = a.nitems;
+    } else {
+        int tiles = (a.job0.n + kWaves * 16 * R - 1) / (kWaves * 16 * R);
+        grid = tiles * a.job0.nseg;
+    }
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL((contract_f64_kernel<NTA, NTB, DIST, MODE, R>), dim3(grid), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 template <int DIST, int MODE>
-hipError_t dispatch_ab(int nta, int ntb, const ContractArgs& a, hipStream_t s) {
+hipError_t dispatch_ab(int nta, int ntb, const SweepLaunch& a, hipStream_t s) {
 #define GWM_CASE(A, B) \
     if (nta == A && ntb == B) return launch_f64_r<A, B, DIST, MODE>(a, s);
     if (MODE == kModeHat) {
@@ -221,7 +252,7 @@ hipError_t dispatch_ab(int nta, int ntb, const ContractArgs& a, hipStream_t s) {
 
 int contract_tile_focus(int nta, int ntb) { return kWaves * 16 * (((nta + ntb) <= 8) ? 2 : 1); }
 
-hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const ContractArgs& a, hipStream_t s) {
+hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s) {
     if (dist == kEuclidean) {
         if (mode == kModeCV) return dispatch_ab<kEuclidean, kModeCV>(nta, ntb, a, s);
         if (mode == kModePlain) return dispatch_ab<kEuclidean, kModePlain>(nta, ntb, a, s);

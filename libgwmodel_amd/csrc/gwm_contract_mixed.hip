@@ -179,15 +179,28 @@ __device__ __forceinline__ void split16(float w, _Float16& hi, _Float16& lo) {
 }
 
 template <int NTA, int NTB, int DIST, int MODE, int R>
-__global__ __launch_bounds__(256) void contract_mixed_kernel(ContractMixedArgs args) {
+__global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     __shared__ MShared<NTA, DIST, MODE> sh;
-    constexpr int LDZ = NTA * 16;
-    constexpr int PT = (NTA + NTB) * 16;
+    int tile, seg;
+    const SweepJob* jp;
+    if (L.items) {
+        const int* it = L.items + 3 * blockIdx.x;
+        jp = L.jobs + it[0];
+        tile = it[1];
+        seg = it[2];
+    } else {
+        jp = &L.job0;
+        seg = blockIdx.x % L.job0.nseg;
+        tile = blockIdx.x / L.job0.nseg;
+    }
+    const SweepJob& args = *jp;
+    const int nta = NTA == 1 ? 1 : args.nta;    // the problem's own tile counts
+    const int ntb = NTB == 0 ? 0 : args.ntb;
+    const int ldz = nta * 16;
+    const int PT = (nta + ntb) * 16;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int seg = blockIdx.x % args.nseg;
-    const int tile = blockIdx.x / args.nseg;
     const int fbase = tile * (kWaves * 16 * R) + wave * 16 * R;
     const int n = args.n;
     const int kq = lane >> 4;       // which 8-row group of the 32-row step
@@ -221,13 +234,13 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(ContractMixedArgs a
         __syncthreads();
         // stage the two 32-row blocks of both planes: global [blk][col][32]
         {
-            const size_t base = (size_t)(j0 / 32) * LDZ * 32;
+            const size_t base = (size_t)(j0 / 32) * ldz * 32;
             const uint4* gh = reinterpret_cast<const uint4*>(args.Zh + base);
             const uint4* gl = reinterpret_cast<const uint4*>(args.Zl + base);
-            constexpr int vecs = 2 * LDZ * 4;   // 16-byte vectors per plane (8 halfs each)
+            const int vecs = 2 * ldz * 4;   // 16-byte vectors per plane (8 halfs each)
             for (int e = threadIdx.x; e < vecs; e += 256) {
-                const int blk = e / (LDZ * 4);
-                const int rem = e % (LDZ * 4);
+                const int blk = e / (ldz * 4);
+                const int rem = e % (ldz * 4);
                 const int c = rem >> 2, part = rem & 3;
                 *reinterpret_cast<uint4*>(&sh.zh[blk][c][part * 8]) = gh[e];
                 *reinterpret_cast<uint4*>(&sh.zl[blk][c][part * 8]) = gl[e];
@@ -261,8 +274,10 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(ContractMixedArgs a
             v8h bh[NTA], bl[NTA];
 #pragma unroll
             for (int t = 0; t < NTA; ++t) {
-                bh[t] = *reinterpret_cast<const v8h*>(&sh.zh[s][16 * t + col][8 * kq]);
-                bl[t] = *reinterpret_cast<const v8h*>(&sh.zl[s][16 * t + col][8 * kq]);
+                if (t < nta) {
+                    bh[t] = *reinterpret_cast<const v8h*>(&sh.zh[s][16 * t + col][8 * kq]);
+                    bl[t] = *reinterpret_cast<const v8h*>(&sh.zl[s][16 * t + col][8 * kq]);
+                }
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -307,12 +322,14 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(ContractMixedArgs a
                 }
 #pragma unroll
                 for (int t = 0; t < NTA; ++t) {
+                    if (t >= nta) continue;
                     hh[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[t], hh[r][t], 0, 0, 0);
                     xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[t], xx[r][t], 0, 0, 0);
                     xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], xx[r][t], 0, 0, 0);
                 }
 #pragma unroll
                 for (int t = 0; t < NTB; ++t) {
+                    if (t >= ntb) continue;
                     hh[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bh[t], hh[r][NTA + t], 0, 0, 0);
                     xx[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bl[t], xx[r][NTA + t], 0, 0, 0);
                     xx[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bh[t], xx[r][NTA + t], 0, 0, 0);
@@ -338,9 +355,12 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(ContractMixedArgs a
         const int f0 = fbase + 16 * r + 4 * kq;
 #pragma unroll
         for (int t = 0; t < NTA + NTB; ++t) {
-            const int zc = (t < NTA ? 16 * t : 16 * (t - NTA)) + col;
+            const bool isB = t >= NTA;
+            const int tt = isB ? t - NTA : t;
+            if (tt >= (isB ? ntb : nta)) continue;
+            const int zc = 16 * tt + col;
             const double unscale = args.colscale[zc];
-            double* o = out + (size_t)(16 * t + col) * np + f0;
+            double* o = out + (size_t)((isB ? ldz : 0) + zc) * np + f0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) o[q] = acc[r][t][q] * unscale;
         }
@@ -348,15 +368,22 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(ContractMixedArgs a
 }
 
 template <int NTA, int NTB, int DIST, int MODE>
-hipError_t launch_mixed_r(const ContractMixedArgs& a, hipStream_t s) {
+hipError_t launch_mixed_r(const SweepLaunch& a, hipStream_t s) {
     constexpr int R = (NTA + NTB) <= 6 ? 2 : 1;
-    int tiles = (a.n + kWaves * 16 * R - 1) / (kWaves * 16 * R);
-    hipLaunchKernelGGL((contract_mixed_kernel<NTA, NTB, DIST, MODE, R>), dim3(tiles * a.nseg), dim3(256), 0, s, a);
+    int grid;
+    if (a.items) {
+        grid = a.nitems;
+    } else {
+        int tiles = (a.job0.n + kWaves * 16 * R - 1) / (kWaves * 16 * R);
+        grid = tiles * a.job0.nseg;
+    }
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL((contract_mixed_kernel<NTA, NTB, DIST, MODE, R>), dim3(grid), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 template <int DIST, int MODE>
-hipError_t dispatch_mixed(int nta, int ntb, const ContractMixedArgs& a, hipStream_t s) {
+hipError_t dispatch_mixed(int nta, int ntb, const SweepLaunch& a, hipStream_t s) {
 #define GWM_CASE(A, B) \
     if (nta == A && ntb == B) return launch_mixed_r<A, B, DIST, MODE>(a, s);
     if (MODE == kModeHat) {
@@ -415,7 +442,12 @@ __global__ void split_kernel(const double* Z, int rows, int ldz, const unsigned 
 
 int contract_mixed_tile_focus(int nta, int ntb) { return kWaves * 16 * (((nta + ntb) <= 6) ? 2 : 1); }
 
-hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const ContractMixedArgs& a, hipStream_t s) {
+int contract_r(int nta, int ntb, int precision_mixed) {
+    if (precision_mixed) return (nta + ntb) <= 6 ? 2 : 1;
+    return (nta + ntb) <= 8 ? 2 : 1;
+}
+
+hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s) {
     if (dist == kEuclidean) {
         if (mode == kModeCV) return dispatch_mixed<kEuclidean, kModeCV>(nta, ntb, a, s);
         if (mode == kModePlain) return dispatch_mixed<kEuclidean, kModePlain>(nta, ntb, a, s);

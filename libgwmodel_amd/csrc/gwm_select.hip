@@ -20,8 +20,7 @@ __device__ __forceinline__ unsigned long long key_of(double v) { return (unsigne
 __device__ __forceinline__ double val_of(unsigned long long k) { return __longlong_as_double((long long)k); }
 
 template <int DIST>
-__global__ __launch_bounds__(256) void adaptive_bw_kernel(Points P, double bw, double* out) {
-    const int f = blockIdx.x;
+__device__ __forceinline__ void adaptive_bw_one(const Points& P, double bw, double* out, int f) {
     const int n = P.n;
     const int tid = threadIdx.x;
     __shared__ unsigned int hist[256];
@@ -101,6 +100,17 @@ __global__ __launch_bounds__(256) void adaptive_bw_kernel(Points P, double bw, d
 }
 
 template <int DIST>
+__global__ __launch_bounds__(256) void adaptive_bw_kernel(Points P, double bw, double* out) {
+    adaptive_bw_one<DIST>(P, bw, out, blockIdx.x);
+}
+
+template <int DIST>
+__global__ __launch_bounds__(256) void adaptive_bw_batch_kernel(const SelectJob* jobs, const int* items) {
+    const SelectJob& J = jobs[items[2 * blockIdx.x]];
+    adaptive_bw_one<DIST>(J.P, J.bw, J.out, items[2 * blockIdx.x + 1]);
+}
+
+template <int DIST>
 __global__ __launch_bounds__(256) void max_distance_kernel(Points P, double* block_max) {
     __shared__ double s_red[256];
     const int n = P.n;
@@ -130,6 +140,15 @@ hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out,
         hipLaunchKernelGGL(adaptive_bw_kernel<kEuclidean>, dim3(P.n), dim3(256), 0, s, P, bw, out);
     else
         hipLaunchKernelGGL(adaptive_bw_kernel<kGreatCircle>, dim3(P.n), dim3(256), 0, s, P, bw, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_adaptive_bw_batch(int dist, const SelectJob* jobs, const int* items, int nitems, hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    if (dist == kEuclidean)
+        hipLaunchKernelGGL(adaptive_bw_batch_kernel<kEuclidean>, dim3(nitems), dim3(256), 0, s, jobs, items);
+    else
+        hipLaunchKernelGGL(adaptive_bw_batch_kernel<kGreatCircle>, dim3(nitems), dim3(256), 0, s, jobs, items);
     return hipGetLastError();
 }
 

@@ -89,9 +89,7 @@ __device__ __forceinline__ double quad_form(const double* Bp, const double* g) {
 }
 
 template <int K, int MODE>
-__global__ __launch_bounds__(128) void solve_kernel(SolveArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+__device__ __forceinline__ void solve_one(const SolveArgs& a, int i) {
     const int n = a.n;
     constexpr int M = Sym<K>::M;
 
@@ -175,14 +173,39 @@ __global__ __launch_bounds__(128) void solve_kernel(SolveArgs a) {
         for (int q = p; q < K; ++q) Gi[Sym<K>::zcol(p, q)] = g[p] * g[q];
 }
 
+template <int K, int MODE>
+__global__ __launch_bounds__(128) void solve_kernel(SolveArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    solve_one<K, MODE>(a, i);
+}
+
+// Batched solves: thread t of the launch belongs to the job whose focus range
+// [start[j], start[j+1]) holds t (start is a prefix sum of the job sizes).
+__device__ __forceinline__ int find_job(const int* start, int njobs, int t) {
+    int lo = 0, hi = njobs;   // start[lo] <= t < start[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (start[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int K, int MODE>
+__global__ __launch_bounds__(128) void solve_batch_kernel(const SolveArgs* jobs, const int* start, int njobs) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= start[njobs]) return;
+    const int j = find_job(start, njobs, t);
+    solve_one<K, MODE>(jobs[j], t - start[j]);
+}
+
 template <int K>
-__global__ __launch_bounds__(128) void qdiag_kernel(const double* part, int nseg, int PT, int n, int n_pad,
-                                                    const double* X, const double* sii, double* qdiag) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+__device__ __forceinline__ void qdiag_one(const QDiagJob& J, int j) {
+    const int n = J.n;
     double x[K];
 #pragma unroll
-    for (int p = 0; p < K; ++p) x[p] = X[j + (size_t)p * n];
+    for (int p = 0; p < K; ++p) x[p] = J.X[j + (size_t)p * n];
     double q = 0.0;
 #pragma unroll
     for (int a = 0; a < K; ++a)
@@ -190,10 +213,25 @@ __global__ __launch_bounds__(128) void qdiag_kernel(const double* part, int nseg
         for (int b = a; b < K; ++b) {
             const int c = Sym<K>::zcol(a, b);
             double m = 0.0;
-            for (int g = 0; g < nseg; ++g) m += part[((size_t)g * PT + c) * n_pad + j];
+            for (int g = 0; g < J.nseg; ++g) m += J.part[((size_t)g * J.PT + c) * J.n_pad + j];
             q += (a == b ? 1.0 : 2.0) * m * x[a] * x[b];
         }
-    qdiag[j] = 1.0 - 2.0 * sii[j] + q;
+    J.qdiag[j] = 1.0 - 2.0 * J.sii[j] + q;
+}
+
+template <int K>
+__global__ __launch_bounds__(128) void qdiag_kernel(QDiagJob J) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= J.n) return;
+    qdiag_one<K>(J, j);
+}
+
+template <int K>
+__global__ __launch_bounds__(128) void qdiag_batch_kernel(const QDiagJob* jobs, const int* start, int njobs) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= start[njobs]) return;
+    const int j = find_job(start, njobs, t);
+    qdiag_one<K>(jobs[j], t - start[j]);
 }
 
 template <int DIST>
@@ -236,6 +274,36 @@ __global__ __launch_bounds__(1024) void reduce3_kernel(const double* a, const do
         out[0] = sa[0];
         out[1] = sb[0];
         out[2] = sc[0];
+    }
+}
+
+// Batched fixed-order sums: block b reduces job b's vectors (same order as a
+// one-problem reduce3 launch).
+__global__ __launch_bounds__(1024) void reduce_batch_kernel(const ReduceJob* jobs) {
+    const ReduceJob& J = jobs[blockIdx.x];
+    __shared__ double sa[1024], sb[1024], sc[1024];
+    double xa = 0.0, xb = 0.0, xc = 0.0;
+    for (int i = threadIdx.x; i < J.n; i += 1024) {
+        if (J.a) xa += J.a[i];
+        if (J.b) xb += J.b[i];
+        if (J.c) xc += J.c[i];
+    }
+    sa[threadIdx.x] = xa;
+    sb[threadIdx.x] = xb;
+    sc[threadIdx.x] = xc;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+            sc[threadIdx.x] += sc[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        J.out[0] = sa[0];
+        J.out[1] = sb[0];
+        J.out[2] = sc[0];
     }
 }
 
@@ -283,7 +351,27 @@ hipError_t launch_solve_k(int mode, const SolveArgs& a, hipStream_t s) {
 template <int K>
 hipError_t launch_qdiag_k(const double* part, int nseg, int PT, int n, int n_pad, const double* X,
                           const double* sii, double* q, hipStream_t s) {
-    hipLaunchKernelGGL(qdiag_kernel<K>, dim3((n + 127) / 128), dim3(128), 0, s, part, nseg, PT, n, n_pad, X, sii, q);
+    QDiagJob J{part, nseg, PT, n, n_pad, X, sii, q};
+    hipLaunchKernelGGL(qdiag_kernel<K>, dim3((n + 127) / 128), dim3(128), 0, s, J);
+    return hipGetLastError();
+}
+
+template <int K>
+hipError_t launch_solve_batch_k(int mode, const SolveArgs* jobs, const int* start, int njobs, int total,
+                                hipStream_t s) {
+    dim3 grid((total + 127) / 128);
+    if (total <= 0) return hipSuccess;
+    if (mode == kSolveCV) hipLaunchKernelGGL((solve_batch_kernel<K, kSolveCV>), grid, dim3(128), 0, s, jobs, start, njobs);
+    else if (mode == kSolveAIC) hipLaunchKernelGGL((solve_batch_kernel<K, kSolveAIC>), grid, dim3(128), 0, s, jobs, start, njobs);
+    else if (mode == kSolveBeta) hipLaunchKernelGGL((solve_batch_kernel<K, kSolveBeta>), grid, dim3(128), 0, s, jobs, start, njobs);
+    else hipLaunchKernelGGL((solve_batch_kernel<K, kSolveFit>), grid, dim3(128), 0, s, jobs, start, njobs);
+    return hipGetLastError();
+}
+
+template <int K>
+hipError_t launch_qdiag_batch_k(const QDiagJob* jobs, const int* start, int njobs, int total, hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(qdiag_batch_kernel<K>, dim3((total + 127) / 128), dim3(128), 0, s, jobs, start, njobs);
     return hipGetLastError();
 }
 
@@ -309,6 +397,25 @@ hipError_t launch_qdiag(int k, const double* part, int nseg, int PT, int n, int 
 #define GWM_QD(K) launch_qdiag_k<K>(part, nseg, PT, n, n_pad, X, sii, qdiag, s)
     GWM_K_SWITCH(k, GWM_QD)
 #undef GWM_QD
+}
+
+hipError_t launch_solve_batch(int k, int mode, const SolveArgs* jobs, const int* start, int njobs, int total,
+                              hipStream_t s) {
+#define GWM_SB(K) launch_solve_batch_k<K>(mode, jobs, start, njobs, total, s)
+    GWM_K_SWITCH(k, GWM_SB)
+#undef GWM_SB
+}
+
+hipError_t launch_qdiag_batch(int k, const QDiagJob* jobs, const int* start, int njobs, int total, hipStream_t s) {
+#define GWM_QB(K) launch_qdiag_batch_k<K>(jobs, start, njobs, total, s)
+    GWM_K_SWITCH(k, GWM_QB)
+#undef GWM_QB
+}
+
+hipError_t launch_reduce_batch(const ReduceJob* jobs, int njobs, hipStream_t s) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_batch_kernel, dim3(njobs), dim3(1024), 0, s, jobs);
+    return hipGetLastError();
 }
 
 hipError_t launch_smatrix(int dist, int k, const Points& P, const double* X, const double* gvec,
