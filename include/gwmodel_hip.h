@@ -145,6 +145,40 @@ int gwm_hip_reset_timers(gwm_hip_ctx* ctx);
 /* Local solves (focus x bandwidth pairs) evaluated since the last reset. */
 int gwm_hip_solve_count(gwm_hip_ctx* ctx, long long* solves);
 
+/* ------------------------------------------------------------------------
+ * Batched / continuous-batching execution.  No counterpart in the reference,
+ * which runs one problem per fit() and has no batching or serving layer.
+ *
+ * A server owns one device stream and runs many GWRBasic requests at once: a
+ * fit at a given bandwidth, or a golden-section bandwidth search followed by
+ * the fit (select_bw).  Requests are admitted first come, first served while
+ * max_inflight slots and the device memory budget allow (a request's footprint
+ * is O(n k^2), never n^2); the criterion and fit evaluations of all in-flight
+ * requests are executed together in rounds of batched launches.  A request's
+ * result does not depend on which other requests share its rounds.  Output
+ * pointers (nullable; betas and betasSE n x k, shat 2, qDiag n, diag 7, bw_out
+ * 1) must stay valid until gwm_hip_server_wait returns for the request.
+ * hbm_budget_bytes <= 0 means 80% of the free device memory at creation.
+ */
+typedef struct gwm_hip_server gwm_hip_server;
+int gwm_hip_server_create(int device, int precision, double hbm_budget_bytes, int max_inflight,
+                          gwm_hip_server** out);
+int gwm_hip_server_set_certify_tolerance(gwm_hip_server* server, double rel_tol);
+int gwm_hip_server_submit(gwm_hip_server* server, int n, int k, const double* coords, const double* X,
+                          const double* y, int distance, int kernel, int adaptive, double bw, int select_bw,
+                          int criterion, int has_lower, double lower, int has_upper, double upper, double* betas,
+                          double* betasSE, double* shat, double* qDiag, double* diag, double* bw_out,
+                          long long* request_id);
+/* Blocks until the request is done; status is the request's GWM_* code,
+ * latency_ms runs from submission to completion, solves counts its local
+ * solves.  The request id is released. */
+int gwm_hip_server_wait(gwm_hip_server* server, long long request_id, int* status, double* latency_ms,
+                        long long* solves);
+int gwm_hip_server_stats(gwm_hip_server* server, long long* rounds, long long* solves, double* busy_ms,
+                         double* peak_bytes, int* peak_inflight);
+const char* gwm_hip_server_last_error(gwm_hip_server* server);
+void gwm_hip_server_destroy(gwm_hip_server* server);
+
 #ifdef __cplusplus
 }
 #endif

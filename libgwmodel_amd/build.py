@@ -22,7 +22,8 @@ FACADE_SRC = os.path.join(HERE, "cpp", "gwr_facade.cpp")
 FACADE_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
 FACADE_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_facade")
 
-SOURCES = ["gwm_contract.hip", "gwm_contract_mixed.hip", "gwm_select.hip", "gwm_solve.hip", "gwm_api.cpp"]
+SOURCES = ["gwm_contract.hip", "gwm_contract_mixed.hip", "gwm_select.hip", "gwm_solve.hip", "gwm_api.cpp",
+           "gwm_batch.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -54,7 +55,7 @@ def build(verbose=False):
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))

@@ -56,6 +56,15 @@ SIGNATURES = {
     "gwm_hip_kernel_time": (_i, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong), _dp]),
     "gwm_hip_reset_timers": (_i, [_vp]),
     "gwm_hip_solve_count": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    "gwm_hip_server_create": (_i, [_i, _i, _d, _i, ctypes.POINTER(_vp)]),
+    "gwm_hip_server_set_certify_tolerance": (_i, [_vp, _d]),
+    "gwm_hip_server_submit": (_i, [_vp, _i, _i, _dp, _dp, _dp, _i, _i, _i, _d, _i, _i, _i, _d, _i, _d, _dp, _dp, _dp,
+                                   _dp, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "gwm_hip_server_wait": (_i, [_vp, ctypes.c_longlong, _ip, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "gwm_hip_server_stats": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), _dp, _dp,
+                                  _ip]),
+    "gwm_hip_server_last_error": (ctypes.c_char_p, [_vp]),
+    "gwm_hip_server_destroy": (None, [_vp]),
 }
 
 _lib = None
@@ -271,6 +280,85 @@ class Context:
         n = ctypes.c_longlong(0)
         self._check(self._lib.gwm_hip_solve_count(self.h, ctypes.byref(n)))
         return n.value
+
+
+class Server:
+    """Continuous-batching server: many GWRBasic requests in flight on one device."""
+
+    def __init__(self, device=0, precision=FP64, hbm_budget_bytes=0.0, max_inflight=64):
+        self._lib = load()
+        h = _vp()
+        rc = self._lib.gwm_hip_server_create(device, int(precision), float(hbm_budget_bytes), int(max_inflight),
+                                             ctypes.byref(h))
+        if rc != GWM_OK:
+            raise GwmError(rc, "cannot create a server on device %d" % device)
+        self.h = h
+        self._live = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.gwm_hip_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_certify_tolerance(self, rel):
+        rc = self._lib.gwm_hip_server_set_certify_tolerance(self.h, float(rel))
+        if rc != GWM_OK:
+            raise GwmError(rc)
+
+    def submit(self, coords, X, y, kernel, adaptive, bw, select_bw=False, crit=CRIT_CV, distance=EUCLIDEAN,
+               lower=None, upper=None):
+        """Queue one request; returns its id.  Results are collected by wait()."""
+        c = np.asfortranarray(coords, dtype=np.float64)
+        Xf = np.asfortranarray(X, dtype=np.float64)
+        yf = np.ascontiguousarray(y, dtype=np.float64)
+        n, k = Xf.shape
+        out = dict(betas=np.zeros((n, k), order="F"), betasSE=np.zeros((n, k), order="F"), shat=np.zeros(2),
+                   qDiag=np.zeros(n), diag=np.zeros(7), bandwidth=np.zeros(1))
+        rid = ctypes.c_longlong(0)
+        rc = self._lib.gwm_hip_server_submit(
+            self.h, n, k, _p(c), _p(Xf), _p(yf), int(distance), int(kernel), int(adaptive), float(bw),
+            int(select_bw), int(crit), int(lower is not None), float(lower or 0), int(upper is not None),
+            float(upper or 0), _p(out["betas"]), _p(out["betasSE"]), _p(out["shat"]), _p(out["qDiag"]),
+            _p(out["diag"]), _p(out["bandwidth"]), ctypes.byref(rid))
+        if rc != GWM_OK:
+            raise GwmError(rc, "submit failed")
+        self._live[rid.value] = (out, c, Xf, yf)
+        return rid.value
+
+    def wait(self, rid):
+        """Blocks until request rid is done; returns its results (raises on error)."""
+        st = ctypes.c_int(0)
+        lat = ctypes.c_double(0)
+        sv = ctypes.c_longlong(0)
+        rc = self._lib.gwm_hip_server_wait(self.h, ctypes.c_longlong(rid), ctypes.byref(st), ctypes.byref(lat),
+                                           ctypes.byref(sv))
+        if rc != GWM_OK:
+            raise GwmError(rc, "wait failed")
+        out = self._live.pop(rid)[0]
+        res = dict(status=st.value, latency_ms=lat.value, solves=sv.value, betas=out["betas"],
+                   betasSE=out["betasSE"], shat=out["shat"], qDiag=out["qDiag"], bandwidth=float(out["bandwidth"][0]),
+                   diagnostic=dict(zip(["RSS", "AIC", "AICc", "ENP", "EDF", "RSquare", "RSquareAdjust"],
+                                       out["diag"])))
+        return res
+
+    def stats(self):
+        r = ctypes.c_longlong(0)
+        s = ctypes.c_longlong(0)
+        b = ctypes.c_double(0)
+        pb = ctypes.c_double(0)
+        pi = ctypes.c_int(0)
+        self._lib.gwm_hip_server_stats(self.h, ctypes.byref(r), ctypes.byref(s), ctypes.byref(b), ctypes.byref(pb),
+                                       ctypes.byref(pi))
+        return dict(rounds=r.value, solves=s.value, busy_ms=b.value, peak_bytes=pb.value, peak_inflight=pi.value)
 
 
 def header_symbols(path=HEADER_PATH):

@@ -16,75 +16,23 @@
 #include <string>
 #include <vector>
 
-#include "../../include/gwmodel_hip.h"
-#include "gwm_contract.h"
+#include "gwm_ctx.h"
 
 using namespace gwm;
+using namespace gwm::detail;
 
-namespace {
-
-const double kPi = 3.14159265358979323846;
+namespace gwm {
+namespace detail {
 
 int nta_of(int k) { return (k * (k + 1) / 2 + k + 15) / 16; }
 int ntb_of(int k) { return (k * (k + 1) / 2 + 15) / 16; }
 
-struct KernelStat {
-    long long launches = 0;
-    double ms = 0.0;
-};
-
-}  // namespace
-
-struct gwm_hip_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    int (*stop)(void*) = nullptr;
-    void* stop_user = nullptr;
-
-    // problem
-    int n = 0, k = 0, dist = 0;
-    int nta = 0, ntb = 0, m = 0;
-    int n_pad = 0, rows_pad = 0;
-    std::vector<double> hcoords, hX, hy;
-
-    // device buffers
-    double *px = nullptr, *py = nullptr, *hx = nullptr, *hy_ = nullptr, *hz = nullptr, *sl = nullptr;
-    double *X = nullptr, *y = nullptr, *Z = nullptr, *G = nullptr;
-    double* part = nullptr;
-    size_t part_bytes = 0;
-    double *bwi = nullptr, *res2 = nullptr, *sii = nullptr, *ssi = nullptr;
-    double *betas = nullptr, *se = nullptr, *gvec = nullptr, *qd = nullptr, *red = nullptr, *blockmax = nullptr;
-    double *vx = nullptr, *vy = nullptr, *vZ = nullptr;
-    double* S = nullptr;
-    int* fail = nullptr;
-    // FP16 hi/lo planes for the mixed-precision sweeps (Z, G and the
-    // single-column table) with their per-column unscale factors
-    _Float16 *Zh = nullptr, *Zl = nullptr, *Gh = nullptr, *Gl = nullptr, *vZh = nullptr, *vZl = nullptr;
-    double *zscale = nullptr, *gscale = nullptr, *vscale = nullptr;
-    unsigned long long* colmax = nullptr;
-    double certify_rel = 1e-7;   // relative bound used to certify mixed comparisons
-
-    // timing
-    bool timing = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::map<std::string, KernelStat> stats;
-    long long solves = 0;
-
-    Points points() const {
-        Points P;
-        P.px = px;
-        P.py = py;
-        P.hx = hx;
-        P.hy = hy_;
-        P.hz = hz;
-        P.sl = sl;
-        P.n = n;
-        return P;
-    }
-};
+}  // namespace detail
+}  // namespace gwm
 
 namespace {
+
+const double kPi = 3.14159265358979323846;
 
 #define GWM_HIP_CHECK(ctx, expr)                                                         \
     do {                                                                                 \
@@ -262,9 +210,43 @@ SolveArgs solve_args(gwm_hip_ctx* c, int PT, int nseg, int colB, const double* b
     return s;
 }
 
+}  // namespace
+
+namespace gwm {
+namespace detail {
+
+void calc_diagnostic(int n_, int k, const double* X, const double* y, const double* betas, const double* shat,
+                     double* diag) {
+    double rss = 0.0;
+    for (int i = 0; i < n_; ++i) {
+        double f = 0.0;
+        for (int a = 0; a < k; ++a) f += betas[i + (size_t)a * n_] * X[i + (size_t)a * n_];
+        double r = y[i] - f;
+        rss += r * r;
+    }
+    const double n = (double)n_;
+    double ym = 0.0;
+    for (int i = 0; i < n_; ++i) ym += y[i];
+    ym /= n;
+    double yss = 0.0;
+    for (int i = 0; i < n_; ++i) yss += (y[i] - ym) * (y[i] - ym);
+    double AIC = n * std::log(rss / n) + n * std::log(2 * kPi) + n + shat[0];
+    double AICc = aicc(rss, n, shat[0]);
+    double edf = n - 2 * shat[0] + shat[1];
+    double enp = 2 * shat[0] - shat[1];
+    double r2 = 1 - rss / yss;
+    double r2adj = 1 - (1 - r2) * (n - 1) / (edf - 1);
+    diag[0] = rss; diag[1] = AIC; diag[2] = AICc; diag[3] = enp; diag[4] = edf; diag[5] = r2; diag[6] = r2adj;
+}
+
 double aicc(double rss, double n, double trS) {
     return n * std::log(rss / n) + n * std::log(2 * kPi) + n * ((n + trS) / (n - 2 - trS));
 }
+
+}  // namespace detail
+}  // namespace gwm
+
+namespace {
 
 int check_ready(gwm_hip_ctx* c) {
     if (!c) return GWM_ERR_INVALID;
@@ -330,7 +312,10 @@ int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit
 // values, the search has entered the region where the criterion varies less
 // than the mixed-precision error, and later candidates are evaluated in FP64
 // directly instead of twice.
-typedef std::function<int(double, bool, double*)> CritFn;
+}  // namespace
+
+namespace gwm {
+namespace detail {
 
 int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certify, double tol_rel,
            double* xopt_out, std::map<double, double>& log) {
@@ -426,6 +411,16 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
     return GWM_OK;
 }
 
+}  // namespace detail
+}  // namespace gwm
+
+namespace {
+
+}  // namespace
+
+namespace gwm {
+namespace detail {
+
 void write_trace(const std::map<double, double>& log, double* tx, double* tf, int cap, int* len) {
     int m = 0;
     for (const auto& kv : log) {
@@ -438,6 +433,16 @@ void write_trace(const std::map<double, double>& log, double* tx, double* tf, in
     if (len) *len = m;
 }
 
+}  // namespace detail
+}  // namespace gwm
+
+namespace {
+
+}  // namespace
+
+namespace gwm {
+namespace detail {
+
 int max_distance_impl(gwm_hip_ctx* c, double* out) {
     int nblocks = (c->n + 255) / 256;
     GWM_HIP_CHECK(c, launch_max_distance(c->dist, c->points(), c->blockmax, nblocks, c->stream));
@@ -449,6 +454,11 @@ int max_distance_impl(gwm_hip_ctx* c, double* out) {
     *out = m;
     return GWM_OK;
 }
+
+}  // namespace detail
+}  // namespace gwm
+
+namespace {
 
 // Upload the design and its product table Z = [x_a x_b (a <= b) | x_a y].
 int upload_design(gwm_hip_ctx* c, const double* X, const double* y) {
@@ -761,28 +771,8 @@ int gwm_hip_gwr_run(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int sel
     if ((rc = gwm_hip_fit(c, kernel, adaptive, bw, precision, betas, betasSE, shat, qDiag, S))) return rc;
     // CalcDiagnostic (GWRBasic.cpp:21-34) and SE scaling (:100-103)
     const int n_ = c->n, k = c->k;
-    const double* X = c->hX.data();
-    const double* y = c->hy.data();
-    double rss = 0.0;
-    for (int i = 0; i < n_; ++i) {
-        double f = 0.0;
-        for (int a = 0; a < k; ++a) f += betas[i + (size_t)a * n_] * X[i + (size_t)a * n_];
-        double r = y[i] - f;
-        rss += r * r;
-    }
-    const double n = (double)n_;
-    double ym = 0.0;
-    for (int i = 0; i < n_; ++i) ym += y[i];
-    ym /= n;
-    double yss = 0.0;
-    for (int i = 0; i < n_; ++i) yss += (y[i] - ym) * (y[i] - ym);
-    double AIC = n * std::log(rss / n) + n * std::log(2 * kPi) + n + shat[0];
-    double AICc = aicc(rss, n, shat[0]);
-    double edf = n - 2 * shat[0] + shat[1];
-    double enp = 2 * shat[0] - shat[1];
-    double r2 = 1 - rss / yss;
-    double r2adj = 1 - (1 - r2) * (n - 1) / (edf - 1);
-    diag[0] = rss; diag[1] = AIC; diag[2] = AICc; diag[3] = enp; diag[4] = edf; diag[5] = r2; diag[6] = r2adj;
+    calc_diagnostic(n_, k, c->hX.data(), c->hy.data(), betas, shat, diag);
+    const double n = (double)n_, rss = diag[0];
     double sigma = rss / (n - 2 * shat[0] + shat[1]);
     for (size_t i = 0; i < (size_t)n_ * k; ++i) betasSE[i] = std::sqrt(sigma * betasSE[i]);
     return GWM_OK;

@@ -1,0 +1,724 @@
+// Continuous-batching scheduler: many GWR problems in flight on one device.
+//
+// The reference runs one problem per fit() call (src/gwmodelpp/GWRBasic.cpp:36-128)
+// and has no batching or serving layer; this file adds one.  Each request (a
+// GWRBasic fit at a given bandwidth, or a golden-section bandwidth search
+// followed by the fit) runs the unchanged host algorithm -- the golden section
+// of BandwidthSelector::optimize (BandwidthSelector.cpp:6-81) and the fit /
+// diagnostics of GWRBasic::fit -- on a worker thread of its own.  Every
+// criterion or fit evaluation the worker needs is posted to the scheduler and
+// the worker blocks.  The scheduler runs in rounds: once every in-flight
+// request has posted (or finished), it executes all posted evaluations as
+// batched launches -- one adaptive-bandwidth launch, one sweep launch per
+// kernel class, one solve launch per k, one reduction launch -- on its stream,
+// then hands the results back.  Between rounds it retires finished requests
+// and admits waiting ones while their device footprint (O(n k^2): coordinates,
+// design, product table and its FP16 planes, partial sums; never n^2) fits
+// the memory budget.
+//
+// Each request's arithmetic depends only on its own data: its data segments
+// are a function of its n alone and every reduction has a fixed order, so a
+// request returns the same bits alone or in any mix.
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "gwm_ctx.h"
+
+using namespace gwm;
+using namespace gwm::detail;
+
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+enum { kEvalCrit = 0, kEvalFit = 1 };
+
+struct Eval {
+    int type = kEvalCrit;
+    double bw = 0.0;
+    int precision = GWM_FP64;
+    int crit = GWM_CRIT_CV;
+    int rc = GWM_OK;
+    double value = DBL_MAX;
+    double shat[2] = {0.0, 0.0};
+};
+
+struct Request {
+    long long id = 0;
+    int n = 0, k = 0, dist = 0, kernel = 0, adaptive = 0, select_bw = 0, crit = 0;
+    double bw = 0.0, lower = 0.0, upper = 0.0;
+    int has_lower = 0, has_upper = 0;
+    std::vector<double> coords, X, y;
+    double *o_betas = nullptr, *o_se = nullptr, *o_shat = nullptr, *o_qdiag = nullptr, *o_diag = nullptr,
+           *o_bw = nullptr;
+    // device state (from admission to retirement)
+    gwm_hip_ctx* ctx = nullptr;
+    double* part = nullptr;
+    int nseg = 1, seg_len = 64;
+    size_t footprint = 0;
+    double maxdist = 0.0;
+    // timing and status
+    double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
+    int status = GWM_OK;
+    bool admitted = false, finished = false, done = false;
+    long long solves = 0;
+    // worker handshake
+    std::thread worker;
+    bool posted = false;
+    Eval ev;
+    // host staging of fit outputs
+    std::vector<double> hbetas, hse, hq;
+};
+
+// Data segments of a request in the batched sweeps: a function of n alone.
+void batch_segments(int n, int& nseg, int& seg_len) {
+    nseg = std::max(1, (n + 8191) / 8192);
+    seg_len = ((n + nseg - 1) / nseg + 63) / 64 * 64;
+    nseg = (n + seg_len - 1) / seg_len;
+}
+
+size_t footprint_bytes(int n, int k, int dist, int nseg) {
+    const size_t rows = (size_t)(n + 63) / 64 * 64, np = (size_t)(n + 127) / 128 * 128;
+    const size_t nta = nta_of(k), ntb = ntb_of(k);
+    size_t b = 0;
+    b += (size_t)n * (dist == GWM_GREAT_CIRCLE ? 6 : 2) * 8;        // coordinates
+    b += (size_t)n * (k + 1) * 8;                                       // X, y
+    b += rows * nta * 16 * (8 + 4) + rows * ntb * 16 * (8 + 4);        // Z, G and FP16 planes
+    b += rows * 16 * (8 + 4) + (size_t)n * 16;                          // single-column tables
+    b += (size_t)n * 7 * 8 + (size_t)n * k * 3 * 8;                     // per-focus vectors
+    b += (size_t)nseg * (nta + ntb) * 16 * np * 8;                      // partial sums
+    return b;
+}
+
+// Device arena for the per-round job tables.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct gwm_hip_server {
+    int device = 0;
+    int precision = GWM_FP64;
+    size_t budget = 0;
+    int max_inflight = 1;
+    double certify_rel = 1e-7;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Request*> pending;
+    std::vector<Request*> active;
+    std::map<long long, Request*> reqs;
+    long long next_id = 1;
+    bool stopping = false;
+    std::thread sched;
+    size_t inflight_bytes = 0, peak_bytes = 0;
+    int peak_inflight = 0;
+    long long rounds = 0, solves = 0;
+    double busy_ms = 0.0;
+    std::string err;
+    Arena arena;
+    double* round_red = nullptr;
+    int* round_fail = nullptr;
+    int round_cap = 0;
+};
+
+namespace {
+
+#define SRV_CHECK(S, expr)                                                 \
+    do {                                                                   \
+        hipError_t e__ = (expr);                                           \
+        if (e__ != hipSuccess) {                                           \
+            (S)->err = std::string(#expr) + ": " + hipGetErrorString(e__); \
+            return e__ == hipErrorOutOfMemory ? GWM_ERR_OOM : GWM_ERR_HIP; \
+        }                                                                  \
+    } while (0)
+
+// Host-side job tables of one round, uploaded into the arena in one pass.
+struct Tables {
+    std::vector<char> host;
+    size_t push(const void* p, size_t bytes) {
+        size_t off = (host.size() + 255) / 256 * 256;
+        host.resize(off + bytes);
+        std::memcpy(host.data() + off, p, bytes);
+        return off;
+    }
+};
+
+struct SweepGroup {
+    int dist, mode, precision, nta, ntb;
+    std::vector<SweepJob> jobs;
+    std::vector<int> items;
+    size_t off_jobs = 0, off_items = 0;
+};
+
+struct SolveGroup {
+    int k, mode;
+    std::vector<SolveArgs> jobs;
+    std::vector<int> start;
+    size_t off_jobs = 0, off_start = 0;
+};
+
+struct QdGroup {
+    int k;
+    std::vector<QDiagJob> jobs;
+    std::vector<int> start;
+    size_t off_jobs = 0, off_start = 0;
+};
+
+SweepJob sweep_job(Request* r, int mode, int nta, int ntb, const double* bw_pt, double bw) {
+    gwm_hip_ctx* c = r->ctx;
+    SweepJob j;
+    j.P = c->points();
+    if (mode == kModeQDiag) {
+        j.Z = c->G;
+        j.Zh = c->Gh;
+        j.Zl = c->Gl;
+        j.colscale = c->gscale;
+    } else {
+        j.Z = c->Z;
+        j.Zh = c->Zh;
+        j.Zl = c->Zl;
+        j.colscale = c->zscale;
+    }
+    j.bw_pt = bw_pt;
+    j.bw = bw;
+    j.kernel = r->kernel;
+    j.n = c->n;
+    j.n_pad = c->n_pad;
+    j.seg_len = r->seg_len;
+    j.nseg = r->nseg;
+    j.nta = nta;
+    j.ntb = ntb;
+    j.out = r->part;
+    return j;
+}
+
+void add_items(SweepGroup& g, int job, int n, int nseg, int R) {
+    const int tiles = (n + 64 * R - 1) / (64 * R);
+    for (int t = 0; t < tiles; ++t)
+        for (int s = 0; s < nseg; ++s) {
+            g.items.push_back(job);
+            g.items.push_back(t);
+            g.items.push_back(s);
+        }
+}
+
+SweepGroup& group_of(std::vector<SweepGroup>& gs, int dist, int mode, int prec, int nta, int ntb) {
+    for (auto& g : gs)
+        if (g.dist == dist && g.mode == mode && g.precision == prec && g.nta == nta && g.ntb == ntb) return g;
+    gs.push_back(SweepGroup{dist, mode, prec, nta, ntb, {}, {}, 0, 0});
+    return gs.back();
+}
+
+template <class G>
+G& kgroup_of(std::vector<G>& gs, int k) {
+    for (auto& g : gs)
+        if (g.k == k) return g;
+    gs.push_back(G{});
+    gs.back().k = k;
+    return gs.back();
+}
+
+SolveGroup& sgroup_of(std::vector<SolveGroup>& gs, int k, int mode) {
+    for (auto& g : gs)
+        if (g.k == k && g.mode == mode) return g;
+    gs.push_back(SolveGroup{k, mode, {}, {}, 0, 0});
+    return gs.back();
+}
+
+int launch_sweeps(gwm_hip_server* S, std::vector<SweepGroup>& gs, const char* dev) {
+    for (auto& g : gs) {
+        SweepLaunch L;
+        std::memset(&L.job0, 0, sizeof(L.job0));
+        L.jobs = reinterpret_cast<const SweepJob*>(dev + g.off_jobs);
+        L.items = reinterpret_cast<const int*>(dev + g.off_items);
+        L.nitems = (int)g.items.size() / 3;
+        if (g.precision == GWM_MIXED) SRV_CHECK(S, launch_contract_mixed(g.dist, g.mode, g.nta, g.ntb, L, S->stream));
+        else SRV_CHECK(S, launch_contract_f64(g.dist, g.mode, g.nta, g.ntb, L, S->stream));
+    }
+    return GWM_OK;
+}
+
+// Execute the posted evaluations of one round.
+int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
+    const double t0 = now_ms();
+    const int ne = (int)batch.size();
+    if (ne > S->round_cap) {
+        if (S->round_red) hipFree(S->round_red);
+        if (S->round_fail) hipFree(S->round_fail);
+        S->round_red = nullptr;
+        S->round_fail = nullptr;
+        S->round_cap = 0;
+        SRV_CHECK(S, hipMalloc((void**)&S->round_red, (size_t)ne * 2 * 3 * sizeof(double)));
+        SRV_CHECK(S, hipMalloc((void**)&S->round_fail, (size_t)ne * 2 * sizeof(int)));
+        S->round_cap = ne * 2;
+    }
+    Tables T;
+    std::vector<SelectJob> sel[2];
+    std::vector<int> sel_items[2];
+    std::vector<SweepGroup> sweeps, qsweeps;
+    std::vector<SolveGroup> solves;
+    std::vector<QdGroup> qds;
+    std::vector<ReduceJob> reds;
+    std::vector<const double*> bwpts(ne, nullptr);
+    for (int e = 0; e < ne; ++e) {
+        Request* r = batch[e];
+        const Eval& ev = r->ev;
+        if (r->adaptive) {
+            const int d = r->dist;
+            SelectJob sj{r->ctx->points(), ev.bw, r->ctx->bwi};
+            const int job = (int)sel[d].size();
+            sel[d].push_back(sj);
+            for (int f = 0; f < r->n; ++f) {
+                sel_items[d].push_back(job);
+                sel_items[d].push_back(f);
+            }
+            bwpts[e] = r->ctx->bwi;
+        }
+        const int k = r->k, nta = nta_of(k), ntb = ntb_of(k);
+        const int prec = ev.precision;
+        int mode, smode, PT, colB = 0;
+        if (ev.type == kEvalCrit) {
+            mode = ev.crit == GWM_CRIT_CV ? kModeCV : kModePlain;
+            smode = ev.crit == GWM_CRIT_CV ? kSolveCV : kSolveAIC;
+            PT = nta * 16;
+        } else {
+            mode = kModeHat;
+            smode = kSolveFit;
+            PT = (nta + ntb) * 16;
+            colB = nta * 16;
+        }
+        SweepGroup& g = group_of(sweeps, r->dist, mode, prec, nta, mode == kModeHat ? ntb : 0);
+        g.jobs.push_back(sweep_job(r, mode, nta, mode == kModeHat ? ntb : 0, bwpts[e], ev.bw));
+        add_items(g, (int)g.jobs.size() - 1, r->n, r->nseg, contract_r(nta, mode == kModeHat ? ntb : 0, prec == GWM_MIXED));
+        gwm_hip_ctx* c = r->ctx;
+        SolveArgs sa;
+        sa.part = r->part;
+        sa.nseg = r->nseg;
+        sa.PT = PT;
+        sa.colB = colB;
+        sa.n = c->n;
+        sa.n_pad = c->n_pad;
+        sa.X = c->X;
+        sa.y = c->y;
+        sa.bw_pt = bwpts[e];
+        sa.bw = ev.bw;
+        sa.kernel = r->kernel;
+        sa.res2 = c->res2;
+        sa.sii = c->sii;
+        sa.ssi = c->ssi;
+        sa.betas = c->betas;
+        sa.se_raw = c->se;
+        sa.G = c->G;
+        sa.ldg = ntb * 16;
+        sa.gvec = c->gvec;
+        sa.fail = S->round_fail + e;
+        SolveGroup& sg = sgroup_of(solves, k, smode);
+        if (sg.start.empty()) sg.start.push_back(0);
+        sg.jobs.push_back(sa);
+        sg.start.push_back(sg.start.back() + c->n);
+        ReduceJob rj;
+        rj.n = c->n;
+        rj.out = S->round_red + 3 * e;
+        if (ev.type == kEvalCrit) {
+            rj.a = c->res2;
+            rj.b = ev.crit == GWM_CRIT_CV ? nullptr : c->sii;
+            rj.c = nullptr;
+        } else {
+            rj.a = c->sii;
+            rj.b = c->ssi;
+            rj.c = c->res2;
+            SweepGroup& q = group_of(qsweeps, r->dist, kModeQDiag, prec, ntb, 0);
+            q.jobs.push_back(sweep_job(r, kModeQDiag, ntb, 0, bwpts[e], ev.bw));
+            add_items(q, (int)q.jobs.size() - 1, r->n, r->nseg, contract_r(ntb, 0, prec == GWM_MIXED));
+            QdGroup& qg = kgroup_of(qds, k);
+            if (qg.start.empty()) qg.start.push_back(0);
+            qg.jobs.push_back(QDiagJob{r->part, r->nseg, ntb * 16, c->n, c->n_pad, c->X, c->sii, c->qd});
+            qg.start.push_back(qg.start.back() + c->n);
+        }
+        reds.push_back(rj);
+    }
+    // pack every table, upload once
+    size_t off_sel[2] = {0, 0}, off_seli[2] = {0, 0};
+    for (int d = 0; d < 2; ++d)
+        if (!sel[d].empty()) {
+            off_sel[d] = T.push(sel[d].data(), sel[d].size() * sizeof(SelectJob));
+            off_seli[d] = T.push(sel_items[d].data(), sel_items[d].size() * sizeof(int));
+        }
+    for (auto* gv : {&sweeps, &qsweeps})
+        for (auto& g : *gv) {
+            g.off_jobs = T.push(g.jobs.data(), g.jobs.size() * sizeof(SweepJob));
+            g.off_items = T.push(g.items.data(), g.items.size() * sizeof(int));
+        }
+    for (auto& g : solves) {
+        g.off_jobs = T.push(g.jobs.data(), g.jobs.size() * sizeof(SolveArgs));
+        g.off_start = T.push(g.start.data(), g.start.size() * sizeof(int));
+    }
+    for (auto& g : qds) {
+        g.off_jobs = T.push(g.jobs.data(), g.jobs.size() * sizeof(QDiagJob));
+        g.off_start = T.push(g.start.data(), g.start.size() * sizeof(int));
+    }
+    size_t off_red = T.push(reds.data(), reds.size() * sizeof(ReduceJob));
+    if (T.host.size() > S->arena.cap) {
+        if (S->arena.base) hipFree(S->arena.base);
+        S->arena.base = nullptr;
+        S->arena.cap = 0;
+        size_t cap = std::max<size_t>(T.host.size() * 2, 1 << 20);
+        SRV_CHECK(S, hipMalloc((void**)&S->arena.base, cap));
+        S->arena.cap = cap;
+    }
+    const char* dev = S->arena.base;
+    SRV_CHECK(S, hipMemcpyAsync(S->arena.base, T.host.data(), T.host.size(), hipMemcpyHostToDevice, S->stream));
+    SRV_CHECK(S, hipMemsetAsync(S->round_fail, 0, (size_t)ne * sizeof(int), S->stream));
+    // 1. adaptive bandwidths
+    for (int d = 0; d < 2; ++d)
+        if (!sel[d].empty())
+            SRV_CHECK(S, launch_adaptive_bw_batch(d, reinterpret_cast<const SelectJob*>(dev + off_sel[d]),
+                                                  reinterpret_cast<const int*>(dev + off_seli[d]),
+                                                  (int)sel_items[d].size() / 2, S->stream));
+    // 2. criterion and hat sweeps
+    int rc = launch_sweeps(S, sweeps, dev);
+    if (rc) return rc;
+    // 3. local solves
+    for (auto& g : solves)
+        SRV_CHECK(S, launch_solve_batch(g.k, g.mode, reinterpret_cast<const SolveArgs*>(dev + g.off_jobs),
+                                        reinterpret_cast<const int*>(dev + g.off_start), (int)g.jobs.size(),
+                                        g.start.back(), S->stream));
+    // 4. qDiag second sweeps of the fits
+    if (!qsweeps.empty()) {
+        for (int e = 0; e < ne; ++e) {
+            Request* r = batch[e];
+            if (r->ev.type == kEvalFit && r->ev.precision == GWM_MIXED)
+                SRV_CHECK(S, launch_split_fp16(r->ctx->G, r->ctx->rows_pad, ntb_of(r->k) * 16, r->ctx->colmax,
+                                               r->ctx->Gh, r->ctx->Gl, r->ctx->gscale, S->stream));
+        }
+        if ((rc = launch_sweeps(S, qsweeps, dev))) return rc;
+        for (auto& g : qds)
+            SRV_CHECK(S, launch_qdiag_batch(g.k, reinterpret_cast<const QDiagJob*>(dev + g.off_jobs),
+                                            reinterpret_cast<const int*>(dev + g.off_start), (int)g.jobs.size(),
+                                            g.start.back(), S->stream));
+    }
+    // 5. reductions
+    SRV_CHECK(S, launch_reduce_batch(reinterpret_cast<const ReduceJob*>(dev + off_red), ne, S->stream));
+    std::vector<double> red((size_t)ne * 3);
+    std::vector<int> fail(ne);
+    SRV_CHECK(S, hipMemcpyAsync(red.data(), S->round_red, red.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                S->stream));
+    SRV_CHECK(S, hipMemcpyAsync(fail.data(), S->round_fail, fail.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                S->stream));
+    for (int e = 0; e < ne; ++e) {
+        Request* r = batch[e];
+        if (r->ev.type != kEvalFit) continue;
+        const size_t nk = (size_t)r->n * r->k;
+        r->hbetas.resize(nk);
+        r->hse.resize(nk);
+        r->hq.resize(r->n);
+        SRV_CHECK(S, hipMemcpyAsync(r->hbetas.data(), r->ctx->betas, nk * 8, hipMemcpyDeviceToHost, S->stream));
+        SRV_CHECK(S, hipMemcpyAsync(r->hse.data(), r->ctx->se, nk * 8, hipMemcpyDeviceToHost, S->stream));
+        SRV_CHECK(S, hipMemcpyAsync(r->hq.data(), r->ctx->qd, (size_t)r->n * 8, hipMemcpyDeviceToHost, S->stream));
+    }
+    SRV_CHECK(S, hipStreamSynchronize(S->stream));
+    long long nsolves = 0;
+    for (int e = 0; e < ne; ++e) {
+        Request* r = batch[e];
+        Eval& ev = r->ev;
+        nsolves += r->n;
+        r->solves += r->n;
+        if (fail[e]) {
+            ev.rc = GWM_ERR_NOT_SPD;
+            ev.value = DBL_MAX;
+            continue;
+        }
+        if (ev.type == kEvalCrit) {
+            double v = ev.crit == GWM_CRIT_CV ? red[3 * e] : aicc(red[3 * e], (double)r->n, red[3 * e + 1]);
+            ev.rc = std::isfinite(v) ? GWM_OK : GWM_ERR_NOT_SPD;
+            ev.value = std::isfinite(v) ? v : DBL_MAX;
+        } else {
+            ev.rc = GWM_OK;
+            ev.shat[0] = red[3 * e];
+            ev.shat[1] = red[3 * e + 1];
+        }
+    }
+    S->rounds += 1;
+    S->solves += nsolves;
+    S->busy_ms += now_ms() - t0;
+    return GWM_OK;
+}
+
+// Worker: the host algorithm of one request.
+void worker_main(gwm_hip_server* S, Request* R) {
+    auto post = [&](int type, double bw, int prec, int crit, double* value) -> int {
+        std::unique_lock<std::mutex> lk(S->mu);
+        R->ev = Eval();
+        R->ev.type = type;
+        R->ev.bw = bw;
+        R->ev.precision = prec;
+        R->ev.crit = crit;
+        R->posted = true;
+        S->cv.notify_all();
+        S->cv.wait(lk, [&] { return !R->posted; });
+        if (value) *value = R->ev.value;
+        return R->ev.rc;
+    };
+    int rc = GWM_OK;
+    double bw = R->bw;
+    if (R->select_bw) {
+        double lo = R->has_lower ? R->lower : (R->adaptive ? 20 : 0.0);
+        double hi = R->has_upper ? R->upper : (R->adaptive ? (double)R->n : R->maxdist);
+        std::map<double, double> log;
+        rc = golden(R->adaptive != 0, lo, hi,
+                    [&](double x, bool exact, double* fx) {
+                        return post(kEvalCrit, x, exact ? GWM_FP64 : S->precision, R->crit, fx);
+                    },
+                    S->precision == GWM_MIXED, S->certify_rel, &bw, log);
+    }
+    if (rc == GWM_OK) rc = post(kEvalFit, bw, S->precision, 0, nullptr);
+    if (rc == GWM_OK) {
+        double shat[2] = {R->ev.shat[0], R->ev.shat[1]};
+        double diag[7];
+        calc_diagnostic(R->n, R->k, R->X.data(), R->y.data(), R->hbetas.data(), shat, diag);
+        const double sigma = diag[0] / (R->n - 2 * shat[0] + shat[1]);
+        const size_t nk = (size_t)R->n * R->k;
+        if (R->o_betas) std::memcpy(R->o_betas, R->hbetas.data(), nk * 8);
+        if (R->o_se)
+            for (size_t i = 0; i < nk; ++i) R->o_se[i] = std::sqrt(sigma * R->hse[i]);
+        if (R->o_shat) std::memcpy(R->o_shat, shat, sizeof(shat));
+        if (R->o_qdiag) std::memcpy(R->o_qdiag, R->hq.data(), (size_t)R->n * 8);
+        if (R->o_diag) std::memcpy(R->o_diag, diag, sizeof(diag));
+        if (R->o_bw) *R->o_bw = bw;
+    }
+    std::unique_lock<std::mutex> lk(S->mu);
+    R->status = rc;
+    R->finished = true;
+    S->cv.notify_all();
+}
+
+void release(gwm_hip_server* S, Request* R) {
+    if (R->part) hipFree(R->part);
+    R->part = nullptr;
+    if (R->ctx) gwm_hip_destroy(R->ctx);
+    R->ctx = nullptr;
+    S->inflight_bytes -= R->footprint;
+}
+
+// Admission: allocate the request's device state.  Called by the scheduler.
+int admit(gwm_hip_server* S, Request* R) {
+    int rc = gwm_hip_create(S->device, &R->ctx);
+    if (rc) return rc;
+    rc = gwm_hip_set_data(R->ctx, R->n, R->k, R->coords.data(), R->X.data(), R->y.data(), R->dist);
+    if (rc) return rc;
+    R->ctx->certify_rel = S->certify_rel;
+    const size_t bytes = (size_t)R->nseg * (nta_of(R->k) + ntb_of(R->k)) * 16 * R->ctx->n_pad * sizeof(double);
+    if (hipMalloc((void**)&R->part, bytes) != hipSuccess) return GWM_ERR_OOM;
+    if (R->select_bw && !R->adaptive && !R->has_upper) {
+        if ((rc = max_distance_impl(R->ctx, &R->maxdist))) return rc;
+    }
+    return GWM_OK;
+}
+
+void scheduler_main(gwm_hip_server* S) {
+    hipSetDevice(S->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        // retire finished requests
+        for (size_t i = 0; i < S->active.size();) {
+            Request* R = S->active[i];
+            if (!R->finished) {
+                ++i;
+                continue;
+            }
+            lk.unlock();
+            if (R->worker.joinable()) R->worker.join();
+            lk.lock();
+            release(S, R);
+            R->t_done = now_ms();
+            R->done = true;
+            S->active.erase(S->active.begin() + i);
+            S->cv.notify_all();
+        }
+        // admit waiting requests while slots and memory allow (first come, first served)
+        while (!S->pending.empty() && (int)S->active.size() < S->max_inflight) {
+            Request* R = S->pending.front();
+            if (!S->active.empty() && S->inflight_bytes + R->footprint > S->budget) break;
+            S->pending.pop_front();
+            lk.unlock();
+            int rc = admit(S, R);
+            lk.lock();
+            R->t_admit = now_ms();
+            S->inflight_bytes += R->footprint;
+            if (rc) {
+                release(S, R);
+                R->status = rc;
+                R->t_done = now_ms();
+                R->done = true;
+                S->cv.notify_all();
+                continue;
+            }
+            R->admitted = true;
+            S->active.push_back(R);
+            S->peak_bytes = std::max(S->peak_bytes, S->inflight_bytes);
+            S->peak_inflight = std::max<int>(S->peak_inflight, (int)S->active.size());
+            R->worker = std::thread(worker_main, S, R);
+        }
+        if (S->active.empty()) {
+            if (S->stopping && S->pending.empty()) break;
+            S->cv.wait(lk, [&] { return S->stopping || !S->pending.empty(); });
+            continue;
+        }
+        // wait until every in-flight request has posted an evaluation or finished
+        S->cv.wait(lk, [&] {
+            for (Request* R : S->active)
+                if (!R->posted && !R->finished) return false;
+            return true;
+        });
+        std::vector<Request*> batch;
+        for (Request* R : S->active)
+            if (R->posted) batch.push_back(R);
+        if (batch.empty()) continue;
+        lk.unlock();
+        int rc = run_round(S, batch);
+        lk.lock();
+        for (Request* R : batch) {
+            if (rc) {
+                R->ev.rc = rc;
+                R->ev.value = DBL_MAX;
+            }
+            R->posted = false;
+        }
+        S->cv.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gwm_hip_server_create(int device, int precision, double budget_bytes, int max_inflight, gwm_hip_server** out) {
+    if (!out || (precision != GWM_FP64 && precision != GWM_MIXED) || max_inflight < 1) return GWM_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return GWM_ERR_NO_DEVICE;
+    gwm_hip_server* S = new gwm_hip_server();
+    S->device = device;
+    S->precision = precision;
+    S->max_inflight = max_inflight;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete S;
+        return GWM_ERR_HIP;
+    }
+    size_t free_b = 0, total_b = 0;
+    hipMemGetInfo(&free_b, &total_b);
+    S->budget = budget_bytes > 0 ? (size_t)budget_bytes : (size_t)(0.8 * (double)free_b);
+    S->sched = std::thread(scheduler_main, S);
+    *out = S;
+    return GWM_OK;
+}
+
+int gwm_hip_server_set_certify_tolerance(gwm_hip_server* S, double rel) {
+    if (!S || !(rel >= 0.0)) return GWM_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->certify_rel = rel;
+    return GWM_OK;
+}
+
+int gwm_hip_server_submit(gwm_hip_server* S, int n, int k, const double* coords, const double* X, const double* y,
+                          int distance, int kernel, int adaptive, double bw, int select_bw, int criterion,
+                          int has_lower, double lower, int has_upper, double upper, double* betas, double* betasSE,
+                          double* shat, double* qDiag, double* diag, double* bw_out, long long* request_id) {
+    if (!S || n < 2 || k < 1 || k > 16 || !coords || !X || !y || kernel < 0 || kernel > 4 ||
+        (distance != GWM_EUCLIDEAN && distance != GWM_GREAT_CIRCLE))
+        return GWM_ERR_INVALID;
+    if (!select_bw && adaptive && !(bw >= 1.0)) return GWM_ERR_INVALID;
+    Request* R = new Request();
+    R->n = n;
+    R->k = k;
+    R->dist = distance;
+    R->kernel = kernel;
+    R->adaptive = adaptive;
+    R->bw = bw;
+    R->select_bw = select_bw;
+    R->crit = criterion;
+    R->has_lower = has_lower;
+    R->lower = lower;
+    R->has_upper = has_upper;
+    R->upper = upper;
+    R->coords.assign(coords, coords + 2 * (size_t)n);
+    R->X.assign(X, X + (size_t)n * k);
+    R->y.assign(y, y + n);
+    R->o_betas = betas;
+    R->o_se = betasSE;
+    R->o_shat = shat;
+    R->o_qdiag = qDiag;
+    R->o_diag = diag;
+    R->o_bw = bw_out;
+    batch_segments(n, R->nseg, R->seg_len);
+    R->footprint = footprint_bytes(n, k, distance, R->nseg);
+    std::lock_guard<std::mutex> lk(S->mu);
+    R->id = S->next_id++;
+    R->t_submit = now_ms();
+    S->reqs[R->id] = R;
+    S->pending.push_back(R);
+    S->cv.notify_all();
+    if (request_id) *request_id = R->id;
+    return GWM_OK;
+}
+
+int gwm_hip_server_wait(gwm_hip_server* S, long long id, int* status, double* latency_ms, long long* solves) {
+    if (!S) return GWM_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return GWM_ERR_INVALID;
+    Request* R = it->second;
+    S->cv.wait(lk, [&] { return R->done; });
+    if (status) *status = R->status;
+    if (latency_ms) *latency_ms = R->t_done - R->t_submit;
+    if (solves) *solves = R->solves;
+    S->reqs.erase(it);
+    delete R;
+    return GWM_OK;
+}
+
+int gwm_hip_server_stats(gwm_hip_server* S, long long* rounds, long long* solves, double* busy_ms,
+                         double* peak_bytes, int* peak_inflight) {
+    if (!S) return GWM_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (rounds) *rounds = S->rounds;
+    if (solves) *solves = S->solves;
+    if (busy_ms) *busy_ms = S->busy_ms;
+    if (peak_bytes) *peak_bytes = (double)S->peak_bytes;
+    if (peak_inflight) *peak_inflight = S->peak_inflight;
+    return GWM_OK;
+}
+
+const char* gwm_hip_server_last_error(gwm_hip_server* S) { return S ? S->err.c_str() : "null server"; }
+
+void gwm_hip_server_destroy(gwm_hip_server* S) {
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stopping = true;
+        S->cv.notify_all();
+    }
+    if (S->sched.joinable()) S->sched.join();
+    for (auto& kv : S->reqs) delete kv.second;
+    if (S->arena.base) hipFree(S->arena.base);
+    if (S->round_red) hipFree(S->round_red);
+    if (S->round_fail) hipFree(S->round_fail);
+    if (S->stream) hipStreamDestroy(S->stream);
+    delete S;
+}
+
+}  // extern "C"
