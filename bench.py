@@ -15,6 +15,14 @@ golden-section decisions); the same step is run on the FP64 path for the
 mixed/FP64 ratio and the beta error.  Inputs are resident on the device
 before the timed region starts.
 
+Two more workloads ride along in the same JSON line (keys "batch" and
+"serve"): 512 uneven problems (n log-uniform in [200, 20000], k in 3..12,
+Gaussian / bisquare / tricube, adaptive or fixed), each a CV golden-section
+search plus fit, run through the batching server against a loop of
+single-problem calls; and a Poisson stream of fit and CV-search requests
+(n log-uniform in [500, 200000], k in 2..16, clustered coordinates) replayed in
+real time against the continuous-batching server, with p50/p99 latency.
+
 Prints one JSON line (see the keys below).
 """
 import argparse
@@ -60,6 +68,93 @@ def cpu_baseline(d, bw, sample, threads):
     return sample / dt, dt
 
 
+def run_batch(capi, synth, num, loop):
+    """Config-4 batch: every problem is a CV search + fit, mixed precision."""
+    probs = synth.config4(num)
+    sum_n = sum(p["X"].shape[0] for p in probs)
+    log("batch: %d problems, sum n %d" % (num, sum_n))
+    with capi.Server(0, capi.MIXED, max_inflight=num) as srv:
+        t0 = time.perf_counter()
+        ids = [srv.submit(p["coords"], p["X"], p["y"], p["kernel"], p["adaptive"], 0.0, select_bw=True,
+                          crit=capi.CRIT_CV) for p in probs]
+        res = [srv.wait(i) for i in ids]
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+    bad = sum(r["status"] != capi.GWM_OK for r in res)
+    solves = sum(r["solves"] for r in res)
+    out = {"problems": num, "sum_n": sum_n, "solves": solves, "seconds": dt, "solves_per_s": solves / dt,
+           "rounds": st["rounds"], "failed": bad}
+    if loop:
+        log("batch: loop of single-problem calls over the same %d problems" % num)
+        ctx = capi.Context(0)
+        lsolves = 0
+        t0 = time.perf_counter()
+        lres = []
+        for p in probs:
+            ctx.set_data(p["coords"], p["X"], p["y"])
+            ctx.reset_timers()
+            g = ctx.gwr_run(p["kernel"], p["adaptive"], 0.0, select_bw=True, crit=capi.CRIT_CV,
+                            precision=capi.MIXED)
+            lsolves += ctx.solve_count()
+            lres.append(g["bandwidth"])
+        ldt = time.perf_counter() - t0
+        ctx.close()
+        out["loop_seconds"] = ldt
+        out["loop_solves_per_s"] = lsolves / ldt
+        out["batch_over_loop"] = (solves / dt) / (lsolves / ldt)
+        out["bandwidths_equal_loop"] = int(sum(a == r["bandwidth"] for a, r in zip(lres, res)))
+    return out
+
+
+def serve_submit(capi, srv, r):
+    n = r["X"].shape[0]
+    if r["search"]:
+        return srv.submit(r["coords"], r["X"], r["y"], capi.BISQUARE, True, 0.0, select_bw=True, crit=capi.CRIT_CV)
+    return srv.submit(r["coords"], r["X"], r["y"], capi.BISQUARE, True, float(min(n - 1, 100)))
+
+
+def run_serve(capi, synth, num, rate, check):
+    """Config-5 stream replayed in real time; fits use an adaptive bisquare
+    kernel of 100 neighbours, searches CV over [20, n] neighbours."""
+    reqs = synth.config5(num, rate=rate)
+    log("serve: %d requests at %.2f/s (%d searches)" % (num, rate, sum(r["search"] for r in reqs)))
+    with capi.Server(0, capi.MIXED, max_inflight=64) as srv:
+        start = time.perf_counter()
+        ids = []
+        for r in reqs:
+            delay = r["t"] - (time.perf_counter() - start)
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(serve_submit(capi, srv, r))
+        res = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - start
+        st = srv.stats()
+    lat = np.array([r["latency_ms"] for r in res])
+    fits = np.array([not r["search"] for r in reqs])
+    solves = sum(r["solves"] for r in res)
+    out = {"requests": num, "rate_per_s": rate, "searches": int((~fits).sum()), "seconds": wall,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "max_ms": float(lat.max()), "fit_p50_ms": float(np.percentile(lat[fits], 50)),
+           "fit_p99_ms": float(np.percentile(lat[fits], 99)),
+           "search_p50_ms": float(np.percentile(lat[~fits], 50)) if (~fits).any() else None,
+           "solves": solves, "solves_per_s": solves / wall, "rounds": st["rounds"],
+           "peak_inflight": st["peak_inflight"], "failed": int(sum(r["status"] != capi.GWM_OK for r in res))}
+    if check:
+        # beta error vs the FP64 server on the smallest requests of each type
+        idx = sorted(range(num), key=lambda i: reqs[i]["X"].shape[0])
+        pick = [i for i in idx if reqs[i]["search"]][:check] + [i for i in idx if not reqs[i]["search"]][:check]
+        errs, same_bw = [], 0
+        with capi.Server(0, capi.FP64, max_inflight=16) as srv:
+            ref = [srv.wait(serve_submit(capi, srv, reqs[i])) for i in pick]
+        for i, g in zip(pick, ref):
+            errs.append(beta_rel_err(res[i]["betas"], g["betas"]))
+            same_bw += res[i]["bandwidth"] == g["bandwidth"]
+        out["beta_rel_err_vs_fp64"] = max(errs)
+        out["checked"] = len(pick)
+        out["bandwidths_identical"] = same_bw
+    return out
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % name)
     if os.path.exists(path):
@@ -77,6 +172,12 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=512, help="focus points of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-problems", type=int, default=512, help="config-4 batch size (0: skip)")
+    ap.add_argument("--no-loop", action="store_true", help="skip the single-problem loop of the batch")
+    ap.add_argument("--serve-requests", type=int, default=64, help="config-5 stream length (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="config-5 arrival rate, requests/s")
+    ap.add_argument("--serve-check", type=int, default=3,
+                    help="requests of each type re-run in FP64 for the beta error")
     ap.add_argument("--sweeps-only", type=int, default=0,
                     help="profiling mode: run this many mixed AICc sweeps (and as many FP64 ones) and exit")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -165,6 +266,10 @@ def main():
                "sample": "%d focus points x n=%d data, one AICc evaluation at the selected bandwidth, "
                          "great-circle Gaussian, OpenMP (%.1f s)" % (args.cpu_sample, n, dt)}
 
+    batch = run_batch(capi, synth, args.batch_problems, not args.no_loop) if args.batch_problems else None
+    serve = run_serve(capi, synth, args.serve_requests, args.serve_rate, args.serve_check) \
+        if args.serve_requests else None
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -200,6 +305,10 @@ def main():
                      "fp64_kernel_avg_launch_ms": avg_ms64,
                      "fp64_frac_of_fp64_peak": flops_per_launch / (avg_ms64 * 1e-3) / PEAK["fp64"] if l64 else None},
         "cpu_baseline": cpu,
+        "serve_p99_ms": serve["p99_ms"] if serve else None,
+        "serve_p50_ms": serve["p50_ms"] if serve else None,
+        "serve": serve,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 

@@ -154,7 +154,10 @@ int gwm_hip_solve_count(gwm_hip_ctx* ctx, long long* solves);
  * the fit (select_bw).  Requests are admitted first come, first served while
  * max_inflight slots and the device memory budget allow (a request's footprint
  * is O(n k^2), never n^2); the criterion and fit evaluations of all in-flight
- * requests are executed together in rounds of batched launches.  A request's
+ * requests are executed together in rounds of batched launches.  A round has a
+ * work budget of about round_ms of device time (default 25), shared shortest-
+ * remaining-work first, so a large evaluation advances in chunks of focus
+ * points over several rounds while small ones complete in one.  A request's
  * result does not depend on which other requests share its rounds.  Output
  * pointers (nullable; betas and betasSE n x k, shat 2, qDiag n, diag 7, bw_out
  * 1) must stay valid until gwm_hip_server_wait returns for the request.
@@ -164,6 +167,9 @@ typedef struct gwm_hip_server gwm_hip_server;
 int gwm_hip_server_create(int device, int precision, double hbm_budget_bytes, int max_inflight,
                           gwm_hip_server** out);
 int gwm_hip_server_set_certify_tolerance(gwm_hip_server* server, double rel_tol);
+/* Round budget in ms of device time; <= 0 runs every posted evaluation whole
+ * in each round. */
+int gwm_hip_server_set_round_ms(gwm_hip_server* server, double round_ms);
 int gwm_hip_server_submit(gwm_hip_server* server, int n, int k, const double* coords, const double* X,
                           const double* y, int distance, int kernel, int adaptive, double bw, int select_bw,
                           int criterion, int has_lower, double lower, int has_upper, double upper, double* betas,

@@ -58,6 +58,7 @@ SIGNATURES = {
     "gwm_hip_solve_count": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "gwm_hip_server_create": (_i, [_i, _i, _d, _i, ctypes.POINTER(_vp)]),
     "gwm_hip_server_set_certify_tolerance": (_i, [_vp, _d]),
+    "gwm_hip_server_set_round_ms": (_i, [_vp, _d]),
     "gwm_hip_server_submit": (_i, [_vp, _i, _i, _dp, _dp, _dp, _i, _i, _i, _d, _i, _i, _i, _d, _i, _d, _dp, _dp, _dp,
                                    _dp, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "gwm_hip_server_wait": (_i, [_vp, ctypes.c_longlong, _ip, _dp, ctypes.POINTER(ctypes.c_longlong)]),
@@ -311,6 +312,12 @@ class Server:
 
     def set_certify_tolerance(self, rel):
         rc = self._lib.gwm_hip_server_set_certify_tolerance(self.h, float(rel))
+        if rc != GWM_OK:
+            raise GwmError(rc)
+
+    def set_round_ms(self, ms):
+        """Work budget of a round in ms of device time (<= 0: whole evaluations)."""
+        rc = self._lib.gwm_hip_server_set_round_ms(self.h, float(ms))
         if rc != GWM_OK:
             raise GwmError(rc)
 

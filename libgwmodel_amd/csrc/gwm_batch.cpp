@@ -8,17 +8,22 @@
 // diagnostics of GWRBasic::fit -- on a worker thread of its own.  Every
 // criterion or fit evaluation the worker needs is posted to the scheduler and
 // the worker blocks.  The scheduler runs in rounds: once every in-flight
-// request has posted (or finished), it executes all posted evaluations as
-// batched launches -- one adaptive-bandwidth launch, one sweep launch per
-// kernel class, one solve launch per k, one reduction launch -- on its stream,
-// then hands the results back.  Between rounds it retires finished requests
-// and admits waiting ones while their device footprint (O(n k^2): coordinates,
-// design, product table and its FP16 planes, partial sums; never n^2) fits
-// the memory budget.
+// request has posted (or finished), it executes a slice of every posted
+// evaluation as batched launches -- one adaptive-bandwidth launch, one sweep
+// launch per kernel class, one solve launch per k, one reduction launch -- on
+// its stream, and hands back the evaluations that completed.  A round has a
+// work budget (about round_ms of device time): the posted evaluations share it
+// shortest-remaining-first, each getting at least one chunk of focus points,
+// so an evaluation of a small problem completes in one short round while a
+// large fit next to it advances chunk by chunk.  Between rounds the scheduler
+// retires finished requests and admits waiting ones while their device
+// footprint (O(n k^2): coordinates, design, product table and its FP16 planes,
+// partial sums; never n^2) fits the memory budget.
 //
 // Each request's arithmetic depends only on its own data: its data segments
-// are a function of its n alone and every reduction has a fixed order, so a
-// request returns the same bits alone or in any mix.
+// are a function of its n alone, a focus point's partial sums and solve do not
+// depend on which round computes them, and every reduction has a fixed order,
+// so a request returns the same bits alone or in any mix.
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -44,6 +49,15 @@ double now_ms() {
 
 enum { kEvalCrit = 0, kEvalFit = 1 };
 
+// Slicing of evaluations across rounds.  Work is counted in units of one data
+// point x one 16-column tile of one focus point in the mixed sweep; an FP64
+// sweep tile costs about 3.5 units and an adaptive-bandwidth selection about 4
+// units per data point.  The device does about 1.5e9 units per ms.
+constexpr int kChunk = 128;              // focus points; a multiple of every sweep's 64 R
+constexpr double kUnitsPerMs = 1.5e9;
+constexpr double kFp64Factor = 3.5;
+constexpr double kSelectUnits = 4.0;
+
 struct Eval {
     int type = kEvalCrit;
     double bw = 0.0;
@@ -65,6 +79,7 @@ struct Request {
     // device state (from admission to retirement)
     gwm_hip_ctx* ctx = nullptr;
     double* part = nullptr;
+    int* dfail = nullptr;        // non-SPD flag of the evaluation in progress
     int nseg = 1, seg_len = 64;
     size_t footprint = 0;
     double maxdist = 0.0;
@@ -77,6 +92,9 @@ struct Request {
     std::thread worker;
     bool posted = false;
     Eval ev;
+    // progress of the posted evaluation: phase 1 = selection, sweep and solves
+    // of focus points [0, next); phase 2 (fits) = qDiag sweep of points [0, next)
+    int phase = 1, next = 0;
     // host staging of fit outputs
     std::vector<double> hbetas, hse, hq;
 };
@@ -131,8 +149,8 @@ struct gwm_hip_server {
     std::string err;
     Arena arena;
     double* round_red = nullptr;
-    int* round_fail = nullptr;
     int round_cap = 0;
+    double round_ms = 25.0;      // work budget of a round; <= 0: whole evaluations
 };
 
 namespace {
@@ -206,9 +224,10 @@ SweepJob sweep_job(Request* r, int mode, int nta, int ntb, const double* bw_pt, 
     return j;
 }
 
-void add_items(SweepGroup& g, int job, int n, int nseg, int R) {
-    const int tiles = (n + 64 * R - 1) / (64 * R);
-    for (int t = 0; t < tiles; ++t)
+// Sweep items of focus points [f0, f1) (f0 a multiple of 64 R).
+void add_items(SweepGroup& g, int job, int f0, int f1, int nseg, int R) {
+    const int t0 = f0 / (64 * R), t1 = (f1 + 64 * R - 1) / (64 * R);
+    for (int t = t0; t < t1; ++t)
         for (int s = 0; s < nseg; ++s) {
             g.items.push_back(job);
             g.items.push_back(t);
@@ -252,111 +271,208 @@ int launch_sweeps(gwm_hip_server* S, std::vector<SweepGroup>& gs, const char* de
     return GWM_OK;
 }
 
-// Execute the posted evaluations of one round.
+// Work units per focus point of phase 1 (selection, sweep, solve) and per
+// point of phase 2 (qDiag sweep) of r's posted evaluation.
+double unit_cost(const Request* r, int phase) {
+    const Eval& ev = r->ev;
+    const double pf = ev.precision == GWM_FP64 ? kFp64Factor : 1.0;
+    if (phase == 1) {
+        const double tiles = ev.type == kEvalCrit ? nta_of(r->k) : nta_of(r->k) + ntb_of(r->k);
+        return r->n * (tiles * pf + (r->adaptive ? kSelectUnits : 0.0));
+    }
+    return (double)r->n * ntb_of(r->k) * pf;
+}
+
+double remaining_cost(const Request* r) {
+    if (r->phase == 1)
+        return (r->n - r->next) * unit_cost(r, 1) + (r->ev.type == kEvalFit ? r->n * unit_cost(r, 2) : 0.0);
+    return (r->n - r->next) * unit_cost(r, 2);
+}
+
+// What one round does for one evaluation.
+struct Slice {
+    Request* r = nullptr;
+    int a0 = 0, a1 = 0;   // phase-1 focus range
+    int q0 = 0, q1 = 0;   // phase-2 point range
+    bool split = false;   // phase 2 starts: the mixed path splits G into FP16 planes first
+    bool done = false;    // the evaluation completes in this round
+    double cost = 0.0;
+};
+
+// Advances r by about `units` of work, at least one chunk (or the rest).
+Slice plan_slice(Request* r, double units, bool unlimited) {
+    Slice sl;
+    sl.r = r;
+    auto count = [&](int phase) {
+        const int left = r->n - r->next;
+        if (unlimited) return left;
+        const long long want = (long long)std::ceil(std::max(units, 0.0) / unit_cost(r, phase));
+        const long long chunks = std::max<long long>(1, (want + kChunk - 1) / kChunk);
+        return (int)std::min<long long>(left, chunks * kChunk);
+    };
+    if (r->phase == 1) {
+        const int c = count(1);
+        sl.a0 = r->next;
+        sl.a1 = r->next + c;
+        sl.cost += c * unit_cost(r, 1);
+        units -= c * unit_cost(r, 1);
+        r->next = sl.a1;
+        if (r->next < r->n) return sl;
+        if (r->ev.type != kEvalFit) {
+            r->phase = 3;
+            sl.done = true;
+            return sl;
+        }
+        r->phase = 2;
+        r->next = 0;
+        if (!unlimited && units <= 0.0) return sl;
+    }
+    sl.split = r->next == 0;
+    const int c = count(2);
+    sl.q0 = r->next;
+    sl.q1 = r->next + c;
+    sl.cost += c * unit_cost(r, 2);
+    r->next = sl.q1;
+    if (r->next == r->n) {
+        r->phase = 3;
+        sl.done = true;
+    }
+    return sl;
+}
+
+SolveArgs solve_job(Request* r, int PT, int colB, const double* bw_pt, int f0) {
+    gwm_hip_ctx* c = r->ctx;
+    SolveArgs sa;
+    sa.part = r->part;
+    sa.nseg = r->nseg;
+    sa.PT = PT;
+    sa.colB = colB;
+    sa.n = c->n;
+    sa.n_pad = c->n_pad;
+    sa.X = c->X;
+    sa.y = c->y;
+    sa.bw_pt = bw_pt;
+    sa.bw = r->ev.bw;
+    sa.kernel = r->kernel;
+    sa.res2 = c->res2;
+    sa.sii = c->sii;
+    sa.ssi = c->ssi;
+    sa.betas = c->betas;
+    sa.se_raw = c->se;
+    sa.G = c->G;
+    sa.ldg = ntb_of(r->k) * 16;
+    sa.gvec = c->gvec;
+    sa.fail = r->dfail;
+    sa.f0 = f0;
+    return sa;
+}
+
+// Executes one round: a slice of every posted evaluation.  Evaluations that
+// complete get their results and phase 3.
 int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
     const double t0 = now_ms();
     const int ne = (int)batch.size();
     if (ne > S->round_cap) {
         if (S->round_red) hipFree(S->round_red);
-        if (S->round_fail) hipFree(S->round_fail);
         S->round_red = nullptr;
-        S->round_fail = nullptr;
         S->round_cap = 0;
-        SRV_CHECK(S, hipMalloc((void**)&S->round_red, (size_t)ne * 2 * 3 * sizeof(double)));
-        SRV_CHECK(S, hipMalloc((void**)&S->round_fail, (size_t)ne * 2 * sizeof(int)));
+        SRV_CHECK(S, hipMalloc((void**)&S->round_red, (size_t)ne * 2 * 4 * sizeof(double)));
         S->round_cap = ne * 2;
     }
+    // share the round's budget, shortest remaining work first
+    std::vector<int> order(ne);
+    std::vector<double> rem(ne);
+    for (int e = 0; e < ne; ++e) {
+        order[e] = e;
+        rem[e] = remaining_cost(batch[e]);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rem[a] < rem[b]; });
+    const bool unlimited = !(S->round_ms > 0.0);
+    double left = unlimited ? 0.0 : S->round_ms * kUnitsPerMs;
+    std::vector<Slice> slices;
+    for (int i = 0; i < ne; ++i) {
+        Slice sl = plan_slice(batch[order[i]], left / (ne - i), unlimited);
+        left -= sl.cost;
+        slices.push_back(sl);
+    }
+
     Tables T;
     std::vector<SelectJob> sel[2];
-    std::vector<int> sel_items[2];
+    std::vector<int> sel_start[2] = {{0}, {0}};
     std::vector<SweepGroup> sweeps, qsweeps;
     std::vector<SolveGroup> solves;
     std::vector<QdGroup> qds;
     std::vector<ReduceJob> reds;
-    std::vector<const double*> bwpts(ne, nullptr);
-    for (int e = 0; e < ne; ++e) {
-        Request* r = batch[e];
+    std::vector<Request*> splits, finished;
+    for (const Slice& sl : slices) {
+        Request* r = sl.r;
         const Eval& ev = r->ev;
-        if (r->adaptive) {
-            const int d = r->dist;
-            SelectJob sj{r->ctx->points(), ev.bw, r->ctx->bwi};
-            const int job = (int)sel[d].size();
-            sel[d].push_back(sj);
-            for (int f = 0; f < r->n; ++f) {
-                sel_items[d].push_back(job);
-                sel_items[d].push_back(f);
-            }
-            bwpts[e] = r->ctx->bwi;
-        }
+        gwm_hip_ctx* c = r->ctx;
         const int k = r->k, nta = nta_of(k), ntb = ntb_of(k);
         const int prec = ev.precision;
-        int mode, smode, PT, colB = 0;
-        if (ev.type == kEvalCrit) {
-            mode = ev.crit == GWM_CRIT_CV ? kModeCV : kModePlain;
-            smode = ev.crit == GWM_CRIT_CV ? kSolveCV : kSolveAIC;
-            PT = nta * 16;
-        } else {
-            mode = kModeHat;
-            smode = kSolveFit;
-            PT = (nta + ntb) * 16;
-            colB = nta * 16;
+        const double* bw_pt = r->adaptive ? c->bwi : nullptr;
+        if (sl.a1 > sl.a0) {
+            if (r->adaptive) {
+                const int d = r->dist;
+                sel[d].push_back(SelectJob{c->points(), ev.bw, c->bwi, sl.a0});
+                sel_start[d].push_back(sel_start[d].back() + sl.a1 - sl.a0);
+            }
+            int mode, smode, PT, colB = 0;
+            if (ev.type == kEvalCrit) {
+                mode = ev.crit == GWM_CRIT_CV ? kModeCV : kModePlain;
+                smode = ev.crit == GWM_CRIT_CV ? kSolveCV : kSolveAIC;
+                PT = nta * 16;
+            } else {
+                mode = kModeHat;
+                smode = kSolveFit;
+                PT = (nta + ntb) * 16;
+                colB = nta * 16;
+            }
+            const int tb = mode == kModeHat ? ntb : 0;
+            SweepGroup& g = group_of(sweeps, r->dist, mode, prec, nta, tb);
+            g.jobs.push_back(sweep_job(r, mode, nta, tb, bw_pt, ev.bw));
+            add_items(g, (int)g.jobs.size() - 1, sl.a0, sl.a1, r->nseg, contract_r(nta, tb, prec == GWM_MIXED));
+            SolveGroup& sg = sgroup_of(solves, k, smode);
+            if (sg.start.empty()) sg.start.push_back(0);
+            sg.jobs.push_back(solve_job(r, PT, colB, bw_pt, sl.a0));
+            sg.start.push_back(sg.start.back() + sl.a1 - sl.a0);
         }
-        SweepGroup& g = group_of(sweeps, r->dist, mode, prec, nta, mode == kModeHat ? ntb : 0);
-        g.jobs.push_back(sweep_job(r, mode, nta, mode == kModeHat ? ntb : 0, bwpts[e], ev.bw));
-        add_items(g, (int)g.jobs.size() - 1, r->n, r->nseg, contract_r(nta, mode == kModeHat ? ntb : 0, prec == GWM_MIXED));
-        gwm_hip_ctx* c = r->ctx;
-        SolveArgs sa;
-        sa.part = r->part;
-        sa.nseg = r->nseg;
-        sa.PT = PT;
-        sa.colB = colB;
-        sa.n = c->n;
-        sa.n_pad = c->n_pad;
-        sa.X = c->X;
-        sa.y = c->y;
-        sa.bw_pt = bwpts[e];
-        sa.bw = ev.bw;
-        sa.kernel = r->kernel;
-        sa.res2 = c->res2;
-        sa.sii = c->sii;
-        sa.ssi = c->ssi;
-        sa.betas = c->betas;
-        sa.se_raw = c->se;
-        sa.G = c->G;
-        sa.ldg = ntb * 16;
-        sa.gvec = c->gvec;
-        sa.fail = S->round_fail + e;
-        SolveGroup& sg = sgroup_of(solves, k, smode);
-        if (sg.start.empty()) sg.start.push_back(0);
-        sg.jobs.push_back(sa);
-        sg.start.push_back(sg.start.back() + c->n);
-        ReduceJob rj;
-        rj.n = c->n;
-        rj.out = S->round_red + 3 * e;
-        if (ev.type == kEvalCrit) {
-            rj.a = c->res2;
-            rj.b = ev.crit == GWM_CRIT_CV ? nullptr : c->sii;
-            rj.c = nullptr;
-        } else {
-            rj.a = c->sii;
-            rj.b = c->ssi;
-            rj.c = c->res2;
+        if (sl.split && prec == GWM_MIXED) splits.push_back(r);
+        if (sl.q1 > sl.q0) {
             SweepGroup& q = group_of(qsweeps, r->dist, kModeQDiag, prec, ntb, 0);
-            q.jobs.push_back(sweep_job(r, kModeQDiag, ntb, 0, bwpts[e], ev.bw));
-            add_items(q, (int)q.jobs.size() - 1, r->n, r->nseg, contract_r(ntb, 0, prec == GWM_MIXED));
+            q.jobs.push_back(sweep_job(r, kModeQDiag, ntb, 0, bw_pt, ev.bw));
+            add_items(q, (int)q.jobs.size() - 1, sl.q0, sl.q1, r->nseg, contract_r(ntb, 0, prec == GWM_MIXED));
             QdGroup& qg = kgroup_of(qds, k);
             if (qg.start.empty()) qg.start.push_back(0);
-            qg.jobs.push_back(QDiagJob{r->part, r->nseg, ntb * 16, c->n, c->n_pad, c->X, c->sii, c->qd});
-            qg.start.push_back(qg.start.back() + c->n);
+            QDiagJob qj{r->part, r->nseg, ntb * 16, c->n, c->n_pad, c->X, c->sii, c->qd};
+            qj.f0 = sl.q0;
+            qg.jobs.push_back(qj);
+            qg.start.push_back(qg.start.back() + sl.q1 - sl.q0);
         }
-        reds.push_back(rj);
+        if (sl.done) {
+            ReduceJob rj;
+            rj.n = c->n;
+            rj.out = S->round_red + 4 * finished.size();
+            rj.fail = r->dfail;
+            if (ev.type == kEvalCrit) {
+                rj.a = c->res2;
+                rj.b = ev.crit == GWM_CRIT_CV ? nullptr : c->sii;
+                rj.c = nullptr;
+            } else {
+                rj.a = c->sii;
+                rj.b = c->ssi;
+                rj.c = c->res2;
+            }
+            reds.push_back(rj);
+            finished.push_back(r);
+        }
     }
     // pack every table, upload once
-    size_t off_sel[2] = {0, 0}, off_seli[2] = {0, 0};
+    size_t off_sel[2] = {0, 0}, off_sels[2] = {0, 0};
     for (int d = 0; d < 2; ++d)
         if (!sel[d].empty()) {
             off_sel[d] = T.push(sel[d].data(), sel[d].size() * sizeof(SelectJob));
-            off_seli[d] = T.push(sel_items[d].data(), sel_items[d].size() * sizeof(int));
+            off_sels[d] = T.push(sel_start[d].data(), sel_start[d].size() * sizeof(int));
         }
     for (auto* gv : {&sweeps, &qsweeps})
         for (auto& g : *gv) {
@@ -371,7 +487,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         g.off_jobs = T.push(g.jobs.data(), g.jobs.size() * sizeof(QDiagJob));
         g.off_start = T.push(g.start.data(), g.start.size() * sizeof(int));
     }
-    size_t off_red = T.push(reds.data(), reds.size() * sizeof(ReduceJob));
+    size_t off_red = reds.empty() ? 0 : T.push(reds.data(), reds.size() * sizeof(ReduceJob));
     if (T.host.size() > S->arena.cap) {
         if (S->arena.base) hipFree(S->arena.base);
         S->arena.base = nullptr;
@@ -381,14 +497,14 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         S->arena.cap = cap;
     }
     const char* dev = S->arena.base;
-    SRV_CHECK(S, hipMemcpyAsync(S->arena.base, T.host.data(), T.host.size(), hipMemcpyHostToDevice, S->stream));
-    SRV_CHECK(S, hipMemsetAsync(S->round_fail, 0, (size_t)ne * sizeof(int), S->stream));
-    // 1. adaptive bandwidths
+    if (!T.host.empty())
+        SRV_CHECK(S, hipMemcpyAsync(S->arena.base, T.host.data(), T.host.size(), hipMemcpyHostToDevice, S->stream));
+    // 1. adaptive bandwidths of the phase-1 focus points
     for (int d = 0; d < 2; ++d)
         if (!sel[d].empty())
             SRV_CHECK(S, launch_adaptive_bw_batch(d, reinterpret_cast<const SelectJob*>(dev + off_sel[d]),
-                                                  reinterpret_cast<const int*>(dev + off_seli[d]),
-                                                  (int)sel_items[d].size() / 2, S->stream));
+                                                  reinterpret_cast<const int*>(dev + off_sels[d]), (int)sel[d].size(),
+                                                  sel_start[d].back(), S->stream));
     // 2. criterion and hat sweeps
     int rc = launch_sweeps(S, sweeps, dev);
     if (rc) return rc;
@@ -397,30 +513,24 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         SRV_CHECK(S, launch_solve_batch(g.k, g.mode, reinterpret_cast<const SolveArgs*>(dev + g.off_jobs),
                                         reinterpret_cast<const int*>(dev + g.off_start), (int)g.jobs.size(),
                                         g.start.back(), S->stream));
-    // 4. qDiag second sweeps of the fits
-    if (!qsweeps.empty()) {
-        for (int e = 0; e < ne; ++e) {
-            Request* r = batch[e];
-            if (r->ev.type == kEvalFit && r->ev.precision == GWM_MIXED)
-                SRV_CHECK(S, launch_split_fp16(r->ctx->G, r->ctx->rows_pad, ntb_of(r->k) * 16, r->ctx->colmax,
-                                               r->ctx->Gh, r->ctx->Gl, r->ctx->gscale, S->stream));
-        }
-        if ((rc = launch_sweeps(S, qsweeps, dev))) return rc;
-        for (auto& g : qds)
-            SRV_CHECK(S, launch_qdiag_batch(g.k, reinterpret_cast<const QDiagJob*>(dev + g.off_jobs),
-                                            reinterpret_cast<const int*>(dev + g.off_start), (int)g.jobs.size(),
-                                            g.start.back(), S->stream));
+    // 4. qDiag second sweeps of the fits (G complete: every solve of the fit precedes on the stream)
+    for (Request* r : splits)
+        SRV_CHECK(S, launch_split_fp16(r->ctx->G, r->ctx->rows_pad, ntb_of(r->k) * 16, r->ctx->colmax, r->ctx->Gh,
+                                       r->ctx->Gl, r->ctx->gscale, S->stream));
+    if ((rc = launch_sweeps(S, qsweeps, dev))) return rc;
+    for (auto& g : qds)
+        SRV_CHECK(S, launch_qdiag_batch(g.k, reinterpret_cast<const QDiagJob*>(dev + g.off_jobs),
+                                        reinterpret_cast<const int*>(dev + g.off_start), (int)g.jobs.size(),
+                                        g.start.back(), S->stream));
+    // 5. reductions and results of the completed evaluations
+    const int nd = (int)finished.size();
+    std::vector<double> red((size_t)nd * 4);
+    if (nd) {
+        SRV_CHECK(S, launch_reduce_batch(reinterpret_cast<const ReduceJob*>(dev + off_red), nd, S->stream));
+        SRV_CHECK(S, hipMemcpyAsync(red.data(), S->round_red, red.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                    S->stream));
     }
-    // 5. reductions
-    SRV_CHECK(S, launch_reduce_batch(reinterpret_cast<const ReduceJob*>(dev + off_red), ne, S->stream));
-    std::vector<double> red((size_t)ne * 3);
-    std::vector<int> fail(ne);
-    SRV_CHECK(S, hipMemcpyAsync(red.data(), S->round_red, red.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                S->stream));
-    SRV_CHECK(S, hipMemcpyAsync(fail.data(), S->round_fail, fail.size() * sizeof(int), hipMemcpyDeviceToHost,
-                                S->stream));
-    for (int e = 0; e < ne; ++e) {
-        Request* r = batch[e];
+    for (Request* r : finished) {
         if (r->ev.type != kEvalFit) continue;
         const size_t nk = (size_t)r->n * r->k;
         r->hbetas.resize(nk);
@@ -432,24 +542,24 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
     }
     SRV_CHECK(S, hipStreamSynchronize(S->stream));
     long long nsolves = 0;
-    for (int e = 0; e < ne; ++e) {
-        Request* r = batch[e];
+    for (int i = 0; i < nd; ++i) {
+        Request* r = finished[i];
         Eval& ev = r->ev;
         nsolves += r->n;
         r->solves += r->n;
-        if (fail[e]) {
+        if (red[4 * i + 3] != 0.0) {
             ev.rc = GWM_ERR_NOT_SPD;
             ev.value = DBL_MAX;
             continue;
         }
         if (ev.type == kEvalCrit) {
-            double v = ev.crit == GWM_CRIT_CV ? red[3 * e] : aicc(red[3 * e], (double)r->n, red[3 * e + 1]);
+            double v = ev.crit == GWM_CRIT_CV ? red[4 * i] : aicc(red[4 * i], (double)r->n, red[4 * i + 1]);
             ev.rc = std::isfinite(v) ? GWM_OK : GWM_ERR_NOT_SPD;
             ev.value = std::isfinite(v) ? v : DBL_MAX;
         } else {
             ev.rc = GWM_OK;
-            ev.shat[0] = red[3 * e];
-            ev.shat[1] = red[3 * e + 1];
+            ev.shat[0] = red[4 * i];
+            ev.shat[1] = red[4 * i + 1];
         }
     }
     S->rounds += 1;
@@ -467,6 +577,8 @@ void worker_main(gwm_hip_server* S, Request* R) {
         R->ev.bw = bw;
         R->ev.precision = prec;
         R->ev.crit = crit;
+        R->phase = 1;
+        R->next = 0;
         R->posted = true;
         S->cv.notify_all();
         S->cv.wait(lk, [&] { return !R->posted; });
@@ -509,6 +621,7 @@ void worker_main(gwm_hip_server* S, Request* R) {
 void release(gwm_hip_server* S, Request* R) {
     if (R->part) hipFree(R->part);
     R->part = nullptr;
+    R->dfail = nullptr;
     if (R->ctx) gwm_hip_destroy(R->ctx);
     R->ctx = nullptr;
     S->inflight_bytes -= R->footprint;
@@ -522,7 +635,9 @@ int admit(gwm_hip_server* S, Request* R) {
     if (rc) return rc;
     R->ctx->certify_rel = S->certify_rel;
     const size_t bytes = (size_t)R->nseg * (nta_of(R->k) + ntb_of(R->k)) * 16 * R->ctx->n_pad * sizeof(double);
-    if (hipMalloc((void**)&R->part, bytes) != hipSuccess) return GWM_ERR_OOM;
+    if (hipMalloc((void**)&R->part, bytes + 256) != hipSuccess) return GWM_ERR_OOM;
+    R->dfail = reinterpret_cast<int*>(reinterpret_cast<char*>(R->part) + bytes);
+    if (hipMemsetAsync(R->dfail, 0, sizeof(int), S->stream) != hipSuccess) return GWM_ERR_HIP;
     if (R->select_bw && !R->adaptive && !R->has_upper) {
         if ((rc = max_distance_impl(R->ctx, &R->maxdist))) return rc;
     }
@@ -595,6 +710,8 @@ void scheduler_main(gwm_hip_server* S) {
             if (rc) {
                 R->ev.rc = rc;
                 R->ev.value = DBL_MAX;
+            } else if (R->phase != 3) {
+                continue;   // in progress: more slices next round
             }
             R->posted = false;
         }
@@ -631,6 +748,13 @@ int gwm_hip_server_set_certify_tolerance(gwm_hip_server* S, double rel) {
     if (!S || !(rel >= 0.0)) return GWM_ERR_INVALID;
     std::lock_guard<std::mutex> lk(S->mu);
     S->certify_rel = rel;
+    return GWM_OK;
+}
+
+int gwm_hip_server_set_round_ms(gwm_hip_server* S, double ms) {
+    if (!S || std::isnan(ms)) return GWM_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->round_ms = ms;
     return GWM_OK;
 }
 
@@ -716,7 +840,6 @@ void gwm_hip_server_destroy(gwm_hip_server* S) {
     for (auto& kv : S->reqs) delete kv.second;
     if (S->arena.base) hipFree(S->arena.base);
     if (S->round_red) hipFree(S->round_red);
-    if (S->round_fail) hipFree(S->round_fail);
     if (S->stream) hipStreamDestroy(S->stream);
     delete S;
 }

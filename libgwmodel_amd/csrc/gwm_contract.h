@@ -14,7 +14,7 @@ enum ContractMode { kModeCV = 0, kModeHat = 1, kModeQDiag = 2, kModePlain = 3 };
 
 // One problem's sweep.  The FP64 kernel reads Z, the mixed kernel the FP16
 // planes Zh/Zl with their per-column unscale factors.  nta/ntb are the
-// problem's own tile counts (<= the kernel's compile-time maxima).
+// problem's tile counts; a launch only groups jobs of equal counts.
 struct SweepJob {
     Points P;
     const double* Z;         // row-major [rows >= ceil(n/64)*64][nta*16], zero padded
@@ -48,7 +48,7 @@ int contract_mixed_tile_focus(int nta, int ntb);
 // row-major FP64 table; colmax is ldz words of scratch.
 hipError_t launch_split_fp16(const double* Z, int rows, int ldz, unsigned long long* colmax, _Float16* Zh,
                              _Float16* Zl, double* unscale, hipStream_t s);
-// nta/ntb select the compile-time tile counts (the batch maxima); grid size is
+// nta/ntb select the compile-time tile counts (those of every job); grid size is
 // nitems (batch) or tiles * nseg (single job).
 hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s);
 hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s);
@@ -88,11 +88,13 @@ struct SolveArgs {
     int ldg;
     double* gvec;          // row-major [n][k] g = A^-1 x_i (fit)
     int* fail;             // set to 1 on a non-SPD local design
+    int f0 = 0;            // first focus of a batched slice
 };
 
 hipError_t launch_solve(int k, int mode, const SolveArgs& a, hipStream_t s);
 // Batched solves of jobs with the same k: start[] (device, njobs+1 entries) is
-// the prefix sum of the jobs' n; total = start[njobs].
+// the prefix sum of the jobs' focus counts; entry t of job j solves focus
+// jobs[j].f0 + t - start[j]; total = start[njobs].
 hipError_t launch_solve_batch(int k, int mode, const SolveArgs* jobs, const int* start, int njobs, int total,
                               hipStream_t s);
 
@@ -102,6 +104,7 @@ struct QDiagJob {
     const double* X;
     const double* sii;
     double* qdiag;
+    int f0 = 0;            // first point of a batched slice
 };
 hipError_t launch_qdiag_batch(int k, const QDiagJob* jobs, const int* start, int njobs, int total, hipStream_t s);
 
@@ -110,7 +113,8 @@ struct ReduceJob {
     const double* b;
     const double* c;
     int n;
-    double* out;   // 3 sums
+    double* out;   // 3 sums, then the failure flag
+    int* fail;     // copied to out[3] and cleared; may be null
 };
 hipError_t launch_reduce_batch(const ReduceJob* jobs, int njobs, hipStream_t s);
 
@@ -118,9 +122,12 @@ struct SelectJob {
     Points P;
     double bw;
     double* out;
+    int f0;        // first focus of the slice
 };
-// Batched adaptive bandwidths: items are (job, focus) pairs, one workgroup each.
-hipError_t launch_adaptive_bw_batch(int dist, const SelectJob* jobs, const int* items, int nitems, hipStream_t s);
+// Batched adaptive bandwidths, one workgroup per focus: start[] is the prefix
+// sum of the jobs' focus counts (as for the batched solves).
+hipError_t launch_adaptive_bw_batch(int dist, const SelectJob* jobs, const int* start, int njobs, int total,
+                                    hipStream_t s);
 
 // qDiag_j = 1 - 2 S_jj + x_j^T M_j x_j, M_j from the kModeQDiag contraction.
 hipError_t launch_qdiag(int k, const double* part, int nseg, int PT, int n, int n_pad, const double* X,

@@ -20,8 +20,8 @@
 //   - Z rows and data coordinates of a 64-point chunk are staged in LDS and
 //     shared by the 4 waves.
 // A batched launch covers several problems: workgroup b takes the work item
-// (job, tile, segment) items[3b..3b+2]; each problem's own tile counts
-// (nta, ntb) may be below the kernel's compile-time maxima.
+// (job, tile, segment) items[3b..3b+2]; every job of a launch has the
+// instance's tile counts (nta, ntb).
 // Output: partial sums out[(seg * PT + c) * n_pad + i], PT = (nta + ntb) * 16,
 // reduced in a fixed order by the solve kernels (deterministic, independent
 // of scheduling and of the other problems of a batch).
@@ -59,7 +59,7 @@ struct FocusCoords {
 };
 
 template <int DIST>
-__device__ __forceinline__ FocusCoords<DIST> load_focus(const Points& P, int f) {
+__device__ __forceinline__ FocusCoords<DIST> load_focus(const GPoints& P, int f) {
     FocusCoords<DIST> fc;
     fc.a = P.px[f];
     fc.b = P.py[f];
@@ -86,21 +86,24 @@ __device__ __forceinline__ double pair_distance(const FocusCoords<DIST>& fc, con
 template <int NTA, int NTB, int DIST, int MODE, int R>
 __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
     __shared__ SharedTile<NTA, DIST, MODE> sh;
+    // The job is copied into registers once (uniform scalar loads): a
+    // reference into memory would be re-read after every LDS store.
     int tile, seg;
-    const SweepJob* jp;
+    SweepJob job;
     if (L.items) {
         const int* it = L.items + 3 * blockIdx.x;
-        jp = L.jobs + it[0];
+        job = L.jobs[it[0]];
         tile = it[1];
         seg = it[2];
     } else {
-        jp = &L.job0;
+        job = L.job0;
         seg = blockIdx.x % L.job0.nseg;
         tile = blockIdx.x / L.job0.nseg;
     }
-    const SweepJob& job = *jp;
-    const int nta = NTA == 1 ? 1 : job.nta;     // the problem's own tile counts
-    const int ntb = NTB == 0 ? 0 : job.ntb;
+    // a launch groups jobs of exactly the instance's tile counts
+    constexpr int nta = NTA, ntb = NTB;
+    const GPoints P = global_points(job.P);
+    GWM_GLOBAL const double* bw_pt = gptr(job.bw_pt);
     const int ldz = nta * 16;
     const int PT = (nta + ntb) * 16;
 
@@ -118,8 +121,8 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
         int f = fbase + 16 * r + (lane & 15);
         fidx[r] = f;
         int fl = f < n ? f : n - 1;
-        fc[r] = load_focus<DIST>(job.P, fl);
-        fbw[r] = make_kconst64((MODE != kModeQDiag && job.bw_pt) ? job.bw_pt[fl] : job.bw);
+        fc[r] = load_focus<DIST>(P, fl);
+        fbw[r] = make_kconst64((MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : job.bw);
     }
 
     v4d accA[R][NTA];
@@ -140,24 +143,24 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
         __syncthreads();
         // stage Z rows [j0, j0 + kChunk) (zero padded beyond n in memory)
         {
-            const double* src = job.Z + (size_t)j0 * ldz;
+            GWM_GLOBAL const double* src = gptr(job.Z) + (size_t)j0 * ldz;
             for (int e = threadIdx.x; e < kChunk * ldz; e += 256) sh.z[e / ldz][e % ldz] = src[e];
         }
         if (threadIdx.x < kChunk) {
             int j = j0 + threadIdx.x;
             int jl = j < n ? j : n - 1;
             if constexpr (DIST == kEuclidean) {
-                sh.c.x[threadIdx.x] = job.P.px[jl];
-                sh.c.y[threadIdx.x] = job.P.py[jl];
+                sh.c.x[threadIdx.x] = P.px[jl];
+                sh.c.y[threadIdx.x] = P.py[jl];
             } else {
-                sh.c.lon[threadIdx.x] = job.P.px[jl];
-                sh.c.lat[threadIdx.x] = job.P.py[jl];
-                sh.c.hx[threadIdx.x] = job.P.hx[jl];
-                sh.c.hy[threadIdx.x] = job.P.hy[jl];
-                sh.c.hz[threadIdx.x] = job.P.hz[jl];
-                sh.c.sl[threadIdx.x] = job.P.sl[jl];
+                sh.c.lon[threadIdx.x] = P.px[jl];
+                sh.c.lat[threadIdx.x] = P.py[jl];
+                sh.c.hx[threadIdx.x] = P.hx[jl];
+                sh.c.hy[threadIdx.x] = P.hy[jl];
+                sh.c.hz[threadIdx.x] = P.hz[jl];
+                sh.c.sl[threadIdx.x] = P.sl[jl];
             }
-            if constexpr (MODE == kModeQDiag) sh.kc[threadIdx.x] = make_kconst64(job.bw_pt ? job.bw_pt[jl] : job.bw);
+            if constexpr (MODE == kModeQDiag) sh.kc[threadIdx.x] = make_kconst64(bw_pt ? bw_pt[jl] : job.bw);
         }
         __syncthreads();
         const int steps = min(kChunk, jend - j0 + 3) / 4;
@@ -191,21 +194,21 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
     // D layout of v_mfma_f64_16x16x4f64 (checked on gfx950): lane holds
     // column (lane % 16), rows (lane / 16) + 4 q, q = 0..3.
     const size_t np = (size_t)job.n_pad;
-    double* out = job.out + (size_t)seg * PT * np;
+    GWM_GLOBAL double* out = gptr(job.out) + (size_t)seg * PT * np;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int f0 = fbase + 16 * r + kk;
 #pragma unroll
         for (int t = 0; t < NTA; ++t) {
             if (t >= nta) continue;
-            double* o = out + (size_t)(16 * t + (lane & 15)) * np + f0;
+            GWM_GLOBAL double* o = out + (size_t)(16 * t + (lane & 15)) * np + f0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) o[4 * q] = accA[r][t][q];
         }
 #pragma unroll
         for (int t = 0; t < NTB; ++t) {
             if (t >= ntb) continue;
-            double* o = out + (size_t)(ldz + 16 * t + (lane & 15)) * np + f0;
+            GWM_GLOBAL double* o = out + (size_t)(ldz + 16 * t + (lane & 15)) * np + f0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) o[4 * q] = accB[r][t][q];
         }

@@ -59,7 +59,7 @@ struct MFocus {
 };
 
 template <int DIST>
-__device__ __forceinline__ MFocus<DIST> load_mfocus(const Points& P, int f) {
+__device__ __forceinline__ MFocus<DIST> load_mfocus(const GPoints& P, int f) {
     MFocus<DIST> fc;
     if constexpr (DIST == kEuclidean) {
         fc.a = P.px[f];
@@ -181,21 +181,24 @@ __device__ __forceinline__ void split16(float w, _Float16& hi, _Float16& lo) {
 template <int NTA, int NTB, int DIST, int MODE, int R>
 __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     __shared__ MShared<NTA, DIST, MODE> sh;
+    // The job is copied into registers once (uniform scalar loads): a
+    // reference into memory would be re-read after every LDS store.
     int tile, seg;
-    const SweepJob* jp;
+    SweepJob args;
     if (L.items) {
         const int* it = L.items + 3 * blockIdx.x;
-        jp = L.jobs + it[0];
+        args = L.jobs[it[0]];
         tile = it[1];
         seg = it[2];
     } else {
-        jp = &L.job0;
+        args = L.job0;
         seg = blockIdx.x % L.job0.nseg;
         tile = blockIdx.x / L.job0.nseg;
     }
-    const SweepJob& args = *jp;
-    const int nta = NTA == 1 ? 1 : args.nta;    // the problem's own tile counts
-    const int ntb = NTB == 0 ? 0 : args.ntb;
+    // a launch groups jobs of exactly the instance's tile counts
+    constexpr int nta = NTA, ntb = NTB;
+    const GPoints P = global_points(args.P);
+    GWM_GLOBAL const double* bw_pt = gptr(args.bw_pt);
     const int ldz = nta * 16;
     const int PT = (nta + ntb) * 16;
 
@@ -214,8 +217,8 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
         int f = fbase + 16 * r + col;
         fidx[r] = f;
         int fl = f < n ? f : n - 1;
-        fc[r] = load_mfocus<DIST>(args.P, fl);
-        double b = (MODE != kModeQDiag && args.bw_pt) ? args.bw_pt[fl] : args.bw;
+        fc[r] = load_mfocus<DIST>(P, fl);
+        double b = (MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : args.bw;
         if (DIST == kEuclidean) fbw[r] = make_bw((float)b);
         else fbw[r] = make_bw((float)b);
     }
@@ -235,30 +238,30 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
         // stage the two 32-row blocks of both planes: global [blk][col][32]
         {
             const size_t base = (size_t)(j0 / 32) * ldz * 32;
-            const uint4* gh = reinterpret_cast<const uint4*>(args.Zh + base);
-            const uint4* gl = reinterpret_cast<const uint4*>(args.Zl + base);
+            GWM_GLOBAL const v4u* gh = reinterpret_cast<GWM_GLOBAL const v4u*>(gptr(args.Zh) + base);
+            GWM_GLOBAL const v4u* gl = reinterpret_cast<GWM_GLOBAL const v4u*>(gptr(args.Zl) + base);
             const int vecs = 2 * ldz * 4;   // 16-byte vectors per plane (8 halfs each)
             for (int e = threadIdx.x; e < vecs; e += 256) {
                 const int blk = e / (ldz * 4);
                 const int rem = e % (ldz * 4);
                 const int c = rem >> 2, part = rem & 3;
-                *reinterpret_cast<uint4*>(&sh.zh[blk][c][part * 8]) = gh[e];
-                *reinterpret_cast<uint4*>(&sh.zl[blk][c][part * 8]) = gl[e];
+                *reinterpret_cast<v4u*>(&sh.zh[blk][c][part * 8]) = gh[e];
+                *reinterpret_cast<v4u*>(&sh.zl[blk][c][part * 8]) = gl[e];
             }
         }
         if (threadIdx.x < kChunkM) {
             int j = j0 + threadIdx.x;
             int jl = j < n ? j : n - 1;
             if constexpr (DIST == kEuclidean) {
-                sh.c.x[threadIdx.x] = args.P.px[jl];
-                sh.c.y[threadIdx.x] = args.P.py[jl];
+                sh.c.x[threadIdx.x] = P.px[jl];
+                sh.c.y[threadIdx.x] = P.py[jl];
             } else {
-                sh.c.hx[threadIdx.x] = args.P.hx[jl];
-                sh.c.hy[threadIdx.x] = args.P.hy[jl];
-                sh.c.hz[threadIdx.x] = args.P.hz[jl];
-                sh.c.sl[threadIdx.x] = args.P.sl[jl];
+                sh.c.hx[threadIdx.x] = P.hx[jl];
+                sh.c.hy[threadIdx.x] = P.hy[jl];
+                sh.c.hz[threadIdx.x] = P.hz[jl];
+                sh.c.sl[threadIdx.x] = P.sl[jl];
             }
-            if constexpr (MODE == kModeQDiag) sh.bw[threadIdx.x] = (float)(args.bw_pt ? args.bw_pt[jl] : args.bw);
+            if constexpr (MODE == kModeQDiag) sh.bw[threadIdx.x] = (float)(bw_pt ? bw_pt[jl] : args.bw);
         }
         __syncthreads();
 #pragma unroll
@@ -349,7 +352,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     // D layout of v_mfma_f32_16x16x32_f16 (checked on gfx950): lane holds
     // column (lane % 16), rows 4 * (lane / 16) + q.
     const size_t np = (size_t)args.n_pad;
-    double* out = args.out + (size_t)seg * PT * np;
+    GWM_GLOBAL double* out = gptr(args.out) + (size_t)seg * PT * np;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int f0 = fbase + 16 * r + 4 * kq;
@@ -359,8 +362,8 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
             const int tt = isB ? t - NTA : t;
             if (tt >= (isB ? ntb : nta)) continue;
             const int zc = 16 * tt + col;
-            const double unscale = args.colscale[zc];
-            double* o = out + (size_t)((isB ? ldz : 0) + zc) * np + f0;
+            const double unscale = gptr(args.colscale)[zc];
+            GWM_GLOBAL double* o = out + (size_t)((isB ? ldz : 0) + zc) * np + f0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) o[q] = acc[r][t][q] * unscale;
         }

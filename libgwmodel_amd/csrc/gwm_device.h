@@ -17,6 +17,7 @@ enum DistId { kEuclidean = 0, kGreatCircle = 1 };
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 // Point table on the device, structure of arrays.  Euclidean uses px/py.
 // Great-circle uses the raw lon/lat (degrees) for the coincident-point test
@@ -31,6 +32,30 @@ struct Points {
     const double* sl;
     int n;
 };
+
+// Pointers into global memory.  Pointers read from a job table in memory are
+// generic, and loads through generic pointers compile to flat loads, which
+// also count against the LDS wait counter; casting them to the global address
+// space once gives global loads.
+#define GWM_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GWM_GLOBAL T* gptr(T* p) {
+    return (GWM_GLOBAL T*)p;
+}
+
+struct GPoints {
+    GWM_GLOBAL const double* px;
+    GWM_GLOBAL const double* py;
+    GWM_GLOBAL const double* hx;
+    GWM_GLOBAL const double* hy;
+    GWM_GLOBAL const double* hz;
+    GWM_GLOBAL const double* sl;
+    int n;
+};
+
+__device__ __forceinline__ GPoints global_points(const Points& P) {
+    return GPoints{gptr(P.px), gptr(P.py), gptr(P.hx), gptr(P.hy), gptr(P.hz), gptr(P.sl), P.n};
+}
 
 // ---------------------------------------------------------------- FP64
 
@@ -145,8 +170,8 @@ __device__ __forceinline__ double kernel_fast_f64(int kernel, double d, const KC
     }
 }
 
-template <int DIST>
-__device__ __forceinline__ double distance_f64(const Points& P, int f, int j) {
+template <int DIST, class PTS>
+__device__ __forceinline__ double distance_f64(const PTS& P, int f, int j) {
     if (DIST == kEuclidean) {
         return sqrt(d2_euclid(P.px[f], P.py[f], P.px[j], P.py[j]));
     } else {
@@ -158,8 +183,8 @@ __device__ __forceinline__ double distance_f64(const Points& P, int f, int j) {
 // Selection key: a monotone image of the distance.  Euclidean uses the
 // squared distance (sqrt is monotone, so order statistics commute with it);
 // great-circle uses the distance itself.
-template <int DIST>
-__device__ __forceinline__ double select_value(const Points& P, int f, int j) {
+template <int DIST, class PTS>
+__device__ __forceinline__ double select_value(const PTS& P, int f, int j) {
     if (DIST == kEuclidean) return d2_euclid(P.px[f], P.py[f], P.px[j], P.py[j]);
     return distance_f64<DIST>(P, f, j);
 }
@@ -223,6 +248,18 @@ __device__ __forceinline__ float kernel_f32(int kernel, float d, float bw, float
     }
     default: return d < bw ? 1.0f : 0.0f;
     }
+}
+
+// Job of batch entry t, given the prefix sums start[0..njobs] of the jobs'
+// entry counts (start[0] = 0): the largest j with start[j] <= t.
+__device__ __forceinline__ int find_job(const int* start, int njobs, int t) {
+    int lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (start[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
 }
 
 }  // namespace gwm

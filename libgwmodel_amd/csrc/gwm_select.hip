@@ -19,8 +19,8 @@ namespace {
 __device__ __forceinline__ unsigned long long key_of(double v) { return (unsigned long long)__double_as_longlong(v); }
 __device__ __forceinline__ double val_of(unsigned long long k) { return __longlong_as_double((long long)k); }
 
-template <int DIST>
-__device__ __forceinline__ void adaptive_bw_one(const Points& P, double bw, double* out, int f) {
+template <int DIST, class PTS, class OUT>
+__device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out, int f) {
     const int n = P.n;
     const int tid = threadIdx.x;
     __shared__ unsigned int hist[256];
@@ -105,9 +105,11 @@ __global__ __launch_bounds__(256) void adaptive_bw_kernel(Points P, double bw, d
 }
 
 template <int DIST>
-__global__ __launch_bounds__(256) void adaptive_bw_batch_kernel(const SelectJob* jobs, const int* items) {
-    const SelectJob& J = jobs[items[2 * blockIdx.x]];
-    adaptive_bw_one<DIST>(J.P, J.bw, J.out, items[2 * blockIdx.x + 1]);
+__global__ __launch_bounds__(256) void adaptive_bw_batch_kernel(const SelectJob* jobs, const int* start, int njobs) {
+    const int t = blockIdx.x;
+    const int j = find_job(start, njobs, t);
+    const SelectJob J = jobs[j];
+    adaptive_bw_one<DIST>(global_points(J.P), J.bw, gptr(J.out), J.f0 + t - start[j]);
 }
 
 template <int DIST>
@@ -143,12 +145,13 @@ hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out,
     return hipGetLastError();
 }
 
-hipError_t launch_adaptive_bw_batch(int dist, const SelectJob* jobs, const int* items, int nitems, hipStream_t s) {
-    if (nitems <= 0) return hipSuccess;
+hipError_t launch_adaptive_bw_batch(int dist, const SelectJob* jobs, const int* start, int njobs, int total,
+                                    hipStream_t s) {
+    if (total <= 0) return hipSuccess;
     if (dist == kEuclidean)
-        hipLaunchKernelGGL(adaptive_bw_batch_kernel<kEuclidean>, dim3(nitems), dim3(256), 0, s, jobs, items);
+        hipLaunchKernelGGL(adaptive_bw_batch_kernel<kEuclidean>, dim3(total), dim3(256), 0, s, jobs, start, njobs);
     else
-        hipLaunchKernelGGL(adaptive_bw_batch_kernel<kGreatCircle>, dim3(nitems), dim3(256), 0, s, jobs, items);
+        hipLaunchKernelGGL(adaptive_bw_batch_kernel<kGreatCircle>, dim3(total), dim3(256), 0, s, jobs, start, njobs);
     return hipGetLastError();
 }
 

@@ -182,22 +182,12 @@ __global__ __launch_bounds__(128) void solve_kernel(SolveArgs a) {
 
 // Batched solves: thread t of the launch belongs to the job whose focus range
 // [start[j], start[j+1]) holds t (start is a prefix sum of the job sizes).
-__device__ __forceinline__ int find_job(const int* start, int njobs, int t) {
-    int lo = 0, hi = njobs;   // start[lo] <= t < start[hi]
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (start[mid] <= t) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 template <int K, int MODE>
 __global__ __launch_bounds__(128) void solve_batch_kernel(const SolveArgs* jobs, const int* start, int njobs) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= start[njobs]) return;
     const int j = find_job(start, njobs, t);
-    solve_one<K, MODE>(jobs[j], t - start[j]);
+    solve_one<K, MODE>(jobs[j], jobs[j].f0 + t - start[j]);
 }
 
 template <int K>
@@ -231,7 +221,7 @@ __global__ __launch_bounds__(128) void qdiag_batch_kernel(const QDiagJob* jobs, 
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= start[njobs]) return;
     const int j = find_job(start, njobs, t);
-    qdiag_one<K>(jobs[j], t - start[j]);
+    qdiag_one<K>(jobs[j], jobs[j].f0 + t - start[j]);
 }
 
 template <int DIST>
@@ -304,6 +294,10 @@ __global__ __launch_bounds__(1024) void reduce_batch_kernel(const ReduceJob* job
         J.out[0] = sa[0];
         J.out[1] = sb[0];
         J.out[2] = sc[0];
+        if (J.fail) {
+            J.out[3] = (double)*J.fail;
+            *J.fail = 0;
+        }
     }
 }
 

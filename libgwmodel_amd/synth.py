@@ -99,3 +99,62 @@ def config3(n=50_000, seed=1003, extent=1000.0, scales=(20, 50, 100, 200, 400, 1
             B[:, c] += a * np.cos((om[0] * u + om[1] * v) / ell + ph)
     y = np.sum(X * B, axis=1) + s.normal(n, 0, 0.5)
     return dict(coords=np.column_stack([u, v]), X=X, y=y, beta=B)
+
+
+KERNELS_CONFIG4 = (0, 2, 3)   # Gaussian, bisquare, tricube
+
+
+def config4(num=512, seed=1004):
+    """Uneven batch: num problems, n ~ logU[200, 20000], k ~ U{3..12},
+    coordinates U[0,100]^2, intercept + N(0,1) covariates, each beta linear
+    plus one sinusoid (frequency U[0.01, 0.1]), noise N(0, 0.2^2); kernel from
+    {Gaussian, bisquare, tricube}, adaptive or fixed with equal probability."""
+    s = Stream(seed)
+    probs = []
+    for p in range(num):
+        n = int(np.exp(s.uniform(1, np.log(200), np.log(20000))[0]))
+        k = 3 + int(s.uniform(1, 0, 10)[0])
+        u = s.uniform(n, 0, 100)
+        v = s.uniform(n, 0, 100)
+        X = np.column_stack([np.ones(n)] + [s.normal(n) for _ in range(k - 1)])
+        B = np.zeros((n, k))
+        for c in range(k):
+            a0, a1, a2 = s.normal(3)
+            fr = s.uniform(1, 0.01, 0.1)[0]
+            th = s.uniform(1, 0, 2 * np.pi)[0]
+            B[:, c] = a0 + 0.01 * (a1 * u + a2 * v) + np.sin(2 * np.pi * fr * (u * np.cos(th) + v * np.sin(th)))
+        y = np.sum(X * B, axis=1) + s.normal(n, 0, 0.2)
+        kernel = KERNELS_CONFIG4[int(s.uniform(1, 0, 3)[0])]
+        adaptive = bool(s.uniform(1)[0] < 0.5)
+        probs.append(dict(coords=np.column_stack([u, v]), X=X, y=y, kernel=kernel, adaptive=adaptive))
+    return probs
+
+
+def config5(num=64, rate=2.0, seed=1005, n_range=(500, 200_000)):
+    """Serving stream: Poisson arrivals at `rate` requests/s; 70% fits at a
+    given bandwidth, 30% CV bandwidth searches; n ~ logU[n_range], k ~ U{2..16};
+    coordinates from 8 Gaussian clusters (sigma 5) on [0,100]^2; N(0,1)
+    covariates, beta as in config 3 (random cosines), noise N(0, 0.5^2)."""
+    s = Stream(seed)
+    centres = s.uniform(16, 10, 90).reshape(8, 2)
+    t = 0.0
+    reqs = []
+    for r in range(num):
+        t += -np.log(1.0 - s.uniform(1)[0]) / rate
+        n = int(np.exp(s.uniform(1, np.log(n_range[0]), np.log(n_range[1]))[0]))
+        k = 2 + int(s.uniform(1, 0, 15)[0])
+        cl = (s.uniform(n, 0, 8)).astype(int)
+        coords = centres[cl] + s.normal(2 * n, 0, 5).reshape(n, 2)
+        X = np.column_stack([np.ones(n)] + [s.normal(n) for _ in range(k - 1)])
+        B = np.zeros((n, k))
+        for c in range(k):
+            ell = 10.0 * (1 + c)
+            for _ in range(4):
+                a = s.normal(1)[0] / 2.0
+                om = s.normal(2)
+                ph = s.uniform(1, 0, 2 * np.pi)[0]
+                B[:, c] += a * np.cos((om[0] * coords[:, 0] + om[1] * coords[:, 1]) / ell + ph)
+        y = np.sum(X * B, axis=1) + s.normal(n, 0, 0.5)
+        search = bool(s.uniform(1)[0] < 0.3)
+        reqs.append(dict(t=t, coords=coords, X=X, y=y, search=search))
+    return reqs

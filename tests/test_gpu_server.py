@@ -90,6 +90,29 @@ def test_alone_equals_in_the_mix(precision):
     assert mixed["diagnostic"] == alone["diagnostic"]
 
 
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+def test_sliced_rounds_equal_whole_rounds(precision):
+    """Evaluations cut into one chunk of focus points per round (a tiny round
+    budget) give the same bits as evaluations run whole in one round."""
+    say("case: sliced vs whole rounds precision=%d" % precision)
+    cases = [MIX[0], MIX[3], MIX[4]]   # adaptive CV search, Gaussian fit k=12, fixed AIC search
+    out = {}
+    for ms in (0.0, 1e-6):
+        with capi.Server(0, precision, max_inflight=8) as srv:
+            srv.set_round_ms(ms)
+            ids = [submit(srv, c) for c in cases]
+            out[ms] = [srv.wait(i) for i in ids]
+            st = srv.stats()
+        say("round_ms %g: rounds %d" % (ms, st["rounds"]))
+    for a, b in zip(out[0.0], out[1e-6]):
+        assert a["status"] == capi.GWM_OK and b["status"] == capi.GWM_OK
+        assert a["bandwidth"] == b["bandwidth"]
+        assert np.array_equal(a["betas"], b["betas"])
+        assert np.array_equal(a["betasSE"], b["betasSE"])
+        assert np.array_equal(a["qDiag"], b["qDiag"])
+        assert a["diagnostic"] == b["diagnostic"]
+
+
 def test_server_search_matches_oracle(oracle):
     say("case: server CV search vs oracle (config1 n=800)")
     d = synth.config1(n=800)
