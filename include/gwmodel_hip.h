@@ -135,6 +135,18 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* ctx, const int* kernels, const int* adaptive, 
  * closer than the bound is re-taken on FP64 re-evaluations.  No counterpart in
  * the reference, which has no mixed-precision path. */
 int gwm_hip_set_certify_tolerance(gwm_hip_ctx* ctx, double rel_tol);
+/* Mixed precision: the error of a mixed local solve grows with the
+ * conditioning of its local design.  Every mixed evaluation computes, per
+ * focus point, the indicator max_a A_aa / L_aa^2 (A = X^T W X, L its Cholesky
+ * factor: the inverse smallest squared pivot of the Jacobi-scaled design,
+ * >= 1, infinite when not SPD), and recomputes in FP64 -- the FP64 path's own
+ * kernel and data segments, hence its bits -- every focus point whose
+ * indicator exceeds the limit: criterion_limit in criterion evaluations
+ * (default 100), fit_limit in fits (default 2).  gwm_hip_last_condition
+ * returns the indicator maximum of the last mixed evaluation and the number
+ * of focus points it recomputed.  No counterpart in the reference. */
+int gwm_hip_set_refine_condition(gwm_hip_ctx* ctx, double criterion_limit, double fit_limit);
+int gwm_hip_last_condition(gwm_hip_ctx* ctx, double* cond, long long* refined);
 
 /* Timing of the device kernels with HIP events on the context stream
  * (enabled by gwm_hip_set_timing).  name: "contract_<fp64|mixed>_<cv|plain|
@@ -167,6 +179,7 @@ typedef struct gwm_hip_server gwm_hip_server;
 int gwm_hip_server_create(int device, int precision, double hbm_budget_bytes, int max_inflight,
                           gwm_hip_server** out);
 int gwm_hip_server_set_certify_tolerance(gwm_hip_server* server, double rel_tol);
+int gwm_hip_server_set_refine_condition(gwm_hip_server* server, double criterion_limit, double fit_limit);
 /* Round budget in ms of device time; <= 0 runs every posted evaluation whole
  * in each round. */
 int gwm_hip_server_set_round_ms(gwm_hip_server* server, double round_ms);

@@ -52,12 +52,15 @@ SIGNATURES = {
     "gwm_hip_mgwr_run": (_i, [_vp, _ip, _ip, _dp, _ip, _ip, _ip, _dp, _i, _i, _d, _i, _i, _i, _i, _d, _i, _d, _i,
                               _dp, _dp, _dp, _dp, _ip]),
     "gwm_hip_set_certify_tolerance": (_i, [_vp, _d]),
+    "gwm_hip_set_refine_condition": (_i, [_vp, _d, _d]),
+    "gwm_hip_last_condition": (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "gwm_hip_set_timing": (_i, [_vp, _i]),
     "gwm_hip_kernel_time": (_i, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong), _dp]),
     "gwm_hip_reset_timers": (_i, [_vp]),
     "gwm_hip_solve_count": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "gwm_hip_server_create": (_i, [_i, _i, _d, _i, ctypes.POINTER(_vp)]),
     "gwm_hip_server_set_certify_tolerance": (_i, [_vp, _d]),
+    "gwm_hip_server_set_refine_condition": (_i, [_vp, _d, _d]),
     "gwm_hip_server_set_round_ms": (_i, [_vp, _d]),
     "gwm_hip_server_submit": (_i, [_vp, _i, _i, _dp, _dp, _dp, _i, _i, _i, _d, _i, _i, _i, _d, _i, _d, _dp, _dp, _dp,
                                    _dp, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
@@ -265,6 +268,18 @@ class Context:
     def set_certify_tolerance(self, rel):
         self._check(self._lib.gwm_hip_set_certify_tolerance(self.h, float(rel)))
 
+    def set_refine_condition(self, criterion_limit, fit_limit):
+        """Mixed precision: focus points whose conditioning indicator exceeds
+        the limit are recomputed in FP64 (criterion evaluations / fits)."""
+        self._check(self._lib.gwm_hip_set_refine_condition(self.h, float(criterion_limit), float(fit_limit)))
+
+    def last_condition(self):
+        """(indicator maximum, focus points recomputed in FP64) of the last mixed evaluation."""
+        v = np.zeros(1)
+        r = ctypes.c_longlong(0)
+        self._check(self._lib.gwm_hip_last_condition(self.h, _p(v), ctypes.byref(r)))
+        return float(v[0]), r.value
+
     def set_timing(self, on=True):
         self._check(self._lib.gwm_hip_set_timing(self.h, int(on)))
 
@@ -312,6 +327,11 @@ class Server:
 
     def set_certify_tolerance(self, rel):
         rc = self._lib.gwm_hip_server_set_certify_tolerance(self.h, float(rel))
+        if rc != GWM_OK:
+            raise GwmError(rc)
+
+    def set_refine_condition(self, criterion_limit, fit_limit):
+        rc = self._lib.gwm_hip_server_set_refine_condition(self.h, float(criterion_limit), float(fit_limit))
         if rc != GWM_OK:
             raise GwmError(rc)
 

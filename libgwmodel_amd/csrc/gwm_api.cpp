@@ -52,7 +52,9 @@ void dfree(T*& p) {
 void free_problem(gwm_hip_ctx* c) {
     dfree(c->px); dfree(c->py); dfree(c->hx); dfree(c->hy_); dfree(c->hz); dfree(c->sl);
     dfree(c->X); dfree(c->y); dfree(c->Z); dfree(c->G); dfree(c->part);
-    dfree(c->bwi); dfree(c->res2); dfree(c->sii); dfree(c->ssi); dfree(c->betas); dfree(c->se);
+    dfree(c->bwi); dfree(c->res2); dfree(c->cond); dfree(c->sii); dfree(c->rlist); dfree(c->rcount);
+    dfree(c->rpart);
+    c->rpart_bytes = 0; dfree(c->ssi); dfree(c->betas); dfree(c->se);
     dfree(c->gvec); dfree(c->qd); dfree(c->red); dfree(c->blockmax); dfree(c->vx); dfree(c->vy);
     dfree(c->vZ); dfree(c->S); dfree(c->fail);
     dfree(c->Zh); dfree(c->Zl); dfree(c->Gh); dfree(c->Gl); dfree(c->vZh); dfree(c->vZl);
@@ -67,18 +69,6 @@ int dalloc(gwm_hip_ctx* c, T** p, size_t count) {
     return GWM_OK;
 }
 
-// Segmentation of the data range: enough workgroups to fill 256 CUs several
-// times over, segment count a multiple of 8 (one slice of Z per XCD).
-void plan_segments(int n, int tile_focus, int& nseg, int& seg_len) {
-    int tiles = (n + tile_focus - 1) / tile_focus;
-    int want = (2048 + tiles - 1) / tiles;
-    want = std::max(1, want);
-    if (want > 1) want = ((want + 7) / 8) * 8;
-    int max_seg = (n + 63) / 64;
-    want = std::min(want, max_seg);
-    seg_len = ((n + want - 1) / want + 63) / 64 * 64;
-    nseg = (n + seg_len - 1) / seg_len;
-}
 
 int ensure_part(gwm_hip_ctx* c, size_t bytes) {
     if (bytes <= c->part_bytes) return GWM_OK;
@@ -143,9 +133,8 @@ const char* mode_name(int mode) {
 int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const double* bw_pt, double bw, int kernel,
           int precision, int* PT_out, int* nseg_out) {
     const bool mixed = precision == GWM_MIXED;
-    int tile_focus = mixed ? contract_mixed_tile_focus(nta, ntb) : contract_tile_focus(nta, ntb);
     int nseg, seg_len;
-    plan_segments(c->n, tile_focus, nseg, seg_len);
+    data_segments(c->n, nseg, seg_len);
     const int PT = (nta + ntb) * 16;
     int rc = ensure_part(c, (size_t)nseg * PT * c->n_pad * sizeof(double));
     if (rc) return rc;
@@ -207,7 +196,73 @@ SolveArgs solve_args(gwm_hip_ctx* c, int PT, int nseg, int colB, const double* b
     s.ldg = c->ntb * 16;
     s.gvec = c->gvec;
     s.fail = c->fail;
+    s.cond = c->cond;
     return s;
+}
+
+// After a mixed-precision sweep and solve over all points: recompute in FP64
+// the focus points whose local design is ill conditioned (indicator above
+// limit, or not SPD), so that their results are those of the FP64 path (same
+// kernel, same data segments, so the same bits).  The failure flag is left as
+// the FP64 solves set it: every point the mixed solves failed on is in the
+// list.  *count receives the number of recomputed points; the indicator
+// maximum of the mixed pass goes to c->last_cond.
+int refine_fp64(gwm_hip_ctx* c, int mode, int smode, int ntb, int colB, const double* bw_pt, double bw, int kernel,
+                double limit, int* count) {
+    GWM_HIP_CHECK(c, launch_reduce(nullptr, nullptr, nullptr, c->cond, c->n, c->red, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(&c->last_cond, c->red + 3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipMemsetAsync(c->rcount, 0, sizeof(int), c->stream));
+    GWM_HIP_CHECK(c, launch_compact(c->cond, c->n, limit, c->rlist, c->rcount, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(count, c->rcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->last_refined = *count;
+    if (*count == 0) return GWM_OK;
+    const int nta = c->nta;
+    int nseg, seg_len;
+    data_segments(c->n, nseg, seg_len);
+    const int PT = (nta + ntb) * 16;
+    const int npad = (*count + 127) / 128 * 128;
+    const size_t bytes = (size_t)nseg * PT * npad * sizeof(double);
+    if (bytes > c->rpart_bytes) {
+        dfree(c->rpart);
+        c->rpart_bytes = 0;
+        GWM_HIP_CHECK(c, hipMalloc((void**)&c->rpart, bytes));
+        c->rpart_bytes = bytes;
+    }
+    SweepLaunch L;
+    SweepJob& j = L.job0;
+    j.P = c->points();
+    j.Z = c->Z;
+    j.Zh = nullptr;
+    j.Zl = nullptr;
+    j.colscale = nullptr;
+    j.bw_pt = bw_pt;
+    j.bw = bw;
+    j.kernel = kernel;
+    j.n = c->n;
+    j.n_pad = npad;
+    j.seg_len = seg_len;
+    j.nseg = nseg;
+    j.nta = nta;
+    j.ntb = ntb;
+    j.out = c->rpart;
+    j.focus_idx = c->rlist;
+    j.nfocus = *count;
+    L.jobs = nullptr;
+    L.items = nullptr;
+    L.nitems = 0;
+    timer_begin(c);
+    GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, L, c->stream));
+    int rc = timer_end(c, "refine");
+    if (rc) return rc;
+    GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
+    SolveArgs s = solve_args(c, PT, nseg, colB, bw_pt, bw, kernel);
+    s.part = c->rpart;
+    s.n_pad = npad;
+    s.focus_idx = c->rlist;
+    s.nfocus = *count;
+    GWM_HIP_CHECK(c, launch_solve(c->k, smode, s, c->stream));
+    return GWM_OK;
 }
 
 }  // namespace
@@ -282,11 +337,17 @@ int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit
     const int mode = crit == GWM_CRIT_CV ? kModeCV : kModePlain;
     if ((rc = sweep(c, mode, c->nta, 0, design_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
     SolveArgs s = solve_args(c, PT, nseg, 0, bw_pt, bw, kernel);
+    const int smode = crit == GWM_CRIT_CV ? kSolveCV : kSolveAIC;
     timer_begin(c);
-    GWM_HIP_CHECK(c, launch_solve(c->k, crit == GWM_CRIT_CV ? kSolveCV : kSolveAIC, s, c->stream));
+    GWM_HIP_CHECK(c, launch_solve(c->k, smode, s, c->stream));
     if ((rc = timer_end(c, "solve"))) return rc;
-    GWM_HIP_CHECK(c, launch_reduce(c->res2, crit == GWM_CRIT_CV ? nullptr : c->sii, nullptr, c->n, c->red, c->stream));
-    double red[3];
+    if (precision == GWM_MIXED) {
+        int count = 0;
+        if ((rc = refine_fp64(c, mode, smode, 0, 0, bw_pt, bw, kernel, c->refine_crit, &count))) return rc;
+    }
+    GWM_HIP_CHECK(c, launch_reduce(c->res2, crit == GWM_CRIT_CV ? nullptr : c->sii, nullptr, nullptr, c->n, c->red,
+                                   c->stream));
+    double red[4];
     int fail = 0;
     GWM_HIP_CHECK(c, hipMemcpyAsync(red, c->red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
     GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -509,7 +570,7 @@ int var_impl(gwm_hip_ctx* c, const double* x, const double* y, int kernel, int a
                                       c->res2, c->sii, beta ? c->betas : nullptr, c->fail, c->stream));
     int fail = 0;
     if (value) {
-        GWM_HIP_CHECK(c, launch_reduce(c->res2, cv ? nullptr : c->sii, nullptr, n, c->red, c->stream));
+        GWM_HIP_CHECK(c, launch_reduce(c->res2, cv ? nullptr : c->sii, nullptr, nullptr, n, c->red, c->stream));
         double red[3];
         GWM_HIP_CHECK(c, hipMemcpyAsync(red, c->red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
         GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -539,6 +600,10 @@ int fit_betas_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int prec
     if ((rc = sweep(c, kModePlain, c->nta, 0, design_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
     SolveArgs s = solve_args(c, PT, nseg, 0, bw_pt, bw, kernel);
     GWM_HIP_CHECK(c, launch_solve(c->k, kSolveBeta, s, c->stream));
+    if (precision == GWM_MIXED) {
+        int count = 0;
+        if ((rc = refine_fp64(c, kModePlain, kSolveBeta, 0, 0, bw_pt, bw, kernel, c->refine_fit, &count))) return rc;
+    }
     int fail = 0;
     GWM_HIP_CHECK(c, hipMemcpyAsync(betas, c->betas, (size_t)c->n * c->k * sizeof(double), hipMemcpyDeviceToHost,
                                     c->stream));
@@ -622,9 +687,10 @@ int gwm_hip_set_data(gwm_hip_ctx* c, int n, int k, const double* coords, const d
     if ((rc = dalloc(c, &c->px, N)) || (rc = dalloc(c, &c->py, N)) || (rc = dalloc(c, &c->X, N * k)) ||
         (rc = dalloc(c, &c->y, N)) || (rc = dalloc(c, &c->Z, (size_t)c->rows_pad * c->nta * 16)) ||
         (rc = dalloc(c, &c->G, (size_t)c->rows_pad * c->ntb * 16)) || (rc = dalloc(c, &c->bwi, N)) ||
-        (rc = dalloc(c, &c->res2, N)) || (rc = dalloc(c, &c->sii, N)) || (rc = dalloc(c, &c->ssi, N)) ||
+        (rc = dalloc(c, &c->res2, N)) || (rc = dalloc(c, &c->cond, N)) ||
+        (rc = dalloc(c, &c->rlist, N)) || (rc = dalloc(c, &c->rcount, 1)) || (rc = dalloc(c, &c->sii, N)) || (rc = dalloc(c, &c->ssi, N)) ||
         (rc = dalloc(c, &c->betas, N * k)) || (rc = dalloc(c, &c->se, N * k)) || (rc = dalloc(c, &c->gvec, N * k)) ||
-        (rc = dalloc(c, &c->qd, N)) || (rc = dalloc(c, &c->red, 4)) || (rc = dalloc(c, &c->blockmax, (N + 255) / 256)) ||
+        (rc = dalloc(c, &c->qd, N)) || (rc = dalloc(c, &c->red, 8)) || (rc = dalloc(c, &c->blockmax, (N + 255) / 256)) ||
         (rc = dalloc(c, &c->vx, N)) || (rc = dalloc(c, &c->vy, N)) || (rc = dalloc(c, &c->vZ, (size_t)c->rows_pad * 16)) ||
         (rc = dalloc(c, &c->fail, 1)) || (rc = dalloc(c, &c->Zh, (size_t)c->rows_pad * c->nta * 16)) ||
         (rc = dalloc(c, &c->Zl, (size_t)c->rows_pad * c->nta * 16)) ||
@@ -688,6 +754,11 @@ int gwm_hip_fit(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int precisi
     timer_begin(c);
     GWM_HIP_CHECK(c, launch_solve(c->k, kSolveFit, s, c->stream));
     if ((rc = timer_end(c, "solve"))) return rc;
+    if (precision == GWM_MIXED) {
+        int count = 0;
+        if ((rc = refine_fp64(c, kModeHat, kSolveFit, c->ntb, c->nta * 16, bw_pt, bw, kernel, c->refine_fit, &count)))
+            return rc;
+    }
     int fail = 0;
     GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -714,7 +785,7 @@ int gwm_hip_fit(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int precisi
         GWM_HIP_CHECK(c, launch_smatrix(c->dist, c->k, c->points(), c->X, c->gvec, bw_pt, bw, kernel, c->S, c->stream));
         GWM_HIP_CHECK(c, hipMemcpyAsync(S, c->S, (size_t)c->n * c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
-    GWM_HIP_CHECK(c, launch_reduce(c->sii, c->ssi, c->res2, c->n, c->red, c->stream));
+    GWM_HIP_CHECK(c, launch_reduce(c->sii, c->ssi, c->res2, nullptr, c->n, c->red, c->stream));
     double red[3];
     GWM_HIP_CHECK(c, hipMemcpyAsync(red, c->red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
     const size_t nk = (size_t)c->n * c->k;
@@ -962,6 +1033,20 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
 int gwm_hip_set_certify_tolerance(gwm_hip_ctx* c, double rel) {
     if (!c || !(rel >= 0.0)) return GWM_ERR_INVALID;
     c->certify_rel = rel;
+    return GWM_OK;
+}
+
+int gwm_hip_set_refine_condition(gwm_hip_ctx* c, double criterion_limit, double fit_limit) {
+    if (!c || std::isnan(criterion_limit) || std::isnan(fit_limit)) return GWM_ERR_INVALID;
+    c->refine_crit = criterion_limit;
+    c->refine_fit = fit_limit;
+    return GWM_OK;
+}
+
+int gwm_hip_last_condition(gwm_hip_ctx* c, double* cond, long long* refined) {
+    if (!c) return GWM_ERR_INVALID;
+    if (cond) *cond = c->last_cond;
+    if (refined) *refined = c->last_refined;
     return GWM_OK;
 }
 

@@ -21,9 +21,10 @@
 // partial sums; never n^2) fits the memory budget.
 //
 // Each request's arithmetic depends only on its own data: its data segments
-// are a function of its n alone, a focus point's partial sums and solve do not
-// depend on which round computes them, and every reduction has a fixed order,
-// so a request returns the same bits alone or in any mix.
+// are a function of its n alone (those of the single-problem API), a focus
+// point's partial sums and solve do not depend on which round computes them,
+// and every reduction has a fixed order, so a request returns the same bits
+// alone, in any mix, or through the single-problem API.
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -49,6 +50,17 @@ double now_ms() {
 
 enum { kEvalCrit = 0, kEvalFit = 1 };
 
+// Phases of an evaluation.  Mixed precision adds a check at the end of the
+// main sweep (which focus points have an ill-conditioned local design?) and,
+// when there are any, an FP64 recomputation of those points.
+enum Phase {
+    kMain = 1,     // adaptive bandwidths, sweep and solves over the focus points
+    kQDiag = 2,    // fits: second sweep and qDiag
+    kDone = 3,
+    kRefine = 4,   // FP64 sweep and solves over the listed focus points
+    kCheck = 5     // main sweep done; the list is counted at the end of the round
+};
+
 // Slicing of evaluations across rounds.  Work is counted in units of one data
 // point x one 16-column tile of one focus point in the mixed sweep; an FP64
 // sweep tile costs about 3.5 units and an adaptive-bandwidth selection about 4
@@ -65,6 +77,7 @@ struct Eval {
     int crit = GWM_CRIT_CV;
     int rc = GWM_OK;
     double value = DBL_MAX;
+    double cond = 0.0;   // conditioning indicator maximum (criterion evaluations)
     double shat[2] = {0.0, 0.0};
 };
 
@@ -92,19 +105,15 @@ struct Request {
     std::thread worker;
     bool posted = false;
     Eval ev;
-    // progress of the posted evaluation: phase 1 = selection, sweep and solves
-    // of focus points [0, next); phase 2 (fits) = qDiag sweep of points [0, next)
+    // progress of the posted evaluation (Phase): focus points, list entries or
+    // qDiag points [0, next) of the current phase are done
     int phase = 1, next = 0;
+    int nrefine = 0;             // focus points recomputed in FP64 (mixed)
+    double refine_crit = 100.0, refine_fit = 2.0;
     // host staging of fit outputs
     std::vector<double> hbetas, hse, hq;
 };
 
-// Data segments of a request in the batched sweeps: a function of n alone.
-void batch_segments(int n, int& nseg, int& seg_len) {
-    nseg = std::max(1, (n + 8191) / 8192);
-    seg_len = ((n + nseg - 1) / nseg + 63) / 64 * 64;
-    nseg = (n + seg_len - 1) / seg_len;
-}
 
 size_t footprint_bytes(int n, int k, int dist, int nseg) {
     const size_t rows = (size_t)(n + 63) / 64 * 64, np = (size_t)(n + 127) / 128 * 128;
@@ -151,6 +160,7 @@ struct gwm_hip_server {
     double* round_red = nullptr;
     int round_cap = 0;
     double round_ms = 25.0;      // work budget of a round; <= 0: whole evaluations
+    double refine_crit = 100.0, refine_fit = 2.0;   // FP64 recomputation limits (mixed)
 };
 
 namespace {
@@ -271,31 +281,37 @@ int launch_sweeps(gwm_hip_server* S, std::vector<SweepGroup>& gs, const char* de
     return GWM_OK;
 }
 
-// Work units per focus point of phase 1 (selection, sweep, solve) and per
-// point of phase 2 (qDiag sweep) of r's posted evaluation.
+// Work units per focus point of the main sweep (selection, sweep, solve), per
+// listed point of the FP64 recomputation, and per point of the qDiag sweep of
+// r's posted evaluation.
 double unit_cost(const Request* r, int phase) {
     const Eval& ev = r->ev;
-    const double pf = ev.precision == GWM_FP64 ? kFp64Factor : 1.0;
-    if (phase == 1) {
-        const double tiles = ev.type == kEvalCrit ? nta_of(r->k) : nta_of(r->k) + ntb_of(r->k);
+    const double tiles = ev.type == kEvalCrit ? nta_of(r->k) : nta_of(r->k) + ntb_of(r->k);
+    if (phase == kMain) {
+        const double pf = ev.precision == GWM_FP64 ? kFp64Factor : 1.0;
         return r->n * (tiles * pf + (r->adaptive ? kSelectUnits : 0.0));
     }
+    if (phase == kRefine) return r->n * tiles * kFp64Factor;
+    const double pf = ev.precision == GWM_FP64 ? kFp64Factor : 1.0;
     return (double)r->n * ntb_of(r->k) * pf;
 }
 
 double remaining_cost(const Request* r) {
-    if (r->phase == 1)
-        return (r->n - r->next) * unit_cost(r, 1) + (r->ev.type == kEvalFit ? r->n * unit_cost(r, 2) : 0.0);
-    return (r->n - r->next) * unit_cost(r, 2);
+    const double q = r->ev.type == kEvalFit ? r->n * unit_cost(r, kQDiag) : 0.0;
+    if (r->phase == kMain) return (r->n - r->next) * unit_cost(r, kMain) + q;
+    if (r->phase == kRefine) return (r->nrefine - r->next) * unit_cost(r, kRefine) + q;
+    return (r->n - r->next) * unit_cost(r, kQDiag);
 }
 
 // What one round does for one evaluation.
 struct Slice {
     Request* r = nullptr;
-    int a0 = 0, a1 = 0;   // phase-1 focus range
-    int q0 = 0, q1 = 0;   // phase-2 point range
-    bool split = false;   // phase 2 starts: the mixed path splits G into FP16 planes first
-    bool done = false;    // the evaluation completes in this round
+    int a0 = 0, a1 = 0;   // main-sweep focus range
+    int e0 = 0, e1 = 0;   // range of list entries recomputed in FP64
+    int q0 = 0, q1 = 0;   // qDiag point range
+    bool split = false;   // qDiag starts: the mixed path splits G into FP16 planes first
+    bool check = false;   // main sweep completes (mixed): count the ill-conditioned points
+    bool done = false;    // the evaluation completes in this round (after a check: if none)
     double cost = 0.0;
 };
 
@@ -303,38 +319,55 @@ struct Slice {
 Slice plan_slice(Request* r, double units, bool unlimited) {
     Slice sl;
     sl.r = r;
-    auto count = [&](int phase) {
-        const int left = r->n - r->next;
+    const bool fit = r->ev.type == kEvalFit;
+    auto count = [&](int phase, int total) {
+        const int left = total - r->next;
         if (unlimited) return left;
         const long long want = (long long)std::ceil(std::max(units, 0.0) / unit_cost(r, phase));
         const long long chunks = std::max<long long>(1, (want + kChunk - 1) / kChunk);
         return (int)std::min<long long>(left, chunks * kChunk);
     };
-    if (r->phase == 1) {
-        const int c = count(1);
-        sl.a0 = r->next;
-        sl.a1 = r->next + c;
-        sl.cost += c * unit_cost(r, 1);
-        units -= c * unit_cost(r, 1);
-        r->next = sl.a1;
-        if (r->next < r->n) return sl;
-        if (r->ev.type != kEvalFit) {
-            r->phase = 3;
+    auto take = [&](int phase, int total, int& lo, int& hi) {
+        const int c = count(phase, total);
+        lo = r->next;
+        hi = r->next + c;
+        sl.cost += c * unit_cost(r, phase);
+        units -= c * unit_cost(r, phase);
+        r->next = hi;
+        return r->next == total;
+    };
+    if (r->phase == kMain) {
+        if (!take(kMain, r->n, sl.a0, sl.a1)) return sl;
+        if (r->ev.precision == GWM_MIXED) {
+            // the refinement list is known after this round; a criterion
+            // completes now if it is empty
+            sl.check = true;
+            sl.done = !fit;
+            r->phase = kCheck;
+            return sl;
+        }
+        if (!fit) {
+            r->phase = kDone;
             sl.done = true;
             return sl;
         }
-        r->phase = 2;
+        r->phase = kQDiag;
+        r->next = 0;
+        if (!unlimited && units <= 0.0) return sl;
+    } else if (r->phase == kRefine) {
+        if (!take(kRefine, r->nrefine, sl.e0, sl.e1)) return sl;
+        if (!fit) {
+            r->phase = kDone;
+            sl.done = true;
+            return sl;
+        }
+        r->phase = kQDiag;
         r->next = 0;
         if (!unlimited && units <= 0.0) return sl;
     }
     sl.split = r->next == 0;
-    const int c = count(2);
-    sl.q0 = r->next;
-    sl.q1 = r->next + c;
-    sl.cost += c * unit_cost(r, 2);
-    r->next = sl.q1;
-    if (r->next == r->n) {
-        r->phase = 3;
+    if (take(kQDiag, r->n, sl.q0, sl.q1)) {
+        r->phase = kDone;
         sl.done = true;
     }
     return sl;
@@ -364,11 +397,12 @@ SolveArgs solve_job(Request* r, int PT, int colB, const double* bw_pt, int f0) {
     sa.gvec = c->gvec;
     sa.fail = r->dfail;
     sa.f0 = f0;
+    sa.cond = c->cond;
     return sa;
 }
 
 // Executes one round: a slice of every posted evaluation.  Evaluations that
-// complete get their results and phase 3.
+// complete get their results and phase kDone.
 int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
     const double t0 = now_ms();
     const int ne = (int)batch.size();
@@ -376,7 +410,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         if (S->round_red) hipFree(S->round_red);
         S->round_red = nullptr;
         S->round_cap = 0;
-        SRV_CHECK(S, hipMalloc((void**)&S->round_red, (size_t)ne * 2 * 4 * sizeof(double)));
+        SRV_CHECK(S, hipMalloc((void**)&S->round_red, (size_t)ne * 2 * 5 * sizeof(double)));
         S->round_cap = ne * 2;
     }
     // share the round's budget, shortest remaining work first
@@ -403,7 +437,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
     std::vector<SolveGroup> solves;
     std::vector<QdGroup> qds;
     std::vector<ReduceJob> reds;
-    std::vector<Request*> splits, finished;
+    std::vector<Request*> splits, finished, checks, refine_starts;
     for (const Slice& sl : slices) {
         Request* r = sl.r;
         const Eval& ev = r->ev;
@@ -411,24 +445,24 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         const int k = r->k, nta = nta_of(k), ntb = ntb_of(k);
         const int prec = ev.precision;
         const double* bw_pt = r->adaptive ? c->bwi : nullptr;
+        int mode, smode, PT, colB = 0;
+        if (ev.type == kEvalCrit) {
+            mode = ev.crit == GWM_CRIT_CV ? kModeCV : kModePlain;
+            smode = ev.crit == GWM_CRIT_CV ? kSolveCV : kSolveAIC;
+            PT = nta * 16;
+        } else {
+            mode = kModeHat;
+            smode = kSolveFit;
+            PT = (nta + ntb) * 16;
+            colB = nta * 16;
+        }
+        const int tb = mode == kModeHat ? ntb : 0;
         if (sl.a1 > sl.a0) {
             if (r->adaptive) {
                 const int d = r->dist;
                 sel[d].push_back(SelectJob{c->points(), ev.bw, c->bwi, sl.a0});
                 sel_start[d].push_back(sel_start[d].back() + sl.a1 - sl.a0);
             }
-            int mode, smode, PT, colB = 0;
-            if (ev.type == kEvalCrit) {
-                mode = ev.crit == GWM_CRIT_CV ? kModeCV : kModePlain;
-                smode = ev.crit == GWM_CRIT_CV ? kSolveCV : kSolveAIC;
-                PT = nta * 16;
-            } else {
-                mode = kModeHat;
-                smode = kSolveFit;
-                PT = (nta + ntb) * 16;
-                colB = nta * 16;
-            }
-            const int tb = mode == kModeHat ? ntb : 0;
             SweepGroup& g = group_of(sweeps, r->dist, mode, prec, nta, tb);
             g.jobs.push_back(sweep_job(r, mode, nta, tb, bw_pt, ev.bw));
             add_items(g, (int)g.jobs.size() - 1, sl.a0, sl.a1, r->nseg, contract_r(nta, tb, prec == GWM_MIXED));
@@ -437,6 +471,29 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
             sg.jobs.push_back(solve_job(r, PT, colB, bw_pt, sl.a0));
             sg.start.push_back(sg.start.back() + sl.a1 - sl.a0);
         }
+        if (sl.e1 > sl.e0) {
+            // FP64 recomputation of listed focus points: FP64 kernel, the
+            // request's own segments, compact partial-sum columns in part
+            const int npad = (r->nrefine + 127) / 128 * 128;
+            SweepGroup& g = group_of(sweeps, r->dist, mode, GWM_FP64, nta, tb);
+            SweepJob j = sweep_job(r, mode, nta, tb, bw_pt, ev.bw);
+            j.Z = c->Z;
+            j.focus_idx = c->rlist;
+            j.nfocus = r->nrefine;
+            j.n_pad = npad;
+            g.jobs.push_back(j);
+            add_items(g, (int)g.jobs.size() - 1, sl.e0, sl.e1, r->nseg, contract_r(nta, tb, 0));
+            SolveGroup& sg = sgroup_of(solves, k, smode);
+            if (sg.start.empty()) sg.start.push_back(0);
+            SolveArgs sa = solve_job(r, PT, colB, bw_pt, sl.e0);
+            sa.n_pad = npad;
+            sa.focus_idx = c->rlist;
+            sa.nfocus = r->nrefine;
+            sg.jobs.push_back(sa);
+            sg.start.push_back(sg.start.back() + sl.e1 - sl.e0);
+            if (sl.e0 == 0) refine_starts.push_back(r);
+        }
+        if (sl.check) checks.push_back(r);
         if (sl.split && prec == GWM_MIXED) splits.push_back(r);
         if (sl.q1 > sl.q0) {
             SweepGroup& q = group_of(qsweeps, r->dist, kModeQDiag, prec, ntb, 0);
@@ -452,12 +509,13 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         if (sl.done) {
             ReduceJob rj;
             rj.n = c->n;
-            rj.out = S->round_red + 4 * finished.size();
+            rj.out = S->round_red + 5 * finished.size();
             rj.fail = r->dfail;
             if (ev.type == kEvalCrit) {
                 rj.a = c->res2;
                 rj.b = ev.crit == GWM_CRIT_CV ? nullptr : c->sii;
                 rj.c = nullptr;
+                rj.m = c->cond;
             } else {
                 rj.a = c->sii;
                 rj.b = c->ssi;
@@ -499,7 +557,10 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
     const char* dev = S->arena.base;
     if (!T.host.empty())
         SRV_CHECK(S, hipMemcpyAsync(S->arena.base, T.host.data(), T.host.size(), hipMemcpyHostToDevice, S->stream));
-    // 1. adaptive bandwidths of the phase-1 focus points
+    // the failure flag of an FP64 recomputation is that of its own solves
+    // (every point the mixed solves failed on is listed)
+    for (Request* r : refine_starts) SRV_CHECK(S, hipMemsetAsync(r->dfail, 0, sizeof(int), S->stream));
+    // 1. adaptive bandwidths of the main-sweep focus points
     for (int d = 0; d < 2; ++d)
         if (!sel[d].empty())
             SRV_CHECK(S, launch_adaptive_bw_batch(d, reinterpret_cast<const SelectJob*>(dev + off_sel[d]),
@@ -513,6 +574,14 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         SRV_CHECK(S, launch_solve_batch(g.k, g.mode, reinterpret_cast<const SolveArgs*>(dev + g.off_jobs),
                                         reinterpret_cast<const int*>(dev + g.off_start), (int)g.jobs.size(),
                                         g.start.back(), S->stream));
+    // 3b. mixed evaluations whose main sweep completed: list the focus points
+    //     with an ill-conditioned local design
+    for (Request* r : checks) {
+        const double limit = r->ev.type == kEvalCrit ? r->refine_crit : r->refine_fit;
+        SRV_CHECK(S, hipMemsetAsync(r->ctx->rcount, 0, sizeof(int), S->stream));
+        SRV_CHECK(S, launch_compact(r->ctx->cond, r->n, limit, r->ctx->rlist, r->ctx->rcount, S->stream));
+        SRV_CHECK(S, hipMemcpyAsync(&r->nrefine, r->ctx->rcount, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    }
     // 4. qDiag second sweeps of the fits (G complete: every solve of the fit precedes on the stream)
     for (Request* r : splits)
         SRV_CHECK(S, launch_split_fp16(r->ctx->G, r->ctx->rows_pad, ntb_of(r->k) * 16, r->ctx->colmax, r->ctx->Gh,
@@ -524,7 +593,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
                                         g.start.back(), S->stream));
     // 5. reductions and results of the completed evaluations
     const int nd = (int)finished.size();
-    std::vector<double> red((size_t)nd * 4);
+    std::vector<double> red((size_t)nd * 5);
     if (nd) {
         SRV_CHECK(S, launch_reduce_batch(reinterpret_cast<const ReduceJob*>(dev + off_red), nd, S->stream));
         SRV_CHECK(S, hipMemcpyAsync(red.data(), S->round_red, red.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -541,25 +610,40 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         SRV_CHECK(S, hipMemcpyAsync(r->hq.data(), r->ctx->qd, (size_t)r->n * 8, hipMemcpyDeviceToHost, S->stream));
     }
     SRV_CHECK(S, hipStreamSynchronize(S->stream));
+    // checked evaluations: recompute the listed points next, or go on
+    for (Request* r : checks) {
+        if (r->nrefine > 0) {
+            r->phase = kRefine;
+            r->next = 0;
+        } else if (r->ev.type == kEvalCrit) {
+            r->phase = kDone;
+        } else {
+            r->phase = kQDiag;
+            r->next = 0;
+        }
+    }
     long long nsolves = 0;
     for (int i = 0; i < nd; ++i) {
         Request* r = finished[i];
+        if (r->phase != kDone) continue;   // a criterion whose check listed points
         Eval& ev = r->ev;
         nsolves += r->n;
         r->solves += r->n;
-        if (red[4 * i + 3] != 0.0) {
+        const double* o = &red[5 * i];
+        if (o[3] != 0.0) {
             ev.rc = GWM_ERR_NOT_SPD;
             ev.value = DBL_MAX;
             continue;
         }
         if (ev.type == kEvalCrit) {
-            double v = ev.crit == GWM_CRIT_CV ? red[4 * i] : aicc(red[4 * i], (double)r->n, red[4 * i + 1]);
+            double v = ev.crit == GWM_CRIT_CV ? o[0] : aicc(o[0], (double)r->n, o[1]);
             ev.rc = std::isfinite(v) ? GWM_OK : GWM_ERR_NOT_SPD;
             ev.value = std::isfinite(v) ? v : DBL_MAX;
+            ev.cond = o[4];
         } else {
             ev.rc = GWM_OK;
-            ev.shat[0] = red[4 * i];
-            ev.shat[1] = red[4 * i + 1];
+            ev.shat[0] = o[0];
+            ev.shat[1] = o[1];
         }
     }
     S->rounds += 1;
@@ -570,34 +654,41 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
 
 // Worker: the host algorithm of one request.
 void worker_main(gwm_hip_server* S, Request* R) {
-    auto post = [&](int type, double bw, int prec, int crit, double* value) -> int {
+    auto post = [&](int type, double bw, int prec, int crit, double* value, double* cond) -> int {
         std::unique_lock<std::mutex> lk(S->mu);
         R->ev = Eval();
         R->ev.type = type;
         R->ev.bw = bw;
         R->ev.precision = prec;
         R->ev.crit = crit;
-        R->phase = 1;
+        R->phase = kMain;
         R->next = 0;
+        R->nrefine = 0;
         R->posted = true;
         S->cv.notify_all();
         S->cv.wait(lk, [&] { return !R->posted; });
         if (value) *value = R->ev.value;
+        if (cond) *cond = R->ev.cond;
         return R->ev.rc;
     };
     int rc = GWM_OK;
     double bw = R->bw;
+    double certify_rel;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        certify_rel = S->certify_rel;
+    }
     if (R->select_bw) {
         double lo = R->has_lower ? R->lower : (R->adaptive ? 20 : 0.0);
         double hi = R->has_upper ? R->upper : (R->adaptive ? (double)R->n : R->maxdist);
         std::map<double, double> log;
         rc = golden(R->adaptive != 0, lo, hi,
                     [&](double x, bool exact, double* fx) {
-                        return post(kEvalCrit, x, exact ? GWM_FP64 : S->precision, R->crit, fx);
+                        return post(kEvalCrit, x, exact ? GWM_FP64 : S->precision, R->crit, fx, nullptr);
                     },
-                    S->precision == GWM_MIXED, S->certify_rel, &bw, log);
+                    S->precision == GWM_MIXED, certify_rel, &bw, log);
     }
-    if (rc == GWM_OK) rc = post(kEvalFit, bw, S->precision, 0, nullptr);
+    if (rc == GWM_OK) rc = post(kEvalFit, bw, S->precision, 0, nullptr, nullptr);
     if (rc == GWM_OK) {
         double shat[2] = {R->ev.shat[0], R->ev.shat[1]};
         double diag[7];
@@ -710,7 +801,7 @@ void scheduler_main(gwm_hip_server* S) {
             if (rc) {
                 R->ev.rc = rc;
                 R->ev.value = DBL_MAX;
-            } else if (R->phase != 3) {
+            } else if (R->phase != kDone) {
                 continue;   // in progress: more slices next round
             }
             R->posted = false;
@@ -751,6 +842,14 @@ int gwm_hip_server_set_certify_tolerance(gwm_hip_server* S, double rel) {
     return GWM_OK;
 }
 
+int gwm_hip_server_set_refine_condition(gwm_hip_server* S, double criterion_limit, double fit_limit) {
+    if (!S || std::isnan(criterion_limit) || std::isnan(fit_limit)) return GWM_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->refine_crit = criterion_limit;
+    S->refine_fit = fit_limit;
+    return GWM_OK;
+}
+
 int gwm_hip_server_set_round_ms(gwm_hip_server* S, double ms) {
     if (!S || std::isnan(ms)) return GWM_ERR_INVALID;
     std::lock_guard<std::mutex> lk(S->mu);
@@ -788,9 +887,11 @@ int gwm_hip_server_submit(gwm_hip_server* S, int n, int k, const double* coords,
     R->o_qdiag = qDiag;
     R->o_diag = diag;
     R->o_bw = bw_out;
-    batch_segments(n, R->nseg, R->seg_len);
+    data_segments(n, R->nseg, R->seg_len);
     R->footprint = footprint_bytes(n, k, distance, R->nseg);
     std::lock_guard<std::mutex> lk(S->mu);
+    R->refine_crit = S->refine_crit;
+    R->refine_fit = S->refine_fit;
     R->id = S->next_id++;
     R->t_submit = now_ms();
     S->reqs[R->id] = R;

@@ -30,6 +30,10 @@ struct SweepJob {
     int nseg;
     int nta, ntb;
     double* out;             // [nseg][(nta+ntb)*16][n_pad]
+    // optional focus list: output column t holds focus focus_idx[t],
+    // t < nfocus (n_pad then pads nfocus); null = every point in order
+    const int* focus_idx = nullptr;
+    int nfocus = 0;
 };
 
 // Launch of one sweep: either a single job (items == nullptr; workgroup b
@@ -89,6 +93,11 @@ struct SolveArgs {
     double* gvec;          // row-major [n][k] g = A^-1 x_i (fit)
     int* fail;             // set to 1 on a non-SPD local design
     int f0 = 0;            // first focus of a batched slice
+    double* cond = nullptr;  // per focus: max_a A_aa / L_aa^2 (Jacobi-scaled inverse pivot), or null
+    // optional focus list (as in SweepJob): entry t solves focus focus_idx[t]
+    // from partial-sum column t
+    const int* focus_idx = nullptr;
+    int nfocus = 0;
 };
 
 hipError_t launch_solve(int k, int mode, const SolveArgs& a, hipStream_t s);
@@ -113,8 +122,9 @@ struct ReduceJob {
     const double* b;
     const double* c;
     int n;
-    double* out;   // 3 sums, then the failure flag
+    double* out;   // 3 sums, the failure flag, max of m
     int* fail;     // copied to out[3] and cleared; may be null
+    const double* m = nullptr;   // out[4] = max over n (0 when null)
 };
 hipError_t launch_reduce_batch(const ReduceJob* jobs, int njobs, hipStream_t s);
 
@@ -137,8 +147,15 @@ hipError_t launch_qdiag(int k, const double* part, int nseg, int PT, int n, int 
 hipError_t launch_smatrix(int dist, int k, const Points& P, const double* X, const double* gvec,
                           const double* bw_pt, double bw, int kernel, double* S, hipStream_t s);
 
-// Deterministic sums of up to 3 vectors of length n into out[0..m).
-hipError_t launch_reduce(const double* a, const double* b, const double* c, int n, double* out, hipStream_t s);
+// Focus points whose conditioning indicator exceeds limit (or is not finite):
+// their indices go to list[] (unordered), their number to *count, which must
+// be zero beforehand.
+hipError_t launch_compact(const double* cond, int n, double limit, int* list, int* count, hipStream_t s);
+
+// Deterministic sums of up to 3 vectors of length n into out[0..2], and the
+// maximum of m into out[3] (0 when m is null).
+hipError_t launch_reduce(const double* a, const double* b, const double* c, const double* m, int n, double* out,
+                         hipStream_t s);
 
 // Single-column local solves for the multiscale path (GWRMultiscale.cpp:420-482,
 // 560-636): part columns 0 = sum w x^2, 1 = sum w x y, colB = sum w^2 x^2.

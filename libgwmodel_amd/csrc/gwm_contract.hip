@@ -111,6 +111,8 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
     const int wave = threadIdx.x >> 6;
     const int fbase = tile * (kWaves * 16 * R) + wave * 16 * R;
     const int n = job.n;
+    GWM_GLOBAL const int* focus_idx = gptr(job.focus_idx);
+    const int nf = focus_idx ? job.nfocus : n;   // focus points of this job
 
     // focus points owned by this lane (one per A tile)
     FocusCoords<DIST> fc[R];
@@ -119,8 +121,9 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int f = fbase + 16 * r + (lane & 15);
-        fidx[r] = f;
-        int fl = f < n ? f : n - 1;
+        int fl = f < nf ? f : nf - 1;
+        if (focus_idx) fl = focus_idx[fl];
+        fidx[r] = focus_idx ? (f < nf ? fl : -1) : f;   // self index for the CV exclusion
         fc[r] = load_focus<DIST>(P, fl);
         fbw[r] = make_kconst64((MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : job.bw);
     }
@@ -227,7 +230,8 @@ hipError_t launch_f64_r(const SweepLaunch& a, hipStream_t s) {
     if (a.items) {
         grid = a.nitems;
     } else {
-        int tiles = (a.job0.n + kWaves * 16 * R - 1) / (kWaves * 16 * R);
+        const int nf = a.job0.focus_idx ? a.job0.nfocus : a.job0.n;
+        int tiles = (nf + kWaves * 16 * R - 1) / (kWaves * 16 * R);
         grid = tiles * a.job0.nseg;
     }
     if (grid <= 0) return hipSuccess;

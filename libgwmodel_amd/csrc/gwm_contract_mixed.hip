@@ -8,8 +8,11 @@
 //   - w and Z split into FP16 hi + lo parts, lo scaled by 2^11 so small
 //     values stay normal: w = wh + wl * 2^-11, z = zh + zl * 2^-11 (each part
 //     carries 11 significant bits, the pair ~22);
-//   - w z ~= wh zh + 2^-11 (wh zl + wl zh) on v_mfma_f32_16x16x32_f16 with
-//     FP32 accumulation (the dropped wl zl term is below 2^-24 relative);
+//   - w z = wh zh + 2^-11 (wh zl + wl zh) + 2^-22 wl zl on
+//     v_mfma_f32_16x16x32_f16 with FP32 accumulation, the three scales in
+//     separate accumulators (the wl zl term, about 2^-22 relative, roughly
+//     halves the error of the betas for 1/4 more matrix-core work, which the
+//     VALU-bound producer hides);
 //   - every 64 data points the FP32 accumulators are flushed into FP64 ones,
 //     so FP32 rounding only ever acts on 64-term partial sums;
 //   - Z columns carry a power-of-two scale (exact) that keeps them inside the
@@ -206,6 +209,8 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     const int wave = threadIdx.x >> 6;
     const int fbase = tile * (kWaves * 16 * R) + wave * 16 * R;
     const int n = args.n;
+    GWM_GLOBAL const int* focus_idx = gptr(args.focus_idx);
+    const int nf = focus_idx ? args.nfocus : n;   // focus points of this job
     const int kq = lane >> 4;       // which 8-row group of the 32-row step
     const int col = lane & 15;
 
@@ -215,15 +220,16 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int f = fbase + 16 * r + col;
-        fidx[r] = f;
-        int fl = f < n ? f : n - 1;
+        int fl = f < nf ? f : nf - 1;
+        if (focus_idx) fl = focus_idx[fl];
+        fidx[r] = focus_idx ? (f < nf ? fl : -1) : f;   // self index for the CV exclusion
         fc[r] = load_mfocus<DIST>(P, fl);
         double b = (MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : args.bw;
         if (DIST == kEuclidean) fbw[r] = make_bw((float)b);
         else fbw[r] = make_bw((float)b);
     }
 
-    v4f hh[R][NTA + NTB], xx[R][NTA + NTB];
+    v4f hh[R][NTA + NTB], xx[R][NTA + NTB], ll[R][NTA + NTB];
     v4d acc[R][NTA + NTB];
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -270,6 +276,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
             for (int t = 0; t < NTA + NTB; ++t) {
                 hh[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
                 xx[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
+                ll[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
             }
         const int nsteps = (jend - j0) > 32 ? 2 : 1;
         for (int s = 0; s < nsteps; ++s) {
@@ -329,6 +336,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                     hh[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[t], hh[r][t], 0, 0, 0);
                     xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[t], xx[r][t], 0, 0, 0);
                     xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], xx[r][t], 0, 0, 0);
+                    ll[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bl[t], ll[r][t], 0, 0, 0);
                 }
 #pragma unroll
                 for (int t = 0; t < NTB; ++t) {
@@ -336,6 +344,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                     hh[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bh[t], hh[r][NTA + t], 0, 0, 0);
                     xx[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bl[t], xx[r][NTA + t], 0, 0, 0);
                     xx[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bh[t], xx[r][NTA + t], 0, 0, 0);
+                    ll[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bl[t], ll[r][NTA + t], 0, 0, 0);
                 }
             }
         }
@@ -346,7 +355,8 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
             for (int t = 0; t < NTA + NTB; ++t)
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    acc[r][t][q] += (double)hh[r][t][q] + (double)xx[r][t][q] * (1.0 / 2048.0);
+                    acc[r][t][q] += (double)hh[r][t][q] + (double)xx[r][t][q] * (1.0 / 2048.0) +
+                                    (double)ll[r][t][q] * (1.0 / 4194304.0);
     }
 
     // D layout of v_mfma_f32_16x16x32_f16 (checked on gfx950): lane holds
@@ -377,7 +387,8 @@ hipError_t launch_mixed_r(const SweepLaunch& a, hipStream_t s) {
     if (a.items) {
         grid = a.nitems;
     } else {
-        int tiles = (a.job0.n + kWaves * 16 * R - 1) / (kWaves * 16 * R);
+        const int nf = a.job0.focus_idx ? a.job0.nfocus : a.job0.n;
+        int tiles = (nf + kWaves * 16 * R - 1) / (kWaves * 16 * R);
         grid = tiles * a.job0.nseg;
     }
     if (grid <= 0) return hipSuccess;

@@ -40,7 +40,7 @@ struct gwm_hip_ctx {
     double *X = nullptr, *y = nullptr, *Z = nullptr, *G = nullptr;
     double* part = nullptr;
     size_t part_bytes = 0;
-    double *bwi = nullptr, *res2 = nullptr, *sii = nullptr, *ssi = nullptr;
+    double *bwi = nullptr, *res2 = nullptr, *sii = nullptr, *ssi = nullptr, *cond = nullptr;
     double *betas = nullptr, *se = nullptr, *gvec = nullptr, *qd = nullptr, *red = nullptr, *blockmax = nullptr;
     double *vx = nullptr, *vy = nullptr, *vZ = nullptr;
     double* S = nullptr;
@@ -51,6 +51,16 @@ struct gwm_hip_ctx {
     double *zscale = nullptr, *gscale = nullptr, *vscale = nullptr;
     unsigned long long* colmax = nullptr;
     double certify_rel = 1e-7;   // relative bound used to certify mixed comparisons
+    // Mixed precision: focus points whose local design has a conditioning
+    // indicator above the limit (criterion evaluations / fits) are recomputed
+    // in FP64 (gwm_hip_set_refine_condition).
+    double refine_crit = 100.0, refine_fit = 2.0;
+    double last_cond = 0.0;      // indicator maximum of the last mixed evaluation
+    long long last_refined = 0;  // focus points it recomputed in FP64
+    int* rlist = nullptr;        // [n] focus list of the refinement
+    int* rcount = nullptr;
+    double* rpart = nullptr;     // partial sums of the refinement sweep
+    size_t rpart_bytes = 0;
 
     // timing
     bool timing = false;
@@ -75,6 +85,16 @@ struct gwm_hip_ctx {
 namespace gwm {
 namespace detail {
 
+// Data segments of a problem's sweeps: a function of n alone, so that every
+// path (single problem, batched, mixed, FP64, FP64 recomputation) sums a focus
+// point's partial sums over the same segments in the same order.  Up to 16
+// segments (consecutive workgroups take consecutive segments, so each of the
+// 8 XCDs streams two slices of the table), each 64..8192 points.
+inline void data_segments(int n, int& nseg, int& seg_len) {
+    seg_len = ((n + 15) / 16 + 63) / 64 * 64;
+    seg_len = seg_len < 64 ? 64 : (seg_len > 8192 ? 8192 : seg_len);
+    nseg = (n + seg_len - 1) / seg_len;
+}
 int nta_of(int k);
 int ntb_of(int k);
 double aicc(double rss, double n, double trS);
@@ -86,6 +106,7 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
            std::map<double, double>& log);
 void write_trace(const std::map<double, double>& log, double* tx, double* tf, int cap, int* len);
 int max_distance_impl(gwm_hip_ctx* c, double* out);
+
 
 }  // namespace detail
 }  // namespace gwm

@@ -5,11 +5,16 @@
 // floor(bw)-th and floor(bw)+1-th smallest (the focus itself has rank 1):
 //     fixbw = d0 + (d1 - d0) * (bw - floor(bw)),
 // or uses (bw/n) * max(dist) when bw >= n.  Here the two order statistics are
-// found without sorting, by an MSD radix select (8-bit digits, 8 passes) over
-// the IEEE-754 bit patterns of the non-negative selection values, which order
-// like the values themselves.  One workgroup per focus point; the histogram
-// lives in LDS.  The result is bit-identical to the sort because the selected
-// values are the very doubles the sort would return.
+// found without sorting, by an MSD radix select over the IEEE-754 bit patterns
+// of the non-negative selection values, which order like the values
+// themselves.  One workgroup per focus point; the histogram lives in LDS.
+// Digits are 12 bits (sign + exponent, then the top mantissa bits), then 8.
+// Passes stream the n distances (recomputed each time) until the bin holding
+// the wanted rank has at most kCap keys; one more pass gathers that bin into
+// LDS (and the smallest key above it), and the remaining digits run on the
+// LDS copy.  Typically 2-3 streaming passes instead of one per byte.  The
+// result is bit-identical to the sort because the selected values are the
+// very doubles the sort would return.
 #include "gwm_contract.h"
 
 namespace gwm {
@@ -19,15 +24,19 @@ namespace {
 __device__ __forceinline__ unsigned long long key_of(double v) { return (unsigned long long)__double_as_longlong(v); }
 __device__ __forceinline__ double val_of(unsigned long long k) { return __longlong_as_double((long long)k); }
 
+constexpr int kSelBins = 4096;   // 12-bit digits
+constexpr int kSelCap = 2048;    // keys gathered into LDS
+
 template <int DIST, class PTS, class OUT>
 __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out, int f) {
     const int n = P.n;
     const int tid = threadIdx.x;
-    __shared__ unsigned int hist[256];
-    __shared__ unsigned long long s_prefix;
+    __shared__ unsigned int hist[kSelBins];
+    __shared__ unsigned int chunk_sum[256];
+    __shared__ unsigned long long cand[kSelCap];
+    __shared__ unsigned long long s_prefix, s_above, s_next;
     __shared__ long long s_rank;
-    __shared__ unsigned int s_equal;
-    __shared__ unsigned long long s_next;
+    __shared__ unsigned int s_count, s_ncand;
     __shared__ double s_red[256];
 
     const double dn = bw / n;
@@ -49,52 +58,106 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
         s_prefix = 0ull;
         s_rank = (long long)b0 - 1;   // 0-based rank of d0
     }
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = 56 - 8 * pass;
-        hist[tid] = 0u;
+    int shift = 64;
+    bool in_lds = false;
+    int ncand = 0;
+    while (shift > 0) {
+        const int width = shift > 40 ? 12 : 8;   // 12, 12, then 8 x 5
+        shift -= width;
+        const int nb = 1 << width;
+        const int hs = shift + width;           // bits above this digit (64: none)
+        for (int i = tid; i < nb; i += 256) hist[i] = 0u;
         __syncthreads();
         const unsigned long long prefix = s_prefix;
-        for (int j = tid; j < n; j += 256) {
-            unsigned long long key = key_of(select_value<DIST>(P, f, j));
-            bool match = pass == 0 || (key >> (shift + 8)) == (prefix >> (shift + 8));
-            if (match) atomicAdd(&hist[(key >> shift) & 255ull], 1u);
+        const unsigned long long mask = (unsigned long long)(nb - 1);
+        if (in_lds) {
+            for (int i = tid; i < ncand; i += 256) {
+                const unsigned long long key = cand[i];
+                if ((key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & mask], 1u);
+            }
+        } else {
+            for (int j = tid; j < n; j += 256) {
+                const unsigned long long key = key_of(select_value<DIST>(P, f, j));
+                if (hs == 64 || (key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & mask], 1u);
+            }
+        }
+        __syncthreads();
+        // the bin holding rank s_rank: per-thread chunk sums, then a scan by thread 0
+        const int per = nb / 256;
+        {
+            unsigned int sum = 0u;
+            for (int i = 0; i < per; ++i) sum += hist[tid * per + i];
+            chunk_sum[tid] = sum;
         }
         __syncthreads();
         if (tid == 0) {
             long long rank = s_rank;
-            int bin = 0;
-            for (; bin < 255; ++bin) {
+            int c = 0;
+            for (; c < 255; ++c) {
+                if (rank < (long long)chunk_sum[c]) break;
+                rank -= chunk_sum[c];
+            }
+            int bin = c * per;
+            for (int i = 0; i < per - 1; ++i, ++bin) {
                 if (rank < (long long)hist[bin]) break;
                 rank -= hist[bin];
             }
             s_prefix = prefix | ((unsigned long long)bin << shift);
             s_rank = rank;
-            s_equal = hist[bin];
+            s_count = hist[bin];
         }
         __syncthreads();
+        if (!in_lds && shift > 0 && s_count <= (unsigned int)kSelCap) {
+            // gather the bin (its keys share the top 64 - shift bits) and the
+            // smallest key above it
+            if (tid == 0) {
+                s_ncand = 0u;
+                s_above = ~0ull;
+            }
+            __syncthreads();
+            const unsigned long long top = s_prefix >> shift;
+            unsigned long long above = ~0ull;
+            for (int j = tid; j < n; j += 256) {
+                const unsigned long long key = key_of(select_value<DIST>(P, f, j));
+                const unsigned long long kt = key >> shift;
+                if (kt == top) cand[atomicAdd(&s_ncand, 1u)] = key;
+                else if (kt > top && key < above) above = key;
+            }
+            atomicMin(&s_above, above);
+            __syncthreads();
+            ncand = (int)s_ncand;
+            in_lds = true;
+        }
     }
+    // all 64 bits resolved: key0 = s_prefix, s_count keys equal it
     const unsigned long long key0 = s_prefix;
-    // second order statistic: equal to d0 when ties cover rank b0, else the
-    // smallest value above d0
-    const bool tie = (s_rank + 1) < (long long)s_equal;
+    const bool tie = (s_rank + 1) < (long long)s_count;
     double d1v;
     if (tie) {
         d1v = val_of(key0);
     } else {
-        if (tid == 0) s_next = ~0ull;
+        // second order statistic: the smallest key above key0
+        if (tid == 0) s_next = in_lds ? s_above : ~0ull;
         __syncthreads();
         unsigned long long best = ~0ull;
-        for (int j = tid; j < n; j += 256) {
-            unsigned long long key = key_of(select_value<DIST>(P, f, j));
-            if (key > key0 && key < best) best = key;
+        if (in_lds) {
+            for (int i = tid; i < ncand; i += 256) {
+                const unsigned long long key = cand[i];
+                if (key > key0 && key < best) best = key;
+            }
+        } else {
+            for (int j = tid; j < n; j += 256) {
+                const unsigned long long key = key_of(select_value<DIST>(P, f, j));
+                if (key > key0 && key < best) best = key;
+            }
         }
         atomicMin(&s_next, best);
         __syncthreads();
         d1v = val_of(s_next);
     }
     if (tid == 0) {
-        double d0 = select_to_distance<DIST>(val_of(key0));
-        double d1 = select_to_distance<DIST>(d1v);
+        const double d0 = select_to_distance<DIST>(val_of(key0));
+        const double d1 = select_to_distance<DIST>(d1v);
         out[f] = d0 + (d1 - d0) * bx;
     }
 }

@@ -89,7 +89,9 @@ __device__ __forceinline__ double quad_form(const double* Bp, const double* g) {
 }
 
 template <int K, int MODE>
-__device__ __forceinline__ void solve_one(const SolveArgs& a, int i) {
+__device__ __forceinline__ void solve_one(const SolveArgs& a, int t) {
+    // partial sums in column t; the focus is t itself or focus_idx[t]
+    const int i = a.focus_idx ? a.focus_idx[t] : t;
     const int n = a.n;
     constexpr int M = Sym<K>::M;
 
@@ -97,9 +99,9 @@ __device__ __forceinline__ void solve_one(const SolveArgs& a, int i) {
 #pragma unroll
     for (int r = 0; r < K; ++r)
 #pragma unroll
-        for (int c = 0; c <= r; ++c) A[Sym<K>::low(r, c)] = sum_parts<K>(a, Sym<K>::zcol(c, r), i);
+        for (int c = 0; c <= r; ++c) A[Sym<K>::low(r, c)] = sum_parts<K>(a, Sym<K>::zcol(c, r), t);
 #pragma unroll
-    for (int r = 0; r < K; ++r) b[r] = sum_parts<K>(a, M + r, i);
+    for (int r = 0; r < K; ++r) b[r] = sum_parts<K>(a, M + r, t);
 #pragma unroll
     for (int r = 0; r < K; ++r) xi[r] = a.X[i + (size_t)r * n];
 
@@ -107,7 +109,20 @@ __device__ __forceinline__ void solve_one(const SolveArgs& a, int i) {
         *a.fail = 1;
         a.res2[i] = NAN;
         if (MODE == kSolveAIC || MODE == kSolveFit) a.sii[i] = NAN;
+        if (a.cond) a.cond[i] = INFINITY;
         return;
+    }
+    if (a.cond) {
+        // conditioning indicator of the Jacobi-scaled design: the inverse of
+        // its smallest squared Cholesky pivot (>= 1; large when a column is
+        // nearly a combination of the earlier ones)
+        double c = 1.0;
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+            const double l = L[Sym<K>::low(p, p)];
+            c = fmax(c, A[Sym<K>::low(p, p)] / (l * l));
+        }
+        a.cond[i] = c;
     }
     chol_solve<K>(L, b, beta);
     {
@@ -152,7 +167,7 @@ __device__ __forceinline__ void solve_one(const SolveArgs& a, int i) {
 #pragma unroll
     for (int r = 0; r < K; ++r)
 #pragma unroll
-        for (int c = 0; c <= r; ++c) B[Sym<K>::low(r, c)] = sum_parts<K>(a, a.colB + Sym<K>::zcol(c, r), i);
+        for (int c = 0; c <= r; ++c) B[Sym<K>::low(r, c)] = sum_parts<K>(a, a.colB + Sym<K>::zcol(c, r), t);
     a.ssi[i] = quad_form<K>(B, g);
 #pragma unroll
     for (int p = 0; p < K; ++p) a.betas[i + (size_t)p * n] = beta[p];
@@ -175,9 +190,9 @@ __device__ __forceinline__ void solve_one(const SolveArgs& a, int i) {
 
 template <int K, int MODE>
 __global__ __launch_bounds__(128) void solve_kernel(SolveArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    solve_one<K, MODE>(a, i);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (a.focus_idx ? a.nfocus : a.n)) return;
+    solve_one<K, MODE>(a, t);
 }
 
 // Batched solves: thread t of the launch belongs to the job whose focus range
@@ -238,25 +253,37 @@ __global__ void smatrix_kernel(Points P, int k, const double* X, const double* g
     S[i + (size_t)j * n] = w * s;
 }
 
-// Deterministic fixed-order sums (one workgroup).
-__global__ __launch_bounds__(1024) void reduce3_kernel(const double* a, const double* b, const double* c, int n,
-                                                       double* out) {
-    __shared__ double sa[1024], sb[1024], sc[1024];
-    double xa = 0.0, xb = 0.0, xc = 0.0;
+// Indices of the focus points whose indicator exceeds the limit.
+__global__ __launch_bounds__(256) void compact_kernel(const double* cond, int n, double limit, int* list, int* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double c = cond[i];
+    if (!(c <= limit)) list[atomicAdd(count, 1)] = i;
+}
+
+// Deterministic fixed-order sums of a, b, c and the maximum of m (one
+// workgroup): out[0..2] = sums, *mx = max (0 when m is null).
+__device__ __forceinline__ void reduce_body(const double* a, const double* b, const double* c, const double* m, int n,
+                                           double* out, double* mx) {
+    __shared__ double sa[1024], sb[1024], sc[1024], sm[1024];
+    double xa = 0.0, xb = 0.0, xc = 0.0, xm = 0.0;
     for (int i = threadIdx.x; i < n; i += 1024) {
         if (a) xa += a[i];
         if (b) xb += b[i];
         if (c) xc += c[i];
+        if (m) xm = fmax(xm, m[i]);
     }
     sa[threadIdx.x] = xa;
     sb[threadIdx.x] = xb;
     sc[threadIdx.x] = xc;
+    sm[threadIdx.x] = xm;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
         if (threadIdx.x < w) {
             sa[threadIdx.x] += sa[threadIdx.x + w];
             sb[threadIdx.x] += sb[threadIdx.x + w];
             sc[threadIdx.x] += sc[threadIdx.x + w];
+            sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
         }
         __syncthreads();
     }
@@ -264,40 +291,24 @@ __global__ __launch_bounds__(1024) void reduce3_kernel(const double* a, const do
         out[0] = sa[0];
         out[1] = sb[0];
         out[2] = sc[0];
+        *mx = sm[0];
     }
 }
 
+__global__ __launch_bounds__(1024) void reduce3_kernel(const double* a, const double* b, const double* c,
+                                                       const double* m, int n, double* out) {
+    reduce_body(a, b, c, m, n, out, out + 3);
+}
+
 // Batched fixed-order sums: block b reduces job b's vectors (same order as a
-// one-problem reduce3 launch).
+// one-problem reduce3 launch); out[3] receives the job's failure flag, which
+// is cleared for its next evaluation.
 __global__ __launch_bounds__(1024) void reduce_batch_kernel(const ReduceJob* jobs) {
     const ReduceJob& J = jobs[blockIdx.x];
-    __shared__ double sa[1024], sb[1024], sc[1024];
-    double xa = 0.0, xb = 0.0, xc = 0.0;
-    for (int i = threadIdx.x; i < J.n; i += 1024) {
-        if (J.a) xa += J.a[i];
-        if (J.b) xb += J.b[i];
-        if (J.c) xc += J.c[i];
-    }
-    sa[threadIdx.x] = xa;
-    sb[threadIdx.x] = xb;
-    sc[threadIdx.x] = xc;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            sa[threadIdx.x] += sa[threadIdx.x + w];
-            sb[threadIdx.x] += sb[threadIdx.x + w];
-            sc[threadIdx.x] += sc[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        J.out[0] = sa[0];
-        J.out[1] = sb[0];
-        J.out[2] = sc[0];
-        if (J.fail) {
-            J.out[3] = (double)*J.fail;
-            *J.fail = 0;
-        }
+    reduce_body(J.a, J.b, J.c, J.m, J.n, J.out, J.out + 4);
+    if (threadIdx.x == 0 && J.fail) {
+        J.out[3] = (double)*J.fail;
+        *J.fail = 0;
     }
 }
 
@@ -332,9 +343,12 @@ __global__ __launch_bounds__(256) void solve_var_kernel(int mode, const double* 
     }
 }
 
+
 template <int K>
 hipError_t launch_solve_k(int mode, const SolveArgs& a, hipStream_t s) {
-    dim3 grid((a.n + 127) / 128);
+    const int count = a.focus_idx ? a.nfocus : a.n;
+    if (count <= 0) return hipSuccess;
+    dim3 grid((count + 127) / 128);
     if (mode == kSolveCV) hipLaunchKernelGGL((solve_kernel<K, kSolveCV>), grid, dim3(128), 0, s, a);
     else if (mode == kSolveAIC) hipLaunchKernelGGL((solve_kernel<K, kSolveAIC>), grid, dim3(128), 0, s, a);
     else if (mode == kSolveBeta) hipLaunchKernelGGL((solve_kernel<K, kSolveBeta>), grid, dim3(128), 0, s, a);
@@ -422,8 +436,15 @@ hipError_t launch_smatrix(int dist, int k, const Points& P, const double* X, con
     return hipGetLastError();
 }
 
-hipError_t launch_reduce(const double* a, const double* b, const double* c, int n, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(reduce3_kernel, dim3(1), dim3(1024), 0, s, a, b, c, n, out);
+hipError_t launch_compact(const double* cond, int n, double limit, int* list, int* count, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(compact_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cond, n, limit, list, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double* a, const double* b, const double* c, const double* m, int n, double* out,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(reduce3_kernel, dim3(1), dim3(1024), 0, s, a, b, c, m, n, out);
     return hipGetLastError();
 }
 

@@ -1,0 +1,117 @@
+"""FP64 recomputation of ill-conditioned local designs in mixed precision.
+
+Mixed-precision local solves lose accuracy in proportion to the conditioning
+of the local design.  Every mixed evaluation lists the focus points whose
+conditioning indicator (inverse smallest squared Cholesky pivot of the
+Jacobi-scaled X^T W X) exceeds a limit and recomputes them with the FP64
+kernel and the FP64 path's data segments.
+
+- With a limit of 0 every point is recomputed, so criteria, betas and the hat
+  traces equal the FP64 path's bit for bit.
+- With the default limits, bandwidth searches on problems whose small fixed
+  bisquare bandwidths leave some local designs nearly singular select the FP64
+  bandwidth, and the fit's betas stay within 1e-6 of the FP64 betas.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from libgwmodel_amd import capi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def problem(seed, n, k):
+    rng = np.random.default_rng(seed)
+    coords = rng.uniform(0, 100, (n, 2))
+    X = np.column_stack([np.ones(n)] + [rng.normal(size=n) for _ in range(k - 1)])
+    beta = np.column_stack([1 + coords[:, 0] / 50] + [np.sin(coords[:, 1] / (10 + 5 * c)) for c in range(k - 1)])
+    y = np.sum(X * beta, axis=1) + rng.normal(0, 0.3, n)
+    return coords, X, y
+
+
+def col_err(b, ref):
+    scale = np.maximum(np.max(np.abs(ref), axis=0), 1e-300)
+    return float(np.max(np.max(np.abs(b - ref), axis=0) / scale))
+
+
+# Problems of the 512-problem batch workload (fixed bisquare bandwidths) whose
+# criterion landscape has nearly singular local designs near the optimum.
+HARD = (209, 221, 254)
+
+
+@pytest.fixture(scope="module")
+def hard_problems():
+    probs = synth.config4(max(HARD) + 1)
+    return [probs[i] for i in HARD]
+
+
+def test_refine_everything_reproduces_fp64():
+    say("case: refine limit 0, n=900 k=5 fixed bisquare 15")
+    coords, X, y = problem(11, 900, 5)
+    with capi.Context(0) as ctx:
+        ctx.set_data(coords, X, y)
+        ctx.set_refine_condition(0.0, 0.0)
+        for crit in (capi.CRIT_CV, capi.CRIT_AIC):
+            for bw in (12.0, 15.0, 30.0):
+                m = ctx.criterion(capi.BISQUARE, False, bw, crit, capi.MIXED)
+                cond, refined = ctx.last_condition()
+                f = ctx.criterion(capi.BISQUARE, False, bw, crit, capi.FP64)
+                assert refined == 900
+                assert m == f, (crit, bw, m, f)
+        m = ctx.gwr_run(capi.BISQUARE, False, 15.0, precision=capi.MIXED)
+        f = ctx.gwr_run(capi.BISQUARE, False, 15.0, precision=capi.FP64)
+        assert np.array_equal(m["betas"], f["betas"])
+        assert np.array_equal(m["shat"], f["shat"])
+        assert np.max(np.abs(m["qDiag"] - f["qDiag"])) < 1e-4
+
+
+def test_refine_default_limits_on_hard_searches(hard_problems):
+    with capi.Context(0) as ctx:
+        for idx, p in zip(HARD, hard_problems):
+            n, k = p["X"].shape
+            say("case: batch-workload problem %d (n=%d k=%d kernel=%d adaptive=%d)" % (idx, n, k, p["kernel"],
+                                                                                       p["adaptive"]))
+            ctx.set_data(p["coords"], p["X"], p["y"])
+            m = ctx.gwr_run(p["kernel"], p["adaptive"], 0.0, select_bw=True, crit=capi.CRIT_CV, precision=capi.MIXED)
+            f = ctx.gwr_run(p["kernel"], p["adaptive"], 0.0, select_bw=True, crit=capi.CRIT_CV, precision=capi.FP64)
+            assert m["bandwidth"] == f["bandwidth"], idx
+            err = col_err(m["betas"], f["betas"])
+            say("  bandwidth %.17g beta err %.3g" % (m["bandwidth"], err))
+            assert err < 1e-6, idx
+
+
+def test_server_refine_everything_reproduces_fp64(hard_problems):
+    say("case: server, refine limit 0, hard problems")
+    out = {}
+    for prec in (capi.FP64, capi.MIXED):
+        with capi.Server(0, prec, max_inflight=8) as srv:
+            srv.set_refine_condition(0.0, 0.0)
+            srv.set_round_ms(0.5)
+            ids = [srv.submit(p["coords"], p["X"], p["y"], p["kernel"], p["adaptive"], 0.0, select_bw=True,
+                              crit=capi.CRIT_CV) for p in hard_problems]
+            out[prec] = [srv.wait(i) for i in ids]
+    for a, b in zip(out[capi.MIXED], out[capi.FP64]):
+        assert a["status"] == capi.GWM_OK and b["status"] == capi.GWM_OK
+        assert a["bandwidth"] == b["bandwidth"]
+        assert np.array_equal(a["betas"], b["betas"])
+        assert np.array_equal(a["shat"], b["shat"])
+
+
+def test_server_default_limits_on_hard_searches(hard_problems):
+    say("case: server, default limits, hard problems")
+    out = {}
+    for prec in (capi.FP64, capi.MIXED):
+        with capi.Server(0, prec, max_inflight=8) as srv:
+            ids = [srv.submit(p["coords"], p["X"], p["y"], p["kernel"], p["adaptive"], 0.0, select_bw=True,
+                              crit=capi.CRIT_CV) for p in hard_problems]
+            out[prec] = [srv.wait(i) for i in ids]
+    for idx, a, b in zip(HARD, out[capi.MIXED], out[capi.FP64]):
+        assert a["bandwidth"] == b["bandwidth"], idx
+        assert col_err(a["betas"], b["betas"]) < 1e-6, idx
