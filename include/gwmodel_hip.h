@@ -146,6 +146,16 @@ int gwm_hip_set_certify_tolerance(gwm_hip_ctx* ctx, double rel_tol);
  * returns the indicator maximum of the last mixed evaluation and the number
  * of focus points it recomputed.  No counterpart in the reference. */
 int gwm_hip_set_refine_condition(gwm_hip_ctx* ctx, double criterion_limit, double fit_limit);
+
+/* Spatial pruning (default on): the points are kept in a spatial order in 64-
+ * point chunks with bounding boxes, and a sweep skips every chunk whose
+ * points are all farther from a workgroup's focus points than the distance
+ * at which the kernel weight is exactly zero in the sweep's arithmetic
+ * (compact kernels: the bandwidth; Gaussian / exponential: exp underflow in
+ * FP64, FP16 underflow of the split weight in mixed precision).  Skipped
+ * chunks contribute exactly zero, so results are bit-identical with pruning
+ * on or off.  No counterpart in the reference, which sums over all n. */
+int gwm_hip_set_pruning(gwm_hip_ctx* ctx, int enabled);
 int gwm_hip_last_condition(gwm_hip_ctx* ctx, double* cond, long long* refined);
 
 /* Timing of the device kernels with HIP events on the context stream

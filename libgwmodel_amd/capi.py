@@ -53,6 +53,7 @@ SIGNATURES = {
                               _dp, _dp, _dp, _dp, _ip]),
     "gwm_hip_set_certify_tolerance": (_i, [_vp, _d]),
     "gwm_hip_set_refine_condition": (_i, [_vp, _d, _d]),
+    "gwm_hip_set_pruning": (_i, [_vp, _i]),
     "gwm_hip_last_condition": (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "gwm_hip_set_timing": (_i, [_vp, _i]),
     "gwm_hip_kernel_time": (_i, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong), _dp]),
@@ -272,6 +273,10 @@ class Context:
         """Mixed precision: focus points whose conditioning indicator exceeds
         the limit are recomputed in FP64 (criterion evaluations / fits)."""
         self._check(self._lib.gwm_hip_set_refine_condition(self.h, float(criterion_limit), float(fit_limit)))
+
+    def set_pruning(self, on=True):
+        """Skip data chunks whose kernel weights are all exactly zero (bit-identical results)."""
+        self._check(self._lib.gwm_hip_set_pruning(self.h, int(on)))
 
     def last_condition(self):
         """(indicator maximum, focus points recomputed in FP64) of the last mixed evaluation."""

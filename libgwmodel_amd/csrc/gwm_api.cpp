@@ -52,6 +52,7 @@ void dfree(T*& p) {
 void free_problem(gwm_hip_ctx* c) {
     dfree(c->px); dfree(c->py); dfree(c->hx); dfree(c->hy_); dfree(c->hz); dfree(c->sl);
     dfree(c->X); dfree(c->y); dfree(c->Z); dfree(c->G); dfree(c->part);
+    dfree(c->boxes); dfree(c->chunk_bw);
     dfree(c->bwi); dfree(c->res2); dfree(c->cond); dfree(c->sii); dfree(c->rlist); dfree(c->rcount);
     dfree(c->rpart);
     c->rpart_bytes = 0; dfree(c->ssi); dfree(c->betas); dfree(c->se);
@@ -67,6 +68,68 @@ template <class T>
 int dalloc(gwm_hip_ctx* c, T** p, size_t count) {
     GWM_HIP_CHECK(c, hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
     return GWM_OK;
+}
+
+// Download a sorted-order device array (n x cols) into a caller-order buffer.
+int download_caller(gwm_hip_ctx* c, const double* dev, double* host, int cols) {
+    std::vector<double> tmp((size_t)c->n * cols);
+    GWM_HIP_CHECK(c, hipMemcpyAsync(tmp.data(), dev, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    to_caller(c, tmp.data(), host, cols);
+    return GWM_OK;
+}
+
+// Interleave the bits of up to 3 coordinates quantized to `bits` bits.
+uint64_t morton(const uint32_t* q, int dims, int bits) {
+    uint64_t code = 0;
+    for (int b = bits - 1; b >= 0; --b)
+        for (int d = 0; d < dims; ++d) code = (code << 1) | ((q[d] >> b) & 1u);
+    return code;
+}
+
+// Sorted order of the points along a Morton curve of their coordinates (the
+// plane, or the unit sphere's embedding in 3D for great-circle distances).
+std::vector<int> spatial_order(int n, int dims, const double* const* xs) {
+    const int bits = dims == 2 ? 16 : 21;
+    double lo[3], hi[3];
+    for (int d = 0; d < dims; ++d) {
+        lo[d] = *std::min_element(xs[d], xs[d] + n);
+        hi[d] = *std::max_element(xs[d], xs[d] + n);
+    }
+    std::vector<std::pair<uint64_t, int>> key(n);
+    const double top = (double)((1u << bits) - 1);
+    for (int i = 0; i < n; ++i) {
+        uint32_t q[3] = {0, 0, 0};
+        for (int d = 0; d < dims; ++d) {
+            const double span = hi[d] - lo[d];
+            const double t = span > 0 ? (xs[d][i] - lo[d]) / span : 0.0;
+            q[d] = (uint32_t)std::min(top, std::max(0.0, std::floor(t * top + 0.5)));
+        }
+        key[i] = {morton(q, dims, bits), i};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = key[i].second;
+    return perm;
+}
+
+// Bounding boxes [lo_0, hi_0, lo_1, hi_1, ...] of consecutive 64-point chunks.
+std::vector<double> chunk_boxes(int n, int dims, const double* const* xs) {
+    const int nc = (n + 63) / 64;
+    std::vector<double> b((size_t)nc * 2 * dims);
+    for (int ch = 0; ch < nc; ++ch) {
+        const int j0 = ch * 64, j1 = std::min(n, j0 + 64);
+        for (int d = 0; d < dims; ++d) {
+            double mn = xs[d][j0], mx = xs[d][j0];
+            for (int j = j0 + 1; j < j1; ++j) {
+                mn = std::min(mn, xs[d][j]);
+                mx = std::max(mx, xs[d][j]);
+            }
+            b[(size_t)ch * 2 * dims + 2 * d] = mn;
+            b[(size_t)ch * 2 * dims + 2 * d + 1] = mx;
+        }
+    }
+    return b;
 }
 
 
@@ -155,6 +218,13 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
     j.nta = nta;
     j.ntb = ntb;
     j.out = c->part;
+    if (c->prune) {
+        j.boxes = c->boxes;
+        if (mode == kModeQDiag && bw_pt) {
+            GWM_HIP_CHECK(c, launch_chunk_max(bw_pt, c->n, c->chunk_bw, c->stream));
+            j.chunk_bw = c->chunk_bw;
+        }
+    }
     L.jobs = nullptr;
     L.items = nullptr;
     L.nitems = 0;
@@ -248,6 +318,7 @@ int refine_fp64(gwm_hip_ctx* c, int mode, int smode, int ntb, int colB, const do
     j.out = c->rpart;
     j.focus_idx = c->rlist;
     j.nfocus = *count;
+    if (c->prune) j.boxes = c->boxes;
     L.jobs = nullptr;
     L.items = nullptr;
     L.nitems = 0;
@@ -269,6 +340,18 @@ int refine_fp64(gwm_hip_ctx* c, int mode, int smode, int ntb, int colB, const do
 
 namespace gwm {
 namespace detail {
+
+void to_sorted(const gwm_hip_ctx* c, const double* src, double* dst, int cols) {
+    const size_t n = c->n;
+    for (int j = 0; j < cols; ++j)
+        for (size_t s = 0; s < n; ++s) dst[s + j * n] = src[(size_t)c->perm[s] + j * n];
+}
+
+void to_caller(const gwm_hip_ctx* c, const double* src, double* dst, int cols) {
+    const size_t n = c->n;
+    for (int j = 0; j < cols; ++j)
+        for (size_t s = 0; s < n; ++s) dst[(size_t)c->perm[s] + j * n] = src[s + j * n];
+}
 
 void calc_diagnostic(int n_, int k, const double* X, const double* y, const double* betas, const double* shat,
                      double* diag) {
@@ -521,9 +604,15 @@ int max_distance_impl(gwm_hip_ctx* c, double* out) {
 
 namespace {
 
-// Upload the design and its product table Z = [x_a x_b (a <= b) | x_a y].
-int upload_design(gwm_hip_ctx* c, const double* X, const double* y) {
+// Upload the design (caller's order) in sorted order with its product table
+// Z = [x_a x_b (a <= b) | x_a y].
+int upload_design(gwm_hip_ctx* c, const double* X_caller, const double* y_caller) {
     const int n = c->n, k = c->k, ldz = c->nta * 16;
+    std::vector<double> Xs((size_t)n * k), ys(n);
+    to_sorted(c, X_caller, Xs.data(), k);
+    to_sorted(c, y_caller, ys.data(), 1);
+    const double* X = Xs.data();
+    const double* y = ys.data();
     std::vector<double> Z((size_t)c->rows_pad * ldz, 0.0);
     for (int j = 0; j < n; ++j) {
         double* z = Z.data() + (size_t)j * ldz;
@@ -541,13 +630,18 @@ int upload_design(gwm_hip_ctx* c, const double* X, const double* y) {
 }
 
 // Single-column criterion / fit (GWRMultiscale.cpp:420-482, 560-636).
-int var_impl(gwm_hip_ctx* c, const double* x, const double* y, int kernel, int adaptive, double bw, int crit,
-             int precision, double* value, double* beta) {
+int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int kernel, int adaptive, double bw,
+             int crit, int precision, double* value, double* beta) {
     int rc = check_kernel(c, kernel, precision);
     if (rc) return rc;
     if (value) *value = DBL_MAX;
     if (stopped(c)) return GWM_ERR_CANCELLED;
     const int n = c->n;
+    std::vector<double> xs(n), ys(n);
+    to_sorted(c, x_caller, xs.data(), 1);
+    to_sorted(c, y_caller, ys.data(), 1);
+    const double* x = xs.data();
+    const double* y = ys.data();
     std::vector<double> Z((size_t)c->rows_pad * 16, 0.0);
     for (int j = 0; j < n; ++j) {
         Z[(size_t)j * 16 + 0] = x[j] * x[j];
@@ -582,9 +676,8 @@ int var_impl(gwm_hip_ctx* c, const double* x, const double* y, int kernel, int a
         *value = v;
         return GWM_OK;
     }
-    GWM_HIP_CHECK(c, hipMemcpyAsync(beta, c->betas, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if ((rc = download_caller(c, c->betas, beta, 1))) return rc;
     c->solves += n;
     return fail ? GWM_ERR_NOT_SPD : GWM_OK;
 }
@@ -605,10 +698,8 @@ int fit_betas_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int prec
         if ((rc = refine_fp64(c, kModePlain, kSolveBeta, 0, 0, bw_pt, bw, kernel, c->refine_fit, &count))) return rc;
     }
     int fail = 0;
-    GWM_HIP_CHECK(c, hipMemcpyAsync(betas, c->betas, (size_t)c->n * c->k * sizeof(double), hipMemcpyDeviceToHost,
-                                    c->stream));
     GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if ((rc = download_caller(c, c->betas, betas, c->k))) return rc;
     c->solves += c->n;
     return fail ? GWM_ERR_NOT_SPD : GWM_OK;
 }
@@ -703,15 +794,13 @@ int gwm_hip_set_data(gwm_hip_ctx* c, int n, int k, const double* coords, const d
         return rc;
     }
     GWM_HIP_CHECK(c, hipMemsetAsync(c->G, 0, (size_t)c->rows_pad * c->ntb * 16 * sizeof(double), c->stream));
-    GWM_HIP_CHECK(c, hipMemcpyAsync(c->px, coords, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    GWM_HIP_CHECK(c, hipMemcpyAsync(c->py, coords + N, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // great circle: the half unit vectors (hx, hy, hz) and sin(lat) of every point
+    std::vector<double> hx, hy, hz, sl;
     if (distance == GWM_GREAT_CIRCLE) {
-        if ((rc = dalloc(c, &c->hx, N)) || (rc = dalloc(c, &c->hy_, N)) || (rc = dalloc(c, &c->hz, N)) ||
-            (rc = dalloc(c, &c->sl, N))) {
-            free_problem(c);
-            return rc;
-        }
-        std::vector<double> hx(N), hy(N), hz(N), sl(N);
+        hx.resize(N);
+        hy.resize(N);
+        hz.resize(N);
+        sl.resize(N);
         const double de2ra = kPi / 180;
         for (size_t j = 0; j < N; ++j) {
             double lon = coords[j] * de2ra, lat = coords[N + j] * de2ra;
@@ -720,12 +809,52 @@ int gwm_hip_set_data(gwm_hip_ctx* c, int n, int k, const double* coords, const d
             hz[j] = 0.5 * std::sin(lat);
             sl[j] = std::sin(lat);
         }
-        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hx, hx.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hy_, hy.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hz, hz.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        GWM_HIP_CHECK(c, hipMemcpyAsync(c->sl, sl.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     }
+    // spatial order, sorted point tables and the 64-point chunk boxes
+    const int dims = distance == GWM_GREAT_CIRCLE ? 3 : 2;
+    {
+        const double* xs[3] = {coords, coords + N, nullptr};
+        if (dims == 3) {
+            xs[0] = hx.data();
+            xs[1] = hy.data();
+            xs[2] = hz.data();
+        }
+        c->perm = spatial_order(n, dims, xs);
+    }
+    std::vector<double> cs(2 * N);
+    to_sorted(c, coords, cs.data(), 2);
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->px, cs.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->py, cs.data() + N, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<double> box;
+    if (distance == GWM_GREAT_CIRCLE) {
+        if ((rc = dalloc(c, &c->hx, N)) || (rc = dalloc(c, &c->hy_, N)) || (rc = dalloc(c, &c->hz, N)) ||
+            (rc = dalloc(c, &c->sl, N))) {
+            free_problem(c);
+            return rc;
+        }
+        std::vector<double> h(4 * N);
+        to_sorted(c, hx.data(), h.data(), 1);
+        to_sorted(c, hy.data(), h.data() + N, 1);
+        to_sorted(c, hz.data(), h.data() + 2 * N, 1);
+        to_sorted(c, sl.data(), h.data() + 3 * N, 1);
+        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hx, h.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hy_, h.data() + N, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hz, h.data() + 2 * N, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        GWM_HIP_CHECK(c, hipMemcpyAsync(c->sl, h.data() + 3 * N, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        const double* xs[3] = {h.data(), h.data() + N, h.data() + 2 * N};
+        box = chunk_boxes(n, 3, xs);
+        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    } else {
+        const double* xs[2] = {cs.data(), cs.data() + N};
+        box = chunk_boxes(n, 2, xs);
+    }
+    if ((rc = dalloc(c, &c->boxes, box.size())) || (rc = dalloc(c, &c->chunk_bw, (N + 63) / 64))) {
+        free_problem(c);
+        return rc;
+    }
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->boxes, box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice,
+                                    c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return upload_design(c, X, y);
 }
 
@@ -774,7 +903,7 @@ int gwm_hip_fit(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int precisi
                                                c->stream));
         if ((rc = sweep(c, kModeQDiag, c->ntb, 0, g_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
         GWM_HIP_CHECK(c, launch_qdiag(c->k, c->part, nseg, PT, c->n, c->n_pad, c->X, c->sii, c->qd, c->stream));
-        GWM_HIP_CHECK(c, hipMemcpyAsync(qDiag, c->qd, c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if ((rc = download_caller(c, c->qd, qDiag, 1))) return rc;
     }
     if (S) {
         if (c->n > 65535) {
@@ -783,15 +912,19 @@ int gwm_hip_fit(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int precisi
         }
         if (!c->S) GWM_HIP_CHECK(c, hipMalloc((void**)&c->S, (size_t)c->n * c->n * sizeof(double)));
         GWM_HIP_CHECK(c, launch_smatrix(c->dist, c->k, c->points(), c->X, c->gvec, bw_pt, bw, kernel, c->S, c->stream));
-        GWM_HIP_CHECK(c, hipMemcpyAsync(S, c->S, (size_t)c->n * c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        std::vector<double> Ss((size_t)c->n * c->n);
+        GWM_HIP_CHECK(c, hipMemcpyAsync(Ss.data(), c->S, Ss.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        const size_t nn = c->n;
+        for (size_t j = 0; j < nn; ++j)
+            for (size_t i = 0; i < nn; ++i) S[(size_t)c->perm[i] + (size_t)c->perm[j] * nn] = Ss[i + j * nn];
     }
     GWM_HIP_CHECK(c, launch_reduce(c->sii, c->ssi, c->res2, nullptr, c->n, c->red, c->stream));
     double red[3];
     GWM_HIP_CHECK(c, hipMemcpyAsync(red, c->red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
-    const size_t nk = (size_t)c->n * c->k;
-    if (betas) GWM_HIP_CHECK(c, hipMemcpyAsync(betas, c->betas, nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (se_raw) GWM_HIP_CHECK(c, hipMemcpyAsync(se_raw, c->se, nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (betas && (rc = download_caller(c, c->betas, betas, c->k))) return rc;
+    if (se_raw && (rc = download_caller(c, c->se, se_raw, c->k))) return rc;
     if (shat) {
         shat[0] = red[0];
         shat[1] = red[1];
@@ -862,10 +995,10 @@ int gwm_hip_distance_row(gwm_hip_ctx* c, int focus, double* out) {
     if (rc) return rc;
     if (!out || focus < 0 || focus >= c->n) return GWM_ERR_INVALID;
     GWM_HIP_CHECK(c, hipSetDevice(c->device));
-    GWM_HIP_CHECK(c, launch_distance_row(c->dist, c->points(), focus, c->res2, c->stream));
-    GWM_HIP_CHECK(c, hipMemcpyAsync(out, c->res2, c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return GWM_OK;
+    int pos = 0;   // sorted position of the caller's point `focus`
+    while (c->perm[pos] != focus) ++pos;
+    GWM_HIP_CHECK(c, launch_distance_row(c->dist, c->points(), pos, c->res2, c->stream));
+    return download_caller(c, c->res2, out, 1);
 }
 
 int gwm_hip_adaptive_bandwidths(gwm_hip_ctx* c, double bw, double* out) {
@@ -875,9 +1008,7 @@ int gwm_hip_adaptive_bandwidths(gwm_hip_ctx* c, double bw, double* out) {
     GWM_HIP_CHECK(c, hipSetDevice(c->device));
     const double* bw_pt;
     if ((rc = resolve_bandwidth(c, 1, bw, &bw_pt))) return rc;
-    GWM_HIP_CHECK(c, hipMemcpyAsync(out, c->bwi, c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return GWM_OK;
+    return download_caller(c, c->bwi, out, 1);
 }
 
 int gwm_hip_var_criterion(gwm_hip_ctx* c, const double* x, const double* y, int kernel, int adaptive, double bw,
@@ -1033,6 +1164,12 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
 int gwm_hip_set_certify_tolerance(gwm_hip_ctx* c, double rel) {
     if (!c || !(rel >= 0.0)) return GWM_ERR_INVALID;
     c->certify_rel = rel;
+    return GWM_OK;
+}
+
+int gwm_hip_set_pruning(gwm_hip_ctx* c, int enabled) {
+    if (!c) return GWM_ERR_INVALID;
+    c->prune = enabled != 0;
     return GWM_OK;
 }
 

@@ -231,6 +231,10 @@ SweepJob sweep_job(Request* r, int mode, int nta, int ntb, const double* bw_pt, 
     j.nta = nta;
     j.ntb = ntb;
     j.out = r->part;
+    if (c->prune) {
+        j.boxes = c->boxes;
+        if (mode == kModeQDiag && bw_pt) j.chunk_bw = c->chunk_bw;
+    }
     return j;
 }
 
@@ -437,7 +441,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
     std::vector<SolveGroup> solves;
     std::vector<QdGroup> qds;
     std::vector<ReduceJob> reds;
-    std::vector<Request*> splits, finished, checks, refine_starts;
+    std::vector<Request*> splits, qstarts, finished, checks, refine_starts;
     for (const Slice& sl : slices) {
         Request* r = sl.r;
         const Eval& ev = r->ev;
@@ -495,6 +499,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         }
         if (sl.check) checks.push_back(r);
         if (sl.split && prec == GWM_MIXED) splits.push_back(r);
+        if (sl.split && r->adaptive && c->prune) qstarts.push_back(r);
         if (sl.q1 > sl.q0) {
             SweepGroup& q = group_of(qsweeps, r->dist, kModeQDiag, prec, ntb, 0);
             q.jobs.push_back(sweep_job(r, kModeQDiag, ntb, 0, bw_pt, ev.bw));
@@ -583,6 +588,8 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         SRV_CHECK(S, hipMemcpyAsync(&r->nrefine, r->ctx->rcount, sizeof(int), hipMemcpyDeviceToHost, S->stream));
     }
     // 4. qDiag second sweeps of the fits (G complete: every solve of the fit precedes on the stream)
+    for (Request* r : qstarts)
+        SRV_CHECK(S, launch_chunk_max(r->ctx->bwi, r->n, r->ctx->chunk_bw, S->stream));
     for (Request* r : splits)
         SRV_CHECK(S, launch_split_fp16(r->ctx->G, r->ctx->rows_pad, ntb_of(r->k) * 16, r->ctx->colmax, r->ctx->Gh,
                                        r->ctx->Gl, r->ctx->gscale, S->stream));
@@ -690,6 +697,17 @@ void worker_main(gwm_hip_server* S, Request* R) {
     }
     if (rc == GWM_OK) rc = post(kEvalFit, bw, S->precision, 0, nullptr, nullptr);
     if (rc == GWM_OK) {
+        // device order -> caller's order
+        {
+            std::vector<double> t(R->hbetas.size());
+            to_caller(R->ctx, R->hbetas.data(), t.data(), R->k);
+            R->hbetas.swap(t);
+            to_caller(R->ctx, R->hse.data(), t.data(), R->k);
+            R->hse.swap(t);
+            t.resize(R->n);
+            to_caller(R->ctx, R->hq.data(), t.data(), 1);
+            R->hq.swap(t);
+        }
         double shat[2] = {R->ev.shat[0], R->ev.shat[1]};
         double diag[7];
         calc_diagnostic(R->n, R->k, R->X.data(), R->y.data(), R->hbetas.data(), shat, diag);

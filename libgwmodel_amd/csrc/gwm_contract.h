@@ -34,6 +34,10 @@ struct SweepJob {
     // t < nfocus (n_pad then pads nfocus); null = every point in order
     const int* focus_idx = nullptr;
     int nfocus = 0;
+    // pruning (see gwm_device.h): boxes of the 64-point data chunks (null: no
+    // pruning) and, for adaptive qDiag sweeps, each chunk's largest bandwidth
+    const double* boxes = nullptr;
+    const double* chunk_bw = nullptr;
 };
 
 // Launch of one sweep: either a single job (items == nullptr; workgroup b
@@ -58,6 +62,10 @@ hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const Sweep
 hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s);
 // the R (focus tiles per wave) each kernel uses for a (nta, ntb) instance
 int contract_r(int nta, int ntb, int precision_mixed);
+
+// Largest value of each 64-entry chunk of v[0..n) (chunk bandwidths for the
+// pruned qDiag sweeps).
+hipError_t launch_chunk_max(const double* v, int n, double* out, hipStream_t s);
 
 // Adaptive bandwidth resolution (BandwidthWeight.cpp:40-55): per focus the
 // interpolated distance between the floor(bw)-th and floor(bw)+1-th nearest

@@ -118,6 +118,7 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
     FocusCoords<DIST> fc[R];
     KConst64 fbw[R];
     int fidx[R];
+    double fb = 0.0;   // largest focus bandwidth of this lane
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int f = fbase + 16 * r + (lane & 15);
@@ -125,7 +126,32 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
         if (focus_idx) fl = focus_idx[fl];
         fidx[r] = focus_idx ? (f < nf ? fl : -1) : f;   // self index for the CV exclusion
         fc[r] = load_focus<DIST>(P, fl);
-        fbw[r] = make_kconst64((MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : job.bw);
+        const double b = (MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : job.bw;
+        fbw[r] = make_kconst64(b);
+        fb = fmax(fb, b);
+    }
+    // pruning: the workgroup's focus box and the zero-weight threshold
+    constexpr int BD = DIST == kEuclidean ? 2 : 3;
+    const bool prune = job.boxes != nullptr;
+    double flo[BD], fhi[BD], thr = INFINITY;
+    if (prune) {
+#pragma unroll
+        for (int d = 0; d < BD; ++d) {
+            flo[d] = INFINITY;
+            fhi[d] = -INFINITY;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double x[3] = {DIST == kEuclidean ? fc[r].a : fc[r].hx, DIST == kEuclidean ? fc[r].b : fc[r].hy,
+                                 fc[r].hz};
+#pragma unroll
+            for (int d = 0; d < BD; ++d) {
+                flo[d] = fmin(flo[d], x[d]);
+                fhi[d] = fmax(fhi[d], x[d]);
+            }
+        }
+        workgroup_box<BD>(flo, fhi, fb, &fb);
+        if (MODE != kModeQDiag) thr = prune_threshold<DIST>(zero_weight_distance(job.kernel, fb, false));
     }
 
     v4d accA[R][NTA];
@@ -143,6 +169,15 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
     const int kk = lane >> 4;
 
     for (int j0 = jbeg; j0 < jend; j0 += kChunk) {
+        if (prune) {
+            const int ch = j0 / kChunk;
+            double t = thr;
+            if (MODE == kModeQDiag)
+                t = prune_threshold<DIST>(
+                    zero_weight_distance(job.kernel, job.chunk_bw ? gptr(job.chunk_bw)[ch] : job.bw, false));
+            const int skip = box_gap2<DIST>(flo, fhi, gptr(job.boxes) + (size_t)ch * 2 * BD) > t;
+            if (__builtin_amdgcn_readfirstlane(skip)) continue;
+        }
         __syncthreads();
         // stage Z rows [j0, j0 + kChunk) (zero padded beyond n in memory)
         {

@@ -217,6 +217,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     MFocus<DIST> fc[R];
     BwConst fbw[R];
     int fidx[R];
+    double fb = 0.0;   // largest focus bandwidth of this lane
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int f = fbase + 16 * r + col;
@@ -224,9 +225,31 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
         if (focus_idx) fl = focus_idx[fl];
         fidx[r] = focus_idx ? (f < nf ? fl : -1) : f;   // self index for the CV exclusion
         fc[r] = load_mfocus<DIST>(P, fl);
-        double b = (MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : args.bw;
-        if (DIST == kEuclidean) fbw[r] = make_bw((float)b);
-        else fbw[r] = make_bw((float)b);
+        const double b = (MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : args.bw;
+        fbw[r] = make_bw((float)b);
+        fb = fmax(fb, b);
+    }
+    // pruning: the workgroup's focus box and the zero-weight threshold
+    constexpr int BD = DIST == kEuclidean ? 2 : 3;
+    const bool prune = args.boxes != nullptr;
+    double flo[BD], fhi[BD], thr = INFINITY;
+    if (prune) {
+#pragma unroll
+        for (int d = 0; d < BD; ++d) {
+            flo[d] = INFINITY;
+            fhi[d] = -INFINITY;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double x[3] = {fc[r].a, fc[r].b, fc[r].c};
+#pragma unroll
+            for (int d = 0; d < BD; ++d) {
+                flo[d] = fmin(flo[d], x[d]);
+                fhi[d] = fmax(fhi[d], x[d]);
+            }
+        }
+        workgroup_box<BD>(flo, fhi, fb, &fb);
+        if (MODE != kModeQDiag) thr = prune_threshold<DIST>(zero_weight_distance(args.kernel, fb, true));
     }
 
     v4f hh[R][NTA + NTB], xx[R][NTA + NTB], ll[R][NTA + NTB];
@@ -240,6 +263,15 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     const int jend = min(n, jbeg + args.seg_len);
 
     for (int j0 = jbeg; j0 < jend; j0 += kChunkM) {
+        if (prune) {
+            const int ch = j0 / kChunkM;
+            double t = thr;
+            if (MODE == kModeQDiag)
+                t = prune_threshold<DIST>(
+                    zero_weight_distance(args.kernel, args.chunk_bw ? gptr(args.chunk_bw)[ch] : args.bw, true));
+            const int skip = box_gap2<DIST>(flo, fhi, gptr(args.boxes) + (size_t)ch * 2 * BD) > t;
+            if (__builtin_amdgcn_readfirstlane(skip)) continue;
+        }
         __syncthreads();
         // stage the two 32-row blocks of both planes: global [blk][col][32]
         {

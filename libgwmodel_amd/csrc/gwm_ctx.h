@@ -33,7 +33,15 @@ struct gwm_hip_ctx {
     int n = 0, k = 0, dist = 0;
     int nta = 0, ntb = 0, m = 0;
     int n_pad = 0, rows_pad = 0;
-    std::vector<double> hcoords, hX, hy;
+    std::vector<double> hcoords, hX, hy;   // the caller's order
+    // Spatial order: every per-point device array holds the points sorted
+    // along a Morton curve, so that runs of 64 points are compact; perm[s] is
+    // the caller's index of sorted position s.  Host copies in and out of the
+    // device are permuted at the API boundary.
+    std::vector<int> perm;
+    bool prune = true;             // skip data chunks whose weights are all exactly zero
+    double* boxes = nullptr;       // bounding boxes of the 64-point chunks (sorted order)
+    double* chunk_bw = nullptr;    // largest per-point bandwidth of each chunk (qDiag sweeps)
 
     // device buffers
     double *px = nullptr, *py = nullptr, *hx = nullptr, *hy_ = nullptr, *hz = nullptr, *sl = nullptr;
@@ -95,6 +103,10 @@ inline void data_segments(int n, int& nseg, int& seg_len) {
     seg_len = seg_len < 64 ? 64 : (seg_len > 8192 ? 8192 : seg_len);
     nseg = (n + seg_len - 1) / seg_len;
 }
+// Column-major n x cols arrays from the caller's order to the context's
+// sorted order and back.
+void to_sorted(const gwm_hip_ctx* c, const double* src, double* dst, int cols);
+void to_caller(const gwm_hip_ctx* c, const double* src, double* dst, int cols);
 int nta_of(int k);
 int ntb_of(int k);
 double aicc(double rss, double n, double trS);

@@ -250,6 +250,91 @@ __device__ __forceinline__ float kernel_f32(int kernel, float d, float bw, float
     }
 }
 
+// ---------------------------------------------------------------- pruning
+//
+// The points are stored in a spatial order and cut into chunks of 64 with a
+// bounding box each.  A chunk whose points are all farther from every focus
+// point of a workgroup than the distance at which the kernel weight is exactly
+// zero contributes exactly nothing, and the sweep skips it; results are the
+// same bits as without skipping.
+
+// Distance beyond which the kernel weight is exactly zero: in FP64 (exp
+// underflows to 0 below e^-745; compact kernels vanish for d >= bw), or in the
+// mixed path, whose FP32 weight is split into FP16 hi + lo parts that are both
+// zero below 2^-36 (exp2 arguments below -37 keep a margin).  Never zero:
+// infinity.
+__device__ __forceinline__ double zero_weight_distance(int kernel, double bw, bool mixed) {
+    switch (kernel) {
+    case kGaussian: return (mixed ? 7.25 : 40.0) * bw;      // d^2 / (2 b^2) > 26.3 (mixed) / 800 (FP64)
+    case kExponential: return (mixed ? 26.0 : 800.0) * bw;
+    default: return bw * (mixed ? 1.0 + 1e-5 : 1.0 + 1e-9);
+    }
+}
+
+// Threshold on the squared box gap (box_gap2) beyond which every pair is
+// farther than `cut`.  Euclidean: cut^2.  Great circle: the gap is measured
+// between boxes of the half unit vectors h, whose squared difference is
+// S = sin^2(theta/2) (theta the central angle), and the ellipsoidal distance
+// is at least 0.98 a theta (the flattening terms change it by less than 4f).
+template <int DIST>
+__device__ __forceinline__ double prune_threshold(double cut) {
+    if (DIST == kEuclidean) return cut * cut;
+    const double a = 6378.137;
+    const double theta = cut / (0.98 * a);
+    if (!(theta < 3.14159)) return INFINITY;
+    const double s = sin(0.5 * theta);
+    return s * s;
+}
+
+// Squared gap between a focus box (lo[d], hi[d]) and a chunk box stored as
+// [lo_0, hi_0, lo_1, hi_1, ...].
+template <int DIST>
+__device__ __forceinline__ double box_gap2(const double* lo, const double* hi, GWM_GLOBAL const double* box) {
+    constexpr int D = DIST == kEuclidean ? 2 : 3;
+    double g2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const double g = fmax(0.0, fmax(box[2 * d] - hi[d], lo[d] - box[2 * d + 1]));
+        g2 += g * g;
+    }
+    return g2;
+}
+
+// Workgroup-wide box of the focus points and their largest bandwidth: every
+// thread of the 256-thread workgroup passes the box lo/hi of its own focus
+// points and their largest bandwidth b, and receives the workgroup's.
+template <int D>
+__device__ __forceinline__ void workgroup_box(double* lo, double* hi, double b, double* bmax) {
+    __shared__ double s_part[4][2 * D + 1];
+    double v[2 * D + 1];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        v[2 * d] = lo[d];
+        v[2 * d + 1] = hi[d];
+    }
+    v[2 * D] = b;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            v[2 * d] = fmin(v[2 * d], __shfl_xor(v[2 * d], m));
+            v[2 * d + 1] = fmax(v[2 * d + 1], __shfl_xor(v[2 * d + 1], m));
+        }
+        v[2 * D] = fmax(v[2 * D], __shfl_xor(v[2 * D], m));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int e = 0; e < 2 * D + 1; ++e) s_part[wave][e] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        lo[d] = fmin(fmin(s_part[0][2 * d], s_part[1][2 * d]), fmin(s_part[2][2 * d], s_part[3][2 * d]));
+        hi[d] = fmax(fmax(s_part[0][2 * d + 1], s_part[1][2 * d + 1]), fmax(s_part[2][2 * d + 1], s_part[3][2 * d + 1]));
+    }
+    *bmax = fmax(fmax(s_part[0][2 * D], s_part[1][2 * D]), fmax(s_part[2][2 * D], s_part[3][2 * D]));
+}
+
 // Job of batch entry t, given the prefix sums start[0..njobs] of the jobs'
 // entry counts (start[0] = 0): the largest j with start[j] <= t.
 __device__ __forceinline__ int find_job(const int* start, int njobs, int t) {

@@ -253,6 +253,16 @@ __global__ void smatrix_kernel(Points P, int k, const double* X, const double* g
     S[i + (size_t)j * n] = w * s;
 }
 
+// Largest value of each 64-entry chunk.
+__global__ __launch_bounds__(256) void chunk_max_kernel(const double* v, int n, double* out) {
+    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ch * 64 >= n) return;
+    const int j1 = min(n, ch * 64 + 64);
+    double m = v[ch * 64];
+    for (int j = ch * 64 + 1; j < j1; ++j) m = fmax(m, v[j]);
+    out[ch] = m;
+}
+
 // Indices of the focus points whose indicator exceeds the limit.
 __global__ __launch_bounds__(256) void compact_kernel(const double* cond, int n, double limit, int* list, int* count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -433,6 +443,13 @@ hipError_t launch_smatrix(int dist, int k, const Points& P, const double* X, con
         hipLaunchKernelGGL(smatrix_kernel<kEuclidean>, grid, dim3(256), 0, s, P, k, X, gvec, bw_pt, bw, kernel, S);
     else
         hipLaunchKernelGGL(smatrix_kernel<kGreatCircle>, grid, dim3(256), 0, s, P, k, X, gvec, bw_pt, bw, kernel, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_chunk_max(const double* v, int n, double* out, hipStream_t s) {
+    const int nc = (n + 63) / 64;
+    if (nc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(chunk_max_kernel, dim3((nc + 255) / 256), dim3(256), 0, s, v, n, out);
     return hipGetLastError();
 }
 
