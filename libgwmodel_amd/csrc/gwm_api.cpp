@@ -168,7 +168,7 @@ int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw
         return GWM_ERR_INVALID;
     }
     timer_begin(c);
-    GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, c->bwi, c->stream));
+    GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, c->bwi, c->prune ? c->boxes : nullptr, c->stream));
     int rc = timer_end(c, "select");
     if (rc) return rc;
     *bw_pt = c->bwi;

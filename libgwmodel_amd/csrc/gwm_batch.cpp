@@ -464,7 +464,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         if (sl.a1 > sl.a0) {
             if (r->adaptive) {
                 const int d = r->dist;
-                sel[d].push_back(SelectJob{c->points(), ev.bw, c->bwi, sl.a0});
+                sel[d].push_back(SelectJob{c->points(), ev.bw, c->bwi, sl.a0, c->prune ? c->boxes : nullptr});
                 sel_start[d].push_back(sel_start[d].back() + sl.a1 - sl.a0);
             }
             SweepGroup& g = group_of(sweeps, r->dist, mode, prec, nta, tb);

@@ -69,8 +69,9 @@ hipError_t launch_chunk_max(const double* v, int n, double* out, hipStream_t s);
 
 // Adaptive bandwidth resolution (BandwidthWeight.cpp:40-55): per focus the
 // interpolated distance between the floor(bw)-th and floor(bw)+1-th nearest
-// neighbours, or (bw/n)*max distance when bw >= n.
-hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out, hipStream_t s);
+// neighbours, or (bw/n)*max distance when bw >= n.  With the chunk boxes of
+// spatially ordered points (boxes, nullable) only candidate chunks are read.
+hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out, const double* boxes, hipStream_t s);
 // Maximum pairwise distance (CRSDistance::maxDistance, CRSDistance.cpp:133-143).
 hipError_t launch_max_distance(int dist, const Points& P, double* block_max, int nblocks, hipStream_t s);
 // Distances from one focus to all points (test hook).
@@ -140,7 +141,8 @@ struct SelectJob {
     Points P;
     double bw;
     double* out;
-    int f0;        // first focus of the slice
+    int f0;                 // first focus of the slice
+    const double* boxes;    // chunk boxes (pruning) or null
 };
 // Batched adaptive bandwidths, one workgroup per focus: start[] is the prefix
 // sum of the jobs' focus counts (as for the batched solves).

@@ -67,3 +67,31 @@ def test_pruning_is_exact(geographic, precision):
                 if "error" not in f1:
                     for key in ("betas", "betasSE", "qDiag", "shat"):
                         assert np.array_equal(f1[key], f0[key], equal_nan=True), key
+
+
+@pytest.mark.parametrize("geographic", [False, True])
+def test_pruned_adaptive_select_is_exact(geographic, oracle):
+    """Adaptive bandwidths (rank floor(bw) / floor(bw)+1 order statistics)
+    from the pruned select equal the unpruned select bit for bit and the
+    oracle's sort (Euclidean: bit for bit), with duplicated points (ties) and
+    ranks up to n - 1."""
+    n = 5000
+    c, X, y = clustered(9, n, 2, geographic)
+    c[::7] = c[3]            # a block of identical points: ties at every rank they cover
+    c[1::11] = np.round(c[1::11], 1)
+    dist = capi.GREAT_CIRCLE if geographic else capi.EUCLIDEAN
+    with capi.Context(0) as ctx:
+        ctx.set_data(c, X, y, dist)
+        for bw in (20.0, 20.5, 64.0, 100.25, 700.0, 1000.0, 4999.5, 4998.0):
+            say("case: geographic=%s bw=%g" % (geographic, bw))
+            ctx.set_pruning(True)
+            a = ctx.adaptive_bandwidths(bw)
+            ctx.set_pruning(False)
+            b = ctx.adaptive_bandwidths(bw)
+            assert np.array_equal(a, b)
+            ref = np.array([oracle.adaptive_threshold(c, bw, f, geographic=geographic) for f in range(0, n, 97)])
+            if geographic:
+                # the device's great-circle formula agrees with the reference's to rounding, not bitwise
+                assert np.allclose(a[::97], ref, rtol=1e-12, atol=0)
+            else:
+                assert np.array_equal(a[::97], ref)
