@@ -52,7 +52,7 @@ void dfree(T*& p) {
 void free_problem(gwm_hip_ctx* c) {
     dfree(c->px); dfree(c->py); dfree(c->hx); dfree(c->hy_); dfree(c->hz); dfree(c->sl);
     dfree(c->X); dfree(c->y); dfree(c->Z); dfree(c->G); dfree(c->part);
-    dfree(c->boxes); dfree(c->chunk_bw);
+    dfree(c->boxes); dfree(c->chunk_bw); dfree(c->sel_left); dfree(c->sel_nleft);
     dfree(c->bwi); dfree(c->res2); dfree(c->cond); dfree(c->sii); dfree(c->rlist); dfree(c->rcount);
     dfree(c->rpart);
     c->rpart_bytes = 0; dfree(c->ssi); dfree(c->betas); dfree(c->se);
@@ -168,7 +168,8 @@ int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw
         return GWM_ERR_INVALID;
     }
     timer_begin(c);
-    GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, c->bwi, c->prune ? c->boxes : nullptr, c->stream));
+    GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, c->bwi, c->prune ? c->boxes : nullptr, c->sel_left,
+                                        c->sel_nleft, c->stream));
     int rc = timer_end(c, "select");
     if (rc) return rc;
     *bw_pt = c->bwi;
@@ -779,7 +780,8 @@ int gwm_hip_set_data(gwm_hip_ctx* c, int n, int k, const double* coords, const d
         (rc = dalloc(c, &c->y, N)) || (rc = dalloc(c, &c->Z, (size_t)c->rows_pad * c->nta * 16)) ||
         (rc = dalloc(c, &c->G, (size_t)c->rows_pad * c->ntb * 16)) || (rc = dalloc(c, &c->bwi, N)) ||
         (rc = dalloc(c, &c->res2, N)) || (rc = dalloc(c, &c->cond, N)) ||
-        (rc = dalloc(c, &c->rlist, N)) || (rc = dalloc(c, &c->rcount, 1)) || (rc = dalloc(c, &c->sii, N)) || (rc = dalloc(c, &c->ssi, N)) ||
+        (rc = dalloc(c, &c->rlist, N)) || (rc = dalloc(c, &c->rcount, 1)) || (rc = dalloc(c, &c->sel_left, N)) ||
+        (rc = dalloc(c, &c->sel_nleft, 1)) || (rc = dalloc(c, &c->sii, N)) || (rc = dalloc(c, &c->ssi, N)) ||
         (rc = dalloc(c, &c->betas, N * k)) || (rc = dalloc(c, &c->se, N * k)) || (rc = dalloc(c, &c->gvec, N * k)) ||
         (rc = dalloc(c, &c->qd, N)) || (rc = dalloc(c, &c->red, 8)) || (rc = dalloc(c, &c->blockmax, (N + 255) / 256)) ||
         (rc = dalloc(c, &c->vx, N)) || (rc = dalloc(c, &c->vy, N)) || (rc = dalloc(c, &c->vZ, (size_t)c->rows_pad * 16)) ||

@@ -159,6 +159,9 @@ struct gwm_hip_server {
     Arena arena;
     double* round_red = nullptr;
     int round_cap = 0;
+    int* sel_left = nullptr;           // scratch of the batched adaptive-bandwidth select
+    unsigned int* sel_nleft = nullptr;
+    int sel_cap = 0;
     double round_ms = 25.0;      // work budget of a round; <= 0: whole evaluations
     double refine_crit = 100.0, refine_fit = 2.0;   // FP64 recomputation limits (mixed)
 };
@@ -566,11 +569,20 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
     // (every point the mixed solves failed on is listed)
     for (Request* r : refine_starts) SRV_CHECK(S, hipMemsetAsync(r->dfail, 0, sizeof(int), S->stream));
     // 1. adaptive bandwidths of the main-sweep focus points
+    const int sel_need = std::max(sel_start[0].back(), sel_start[1].back());
+    if (sel_need > S->sel_cap) {
+        if (S->sel_left) hipFree(S->sel_left);
+        S->sel_left = nullptr;
+        S->sel_cap = 0;
+        SRV_CHECK(S, hipMalloc((void**)&S->sel_left, (size_t)sel_need * 2 * sizeof(int)));
+        S->sel_cap = sel_need * 2;
+    }
+    if (!S->sel_nleft) SRV_CHECK(S, hipMalloc((void**)&S->sel_nleft, 2 * sizeof(unsigned int)));
     for (int d = 0; d < 2; ++d)
         if (!sel[d].empty())
             SRV_CHECK(S, launch_adaptive_bw_batch(d, reinterpret_cast<const SelectJob*>(dev + off_sel[d]),
                                                   reinterpret_cast<const int*>(dev + off_sels[d]), (int)sel[d].size(),
-                                                  sel_start[d].back(), S->stream));
+                                                  sel_start[d].back(), S->sel_left, S->sel_nleft + d, S->stream));
     // 2. criterion and hat sweeps
     int rc = launch_sweeps(S, sweeps, dev);
     if (rc) return rc;
@@ -959,6 +971,8 @@ void gwm_hip_server_destroy(gwm_hip_server* S) {
     for (auto& kv : S->reqs) delete kv.second;
     if (S->arena.base) hipFree(S->arena.base);
     if (S->round_red) hipFree(S->round_red);
+    if (S->sel_left) hipFree(S->sel_left);
+    if (S->sel_nleft) hipFree(S->sel_nleft);
     if (S->stream) hipStreamDestroy(S->stream);
     delete S;
 }

@@ -71,7 +71,10 @@ hipError_t launch_chunk_max(const double* v, int n, double* out, hipStream_t s);
 // interpolated distance between the floor(bw)-th and floor(bw)+1-th nearest
 // neighbours, or (bw/n)*max distance when bw >= n.  With the chunk boxes of
 // spatially ordered points (boxes, nullable) only candidate chunks are read.
-hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out, const double* boxes, hipStream_t s);
+// left (n ints) and nleft (one counter) are scratch for the points the
+// one-wave fast path leaves to the workgroup select.
+hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out, const double* boxes, int* left,
+                              unsigned int* nleft, hipStream_t s);
 // Maximum pairwise distance (CRSDistance::maxDistance, CRSDistance.cpp:133-143).
 hipError_t launch_max_distance(int dist, const Points& P, double* block_max, int nblocks, hipStream_t s);
 // Distances from one focus to all points (test hook).
@@ -144,10 +147,10 @@ struct SelectJob {
     int f0;                 // first focus of the slice
     const double* boxes;    // chunk boxes (pruning) or null
 };
-// Batched adaptive bandwidths, one workgroup per focus: start[] is the prefix
-// sum of the jobs' focus counts (as for the batched solves).
+// Batched adaptive bandwidths: start[] is the prefix sum of the jobs' focus
+// counts (as for the batched solves); left (total ints) and nleft are scratch.
 hipError_t launch_adaptive_bw_batch(int dist, const SelectJob* jobs, const int* start, int njobs, int total,
-                                    hipStream_t s);
+                                    int* left, unsigned int* nleft, hipStream_t s);
 
 // qDiag_j = 1 - 2 S_jj + x_j^T M_j x_j, M_j from the kModeQDiag contraction.
 hipError_t launch_qdiag(int k, const double* part, int nseg, int PT, int n, int n_pad, const double* X,

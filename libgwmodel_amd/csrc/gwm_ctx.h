@@ -42,6 +42,8 @@ struct gwm_hip_ctx {
     bool prune = true;             // skip data chunks whose weights are all exactly zero
     double* boxes = nullptr;       // bounding boxes of the 64-point chunks (sorted order)
     double* chunk_bw = nullptr;    // largest per-point bandwidth of each chunk (qDiag sweeps)
+    int* sel_left = nullptr;       // [n] points the one-wave select leaves to the workgroup select
+    unsigned int* sel_nleft = nullptr;
 
     // device buffers
     double *px = nullptr, *py = nullptr, *hx = nullptr, *hy_ = nullptr, *hz = nullptr, *sl = nullptr;

@@ -23,6 +23,8 @@
 // farther than U), and every point with a key <= U lies in a chunk whose
 // lower bound is <= U.  The two order statistics are therefore the same among
 // the candidates as among all n points.
+#include <algorithm>
+
 #include "gwm_contract.h"
 
 namespace gwm {
@@ -72,6 +74,55 @@ __device__ __forceinline__ void key_bounds(const PTS& P, int f, GWM_GLOBAL const
     }
 }
 
+// One radix digit of a 256-thread select: hist[0..nb) holds the counts of
+// the keys matching `prefix` above the digit; the bin holding 0-based rank
+// `rank` is located by a block-wide scan of per-thread bin groups and its
+// value appended to the prefix.  Results in s_prefix, s_rank (rank within
+// the bin) and s_count (keys in the bin), valid after the final barrier.
+__device__ __forceinline__ void resolve_digit(const unsigned int* hist, int nb, long long rank,
+                                              unsigned long long prefix, int shift, unsigned long long* s_prefix,
+                                              long long* s_rank, unsigned int* s_count) {
+    __shared__ unsigned int s_wsum[4];
+    __shared__ int s_group;
+    __shared__ long long s_rem;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = nb / 256;
+    unsigned int c = 0u;
+    for (int i = 0; i < per; ++i) c += hist[tid * per + i];
+    unsigned int x = c;   // inclusive scan over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_wsum[wave] = x;
+    if (tid == 0) {
+        s_group = 255;
+        s_rem = 0;
+    }
+    __syncthreads();
+    long long off = 0;
+    for (int w = 0; w < wave; ++w) off += s_wsum[w];
+    const long long incl = off + x, excl = incl - c;
+    if (excl <= rank && rank < incl) {
+        s_group = tid;
+        s_rem = rank - excl;
+    }
+    __syncthreads();
+    if (tid == s_group) {
+        long long r = s_rem;
+        int bin = tid * per;
+        for (int i = 0; i < per - 1; ++i, ++bin) {
+            if (r < (long long)hist[bin]) break;
+            r -= hist[bin];
+        }
+        *s_prefix = prefix | ((unsigned long long)bin << shift);
+        *s_rank = r;
+        *s_count = hist[bin];
+    }
+    __syncthreads();
+}
+
 template <int DIST, class PTS, class OUT>
 __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out, int f,
                                                 GWM_GLOBAL const double* boxes) {
@@ -79,7 +130,6 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
     const int tid = threadIdx.x;
     constexpr int BD = DIST == kEuclidean ? 4 : 6;   // doubles per chunk box
     __shared__ unsigned int hist[kSelBins];
-    __shared__ unsigned int chunk_sum[256];
     __shared__ unsigned long long cand[kSelCap];
     __shared__ unsigned long long s_prefix, s_above, s_next;
     __shared__ long long s_rank;
@@ -126,28 +176,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
                 if (hs == 64 || (key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & 4095ull], 1u);
             }
             __syncthreads();
-            {
-                unsigned int sum = 0u;
-                for (int i = 0; i < 16; ++i) sum += hist[tid * 16 + i];
-                chunk_sum[tid] = sum;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                long long r = rank;
-                int c = 0;
-                for (; c < 255; ++c) {
-                    if (r < (long long)chunk_sum[c]) break;
-                    r -= chunk_sum[c];
-                }
-                int bin = c * 16;
-                for (int i = 0; i < 15; ++i, ++bin) {
-                    if (r < (long long)hist[bin]) break;
-                    r -= hist[bin];
-                }
-                s_prefix = prefix | ((unsigned long long)bin << shift);
-                s_rank = r;
-            }
-            __syncthreads();
+            resolve_digit(hist, kSelBins, rank, prefix, shift, &s_prefix, &s_rank, &s_count);
             prefix = s_prefix;
             rank = s_rank;
             __syncthreads();
@@ -174,13 +203,26 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
             for (int j = tid; j < n; j += 256) fn(key_of(select_value<DIST>(P, f, j)));
         }
     };
+    bool in_lds = false;
+    int ncand = 0;
+    if (pruned && nclist * 64 <= kSelCap) {
+        // few candidates: hold all their keys in LDS (every key up to rank
+        // floor(bw) is among them, so nothing lies "above" the set that matters)
+        if (tid == 0) {
+            s_ncand = 0u;
+            s_above = ~0ull;
+        }
+        __syncthreads();
+        for_each_key([&](unsigned long long key) { cand[atomicAdd(&s_ncand, 1u)] = key; });
+        __syncthreads();
+        ncand = (int)s_ncand;
+        in_lds = true;
+    }
     if (tid == 0) {
         s_prefix = 0ull;
         s_rank = (long long)b0 - 1;   // 0-based rank of d0
     }
     int shift = 64;
-    bool in_lds = false;
-    int ncand = 0;
     while (shift > 0) {
         const int width = shift > 40 ? 12 : 8;   // 12, 12, then 8 x 5
         shift -= width;
@@ -193,7 +235,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
         if (in_lds) {
             for (int i = tid; i < ncand; i += 256) {
                 const unsigned long long key = cand[i];
-                if ((key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & mask], 1u);
+                if (hs == 64 || (key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & mask], 1u);
             }
         } else {
             for_each_key([&](unsigned long long key) {
@@ -201,31 +243,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
             });
         }
         __syncthreads();
-        // the bin holding rank s_rank: per-thread chunk sums, then a scan by thread 0
-        const int per = nb / 256;
-        {
-            unsigned int sum = 0u;
-            for (int i = 0; i < per; ++i) sum += hist[tid * per + i];
-            chunk_sum[tid] = sum;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            long long rank = s_rank;
-            int c = 0;
-            for (; c < 255; ++c) {
-                if (rank < (long long)chunk_sum[c]) break;
-                rank -= chunk_sum[c];
-            }
-            int bin = c * per;
-            for (int i = 0; i < per - 1; ++i, ++bin) {
-                if (rank < (long long)hist[bin]) break;
-                rank -= hist[bin];
-            }
-            s_prefix = prefix | ((unsigned long long)bin << shift);
-            s_rank = rank;
-            s_count = hist[bin];
-        }
-        __syncthreads();
+        resolve_digit(hist, nb, s_rank, prefix, shift, &s_prefix, &s_rank, &s_count);
         if (!in_lds && shift > 0 && s_count <= (unsigned int)kSelCap) {
             // gather the bin (its keys share the top 64 - shift bits) and the
             // smallest key above it
@@ -279,18 +297,237 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
     }
 }
 
+// ---------------------------------------------------------------- one wave
+//
+// Fast path for pruned selections with few candidates (the usual adaptive
+// bandwidths, tens to hundreds of neighbours): one wave per focus point, no
+// workgroup barriers.  U comes from two 8-bit radix digits of the chunk upper
+// bounds (the upper edge of a 16-bit bin), the candidate chunks' keys (at most
+// kWaveKeys, padded with the largest key) are gathered into the wave's LDS,
+// and eight 8-bit radix digits resolve the two order statistics there.  A
+// focus point outside this case is appended to a list that the workgroup
+// select (adaptive_bw_one) then works through with a fixed grid.
+constexpr int kWaveKeys = 1024;
+constexpr int kWaveChunks = kWaveKeys / 64;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inclusive prefix sum over the 64 lanes.
+__device__ __forceinline__ unsigned int wave_scan(unsigned int x, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+// One 8-bit digit over a 256-bin histogram in LDS: the bin holding `rank`;
+// updates prefix / rank and returns the bin's count.
+__device__ __forceinline__ unsigned int wave_digit(const unsigned int* hist, int lane, int shift,
+                                                   unsigned long long& prefix, long long& rank) {
+    unsigned int c[4], sum = 0u;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        c[i] = hist[4 * lane + i];
+        sum += c[i];
+    }
+    const unsigned int incl = wave_scan(sum, lane);
+    const unsigned int excl = incl - sum;
+    const bool mine = (long long)excl <= rank && rank < (long long)incl;
+    const unsigned long long who = __ballot(mine);
+    const int src = who ? __builtin_ctzll(who) : 63;
+    int bin = 4 * lane;
+    long long r = rank - (long long)excl;
+    unsigned int cnt = c[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (r < (long long)c[i]) {
+            cnt = c[i];
+            break;
+        }
+        r -= c[i];
+        ++bin;
+    }
+    bin = __shfl(bin, src);
+    r = __shfl(r, src);
+    cnt = __shfl(cnt, src);
+    prefix |= (unsigned long long)bin << shift;
+    rank = r;
+    return cnt;
+}
+
+// Returns false (nothing written) when the point is left to the workgroup select.
+template <int DIST, class PTS, class OUT>
+__device__ __forceinline__ bool adaptive_bw_wave(const PTS& P, double bw, OUT out, int f,
+                                                 GWM_GLOBAL const double* boxes, unsigned int* hist,
+                                                 unsigned long long* keys, int* clist) {
+    const int n = P.n;
+    const int lane = threadIdx.x & 63;
+    constexpr int BD = DIST == kEuclidean ? 4 : 6;
+    const int nch = (n + 63) / 64, nfull = n / 64;
+    const double b0 = floor(bw), bx = bw - b0;
+    const long long need = (long long)b0 + 1;
+    if (!(bw / n < 1) || !boxes || b0 < 1 || need > kWaveKeys || need > 64LL * nfull) return false;
+    // U: upper edge of the 16-bit bin holding the m-th smallest chunk upper bound
+    unsigned long long prefix = 0ull;
+    long long rank = (need + 63) / 64 - 1;
+    int shift = 64;
+    for (int pass = 0; pass < 2; ++pass) {
+        shift -= 8;
+        const int hs = shift + 8;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) hist[4 * lane + i] = 0u;
+        wave_sync();
+        for (int c = lane; c < nfull; c += 64) {
+            double lb, ub;
+            key_bounds<DIST>(P, f, boxes + (size_t)c * BD, &lb, &ub);
+            const unsigned long long key = key_of(ub);
+            if (hs == 64 || (key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & 255ull], 1u);
+        }
+        wave_sync();
+        wave_digit(hist, lane, shift, prefix, rank);
+        wave_sync();
+    }
+    const double U = val_of(prefix | ((1ull << shift) - 1ull));
+    // candidate chunks, in chunk order
+    int ncl = 0;
+    bool overflow = false;
+    for (int c0 = 0; c0 < nch; c0 += 64) {
+        const int c = c0 + lane;
+        bool take = false;
+        if (c < nch) {
+            double lb, ub;
+            key_bounds<DIST>(P, f, boxes + (size_t)c * BD, &lb, &ub);
+            take = lb <= U;
+        }
+        const unsigned long long m = __ballot(take);
+        const int pos = ncl + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (take && pos < kWaveChunks) clist[pos] = c;
+        ncl += __popcll(m);
+        if (ncl > kWaveChunks) {
+            overflow = true;
+            break;
+        }
+    }
+    if (overflow) return false;
+    wave_sync();
+    // the candidates' keys; padding keys are the largest value
+    const int nk = ncl * 64;
+    for (int e = lane; e < nk; e += 64) {
+        const int j = clist[e >> 6] * 64 + (e & 63);
+        keys[e] = j < n ? key_of(select_value<DIST>(P, f, j)) : ~0ull;
+    }
+    // eight 8-bit digits: rank floor(bw) - 1
+    prefix = 0ull;
+    rank = (long long)b0 - 1;
+    shift = 64;
+    unsigned int cnt = 0u;
+    while (shift > 0) {
+        shift -= 8;
+        const int hs = shift + 8;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) hist[4 * lane + i] = 0u;
+        wave_sync();
+        for (int e = lane; e < nk; e += 64) {
+            const unsigned long long key = keys[e];
+            if (hs == 64 || (key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & 255ull], 1u);
+        }
+        wave_sync();
+        cnt = wave_digit(hist, lane, shift, prefix, rank);
+        wave_sync();
+    }
+    const unsigned long long key0 = prefix;
+    unsigned long long key1 = key0;
+    if (!(rank + 1 < (long long)cnt)) {
+        unsigned long long best = ~0ull;
+        for (int e = lane; e < nk; e += 64) {
+            const unsigned long long key = keys[e];
+            if (key > key0 && key < best) best = key;
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const unsigned long long o = __shfl_xor(best, d);
+            best = o < best ? o : best;
+        }
+        key1 = best;
+    }
+    if (lane == 0) {
+        const double d0 = select_to_distance<DIST>(val_of(key0));
+        const double d1 = select_to_distance<DIST>(val_of(key1));
+        out[f] = d0 + (d1 - d0) * bx;
+    }
+    return true;
+}
+
+// LDS of the four waves of a fast-path workgroup.
+struct WaveSelectShared {
+    unsigned int hist[4][256];
+    unsigned long long keys[4][kWaveKeys];
+    int clist[4][kWaveChunks];
+};
+
+// Fast path over points 0..n-1 (entries 0..total-1 of a batch); the points
+// it leaves are appended to left[] (*nleft must be zero beforehand).
 template <int DIST>
-__global__ __launch_bounds__(256) void adaptive_bw_kernel(Points P, double bw, double* out, const double* boxes) {
-    adaptive_bw_one<DIST>(global_points(P), bw, gptr(out), blockIdx.x, gptr(boxes));
+__global__ __launch_bounds__(256) void adaptive_bw_wave_kernel(Points P, double bw, double* out, const double* boxes,
+                                                               int* left, unsigned int* nleft) {
+    __shared__ WaveSelectShared sh;
+    const int w = threadIdx.x >> 6;
+    const int f = blockIdx.x * 4 + w;
+    if (f >= P.n) return;
+    if (!adaptive_bw_wave<DIST>(global_points(P), bw, gptr(out), f, gptr(boxes), sh.hist[w], sh.keys[w],
+                                sh.clist[w]) &&
+        (threadIdx.x & 63) == 0)
+        left[atomicAdd(nleft, 1u)] = f;
 }
 
 template <int DIST>
-__global__ __launch_bounds__(256) void adaptive_bw_batch_kernel(const SelectJob* jobs, const int* start, int njobs) {
-    const int t = blockIdx.x;
+__global__ __launch_bounds__(256) void adaptive_bw_wave_batch_kernel(const SelectJob* jobs, const int* start,
+                                                                     int njobs, int* left, unsigned int* nleft) {
+    __shared__ WaveSelectShared sh;
+    const int w = threadIdx.x >> 6;
+    const int t = blockIdx.x * 4 + w;
+    if (t >= start[njobs]) return;
     const int j = find_job(start, njobs, t);
     const SelectJob J = jobs[j];
-    adaptive_bw_one<DIST>(global_points(J.P), J.bw, gptr(J.out), J.f0 + t - start[j], gptr(J.boxes));
+    if (!adaptive_bw_wave<DIST>(global_points(J.P), J.bw, gptr(J.out), J.f0 + t - start[j], gptr(J.boxes),
+                                sh.hist[w], sh.keys[w], sh.clist[w]) &&
+        (threadIdx.x & 63) == 0)
+        left[atomicAdd(nleft, 1u)] = t;
 }
+
+// Workgroup select over the listed points (every point when the list is
+// null), one point per workgroup iteration on a fixed grid.
+template <int DIST>
+__global__ __launch_bounds__(256) void adaptive_bw_kernel(Points P, double bw, double* out, const double* boxes,
+                                                          const int* left, const unsigned int* nleft) {
+    const unsigned int total = left ? *nleft : (unsigned int)P.n;
+    for (unsigned int i = blockIdx.x; i < total; i += gridDim.x) {
+        const int f = left ? left[i] : (int)i;
+        adaptive_bw_one<DIST>(global_points(P), bw, gptr(out), f, gptr(boxes));
+        __syncthreads();
+    }
+}
+
+template <int DIST>
+__global__ __launch_bounds__(256) void adaptive_bw_batch_kernel(const SelectJob* jobs, const int* start, int njobs,
+                                                                const int* left, const unsigned int* nleft) {
+    const unsigned int total = *nleft;
+    for (unsigned int i = blockIdx.x; i < total; i += gridDim.x) {
+        const int t = left[i];
+        const int j = find_job(start, njobs, t);
+        const SelectJob J = jobs[j];
+        adaptive_bw_one<DIST>(global_points(J.P), J.bw, gptr(J.out), J.f0 + t - start[j], gptr(J.boxes));
+        __syncthreads();
+    }
+}
+
+constexpr int kFallbackGrid = 2048;
 
 template <int DIST>
 __global__ __launch_bounds__(256) void max_distance_kernel(Points P, double* block_max) {
@@ -317,21 +554,46 @@ __global__ void distance_row_kernel(Points P, int focus, double* out) {
 
 }  // namespace
 
-hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out, const double* boxes, hipStream_t s) {
-    if (dist == kEuclidean)
-        hipLaunchKernelGGL(adaptive_bw_kernel<kEuclidean>, dim3(P.n), dim3(256), 0, s, P, bw, out, boxes);
-    else
-        hipLaunchKernelGGL(adaptive_bw_kernel<kGreatCircle>, dim3(P.n), dim3(256), 0, s, P, bw, out, boxes);
+hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out, const double* boxes, int* left,
+                              unsigned int* nleft, hipStream_t s) {
+    const dim3 wgrid((P.n + 3) / 4), fgrid(std::min(P.n, kFallbackGrid));
+    const int* lst = boxes ? left : nullptr;
+    if (boxes) {
+        hipError_t e = hipMemsetAsync(nleft, 0, sizeof(unsigned int), s);
+        if (e != hipSuccess) return e;
+    }
+    if (dist == kEuclidean) {
+        if (boxes)
+            hipLaunchKernelGGL(adaptive_bw_wave_kernel<kEuclidean>, wgrid, dim3(256), 0, s, P, bw, out, boxes, left,
+                               nleft);
+        hipLaunchKernelGGL(adaptive_bw_kernel<kEuclidean>, fgrid, dim3(256), 0, s, P, bw, out, boxes, lst, nleft);
+    } else {
+        if (boxes)
+            hipLaunchKernelGGL(adaptive_bw_wave_kernel<kGreatCircle>, wgrid, dim3(256), 0, s, P, bw, out, boxes, left,
+                               nleft);
+        hipLaunchKernelGGL(adaptive_bw_kernel<kGreatCircle>, fgrid, dim3(256), 0, s, P, bw, out, boxes, lst, nleft);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_adaptive_bw_batch(int dist, const SelectJob* jobs, const int* start, int njobs, int total,
-                                    hipStream_t s) {
+                                    int* left, unsigned int* nleft, hipStream_t s) {
     if (total <= 0) return hipSuccess;
-    if (dist == kEuclidean)
-        hipLaunchKernelGGL(adaptive_bw_batch_kernel<kEuclidean>, dim3(total), dim3(256), 0, s, jobs, start, njobs);
-    else
-        hipLaunchKernelGGL(adaptive_bw_batch_kernel<kGreatCircle>, dim3(total), dim3(256), 0, s, jobs, start, njobs);
+    // the fast path runs for every entry; jobs without boxes leave all their points to the workgroup select
+    const dim3 wgrid((total + 3) / 4), fgrid(std::min(total, kFallbackGrid));
+    hipError_t e = hipMemsetAsync(nleft, 0, sizeof(unsigned int), s);
+    if (e != hipSuccess) return e;
+    if (dist == kEuclidean) {
+        hipLaunchKernelGGL(adaptive_bw_wave_batch_kernel<kEuclidean>, wgrid, dim3(256), 0, s, jobs, start, njobs, left,
+                           nleft);
+        hipLaunchKernelGGL(adaptive_bw_batch_kernel<kEuclidean>, fgrid, dim3(256), 0, s, jobs, start, njobs, left,
+                           nleft);
+    } else {
+        hipLaunchKernelGGL(adaptive_bw_wave_batch_kernel<kGreatCircle>, wgrid, dim3(256), 0, s, jobs, start, njobs,
+                           left, nleft);
+        hipLaunchKernelGGL(adaptive_bw_batch_kernel<kGreatCircle>, fgrid, dim3(256), 0, s, jobs, start, njobs, left,
+                           nleft);
+    }
     return hipGetLastError();
 }
 
