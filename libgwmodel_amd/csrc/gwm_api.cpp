@@ -53,6 +53,8 @@ void free_problem(gwm_hip_ctx* c) {
     dfree(c->px); dfree(c->py); dfree(c->hx); dfree(c->hy_); dfree(c->hz); dfree(c->sl);
     dfree(c->X); dfree(c->y); dfree(c->Z); dfree(c->G); dfree(c->part);
     dfree(c->boxes); dfree(c->chunk_bw); dfree(c->sel_left); dfree(c->sel_nleft);
+    for (auto& e : c->bw_cache) dfree(e.buf);
+    c->bw_cache.clear();
     dfree(c->bwi); dfree(c->res2); dfree(c->cond); dfree(c->sii); dfree(c->rlist); dfree(c->rcount);
     dfree(c->rpart);
     c->rpart_bytes = 0; dfree(c->ssi); dfree(c->betas); dfree(c->se);
@@ -159,7 +161,8 @@ int timer_end(gwm_hip_ctx* c, const char* name) {
     return GWM_OK;
 }
 
-// Resolve the bandwidth: adaptive -> per-focus thresholds in c->bwi.
+// Resolve the bandwidth: adaptive -> per-focus thresholds (a vector of the
+// bandwidth cache, filled by the select on a miss).
 int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw_pt) {
     *bw_pt = nullptr;
     if (!adaptive) return GWM_OK;
@@ -167,12 +170,33 @@ int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw
         c->err = "adaptive bandwidth must be >= 1 neighbour";
         return GWM_ERR_INVALID;
     }
+    for (auto& e : c->bw_cache)
+        if (e.bw == bw) {
+            e.used = ++c->bw_clock;
+            *bw_pt = e.buf;
+            return GWM_OK;
+        }
+    const size_t cap = std::min<size_t>(64, std::max<size_t>(4, (256u << 20) / ((size_t)c->n * sizeof(double))));
+    gwm_hip_ctx::BwEntry* slot = nullptr;
+    if (c->bw_cache.size() < cap) {
+        double* buf = nullptr;
+        GWM_HIP_CHECK(c, hipMalloc((void**)&buf, (size_t)c->n * sizeof(double)));
+        c->bw_cache.push_back(gwm_hip_ctx::BwEntry{bw, buf, 0});
+        slot = &c->bw_cache.back();
+    } else {
+        slot = &c->bw_cache[0];
+        for (auto& e : c->bw_cache)
+            if (e.used < slot->used) slot = &e;
+    }
+    slot->bw = NAN;   // not valid until the select succeeds
     timer_begin(c);
-    GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, c->bwi, c->prune ? c->boxes : nullptr, c->sel_left,
-                                        c->sel_nleft, c->stream));
+    GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, slot->buf, c->prune ? c->boxes : nullptr,
+                                        c->sel_left, c->sel_nleft, c->stream));
     int rc = timer_end(c, "select");
     if (rc) return rc;
-    *bw_pt = c->bwi;
+    slot->bw = bw;
+    slot->used = ++c->bw_clock;
+    *bw_pt = slot->buf;
     return GWM_OK;
 }
 
@@ -1010,7 +1034,7 @@ int gwm_hip_adaptive_bandwidths(gwm_hip_ctx* c, double bw, double* out) {
     GWM_HIP_CHECK(c, hipSetDevice(c->device));
     const double* bw_pt;
     if ((rc = resolve_bandwidth(c, 1, bw, &bw_pt))) return rc;
-    return download_caller(c, c->bwi, out, 1);
+    return download_caller(c, bw_pt, out, 1);
 }
 
 int gwm_hip_var_criterion(gwm_hip_ctx* c, const double* x, const double* y, int kernel, int adaptive, double bw,

@@ -42,6 +42,16 @@ struct gwm_hip_ctx {
     bool prune = true;             // skip data chunks whose weights are all exactly zero
     double* boxes = nullptr;       // bounding boxes of the 64-point chunks (sorted order)
     double* chunk_bw = nullptr;    // largest per-point bandwidth of each chunk (qDiag sweeps)
+    // adaptive bandwidth vectors of recently used bandwidths (least recently
+    // used replaced): searches revisit bandwidths (FP64 confirmations, the
+    // fixed first candidates of every multiscale search)
+    struct BwEntry {
+        double bw;
+        double* buf;
+        unsigned long long used;
+    };
+    std::vector<BwEntry> bw_cache;
+    unsigned long long bw_clock = 0;
     int* sel_left = nullptr;       // [n] points the one-wave select leaves to the workgroup select
     unsigned int* sel_nleft = nullptr;
 

@@ -157,6 +157,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
     const long long need = (long long)b0 + 1;   // keys up to rank floor(bw) are needed
     const bool pruned = boxes && nch <= kSelChunks && need <= 64LL * nfull;
     int nclist = 0;
+    double U = 0.0;
     if (pruned) {
         // U: upper edge of the 24-bit radix bin holding the m-th smallest
         // chunk upper bound over the full chunks
@@ -181,7 +182,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
             rank = s_rank;
             __syncthreads();
         }
-        const double U = val_of(prefix | ((1ull << shift) - 1ull));
+        U = val_of(prefix | ((1ull << shift) - 1ull));
         if (tid == 0) s_nclist = 0u;
         __syncthreads();
         for (int c = tid; c < nch; c += 256) {
@@ -218,9 +219,45 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
         ncand = (int)s_ncand;
         in_lds = true;
     }
+    long long rank0 = (long long)b0 - 1;   // 0-based rank of d0 (within the LDS keys, once gathered)
+    if (pruned && !in_lds && U > 0.0) {
+        // many candidates: one pass of kSelBins linear bins over [0, U] (the
+        // wanted keys are <= U; the map is monotone, larger keys share the last
+        // bin), then gather the bin holding the rank when it fits in LDS
+        const double scale = (double)kSelBins / U;
+        auto lbin = [&](unsigned long long key) {
+            const double b = val_of(key) * scale;
+            return b < (double)(kSelBins - 1) ? (int)b : kSelBins - 1;
+        };
+        for (int i = tid; i < kSelBins; i += 256) hist[i] = 0u;
+        __syncthreads();
+        for_each_key([&](unsigned long long key) { atomicAdd(&hist[lbin(key)], 1u); });
+        __syncthreads();
+        resolve_digit(hist, kSelBins, rank0, 0ull, 0, &s_prefix, &s_rank, &s_count);
+        const int bin = (int)s_prefix;
+        if (s_count <= (unsigned int)kSelCap) {
+            if (tid == 0) {
+                s_ncand = 0u;
+                s_above = ~0ull;
+            }
+            __syncthreads();
+            unsigned long long above = ~0ull;
+            for_each_key([&](unsigned long long key) {
+                const int b = lbin(key);
+                if (b == bin) cand[atomicAdd(&s_ncand, 1u)] = key;
+                else if (b > bin && key < above) above = key;
+            });
+            atomicMin(&s_above, above);
+            __syncthreads();
+            ncand = (int)s_ncand;
+            rank0 = s_rank;
+            in_lds = true;
+        }
+        __syncthreads();
+    }
     if (tid == 0) {
         s_prefix = 0ull;
-        s_rank = (long long)b0 - 1;   // 0-based rank of d0
+        s_rank = rank0;
     }
     int shift = 64;
     while (shift > 0) {
