@@ -15,8 +15,10 @@ golden-section decisions); the same step is run on the FP64 path for the
 mixed/FP64 ratio and the beta error.  Inputs are resident on the device
 before the timed region starts.
 
-Two more workloads ride along in the same JSON line (keys "batch" and
-"serve"): 512 uneven problems (n log-uniform in [200, 20000], k in 3..12,
+Three more workloads ride along in the same JSON line (keys "mgwr", "batch"
+and "serve"): GWRMultiscale backfitting at n = 50,000, k = 6 (bisquare
+adaptive bandwidths chosen by CV for every parameter, CVR threshold 1e-5) in
+mixed precision and FP64; 512 uneven problems (n log-uniform in [200, 20000], k in 3..12,
 Gaussian / bisquare / tricube, adaptive or fixed), each a CV golden-section
 search plus fit, run through the batching server against a loop of
 single-problem calls; and a Poisson stream of fit and CV-search requests
@@ -106,6 +108,33 @@ def run_batch(capi, synth, num, loop):
     return out
 
 
+def run_mgwr(capi, synth, n):
+    """Config-3 multiscale backfitting, mixed precision and FP64."""
+    d = synth.config3(n=n)
+    k = d["X"].shape[1]
+    out = {"n": n, "k": k}
+    res = {}
+    with capi.Context(0) as ctx:
+        ctx.set_data(d["coords"], d["X"], d["y"])
+        for name, prec in (("mixed", capi.MIXED), ("fp64", capi.FP64)):
+            log("mgwr %s n=%d" % (name, n))
+            ctx.reset_timers()
+            t0 = time.perf_counter()
+            r = ctx.mgwr_run([capi.BISQUARE] * k, [1] * k, [0.0] * k, [0] * k, [capi.CRIT_CV] * k,
+                             [0] + [1] * (k - 1), [1e-5] * k, crit_type=0, crit_threshold=1e-5, precision=prec)
+            dt = time.perf_counter() - t0
+            res[name] = r
+            out[name + "_seconds"] = dt
+            out[name + "_solves_per_s"] = ctx.solve_count() / dt
+            out[name + "_iterations"] = r["iterations"]
+    out["bandwidths"] = [float(b) for b in res["mixed"]["bandwidths"]]
+    out["bandwidths_identical"] = bool(np.array_equal(res["mixed"]["bandwidths"], res["fp64"]["bandwidths"]))
+    out["beta_rel_err_vs_fp64"] = beta_rel_err(res["mixed"]["betas"], res["fp64"]["betas"])
+    out["aicc_rel_err_vs_fp64"] = abs(res["mixed"]["diagnostic"]["AICc"] - res["fp64"]["diagnostic"]["AICc"]) / \
+        abs(res["fp64"]["diagnostic"]["AICc"])
+    return out
+
+
 def serve_submit(capi, srv, r):
     n = r["X"].shape[0]
     if r["search"]:
@@ -172,6 +201,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=512, help="focus points of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mgwr-n", type=int, default=50_000, help="config-3 multiscale size (0: skip)")
     ap.add_argument("--batch-problems", type=int, default=512, help="config-4 batch size (0: skip)")
     ap.add_argument("--no-loop", action="store_true", help="skip the single-problem loop of the batch")
     ap.add_argument("--serve-requests", type=int, default=64, help="config-5 stream length (0: skip)")
@@ -180,6 +210,8 @@ def main():
                     help="requests of each type re-run in FP64 for the beta error")
     ap.add_argument("--sweeps-only", type=int, default=0,
                     help="profiling mode: run this many mixed AICc sweeps (and as many FP64 ones) and exit")
+    ap.add_argument("--sweep-bw", type=float, default=61.890021059082855,
+                    help="bandwidth (km) of the profiling sweeps (default: the selected one)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="profiling mode: one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -212,7 +244,7 @@ def main():
     if args.sweeps_only:
         for prec in (capi.MIXED, capi.FP64):
             for _ in range(args.sweeps_only):
-                ctx.criterion(capi.GAUSSIAN, False, 100.0, capi.CRIT_AIC, prec)
+                ctx.criterion(capi.GAUSSIAN, False, args.sweep_bw, capi.CRIT_AIC, prec)
         print(json.dumps({"sweeps": args.sweeps_only, "n": n}))
         ctx.close()
         return
@@ -233,6 +265,7 @@ def main():
     ctx.set_timing(False)
     solves_mixed = ctx.solve_count()
     launches, ms = ctx.kernel_time("contract_mixed_plain")
+    swept_pairs = ctx.kernel_pairs("contract_mixed_plain")
     l64_in_mixed, _ = ctx.kernel_time("contract_fp64_plain")
     hat_l, hat_ms = ctx.kernel_time("contract_mixed_hat")
     value = solves_mixed / t_mixed
@@ -247,13 +280,19 @@ def main():
     ctx.set_timing(False)
     solves_fp64 = ctx.solve_count()
     l64, ms64 = ctx.kernel_time("contract_fp64_plain")
+    swept64 = ctx.kernel_pairs("contract_fp64_plain")
     value_fp64 = solves_fp64 / t_fp64
     ctx.close()
 
     P = k * (k + 1) // 2 + k
     flops_per_launch = 2.0 * n * n * P          # n focus x n data x P columns, multiply-add = 2
     avg_ms = ms / max(launches, 1)
-    achieved = flops_per_launch / (avg_ms * 1e-3)
+    # executed work: the pairs the sweeps processed (pruning skips chunks whose
+    # weights are exactly zero); algorithmic: all n^2 pairs
+    swept_per_launch = swept_pairs / max(launches, 1)
+    executed_flops = 2.0 * swept_per_launch * P
+    achieved = executed_flops / (avg_ms * 1e-3)
+    effective = flops_per_launch / (avg_ms * 1e-3)
     avg_ms64 = ms64 / max(l64, 1)
     traffic = load_traffic("contract_mixed_plain_n%d" % n)
 
@@ -266,6 +305,7 @@ def main():
                "sample": "%d focus points x n=%d data, one AICc evaluation at the selected bandwidth, "
                          "great-circle Gaussian, OpenMP (%.1f s)" % (args.cpu_sample, n, dt)}
 
+    mgwr = run_mgwr(capi, synth, args.mgwr_n) if args.mgwr_n else None
     batch = run_batch(capi, synth, args.batch_problems, not args.no_loop) if args.batch_problems else None
     serve = run_serve(capi, synth, args.serve_requests, args.serve_rate, args.serve_check) \
         if args.serve_requests else None
@@ -301,14 +341,19 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK["mixed"],
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "kernel": "contract_mixed_kernel (AICc sweep)", "avg_launch_ms": avg_ms,
-                     "flops_per_launch": flops_per_launch,
+                     "flops_per_launch": executed_flops,
+                     "swept_pair_fraction": swept_per_launch / (float(n) * n),
+                     "effective_tflops_all_pairs": effective / 1e12,
                      "fp64_kernel_avg_launch_ms": avg_ms64,
-                     "fp64_frac_of_fp64_peak": flops_per_launch / (avg_ms64 * 1e-3) / PEAK["fp64"] if l64 else None},
+                     "fp64_frac_of_fp64_peak": 2.0 * (swept64 / max(l64, 1)) * P / (avg_ms64 * 1e-3) / PEAK["fp64"]
+                     if l64 else None,
+                     "fp64_swept_pair_fraction": swept64 / max(l64, 1) / (float(n) * n)},
         "cpu_baseline": cpu,
         "serve_p99_ms": serve["p99_ms"] if serve else None,
         "serve_p50_ms": serve["p50_ms"] if serve else None,
         "serve": serve,
         "batch": batch,
+        "mgwr": mgwr,
     }
     print(json.dumps(line), flush=True)
 

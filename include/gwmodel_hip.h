@@ -163,6 +163,9 @@ int gwm_hip_last_condition(gwm_hip_ctx* ctx, double* cond, long long* refined);
  * hat|qdiag>" for the sweeps, "select" (adaptive bandwidths), "solve". */
 int gwm_hip_set_timing(gwm_hip_ctx* ctx, int enabled);
 int gwm_hip_kernel_time(gwm_hip_ctx* ctx, const char* name, long long* launches, double* total_ms);
+/* Focus x data pairs the named sweeps actually processed while timing was on
+ * (spatial pruning skips the rest); padding lanes and points count. */
+int gwm_hip_kernel_pairs(gwm_hip_ctx* ctx, const char* name, double* pairs);
 int gwm_hip_reset_timers(gwm_hip_ctx* ctx);
 /* Local solves (focus x bandwidth pairs) evaluated since the last reset. */
 int gwm_hip_solve_count(gwm_hip_ctx* ctx, long long* solves);

@@ -54,6 +54,7 @@ SIGNATURES = {
     "gwm_hip_set_certify_tolerance": (_i, [_vp, _d]),
     "gwm_hip_set_refine_condition": (_i, [_vp, _d, _d]),
     "gwm_hip_set_pruning": (_i, [_vp, _i]),
+    "gwm_hip_kernel_pairs": (_i, [_vp, ctypes.c_char_p, _dp]),
     "gwm_hip_last_condition": (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "gwm_hip_set_timing": (_i, [_vp, _i]),
     "gwm_hip_kernel_time": (_i, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong), _dp]),
@@ -293,6 +294,12 @@ class Context:
         ms = ctypes.c_double(0)
         self._check(self._lib.gwm_hip_kernel_time(self.h, name.encode(), ctypes.byref(n), ctypes.byref(ms)))
         return n.value, ms.value
+
+    def kernel_pairs(self, name):
+        """Focus x data pairs the named sweeps processed while timing was on."""
+        v = np.zeros(1)
+        self._check(self._lib.gwm_hip_kernel_pairs(self.h, name.encode(), _p(v)))
+        return float(v[0])
 
     def reset_timers(self):
         self._check(self._lib.gwm_hip_reset_timers(self.h))

@@ -253,12 +253,21 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
     L.jobs = nullptr;
     L.items = nullptr;
     L.nitems = 0;
+    if (c->timing) {
+        if (!c->pairs_dev) GWM_HIP_CHECK(c, hipMalloc((void**)&c->pairs_dev, sizeof(unsigned long long)));
+        GWM_HIP_CHECK(c, hipMemsetAsync(c->pairs_dev, 0, sizeof(unsigned long long), c->stream));
+        j.pairs = c->pairs_dev;
+    }
     timer_begin(c);
     if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(c->dist, mode, nta, ntb, L, c->stream));
     else GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, L, c->stream));
     if (c->timing) {
         std::string name = std::string("contract_") + (mixed ? "mixed_" : "fp64_") + mode_name(mode);
         if ((rc = timer_end(c, name.c_str()))) return rc;
+        unsigned long long pairs = 0;
+        GWM_HIP_CHECK(c, hipMemcpyAsync(&pairs, c->pairs_dev, sizeof(pairs), hipMemcpyDeviceToHost, c->stream));
+        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        c->stats[name].pairs += (double)pairs;
     }
     *PT_out = PT;
     *nseg_out = nseg;
@@ -764,6 +773,7 @@ void gwm_hip_destroy(gwm_hip_ctx* c) {
     free_problem(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->pairs_dev) hipFree(c->pairs_dev);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1224,6 +1234,13 @@ int gwm_hip_kernel_time(gwm_hip_ctx* c, const char* name, long long* launches, d
     auto it = c->stats.find(name);
     if (launches) *launches = it == c->stats.end() ? 0 : it->second.launches;
     if (total_ms) *total_ms = it == c->stats.end() ? 0.0 : it->second.ms;
+    return GWM_OK;
+}
+
+int gwm_hip_kernel_pairs(gwm_hip_ctx* c, const char* name, double* pairs) {
+    if (!c || !name || !pairs) return GWM_ERR_INVALID;
+    auto it = c->stats.find(name);
+    *pairs = it == c->stats.end() ? 0.0 : it->second.pairs;
     return GWM_OK;
 }
 

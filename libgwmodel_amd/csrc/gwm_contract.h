@@ -38,6 +38,9 @@ struct SweepJob {
     // pruning) and, for adaptive qDiag sweeps, each chunk's largest bandwidth
     const double* boxes = nullptr;
     const double* chunk_bw = nullptr;
+    // optional count of the focus x data pairs actually swept (pruning skips
+    // the rest): one atomic add per workgroup and processed chunk
+    unsigned long long* pairs = nullptr;
 };
 
 // Launch of one sweep: either a single job (items == nullptr; workgroup b

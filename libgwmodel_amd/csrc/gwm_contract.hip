@@ -168,6 +168,7 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
     const int jend = min(n, jbeg + job.seg_len);
     const int kk = lane >> 4;
 
+    int swept = 0;   // chunks processed (not pruned)
     for (int j0 = jbeg; j0 < jend; j0 += kChunk) {
         if (prune) {
             const int ch = j0 / kChunk;
@@ -178,6 +179,7 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
             const int skip = box_gap2<DIST>(flo, fhi, gptr(job.boxes) + (size_t)ch * 2 * BD) > t;
             if (__builtin_amdgcn_readfirstlane(skip)) continue;
         }
+        ++swept;
         __syncthreads();
         // stage Z rows [j0, j0 + kChunk) (zero padded beyond n in memory)
         {
@@ -228,6 +230,9 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
             }
         }
     }
+
+    if (job.pairs && threadIdx.x == 0 && swept)
+        atomicAdd(job.pairs, (unsigned long long)swept * (unsigned long long)(kWaves * 16 * R) * 64ull);
 
     // D layout of v_mfma_f64_16x16x4f64 (checked on gfx950): lane holds
     // column (lane % 16), rows (lane / 16) + 4 q, q = 0..3.

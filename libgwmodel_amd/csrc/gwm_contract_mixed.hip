@@ -262,6 +262,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     const int jbeg = seg * args.seg_len;
     const int jend = min(n, jbeg + args.seg_len);
 
+    int swept = 0;   // chunks processed (not pruned)
     for (int j0 = jbeg; j0 < jend; j0 += kChunkM) {
         if (prune) {
             const int ch = j0 / kChunkM;
@@ -272,6 +273,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
             const int skip = box_gap2<DIST>(flo, fhi, gptr(args.boxes) + (size_t)ch * 2 * BD) > t;
             if (__builtin_amdgcn_readfirstlane(skip)) continue;
         }
+        ++swept;
         __syncthreads();
         // stage the two 32-row blocks of both planes: global [blk][col][32]
         {
@@ -390,6 +392,9 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                     acc[r][t][q] += (double)hh[r][t][q] + (double)xx[r][t][q] * (1.0 / 2048.0) +
                                     (double)ll[r][t][q] * (1.0 / 4194304.0);
     }
+
+    if (args.pairs && threadIdx.x == 0 && swept)
+        atomicAdd(args.pairs, (unsigned long long)swept * (unsigned long long)(kWaves * 16 * R) * 64ull);
 
     // D layout of v_mfma_f32_16x16x32_f16 (checked on gfx950): lane holds
     // column (lane % 16), rows 4 * (lane / 16) + q.

@@ -17,6 +17,7 @@ namespace detail {
 struct KernelStat {
     long long launches = 0;
     double ms = 0.0;
+    double pairs = 0.0;   // sweeps: focus x data pairs actually swept
 };
 
 }  // namespace detail
@@ -85,6 +86,7 @@ struct gwm_hip_ctx {
     // timing
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long* pairs_dev = nullptr;   // swept-pair counter (timing only)
     std::map<std::string, gwm::detail::KernelStat> stats;
     long long solves = 0;
 
