@@ -212,6 +212,8 @@ def main():
                     help="profiling mode: run this many mixed AICc sweeps (and as many FP64 ones) and exit")
     ap.add_argument("--sweep-bw", type=float, default=61.890021059082855,
                     help="bandwidth (km) of the profiling sweeps (default: the selected one)")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="profiling mode: run only the multiscale / batch / serve legs and print their JSON")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="profiling mode: one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -232,6 +234,14 @@ def main():
         return
 
     from libgwmodel_amd import capi, synth
+
+    if args.skip_headline:
+        extra = {"mgwr": run_mgwr(capi, synth, args.mgwr_n) if args.mgwr_n else None,
+                 "batch": run_batch(capi, synth, args.batch_problems, not args.no_loop) if args.batch_problems else None,
+                 "serve": run_serve(capi, synth, args.serve_requests, args.serve_rate, args.serve_check)
+                 if args.serve_requests else None}
+        print(json.dumps(extra), flush=True)
+        return
 
     n = args.n
     lo, hi = synth.CONFIG2_BW_KM
