@@ -74,6 +74,133 @@ __device__ __forceinline__ void key_bounds(const PTS& P, int f, GWM_GLOBAL const
     }
 }
 
+// ---------------------------------------------------------------- wave helpers
+
+constexpr int kWaveKeys = 1024;
+constexpr int kWaveChunks = kWaveKeys / 64;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inclusive prefix sum over the 64 lanes.
+__device__ __forceinline__ unsigned int wave_scan(unsigned int x, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+// One 8-bit digit over a 256-bin histogram in LDS: the bin holding `rank`;
+// updates prefix / rank and returns the bin's count.
+__device__ __forceinline__ unsigned int wave_digit(const unsigned int* hist, int lane, int shift,
+                                                   unsigned long long& prefix, long long& rank) {
+    unsigned int c[4], sum = 0u;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        c[i] = hist[4 * lane + i];
+        sum += c[i];
+    }
+    const unsigned int incl = wave_scan(sum, lane);
+    const unsigned int excl = incl - sum;
+    const bool mine = (long long)excl <= rank && rank < (long long)incl;
+    const unsigned long long who = __ballot(mine);
+    const int src = who ? __builtin_ctzll(who) : 63;
+    int bin = 4 * lane;
+    long long r = rank - (long long)excl;
+    unsigned int cnt = c[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (r < (long long)c[i]) {
+            cnt = c[i];
+            break;
+        }
+        r -= c[i];
+        ++bin;
+    }
+    bin = __shfl(bin, src);
+    r = __shfl(r, src);
+    cnt = __shfl(cnt, src);
+    prefix |= (unsigned long long)bin << shift;
+    rank = r;
+    return cnt;
+}
+
+// One wave: the smallest chunk-bound key at 0-based rank `rank` among the
+// first `nch` chunks (upper bounds when upper is set, else lower bounds), to
+// 16 bits: returns the upper edge (upper) or lower edge (lower) of its bin.
+template <int DIST, class PTS>
+__device__ __forceinline__ double wave_chunk_bound(const PTS& P, int f, GWM_GLOBAL const double* boxes, int nch,
+                                                   long long rank, bool upper, unsigned int* hist) {
+    constexpr int BD = DIST == kEuclidean ? 4 : 6;
+    const int lane = threadIdx.x & 63;
+    unsigned long long prefix = 0ull;
+    int shift = 64;
+    for (int pass = 0; pass < 2; ++pass) {
+        shift -= 8;
+        const int hs = shift + 8;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) hist[4 * lane + i] = 0u;
+        wave_sync();
+        for (int c = lane; c < nch; c += 64) {
+            double lb, ub;
+            key_bounds<DIST>(P, f, boxes + (size_t)c * BD, &lb, &ub);
+            const unsigned long long key = key_of(upper ? ub : lb);
+            if (hs == 64 || (key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & 255ull], 1u);
+        }
+        wave_sync();
+        wave_digit(hist, lane, shift, prefix, rank);
+        wave_sync();
+    }
+    return val_of(upper ? (prefix | ((1ull << shift) - 1ull)) : prefix);
+}
+
+// One wave: the keys at 0-based ranks `rank` and rank + 1 among keys[0..nk)
+// in LDS (eight 8-bit radix digits over a 256-word LDS histogram); keys above
+// the array start at `above`.
+__device__ __forceinline__ void wave_rank_select(const unsigned long long* keys, int nk, long long rank,
+                                                 unsigned long long above, unsigned int* hist,
+                                                 unsigned long long* key0, unsigned long long* key1) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long prefix = 0ull;
+    int shift = 64;
+    unsigned int cnt = 0u;
+    while (shift > 0) {
+        shift -= 8;
+        const int hs = shift + 8;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) hist[4 * lane + i] = 0u;
+        wave_sync();
+        for (int e = lane; e < nk; e += 64) {
+            const unsigned long long key = keys[e];
+            if (hs == 64 || (key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & 255ull], 1u);
+        }
+        wave_sync();
+        cnt = wave_digit(hist, lane, shift, prefix, rank);
+        wave_sync();
+    }
+    *key0 = prefix;
+    if (rank + 1 < (long long)cnt) {
+        *key1 = prefix;
+        return;
+    }
+    unsigned long long best = above;
+    for (int e = lane; e < nk; e += 64) {
+        const unsigned long long key = keys[e];
+        if (key > prefix && key < best) best = key;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned long long o = __shfl_xor(best, d);
+        best = o < best ? o : best;
+    }
+    *key1 = best;
+}
+
 // One radix digit of a 256-thread select: hist[0..nb) holds the counts of
 // the keys matching `prefix` above the digit; the bin holding 0-based rank
 // `rank` is located by a block-wide scan of per-thread bin groups and its
@@ -134,6 +261,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
     __shared__ unsigned long long s_prefix, s_above, s_next;
     __shared__ long long s_rank;
     __shared__ unsigned int s_count, s_ncand, s_nclist;
+    __shared__ double s_U;
     __shared__ double s_red[256];
     __shared__ int clist[kSelChunks];
 
@@ -159,30 +287,14 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
     int nclist = 0;
     double U = 0.0;
     if (pruned) {
-        // U: upper edge of the 24-bit radix bin holding the m-th smallest
-        // chunk upper bound over the full chunks
-        const long long m = (need + 63) / 64;
-        unsigned long long prefix = 0ull;
-        long long rank = m - 1;
-        int shift = 64;
-        for (int pass = 0; pass < 2; ++pass) {
-            shift -= 12;
-            const int hs = shift + 12;
-            for (int i = tid; i < kSelBins; i += 256) hist[i] = 0u;
-            __syncthreads();
-            for (int c = tid; c < nfull; c += 256) {
-                double lb, ub;
-                key_bounds<DIST>(P, f, boxes + (size_t)c * BD, &lb, &ub);
-                const unsigned long long key = key_of(ub);
-                if (hs == 64 || (key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & 4095ull], 1u);
-            }
-            __syncthreads();
-            resolve_digit(hist, kSelBins, rank, prefix, shift, &s_prefix, &s_rank, &s_count);
-            prefix = s_prefix;
-            rank = s_rank;
-            __syncthreads();
+        // U: upper edge of the 16-bit bin holding the m-th smallest chunk
+        // upper bound over the full chunks (one wave)
+        if (tid < 64) {
+            const double u = wave_chunk_bound<DIST>(P, f, boxes, nfull, (need + 63) / 64 - 1, true, hist);
+            if (tid == 0) s_U = u;
         }
-        U = val_of(prefix | ((1ull << shift) - 1ull));
+        __syncthreads();
+        U = s_U;
         if (tid == 0) s_nclist = 0u;
         __syncthreads();
         for (int c = tid; c < nch; c += 256) {
@@ -254,6 +366,19 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
             in_lds = true;
         }
         __syncthreads();
+    }
+    if (in_lds) {
+        // the gathered keys: one wave resolves both order statistics
+        if (tid < 64) {
+            unsigned long long key0, key1;
+            wave_rank_select(cand, ncand, rank0, s_above, hist, &key0, &key1);
+            if (tid == 0) {
+                const double d0 = select_to_distance<DIST>(val_of(key0));
+                const double d1 = select_to_distance<DIST>(val_of(key1));
+                out[f] = d0 + (d1 - d0) * bx;
+            }
+        }
+        return;
     }
     if (tid == 0) {
         s_prefix = 0ull;
@@ -344,60 +469,6 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
 // and eight 8-bit radix digits resolve the two order statistics there.  A
 // focus point outside this case is appended to a list that the workgroup
 // select (adaptive_bw_one) then works through with a fixed grid.
-constexpr int kWaveKeys = 1024;
-constexpr int kWaveChunks = kWaveKeys / 64;
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Inclusive prefix sum over the 64 lanes.
-__device__ __forceinline__ unsigned int wave_scan(unsigned int x, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const unsigned int y = __shfl_up(x, d);
-        if (lane >= d) x += y;
-    }
-    return x;
-}
-
-// One 8-bit digit over a 256-bin histogram in LDS: the bin holding `rank`;
-// updates prefix / rank and returns the bin's count.
-__device__ __forceinline__ unsigned int wave_digit(const unsigned int* hist, int lane, int shift,
-                                                   unsigned long long& prefix, long long& rank) {
-    unsigned int c[4], sum = 0u;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        c[i] = hist[4 * lane + i];
-        sum += c[i];
-    }
-    const unsigned int incl = wave_scan(sum, lane);
-    const unsigned int excl = incl - sum;
-    const bool mine = (long long)excl <= rank && rank < (long long)incl;
-    const unsigned long long who = __ballot(mine);
-    const int src = who ? __builtin_ctzll(who) : 63;
-    int bin = 4 * lane;
-    long long r = rank - (long long)excl;
-    unsigned int cnt = c[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        if (r < (long long)c[i]) {
-            cnt = c[i];
-            break;
-        }
-        r -= c[i];
-        ++bin;
-    }
-    bin = __shfl(bin, src);
-    r = __shfl(r, src);
-    cnt = __shfl(cnt, src);
-    prefix |= (unsigned long long)bin << shift;
-    rank = r;
-    return cnt;
-}
-
 // Returns false (nothing written) when the point is left to the workgroup select.
 template <int DIST, class PTS, class OUT>
 __device__ __forceinline__ bool adaptive_bw_wave(const PTS& P, double bw, OUT out, int f,
@@ -411,26 +482,7 @@ __device__ __forceinline__ bool adaptive_bw_wave(const PTS& P, double bw, OUT ou
     const long long need = (long long)b0 + 1;
     if (!(bw / n < 1) || !boxes || b0 < 1 || need > kWaveKeys || need > 64LL * nfull) return false;
     // U: upper edge of the 16-bit bin holding the m-th smallest chunk upper bound
-    unsigned long long prefix = 0ull;
-    long long rank = (need + 63) / 64 - 1;
-    int shift = 64;
-    for (int pass = 0; pass < 2; ++pass) {
-        shift -= 8;
-        const int hs = shift + 8;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) hist[4 * lane + i] = 0u;
-        wave_sync();
-        for (int c = lane; c < nfull; c += 64) {
-            double lb, ub;
-            key_bounds<DIST>(P, f, boxes + (size_t)c * BD, &lb, &ub);
-            const unsigned long long key = key_of(ub);
-            if (hs == 64 || (key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & 255ull], 1u);
-        }
-        wave_sync();
-        wave_digit(hist, lane, shift, prefix, rank);
-        wave_sync();
-    }
-    const double U = val_of(prefix | ((1ull << shift) - 1ull));
+    const double U = wave_chunk_bound<DIST>(P, f, boxes, nfull, (need + 63) / 64 - 1, true, hist);
     // candidate chunks, in chunk order
     int ncl = 0;
     bool overflow = false;
@@ -459,40 +511,9 @@ __device__ __forceinline__ bool adaptive_bw_wave(const PTS& P, double bw, OUT ou
         const int j = clist[e >> 6] * 64 + (e & 63);
         keys[e] = j < n ? key_of(select_value<DIST>(P, f, j)) : ~0ull;
     }
-    // eight 8-bit digits: rank floor(bw) - 1
-    prefix = 0ull;
-    rank = (long long)b0 - 1;
-    shift = 64;
-    unsigned int cnt = 0u;
-    while (shift > 0) {
-        shift -= 8;
-        const int hs = shift + 8;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) hist[4 * lane + i] = 0u;
-        wave_sync();
-        for (int e = lane; e < nk; e += 64) {
-            const unsigned long long key = keys[e];
-            if (hs == 64 || (key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & 255ull], 1u);
-        }
-        wave_sync();
-        cnt = wave_digit(hist, lane, shift, prefix, rank);
-        wave_sync();
-    }
-    const unsigned long long key0 = prefix;
-    unsigned long long key1 = key0;
-    if (!(rank + 1 < (long long)cnt)) {
-        unsigned long long best = ~0ull;
-        for (int e = lane; e < nk; e += 64) {
-            const unsigned long long key = keys[e];
-            if (key > key0 && key < best) best = key;
-        }
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            const unsigned long long o = __shfl_xor(best, d);
-            best = o < best ? o : best;
-        }
-        key1 = best;
-    }
+    wave_sync();
+    unsigned long long key0, key1;
+    wave_rank_select(keys, nk, (long long)b0 - 1, ~0ull, hist, &key0, &key1);
     if (lane == 0) {
         const double d0 = select_to_distance<DIST>(val_of(key0));
         const double d1 = select_to_distance<DIST>(val_of(key1));
