@@ -241,15 +241,19 @@ SweepJob sweep_job(Request* r, int mode, int nta, int ntb, const double* bw_pt, 
     return j;
 }
 
-// Sweep items of focus points [f0, f1) (f0 a multiple of 64 R).
+// Sweep items of focus points [f0, f1) (f0 a multiple of 64 R), tiles in the
+// spread order of tile_stride.
 void add_items(SweepGroup& g, int job, int f0, int f1, int nseg, int R) {
     const int t0 = f0 / (64 * R), t1 = (f1 + 64 * R - 1) / (64 * R);
-    for (int t = t0; t < t1; ++t)
+    const int cnt = t1 - t0, stride = tile_stride(cnt);
+    for (int u = 0; u < cnt; ++u) {
+        const int t = t0 + (int)(((long long)u * stride) % cnt);
         for (int s = 0; s < nseg; ++s) {
             g.items.push_back(job);
             g.items.push_back(t);
             g.items.push_back(s);
         }
+    }
 }
 
 SweepGroup& group_of(std::vector<SweepGroup>& gs, int dist, int mode, int prec, int nta, int ntb) {

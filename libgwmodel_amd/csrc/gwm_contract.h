@@ -51,7 +51,15 @@ struct SweepLaunch {
     const SweepJob* jobs;
     const int* items;
     int nitems;
+    int tile_stride = 1;   // single job: workgroup order t visits tile (t * stride) mod tiles (set by the launcher)
 };
+
+// A stride coprime to `tiles`, near tiles / golden ratio: visiting tiles in
+// the order (t * stride) mod tiles spreads consecutive workgroups over the
+// domain, so that with spatially ordered points the workgroups that pruning
+// leaves busy are spread over all XCDs instead of the few that hold the data
+// segments next to the current tiles.
+int tile_stride(int tiles);
 
 int contract_tile_focus(int nta, int ntb);
 int contract_mixed_tile_focus(int nta, int ntb);

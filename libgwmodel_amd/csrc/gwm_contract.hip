@@ -98,7 +98,8 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
     } else {
         job = L.job0;
         seg = blockIdx.x % L.job0.nseg;
-        tile = blockIdx.x / L.job0.nseg;
+        const int ntiles = gridDim.x / L.job0.nseg;
+        tile = (int)(((long long)(blockIdx.x / L.job0.nseg) * L.tile_stride) % ntiles);
     }
     // a launch groups jobs of exactly the instance's tile counts
     constexpr int nta = NTA, ntb = NTB;
@@ -275,7 +276,9 @@ hipError_t launch_f64_r(const SweepLaunch& a, hipStream_t s) {
         grid = tiles * a.job0.nseg;
     }
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL((contract_f64_kernel<NTA, NTB, DIST, MODE, R>), dim3(grid), dim3(256), 0, s, a);
+    SweepLaunch b = a;
+    if (!a.items) b.tile_stride = tile_stride(grid / a.job0.nseg);
+    hipLaunchKernelGGL((contract_f64_kernel<NTA, NTB, DIST, MODE, R>), dim3(grid), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
@@ -296,6 +299,21 @@ hipError_t dispatch_ab(int nta, int ntb, const SweepLaunch& a, hipStream_t s) {
 }
 
 }  // namespace
+
+int tile_stride(int tiles) {
+    if (tiles <= 2) return 1;
+    int s = (int)(tiles * 0.6180339887498949) | 1;
+    auto gcd = [](int a, int b) {
+        while (b) {
+            const int t = a % b;
+            a = b;
+            b = t;
+        }
+        return a;
+    };
+    while (gcd(s, tiles) != 1) s += 2;
+    return s % tiles;
+}
 
 int contract_tile_focus(int nta, int ntb) { return kWaves * 16 * (((nta + ntb) <= 8) ? 2 : 1); }
 

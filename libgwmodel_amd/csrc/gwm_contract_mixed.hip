@@ -196,7 +196,8 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     } else {
         args = L.job0;
         seg = blockIdx.x % L.job0.nseg;
-        tile = blockIdx.x / L.job0.nseg;
+        const int ntiles = gridDim.x / L.job0.nseg;
+        tile = (int)(((long long)(blockIdx.x / L.job0.nseg) * L.tile_stride) % ntiles);
     }
     // a launch groups jobs of exactly the instance's tile counts
     constexpr int nta = NTA, ntb = NTB;
@@ -429,7 +430,9 @@ hipError_t launch_mixed_r(const SweepLaunch& a, hipStream_t s) {
         grid = tiles * a.job0.nseg;
     }
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL((contract_mixed_kernel<NTA, NTB, DIST, MODE, R>), dim3(grid), dim3(256), 0, s, a);
+    SweepLaunch b = a;
+    if (!a.items) b.tile_stride = tile_stride(grid / a.job0.nseg);
+    hipLaunchKernelGGL((contract_mixed_kernel<NTA, NTB, DIST, MODE, R>), dim3(grid), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
