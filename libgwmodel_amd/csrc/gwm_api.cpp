@@ -82,32 +82,48 @@ int download_caller(gwm_hip_ctx* c, const double* dev, double* host, int cols) {
 }
 
 // Interleave the bits of up to 3 coordinates quantized to `bits` bits.
-uint64_t morton(const uint32_t* q, int dims, int bits) {
-    uint64_t code = 0;
-    for (int b = bits - 1; b >= 0; --b)
-        for (int d = 0; d < dims; ++d) code = (code << 1) | ((q[d] >> b) & 1u);
-    return code;
+// Position of the cell (x, y) along the Hilbert curve of a 2^bits x 2^bits
+// grid (the classic rotate-and-reflect walk, one quadrant per level).
+uint64_t hilbert_index(uint32_t x, uint32_t y, int bits) {
+    uint64_t d = 0;
+    for (uint32_t s = 1u << (bits - 1); s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = s - 1 - (x & (s - 1)) + (x & ~(s - 1));
+                y = s - 1 - (y & (s - 1)) + (y & ~(s - 1));
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
 }
 
-// Sorted order of the points along a Morton curve of their coordinates (the
-// plane, or the unit sphere's embedding in 3D for great-circle distances).
-std::vector<int> spatial_order(int n, int dims, const double* const* xs) {
-    const int bits = dims == 2 ? 16 : 21;
-    double lo[3], hi[3];
-    for (int d = 0; d < dims; ++d) {
-        lo[d] = *std::min_element(xs[d], xs[d] + n);
-        hi[d] = *std::max_element(xs[d], xs[d] + n);
+// Sorted order of the points along a Hilbert curve of their planar
+// coordinates (x, y, or lon, lat in degrees): consecutive points are close,
+// and unlike a Morton order the curve has no long jumps, so runs of 64
+// points have small bounding boxes.
+std::vector<int> spatial_order(int n, const double* xs, const double* ys) {
+    const int bits = 16;
+    const double* v[2] = {xs, ys};
+    double lo[2], hi[2];
+    for (int d = 0; d < 2; ++d) {
+        lo[d] = *std::min_element(v[d], v[d] + n);
+        hi[d] = *std::max_element(v[d], v[d] + n);
     }
     std::vector<std::pair<uint64_t, int>> key(n);
     const double top = (double)((1u << bits) - 1);
     for (int i = 0; i < n; ++i) {
-        uint32_t q[3] = {0, 0, 0};
-        for (int d = 0; d < dims; ++d) {
+        uint32_t q[2];
+        for (int d = 0; d < 2; ++d) {
             const double span = hi[d] - lo[d];
-            const double t = span > 0 ? (xs[d][i] - lo[d]) / span : 0.0;
+            const double t = span > 0 ? (v[d][i] - lo[d]) / span : 0.0;
             q[d] = (uint32_t)std::min(top, std::max(0.0, std::floor(t * top + 0.5)));
         }
-        key[i] = {morton(q, dims, bits), i};
+        key[i] = {hilbert_index(q[0], q[1], bits), i};
     }
     std::sort(key.begin(), key.end());
     std::vector<int> perm(n);
@@ -847,16 +863,7 @@ int gwm_hip_set_data(gwm_hip_ctx* c, int n, int k, const double* coords, const d
         }
     }
     // spatial order, sorted point tables and the 64-point chunk boxes
-    const int dims = distance == GWM_GREAT_CIRCLE ? 3 : 2;
-    {
-        const double* xs[3] = {coords, coords + N, nullptr};
-        if (dims == 3) {
-            xs[0] = hx.data();
-            xs[1] = hy.data();
-            xs[2] = hz.data();
-        }
-        c->perm = spatial_order(n, dims, xs);
-    }
+    c->perm = spatial_order(n, coords, coords + N);
     std::vector<double> cs(2 * N);
     to_sorted(c, coords, cs.data(), 2);
     GWM_HIP_CHECK(c, hipMemcpyAsync(c->px, cs.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));

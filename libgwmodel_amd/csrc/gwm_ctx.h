@@ -36,7 +36,7 @@ struct gwm_hip_ctx {
     int n_pad = 0, rows_pad = 0;
     std::vector<double> hcoords, hX, hy;   // the caller's order
     // Spatial order: every per-point device array holds the points sorted
-    // along a Morton curve, so that runs of 64 points are compact; perm[s] is
+    // along a Hilbert curve, so that runs of 64 points are compact; perm[s] is
     // the caller's index of sorted position s.  Host copies in and out of the
     // device are permuted at the API boundary.
     std::vector<int> perm;
