@@ -48,12 +48,27 @@ struct MCoords<kGreatCircle> {
     double sl[kChunkM];
 };
 
+// Kernel weight from the squared distance (Euclidean) or from the distance.
+struct BwConst {
+    float bw, inv_bw, bw2, inv_bw2, gauss;   // gauss = -log2(e) / (2 bw^2)
+};
+
+__device__ __forceinline__ BwConst make_bw(float b) {
+    BwConst c;
+    c.bw = b;
+    c.inv_bw = 1.0f / b;
+    c.bw2 = b * b;
+    c.inv_bw2 = 1.0f / (b * b);
+    c.gauss = -1.4426950408889634f * 0.5f * c.inv_bw2;
+    return c;
+}
+
 template <int NTA, int DIST, int MODE>
 struct alignas(16) MShared {
     _Float16 zh[2][NTA * 16][kColStride];
     _Float16 zl[2][NTA * 16][kColStride];
     MCoords<DIST> c;
-    float bw[MODE == kModeQDiag ? kChunkM : 1];
+    BwConst bwc[MODE == kModeQDiag ? kChunkM : 1];   // qDiag: constants of each data point's bandwidth
 };
 
 template <int DIST>
@@ -76,21 +91,6 @@ __device__ __forceinline__ MFocus<DIST> load_mfocus(const GPoints& P, int f) {
         fc.sl = P.sl[f];
     }
     return fc;
-}
-
-// Kernel weight from the squared distance (Euclidean) or from the distance.
-struct BwConst {
-    float bw, inv_bw, bw2, inv_bw2, gauss;   // gauss = -log2(e) / (2 bw^2)
-};
-
-__device__ __forceinline__ BwConst make_bw(float b) {
-    BwConst c;
-    c.bw = b;
-    c.inv_bw = 1.0f / b;
-    c.bw2 = b * b;
-    c.inv_bw2 = 1.0f / (b * b);
-    c.gauss = -1.4426950408889634f * 0.5f * c.inv_bw2;
-    return c;
 }
 
 __device__ __forceinline__ float weight_from_d2(int kernel, float d2, const BwConst& c) {
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                 sh.c.hz[threadIdx.x] = P.hz[jl];
                 sh.c.sl[threadIdx.x] = P.sl[jl];
             }
-            if constexpr (MODE == kModeQDiag) sh.bw[threadIdx.x] = (float)(bw_pt ? bw_pt[jl] : args.bw);
+            if constexpr (MODE == kModeQDiag) sh.bwc[threadIdx.x] = make_bw((float)(bw_pt ? bw_pt[jl] : args.bw));
         }
         __syncthreads();
 #pragma unroll
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                 }
                 if constexpr (MODE == kModeQDiag) {
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) w[e] = weight_from_d2(args.kernel, d2[e], make_bw(sh.bw[rb + e]));
+                    for (int e = 0; e < 8; ++e) w[e] = weight_from_d2(args.kernel, d2[e], sh.bwc[rb + e]);
                 } else {
                     weights8_from_d2(args.kernel, d2, fbw[r], w);
                 }
