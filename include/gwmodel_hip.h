@@ -84,6 +84,16 @@ int gwm_hip_criterion(gwm_hip_ctx* ctx, int kernel, int adaptive, double bw, int
 int gwm_hip_fit(gwm_hip_ctx* ctx, int kernel, int adaptive, double bw, int precision, double* betas,
                 double* betasSE_raw, double* shat, double* qDiag, double* S);
 
+/* Betas at m prediction locations: replaces GWRBasic::predictSerial /
+ * predictOmp / predictCuda (src/gwmodelpp/GWRBasic.cpp:167-191, :349-384,
+ * :665-707) behind GWRBasic::predict (:130-156).  locations is m x 2
+ * column-major in the data's coordinate system; betas receives m x k
+ * column-major.  Adaptive bandwidths count the bw nearest data points of each
+ * location.  Returns GWM_ERR_NOT_SPD (betas NaN at those locations) where
+ * predictSerial's inv_sympd throws. */
+int gwm_hip_predict(gwm_hip_ctx* ctx, int m, const double* locations, int kernel, int adaptive, double bw,
+                    int precision, double* betas);
+
 /* Golden-section bandwidth selection on device-resident data: replaces
  * BandwidthSelector::optimize (src/gwmodelpp/BandwidthSelector.cpp:6-81)
  * driving getCriterion.  Bounds follow GWRBasic::fit (GWRBasic.cpp:76-79):

@@ -235,6 +235,9 @@ public:
     Status status() const { return mStatus; }
 
     Mat fit();
+    // IGwmodelPredictable::predict (GWRBasic.cpp:130-156): betas at the rows
+    // of locations (m x 2) with the current spatial weight; also stored in betas().
+    Mat predict(const Mat& locations);
 
     const Mat& betas() const { return mBetas; }
     const Mat& betasSE() const { return mBetasSE; }

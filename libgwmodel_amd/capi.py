@@ -41,6 +41,7 @@ SIGNATURES = {
     "gwm_hip_set_data": (_i, [_vp, _i, _i, _dp, _dp, _dp, _i]),
     "gwm_hip_criterion": (_i, [_vp, _i, _i, _d, _i, _i, _dp]),
     "gwm_hip_fit": (_i, [_vp, _i, _i, _d, _i, _dp, _dp, _dp, _dp, _dp]),
+    "gwm_hip_predict": (_i, [_vp, _i, _dp, _i, _i, _d, _i, _dp]),
     "gwm_hip_select_bandwidth": (_i, [_vp, _i, _i, _i, _i, _i, _d, _i, _d, _dp, _dp, _dp, _i, _ip]),
     "gwm_hip_gwr_run": (_i, [_vp, _i, _i, _d, _i, _i, _i, _i, _d, _i, _d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                              _dp, _i, _ip]),
@@ -182,6 +183,15 @@ class Context:
         self._check(self._lib.gwm_hip_fit(self.h, int(kernel), int(adaptive), float(bw), int(precision), _p(betas),
                                           _p(se), _p(shat), _p(q), _p(S)))
         return dict(betas=betas, betasSE_raw=se, shat=shat, qDiag=q, S=S)
+
+    def predict(self, locations, kernel, adaptive, bw, precision=FP64):
+        """Betas (m x k) at the rows of `locations` (m x 2)."""
+        loc = np.asfortranarray(np.asarray(locations, dtype=np.float64).reshape(-1, 2))
+        m = loc.shape[0]
+        betas = np.zeros((m, self.k), order="F")
+        self._check(self._lib.gwm_hip_predict(self.h, m, _p(loc), int(kernel), int(adaptive), float(bw),
+                                              int(precision), _p(betas)))
+        return betas
 
     def select_bandwidth(self, kernel, adaptive, crit, precision=FP64, lower=None, upper=None, trace_cap=256):
         bw = np.zeros(1)

@@ -90,6 +90,32 @@ Mat GWRBasic::fit() {
     return mBetas;
 }
 
+Mat GWRBasic::predict(const Mat& locations) {
+    mStatus = Status::Success;
+    const std::size_t n = mCoords.n_rows(), k = mX.n_cols(), m = locations.n_rows();
+    if (mParallelType != ParallelType::HIP) throw std::logic_error("parallel type not available in this build");
+    if (mCoords.n_cols() != 2 || mX.n_rows() != n || mY.n_elem() != n || locations.n_cols() != 2)
+        throw std::invalid_argument("coords and locations must have 2 columns, X n x k, y n");
+    const int dist = distance_id(mSpatialWeight);
+    BandwidthWeight* bw = mSpatialWeight.weight<BandwidthWeight>();
+    HipContext ctx(mGpuId);
+    gwm_hip_set_stop_callback(ctx.get(), &stop_adapter, mTelegram);
+    int rc = gwm_hip_set_data(ctx.get(), (int)n, (int)k, mCoords.memptr(), mX.memptr(), mY.memptr(), dist);
+    if (rc) raise(ctx.get(), rc);
+    const int prec = mPrecision == Precision::Mixed ? GWM_MIXED : GWM_FP64;
+    Mat betas(m, k);
+    rc = gwm_hip_predict(ctx.get(), (int)m, locations.memptr(), (int)bw->kernel(), bw->adaptive() ? 1 : 0,
+                         bw->bandwidth(), prec, betas.memptr());
+    if (rc == GWM_ERR_CANCELLED) {
+        // the reference returns an n x k zero matrix when stopped (GWRBasic.cpp:146)
+        mStatus = Status::Terminated;
+        return Mat(n, k);
+    }
+    if (rc) raise(ctx.get(), rc);
+    mBetas = betas;
+    return mBetas;
+}
+
 void GWRMultiscale::setBandwidthInitilize(const std::vector<BandwidthInitilizeType>& v) {
     if (v.size() != mX.n_cols()) throw std::length_error("BandwidthInitilize size do not match indepvars");
     mBandwidthInitilize = v;

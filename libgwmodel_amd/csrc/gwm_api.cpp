@@ -81,7 +81,19 @@ int download_caller(gwm_hip_ctx* c, const double* dev, double* host, int cols) {
     return GWM_OK;
 }
 
-// Interleave the bits of up to 3 coordinates quantized to `bits` bits.
+// Great circle: the half unit vectors (hx, hy, hz) and sin(lat) of points
+// given in degrees.
+void sphere_tables(size_t N, const double* lon, const double* lat, double* hx, double* hy, double* hz, double* sl) {
+    const double de2ra = kPi / 180;
+    for (size_t j = 0; j < N; ++j) {
+        const double lo = lon[j] * de2ra, la = lat[j] * de2ra;
+        hx[j] = 0.5 * std::cos(la) * std::cos(lo);
+        hy[j] = 0.5 * std::cos(la) * std::sin(lo);
+        hz[j] = 0.5 * std::sin(la);
+        sl[j] = std::sin(la);
+    }
+}
+
 // Position of the cell (x, y) along the Hilbert curve of a 2^bits x 2^bits
 // grid (the classic rotate-and-reflect walk, one quadrant per level).
 uint64_t hilbert_index(uint32_t x, uint32_t y, int bits) {
@@ -320,19 +332,32 @@ SolveArgs solve_args(gwm_hip_ctx* c, int PT, int nseg, int colB, const double* b
     return s;
 }
 
+// Focus points of a prediction (gwm_hip_predict): their table in spatial
+// order and the device arrays the solves write.
+struct PredictSet {
+    gwm::Points Q;
+    double* cond;   // [m] conditioning indicator
+    int* rlist;     // [m] refinement list
+    double* betas;  // m x k column-major
+};
+
 // After a mixed-precision sweep and solve over all points: recompute in FP64
 // the focus points whose local design is ill conditioned (indicator above
 // limit, or not SPD), so that their results are those of the FP64 path (same
 // kernel, same data segments, so the same bits).  The failure flag is left as
 // the FP64 solves set it: every point the mixed solves failed on is in the
 // list.  *count receives the number of recomputed points; the indicator
-// maximum of the mixed pass goes to c->last_cond.
+// maximum of the mixed pass goes to c->last_cond.  q: the focus points are
+// prediction locations rather than the data points.
 int refine_fp64(gwm_hip_ctx* c, int mode, int smode, int ntb, int colB, const double* bw_pt, double bw, int kernel,
-                double limit, int* count) {
-    GWM_HIP_CHECK(c, launch_reduce(nullptr, nullptr, nullptr, c->cond, c->n, c->red, c->stream));
+                double limit, int* count, const PredictSet* q = nullptr) {
+    const int nf = q ? q->Q.n : c->n;
+    double* cond = q ? q->cond : c->cond;
+    int* rlist = q ? q->rlist : c->rlist;
+    GWM_HIP_CHECK(c, launch_reduce(nullptr, nullptr, nullptr, cond, nf, c->red, c->stream));
     GWM_HIP_CHECK(c, hipMemcpyAsync(&c->last_cond, c->red + 3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     GWM_HIP_CHECK(c, hipMemsetAsync(c->rcount, 0, sizeof(int), c->stream));
-    GWM_HIP_CHECK(c, launch_compact(c->cond, c->n, limit, c->rlist, c->rcount, c->stream));
+    GWM_HIP_CHECK(c, launch_compact(cond, nf, limit, rlist, c->rcount, c->stream));
     GWM_HIP_CHECK(c, hipMemcpyAsync(count, c->rcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->last_refined = *count;
@@ -366,8 +391,9 @@ int refine_fp64(gwm_hip_ctx* c, int mode, int smode, int ntb, int colB, const do
     j.nta = nta;
     j.ntb = ntb;
     j.out = c->rpart;
-    j.focus_idx = c->rlist;
+    j.focus_idx = rlist;
     j.nfocus = *count;
+    if (q) j.Q = q->Q;
     if (c->prune) j.boxes = c->boxes;
     L.jobs = nullptr;
     L.items = nullptr;
@@ -380,8 +406,15 @@ int refine_fp64(gwm_hip_ctx* c, int mode, int smode, int ntb, int colB, const do
     SolveArgs s = solve_args(c, PT, nseg, colB, bw_pt, bw, kernel);
     s.part = c->rpart;
     s.n_pad = npad;
-    s.focus_idx = c->rlist;
+    s.focus_idx = rlist;
     s.nfocus = *count;
+    if (q) {
+        s.n = q->Q.n;
+        s.X = nullptr;
+        s.y = nullptr;
+        s.betas = q->betas;
+        s.cond = q->cond;
+    }
     GWM_HIP_CHECK(c, launch_solve(c->k, smode, s, c->stream));
     return GWM_OK;
 }
@@ -754,6 +787,136 @@ int fit_betas_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int prec
     return fail ? GWM_ERR_NOT_SPD : GWM_OK;
 }
 
+// Betas at prediction locations (GWRBasic::predictSerial, GWRBasic.cpp:167-191):
+// beta(u) = (X^T W(u) X)^-1 X^T W(u) y with W(u) the weights of the distances
+// from location u to every data point; adaptive bandwidths are the bw-th
+// nearest data point of u (BandwidthWeight::weight on that distance vector).
+// The locations are swept in their own spatial order; results come back in
+// the caller's order.
+int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adaptive, double bw, int precision,
+                 double* betas_out) {
+    int rc;
+    if ((rc = check_kernel(c, kernel, precision))) return rc;
+    if (adaptive && !(bw >= 1.0)) {
+        c->err = "adaptive bandwidth must be >= 1 neighbour";
+        return GWM_ERR_INVALID;
+    }
+    if (!adaptive && !(bw > 0.0)) {
+        c->err = "fixed bandwidth must be > 0";
+        return GWM_ERR_INVALID;
+    }
+    if (stopped(c)) return GWM_ERR_CANCELLED;
+    const size_t M = m;
+    const int k = c->k;
+    const bool gc = c->dist == GWM_GREAT_CIRCLE;
+    // spatial order of the locations and their sorted tables
+    const std::vector<int> qperm = spatial_order(m, loc, loc + M);
+    const int ntab = gc ? 6 : 2;
+    std::vector<double> tab(ntab * M);
+    for (size_t s = 0; s < M; ++s) {
+        tab[s] = loc[qperm[s]];
+        tab[M + s] = loc[M + qperm[s]];
+    }
+    if (gc) sphere_tables(M, tab.data(), tab.data() + M, &tab[2 * M], &tab[3 * M], &tab[4 * M], &tab[5 * M]);
+    // device arrays: tables, bandwidths, betas, indicator; lists
+    struct Owned {
+        void* p = nullptr;
+        ~Owned() {
+            if (p) hipFree(p);
+        }
+    } dbuf, ibuf;
+    const size_t nd = M * (ntab + 2 + k);
+    GWM_HIP_CHECK(c, hipMalloc(&dbuf.p, nd * sizeof(double)));
+    GWM_HIP_CHECK(c, hipMalloc(&ibuf.p, (2 * M + 1) * sizeof(int)));
+    double* d = static_cast<double*>(dbuf.p);
+    int* left = static_cast<int*>(ibuf.p);
+    int* rlist = left + M;
+    unsigned int* nleft = reinterpret_cast<unsigned int*>(left + 2 * M);
+    GWM_HIP_CHECK(c, hipMemcpyAsync(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    gwm::Points Q{};
+    Q.px = d;
+    Q.py = d + M;
+    if (gc) {
+        Q.hx = d + 2 * M;
+        Q.hy = d + 3 * M;
+        Q.hz = d + 4 * M;
+        Q.sl = d + 5 * M;
+    }
+    Q.n = m;
+    double* qbw = d + ntab * M;
+    double* qcond = qbw + M;
+    double* qbetas = qcond + M;
+    const double* bw_pt = nullptr;
+    if (adaptive) {
+        timer_begin(c);
+        GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, qbw, c->prune ? c->boxes : nullptr, left, nleft,
+                                            c->stream, &Q));
+        if ((rc = timer_end(c, "select"))) return rc;
+        bw_pt = qbw;
+    }
+    // one sweep over the locations
+    const bool mixed = precision == GWM_MIXED;
+    int nseg, seg_len;
+    data_segments(c->n, nseg, seg_len);
+    const int PT = c->nta * 16;
+    const int mpad = (m + 127) / 128 * 128;
+    if ((rc = ensure_part(c, (size_t)nseg * PT * mpad * sizeof(double)))) return rc;
+    SweepLaunch L;
+    SweepJob& j = L.job0;
+    j.P = c->points();
+    j.Q = Q;
+    j.Z = c->Z;
+    j.Zh = c->Zh;
+    j.Zl = c->Zl;
+    j.colscale = c->zscale;
+    j.bw_pt = bw_pt;
+    j.bw = bw;
+    j.kernel = kernel;
+    j.n = c->n;
+    j.n_pad = mpad;
+    j.seg_len = seg_len;
+    j.nseg = nseg;
+    j.nta = c->nta;
+    j.ntb = 0;
+    j.out = c->part;
+    if (c->prune) j.boxes = c->boxes;
+    L.jobs = nullptr;
+    L.items = nullptr;
+    L.nitems = 0;
+    GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
+    timer_begin(c);
+    if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(c->dist, kModePlain, c->nta, 0, L, c->stream));
+    else GWM_HIP_CHECK(c, launch_contract_f64(c->dist, kModePlain, c->nta, 0, L, c->stream));
+    if ((rc = timer_end(c, mixed ? "contract_mixed_predict" : "contract_fp64_predict"))) return rc;
+    SolveArgs sa = solve_args(c, PT, nseg, 0, bw_pt, bw, kernel);
+    sa.n = m;
+    sa.n_pad = mpad;
+    sa.X = nullptr;
+    sa.y = nullptr;
+    sa.betas = qbetas;
+    sa.cond = qcond;
+    GWM_HIP_CHECK(c, launch_solve(k, kSolvePredict, sa, c->stream));
+    if (mixed) {
+        const PredictSet q{Q, qcond, rlist, qbetas};
+        int count = 0;
+        if ((rc = refine_fp64(c, kModePlain, kSolvePredict, 0, 0, bw_pt, bw, kernel, c->refine_fit, &count, &q)))
+            return rc;
+    }
+    int fail = 0;
+    std::vector<double> b(M * k);
+    GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(b.data(), qbetas, b.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    for (int p = 0; p < k; ++p)
+        for (size_t s = 0; s < M; ++s) betas_out[(size_t)qperm[s] + p * M] = b[s + p * M];
+    c->solves += m;
+    if (fail) {
+        c->err = "inv_sympd(): matrix is singular or not positive definite";
+        return GWM_ERR_NOT_SPD;
+    }
+    return GWM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -853,14 +1016,7 @@ int gwm_hip_set_data(gwm_hip_ctx* c, int n, int k, const double* coords, const d
         hy.resize(N);
         hz.resize(N);
         sl.resize(N);
-        const double de2ra = kPi / 180;
-        for (size_t j = 0; j < N; ++j) {
-            double lon = coords[j] * de2ra, lat = coords[N + j] * de2ra;
-            hx[j] = 0.5 * std::cos(lat) * std::cos(lon);
-            hy[j] = 0.5 * std::cos(lat) * std::sin(lon);
-            hz[j] = 0.5 * std::sin(lat);
-            sl[j] = std::sin(lat);
-        }
+        sphere_tables(N, coords, coords + N, hx.data(), hy.data(), hz.data(), sl.data());
     }
     // spatial order, sorted point tables and the 64-point chunk boxes
     c->perm = spatial_order(n, coords, coords + N);
@@ -1202,6 +1358,18 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
     std::copy(bw.begin(), bw.end(), bws_out);
     restore();
     return GWM_OK;
+}
+
+int gwm_hip_predict(gwm_hip_ctx* c, int m, const double* locations, int kernel, int adaptive, double bw,
+                    int precision, double* betas) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (m < 1 || !locations || !betas) {
+        c->err = "predict: need m >= 1 locations and an output buffer";
+        return GWM_ERR_INVALID;
+    }
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    return predict_impl(c, m, locations, kernel, adaptive, bw, precision, betas);
 }
 
 int gwm_hip_set_certify_tolerance(gwm_hip_ctx* c, double rel) {

@@ -34,6 +34,10 @@ struct SweepJob {
     // t < nfocus (n_pad then pads nfocus); null = every point in order
     const int* focus_idx = nullptr;
     int nfocus = 0;
+    // optional focus table (predict): when Q.n > 0 the focus points are the
+    // points of Q (n_pad pads Q.n, focus_idx indexes Q, bw_pt is per point
+    // of Q) and no focus is excluded from its own sums
+    Points Q{};
     // pruning (see gwm_device.h): boxes of the 64-point data chunks (null: no
     // pruning) and, for adaptive qDiag sweeps, each chunk's largest bandwidth
     const double* boxes = nullptr;
@@ -84,14 +88,17 @@ hipError_t launch_chunk_max(const double* v, int n, double* out, hipStream_t s);
 // spatially ordered points (boxes, nullable) only candidate chunks are read.
 // left (n ints) and nleft (one counter) are scratch for the points the
 // one-wave fast path leaves to the workgroup select.
+// focus (nullable): thresholds at these points instead of at P's own (predict).
 hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out, const double* boxes, int* left,
-                              unsigned int* nleft, hipStream_t s);
+                              unsigned int* nleft, hipStream_t s, const Points* focus = nullptr);
 // Maximum pairwise distance (CRSDistance::maxDistance, CRSDistance.cpp:133-143).
 hipError_t launch_max_distance(int dist, const Points& P, double* block_max, int nblocks, hipStream_t s);
 // Distances from one focus to all points (test hook).
 hipError_t launch_distance_row(int dist, const Points& P, int focus, double* out, hipStream_t s);
 
-enum SolveMode { kSolveCV = 0, kSolveAIC = 1, kSolveFit = 2, kSolveBeta = 3 };
+// kSolvePredict: betas only, at focus points that are not data points (X and
+// y are not read; n counts the focus points).
+enum SolveMode { kSolveCV = 0, kSolveAIC = 1, kSolveFit = 2, kSolveBeta = 3, kSolvePredict = 4 };
 
 struct SolveArgs {
     const double* part;    // contraction partial sums [nseg][PT][n_pad]

@@ -113,7 +113,9 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
     const int fbase = tile * (kWaves * 16 * R) + wave * 16 * R;
     const int n = job.n;
     GWM_GLOBAL const int* focus_idx = gptr(job.focus_idx);
-    const int nf = focus_idx ? job.nfocus : n;   // focus points of this job
+    const bool qtab = job.Q.n > 0;   // focus points from a table of their own
+    const GPoints FP = qtab ? global_points(job.Q) : P;
+    const int nf = focus_idx ? job.nfocus : (qtab ? job.Q.n : n);   // focus points of this job
 
     // focus points owned by this lane (one per A tile)
     FocusCoords<DIST> fc[R];
@@ -125,8 +127,8 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
         int f = fbase + 16 * r + (lane & 15);
         int fl = f < nf ? f : nf - 1;
         if (focus_idx) fl = focus_idx[fl];
-        fidx[r] = focus_idx ? (f < nf ? fl : -1) : f;   // self index for the CV exclusion
-        fc[r] = load_focus<DIST>(P, fl);
+        fidx[r] = qtab ? -1 : (focus_idx ? (f < nf ? fl : -1) : f);   // self index for the CV exclusion
+        fc[r] = load_focus<DIST>(FP, fl);
         const double b = (MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : job.bw;
         fbw[r] = make_kconst64(b);
         fb = fmax(fb, b);
@@ -271,7 +273,7 @@ hipError_t launch_f64_r(const SweepLaunch& a, hipStream_t s) {
     if (a.items) {
         grid = a.nitems;
     } else {
-        const int nf = a.job0.focus_idx ? a.job0.nfocus : a.job0.n;
+        const int nf = a.job0.focus_idx ? a.job0.nfocus : (a.job0.Q.n > 0 ? a.job0.Q.n : a.job0.n);
         int tiles = (nf + kWaves * 16 * R - 1) / (kWaves * 16 * R);
         grid = tiles * a.job0.nseg;
     }

@@ -211,7 +211,9 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     const int fbase = tile * (kWaves * 16 * R) + wave * 16 * R;
     const int n = args.n;
     GWM_GLOBAL const int* focus_idx = gptr(args.focus_idx);
-    const int nf = focus_idx ? args.nfocus : n;   // focus points of this job
+    const bool qtab = args.Q.n > 0;   // focus points from a table of their own
+    const GPoints FP = qtab ? global_points(args.Q) : P;
+    const int nf = focus_idx ? args.nfocus : (qtab ? args.Q.n : n);   // focus points of this job
     const int kq = lane >> 4;       // which 8-row group of the 32-row step
     const int col = lane & 15;
 
@@ -224,8 +226,8 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
         int f = fbase + 16 * r + col;
         int fl = f < nf ? f : nf - 1;
         if (focus_idx) fl = focus_idx[fl];
-        fidx[r] = focus_idx ? (f < nf ? fl : -1) : f;   // self index for the CV exclusion
-        fc[r] = load_mfocus<DIST>(P, fl);
+        fidx[r] = qtab ? -1 : (focus_idx ? (f < nf ? fl : -1) : f);   // self index for the CV exclusion
+        fc[r] = load_mfocus<DIST>(FP, fl);
         const double b = (MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : args.bw;
         fbw[r] = make_bw((float)b);
         fb = fmax(fb, b);
@@ -425,7 +427,7 @@ hipError_t launch_mixed_r(const SweepLaunch& a, hipStream_t s) {
     if (a.items) {
         grid = a.nitems;
     } else {
-        const int nf = a.job0.focus_idx ? a.job0.nfocus : a.job0.n;
+        const int nf = a.job0.focus_idx ? a.job0.nfocus : (a.job0.Q.n > 0 ? a.job0.Q.n : a.job0.n);
         int tiles = (nf + kWaves * 16 * R - 1) / (kWaves * 16 * R);
         grid = tiles * a.job0.nseg;
     }

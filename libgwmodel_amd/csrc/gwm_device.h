@@ -170,23 +170,29 @@ __device__ __forceinline__ double kernel_fast_f64(int kernel, double d, const KC
     }
 }
 
+// Distance from point f of the focus table F to point j of the data table P
+// (F is P itself for fits and criteria, the prediction locations for predict).
 template <int DIST, class PTS>
-__device__ __forceinline__ double distance_f64(const PTS& P, int f, int j) {
+__device__ __forceinline__ double distance_f64(const PTS& F, int f, const PTS& P, int j) {
     if (DIST == kEuclidean) {
-        return sqrt(d2_euclid(P.px[f], P.py[f], P.px[j], P.py[j]));
+        return sqrt(d2_euclid(F.px[f], F.py[f], P.px[j], P.py[j]));
     } else {
-        return gcdist_f64(P.px[f], P.py[f], P.hx[f], P.hy[f], P.hz[f], P.sl[f], P.px[j], P.py[j], P.hx[j],
+        return gcdist_f64(F.px[f], F.py[f], F.hx[f], F.hy[f], F.hz[f], F.sl[f], P.px[j], P.py[j], P.hx[j],
                           P.hy[j], P.hz[j], P.sl[j]);
     }
+}
+template <int DIST, class PTS>
+__device__ __forceinline__ double distance_f64(const PTS& P, int f, int j) {
+    return distance_f64<DIST>(P, f, P, j);
 }
 
 // Selection key: a monotone image of the distance.  Euclidean uses the
 // squared distance (sqrt is monotone, so order statistics commute with it);
 // great-circle uses the distance itself.
 template <int DIST, class PTS>
-__device__ __forceinline__ double select_value(const PTS& P, int f, int j) {
-    if (DIST == kEuclidean) return d2_euclid(P.px[f], P.py[f], P.px[j], P.py[j]);
-    return distance_f64<DIST>(P, f, j);
+__device__ __forceinline__ double select_value(const PTS& F, int f, const PTS& P, int j) {
+    if (DIST == kEuclidean) return d2_euclid(F.px[f], F.py[f], P.px[j], P.py[j]);
+    return distance_f64<DIST>(F, f, P, j);
 }
 template <int DIST>
 __device__ __forceinline__ double select_to_distance(double v) {

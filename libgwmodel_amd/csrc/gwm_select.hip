@@ -41,10 +41,10 @@ constexpr int kSelChunks = 4096; // candidate chunk list capacity (pruning for n
 // Lower and upper bounds of the selection key (squared distance, Euclidean;
 // distance, great circle) between point f and the points of a chunk box.
 template <int DIST, class PTS>
-__device__ __forceinline__ void key_bounds(const PTS& P, int f, GWM_GLOBAL const double* box, double* lb,
+__device__ __forceinline__ void key_bounds(const PTS& F, int f, GWM_GLOBAL const double* box, double* lb,
                                            double* ub) {
     if constexpr (DIST == kEuclidean) {
-        const double x[2] = {P.px[f], P.py[f]};
+        const double x[2] = {F.px[f], F.py[f]};
         double g2 = 0.0, h2 = 0.0;
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
@@ -57,7 +57,7 @@ __device__ __forceinline__ void key_bounds(const PTS& P, int f, GWM_GLOBAL const
         *lb = g2 * (1.0 - 1e-12);
         *ub = h2 * (1.0 + 1e-12);
     } else {
-        const double x[3] = {P.hx[f], P.hy[f], P.hz[f]};
+        const double x[3] = {F.hx[f], F.hy[f], F.hz[f]};
         double g2 = 0.0, h2 = 0.0;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -134,7 +134,7 @@ __device__ __forceinline__ unsigned int wave_digit(const unsigned int* hist, int
 // first `nch` chunks (upper bounds when upper is set, else lower bounds), to
 // 16 bits: returns the upper edge (upper) or lower edge (lower) of its bin.
 template <int DIST, class PTS>
-__device__ __forceinline__ double wave_chunk_bound(const PTS& P, int f, GWM_GLOBAL const double* boxes, int nch,
+__device__ __forceinline__ double wave_chunk_bound(const PTS& F, int f, GWM_GLOBAL const double* boxes, int nch,
                                                    long long rank, bool upper, unsigned int* hist) {
     constexpr int BD = DIST == kEuclidean ? 4 : 6;
     const int lane = threadIdx.x & 63;
@@ -148,7 +148,7 @@ __device__ __forceinline__ double wave_chunk_bound(const PTS& P, int f, GWM_GLOB
         wave_sync();
         for (int c = lane; c < nch; c += 64) {
             double lb, ub;
-            key_bounds<DIST>(P, f, boxes + (size_t)c * BD, &lb, &ub);
+            key_bounds<DIST>(F, f, boxes + (size_t)c * BD, &lb, &ub);
             const unsigned long long key = key_of(upper ? ub : lb);
             if (hs == 64 || (key >> hs) == (prefix >> hs)) atomicAdd(&hist[(key >> shift) & 255ull], 1u);
         }
@@ -251,7 +251,7 @@ __device__ __forceinline__ void resolve_digit(const unsigned int* hist, int nb, 
 }
 
 template <int DIST, class PTS, class OUT>
-__device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out, int f,
+__device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, double bw, OUT out, int f,
                                                 GWM_GLOBAL const double* boxes) {
     const int n = P.n;
     const int tid = threadIdx.x;
@@ -269,7 +269,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
     if (!(dn < 1)) {
         // (bw/n) * max(dist); distances are >= 0 and include the focus (0)
         double m = 0.0;
-        for (int j = tid; j < n; j += 256) m = fmax(m, distance_f64<DIST>(P, f, j));
+        for (int j = tid; j < n; j += 256) m = fmax(m, distance_f64<DIST>(F, f, P, j));
         s_red[tid] = m;
         __syncthreads();
         for (int w = 128; w > 0; w >>= 1) {
@@ -290,7 +290,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
         // U: upper edge of the 16-bit bin holding the m-th smallest chunk
         // upper bound over the full chunks (one wave)
         if (tid < 64) {
-            const double u = wave_chunk_bound<DIST>(P, f, boxes, nfull, (need + 63) / 64 - 1, true, hist);
+            const double u = wave_chunk_bound<DIST>(F, f, boxes, nfull, (need + 63) / 64 - 1, true, hist);
             if (tid == 0) s_U = u;
         }
         __syncthreads();
@@ -299,7 +299,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
         __syncthreads();
         for (int c = tid; c < nch; c += 256) {
             double lb, ub;
-            key_bounds<DIST>(P, f, boxes + (size_t)c * BD, &lb, &ub);
+            key_bounds<DIST>(F, f, boxes + (size_t)c * BD, &lb, &ub);
             if (lb <= U) clist[atomicAdd(&s_nclist, 1u)] = c;
         }
         __syncthreads();
@@ -310,10 +310,10 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
         if (pruned) {
             for (int e = tid; e < nclist * 64; e += 256) {
                 const int j = clist[e >> 6] * 64 + (e & 63);
-                if (j < n) fn(key_of(select_value<DIST>(P, f, j)));
+                if (j < n) fn(key_of(select_value<DIST>(F, f, P, j)));
             }
         } else {
-            for (int j = tid; j < n; j += 256) fn(key_of(select_value<DIST>(P, f, j)));
+            for (int j = tid; j < n; j += 256) fn(key_of(select_value<DIST>(F, f, P, j)));
         }
     };
     bool in_lds = false;
@@ -471,7 +471,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& P, double bw, OUT out
 // select (adaptive_bw_one) then works through with a fixed grid.
 // Returns false (nothing written) when the point is left to the workgroup select.
 template <int DIST, class PTS, class OUT>
-__device__ __forceinline__ bool adaptive_bw_wave(const PTS& P, double bw, OUT out, int f,
+__device__ __forceinline__ bool adaptive_bw_wave(const PTS& F, const PTS& P, double bw, OUT out, int f,
                                                  GWM_GLOBAL const double* boxes, unsigned int* hist,
                                                  unsigned long long* keys, int* clist) {
     const int n = P.n;
@@ -482,7 +482,7 @@ __device__ __forceinline__ bool adaptive_bw_wave(const PTS& P, double bw, OUT ou
     const long long need = (long long)b0 + 1;
     if (!(bw / n < 1) || !boxes || b0 < 1 || need > kWaveKeys || need > 64LL * nfull) return false;
     // U: upper edge of the 16-bit bin holding the m-th smallest chunk upper bound
-    const double U = wave_chunk_bound<DIST>(P, f, boxes, nfull, (need + 63) / 64 - 1, true, hist);
+    const double U = wave_chunk_bound<DIST>(F, f, boxes, nfull, (need + 63) / 64 - 1, true, hist);
     // candidate chunks, in chunk order
     int ncl = 0;
     bool overflow = false;
@@ -491,7 +491,7 @@ __device__ __forceinline__ bool adaptive_bw_wave(const PTS& P, double bw, OUT ou
         bool take = false;
         if (c < nch) {
             double lb, ub;
-            key_bounds<DIST>(P, f, boxes + (size_t)c * BD, &lb, &ub);
+            key_bounds<DIST>(F, f, boxes + (size_t)c * BD, &lb, &ub);
             take = lb <= U;
         }
         const unsigned long long m = __ballot(take);
@@ -509,7 +509,7 @@ __device__ __forceinline__ bool adaptive_bw_wave(const PTS& P, double bw, OUT ou
     const int nk = ncl * 64;
     for (int e = lane; e < nk; e += 64) {
         const int j = clist[e >> 6] * 64 + (e & 63);
-        keys[e] = j < n ? key_of(select_value<DIST>(P, f, j)) : ~0ull;
+        keys[e] = j < n ? key_of(select_value<DIST>(F, f, P, j)) : ~0ull;
     }
     wave_sync();
     unsigned long long key0, key1;
@@ -532,14 +532,14 @@ struct WaveSelectShared {
 // Fast path over points 0..n-1 (entries 0..total-1 of a batch); the points
 // it leaves are appended to left[] (*nleft must be zero beforehand).
 template <int DIST>
-__global__ __launch_bounds__(256) void adaptive_bw_wave_kernel(Points P, double bw, double* out, const double* boxes,
-                                                               int* left, unsigned int* nleft) {
+__global__ __launch_bounds__(256) void adaptive_bw_wave_kernel(Points F, Points P, double bw, double* out,
+                                                               const double* boxes, int* left, unsigned int* nleft) {
     __shared__ WaveSelectShared sh;
     const int w = threadIdx.x >> 6;
     const int f = blockIdx.x * 4 + w;
-    if (f >= P.n) return;
-    if (!adaptive_bw_wave<DIST>(global_points(P), bw, gptr(out), f, gptr(boxes), sh.hist[w], sh.keys[w],
-                                sh.clist[w]) &&
+    if (f >= F.n) return;
+    if (!adaptive_bw_wave<DIST>(global_points(F), global_points(P), bw, gptr(out), f, gptr(boxes), sh.hist[w],
+                                sh.keys[w], sh.clist[w]) &&
         (threadIdx.x & 63) == 0)
         left[atomicAdd(nleft, 1u)] = f;
 }
@@ -553,8 +553,9 @@ __global__ __launch_bounds__(256) void adaptive_bw_wave_batch_kernel(const Selec
     if (t >= start[njobs]) return;
     const int j = find_job(start, njobs, t);
     const SelectJob J = jobs[j];
-    if (!adaptive_bw_wave<DIST>(global_points(J.P), J.bw, gptr(J.out), J.f0 + t - start[j], gptr(J.boxes),
-                                sh.hist[w], sh.keys[w], sh.clist[w]) &&
+    const GPoints P = global_points(J.P);
+    if (!adaptive_bw_wave<DIST>(P, P, J.bw, gptr(J.out), J.f0 + t - start[j], gptr(J.boxes), sh.hist[w], sh.keys[w],
+                                sh.clist[w]) &&
         (threadIdx.x & 63) == 0)
         left[atomicAdd(nleft, 1u)] = t;
 }
@@ -562,12 +563,13 @@ __global__ __launch_bounds__(256) void adaptive_bw_wave_batch_kernel(const Selec
 // Workgroup select over the listed points (every point when the list is
 // null), one point per workgroup iteration on a fixed grid.
 template <int DIST>
-__global__ __launch_bounds__(256) void adaptive_bw_kernel(Points P, double bw, double* out, const double* boxes,
-                                                          const int* left, const unsigned int* nleft) {
-    const unsigned int total = left ? *nleft : (unsigned int)P.n;
+__global__ __launch_bounds__(256) void adaptive_bw_kernel(Points F, Points P, double bw, double* out,
+                                                          const double* boxes, const int* left,
+                                                          const unsigned int* nleft) {
+    const unsigned int total = left ? *nleft : (unsigned int)F.n;
     for (unsigned int i = blockIdx.x; i < total; i += gridDim.x) {
         const int f = left ? left[i] : (int)i;
-        adaptive_bw_one<DIST>(global_points(P), bw, gptr(out), f, gptr(boxes));
+        adaptive_bw_one<DIST>(global_points(F), global_points(P), bw, gptr(out), f, gptr(boxes));
         __syncthreads();
     }
 }
@@ -580,7 +582,8 @@ __global__ __launch_bounds__(256) void adaptive_bw_batch_kernel(const SelectJob*
         const int t = left[i];
         const int j = find_job(start, njobs, t);
         const SelectJob J = jobs[j];
-        adaptive_bw_one<DIST>(global_points(J.P), J.bw, gptr(J.out), J.f0 + t - start[j], gptr(J.boxes));
+        const GPoints P = global_points(J.P);
+        adaptive_bw_one<DIST>(P, P, J.bw, gptr(J.out), J.f0 + t - start[j], gptr(J.boxes));
         __syncthreads();
     }
 }
@@ -613,8 +616,9 @@ __global__ void distance_row_kernel(Points P, int focus, double* out) {
 }  // namespace
 
 hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out, const double* boxes, int* left,
-                              unsigned int* nleft, hipStream_t s) {
-    const dim3 wgrid((P.n + 3) / 4), fgrid(std::min(P.n, kFallbackGrid));
+                              unsigned int* nleft, hipStream_t s, const Points* focus) {
+    const Points F = focus ? *focus : P;
+    const dim3 wgrid((F.n + 3) / 4), fgrid(std::min(F.n, kFallbackGrid));
     const int* lst = boxes ? left : nullptr;
     if (boxes) {
         hipError_t e = hipMemsetAsync(nleft, 0, sizeof(unsigned int), s);
@@ -622,14 +626,15 @@ hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out,
     }
     if (dist == kEuclidean) {
         if (boxes)
-            hipLaunchKernelGGL(adaptive_bw_wave_kernel<kEuclidean>, wgrid, dim3(256), 0, s, P, bw, out, boxes, left,
+            hipLaunchKernelGGL(adaptive_bw_wave_kernel<kEuclidean>, wgrid, dim3(256), 0, s, F, P, bw, out, boxes, left,
                                nleft);
-        hipLaunchKernelGGL(adaptive_bw_kernel<kEuclidean>, fgrid, dim3(256), 0, s, P, bw, out, boxes, lst, nleft);
+        hipLaunchKernelGGL(adaptive_bw_kernel<kEuclidean>, fgrid, dim3(256), 0, s, F, P, bw, out, boxes, lst, nleft);
     } else {
         if (boxes)
-            hipLaunchKernelGGL(adaptive_bw_wave_kernel<kGreatCircle>, wgrid, dim3(256), 0, s, P, bw, out, boxes, left,
-                               nleft);
-        hipLaunchKernelGGL(adaptive_bw_kernel<kGreatCircle>, fgrid, dim3(256), 0, s, P, bw, out, boxes, lst, nleft);
+            hipLaunchKernelGGL(adaptive_bw_wave_kernel<kGreatCircle>, wgrid, dim3(256), 0, s, F, P, bw, out, boxes,
+                               left, nleft);
+        hipLaunchKernelGGL(adaptive_bw_kernel<kGreatCircle>, fgrid, dim3(256), 0, s, F, P, bw, out, boxes, lst,
+                           nleft);
     }
     return hipGetLastError();
 }

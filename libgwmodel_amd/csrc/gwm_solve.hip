@@ -102,12 +102,18 @@ __device__ __forceinline__ void solve_one(const SolveArgs& a, int t) {
         for (int c = 0; c <= r; ++c) A[Sym<K>::low(r, c)] = sum_parts<K>(a, Sym<K>::zcol(c, r), t);
 #pragma unroll
     for (int r = 0; r < K; ++r) b[r] = sum_parts<K>(a, M + r, t);
+    if (MODE != kSolvePredict)
 #pragma unroll
-    for (int r = 0; r < K; ++r) xi[r] = a.X[i + (size_t)r * n];
+        for (int r = 0; r < K; ++r) xi[r] = a.X[i + (size_t)r * n];
 
     if (!cholesky<K>(A, L)) {
         *a.fail = 1;
-        a.res2[i] = NAN;
+        if (MODE == kSolvePredict) {
+#pragma unroll
+            for (int p = 0; p < K; ++p) a.betas[i + (size_t)p * n] = NAN;
+        } else {
+            a.res2[i] = NAN;
+        }
         if (MODE == kSolveAIC || MODE == kSolveFit) a.sii[i] = NAN;
         if (a.cond) a.cond[i] = INFINITY;
         return;
@@ -138,6 +144,11 @@ __device__ __forceinline__ void solve_one(const SolveArgs& a, int t) {
         chol_solve<K>(L, r, dlt);
 #pragma unroll
         for (int p = 0; p < K; ++p) beta[p] += dlt[p];
+    }
+    if (MODE == kSolvePredict) {
+#pragma unroll
+        for (int p = 0; p < K; ++p) a.betas[i + (size_t)p * n] = beta[p];
+        return;
     }
     // residual: y_i - sum_a beta_a x_a (GWRBase::Fitted order)
     double yhat = 0.0;
@@ -362,6 +373,7 @@ hipError_t launch_solve_k(int mode, const SolveArgs& a, hipStream_t s) {
     if (mode == kSolveCV) hipLaunchKernelGGL((solve_kernel<K, kSolveCV>), grid, dim3(128), 0, s, a);
     else if (mode == kSolveAIC) hipLaunchKernelGGL((solve_kernel<K, kSolveAIC>), grid, dim3(128), 0, s, a);
     else if (mode == kSolveBeta) hipLaunchKernelGGL((solve_kernel<K, kSolveBeta>), grid, dim3(128), 0, s, a);
+    else if (mode == kSolvePredict) hipLaunchKernelGGL((solve_kernel<K, kSolvePredict>), grid, dim3(128), 0, s, a);
     else hipLaunchKernelGGL((solve_kernel<K, kSolveFit>), grid, dim3(128), 0, s, a);
     return hipGetLastError();
 }

@@ -102,6 +102,22 @@ void distance_vec(int n, const double* coords, bool geo, int focus, double* out)
     }
 }
 
+/* Distance from a point (fx, fy) that is not a data point to every data
+ * point: CRSDistance::distance(focus) with the prediction parameter
+ * {locations, coords} (GWRBasic.cpp:158-165, CRSDistance.cpp:122-131). */
+void distance_from(int n, const double* coords, bool geo, double fx, double fy, double* out) {
+    if (geo) {
+        for (int j = 0; j < n; ++j) out[j] = spgcdist(coords[j], fx, coords[n + j], fy);
+    } else {
+        for (int j = 0; j < n; ++j) {
+            double dx = coords[j] - fx, dy = coords[n + j] - fy;
+            double s = dx * dx;
+            s = s + dy * dy;
+            out[j] = std::sqrt(s);
+        }
+    }
+}
+
 /* ---------------- kernels ---------------- */
 
 /* BandwidthWeight.h:68-121; the compact kernels use the CPU boundary (d < bw). */
@@ -702,6 +718,36 @@ int orc_gwr_fit_core(int n, int k, const double* coords, const double* X, const 
                      int focus_count) {
     Problem P{n, k, coords, X, y, geographic != 0};
     return hat_sweep(P, kernel, adaptive != 0, bw, nthreads, focus_count, betas, betasSE, qDiag, S, shat) ? 0 : 1;
+}
+
+int orc_gwr_predict(int n, int k, const double* coords, const double* X, const double* y, int geographic,
+                    int m, const double* locations, int kernel, int adaptive, double bw, int nthreads,
+                    double* betas) {
+    const int T = nthreads_or_default(nthreads);
+    int fail = 0;
+#pragma omp parallel num_threads(T)
+    {
+        std::vector<double> d(n), w(n), scratch, A(k * k), b(k);
+#pragma omp for schedule(static)
+        for (int i = 0; i < m; ++i) {
+            distance_from(n, coords, geographic != 0, locations[i], locations[m + i], d.data());
+            weight_vec(d.data(), n, kernel, adaptive != 0, bw, w.data(), scratch);
+            normal_equations(n, k, X, y, w.data(), A.data(), b.data());
+            try {
+                inv_sympd(A.data(), k);
+                for (int a = 0; a < k; ++a) {
+                    double s = 0.0;
+                    for (int c = 0; c < k; ++c) s += A[a + c * k] * b[c];
+                    betas[i + (size_t)a * m] = s;
+                }
+            } catch (const NotSpd&) {
+                for (int a = 0; a < k; ++a) betas[i + (size_t)a * m] = NAN;
+#pragma omp atomic write
+                fail = 1;
+            }
+        }
+    }
+    return fail;
 }
 
 int orc_gwr_run(int n, int k, const double* coords, const double* X, const double* y,

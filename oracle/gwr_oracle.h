@@ -49,6 +49,14 @@ int orc_gwr_fit_core(int n, int k, const double* coords, const double* X, const 
                      double* betas, double* betasSE, double* shat, double* qDiag, double* S,
                      int focus_count);
 
+/* GWRBasic::predictSerial (GWRBasic.cpp:167-191): betas (m*k) at the m
+ * locations (m*2, column-major) from the data (coords, X, y); adaptive
+ * bandwidths count data points nearest each location.  Returns 0 ok, 1 when
+ * a local design is not SPD (those rows NaN; the reference throws). */
+int orc_gwr_predict(int n, int k, const double* coords, const double* X, const double* y, int geographic,
+                    int m, const double* locations, int kernel, int adaptive, double bw, int nthreads,
+                    double* betas);
+
 /* GWRBasic::fit() orchestration (GWRBasic.cpp:36-128): optional golden-section
  * bandwidth selection, fit, diagnostics, SE scaling.
  * diag[7] = RSS, AIC, AICc, ENP, EDF, R2, R2adj (RegressionDiagnostic order).

@@ -45,6 +45,8 @@ def lib():
         L.orc_gwr_fit_core.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
                                        ctypes.c_int]
+        L.orc_gwr_predict.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
         L.orc_gwr_run.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
@@ -128,6 +130,18 @@ def fit_core(coords, X, y, kernel, adaptive, bw, geographic=False, nthreads=0, w
     if rc != 0:
         return None
     return dict(betas=betas, betasSE_raw=se, shat=shat, qDiag=q, S=S)
+
+
+def predict(coords, X, y, locations, kernel, adaptive, bw, geographic=False, nthreads=0):
+    """predictSerial: betas (m x k) at the locations, or None on non-SPD."""
+    c, Xc, yc = _colmajor(coords), _colmajor(X), _f64(y)
+    loc = _colmajor(locations)
+    n, k = Xc.shape
+    m = loc.shape[0]
+    betas = np.zeros((m, k), order="F")
+    rc = lib().orc_gwr_predict(n, k, _p(c), _p(Xc), _p(yc), int(geographic), m, _p(loc), int(kernel),
+                               int(adaptive), float(bw), int(nthreads), _p(betas))
+    return None if rc else betas
 
 
 def gwr_run(coords, X, y, kernel, adaptive, bw, select_bw=False, crit=CRIT_AIC, lower=None, upper=None,

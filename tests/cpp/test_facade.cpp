@@ -87,6 +87,26 @@ int main(int argc, char** argv) {
         REQUIRE_NEAR(d.RSquareAdjust, 0.674975341723766, 1e-8);
         REQUIRE(algorithm.s().n_rows() == n);   // n < 8192: S is stored
     }
+    {   // predict at the data coordinates: the fit's betas (same weights and normal equations)
+        CRSDistance distance(false);
+        BandwidthWeight bandwidth(36, true, BandwidthWeight::Gaussian);
+        SpatialWeight spatial(&bandwidth, &distance);
+        GWRBasic algorithm;
+        algorithm.setCoords(coords);
+        algorithm.setDependentVariable(y);
+        algorithm.setIndependentVariables(x);
+        algorithm.setSpatialWeight(spatial);
+        algorithm.setParallelType(ParallelType::HIP);
+        Mat fitted = algorithm.fit();
+        Mat predicted = algorithm.predict(coords);
+        REQUIRE(predicted.n_rows() == n && predicted.n_cols() == 4);
+        double worst = 0.0;
+        for (std::size_t i = 0; i < n; ++i)
+            for (std::size_t j = 0; j < 4; ++j)
+                worst = std::fmax(worst, std::fabs(predicted(i, j) - fitted(i, j)) / (1.0 + std::fabs(fitted(i, j))));
+        REQUIRE_NEAR(worst, 0.0, 1e-9);
+        REQUIRE(algorithm.status() == Status::Success);
+    }
     {   // fixed bandwidth | no bandwidth optimization
         CRSDistance distance(false);
         BandwidthWeight bandwidth(5000, false, BandwidthWeight::Gaussian);

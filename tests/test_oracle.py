@@ -242,3 +242,85 @@ def test_brute_force_dense(oracle):
         assert np.allclose(f["shat"], shat, rtol=1e-10)
         assert np.allclose(f["qDiag"], q, rtol=1e-10)
         assert np.allclose(f["S"], S, rtol=1e-10, atol=1e-13)
+
+
+# --------------------------------------------------------------- predict (GWRBasic.cpp:167-191)
+
+def dense_predict(coords, X, y, loc, kern, adaptive, bw):
+    """Independent dense restatement of predictSerial at locations that are not data points."""
+    m, k = loc.shape[0], X.shape[1]
+    D = np.sqrt(((loc[:, None, :] - coords[None, :, :]) ** 2).sum(-1))
+    betas = np.zeros((m, k))
+    for i in range(m):
+        d = D[i]
+        b = bw
+        if adaptive:
+            s = np.sort(d)
+            b0 = int(math.floor(bw))
+            b = s[b0 - 1] + (s[b0] - s[b0 - 1]) * (bw - b0)
+        if kern == 0:
+            w = np.exp(-0.5 * (d / b) ** 2)
+        else:  # bisquare
+            w = np.where(d < b, (1 - (d / b) ** 2) ** 2, 0.0)
+        betas[i] = np.linalg.solve(X.T @ (w[:, None] * X), X.T @ (w * y))
+    return betas
+
+
+def test_predict_at_data_points_is_fit(oracle, londonhp100):
+    # the KAT configuration (adaptive 36, Gaussian): at the data coordinates the
+    # prediction weights are the fit's, so the betas are the fit's betas
+    c, X, y = londonhp100["coords"], londonhp100["X"], londonhp100["y"]
+    f = oracle.fit_core(c, X, y, oracle.GAUSSIAN, True, 36)
+    p = oracle.predict(c, X, y, c, oracle.GAUSSIAN, True, 36)
+    assert np.allclose(p, f["betas"], rtol=1e-10, atol=1e-12)
+    f = oracle.fit_core(c, X, y, oracle.BISQUARE, False, 5000)
+    p = oracle.predict(c, X, y, c, oracle.BISQUARE, False, 5000)
+    assert np.allclose(p, f["betas"], rtol=1e-10, atol=1e-12)
+
+
+def test_predict_at_data_points_is_fit_geographic(oracle):
+    rng = np.random.default_rng(3)
+    n = 80
+    coords = np.column_stack([rng.uniform(100, 120, n), rng.uniform(20, 40, n)])
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, 2))])
+    y = X @ np.array([1.0, 0.5, -0.5]) + rng.normal(size=n)
+    f = oracle.fit_core(coords, X, y, oracle.GAUSSIAN, True, 30, geographic=True)
+    p = oracle.predict(coords, X, y, coords, oracle.GAUSSIAN, True, 30, geographic=True)
+    assert np.allclose(p, f["betas"], rtol=1e-10, atol=1e-12)
+
+
+def test_predict_ols_limit(oracle):
+    rng = np.random.default_rng(12)
+    n = 60
+    coords = rng.uniform(0, 10, (n, 2))
+    loc = rng.uniform(-5, 15, (25, 2))
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, 2))])
+    y = X @ np.array([1.0, -2.0, 0.5]) + rng.normal(size=n)
+    beta, *_ = np.linalg.lstsq(X, y, rcond=None)
+    for kern, adaptive, bw in [(oracle.BOXCAR, False, 1e6), (oracle.GAUSSIAN, False, 1e9), (oracle.BOXCAR, True, 2 * n)]:
+        p = oracle.predict(coords, X, y, loc, kern, adaptive, bw)
+        assert np.allclose(p, beta[None, :], rtol=1e-9, atol=1e-12)
+
+
+def test_predict_brute_force_dense(oracle):
+    rng = np.random.default_rng(6)
+    n = 50
+    coords = rng.uniform(0, 10, (n, 2))
+    loc = rng.uniform(0, 10, (30, 2))
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, 2))])
+    y = X @ np.array([0.3, 1.0, -1.0]) + rng.normal(size=n)
+    for kern, adaptive, bw in [(oracle.GAUSSIAN, False, 3.0), (oracle.BISQUARE, True, 25.5),
+                               (oracle.GAUSSIAN, True, 7)]:
+        p = oracle.predict(coords, X, y, loc, kern, adaptive, bw)
+        assert np.allclose(p, dense_predict(coords, X, y, loc, kern, adaptive, bw), rtol=1e-10, atol=1e-12)
+
+
+def test_predict_not_spd(oracle):
+    rng = np.random.default_rng(7)
+    n = 40
+    coords = rng.uniform(0, 10, (n, 2))
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    y = rng.normal(size=n)
+    loc = np.array([[5.0, 5.0], [500.0, 500.0]])
+    p = oracle.predict(coords, X, y, loc, oracle.BISQUARE, False, 20.0)
+    assert p is None
