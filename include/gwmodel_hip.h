@@ -94,6 +94,28 @@ int gwm_hip_fit(gwm_hip_ctx* ctx, int kernel, int adaptive, double bw, int preci
 int gwm_hip_predict(gwm_hip_ctx* ctx, int m, const double* locations, int kernel, int adaptive, double bw,
                     int precision, double* betas);
 
+/* Variable-selection criterion of one column set: replaces
+ * GWRBasic::indepVarsSelectionCriterionSerial / Omp / Cuda
+ * (src/gwmodelpp/GWRBasic.cpp:309-346, :537-580, :838-900), the
+ * IndepVarsSelectCriterionCalculator of GWRBasic.h:53.  Columns are
+ * {0 if has_intercept} + vars (VariableForwardSelector::index2uvec,
+ * VariableForwardSelector.h:26-35); the regression has unit weights (:319), so
+ * *value is the AICc of the global OLS with tr(S) = sum_i x_i^T (X^T X)^-1 x_i,
+ * or DBL_MAX when X^T X is not SPD or the value is not finite. */
+int gwm_hip_indep_vars_criterion(gwm_hip_ctx* ctx, int has_intercept, const int* vars, int nvars, double* value);
+
+/* Forward selection of independent variables: replaces
+ * VariableForwardSelector::optimize / sort / select
+ * (src/gwmodelpp/VariableForwardSelector.cpp:9-77) as GWRBasic::fit runs it
+ * (GWRBasic.cpp:50-72) over the columns after the intercept.  selected (cap k)
+ * receives the chosen columns, *nselected their count.  The criterion list
+ * (GWRBasic::indepVarsSelectionCriterionList) has m(m+1)/2 models for m
+ * candidate columns: model q has model_nvars[q] columns in
+ * model_vars[q*k .. q*k+k) (-1 padded) and criterion model_crit[q]; at most
+ * cap_models are written, *nmodels receives the full count. */
+int gwm_hip_indep_vars_select(gwm_hip_ctx* ctx, int has_intercept, double threshold, int* selected, int* nselected,
+                              int* model_vars, int* model_nvars, double* model_crit, int cap_models, int* nmodels);
+
 /* Golden-section bandwidth selection on device-resident data: replaces
  * BandwidthSelector::optimize (src/gwmodelpp/BandwidthSelector.cpp:6-81)
  * driving getCriterion.  Bounds follow GWRBasic::fit (GWRBasic.cpp:76-79):

@@ -178,6 +178,8 @@ private:
 };
 
 typedef std::vector<std::pair<double, double>> BandwidthCriterionList;
+// IVarialbeSelectable.h:82
+typedef std::vector<std::pair<std::vector<std::size_t>, double>> VariablesCriterionList;
 
 // Owns one device context; shared by the algorithm objects below.
 class HipContext {
@@ -221,6 +223,13 @@ public:
     void setGoldenUpperBounds(double value) { mGoldenUpperBounds = value; }
     void setGoldenLowerBounds(double value) { mGoldenLowerBounds = value; }
     const BandwidthCriterionList& bandwidthSelectionCriterionList() const { return mBandwidthSelectionCriterionList; }
+    // IVarialbeSelectable (GWRBasic.h:213-269, 380-383)
+    void setIsAutoselectIndepVars(bool v) { mIsAutoselectIndepVars = v; }
+    bool isAutoselectIndepVars() const { return mIsAutoselectIndepVars; }
+    void setIndepVarSelectionThreshold(double threshold) { mIndepVarSelectionThreshold = threshold; }
+    double indepVarSelectionThreshold() const { return mIndepVarSelectionThreshold; }
+    const VariablesCriterionList& indepVarsSelectionCriterionList() const { return mIndepVarsSelectionCriterionList; }
+    std::vector<std::size_t> selectedVariables() const { return mSelectedIndepVars; }
 
     // IParallelizable (IParallelizable.h:35-69): only the HIP back-end exists here.
     int parallelAbility() const { return ParallelType::HIP; }
@@ -254,6 +263,10 @@ private:
     bool mHasHatMatrix = true;
     bool mHasIntercept = true;
     bool mIsAutoselectBandwidth = false;
+    bool mIsAutoselectIndepVars = false;
+    double mIndepVarSelectionThreshold = 3.0;
+    VariablesCriterionList mIndepVarsSelectionCriterionList;
+    std::vector<std::size_t> mSelectedIndepVars;
     BandwidthSelectionCriterionType mBandwidthSelectionCriterion = AIC;
     std::optional<double> mGoldenUpperBounds, mGoldenLowerBounds;
     BandwidthCriterionList mBandwidthSelectionCriterionList;

@@ -22,8 +22,8 @@ FACADE_SRC = os.path.join(HERE, "cpp", "gwr_facade.cpp")
 FACADE_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
 FACADE_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_facade")
 
-SOURCES = ["gwm_contract.hip", "gwm_contract_mixed.hip", "gwm_select.hip", "gwm_solve.hip", "gwm_api.cpp",
-           "gwm_batch.cpp"]
+SOURCES = ["gwm_contract.hip", "gwm_contract_mixed.hip", "gwm_select.hip", "gwm_solve.hip", "gwm_varsel.hip",
+           "gwm_api.cpp", "gwm_batch.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

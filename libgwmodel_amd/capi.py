@@ -42,6 +42,8 @@ SIGNATURES = {
     "gwm_hip_criterion": (_i, [_vp, _i, _i, _d, _i, _i, _dp]),
     "gwm_hip_fit": (_i, [_vp, _i, _i, _d, _i, _dp, _dp, _dp, _dp, _dp]),
     "gwm_hip_predict": (_i, [_vp, _i, _dp, _i, _i, _d, _i, _dp]),
+    "gwm_hip_indep_vars_criterion": (_i, [_vp, _i, _ip, _i, _dp]),
+    "gwm_hip_indep_vars_select": (_i, [_vp, _i, _d, _ip, _ip, _ip, _ip, _dp, _i, _ip]),
     "gwm_hip_select_bandwidth": (_i, [_vp, _i, _i, _i, _i, _i, _d, _i, _d, _dp, _dp, _dp, _i, _ip]),
     "gwm_hip_gwr_run": (_i, [_vp, _i, _i, _d, _i, _i, _i, _i, _d, _i, _d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                              _dp, _i, _ip]),
@@ -192,6 +194,28 @@ class Context:
         self._check(self._lib.gwm_hip_predict(self.h, m, _p(loc), int(kernel), int(adaptive), float(bw),
                                               int(precision), _p(betas)))
         return betas
+
+    def indep_vars_criterion(self, vars, has_intercept=True):
+        """AICc of the unit-weight regression on columns {0} + vars (DBL_MAX if singular)."""
+        v = np.ascontiguousarray(vars, dtype=np.int32)
+        out = np.zeros(1)
+        self._check(self._lib.gwm_hip_indep_vars_criterion(self.h, int(has_intercept), _ipa(v), len(v), _p(out)))
+        return float(out[0])
+
+    def indep_vars_select(self, threshold=3.0, has_intercept=True):
+        """Forward selection: (selected columns, [(columns, criterion), ...])."""
+        k = self.k
+        cap = k * (k + 1) // 2
+        sel = np.zeros(k, dtype=np.int32)
+        nsel = np.zeros(1, dtype=np.int32)
+        mv = np.zeros(cap * k, dtype=np.int32)
+        mn = np.zeros(cap, dtype=np.int32)
+        mc = np.zeros(cap)
+        nm = np.zeros(1, dtype=np.int32)
+        self._check(self._lib.gwm_hip_indep_vars_select(self.h, int(has_intercept), float(threshold), _ipa(sel),
+                                                        _ipa(nsel), _ipa(mv), _ipa(mn), _p(mc), cap, _ipa(nm)))
+        models = [(list(mv[q * k:q * k + mn[q]]), float(mc[q])) for q in range(min(int(nm[0]), cap))]
+        return list(sel[:int(nsel[0])]), models
 
     def select_bandwidth(self, kernel, adaptive, crit, precision=FP64, lower=None, upper=None, trace_cap=256):
         bw = np.zeros(1)

@@ -47,7 +47,8 @@ HipContext::~HipContext() { gwm_hip_destroy(mCtx); }
 
 Mat GWRBasic::fit() {
     mStatus = Status::Success;
-    const std::size_t n = mCoords.n_rows(), k = mX.n_cols();
+    const std::size_t n = mCoords.n_rows();
+    std::size_t k = mX.n_cols();
     if (mParallelType != ParallelType::HIP) throw std::logic_error("parallel type not available in this build");
     if (mCoords.n_cols() != 2 || mX.n_rows() != n || mY.n_elem() != n)
         throw std::invalid_argument("coords must be n x 2, X n x k, y n");
@@ -59,6 +60,41 @@ Mat GWRBasic::fit() {
     if (rc) raise(ctx.get(), rc);
     const int prec = mPrecision == Precision::Mixed ? GWM_MIXED : GWM_FP64;
     const int crit = mBandwidthSelectionCriterion == CV ? GWM_CRIT_CV : GWM_CRIT_AIC;
+
+    if (mIsAutoselectIndepVars) {
+        // GWRBasic.cpp:50-72: forward selection over the columns after the
+        // intercept, then the fit runs on {intercept} + the selected columns
+        const std::size_t kall = k, cap = k * (k + 1) / 2;
+        std::vector<int> sel(k), mv(cap * k), mn(cap);
+        std::vector<double> mc(cap);
+        int nsel = 0, nmod = 0;
+        rc = gwm_hip_indep_vars_select(ctx.get(), mHasIntercept ? 1 : 0, mIndepVarSelectionThreshold, sel.data(),
+                                       &nsel, mv.data(), mn.data(), mc.data(), (int)cap, &nmod);
+        if (rc == GWM_ERR_CANCELLED) {
+            mStatus = Status::Terminated;
+            mBetas = Mat(n, k);
+            return mBetas;
+        }
+        if (rc) raise(ctx.get(), rc);
+        mSelectedIndepVars.assign(sel.begin(), sel.begin() + nsel);
+        if (nsel > 0) {
+            std::vector<std::size_t> cols;
+            if (mHasIntercept) cols.push_back(0);
+            for (std::size_t v : mSelectedIndepVars) cols.push_back(v);
+            Mat xs(n, cols.size());
+            for (std::size_t a = 0; a < cols.size(); ++a) xs.set_col(a, mX.col(cols[a]));
+            mX = xs;
+            k = cols.size();
+            mIndepVarsSelectionCriterionList.clear();
+            for (int q = 0; q < nmod && q < (int)cap; ++q) {
+                const std::size_t row = (std::size_t)q * kall;
+                std::vector<std::size_t> vars(mv.begin() + row, mv.begin() + row + mn[q]);
+                mIndepVarsSelectionCriterionList.emplace_back(vars, mc[q]);
+            }
+            rc = gwm_hip_set_data(ctx.get(), (int)n, (int)k, mCoords.memptr(), mX.memptr(), mY.memptr(), dist);
+            if (rc) raise(ctx.get(), rc);
+        }
+    }
 
     mBetas = Mat(n, k);
     mBetasSE = Mat(n, k);

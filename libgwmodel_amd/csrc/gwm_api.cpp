@@ -332,6 +332,17 @@ SolveArgs solve_args(gwm_hip_ctx* c, int PT, int nseg, int colB, const double* b
     return s;
 }
 
+// Device memory owned by one API call.
+struct DevBuf {
+    void* p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) hipFree(p);
+    }
+};
+
 // Focus points of a prediction (gwm_hip_predict): their table in spatial
 // order and the device arrays the solves write.
 struct PredictSet {
@@ -819,12 +830,7 @@ int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adapt
     }
     if (gc) sphere_tables(M, tab.data(), tab.data() + M, &tab[2 * M], &tab[3 * M], &tab[4 * M], &tab[5 * M]);
     // device arrays: tables, bandwidths, betas, indicator; lists
-    struct Owned {
-        void* p = nullptr;
-        ~Owned() {
-            if (p) hipFree(p);
-        }
-    } dbuf, ibuf;
+    DevBuf dbuf, ibuf;
     const size_t nd = M * (ntab + 2 + k);
     GWM_HIP_CHECK(c, hipMalloc(&dbuf.p, nd * sizeof(double)));
     GWM_HIP_CHECK(c, hipMalloc(&ibuf.p, (2 * M + 1) * sizeof(int)));
@@ -914,6 +920,65 @@ int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adapt
         c->err = "inv_sympd(): matrix is singular or not positive definite";
         return GWM_ERR_NOT_SPD;
     }
+    return GWM_OK;
+}
+
+// Criteria of candidate column sets (GWRBasic::indepVarsSelectionCriterion,
+// GWRBasic.cpp:309-346): columns {0 if has_intercept} + vars; RSS and tr(S)
+// on the device from the Gram matrix G of [X y] (launch_gram), AICc here;
+// DBL_MAX when the design is not SPD or the value is not finite
+// (GWRBasic.cpp:893).
+int varsel_criteria(gwm_hip_ctx* c, const double* G, int has_intercept, const std::vector<std::vector<int>>& sets,
+                    std::vector<double>& crit) {
+    const int k = c->k, ns = (int)sets.size();
+    crit.assign(ns, DBL_MAX);
+    if (ns == 0) return GWM_OK;
+    std::vector<int> h(ns * (k + 1), 0);
+    for (int q = 0; q < ns; ++q) {
+        int p = 0;
+        if (has_intercept) h[q * k + p++] = 0;
+        for (int v : sets[q]) h[q * k + p++] = v;
+        h[ns * k + q] = p;
+    }
+    DevBuf ib, ob;
+    GWM_HIP_CHECK(c, hipMalloc(&ib.p, h.size() * sizeof(int)));
+    GWM_HIP_CHECK(c, hipMalloc(&ob.p, 2 * (size_t)ns * sizeof(double)));
+    int* dsets = static_cast<int*>(ib.p);
+    double* dout = static_cast<double*>(ob.p);
+    GWM_HIP_CHECK(c, hipMemcpyAsync(dsets, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    timer_begin(c);
+    GWM_HIP_CHECK(c, launch_varsel(c->X, c->y, c->n, k, G, dsets, dsets + ns * k, ns, dout, c->stream));
+    int rc = timer_end(c, "varsel");
+    if (rc) return rc;
+    std::vector<double> o(2 * ns);
+    GWM_HIP_CHECK(c, hipMemcpyAsync(o.data(), dout, o.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < ns; ++q) {
+        const double v = aicc(o[2 * q], (double)c->n, o[2 * q + 1]);
+        crit[q] = std::isfinite(v) ? v : DBL_MAX;
+    }
+    c->solves += ns;
+    return GWM_OK;
+}
+
+int gram(gwm_hip_ctx* c, DevBuf& g) {
+    const int k = c->k;
+    GWM_HIP_CHECK(c, hipMalloc(&g.p, (size_t)(k + 1) * (k + 2) / 2 * sizeof(double)));
+    GWM_HIP_CHECK(c, launch_gram(c->X, c->y, c->n, k, static_cast<double*>(g.p), c->stream));
+    return GWM_OK;
+}
+
+int check_vars(gwm_hip_ctx* c, int has_intercept, const int* vars, int nvars) {
+    const int first = has_intercept ? 1 : 0;
+    if (nvars < 0 || (nvars > 0 && !vars) || nvars + first < 1 || nvars + first > c->k) {
+        c->err = "variables: need 1 <= columns <= k";
+        return GWM_ERR_INVALID;
+    }
+    for (int i = 0; i < nvars; ++i)
+        if (vars[i] < first || vars[i] >= c->k) {
+            c->err = "variables: column index out of range";
+            return GWM_ERR_INVALID;
+        }
     return GWM_OK;
 }
 
@@ -1370,6 +1435,88 @@ int gwm_hip_predict(gwm_hip_ctx* c, int m, const double* locations, int kernel, 
     }
     GWM_HIP_CHECK(c, hipSetDevice(c->device));
     return predict_impl(c, m, locations, kernel, adaptive, bw, precision, betas);
+}
+
+int gwm_hip_indep_vars_criterion(gwm_hip_ctx* c, int has_intercept, const int* vars, int nvars, double* value) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!value) return GWM_ERR_INVALID;
+    *value = DBL_MAX;
+    if ((rc = check_vars(c, has_intercept, vars, nvars))) return rc;
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    if (stopped(c)) return GWM_ERR_CANCELLED;
+    DevBuf g;
+    if ((rc = gram(c, g))) return rc;
+    std::vector<double> crit;
+    if ((rc = varsel_criteria(c, static_cast<double*>(g.p), has_intercept, {std::vector<int>(vars, vars + nvars)},
+                              crit)))
+        return rc;
+    *value = crit[0];
+    return GWM_OK;
+}
+
+// VariableForwardSelector::optimize / sort / select
+// (src/gwmodelpp/VariableForwardSelector.cpp:9-77): each step adds the
+// remaining column with the smallest criterion (first on ties, like
+// arma::index_min); the model list is then ordered by size and, within a size,
+// by descending criterion (a stable sort: the reference's std::sort leaves
+// equal values in an unspecified order); the selection is the last model
+// whose predecessor's criterion is larger by at least the threshold, else the
+// last model (the reference's loop reads before the first element there).
+int gwm_hip_indep_vars_select(gwm_hip_ctx* c, int has_intercept, double threshold, int* selected, int* nselected,
+                              int* model_vars, int* model_nvars, double* model_crit, int cap_models, int* nmodels) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!selected || !nselected || !nmodels || (cap_models > 0 && (!model_vars || !model_nvars || !model_crit)))
+        return GWM_ERR_INVALID;
+    GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    const int k = c->k;
+    std::vector<int> vars;
+    for (int col = has_intercept ? 1 : 0; col < k; ++col) vars.push_back(col);
+    const int m = (int)vars.size();
+    std::vector<int> cur, rest(vars);
+    std::vector<std::pair<std::vector<int>, double>> models;
+    DevBuf g;
+    if (m > 0 && (rc = gram(c, g))) return rc;
+    for (int step = 0; step < m; ++step) {
+        if (stopped(c)) return GWM_ERR_CANCELLED;
+        std::vector<std::vector<int>> sets;
+        for (int v : rest) {
+            sets.push_back(cur);
+            sets.back().push_back(v);
+        }
+        std::vector<double> crit;
+        if ((rc = varsel_criteria(c, static_cast<double*>(g.p), has_intercept, sets, crit))) return rc;
+        for (size_t j = 0; j < sets.size(); ++j) models.emplace_back(sets[j], crit[j]);
+        const size_t best = std::min_element(crit.begin(), crit.end()) - crit.begin();
+        cur.push_back(rest[best]);
+        rest.erase(rest.begin() + best);
+    }
+    size_t tag = 0;
+    for (int size = m; size > 0; --size) {
+        std::stable_sort(models.begin() + tag, models.begin() + tag + size,
+                         [](const std::pair<std::vector<int>, double>& l,
+                            const std::pair<std::vector<int>, double>& r) { return l.second > r.second; });
+        tag += size;
+    }
+    *nselected = 0;
+    if (!models.empty()) {
+        size_t pick = models.size() - 1;
+        for (size_t i = models.size() - 1; i >= 1; --i)
+            if (models[i - 1].second - models[i].second >= threshold) {
+                pick = i;
+                break;
+            }
+        for (int v : models[pick].first) selected[(*nselected)++] = v;
+    }
+    *nmodels = (int)models.size();
+    for (int q = 0; q < std::min(cap_models, (int)models.size()); ++q) {
+        model_nvars[q] = (int)models[q].first.size();
+        for (int a = 0; a < k; ++a)
+            model_vars[(size_t)q * k + a] = a < model_nvars[q] ? models[q].first[a] : -1;
+        model_crit[q] = models[q].second;
+    }
+    return GWM_OK;
 }
 
 int gwm_hip_set_certify_tolerance(gwm_hip_ctx* c, double rel) {

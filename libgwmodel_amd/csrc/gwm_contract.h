@@ -181,6 +181,13 @@ hipError_t launch_smatrix(int dist, int k, const Points& P, const double* X, con
 // Focus points whose conditioning indicator exceeds limit (or is not finite):
 // their indices go to list[] (unordered), their number to *count, which must
 // be zero beforehand.
+// Independent-variable selection: packed lower triangle of [X y]^T [X y]
+// ((k+1)(k+2)/2 entries), then per candidate column set (sets: nsets x k,
+// nvars[s] used) out[2s] = RSS and out[2s+1] = tr(S) of its OLS (NaN: not SPD).
+hipError_t launch_gram(const double* X, const double* y, int n, int k, double* G, hipStream_t s);
+hipError_t launch_varsel(const double* X, const double* y, int n, int k, const double* G, const int* sets,
+                         const int* nvars, int nsets, double* out, hipStream_t s);
+
 hipError_t launch_compact(const double* cond, int n, double limit, int* list, int* count, hipStream_t s);
 
 // Deterministic sums of up to 3 vectors of length n into out[0..2], and the

@@ -750,6 +750,107 @@ int orc_gwr_predict(int n, int k, const double* coords, const double* X, const d
     return fail;
 }
 
+double orc_indep_vars_criterion(int n, int k, const double* X, const double* y, int has_intercept,
+                                const int* vars, int nvars) {
+    // x = mX.cols(index2uvec(vars, hasIntercept)) (VariableForwardSelector.h:26-35)
+    std::vector<int> cols;
+    if (has_intercept) cols.push_back(0);
+    for (int v = 0; v < nvars; ++v) cols.push_back(vars[v]);
+    const int p = (int)cols.size();
+    std::vector<double> x((size_t)n * p), w(n, 1.0), A(p * p), b(p), beta(p), ci(p);
+    for (int a = 0; a < p; ++a)
+        for (int i = 0; i < n; ++i) x[i + (size_t)a * n] = X[i + (size_t)cols[a] * n];
+    (void)k;
+    // unit weights (GWRBasic.cpp:319): every location has the same xtwx, xtwy
+    normal_equations(n, p, x.data(), y, w.data(), A.data(), b.data());
+    try {
+        inv_sympd(A.data(), p);
+    } catch (const NotSpd&) {
+        return DBL_MAX;
+    }
+    for (int a = 0; a < p; ++a) {
+        double t = 0.0;
+        for (int c = 0; c < p; ++c) t += A[a + c * p] * b[c];
+        beta[a] = t;
+    }
+    // shat(0) = sum_i si(0, i) with si = x_i^T inv(xtwx) xtw, i.e. x_i^T inv x_i
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) {
+        for (int a = 0; a < p; ++a) {
+            double t = 0.0;
+            for (int c = 0; c < p; ++c) t += A[a + c * p] * x[i + (size_t)c * n];
+            ci[a] = t;
+        }
+        double sii = 0.0;
+        for (int a = 0; a < p; ++a) sii += x[i + (size_t)a * n] * ci[a];
+        tr += sii;
+    }
+    std::vector<double> betas((size_t)n * p);
+    for (int a = 0; a < p; ++a)
+        for (int i = 0; i < n; ++i) betas[i + (size_t)a * n] = beta[a];
+    const double v = aicc_of(rss_of(n, p, x.data(), y, betas.data()), n, tr);
+    return std::isfinite(v) ? v : DBL_MAX;
+}
+
+int orc_indep_vars_select(int n, int k, const double* X, const double* y, int has_intercept, double threshold,
+                          int* selected, int* nselected, int* model_vars, int* model_nvars, double* model_crit,
+                          int* nmodels) {
+    // VariableForwardSelector::optimize (VariableForwardSelector.cpp:9-42)
+    std::vector<int> vars;
+    for (int c = has_intercept ? 1 : 0; c < k; ++c) vars.push_back(c);
+    const int m = (int)vars.size();
+    std::vector<int> cur, rest;
+    for (int i = 0; i < m; ++i) rest.push_back(i);
+    std::vector<std::pair<std::vector<int>, double>> models;
+    auto to_vars = [&](const std::vector<int>& idx) {
+        std::vector<int> v;
+        for (int i : idx) v.push_back(vars[i]);
+        return v;
+    };
+    for (int i = 0; i < m; ++i) {
+        std::vector<double> crit(rest.size());
+        for (size_t j = 0; j < rest.size(); ++j) {
+            cur.push_back(rest[j]);
+            const std::vector<int> v = to_vars(cur);
+            crit[j] = orc_indep_vars_criterion(n, k, X, y, has_intercept, v.data(), (int)v.size());
+            models.emplace_back(cur, crit[j]);
+            cur.pop_back();
+        }
+        const size_t best = std::min_element(crit.begin(), crit.end()) - crit.begin();
+        cur.push_back(rest[best]);
+        rest.erase(rest.begin() + best);
+    }
+    // sort (:53-65): each group of equal size in descending criterion
+    size_t tag = 0;
+    for (int g = m; g > 0; --g) {
+        std::stable_sort(models.begin() + tag, models.begin() + tag + g,
+                         [](const std::pair<std::vector<int>, double>& l, const std::pair<std::vector<int>, double>& r) {
+                             return l.second > r.second;
+                         });
+        tag += g;
+    }
+    // select (:67-77): the last model after which the criterion drops by >= threshold
+    *nselected = 0;
+    if (!models.empty()) {
+        size_t pick = models.size() - 1;
+        bool found = false;
+        for (size_t i = models.size() - 1; i >= 1 && !found; --i)
+            if (models[i - 1].second - models[i].second >= threshold) {
+                pick = i;
+                found = true;
+            }
+        for (int v : to_vars(models[pick].first)) selected[(*nselected)++] = v;
+    }
+    *nmodels = (int)models.size();
+    for (size_t q = 0; q < models.size(); ++q) {
+        const std::vector<int> v = to_vars(models[q].first);
+        model_nvars[q] = (int)v.size();
+        for (int a = 0; a < k; ++a) model_vars[q * k + a] = a < (int)v.size() ? v[a] : -1;
+        model_crit[q] = models[q].second;
+    }
+    return 0;
+}
+
 int orc_gwr_run(int n, int k, const double* coords, const double* X, const double* y,
                 int geographic, int kernel, int adaptive, double bw,
                 int select_bw, int crit, int has_lower, double lower, int has_upper, double upper,

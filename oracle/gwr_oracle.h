@@ -57,6 +57,22 @@ int orc_gwr_predict(int n, int k, const double* coords, const double* X, const d
                     int m, const double* locations, int kernel, int adaptive, double bw, int nthreads,
                     double* betas);
 
+/* GWRBasic::indepVarsSelectionCriterionSerial (GWRBasic.cpp:309-346): AICc of
+ * the regressions on columns {0 if has_intercept} + vars with unit weights
+ * (every location solves the same global OLS, :319); DBL_MAX when the design
+ * is not SPD or the value is not finite (GWRBasic.cpp:893). */
+double orc_indep_vars_criterion(int n, int k, const double* X, const double* y, int has_intercept,
+                                const int* vars, int nvars);
+
+/* VariableForwardSelector::optimize / sort / select
+ * (VariableForwardSelector.cpp:9-77) over the columns of X after the
+ * intercept.  selected (cap k) gets the chosen columns; the criterion list
+ * (m(m+1)/2 models, m candidate columns) goes to model_vars (row q: k entries,
+ * -1 padded), model_nvars and model_crit in the reference's sorted order. */
+int orc_indep_vars_select(int n, int k, const double* X, const double* y, int has_intercept, double threshold,
+                          int* selected, int* nselected, int* model_vars, int* model_nvars, double* model_crit,
+                          int* nmodels);
+
 /* GWRBasic::fit() orchestration (GWRBasic.cpp:36-128): optional golden-section
  * bandwidth selection, fit, diagnostics, SE scaling.
  * diag[7] = RSS, AIC, AICc, ENP, EDF, R2, R2adj (RegressionDiagnostic order).

@@ -47,6 +47,10 @@ def lib():
                                        ctypes.c_int]
         L.orc_gwr_predict.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_indep_vars_criterion.restype = ctypes.c_double
+        L.orc_indep_vars_criterion.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, ctypes.c_int]
+        L.orc_indep_vars_select.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                            _ip, _ip, _ip, _ip, _dp, _ip]
         L.orc_gwr_run.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
@@ -142,6 +146,31 @@ def predict(coords, X, y, locations, kernel, adaptive, bw, geographic=False, nth
     rc = lib().orc_gwr_predict(n, k, _p(c), _p(Xc), _p(yc), int(geographic), m, _p(loc), int(kernel),
                                int(adaptive), float(bw), int(nthreads), _p(betas))
     return None if rc else betas
+
+
+def indep_vars_criterion(X, y, vars, has_intercept=True):
+    """indepVarsSelectionCriterion: AICc of the unit-weight regression on the columns."""
+    Xc, yc = _colmajor(X), _f64(y)
+    n, k = Xc.shape
+    v = np.ascontiguousarray(vars, dtype=np.int32)
+    return lib().orc_indep_vars_criterion(n, k, _p(Xc), _p(yc), int(has_intercept), _ip_arr(v), len(v))
+
+
+def indep_vars_select(X, y, threshold=3.0, has_intercept=True):
+    """VariableForwardSelector: (selected columns, [(columns, criterion), ...])."""
+    Xc, yc = _colmajor(X), _f64(y)
+    n, k = Xc.shape
+    sel = np.zeros(k, dtype=np.int32)
+    nsel = np.zeros(1, dtype=np.int32)
+    cap = k * (k + 1) // 2
+    mv = np.zeros(max(cap, 1) * k, dtype=np.int32)
+    mn = np.zeros(max(cap, 1), dtype=np.int32)
+    mc = np.zeros(max(cap, 1))
+    nm = np.zeros(1, dtype=np.int32)
+    lib().orc_indep_vars_select(n, k, _p(Xc), _p(yc), int(has_intercept), float(threshold), _ip_arr(sel),
+                                _ip_arr(nsel), _ip_arr(mv), _ip_arr(mn), _p(mc), _ip_arr(nm))
+    models = [(list(mv[q * k:q * k + mn[q]]), float(mc[q])) for q in range(int(nm[0]))]
+    return list(sel[:int(nsel[0])]), models
 
 
 def gwr_run(coords, X, y, kernel, adaptive, bw, select_bw=False, crit=CRIT_AIC, lower=None, upper=None,

@@ -1,5 +1,5 @@
 // Known-answer tests of the C++ host interface on the device, restating the
-// reference's Catch2 sections for londonhp100 (test/testGWRBasic.cpp:40-137,
+// reference's Catch2 sections for londonhp100 (test/testGWRBasic.cpp:40-240,
 // test/testGWRMultiscale.cpp:135-173) with ParallelType::HIP.
 // Usage: test_facade <dir with londonhp100coords.csv and londonhp100data.csv>
 #include <cmath>
@@ -141,6 +141,43 @@ int main(int argc, char** argv) {
         std::size_t bw = (std::size_t)algorithm.spatialWeight().weight<BandwidthWeight>()->bandwidth();
         REQUIRE(bw == 67);
         REQUIRE(!algorithm.bandwidthSelectionCriterionList().empty());
+    }
+    for (int optimize_bw = 0; optimize_bw < 2; ++optimize_bw) {
+        // adaptive bandwidth | {no, CV} bandwidth optimization | AIC variable optimization
+        CRSDistance distance(false);
+        BandwidthWeight bandwidth(36, true, BandwidthWeight::Gaussian);
+        SpatialWeight spatial(&bandwidth, &distance);
+        GWRBasic algorithm;
+        algorithm.setCoords(coords);
+        algorithm.setDependentVariable(y);
+        algorithm.setIndependentVariables(x);
+        algorithm.setSpatialWeight(spatial);
+        if (optimize_bw) {
+            algorithm.setIsAutoselectBandwidth(true);
+            algorithm.setBandwidthSelectionCriterion(GWRBasic::BandwidthSelectionCriterionType::CV);
+        }
+        algorithm.setIsAutoselectIndepVars(true);
+        algorithm.setIndepVarSelectionThreshold(3.0);
+        algorithm.setParallelType(ParallelType::HIP);
+        algorithm.fit();
+        const VariablesCriterionList& c = algorithm.indepVarsSelectionCriterionList();
+        const std::vector<std::vector<std::size_t>> vars = {{2}, {3}, {1}, {1, 3}, {1, 2}, {1, 2, 3}};
+        const double values[] = {2551.61359020599, 2551.30032201349, 2468.93236280013,
+                                 2452.86447942033, 2450.59642666509, 2452.80388934625};
+        REQUIRE(c.size() == 6);
+        for (std::size_t q = 0; q < c.size() && q < 6; ++q) {
+            REQUIRE(c[q].first == vars[q]);
+            REQUIRE_NEAR(c[q].second, values[q], 1e-8);
+        }
+        REQUIRE(algorithm.selectedVariables() == std::vector<std::size_t>({1, 3}));
+        if (optimize_bw) {
+            REQUIRE(algorithm.spatialWeight().weight<BandwidthWeight>()->bandwidth() == 31.0);
+            RegressionDiagnostic d = algorithm.diagnostic();
+            REQUIRE_NEAR(d.AIC, 2435.8161441795, 1e-8);
+            REQUIRE_NEAR(d.AICc, 2445.49629974057, 1e-8);
+            REQUIRE_NEAR(d.RSquare, 0.706143867720706, 1e-8);
+            REQUIRE_NEAR(d.RSquareAdjust, 0.678982114793865, 1e-8);
+        }
     }
     {   // cancellation: Status::Terminated and zero betas
         CRSDistance distance(false);

@@ -9,6 +9,7 @@
 - A brute-force dense-matrix GWR (explicit W_i, C_i, S) on tiny n.
 """
 import math
+import sys
 
 import numpy as np
 import pytest
@@ -324,3 +325,34 @@ def test_predict_not_spd(oracle):
     loc = np.array([[5.0, 5.0], [500.0, 500.0]])
     p = oracle.predict(coords, X, y, loc, oracle.BISQUARE, False, 20.0)
     assert p is None
+
+
+# --------------------------------------------------------------- variable selection (testGWRBasic.cpp:139-240)
+
+VARSEL_KAT = [([2], 2551.61359020599), ([3], 2551.30032201349), ([1], 2468.93236280013),
+              ([1, 3], 2452.86447942033), ([1, 2], 2450.59642666509), ([1, 2, 3], 2452.80388934625)]
+
+
+def test_kat_indep_vars_select(oracle, londonhp100):
+    sel, models = oracle.indep_vars_select(londonhp100["X"], londonhp100["y"], threshold=3.0)
+    assert sel == [1, 3]
+    assert len(models) == len(VARSEL_KAT)
+    for (v, c), (kv, kc) in zip(models, VARSEL_KAT):
+        assert v == kv
+        assert abs(c - kc) <= 1e-8
+
+
+def test_indep_vars_criterion_is_ols_aicc(oracle, londonhp100):
+    # unit weights: the criterion is the AICc of the global OLS with tr(S) = p
+    X, y = londonhp100["X"], londonhp100["y"]
+    n = len(y)
+    for cols in ([0, 1], [0, 2, 3], [0, 1, 2, 3]):
+        Xs = X[:, cols]
+        beta, *_ = np.linalg.lstsq(Xs, y, rcond=None)
+        rss = float(((y - Xs @ beta) ** 2).sum())
+        p = len(cols)
+        aicc = n * math.log(rss / n) + n * math.log(2 * math.pi) + n * (n + p) / (n - 2 - p)
+        assert oracle.indep_vars_criterion(X, y, cols[1:]) == pytest.approx(aicc, rel=1e-12)
+    # a zero column makes the design singular: DBL_MAX
+    Xd = np.column_stack([X, np.zeros(n)])
+    assert oracle.indep_vars_criterion(Xd, y, [1, 4]) == sys.float_info.max
