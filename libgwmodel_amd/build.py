@@ -21,6 +21,9 @@ FACADE_LIB = os.path.join(OUT_DIR, "libgwmodelpp_hip.so")
 FACADE_SRC = os.path.join(HERE, "cpp", "gwr_facade.cpp")
 FACADE_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
 FACADE_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_facade")
+TASK_SRC = os.path.join(HERE, "cpp", "gwr_gpu_task.cpp")
+TASK_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_gpu_task.cpp")
+TASK_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_gpu_task")
 
 SOURCES = ["gwm_contract.hip", "gwm_contract_mixed.hip", "gwm_select.hip", "gwm_solve.hip", "gwm_varsel.hip",
            "gwm_api.cpp", "gwm_batch.cpp"]
@@ -73,16 +76,19 @@ def _run(cmd):
 
 
 def build_facade():
-    """The C++ host interface (include/gwmodelpp/gwmodel.h) is host-only C++
-    over the C ABI: g++, linked against libgwmodel_hip.so, plus its KAT
-    program tests/cpp/test_facade."""
+    """The C++ host interface (include/gwmodelpp/gwmodel.h) and the
+    IGWRBasicGpuTask entry (include/gwmodelpp/IGWRBasicGpuTask.h) are host-only
+    C++ over the C ABI: g++, linked against libgwmodel_hip.so, plus their KAT
+    programs tests/cpp/test_facade and tests/cpp/test_gpu_task."""
     inc = "-I" + os.path.join(ROOT, "include")
     tmp = FACADE_LIB + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", inc, FACADE_SRC, "-o", tmp, "-L" + OUT_DIR,
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", inc, FACADE_SRC, TASK_SRC, "-o", tmp, "-L" + OUT_DIR,
           "-lgwmodel_hip", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, FACADE_LIB)
     os.makedirs(os.path.dirname(FACADE_TEST_BIN), exist_ok=True)
     _run(["g++", "-O2", "-std=c++17", inc, FACADE_TEST_SRC, "-o", FACADE_TEST_BIN, "-L" + OUT_DIR,
+          "-lgwmodelpp_hip", "-lgwmodel_hip", "-Wl,-rpath,$ORIGIN/../../../libgwmodel_amd/lib"])
+    _run(["g++", "-O2", "-std=c++17", inc, TASK_TEST_SRC, "-o", TASK_TEST_BIN, "-L" + OUT_DIR,
           "-lgwmodelpp_hip", "-lgwmodel_hip", "-Wl,-rpath,$ORIGIN/../../../libgwmodel_amd/lib"])
 
 

@@ -25,3 +25,18 @@ def test_create_fails_cleanly_without_device():
     h = ctypes.c_void_p()
     rc = capi.load().gwm_hip_create(0, ctypes.byref(h))
     assert rc == capi.GWM_ERR_NO_DEVICE
+
+
+def test_gpu_task_library_exports_the_factories():
+    """include/gwmodelpp/IGWRBasicGpuTask.h: the three extern "C" names of the
+    reference's gwmodelcuda entry (IGWRBasicGpuTask.h:63-65)."""
+    import os
+    import re
+    capi.load()   # the host library depends on the C-ABI library
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib = ctypes.CDLL(os.path.join(root, "libgwmodel_amd", "lib", "libgwmodelpp_hip.so"))
+    text = open(os.path.join(root, "include", "gwmodelpp", "IGWRBasicGpuTask.h")).read()
+    names = sorted(set(re.findall(r'extern "C" GWMODELCUDA_API [^(]*?\b(\w+)\s*\(', text)))
+    assert names == ["GWRBasicGpuTaskFit_Create", "GWRBasicGpuTaskPredict_Create", "GWRBasicGpuTask_Del"]
+    for name in names:
+        assert hasattr(lib, name), name

@@ -1,7 +1,8 @@
-"""The C++ host interface (include/gwmodelpp/gwmodel.h) on the device: runs
-tests/cpp/test_facade, which restates the reference's Catch2 KAT sections
-(test/testGWRBasic.cpp:40-137, test/testGWRMultiscale.cpp:135-173) with
-ParallelType::HIP."""
+"""The C++ host interfaces on the device: tests/cpp/test_facade restates the
+reference's Catch2 KAT sections (test/testGWRBasic.cpp:40-240,
+test/testGWRMultiscale.cpp:135-173) on include/gwmodelpp/gwmodel.h with
+ParallelType::HIP; tests/cpp/test_gpu_task drives the IGWRBasicGpuTask entry
+(include/gwmodelpp/IGWRBasicGpuTask.h) through its factory functions."""
 import os
 import subprocess
 
@@ -9,6 +10,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_facade")
+TASK_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_gpu_task")
 
 pytestmark = pytest.mark.gpu
 
@@ -19,3 +21,12 @@ def test_cpp_facade_kats():
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     assert "FACADE TESTS PASSED" in r.stdout
+
+
+def test_cpp_gpu_task_kats():
+    assert os.path.exists(TASK_BIN), "build the library first (__graft_entry__.build())"
+    r = subprocess.run([TASK_BIN, os.path.join(ROOT, "tests", "golden")], capture_output=True, text=True,
+                       timeout=150)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "GPU TASK TESTS PASSED" in r.stdout
