@@ -116,6 +116,16 @@ int gwm_hip_indep_vars_criterion(gwm_hip_ctx* ctx, int has_intercept, const int*
 int gwm_hip_indep_vars_select(gwm_hip_ctx* ctx, int has_intercept, double threshold, int* selected, int* nselected,
                               int* model_vars, int* model_nvars, double* model_crit, int cap_models, int* nmodels);
 
+/* GWRMultiscale::setHasHatMatrix (GWRMultiscale.h:398) for gwm_hip_mgwr_run: keep
+ * S0 and the per-variable operators (GWRMultiscale.cpp:163-168, 215-225,
+ * 296-301) on the device, (k + 3) n^2 doubles (GWM_ERR_OOM when the device
+ * lacks the memory).  The fitAll sweep then runs in FP64 whatever the
+ * precision.  Off by default. */
+int gwm_hip_mgwr_set_hat_matrix(gwm_hip_ctx* ctx, int enabled);
+/* {tr S0, tr S0^T S0} of the last gwm_hip_mgwr_run (GWRMultiscale.cpp:176-179;
+ * zeros without the hat matrix). */
+int gwm_hip_mgwr_last_shat(gwm_hip_ctx* ctx, double* shat);
+
 /* Golden-section bandwidth selection on device-resident data: replaces
  * BandwidthSelector::optimize (src/gwmodelpp/BandwidthSelector.cpp:6-81)
  * driving getCriterion.  Bounds follow GWRBasic::fit (GWRBasic.cpp:76-79):
@@ -150,11 +160,13 @@ int gwm_hip_var_fit(gwm_hip_ctx* ctx, const double* x, const double* y, int kern
                     int precision, double* beta);
 
 /* GWRMultiscale::fit() end to end (src/gwmodelpp/GWRMultiscale.cpp:68-197, backfitting
- * :208-317) without the hat matrix (setHasHatMatrix(false)).  Per-variable arrays
- * have k entries; init_type 0 Null / 1 Initial / 2 Specified; approach uses the
- * GWM_CRIT_* ids; crit_type 0 CVR / 1 dCVR.  Outputs: betas n x k, bws_out k,
- * diag[7] (AIC and the hat-based terms follow the reference with tr S = 0),
- * rss_out, iterations_out. */
+ * :208-317), without the hat matrix (setHasHatMatrix(false)) unless
+ * gwm_hip_mgwr_set_hat_matrix enabled it.  Per-variable arrays have k entries;
+ * init_type 0 Null / 1 Initial / 2 Specified; approach uses the GWM_CRIT_* ids;
+ * crit_type 0 CVR / 1 dCVR.  Outputs: betas n x k, bws_out k, diag[7]
+ * (CalcDiagnostic, :41-66: AIC is left 0 as the reference leaves it unset; the
+ * hat-based terms use tr S = tr S^T S = 0 without the hat matrix), rss_out,
+ * iterations_out. */
 int gwm_hip_mgwr_run(gwm_hip_ctx* ctx, const int* kernels, const int* adaptive, const double* bws,
                      const int* init_type, const int* approach, const int* centered, const double* bw_threshold,
                      int has_intercept, int crit_type, double crit_threshold, int max_iter, int retry_times,

@@ -42,6 +42,8 @@ SIGNATURES = {
     "gwm_hip_criterion": (_i, [_vp, _i, _i, _d, _i, _i, _dp]),
     "gwm_hip_fit": (_i, [_vp, _i, _i, _d, _i, _dp, _dp, _dp, _dp, _dp]),
     "gwm_hip_predict": (_i, [_vp, _i, _dp, _i, _i, _d, _i, _dp]),
+    "gwm_hip_mgwr_set_hat_matrix": (_i, [_vp, _i]),
+    "gwm_hip_mgwr_last_shat": (_i, [_vp, _dp]),
     "gwm_hip_indep_vars_criterion": (_i, [_vp, _i, _ip, _i, _dp]),
     "gwm_hip_indep_vars_select": (_i, [_vp, _i, _d, _ip, _ip, _ip, _ip, _dp, _i, _ip]),
     "gwm_hip_select_bandwidth": (_i, [_vp, _i, _i, _i, _i, _i, _d, _i, _d, _dp, _dp, _dp, _i, _ip]),
@@ -185,6 +187,15 @@ class Context:
         self._check(self._lib.gwm_hip_fit(self.h, int(kernel), int(adaptive), float(bw), int(precision), _p(betas),
                                           _p(se), _p(shat), _p(q), _p(S)))
         return dict(betas=betas, betasSE_raw=se, shat=shat, qDiag=q, S=S)
+
+    def mgwr_set_hat_matrix(self, on=True):
+        """Keep the multiscale hat matrix (tr S0, tr S0^T S0 in the diagnostics)."""
+        self._check(self._lib.gwm_hip_mgwr_set_hat_matrix(self.h, int(on)))
+
+    def mgwr_last_shat(self):
+        out = np.zeros(2)
+        self._check(self._lib.gwm_hip_mgwr_last_shat(self.h, _p(out)))
+        return out
 
     def predict(self, locations, kernel, adaptive, bw, precision=FP64):
         """Betas (m x k) at the rows of `locations` (m x 2)."""

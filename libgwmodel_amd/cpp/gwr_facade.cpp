@@ -171,8 +171,6 @@ Mat GWRMultiscale::fit() {
     if (mSpatialWeights.size() != k || mBandwidthInitilize.size() != k || mBandwidthSelectionApproach.size() != k ||
         mPreditorCentered.size() != k || mBandwidthSelectThreshold.size() != k)
         throw std::invalid_argument("per-variable settings must have one entry per independent variable");
-    if (mHasHatMatrix)
-        throw std::logic_error("the HIP back-end runs GWRMultiscale without the hat matrix: call setHasHatMatrix(false)");
     const int dist = distance_id(mSpatialWeights[0]);
     std::vector<int> kernels(k), adaptive(k), init(k), approach(k), centered(k);
     std::vector<double> bws(k);
@@ -190,6 +188,7 @@ Mat GWRMultiscale::fit() {
     gwm_hip_set_stop_callback(ctx.get(), &stop_adapter, mTelegram);
     int rc = gwm_hip_set_data(ctx.get(), (int)n, (int)k, mCoords.memptr(), mX.memptr(), mY.memptr(), dist);
     if (rc) raise(ctx.get(), rc);
+    gwm_hip_mgwr_set_hat_matrix(ctx.get(), mHasHatMatrix ? 1 : 0);
     mBetas = Mat(n, k);
     std::vector<double> bws_out(k);
     double diag[7] = {0}, rss = 0.0;

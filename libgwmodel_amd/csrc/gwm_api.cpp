@@ -6,6 +6,7 @@
 // memory is O(n * k^2) per problem.  All HIP failures are reported as status
 // codes; the device is never reset.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cfloat>
@@ -778,7 +779,8 @@ int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int
 
 // Full-design fit returning betas only (fitAll without the hat matrix,
 // GWRMultiscale.cpp:394-417).
-int fit_betas_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int precision, double* betas) {
+int fit_betas_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int precision, double* betas,
+                   double* ainv = nullptr) {
     int rc;
     const double* bw_pt;
     if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
@@ -786,6 +788,7 @@ int fit_betas_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int prec
     int PT, nseg;
     if ((rc = sweep(c, kModePlain, c->nta, 0, design_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
     SolveArgs s = solve_args(c, PT, nseg, 0, bw_pt, bw, kernel);
+    s.ainv = ainv;
     GWM_HIP_CHECK(c, launch_solve(c->k, kSolveBeta, s, c->stream));
     if (precision == GWM_MIXED) {
         int count = 0;
@@ -923,6 +926,82 @@ int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adapt
     return GWM_OK;
 }
 
+// Hat matrix of a multiscale run (GWRMultiscale.cpp:163-168, 176-179,
+// 215-225, 296-301), resident on the device: R (k slices) and S0 from fitAll,
+// then one update per backfitting step.  Memory: (k + 3) n^2 doubles.
+struct MgwrHat {
+    DevBuf mats, ainv, scratch;
+    double *R = nullptr, *S0 = nullptr, *M = nullptr, *S = nullptr, *dinv = nullptr, *part = nullptr;
+    size_t nn = 0;
+};
+
+int hat_alloc(gwm_hip_ctx* c, MgwrHat& h) {
+    const size_t n = c->n, k = c->k;
+    h.nn = n * n;
+    const size_t bytes = (k + 3) * h.nn * sizeof(double);
+    size_t free_b = 0, total_b = 0;
+    GWM_HIP_CHECK(c, hipMemGetInfo(&free_b, &total_b));
+    if (bytes > free_b / 10 * 9) {
+        c->err = "the multiscale hat matrix needs (k + 3) n^2 doubles (" + std::to_string(bytes >> 20) +
+                 " MiB), more than the device has free; run without it";
+        return GWM_ERR_OOM;
+    }
+    GWM_HIP_CHECK(c, hipMalloc(&h.mats.p, bytes));
+    GWM_HIP_CHECK(c, hipMalloc(&h.ainv.p, n * k * k * sizeof(double)));
+    GWM_HIP_CHECK(c, hipMalloc(&h.scratch.p, (n + 2048 + 2) * sizeof(double)));
+    h.R = static_cast<double*>(h.mats.p);
+    h.S0 = h.R + k * h.nn;
+    h.M = h.S0 + h.nn;
+    h.S = h.M + h.nn;
+    h.dinv = static_cast<double*>(h.scratch.p);
+    h.part = h.dinv + n;
+    if (!c->blas) {
+        rocblas_handle b = nullptr;
+        if (rocblas_create_handle(&b) != rocblas_status_success) {
+            c->err = "rocblas_create_handle failed";
+            return GWM_ERR_HIP;
+        }
+        c->blas = b;
+    }
+    if (rocblas_set_stream(static_cast<rocblas_handle>(c->blas), c->stream) != rocblas_status_success) {
+        c->err = "rocblas_set_stream failed";
+        return GWM_ERR_HIP;
+    }
+    return GWM_OK;
+}
+
+// Backfitting step of variable i at bandwidth bw: R_i <- S_i (R_i - S0) + S_i,
+// S0 <- R_i(new) - (R_i(old) - S0).
+int hat_step(gwm_hip_ctx* c, MgwrHat& h, int i, int kernel, int adaptive, double bw) {
+    const int n = c->n;
+    const double* bw_pt;
+    int rc;
+    if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
+    double* Ri = h.R + (size_t)i * h.nn;
+    GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
+    timer_begin(c);
+    GWM_HIP_CHECK(c, launch_var_smatrix(c->dist, c->points(), c->X + (size_t)i * n, bw_pt, bw, kernel, h.dinv, c->fail,
+                                        h.S, c->stream));
+    GWM_HIP_CHECK(c, launch_sub(Ri, h.S0, h.M, h.nn, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(Ri, h.S, h.nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    const double one = 1.0;
+    if (rocblas_dgemm(static_cast<rocblas_handle>(c->blas), rocblas_operation_none, rocblas_operation_none, n, n, n,
+                      &one, h.S, n, h.M, n, &one, Ri, n) != rocblas_status_success) {
+        c->err = "rocblas_dgemm failed";
+        return GWM_ERR_HIP;
+    }
+    GWM_HIP_CHECK(c, launch_sub(Ri, h.M, h.S0, h.nn, c->stream));
+    if ((rc = timer_end(c, "mgwr_hat"))) return rc;
+    int fail = 0;
+    GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (fail) {
+        c->err = "inv_sympd(): matrix is singular or not positive definite";
+        return GWM_ERR_NOT_SPD;
+    }
+    return GWM_OK;
+}
+
 // Criteria of candidate column sets (GWRBasic::indepVarsSelectionCriterion,
 // GWRBasic.cpp:309-346): columns {0 if has_intercept} + vars; RSS and tr(S)
 // on the device from the Gram matrix G of [X y] (launch_gram), AICc here;
@@ -1018,6 +1097,7 @@ void gwm_hip_destroy(gwm_hip_ctx* c) {
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->pairs_dev) hipFree(c->pairs_dev);
+    if (c->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(c->blas));
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1353,7 +1433,21 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
     if (rc) return fail_out(rc);
     // backfitting (GWRMultiscale.cpp:208-317)
     std::vector<double> betas((size_t)n * k), resid(n), yi(n), colv(n);
-    if ((rc = fit_betas_impl(c, kernels[0], adaptive[0], initBw, precision, betas.data()))) return fail_out(rc);
+    const bool hat = c->mgwr_hat;
+    MgwrHat H;
+    if (hat) {
+        // fitAll in FP64 with the per-point A^-1 (GWRMultiscale.cpp:362-395), then R and S0 (:215-225)
+        if ((rc = hat_alloc(c, H))) return fail_out(rc);
+        if ((rc = fit_betas_impl(c, kernels[0], adaptive[0], initBw, GWM_FP64, betas.data(),
+                                 static_cast<double*>(H.ainv.p))))
+            return fail_out(rc);
+        const double* bw_pt;
+        if ((rc = resolve_bandwidth(c, adaptive[0], initBw, &bw_pt))) return fail_out(rc);
+        GWM_HIP_CHECK(c, launch_hat_init(c->dist, c->points(), k, c->X, static_cast<double*>(H.ainv.p), bw_pt, initBw,
+                                         kernels[0], H.R, H.S0, c->stream));
+    } else if ((rc = fit_betas_impl(c, kernels[0], adaptive[0], initBw, precision, betas.data()))) {
+        return fail_out(rc);
+    }
     auto update_resid = [&]() {
         for (int j = 0; j < n; ++j) {
             double f = 0.0;
@@ -1394,6 +1488,7 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
                                colv.data())))
                 return fail_out(rc);
             std::copy(colv.begin(), colv.end(), betas.begin() + (size_t)i * n);
+            if (hat && (rc = hat_step(c, H, i, kernels[i], adaptive[i], bw[i]))) return fail_out(rc);
             update_resid();
         }
         RSS1 = 0.0;
@@ -1402,7 +1497,16 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         RSS0 = RSS1;
     }
     if (iterations_out) *iterations_out = iteration - 1;
-    // CalcDiagnostic (GWRMultiscale.cpp:41-66) with shat = {0, 0} (no hat matrix)
+    // CalcDiagnostic (GWRMultiscale.cpp:41-66) with shat = {tr S0, tr S0^T S0}
+    // ({0, 0} without the hat matrix, :176-179)
+    double shat[2] = {0.0, 0.0};
+    if (hat) {
+        GWM_HIP_CHECK(c, launch_trace(H.S0, n, H.part, H.part + 2048, c->stream));
+        GWM_HIP_CHECK(c, hipMemcpyAsync(shat, H.part + 2048, sizeof(shat), hipMemcpyDeviceToHost, c->stream));
+        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    }
+    c->mgwr_shat[0] = shat[0];
+    c->mgwr_shat[1] = shat[1];
     const double nd = n;
     double ym = 0.0;
     for (int j = 0; j < n; ++j) ym += y[j];
@@ -1410,11 +1514,12 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
     double TSS = 0.0;
     for (int j = 0; j < n; ++j) TSS += (y[j] - ym) * (y[j] - ym);
     double r2 = 1 - RSS0 / TSS;
-    double edf = nd;
+    const double trS = shat[0], trStS = shat[1];
+    double edf = nd - 2 * trS + trStS;
     diag[0] = RSS0;
     diag[1] = 0.0;
-    diag[2] = nd * std::log(RSS0 / nd) + nd * std::log(2 * kPi) + nd * (nd / (nd - 2));
-    diag[3] = 0.0;
+    diag[2] = nd * std::log(RSS0 / nd) + nd * std::log(2 * kPi) + nd * ((nd + trS) / (nd - 2 - trS));
+    diag[3] = 2 * trS - trStS;
     diag[4] = edf;
     diag[5] = r2;
     diag[6] = 1 - (1 - r2) * (nd - 1) / (edf - 1);
@@ -1516,6 +1621,19 @@ int gwm_hip_indep_vars_select(gwm_hip_ctx* c, int has_intercept, double threshol
             model_vars[(size_t)q * k + a] = a < model_nvars[q] ? models[q].first[a] : -1;
         model_crit[q] = models[q].second;
     }
+    return GWM_OK;
+}
+
+int gwm_hip_mgwr_set_hat_matrix(gwm_hip_ctx* c, int enabled) {
+    if (!c) return GWM_ERR_INVALID;
+    c->mgwr_hat = enabled != 0;
+    return GWM_OK;
+}
+
+int gwm_hip_mgwr_last_shat(gwm_hip_ctx* c, double* shat) {
+    if (!c || !shat) return GWM_ERR_INVALID;
+    shat[0] = c->mgwr_shat[0];
+    shat[1] = c->mgwr_shat[1];
     return GWM_OK;
 }
 

@@ -128,6 +128,7 @@ struct SolveArgs {
     // from partial-sum column t
     const int* focus_idx = nullptr;
     int nfocus = 0;
+    double* ainv = nullptr;  // per focus: A^-1 row-major k x k (kSolveBeta), or null
 };
 
 hipError_t launch_solve(int k, int mode, const SolveArgs& a, hipStream_t s);
@@ -187,6 +188,18 @@ hipError_t launch_smatrix(int dist, int k, const Points& P, const double* X, con
 hipError_t launch_gram(const double* X, const double* y, int n, int k, double* G, hipStream_t s);
 hipError_t launch_varsel(const double* X, const double* y, int n, int k, const double* G, const int* sets,
                          const int* nvars, int nsets, double* out, hipStream_t s);
+
+// Multiscale hat matrix (n x n column-major, context point order):
+// R (k slices) and S0 after fitAll from the per-point A^-1; the single-column
+// hat matrix S of x at a bandwidth (dinv: n scratch, fail set when a
+// denominator is not positive); out = a - b; out[0..1] = tr(S), sum S^2
+// (part: 2048 scratch doubles).
+hipError_t launch_hat_init(int dist, const Points& P, int k, const double* X, const double* ainv, const double* bw_pt,
+                           double bw, int kernel, double* R, double* S0, hipStream_t s);
+hipError_t launch_var_smatrix(int dist, const Points& P, const double* x, const double* bw_pt, double bw, int kernel,
+                              double* dinv, int* fail, double* S, hipStream_t s);
+hipError_t launch_sub(const double* a, const double* b, double* out, size_t count, hipStream_t s);
+hipError_t launch_trace(const double* S, int n, double* part, double* out, hipStream_t s);
 
 hipError_t launch_compact(const double* cond, int n, double limit, int* list, int* count, hipStream_t s);
 

@@ -87,6 +87,11 @@ struct gwm_hip_ctx {
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long* pairs_dev = nullptr;   // swept-pair counter (timing only)
+    // Multiscale runs: keep the hat matrix (gwm_hip_mgwr_set_hat_matrix); the
+    // rocBLAS handle of its products (created on first use); last {tr S, tr S^T S}
+    bool mgwr_hat = false;
+    void* blas = nullptr;
+    double mgwr_shat[2] = {0.0, 0.0};
     std::map<std::string, gwm::detail::KernelStat> stats;
     long long solves = 0;
 

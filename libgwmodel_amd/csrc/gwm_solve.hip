@@ -160,6 +160,17 @@ __device__ __forceinline__ void solve_one(const SolveArgs& a, int t) {
     if (MODE == kSolveBeta) {
 #pragma unroll
         for (int p = 0; p < K; ++p) a.betas[i + (size_t)p * n] = beta[p];
+        if (a.ainv) {
+            // columns of A^-1 (symmetric: stored row-major as they come)
+            double e[K], col[K];
+            for (int p = 0; p < K; ++p) {
+#pragma unroll
+                for (int q = 0; q < K; ++q) e[q] = q == p ? 1.0 : 0.0;
+                chol_solve<K>(L, e, col);
+#pragma unroll
+                for (int q = 0; q < K; ++q) a.ainv[(size_t)i * K * K + q * K + p] = col[q];
+            }
+        }
         return;
     }
 

@@ -1,6 +1,6 @@
 // Known-answer tests of the C++ host interface on the device, restating the
 // reference's Catch2 sections for londonhp100 (test/testGWRBasic.cpp:40-240,
-// test/testGWRMultiscale.cpp:135-173) with ParallelType::HIP.
+// test/testGWRMultiscale.cpp:45-88, 135-173) with ParallelType::HIP.
 // Usage: test_facade <dir with londonhp100coords.csv and londonhp100data.csv>
 #include <cmath>
 #include <cstdio>
@@ -225,6 +225,40 @@ int main(int argc, char** argv) {
         REQUIRE(sw[1].weight<BandwidthWeight>()->bandwidth() == 98);
         REQUIRE(sw[2].weight<BandwidthWeight>()->bandwidth() == 98);
         REQUIRE_NEAR(algorithm.diagnostic().RSquare, 0.757377391669, 1e-6);
+    }
+    {   // MGWR: optim bw cv | null init bw | fixed | bisquare | with hat matrix (testGWRMultiscale.cpp:45-88)
+        std::vector<SpatialWeight> spatials;
+        std::vector<bool> centered;
+        std::vector<GWRMultiscale::BandwidthInitilizeType> init;
+        std::vector<GWRMultiscale::BandwidthSelectionCriterionType> approach;
+        for (int i = 0; i < 3; ++i) {
+            CRSDistance distance;
+            BandwidthWeight bandwidth(0, false, BandwidthWeight::Bisquare);
+            spatials.push_back(SpatialWeight(&bandwidth, &distance));
+            centered.push_back(i != 0);
+            init.push_back(GWRMultiscale::BandwidthInitilizeType::Null);
+            approach.push_back(GWRMultiscale::BandwidthSelectionCriterionType::CV);
+        }
+        GWRMultiscale algorithm;
+        algorithm.setCoords(coords);
+        algorithm.setDependentVariable(y);
+        algorithm.setIndependentVariables(xm);
+        algorithm.setSpatialWeights(spatials);
+        algorithm.setHasHatMatrix(true);
+        algorithm.setPreditorCentered(centered);
+        algorithm.setBandwidthInitilize(init);
+        algorithm.setBandwidthSelectionApproach(approach);
+        algorithm.setBandwidthSelectThreshold(std::vector<double>(3, 1e-5));
+        algorithm.setParallelType(ParallelType::HIP);
+        algorithm.fit();
+        const std::vector<SpatialWeight>& sw = algorithm.spatialWeights();
+        REQUIRE_NEAR(sw[0].weight<BandwidthWeight>()->bandwidth(), 4623.78, 0.1);
+        REQUIRE_NEAR(sw[1].weight<BandwidthWeight>()->bandwidth(), 12665.70, 0.1);
+        REQUIRE_NEAR(sw[2].weight<BandwidthWeight>()->bandwidth(), 12665.70, 0.1);
+        RegressionDiagnostic d = algorithm.diagnostic();
+        REQUIRE_NEAR(d.AICc, 2437.09277417389, 1e-6);
+        REQUIRE_NEAR(d.RSquare, 0.744649364494, 1e-6);
+        REQUIRE_NEAR(d.RSquareAdjust, 0.712344894394, 1e-6);
     }
     std::printf(failures ? "FACADE TESTS FAILED: %d\n" : "FACADE TESTS PASSED\n", failures);
     return failures ? 1 : 0;

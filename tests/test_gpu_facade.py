@@ -1,6 +1,6 @@
 """The C++ host interfaces on the device: tests/cpp/test_facade restates the
 reference's Catch2 KAT sections (test/testGWRBasic.cpp:40-240,
-test/testGWRMultiscale.cpp:135-173) on include/gwmodelpp/gwmodel.h with
+test/testGWRMultiscale.cpp:45-88, 135-173) on include/gwmodelpp/gwmodel.h with
 ParallelType::HIP; tests/cpp/test_gpu_task drives the IGWRBasicGpuTask entry
 (include/gwmodelpp/IGWRBasicGpuTask.h) through its factory functions."""
 import os
