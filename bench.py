@@ -135,6 +135,72 @@ def run_mgwr(capi, synth, n):
     return out
 
 
+def run_mgwr_hat(capi, synth, n):
+    """Multiscale backfitting with the hat matrix (S0 and one n x n operator
+    per variable on the device, an FP64 GEMM per backfitting step)."""
+    d = synth.config3(n=n)
+    X = d["X"][:, :3]
+    k = X.shape[1]
+    out = {"n": n, "k": k}
+    with capi.Context(0) as ctx:
+        ctx.set_data(d["coords"], X, d["y"])
+        ctx.mgwr_set_hat_matrix(True)
+        log("mgwr with hat matrix n=%d k=%d" % (n, k))
+        ctx.reset_timers()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        r = ctx.mgwr_run([capi.BISQUARE] * k, [1] * k, [0.0] * k, [0] * k, [capi.CRIT_CV] * k,
+                         [0] + [1] * (k - 1), [1e-5] * k, crit_type=0, crit_threshold=1e-5, precision=capi.MIXED)
+        dt = time.perf_counter() - t0
+        ctx.set_timing(False)
+        steps, ms = ctx.kernel_time("mgwr_hat")
+        shat = ctx.mgwr_last_shat()
+    out["seconds"] = dt
+    out["iterations"] = r["iterations"]
+    out["hat_steps"] = steps
+    out["hat_ms_per_step"] = ms / max(steps, 1)
+    # one step: S_i (n^2 weights), M = R_i - S0, R_i = S_i M + S_i (2 n^3 flops), S0 = R_i - M
+    out["hat_step_tflops"] = 2.0 * float(n) ** 3 / (out["hat_ms_per_step"] * 1e-3) / 1e12 if steps else None
+    out["hat_step_frac_of_fp64_peak"] = out["hat_step_tflops"] * 1e12 / PEAK["fp64"] if steps else None
+    out["trS"], out["trStS"] = float(shat[0]), float(shat[1])
+    out["AICc"] = r["diagnostic"]["AICc"]
+    out["bandwidths"] = [float(b) for b in r["bandwidths"]]
+    return out
+
+
+def run_predict(ctx, capi, d, bw, m, seed=2024):
+    """Betas at m new locations of the config-2 box at the selected bandwidth."""
+    rng = np.random.default_rng(seed)
+    loc = np.column_stack([rng.uniform(-10, 30, m), rng.uniform(35, 60, m)])
+    out = {"m": m, "bandwidth_km": bw}
+    res = {}
+    for name, prec in (("mixed", capi.MIXED), ("fp64", capi.FP64)):
+        ctx.predict(loc[:1024], capi.GAUSSIAN, False, bw, precision=prec)   # warm
+        log("predict %s m=%d" % (name, m))
+        ctx.reset_timers()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        res[name] = ctx.predict(loc, capi.GAUSSIAN, False, bw, precision=prec)
+        dt = time.perf_counter() - t0
+        ctx.set_timing(False)
+        _, kms = ctx.kernel_time("contract_%s_predict" % name)
+        out[name + "_seconds"] = dt
+        out[name + "_solves_per_s"] = m / dt
+        out[name + "_sweep_ms"] = kms
+    out["mixed_over_fp64"] = out["mixed_solves_per_s"] / out["fp64_solves_per_s"]
+    out["beta_rel_err_vs_fp64"] = beta_rel_err(res["mixed"], res["fp64"])
+    return out
+
+
+def run_varsel(ctx):
+    """Forward selection over the config-2 covariates (unit-weight AICc per candidate set)."""
+    log("variable selection")
+    t0 = time.perf_counter()
+    sel, models = ctx.indep_vars_select(threshold=3.0)
+    dt = time.perf_counter() - t0
+    return {"models": len(models), "selected": [int(v) for v in sel], "seconds": dt}
+
+
 def serve_submit(capi, srv, r):
     n = r["X"].shape[0]
     if r["search"]:
@@ -202,6 +268,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=512, help="focus points of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mgwr-n", type=int, default=50_000, help="config-3 multiscale size (0: skip)")
+    ap.add_argument("--mgwr-hat-n", type=int, default=4000, help="multiscale size with the hat matrix (0: skip)")
+    ap.add_argument("--predict-m", type=int, default=-1, help="prediction locations (default n; 0: skip)")
     ap.add_argument("--batch-problems", type=int, default=512, help="config-4 batch size (0: skip)")
     ap.add_argument("--no-loop", action="store_true", help="skip the single-problem loop of the batch")
     ap.add_argument("--serve-requests", type=int, default=64, help="config-5 stream length (0: skip)")
@@ -237,6 +305,7 @@ def main():
 
     if args.skip_headline:
         extra = {"mgwr": run_mgwr(capi, synth, args.mgwr_n) if args.mgwr_n else None,
+                 "mgwr_hat": run_mgwr_hat(capi, synth, args.mgwr_hat_n) if args.mgwr_hat_n else None,
                  "batch": run_batch(capi, synth, args.batch_problems, not args.no_loop) if args.batch_problems else None,
                  "serve": run_serve(capi, synth, args.serve_requests, args.serve_rate, args.serve_check)
                  if args.serve_requests else None}
@@ -292,6 +361,9 @@ def main():
     l64, ms64 = ctx.kernel_time("contract_fp64_plain")
     swept64 = ctx.kernel_pairs("contract_fp64_plain")
     value_fp64 = solves_fp64 / t_fp64
+    m = n if args.predict_m < 0 else args.predict_m
+    predict = run_predict(ctx, capi, d, mixed["bandwidth"], m) if m else None
+    varsel = run_varsel(ctx)
     ctx.close()
 
     P = k * (k + 1) // 2 + k
@@ -316,6 +388,7 @@ def main():
                          "great-circle Gaussian, OpenMP (%.1f s)" % (args.cpu_sample, n, dt)}
 
     mgwr = run_mgwr(capi, synth, args.mgwr_n) if args.mgwr_n else None
+    mgwr_hat = run_mgwr_hat(capi, synth, args.mgwr_hat_n) if args.mgwr_hat_n else None
     batch = run_batch(capi, synth, args.batch_problems, not args.no_loop) if args.batch_problems else None
     serve = run_serve(capi, synth, args.serve_requests, args.serve_rate, args.serve_check) \
         if args.serve_requests else None
@@ -364,6 +437,9 @@ def main():
         "serve": serve,
         "batch": batch,
         "mgwr": mgwr,
+        "mgwr_hat": mgwr_hat,
+        "predict": predict,
+        "varsel": varsel,
     }
     print(json.dumps(line), flush=True)
 
