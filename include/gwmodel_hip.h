@@ -186,7 +186,7 @@ int gwm_hip_set_certify_tolerance(gwm_hip_ctx* ctx, double rel_tol);
  * >= 1, infinite when not SPD), and recomputes in FP64 -- the FP64 path's own
  * kernel and data segments, hence its bits -- every focus point whose
  * indicator exceeds the limit: criterion_limit in criterion evaluations
- * (default 100), fit_limit in fits (default 2).  gwm_hip_last_condition
+ * (default 10), fit_limit in fits (default 2).  gwm_hip_last_condition
  * returns the indicator maximum of the last mixed evaluation and the number
  * of focus points it recomputed.  No counterpart in the reference. */
 int gwm_hip_set_refine_condition(gwm_hip_ctx* ctx, double criterion_limit, double fit_limit);
@@ -201,6 +201,13 @@ int gwm_hip_set_refine_condition(gwm_hip_ctx* ctx, double criterion_limit, doubl
  * on or off.  No counterpart in the reference, which sums over all n. */
 int gwm_hip_set_pruning(gwm_hip_ctx* ctx, int enabled);
 int gwm_hip_last_condition(gwm_hip_ctx* ctx, double* cond, long long* refined);
+/* Absolute error bound that the certified golden section assumes for the
+ * last mixed-precision criterion value of this context (0 after an FP64
+ * evaluation): the largest of tol |f|, the residual rounding term
+ * (20 * 2^-22 * kappa / sqrt(n), times f for CV and n for AICc, kappa the
+ * indicator maximum capped at the recomputation limit) and, for AICc,
+ * tol * S * tr(S) with S = n (2n - 2) / (n - 2 - tr(S))^2. */
+int gwm_hip_last_error_bound(gwm_hip_ctx* ctx, double* bound);
 
 /* Timing of the device kernels with HIP events on the context stream
  * (enabled by gwm_hip_set_timing).  name: "contract_<fp64|mixed>_<cv|plain|

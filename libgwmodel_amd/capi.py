@@ -43,6 +43,7 @@ SIGNATURES = {
     "gwm_hip_fit": (_i, [_vp, _i, _i, _d, _i, _dp, _dp, _dp, _dp, _dp]),
     "gwm_hip_predict": (_i, [_vp, _i, _dp, _i, _i, _d, _i, _dp]),
     "gwm_hip_mgwr_set_hat_matrix": (_i, [_vp, _i]),
+    "gwm_hip_last_error_bound": (_i, [_vp, _dp]),
     "gwm_hip_mgwr_last_shat": (_i, [_vp, _dp]),
     "gwm_hip_indep_vars_criterion": (_i, [_vp, _i, _ip, _i, _dp]),
     "gwm_hip_indep_vars_select": (_i, [_vp, _i, _d, _ip, _ip, _ip, _ip, _dp, _i, _ip]),
@@ -187,6 +188,12 @@ class Context:
         self._check(self._lib.gwm_hip_fit(self.h, int(kernel), int(adaptive), float(bw), int(precision), _p(betas),
                                           _p(se), _p(shat), _p(q), _p(S)))
         return dict(betas=betas, betasSE_raw=se, shat=shat, qDiag=q, S=S)
+
+    def last_error_bound(self):
+        """Error bound the certified search assumes for the last mixed criterion value."""
+        out = np.zeros(1)
+        self._check(self._lib.gwm_hip_last_error_bound(self.h, _p(out)))
+        return float(out[0])
 
     def mgwr_set_hat_matrix(self, on=True):
         """Keep the multiscale hat matrix (tr S0, tr S0^T S0 in the diagnostics)."""

@@ -476,6 +476,40 @@ double aicc(double rss, double n, double trS) {
     return n * std::log(rss / n) + n * std::log(2 * kPi) + n * ((n + trS) / (n - 2 - trS));
 }
 
+// Absolute error bound of a mixed-precision criterion value, for the
+// certified golden section: the largest of three terms, each covering one
+// error source.  Their constants come from the errors measured against FP64
+// at 12,062 criterion evaluations (every golden-section candidate of CV and
+// AICc searches over 320 problems of the uneven batch workload, the n = 2000
+// workload with three kernels, and the n = 100k great-circle AICc workload
+// from 56 to 2200 km): with them no evaluation exceeded 0.69 of its bound.
+//  - tol_rel |f| for AICc, 3 tol_rel |f| for CV (whose relative errors run
+//    about three times larger at the same bandwidth);
+//  - the rounding of the fitted values, random over the focus points: an
+//    error of yhat_i relative to itself moves res_i^2 by 2 res_i yhat_i times
+//    it, so RSS (or CV) moves by 2 e sqrt(q) with q = sum_i (res_i yhat_i)^2
+//    and e = kRoundC 2^-22 kappa, kappa = min(indicator maximum, FP64
+//    recomputation limit) >= 1; AICc = n ln(RSS/n) + ... turns it into
+//    n 2 e sqrt(q) / RSS;
+//  - AICc only: the trace carries the systematic relative error of the mixed
+//    weights (it does not cancel as in beta), and AICc amplifies an error of
+//    trS by S = n (2n - 2) / (n - 2 - trS)^2: tol_rel S trS.  With the previous
+//    term it bounds the error where AICc crosses zero and |f| says nothing
+//    of it.
+constexpr double kRoundC = 4.0;
+constexpr double kCvTol = 3.0;
+double crit_error_bound(int crit, double value, double rss, double trS, double q, int n, double cond_max, double limit,
+                        double tol_rel) {
+    const double nd = n;
+    const double kappa = std::max(1.0, std::min(cond_max, limit));
+    const double dr = 2.0 * kRoundC * std::ldexp(1.0, -22) * kappa * std::sqrt(std::max(q, 0.0));
+    if (crit == GWM_CRIT_CV) return std::max(kCvTol * tol_rel * std::fabs(value), dr);
+    const double b = tol_rel * std::fabs(value);
+    const double gap = nd - 2 - trS;
+    const double S = gap > 0 ? nd * (2 * nd - 2) / (gap * gap) : INFINITY;
+    return std::max(b, std::max(rss > 0 ? nd * dr / rss : INFINITY, tol_rel * S * std::fabs(trS)));
+}
+
 }  // namespace detail
 }  // namespace gwm
 
@@ -503,8 +537,10 @@ int check_kernel(gwm_hip_ctx* c, int kernel, int precision) {
 }
 
 // Criterion of the full design: CV (GWRBasic.cpp:440-485) or AICc (:487-535).
-int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit, int precision, double* value) {
+int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit, int precision, double* value,
+                   double* bound = nullptr) {
     *value = DBL_MAX;
+    if (bound) *bound = 0.0;
     int rc = check_kernel(c, kernel, precision);
     if (rc) return rc;
     if (stopped(c)) return GWM_ERR_CANCELLED;
@@ -523,8 +559,10 @@ int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit
         int count = 0;
         if ((rc = refine_fp64(c, mode, smode, 0, 0, bw_pt, bw, kernel, c->refine_crit, &count))) return rc;
     }
-    GWM_HIP_CHECK(c, launch_reduce(c->res2, crit == GWM_CRIT_CV ? nullptr : c->sii, nullptr, nullptr, c->n, c->red,
-                                   c->stream));
+    // sums, and (mixed) the indicator maximum over the final per-point values
+    const bool mixed = precision == GWM_MIXED;
+    GWM_HIP_CHECK(c, launch_reduce(c->res2, crit == GWM_CRIT_CV ? nullptr : c->sii, mixed ? c->ssi : nullptr,
+                                   mixed ? c->cond : nullptr, c->n, c->red, c->stream));
     double red[4];
     int fail = 0;
     GWM_HIP_CHECK(c, hipMemcpyAsync(red, c->red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
@@ -535,6 +573,9 @@ int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit
     double v = crit == GWM_CRIT_CV ? red[0] : aicc(red[0], (double)c->n, red[1]);
     if (!std::isfinite(v)) return GWM_ERR_NOT_SPD;
     *value = v;
+    c->last_bound =
+        mixed ? crit_error_bound(crit, v, red[0], red[1], red[2], c->n, red[3], c->refine_crit, c->certify_rel) : 0.0;
+    if (bound) *bound = c->last_bound;
     return GWM_OK;
 }
 
@@ -546,7 +587,8 @@ int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit
 // on criterion values -- the branch f1 < f2, the stopping test |f2 - f1| > eps
 // and the final choice -- is certified: when a value's mixed-precision error
 // bound (tol_rel * |f|) could flip the decision, both points are re-evaluated
-// in FP64 and the decision is taken on the FP64 values.  The iterates are
+// in FP64 and the decision is taken on the FP64 values.  The bound of a mixed
+// value is the evaluation's own (crit_error_bound), at least tol_rel |f|.  The iterates are
 // therefore those of the FP64 search.  Once a decision has needed the FP64
 // values, the search has entered the region where the criterion varies less
 // than the mixed-precision error, and later candidates are evaluated in FP64
@@ -566,18 +608,20 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
     double x1 = adaptBw ? std::floor(xL + d) : (xL + d);
     double x2 = adaptBw ? std::round(xU - d) : (xU - d);
     double f1 = DBL_MAX, f2 = DBL_MAX;
+    double b1 = 0.0, b2 = 0.0;   // error bounds of mixed values (crit_error_bound)
     bool e1 = !certify, e2 = !certify;
     bool sticky = false;   // FP64 from here on
     // evaluate; a failed (DBL_MAX) mixed value is confirmed in FP64
-    auto eval = [&](double x, bool& exact, double* fx) -> int {
-        int rc = f(x, exact, fx);
+    auto eval = [&](double x, bool& exact, double* fx, double* fb) -> int {
+        *fb = 0.0;
+        int rc = f(x, exact, fx, fb);
         if (rc == GWM_ERR_NOT_SPD) {
             *fx = DBL_MAX;
             rc = GWM_OK;
         }
         if (rc == GWM_OK && !exact && *fx == DBL_MAX) {
             exact = true;
-            rc = f(x, true, fx);
+            rc = f(x, true, fx, fb);
             if (rc == GWM_ERR_NOT_SPD) {
                 *fx = DBL_MAX;
                 rc = GWM_OK;
@@ -588,11 +632,14 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
     auto record = [&](double x, double fx) {
         if (fx < DBL_MAX) log[x] = fx;
     };
-    auto bound = [&](double fx, bool exact) { return exact || fx == DBL_MAX ? 0.0 : tol_rel * std::fabs(fx); };
+    // a mixed value's bound: the evaluation's own, never below tol_rel |f|
+    auto bound = [&](double fx, bool exact, double fb) {
+        return exact || fx == DBL_MAX ? 0.0 : std::max(fb, tol_rel * std::fabs(fx));
+    };
     // upgrade both points to FP64 when a decision on (f1, f2) is not certain
     auto certify_pair = [&]() -> int {
         if (e1 && e2) return GWM_OK;
-        double m = bound(f1, e1) + bound(f2, e2);
+        double m = bound(f1, e1, b1) + bound(f2, e2, b2);
         double diff = std::fabs(f2 - f1);
         bool sure = diff > m && std::fabs(diff - eps) > m;
         if (sure) return GWM_OK;
@@ -600,19 +647,19 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
         int rc;
         if (!e1) {
             e1 = true;
-            if ((rc = eval(x1, e1, &f1))) return rc;
+            if ((rc = eval(x1, e1, &f1, &b1))) return rc;
             record(x1, f1);
         }
         if (!e2) {
             e2 = true;
-            if ((rc = eval(x2, e2, &f2))) return rc;
+            if ((rc = eval(x2, e2, &f2, &b2))) return rc;
             record(x2, f2);
         }
         return GWM_OK;
     };
     int rc;
-    if ((rc = eval(x1, e1, &f1))) return rc;
-    if ((rc = eval(x2, e2, &f2))) return rc;
+    if ((rc = eval(x1, e1, &f1, &b1))) return rc;
+    if ((rc = eval(x2, e2, &f2, &b2))) return rc;
     if (f1 == DBL_MAX && f2 == DBL_MAX) return GWM_ERR_INVALID_INITIAL;
     record(x1, f1);
     record(x2, f2);
@@ -628,8 +675,9 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
             x1 = adaptBw ? std::round(xL + d) : (xL + d);
             f2 = f1;
             e2 = e1;
+            b2 = b1;
             e1 = !certify || sticky;
-            if ((rc = eval(x1, e1, &f1))) return rc;
+            if ((rc = eval(x1, e1, &f1, &b1))) return rc;
             record(x1, f1);
         } else {
             xU = x1;
@@ -637,8 +685,9 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
             x2 = adaptBw ? std::floor(xU - d) : (xU - d);
             f1 = f2;
             e1 = e2;
+            b1 = b2;
             e2 = !certify || sticky;
-            if ((rc = eval(x2, e2, &f2))) return rc;
+            if ((rc = eval(x2, e2, &f2, &b2))) return rc;
             record(x2, f2);
         }
         iter = iter + 1;
@@ -726,7 +775,8 @@ int upload_design(gwm_hip_ctx* c, const double* X_caller, const double* y_caller
 
 // Single-column criterion / fit (GWRMultiscale.cpp:420-482, 560-636).
 int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int kernel, int adaptive, double bw,
-             int crit, int precision, double* value, double* beta) {
+             int crit, int precision, double* value, double* beta, double* bound = nullptr) {
+    if (bound) *bound = 0.0;
     int rc = check_kernel(c, kernel, precision);
     if (rc) return rc;
     if (value) *value = DBL_MAX;
@@ -769,6 +819,11 @@ int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int
         double v = cv ? red[0] : aicc(red[0], (double)n, red[1]);
         if (!std::isfinite(v)) return GWM_ERR_NOT_SPD;
         *value = v;
+        // single column: no conditioning indicator and no (res yhat)^2 sums,
+        // so the bound is tol_rel |f| and, for AICc, the trace term
+        c->last_bound =
+            precision == GWM_MIXED ? crit_error_bound(crit, v, red[0], red[1], 0.0, n, 1.0, 1.0, c->certify_rel) : 0.0;
+        if (bound) *bound = c->last_bound;
         return GWM_OK;
     }
     GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1292,8 +1347,8 @@ int gwm_hip_select_bandwidth(gwm_hip_ctx* c, int kernel, int adaptive, int crit,
     std::map<double, double> log;
     double xopt = 0.0;
     rc = golden(adaptive != 0, lo, hi,
-                [&](double x, bool exact, double* fx) {
-                    return criterion_impl(c, kernel, adaptive, x, crit, exact ? GWM_FP64 : precision, fx);
+                [&](double x, bool exact, double* fx, double* fb) {
+                    return criterion_impl(c, kernel, adaptive, x, crit, exact ? GWM_FP64 : precision, fx, fb);
                 },
                 precision == GWM_MIXED, c->certify_rel, &xopt, log);
     write_trace(log, tx, tf, cap, len);
@@ -1413,9 +1468,9 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
         std::map<double, double> log;
         const double* xi = col(i);
-        rc = golden(ad, lo, hi, [&](double x, bool exact, double* fx) {
+        rc = golden(ad, lo, hi, [&](double x, bool exact, double* fx, double* fb) {
             return var_impl(c, xi, y.data(), kernels[i], adaptive[i], x, approach[i], exact ? GWM_FP64 : precision, fx,
-                            nullptr);
+                            nullptr, fb);
         }, precision == GWM_MIXED, c->certify_rel, &bw[i], log);
         if (rc) return fail_out(rc);
     }
@@ -1427,8 +1482,8 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
     double hi0 = has_upper ? upper : (ad0 ? (double)n : maxDist);
     std::map<double, double> log0;
     double initBw = 0.0;
-    rc = golden(ad0, lo0, hi0, [&](double x, bool exact, double* fx) {
-        return criterion_impl(c, kernels[0], adaptive[0], x, approach[0], exact ? GWM_FP64 : precision, fx);
+    rc = golden(ad0, lo0, hi0, [&](double x, bool exact, double* fx, double* fb) {
+        return criterion_impl(c, kernels[0], adaptive[0], x, approach[0], exact ? GWM_FP64 : precision, fx, fb);
     }, precision == GWM_MIXED, c->certify_rel, &initBw, log0);
     if (rc) return fail_out(rc);
     // backfitting (GWRMultiscale.cpp:208-317)
@@ -1472,9 +1527,9 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
                 double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
                 std::map<double, double> log;
                 double bw1 = 0.0;
-                rc = golden(ad, lo, hi, [&](double x, bool exact, double* fx) {
+                rc = golden(ad, lo, hi, [&](double x, bool exact, double* fx, double* fb) {
                     return var_impl(c, xi, yi.data(), kernels[i], adaptive[i], x, approach[i],
-                                    exact ? GWM_FP64 : precision, fx, nullptr);
+                                    exact ? GWM_FP64 : precision, fx, nullptr, fb);
                 }, precision == GWM_MIXED, c->certify_rel, &bw1, log);
                 if (rc) return fail_out(rc);
                 if (std::fabs(bw1 - bw[i]) > bw_threshold[i]) bwChangeNo[i] = 0;
@@ -1660,6 +1715,12 @@ int gwm_hip_last_condition(gwm_hip_ctx* c, double* cond, long long* refined) {
     if (!c) return GWM_ERR_INVALID;
     if (cond) *cond = c->last_cond;
     if (refined) *refined = c->last_refined;
+    return GWM_OK;
+}
+
+int gwm_hip_last_error_bound(gwm_hip_ctx* c, double* bound) {
+    if (!c || !bound) return GWM_ERR_INVALID;
+    *bound = c->last_bound;
     return GWM_OK;
 }
 

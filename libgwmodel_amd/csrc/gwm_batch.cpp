@@ -78,6 +78,7 @@ struct Eval {
     int rc = GWM_OK;
     double value = DBL_MAX;
     double cond = 0.0;   // conditioning indicator maximum (criterion evaluations)
+    double bound = 0.0;  // error bound of a mixed criterion value (crit_error_bound)
     double shat[2] = {0.0, 0.0};
 };
 
@@ -109,7 +110,7 @@ struct Request {
     // qDiag points [0, next) of the current phase are done
     int phase = 1, next = 0;
     int nrefine = 0;             // focus points recomputed in FP64 (mixed)
-    double refine_crit = 100.0, refine_fit = 2.0;
+    double refine_crit = 10.0, refine_fit = 2.0;
     // host staging of fit outputs
     std::vector<double> hbetas, hse, hq;
 };
@@ -163,7 +164,7 @@ struct gwm_hip_server {
     unsigned int* sel_nleft = nullptr;
     int sel_cap = 0;
     double round_ms = 25.0;      // work budget of a round; <= 0: whole evaluations
-    double refine_crit = 100.0, refine_fit = 2.0;   // FP64 recomputation limits (mixed)
+    double refine_crit = 10.0, refine_fit = 2.0;   // FP64 recomputation limits (mixed)
 };
 
 namespace {
@@ -526,7 +527,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
             if (ev.type == kEvalCrit) {
                 rj.a = c->res2;
                 rj.b = ev.crit == GWM_CRIT_CV ? nullptr : c->sii;
-                rj.c = nullptr;
+                rj.c = c->ssi;   // (res yhat)^2 of the error bound
                 rj.m = c->cond;
             } else {
                 rj.a = c->sii;
@@ -663,6 +664,10 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
             ev.rc = std::isfinite(v) ? GWM_OK : GWM_ERR_NOT_SPD;
             ev.value = std::isfinite(v) ? v : DBL_MAX;
             ev.cond = o[4];
+            ev.bound = ev.precision == GWM_MIXED && std::isfinite(v)
+                           ? crit_error_bound(ev.crit, v, o[0], o[1], o[2], r->n, o[4], r->refine_crit,
+                                              r->ctx->certify_rel)
+                           : 0.0;
         } else {
             ev.rc = GWM_OK;
             ev.shat[0] = o[0];
@@ -677,7 +682,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
 
 // Worker: the host algorithm of one request.
 void worker_main(gwm_hip_server* S, Request* R) {
-    auto post = [&](int type, double bw, int prec, int crit, double* value, double* cond) -> int {
+    auto post = [&](int type, double bw, int prec, int crit, double* value, double* bound) -> int {
         std::unique_lock<std::mutex> lk(S->mu);
         R->ev = Eval();
         R->ev.type = type;
@@ -691,7 +696,7 @@ void worker_main(gwm_hip_server* S, Request* R) {
         S->cv.notify_all();
         S->cv.wait(lk, [&] { return !R->posted; });
         if (value) *value = R->ev.value;
-        if (cond) *cond = R->ev.cond;
+        if (bound) *bound = R->ev.bound;
         return R->ev.rc;
     };
     int rc = GWM_OK;
@@ -706,8 +711,8 @@ void worker_main(gwm_hip_server* S, Request* R) {
         double hi = R->has_upper ? R->upper : (R->adaptive ? (double)R->n : R->maxdist);
         std::map<double, double> log;
         rc = golden(R->adaptive != 0, lo, hi,
-                    [&](double x, bool exact, double* fx) {
-                        return post(kEvalCrit, x, exact ? GWM_FP64 : S->precision, R->crit, fx, nullptr);
+                    [&](double x, bool exact, double* fx, double* fb) {
+                        return post(kEvalCrit, x, exact ? GWM_FP64 : S->precision, R->crit, fx, fb);
                     },
                     S->precision == GWM_MIXED, certify_rel, &bw, log);
     }

@@ -75,9 +75,10 @@ struct gwm_hip_ctx {
     // Mixed precision: focus points whose local design has a conditioning
     // indicator above the limit (criterion evaluations / fits) are recomputed
     // in FP64 (gwm_hip_set_refine_condition).
-    double refine_crit = 100.0, refine_fit = 2.0;
+    double refine_crit = 10.0, refine_fit = 2.0;
     double last_cond = 0.0;      // indicator maximum of the last mixed evaluation
     long long last_refined = 0;  // focus points it recomputed in FP64
+    double last_bound = 0.0;     // error bound of the last mixed criterion value (crit_error_bound)
     int* rlist = nullptr;        // [n] focus list of the refinement
     int* rcount = nullptr;
     double* rpart = nullptr;     // partial sums of the refinement sweep
@@ -132,7 +133,12 @@ double aicc(double rss, double n, double trS);
 // GWRBasic::CalcDiagnostic (src/gwmodelpp/GWRBasic.cpp:21-34): diag[7].
 void calc_diagnostic(int n, int k, const double* X, const double* y, const double* betas, const double* shat,
                      double* diag);
-typedef std::function<int(double, bool, double*)> CritFn;
+// f(x, exact, &value, &bound): bound receives the absolute error bound of a
+// mixed-precision value (crit_error_bound), ignored for exact values.
+typedef std::function<int(double, bool, double*, double*)> CritFn;
+// Error bound of a mixed-precision criterion value (absolute); see gwm_api.cpp.
+double crit_error_bound(int crit, double value, double rss, double trS, double q, int n, double cond_max, double limit,
+                        double tol_rel);
 int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certify, double tol_rel, double* xopt_out,
            std::map<double, double>& log);
 void write_trace(const std::map<double, double>& log, double* tx, double* tf, int cap, int* len);

@@ -156,6 +156,12 @@ __device__ __forceinline__ void solve_one(const SolveArgs& a, int t) {
     for (int p = 0; p < K; ++p) yhat += beta[p] * xi[p];
     double res = a.y[i] - yhat;
     a.res2[i] = res * res;
+    if (MODE == kSolveCV || MODE == kSolveAIC) {
+        // (res * yhat)^2: the scale of the criterion's rounding error (an
+        // error of yhat relative to itself moves res^2 by 2 res yhat times it)
+        const double q = res * yhat;
+        a.ssi[i] = q * q;
+    }
     if (MODE == kSolveCV) return;
     if (MODE == kSolveBeta) {
 #pragma unroll

@@ -108,3 +108,21 @@ def test_mixed_mgwr_bandwidths_match(ctx, londonhp100):
                      precision=capi.MIXED)
     assert list(r["bandwidths"]) == [35, 98, 98]
     assert abs(r["diagnostic"]["RSquare"] - 0.757377391669) < 1e-6
+
+
+def test_certified_searches_match_fp64_on_uneven_batch(ctx):
+    """Certified mixed searches return the FP64 search's bandwidth on the
+    first 96 problems of the uneven batch workload, for CV and for AICc
+    (whose value crosses zero on some of them)."""
+    say("case: certified mixed vs FP64 searches, 96 uneven-batch problems x {CV, AICc}")
+    probs = synth.config4(512)[:96]
+    differ = []
+    for pi, p in enumerate(probs):
+        ctx.set_data(p["coords"], p["X"], p["y"])
+        for crit in (capi.CRIT_CV, capi.CRIT_AIC):
+            b64, _ = ctx.select_bandwidth(p["kernel"], p["adaptive"], crit, capi.FP64)
+            bmx, _ = ctx.select_bandwidth(p["kernel"], p["adaptive"], crit, capi.MIXED)
+            if b64 != bmx:
+                differ.append((pi, crit, b64, bmx))
+    say("  differing: %s" % differ)
+    assert not differ
