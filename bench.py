@@ -201,6 +201,21 @@ def run_varsel(ctx):
     return {"models": len(models), "selected": [int(v) for v in sel], "seconds": dt}
 
 
+def autotune_summary(lines):
+    """Classes the sweep autotuner has seen, and the largest gains of the
+    shapes it chose over the default shape (time of shape 0 / chosen)."""
+    gains = []
+    for l in lines:
+        chosen = int(l.split("chosen=")[1].split()[0])
+        ms = [float(v) for v in l.split("ms=")[1].split(",")]
+        if chosen > 0 and ms[0] > 0 and ms[chosen] > 0:
+            gains.append((ms[0] / ms[chosen], l))
+    gains.sort(reverse=True)
+    return {"classes": len(lines), "tuned_to_non_default": len(gains),
+            "median_gain": float(np.median([g for g, _ in gains])) if gains else None,
+            "top": [l for _, l in gains[:3]]}
+
+
 def serve_submit(capi, srv, r):
     n = r["X"].shape[0]
     if r["search"]:
@@ -440,6 +455,7 @@ def main():
         "mgwr_hat": mgwr_hat,
         "predict": predict,
         "varsel": varsel,
+        "autotune": autotune_summary(capi.autotune_report()),
     }
     print(json.dumps(line), flush=True)
 

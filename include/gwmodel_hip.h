@@ -209,6 +209,18 @@ int gwm_hip_last_condition(gwm_hip_ctx* ctx, double* cond, long long* refined);
  * tol * S * tr(S) with S = n (2n - 2) / (n - 2 - tr(S))^2. */
 int gwm_hip_last_error_bound(gwm_hip_ctx* ctx, double* bound);
 
+/* Online autotuning of the single-problem sweeps (no counterpart in the
+ * reference; on by default).  Per class (distance, sweep mode, precision,
+ * column tiles, n to a power of two) the first sweeps rotate through the
+ * launch shapes that leave results bit-identical -- one or two focus tiles
+ * per wave, tiles visited in order or by a coprime stride -- each timed twice
+ * with HIP events; the fastest is used from then on, process-wide.  Batched
+ * server launches keep the default shape. */
+int gwm_hip_set_autotune(gwm_hip_ctx* ctx, int enabled);
+/* The tuned classes, one text line each (shape chosen, best time of every
+ * shape in ms), NUL-terminated and truncated to cap bytes. */
+int gwm_hip_autotune_report(char* buf, int cap);
+
 /* Timing of the device kernels with HIP events on the context stream
  * (enabled by gwm_hip_set_timing).  name: "contract_<fp64|mixed>_<cv|plain|
  * hat|qdiag>" for the sweeps, "select" (adaptive bandwidths), "solve". */

@@ -44,6 +44,8 @@ SIGNATURES = {
     "gwm_hip_predict": (_i, [_vp, _i, _dp, _i, _i, _d, _i, _dp]),
     "gwm_hip_mgwr_set_hat_matrix": (_i, [_vp, _i]),
     "gwm_hip_last_error_bound": (_i, [_vp, _dp]),
+    "gwm_hip_set_autotune": (_i, [_vp, _i]),
+    "gwm_hip_autotune_report": (_i, [ctypes.c_char_p, _i]),
     "gwm_hip_mgwr_last_shat": (_i, [_vp, _dp]),
     "gwm_hip_indep_vars_criterion": (_i, [_vp, _i, _ip, _i, _dp]),
     "gwm_hip_indep_vars_select": (_i, [_vp, _i, _d, _ip, _ip, _ip, _ip, _dp, _i, _ip]),
@@ -188,6 +190,10 @@ class Context:
         self._check(self._lib.gwm_hip_fit(self.h, int(kernel), int(adaptive), float(bw), int(precision), _p(betas),
                                           _p(se), _p(shat), _p(q), _p(S)))
         return dict(betas=betas, betasSE_raw=se, shat=shat, qDiag=q, S=S)
+
+    def set_autotune(self, on=True):
+        """Online autotuning of the sweep launch shapes (results do not change)."""
+        self._check(self._lib.gwm_hip_set_autotune(self.h, int(on)))
 
     def last_error_bound(self):
         """Error bound the certified search assumes for the last mixed criterion value."""
@@ -450,6 +456,13 @@ class Server:
         self._lib.gwm_hip_server_stats(self.h, ctypes.byref(r), ctypes.byref(s), ctypes.byref(b), ctypes.byref(pb),
                                        ctypes.byref(pi))
         return dict(rounds=r.value, solves=s.value, busy_ms=b.value, peak_bytes=pb.value, peak_inflight=pi.value)
+
+
+def autotune_report():
+    """Lines of the process-wide autotuning table."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    load().gwm_hip_autotune_report(buf, len(buf))
+    return [l for l in buf.value.decode().splitlines() if l]
 
 
 def header_symbols(path=HEADER_PATH):

@@ -14,6 +14,8 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -246,6 +248,36 @@ const char* mode_name(int mode) {
     }
 }
 
+// ---------------------------------------------------------------- autotuner
+//
+// Launch shapes of a single-problem sweep that leave every result bit unchanged
+// (each focus point sums its data in the same order whatever the shape):
+//   0: default focus tiles per wave, tiles visited by the coprime stride
+//   1: default tiles per wave, tiles visited in order
+//   2: one focus tile per wave (where the default is two), stride order
+//   3: one focus tile per wave, in order
+// The shape matters where the default grid leaves the chip under-occupied:
+// fewer than kTuneGridMax workgroups (4 per CU).  The first sweep of such a
+// class (distance, mode, precision, column tiles, n to a power of two) runs
+// every candidate kTuneSamples times on the same work, timed with HIP events,
+// and the fastest is used from then on; the table is process-wide.  Larger
+// grids keep the default, whose stride spreads pruned work over the XCDs.
+struct TuneKey {
+    int dist, mode, prec, nta, ntb, nlog2;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(dist, mode, prec, nta, ntb, nlog2) < std::tie(o.dist, o.mode, o.prec, o.nta, o.ntb, o.nlog2);
+    }
+};
+struct TuneState {
+    double best_ms[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+    int ncand = 1;
+    int chosen = -1;
+};
+constexpr int kTuneSamples = 2;
+constexpr int kTuneGridMax = 1024;
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneState> g_tune;
+
 // One sweep of the fused contraction.
 int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const double* bw_pt, double bw, int kernel,
           int precision, int* PT_out, int* nseg_out) {
@@ -286,6 +318,51 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
         if (!c->pairs_dev) GWM_HIP_CHECK(c, hipMalloc((void**)&c->pairs_dev, sizeof(unsigned long long)));
         GWM_HIP_CHECK(c, hipMemsetAsync(c->pairs_dev, 0, sizeof(unsigned long long), c->stream));
         j.pairs = c->pairs_dev;
+    }
+    // launch shape: from the autotuner (results are the same for every shape)
+    if (c->autotune) {
+        TuneKey key{c->dist, mode, mixed ? 1 : 0, nta, ntb, 0};
+        while ((2 << key.nlog2) <= c->n) ++key.nlog2;
+        int chosen;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune.find(key);
+            chosen = it == g_tune.end() ? -1 : it->second.chosen;
+        }
+        if (chosen < 0) {
+            const int r = contract_r(nta, ntb, mixed ? 1 : 0);
+            const int grid = (c->n + 64 * r - 1) / (64 * r) * nseg;
+            TuneState st;
+            st.ncand = grid >= kTuneGridMax ? 1 : (r == 2 ? 4 : 2);
+            if (st.ncand > 1) {
+                if (!c->tev0) GWM_HIP_CHECK(c, hipEventCreate(&c->tev0));
+                if (!c->tev1) GWM_HIP_CHECK(c, hipEventCreate(&c->tev1));
+                SweepLaunch T = L;
+                T.job0.pairs = nullptr;
+                for (int q = 0; q < st.ncand; ++q)
+                    for (int rep = 0; rep < kTuneSamples; ++rep) {
+                        T.linear_order = q & 1;
+                        T.shape_r = q >= 2 ? 1 : 0;
+                        float ms = 0.0f;
+                        GWM_HIP_CHECK(c, hipEventRecord(c->tev0, c->stream));
+                        if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(c->dist, mode, nta, ntb, T, c->stream));
+                        else GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, T, c->stream));
+                        GWM_HIP_CHECK(c, hipEventRecord(c->tev1, c->stream));
+                        GWM_HIP_CHECK(c, hipEventSynchronize(c->tev1));
+                        GWM_HIP_CHECK(c, hipEventElapsedTime(&ms, c->tev0, c->tev1));
+                        st.best_ms[q] = std::min(st.best_ms[q], (double)ms);
+                    }
+            }
+            int best = 0;
+            for (int q = 1; q < st.ncand; ++q)
+                if (st.best_ms[q] < st.best_ms[best]) best = q;
+            st.chosen = best;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune.emplace(key, st);
+            chosen = g_tune[key].chosen;
+        }
+        L.linear_order = chosen & 1;
+        L.shape_r = chosen >= 2 ? 1 : 0;
     }
     timer_begin(c);
     if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(c->dist, mode, nta, ntb, L, c->stream));
@@ -1151,6 +1228,8 @@ void gwm_hip_destroy(gwm_hip_ctx* c) {
     free_problem(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->tev0) hipEventDestroy(c->tev0);
+    if (c->tev1) hipEventDestroy(c->tev1);
     if (c->pairs_dev) hipFree(c->pairs_dev);
     if (c->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(c->blas));
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1721,6 +1800,37 @@ int gwm_hip_last_condition(gwm_hip_ctx* c, double* cond, long long* refined) {
 int gwm_hip_last_error_bound(gwm_hip_ctx* c, double* bound) {
     if (!c || !bound) return GWM_ERR_INVALID;
     *bound = c->last_bound;
+    return GWM_OK;
+}
+
+int gwm_hip_set_autotune(gwm_hip_ctx* c, int enabled) {
+    if (!c) return GWM_ERR_INVALID;
+    c->autotune = enabled != 0;
+    return GWM_OK;
+}
+
+int gwm_hip_autotune_report(char* buf, int cap) {
+    if (!buf || cap <= 0) return GWM_ERR_INVALID;
+    static const char* modes[] = {"cv", "hat", "qdiag", "plain"};   // ContractMode order
+    std::string out;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (const auto& kv : g_tune) {
+            const TuneKey& k = kv.first;
+            const TuneState& st = kv.second;
+            char line[256];
+            int m = std::snprintf(line, sizeof(line), "%s %s %s nta=%d ntb=%d n<2^%d chosen=%d ms=",
+                                  k.dist == GWM_GREAT_CIRCLE ? "great_circle" : "euclidean",
+                                  k.mode >= 0 && k.mode < 4 ? modes[k.mode] : "?", k.prec ? "mixed" : "fp64", k.nta,
+                                  k.ntb, k.nlog2 + 1, st.chosen);
+            for (int q = 0; q < st.ncand && m < (int)sizeof(line) - 16; ++q)
+                m += std::snprintf(line + m, sizeof(line) - m, q ? ",%.4f" : "%.4f",
+                                   std::isfinite(st.best_ms[q]) ? st.best_ms[q] : -1.0);
+            out += line;
+            out += "\n";
+        }
+    }
+    std::snprintf(buf, cap, "%s", out.c_str());
     return GWM_OK;
 }
 

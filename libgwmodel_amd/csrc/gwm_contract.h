@@ -56,6 +56,12 @@ struct SweepLaunch {
     const int* items;
     int nitems;
     int tile_stride = 1;   // single job: workgroup order t visits tile (t * stride) mod tiles (set by the launcher)
+    // launch shape of a single job, chosen by the autotuner (results do not
+    // depend on it: every focus point sums its data in the same order):
+    // shape_r = 1 asks for one focus tile per wave where the default is two;
+    // linear_order = 1 visits the tiles in order instead of by the stride
+    int shape_r = 0;
+    int linear_order = 0;
 };
 
 // A stride coprime to `tiles`, near tiles / golden ratio: visiting tiles in

@@ -266,9 +266,8 @@ constexpr int r_of() {
     return (NTA + NTB) <= 8 ? 2 : 1;
 }
 
-template <int NTA, int NTB, int DIST, int MODE>
-hipError_t launch_f64_r(const SweepLaunch& a, hipStream_t s) {
-    constexpr int R = r_of<NTA, NTB>();
+template <int NTA, int NTB, int DIST, int MODE, int R>
+hipError_t launch_f64_rr(const SweepLaunch& a, hipStream_t s) {
     int grid;
     if (a.items) {
         grid = a.nitems;
@@ -279,9 +278,19 @@ hipError_t launch_f64_r(const SweepLaunch& a, hipStream_t s) {
     }
     if (grid <= 0) return hipSuccess;
     SweepLaunch b = a;
-    if (!a.items) b.tile_stride = tile_stride(grid / a.job0.nseg);
+    if (!a.items) b.tile_stride = a.linear_order ? 1 : tile_stride(grid / a.job0.nseg);
     hipLaunchKernelGGL((contract_f64_kernel<NTA, NTB, DIST, MODE, R>), dim3(grid), dim3(256), 0, s, b);
     return hipGetLastError();
+}
+
+// The default shape, or one focus tile per wave when asked for (single jobs).
+template <int NTA, int NTB, int DIST, int MODE>
+hipError_t launch_f64_r(const SweepLaunch& a, hipStream_t s) {
+    constexpr int RD = r_of<NTA, NTB>();
+    if constexpr (RD == 2) {
+        if (a.shape_r == 1 && !a.items) return launch_f64_rr<NTA, NTB, DIST, MODE, 1>(a, s);
+    }
+    return launch_f64_rr<NTA, NTB, DIST, MODE, RD>(a, s);
 }
 
 template <int DIST, int MODE>

@@ -91,6 +91,9 @@ struct gwm_hip_ctx {
     // Multiscale runs: keep the hat matrix (gwm_hip_mgwr_set_hat_matrix); the
     // rocBLAS handle of its products (created on first use); last {tr S, tr S^T S}
     bool mgwr_hat = false;
+    // online autotuning of single-problem sweep shapes (gwm_hip_set_autotune)
+    bool autotune = true;
+    hipEvent_t tev0 = nullptr, tev1 = nullptr;
     void* blas = nullptr;
     double mgwr_shat[2] = {0.0, 0.0};
     std::map<std::string, gwm::detail::KernelStat> stats;
