@@ -1153,12 +1153,14 @@ int varsel_criteria(gwm_hip_ctx* c, const double* G, int has_intercept, const st
     }
     DevBuf ib, ob;
     GWM_HIP_CHECK(c, hipMalloc(&ib.p, h.size() * sizeof(int)));
-    GWM_HIP_CHECK(c, hipMalloc(&ob.p, 2 * (size_t)ns * sizeof(double)));
+    const size_t nblk = varsel_blocks(c->n);
+    GWM_HIP_CHECK(c, hipMalloc(&ob.p, 2 * (size_t)ns * (1 + nblk) * sizeof(double)));
     int* dsets = static_cast<int*>(ib.p);
     double* dout = static_cast<double*>(ob.p);
+    double* dpart = dout + 2 * (size_t)ns;
     GWM_HIP_CHECK(c, hipMemcpyAsync(dsets, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     timer_begin(c);
-    GWM_HIP_CHECK(c, launch_varsel(c->X, c->y, c->n, k, G, dsets, dsets + ns * k, ns, dout, c->stream));
+    GWM_HIP_CHECK(c, launch_varsel(c->X, c->y, c->n, k, G, dsets, dsets + ns * k, ns, dpart, dout, c->stream));
     int rc = timer_end(c, "varsel");
     if (rc) return rc;
     std::vector<double> o(2 * ns);

@@ -192,8 +192,10 @@ hipError_t launch_smatrix(int dist, int k, const Points& P, const double* X, con
 // ((k+1)(k+2)/2 entries), then per candidate column set (sets: nsets x k,
 // nvars[s] used) out[2s] = RSS and out[2s+1] = tr(S) of its OLS (NaN: not SPD).
 hipError_t launch_gram(const double* X, const double* y, int n, int k, double* G, hipStream_t s);
+// part: 2 * nsets * varsel_blocks(n) doubles of scratch.
+int varsel_blocks(int n);
 hipError_t launch_varsel(const double* X, const double* y, int n, int k, const double* G, const int* sets,
-                         const int* nvars, int nsets, double* out, hipStream_t s);
+                         const int* nvars, int nsets, double* part, double* out, hipStream_t s);
 
 // Multiscale hat matrix (n x n column-major, context point order):
 // R (k slices) and S0 after fitAll from the per-point A^-1; the single-column

@@ -6,10 +6,13 @@
 // OLS n times, and scores the candidate column set by AICc with
 // tr(S) = sum_i x_i^T (X^T X)^-1 x_i.  Here the Gram matrix of [X y] is formed
 // once per selection (one workgroup per entry, a fixed-order reduction), and
-// each candidate set is one workgroup: Cholesky of its sub-Gram, beta with one
-// refinement step, then one pass over the points for the residual sum of
-// squares (GWRBase::RSS order) and the hat diagonal sum.
+// each candidate set is a group of workgroups over slices of the points: each
+// factors the set's sub-Gram (Cholesky, beta with one refinement step) and
+// sums the squared residuals (GWRBase::RSS order within a point) and hat
+// diagonals of its slice; a last kernel adds the slices in order.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "gwm_contract.h"
 
@@ -52,19 +55,22 @@ __device__ __forceinline__ double gram_at(const double* G, int r, int c) {
     return r >= c ? G[r * (r + 1) / 2 + c] : G[c * (c + 1) / 2 + r];
 }
 
-// One candidate set per workgroup: sets[s * k + a], a < nvars[s], are column
-// indices of X.  out[2s] = RSS, out[2s + 1] = tr(S); NaN for both when the
-// sub-Gram is not positive definite (inv_sympd's failure).
+// nblk workgroups per candidate set (set = blockIdx / nblk), each over one
+// contiguous slice of the points: sets[s * k + a], a < nvars[s], are column
+// indices of X.  part[2 (s nblk + b)] = the slice's RSS, [.. + 1] its tr(S);
+// NaN for both when the sub-Gram is not positive definite (inv_sympd's failure).
 __global__ __launch_bounds__(256) void varsel_kernel(const double* X, const double* y, int n, int k,
-                                                     const double* G, const int* sets, const int* nvars,
-                                                     double* out) {
+                                                     const double* G, const int* sets, const int* nvars, int nblk,
+                                                     double* part) {
     __shared__ double red[256];
     __shared__ double L[kVarMax * kVarMax], A[kVarMax * kVarMax];
     __shared__ double beta[kVarMax], b[kVarMax], z[kVarMax], r[kVarMax], bt[kVarMax], d[kVarMax];
     __shared__ int cols[kVarMax];
     __shared__ int ok;
-    const int s = blockIdx.x;
+    const int s = blockIdx.x / nblk, blk = blockIdx.x % nblk;
     const int p = nvars[s];
+    const int chunk = (n + nblk - 1) / nblk;
+    const int j0 = blk * chunk, j1 = min(n, j0 + chunk);
     if (threadIdx.x < p) cols[threadIdx.x] = sets[(size_t)s * k + threadIdx.x];
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -115,12 +121,13 @@ __global__ __launch_bounds__(256) void varsel_kernel(const double* X, const doub
         ok = good;
     }
     __syncthreads();
+    double* out = part + 2 * (size_t)blockIdx.x;
     if (!ok) {
-        if (threadIdx.x == 0) out[2 * s] = out[2 * s + 1] = NAN;
+        if (threadIdx.x == 0) out[0] = out[1] = NAN;
         return;
     }
     double rss = 0.0, tr = 0.0;
-    for (int j = threadIdx.x; j < n; j += 256) {
+    for (int j = j0 + threadIdx.x; j < j1; j += 256) {
         double x[kVarMax], u[kVarMax];
         double f = 0.0;
         for (int a = 0; a < p; ++a) {
@@ -142,9 +149,22 @@ __global__ __launch_bounds__(256) void varsel_kernel(const double* X, const doub
     rss = block_sum(rss, red);
     tr = block_sum(tr, red);
     if (threadIdx.x == 0) {
-        out[2 * s] = rss;
-        out[2 * s + 1] = tr;
+        out[0] = rss;
+        out[1] = tr;
     }
+}
+
+// out[2s], out[2s + 1]: the slices' sums of set s, in slice order.
+__global__ void varsel_final_kernel(const double* part, int nblk, int nsets, double* out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsets) return;
+    double rss = 0.0, tr = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        rss += part[2 * ((size_t)s * nblk + b)];
+        tr += part[2 * ((size_t)s * nblk + b) + 1];
+    }
+    out[2 * s] = rss;
+    out[2 * s + 1] = tr;
 }
 
 }  // namespace
@@ -155,11 +175,15 @@ hipError_t launch_gram(const double* X, const double* y, int n, int k, double* G
     return hipGetLastError();
 }
 
+int varsel_blocks(int n) { return std::max(1, std::min(256, (n + 4095) / 4096)); }
+
 hipError_t launch_varsel(const double* X, const double* y, int n, int k, const double* G, const int* sets,
-                         const int* nvars, int nsets, double* out, hipStream_t s) {
+                         const int* nvars, int nsets, double* part, double* out, hipStream_t s) {
     if (nsets <= 0) return hipSuccess;
     if (k < 1 || k + 1 > kVarMax) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(varsel_kernel, dim3(nsets), dim3(256), 0, s, X, y, n, k, G, sets, nvars, out);
+    const int nblk = varsel_blocks(n);
+    hipLaunchKernelGGL(varsel_kernel, dim3(nsets * nblk), dim3(256), 0, s, X, y, n, k, G, sets, nvars, nblk, part);
+    hipLaunchKernelGGL(varsel_final_kernel, dim3((nsets + 63) / 64), dim3(64), 0, s, part, nblk, nsets, out);
     return hipGetLastError();
 }
 
