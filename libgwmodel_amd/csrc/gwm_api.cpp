@@ -15,8 +15,9 @@
 #include <functional>
 #include <map>
 #include <mutex>
-#include <tuple>
 #include <string>
+#include <thread>
+#include <tuple>
 #include <vector>
 
 #include "gwm_ctx.h"
@@ -84,17 +85,35 @@ int download_caller(gwm_hip_ctx* c, const double* dev, double* host, int cols) {
     return GWM_OK;
 }
 
+// f(i) for i in [0, n) on the host's cores (element-wise work, so the
+// result does not depend on the split).
+template <class F>
+void parallel_for(size_t n, const F& f) {
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t T = std::min<size_t>(std::min<size_t>(hw, 16), (n + 16383) / 16384);
+    if (T <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            for (size_t i = n * t / T; i < n * (t + 1) / T; ++i) f(i);
+        });
+    for (auto& x : th) x.join();
+}
+
 // Great circle: the half unit vectors (hx, hy, hz) and sin(lat) of points
 // given in degrees.
 void sphere_tables(size_t N, const double* lon, const double* lat, double* hx, double* hy, double* hz, double* sl) {
     const double de2ra = kPi / 180;
-    for (size_t j = 0; j < N; ++j) {
+    parallel_for(N, [&](size_t j) {
         const double lo = lon[j] * de2ra, la = lat[j] * de2ra;
         hx[j] = 0.5 * std::cos(la) * std::cos(lo);
         hy[j] = 0.5 * std::cos(la) * std::sin(lo);
         hz[j] = 0.5 * std::sin(la);
         sl[j] = std::sin(la);
-    }
+    });
 }
 
 // Position of the cell (x, y) along the Hilbert curve of a 2^bits x 2^bits
@@ -129,21 +148,35 @@ std::vector<int> spatial_order(int n, const double* xs, const double* ys) {
         lo[d] = *std::min_element(v[d], v[d] + n);
         hi[d] = *std::max_element(v[d], v[d] + n);
     }
-    std::vector<std::pair<uint64_t, int>> key(n);
+    // 32-bit curve keys, then a stable two-pass radix sort (16-bit digits):
+    // the order of (key, index) pairs, ties in index order
+    std::vector<uint32_t> key(n);
     const double top = (double)((1u << bits) - 1);
-    for (int i = 0; i < n; ++i) {
+    parallel_for(n, [&](size_t i) {
         uint32_t q[2];
         for (int d = 0; d < 2; ++d) {
             const double span = hi[d] - lo[d];
             const double t = span > 0 ? (v[d][i] - lo[d]) / span : 0.0;
             q[d] = (uint32_t)std::min(top, std::max(0.0, std::floor(t * top + 0.5)));
         }
-        key[i] = {hilbert_index(q[0], q[1], bits), i};
+        key[i] = (uint32_t)hilbert_index(q[0], q[1], bits);
+    });
+    std::vector<int> a(n), b(n);
+    for (int i = 0; i < n; ++i) a[i] = i;
+    std::vector<int> count(1 << 16);
+    for (int shift = 0; shift < 32; shift += 16) {
+        std::fill(count.begin(), count.end(), 0);
+        for (int i = 0; i < n; ++i) ++count[(key[a[i]] >> shift) & 0xffff];
+        int sum = 0;
+        for (int& c : count) {
+            const int t = c;
+            c = sum;
+            sum += t;
+        }
+        for (int i = 0; i < n; ++i) b[count[(key[a[i]] >> shift) & 0xffff]++] = a[i];
+        a.swap(b);
     }
-    std::sort(key.begin(), key.end());
-    std::vector<int> perm(n);
-    for (int i = 0; i < n; ++i) perm[i] = key[i].second;
-    return perm;
+    return a;
 }
 
 // Bounding boxes [lo_0, hi_0, lo_1, hi_1, ...] of consecutive 64-point chunks.
