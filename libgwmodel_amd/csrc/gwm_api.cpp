@@ -54,6 +54,13 @@ void dfree(T*& p) {
 }
 
 void free_problem(gwm_hip_ctx* c) {
+    auto dfree = [c](auto*& p) {
+        if (p) {
+            if (c->async_mem) hipFreeAsync(p, c->stream);
+            else hipFree(p);
+        }
+        p = nullptr;
+    };
     dfree(c->px); dfree(c->py); dfree(c->hx); dfree(c->hy_); dfree(c->hz); dfree(c->sl);
     dfree(c->X); dfree(c->y); dfree(c->Z); dfree(c->G); dfree(c->part);
     dfree(c->boxes); dfree(c->chunk_bw); dfree(c->sel_left); dfree(c->sel_nleft);
@@ -72,7 +79,9 @@ void free_problem(gwm_hip_ctx* c) {
 
 template <class T>
 int dalloc(gwm_hip_ctx* c, T** p, size_t count) {
-    GWM_HIP_CHECK(c, hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (c->async_mem) GWM_HIP_CHECK(c, hipMallocAsync((void**)p, bytes, c->stream));
+    else GWM_HIP_CHECK(c, hipMalloc((void**)p, bytes));
     return GWM_OK;
 }
 

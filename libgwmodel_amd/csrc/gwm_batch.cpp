@@ -105,6 +105,7 @@ struct Request {
     // worker handshake
     std::thread worker;
     bool posted = false;
+    std::condition_variable wcv;   // the worker waits here for its posted evaluation
     Eval ev;
     // progress of the posted evaluation (Phase): focus points, list entries or
     // qDiag points [0, next) of the current phase are done
@@ -559,6 +560,22 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         g.off_start = T.push(g.start.data(), g.start.size() * sizeof(int));
     }
     size_t off_red = reds.empty() ? 0 : T.push(reds.data(), reds.size() * sizeof(ReduceJob));
+    // checks: one compaction launch for every checked request, counts in the
+    // arena (zero from the upload), read back with one copy
+    std::vector<CompactJob> cjobs;
+    int cmax = 0;
+    for (Request* r : checks) {
+        const double limit = r->ev.type == kEvalCrit ? r->refine_crit : r->refine_fit;
+        cjobs.push_back(CompactJob{r->ctx->cond, r->n, limit, r->ctx->rlist, nullptr});
+        cmax = std::max(cmax, r->n);
+    }
+    const std::vector<int> czero(checks.size(), 0);
+    const size_t off_cnt = checks.empty() ? 0 : T.push(czero.data(), czero.size() * sizeof(int));
+    const size_t off_cj = checks.empty() ? 0 : T.push(cjobs.data(), cjobs.size() * sizeof(CompactJob));
+    // failure flags of the FP64 recomputations that start this round
+    std::vector<int*> zeros;
+    for (Request* r : refine_starts) zeros.push_back(r->dfail);
+    const size_t off_zero = zeros.empty() ? 0 : T.push(zeros.data(), zeros.size() * sizeof(int*));
     if (T.host.size() > S->arena.cap) {
         if (S->arena.base) hipFree(S->arena.base);
         S->arena.base = nullptr;
@@ -568,11 +585,16 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         S->arena.cap = cap;
     }
     const char* dev = S->arena.base;
+    for (size_t i = 0; i < cjobs.size(); ++i) {
+        CompactJob j = cjobs[i];
+        j.count = reinterpret_cast<int*>(S->arena.base + off_cnt) + i;
+        std::memcpy(T.host.data() + off_cj + i * sizeof(CompactJob), &j, sizeof(CompactJob));
+    }
     if (!T.host.empty())
         SRV_CHECK(S, hipMemcpyAsync(S->arena.base, T.host.data(), T.host.size(), hipMemcpyHostToDevice, S->stream));
     // the failure flag of an FP64 recomputation is that of its own solves
     // (every point the mixed solves failed on is listed)
-    for (Request* r : refine_starts) SRV_CHECK(S, hipMemsetAsync(r->dfail, 0, sizeof(int), S->stream));
+    SRV_CHECK(S, launch_zero_ints(reinterpret_cast<int* const*>(dev + off_zero), (int)zeros.size(), S->stream));
     // 1. adaptive bandwidths of the main-sweep focus points
     const int sel_need = std::max(sel_start[0].back(), sel_start[1].back());
     if (sel_need > S->sel_cap) {
@@ -598,11 +620,12 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
                                         g.start.back(), S->stream));
     // 3b. mixed evaluations whose main sweep completed: list the focus points
     //     with an ill-conditioned local design
-    for (Request* r : checks) {
-        const double limit = r->ev.type == kEvalCrit ? r->refine_crit : r->refine_fit;
-        SRV_CHECK(S, hipMemsetAsync(r->ctx->rcount, 0, sizeof(int), S->stream));
-        SRV_CHECK(S, launch_compact(r->ctx->cond, r->n, limit, r->ctx->rlist, r->ctx->rcount, S->stream));
-        SRV_CHECK(S, hipMemcpyAsync(&r->nrefine, r->ctx->rcount, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    std::vector<int> ccount(checks.size(), 0);
+    if (!checks.empty()) {
+        SRV_CHECK(S, launch_compact_batch(reinterpret_cast<const CompactJob*>(dev + off_cj), (int)checks.size(), cmax,
+                                          S->stream));
+        SRV_CHECK(S, hipMemcpyAsync(ccount.data(), dev + off_cnt, ccount.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                    S->stream));
     }
     // 4. qDiag second sweeps of the fits (G complete: every solve of the fit precedes on the stream)
     for (Request* r : qstarts)
@@ -634,6 +657,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         SRV_CHECK(S, hipMemcpyAsync(r->hq.data(), r->ctx->qd, (size_t)r->n * 8, hipMemcpyDeviceToHost, S->stream));
     }
     SRV_CHECK(S, hipStreamSynchronize(S->stream));
+    for (size_t i = 0; i < checks.size(); ++i) checks[i]->nrefine = ccount[i];
     // checked evaluations: recompute the listed points next, or go on
     for (Request* r : checks) {
         if (r->nrefine > 0) {
@@ -680,8 +704,22 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
     return GWM_OK;
 }
 
+int setup(gwm_hip_server* S, Request* R);
+void teardown(Request* R);
+
 // Worker: the host algorithm of one request.
 void worker_main(gwm_hip_server* S, Request* R) {
+    {
+        const int rc = setup(S, R);
+        if (rc) {
+            teardown(R);
+            std::unique_lock<std::mutex> lk(S->mu);
+            R->status = rc;
+            R->finished = true;
+            S->cv.notify_all();
+            return;
+        }
+    }
     auto post = [&](int type, double bw, int prec, int crit, double* value, double* bound) -> int {
         std::unique_lock<std::mutex> lk(S->mu);
         R->ev = Eval();
@@ -694,7 +732,7 @@ void worker_main(gwm_hip_server* S, Request* R) {
         R->nrefine = 0;
         R->posted = true;
         S->cv.notify_all();
-        S->cv.wait(lk, [&] { return !R->posted; });
+        R->wcv.wait(lk, [&] { return !R->posted; });
         if (value) *value = R->ev.value;
         if (bound) *bound = R->ev.bound;
         return R->ev.rc;
@@ -742,6 +780,7 @@ void worker_main(gwm_hip_server* S, Request* R) {
         if (R->o_diag) std::memcpy(R->o_diag, diag, sizeof(diag));
         if (R->o_bw) *R->o_bw = bw;
     }
+    teardown(R);
     std::unique_lock<std::mutex> lk(S->mu);
     R->status = rc;
     R->finished = true;
@@ -749,29 +788,40 @@ void worker_main(gwm_hip_server* S, Request* R) {
 }
 
 void release(gwm_hip_server* S, Request* R) {
-    if (R->part) hipFree(R->part);
-    R->part = nullptr;
-    R->dfail = nullptr;
-    if (R->ctx) gwm_hip_destroy(R->ctx);
-    R->ctx = nullptr;
+    teardown(R);
     S->inflight_bytes -= R->footprint;
 }
 
-// Admission: allocate the request's device state.  Called by the scheduler.
-int admit(gwm_hip_server* S, Request* R) {
+// Device state of an admitted request: built by its worker thread (so the
+// set-ups of many requests run on the host's cores while the scheduler runs
+// rounds), from the stream-ordered allocator of the request's own stream, and
+// synchronised before the first evaluation is posted.
+int setup(gwm_hip_server* S, Request* R) {
     int rc = gwm_hip_create(S->device, &R->ctx);
     if (rc) return rc;
+    R->ctx->async_mem = true;
     rc = gwm_hip_set_data(R->ctx, R->n, R->k, R->coords.data(), R->X.data(), R->y.data(), R->dist);
     if (rc) return rc;
     R->ctx->certify_rel = S->certify_rel;
     const size_t bytes = (size_t)R->nseg * (nta_of(R->k) + ntb_of(R->k)) * 16 * R->ctx->n_pad * sizeof(double);
-    if (hipMalloc((void**)&R->part, bytes + 256) != hipSuccess) return GWM_ERR_OOM;
+    if (hipMallocAsync((void**)&R->part, bytes + 256, R->ctx->stream) != hipSuccess) return GWM_ERR_OOM;
     R->dfail = reinterpret_cast<int*>(reinterpret_cast<char*>(R->part) + bytes);
-    if (hipMemsetAsync(R->dfail, 0, sizeof(int), S->stream) != hipSuccess) return GWM_ERR_HIP;
+    if (hipMemsetAsync(R->dfail, 0, sizeof(int), R->ctx->stream) != hipSuccess) return GWM_ERR_HIP;
     if (R->select_bw && !R->adaptive && !R->has_upper) {
         if ((rc = max_distance_impl(R->ctx, &R->maxdist))) return rc;
     }
+    if (hipStreamSynchronize(R->ctx->stream) != hipSuccess) return GWM_ERR_HIP;
     return GWM_OK;
+}
+
+// Device state of a request whose work has completed: returned to the
+// stream-ordered allocator, without a device-wide synchronisation.
+void teardown(Request* R) {
+    if (R->part) hipFreeAsync(R->part, R->ctx->stream);
+    R->part = nullptr;
+    R->dfail = nullptr;
+    if (R->ctx) gwm_hip_destroy(R->ctx);
+    R->ctx = nullptr;
 }
 
 void scheduler_main(gwm_hip_server* S) {
@@ -799,19 +849,8 @@ void scheduler_main(gwm_hip_server* S) {
             Request* R = S->pending.front();
             if (!S->active.empty() && S->inflight_bytes + R->footprint > S->budget) break;
             S->pending.pop_front();
-            lk.unlock();
-            int rc = admit(S, R);
-            lk.lock();
             R->t_admit = now_ms();
             S->inflight_bytes += R->footprint;
-            if (rc) {
-                release(S, R);
-                R->status = rc;
-                R->t_done = now_ms();
-                R->done = true;
-                S->cv.notify_all();
-                continue;
-            }
             R->admitted = true;
             S->active.push_back(R);
             S->peak_bytes = std::max(S->peak_bytes, S->inflight_bytes);
@@ -844,8 +883,8 @@ void scheduler_main(gwm_hip_server* S) {
                 continue;   // in progress: more slices next round
             }
             R->posted = false;
+            R->wcv.notify_one();   // wake only this request's worker
         }
-        S->cv.notify_all();
     }
 }
 

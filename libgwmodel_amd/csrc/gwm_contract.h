@@ -210,6 +210,17 @@ hipError_t launch_sub(const double* a, const double* b, double* out, size_t coun
 hipError_t launch_trace(const double* S, int n, double* part, double* out, hipStream_t s);
 
 hipError_t launch_compact(const double* cond, int n, double limit, int* list, int* count, hipStream_t s);
+// The same for many problems in one launch (*count must be zero beforehand),
+// and one launch that zeroes the ints ptrs[0..count).
+struct CompactJob {
+    const double* cond;
+    int n;
+    double limit;
+    int* list;
+    int* count;
+};
+hipError_t launch_compact_batch(const CompactJob* jobs, int njobs, int max_n, hipStream_t s);
+hipError_t launch_zero_ints(int* const* ptrs, int count, hipStream_t s);
 
 // Deterministic sums of up to 3 vectors of length n into out[0..2], and the
 // maximum of m into out[3] (0 when m is null).

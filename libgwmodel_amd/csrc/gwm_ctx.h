@@ -91,6 +91,9 @@ struct gwm_hip_ctx {
     // Multiscale runs: keep the hat matrix (gwm_hip_mgwr_set_hat_matrix); the
     // rocBLAS handle of its products (created on first use); last {tr S, tr S^T S}
     bool mgwr_hat = false;
+    // problem buffers from the stream-ordered allocator on this context's
+    // stream (server-owned contexts: no device-wide synchronisation at free)
+    bool async_mem = false;
     // online autotuning of single-problem sweep shapes (gwm_hip_set_autotune)
     bool autotune = true;
     hipEvent_t tev0 = nullptr, tev1 = nullptr;

@@ -299,6 +299,20 @@ __global__ __launch_bounds__(256) void compact_kernel(const double* cond, int n,
     if (!(c <= limit)) list[atomicAdd(count, 1)] = i;
 }
 
+// compact_kernel over many problems: problem blockIdx.y, grid-stride in x.
+__global__ __launch_bounds__(256) void compact_batch_kernel(const CompactJob* jobs) {
+    const CompactJob J = jobs[blockIdx.y];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < J.n; i += gridDim.x * 256) {
+        const double c = J.cond[i];
+        if (!(c <= J.limit)) J.list[atomicAdd(J.count, 1)] = i;
+    }
+}
+
+__global__ void zero_ints_kernel(int* const* ptrs, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) *ptrs[i] = 0;
+}
+
 // Deterministic fixed-order sums of a, b, c and the maximum of m (one
 // workgroup): out[0..2] = sums, *mx = max (0 when m is null).
 __device__ __forceinline__ void reduce_body(const double* a, const double* b, const double* c, const double* m, int n,
@@ -485,6 +499,20 @@ hipError_t launch_chunk_max(const double* v, int n, double* out, hipStream_t s) 
 hipError_t launch_compact(const double* cond, int n, double limit, int* list, int* count, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(compact_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cond, n, limit, list, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_batch(const CompactJob* jobs, int njobs, int max_n, hipStream_t s) {
+    if (njobs <= 0 || max_n <= 0) return hipSuccess;
+    if (njobs > 65535) return hipErrorInvalidValue;
+    const int bx = std::min((max_n + 255) / 256, 1024);
+    hipLaunchKernelGGL(compact_batch_kernel, dim3(bx, njobs), dim3(256), 0, s, jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(int* const* ptrs, int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((count + 255) / 256), dim3(256), 0, s, ptrs, count);
     return hipGetLastError();
 }
 
