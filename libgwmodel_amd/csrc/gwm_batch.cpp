@@ -68,6 +68,7 @@ enum Phase {
 constexpr int kChunk = 128;              // focus points; a multiple of every sweep's 64 R
 constexpr double kUnitsPerMs = 1.5e9;
 constexpr double kFp64Factor = 3.5;
+constexpr double kGatherMs = 2.0;   // longest wait for the last posts before a round starts
 constexpr double kSelectUnits = 4.0;
 
 struct Eval {
@@ -862,8 +863,10 @@ void scheduler_main(gwm_hip_server* S) {
             S->cv.wait(lk, [&] { return S->stopping || !S->pending.empty(); });
             continue;
         }
-        // wait until every in-flight request has posted an evaluation or finished
-        S->cv.wait(lk, [&] {
+        // wait until every in-flight request has posted an evaluation or
+        // finished, for at most kGatherMs: a worker that is still setting up
+        // or finishing on the host joins a later round
+        S->cv.wait_for(lk, std::chrono::microseconds((long long)(kGatherMs * 1000)), [&] {
             for (Request* R : S->active)
                 if (!R->posted && !R->finished) return false;
             return true;
@@ -871,7 +874,14 @@ void scheduler_main(gwm_hip_server* S) {
         std::vector<Request*> batch;
         for (Request* R : S->active)
             if (R->posted) batch.push_back(R);
-        if (batch.empty()) continue;
+        if (batch.empty()) {
+            S->cv.wait(lk, [&] {
+                for (Request* R : S->active)
+                    if (R->posted || R->finished) return true;
+                return S->active.empty();
+            });
+            continue;
+        }
         lk.unlock();
         int rc = run_round(S, batch);
         lk.lock();
