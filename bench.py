@@ -280,7 +280,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=100_000)
     ap.add_argument("--fp64-steps", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=512, help="focus points of the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=16384,
+                    help="focus points of the CPU baseline sample (about 10 s of host work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mgwr-n", type=int, default=50_000, help="config-3 multiscale size (0: skip)")
     ap.add_argument("--mgwr-hat-n", type=int, default=4000, help="multiscale size with the hat matrix (0: skip)")
