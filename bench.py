@@ -71,8 +71,8 @@ def cpu_baseline(d, bw, sample, threads):
 
 
 def run_batch(capi, synth, num, loop):
-    """Config-4 batch: every problem is a CV search + fit, mixed precision."""
-    probs = synth.config4(num)
+    """Uneven batch: every problem is a CV search + fit, mixed precision."""
+    probs = synth.uneven_batch(num)
     sum_n = sum(p["X"].shape[0] for p in probs)
     log("batch: %d problems, sum n %d" % (num, sum_n))
     with capi.Server(0, capi.MIXED, max_inflight=num) as srv:
@@ -109,8 +109,8 @@ def run_batch(capi, synth, num, loop):
 
 
 def run_mgwr(capi, synth, n):
-    """Config-3 multiscale backfitting, mixed precision and FP64."""
-    d = synth.config3(n=n)
+    """Multiscale-cosines backfitting, mixed precision and FP64."""
+    d = synth.multiscale_cosines(n=n)
     k = d["X"].shape[1]
     out = {"n": n, "k": k}
     res = {}
@@ -138,7 +138,7 @@ def run_mgwr(capi, synth, n):
 def run_mgwr_hat(capi, synth, n):
     """Multiscale backfitting with the hat matrix (S0 and one n x n operator
     per variable on the device, an FP64 GEMM per backfitting step)."""
-    d = synth.config3(n=n)
+    d = synth.multiscale_cosines(n=n)
     X = d["X"][:, :3]
     k = X.shape[1]
     out = {"n": n, "k": k}
@@ -169,7 +169,7 @@ def run_mgwr_hat(capi, synth, n):
 
 
 def run_predict(ctx, capi, d, bw, m, seed=2024):
-    """Betas at m new locations of the config-2 box at the selected bandwidth."""
+    """Betas at m new locations of the sphere-box workload at the selected bandwidth."""
     rng = np.random.default_rng(seed)
     loc = np.column_stack([rng.uniform(-10, 30, m), rng.uniform(35, 60, m)])
     out = {"m": m, "bandwidth_km": bw}
@@ -193,7 +193,7 @@ def run_predict(ctx, capi, d, bw, m, seed=2024):
 
 
 def run_varsel(ctx):
-    """Forward selection over the config-2 covariates (unit-weight AICc per candidate set)."""
+    """Forward selection over the sphere-box covariates (unit-weight AICc per candidate set)."""
     log("variable selection")
     t0 = time.perf_counter()
     sel, models = ctx.indep_vars_select(threshold=3.0)
@@ -224,9 +224,9 @@ def serve_submit(capi, srv, r):
 
 
 def run_serve(capi, synth, num, rate, check):
-    """Config-5 stream replayed in real time; fits use an adaptive bisquare
+    """Serving stream replayed in real time; fits use an adaptive bisquare
     kernel of 100 neighbours, searches CV over [20, n] neighbours."""
-    reqs = synth.config5(num, rate=rate)
+    reqs = synth.serving_stream(num, rate=rate)
     log("serve: %d requests at %.2f/s (%d searches)" % (num, rate, sum(r["search"] for r in reqs)))
     with capi.Server(0, capi.MIXED, max_inflight=64) as srv:
         start = time.perf_counter()
@@ -283,13 +283,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16384,
                     help="focus points of the CPU baseline sample (about 10 s of host work)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mgwr-n", type=int, default=50_000, help="config-3 multiscale size (0: skip)")
+    ap.add_argument("--mgwr-n", type=int, default=50_000, help="multiscale-cosines size (0: skip)")
     ap.add_argument("--mgwr-hat-n", type=int, default=4000, help="multiscale size with the hat matrix (0: skip)")
     ap.add_argument("--predict-m", type=int, default=-1, help="prediction locations (default n; 0: skip)")
-    ap.add_argument("--batch-problems", type=int, default=512, help="config-4 batch size (0: skip)")
+    ap.add_argument("--batch-problems", type=int, default=512, help="uneven batch size (0: skip)")
     ap.add_argument("--no-loop", action="store_true", help="skip the single-problem loop of the batch")
-    ap.add_argument("--serve-requests", type=int, default=64, help="config-5 stream length (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="config-5 arrival rate, requests/s")
+    ap.add_argument("--serve-requests", type=int, default=64, help="serving stream length (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="serving-stream arrival rate, requests/s")
     ap.add_argument("--serve-check", type=int, default=3,
                     help="requests of each type re-run in FP64 for the beta error")
     ap.add_argument("--sweeps-only", type=int, default=0,
@@ -329,9 +329,9 @@ def main():
         return
 
     n = args.n
-    lo, hi = synth.CONFIG2_BW_KM
-    log("generating config-2 workload n=%d" % n)
-    d = synth.config2(n=n)
+    lo, hi = synth.SPHERE_BOX_BW_KM
+    log("generating sphere-box workload n=%d" % n)
+    d = synth.sphere_box(n=n)
     k = d["X"].shape[1]
     ctx = capi.Context(0)
     ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)

@@ -42,7 +42,7 @@ class Stream:
         return mu + sigma * z
 
 
-def config1(n=2000, seed=1001):
+def uniform_square(n=2000, seed=1001):
     """n points on U[0,100]^2, k = 4: beta0 = 1+(u+v)/50, beta1 = sin(u/20),
     beta2 = cos(v/25), beta3 = 0.5, noise N(0, 0.5^2)."""
     s = Stream(seed)
@@ -54,7 +54,7 @@ def config1(n=2000, seed=1001):
     return dict(coords=np.column_stack([u, v]), X=X, y=y, beta=B)
 
 
-def config2(n=100_000, seed=1002, k=8):
+def sphere_box(n=100_000, seed=1002, k=8):
     """lon ~ U[-10,30], lat ~ U[35,60]; X = [1, 7 covariates N(0,1) with
     equicorrelation 0.3]; each beta is a sum of 3 Gaussian bumps (centre in the
     box, width U[2,8] degrees, amplitude N(0,1)); noise N(0, 0.3^2)."""
@@ -77,12 +77,12 @@ def config2(n=100_000, seed=1002, k=8):
     return dict(coords=np.column_stack([lon, lat]), X=X, y=y, beta=B)
 
 
-# Bandwidth interval of config 2: [0.5, 20] degrees of arc in the reference's
+# Bandwidth interval of the sphere-box workload: [0.5, 20] degrees of arc in the reference's
 # great-circle unit (km), 1 degree = 111.3195 km.
-CONFIG2_BW_KM = (0.5 * 111.3195, 20 * 111.3195)
+SPHERE_BOX_BW_KM = (0.5 * 111.3195, 20 * 111.3195)
 
 
-def config3(n=50_000, seed=1003, extent=1000.0, scales=(20, 50, 100, 200, 400, 1000)):
+def multiscale_cosines(n=50_000, seed=1003, extent=1000.0, scales=(20, 50, 100, 200, 400, 1000)):
     """MGWR workload: U[0,extent]^2, k = 6; beta_j = sum of 16 random cosines
     with length scale scales[j]; noise N(0, 0.5^2)."""
     s = Stream(seed)
@@ -101,10 +101,10 @@ def config3(n=50_000, seed=1003, extent=1000.0, scales=(20, 50, 100, 200, 400, 1
     return dict(coords=np.column_stack([u, v]), X=X, y=y, beta=B)
 
 
-KERNELS_CONFIG4 = (0, 2, 3)   # Gaussian, bisquare, tricube
+UNEVEN_BATCH_KERNELS = (0, 2, 3)   # Gaussian, bisquare, tricube
 
 
-def config4(num=512, seed=1004):
+def uneven_batch(num=512, seed=1004):
     """Uneven batch: num problems, n ~ logU[200, 20000], k ~ U{3..12},
     coordinates U[0,100]^2, intercept + N(0,1) covariates, each beta linear
     plus one sinusoid (frequency U[0.01, 0.1]), noise N(0, 0.2^2); kernel from
@@ -124,17 +124,17 @@ def config4(num=512, seed=1004):
             th = s.uniform(1, 0, 2 * np.pi)[0]
             B[:, c] = a0 + 0.01 * (a1 * u + a2 * v) + np.sin(2 * np.pi * fr * (u * np.cos(th) + v * np.sin(th)))
         y = np.sum(X * B, axis=1) + s.normal(n, 0, 0.2)
-        kernel = KERNELS_CONFIG4[int(s.uniform(1, 0, 3)[0])]
+        kernel = UNEVEN_BATCH_KERNELS[int(s.uniform(1, 0, 3)[0])]
         adaptive = bool(s.uniform(1)[0] < 0.5)
         probs.append(dict(coords=np.column_stack([u, v]), X=X, y=y, kernel=kernel, adaptive=adaptive))
     return probs
 
 
-def config5(num=64, rate=2.0, seed=1005, n_range=(500, 200_000)):
+def serving_stream(num=64, rate=2.0, seed=1005, n_range=(500, 200_000)):
     """Serving stream: Poisson arrivals at `rate` requests/s; 70% fits at a
     given bandwidth, 30% CV bandwidth searches; n ~ logU[n_range], k ~ U{2..16};
     coordinates from 8 Gaussian clusters (sigma 5) on [0,100]^2; N(0,1)
-    covariates, beta as in config 3 (random cosines), noise N(0, 0.5^2)."""
+    covariates, beta as in multiscale_cosines (random cosines), noise N(0, 0.5^2)."""
     s = Stream(seed)
     centres = s.uniform(16, 10, 90).reshape(8, 2)
     t = 0.0

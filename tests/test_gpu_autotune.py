@@ -19,7 +19,7 @@ def say(msg):
 
 @pytest.mark.parametrize("prec", [capi.FP64, capi.MIXED])
 def test_shapes_are_bit_identical(prec):
-    d = synth.config1(n=3000)
+    d = synth.uniform_square(n=3000)
     with capi.Context(0) as ctx:
         ctx.set_data(d["coords"], d["X"], d["y"])
         ctx.set_autotune(False)

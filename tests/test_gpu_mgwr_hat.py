@@ -70,8 +70,8 @@ def test_kat_bounds_hat(ctx, londonhp100):
 
 
 def test_hat_vs_oracle_n600(ctx, oracle):
-    say("case: MGWR with hat vs oracle config1 n=600")
-    d = synth.config1(n=600)
+    say("case: MGWR with hat vs oracle uniform_square n=600")
+    d = synth.uniform_square(n=600)
     Xm = d["X"][:, :3]
     ctx.set_data(d["coords"], Xm, d["y"])
     kw = dict(crit_type=1, lower=None, upper=None)

@@ -48,7 +48,7 @@ MIX_CASES = [
 @pytest.mark.parametrize("kernel,adaptive,bw", MIX_CASES)
 def test_mixed_criteria_and_fit_euclidean(ctx, kernel, adaptive, bw):
     say("case: mixed euclidean kernel=%d adaptive=%d bw=%g" % (kernel, adaptive, bw))
-    d = synth.config1(n=3000)
+    d = synth.uniform_square(n=3000)
     ctx.set_data(d["coords"], d["X"], d["y"])
     for crit in [capi.CRIT_CV, capi.CRIT_AIC]:
         f64 = ctx.criterion(kernel, adaptive, bw, crit, capi.FP64)
@@ -67,9 +67,9 @@ def test_mixed_criteria_and_fit_euclidean(ctx, kernel, adaptive, bw):
     assert es < 1e-4
 
 
-def test_mixed_great_circle_config2(ctx):
-    say("case: mixed great-circle config2 n=6000")
-    d = synth.config2(n=6000)
+def test_mixed_great_circle_sphere_box(ctx):
+    say("case: mixed great-circle sphere_box n=6000")
+    d = synth.sphere_box(n=6000)
     ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
     # at n = 6000 over the 40 x 25 degree box a 60 km bandwidth leaves ~8
     # effective neighbours for k = 8 (near-singular local designs); bandwidths
@@ -84,16 +84,16 @@ def test_mixed_great_circle_config2(ctx):
 
 
 def test_certified_search_matches_fp64(ctx):
-    say("case: certified mixed search config2 n=4000")
-    d = synth.config2(n=4000)
-    lo, hi = synth.CONFIG2_BW_KM
+    say("case: certified mixed search sphere_box n=4000")
+    d = synth.sphere_box(n=4000)
+    lo, hi = synth.SPHERE_BOX_BW_KM
     ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
     b64, t64 = ctx.select_bandwidth(capi.GAUSSIAN, False, capi.CRIT_AIC, capi.FP64, lo, hi)
     bm, tm = ctx.select_bandwidth(capi.GAUSSIAN, False, capi.CRIT_AIC, capi.MIXED, lo, hi)
     assert bm == b64
     assert [x for x, _ in tm] == [x for x, _ in t64]
-    say("case: certified mixed search config1 adaptive bisquare CV n=2000")
-    d = synth.config1(n=2000)
+    say("case: certified mixed search uniform_square adaptive bisquare CV n=2000")
+    d = synth.uniform_square(n=2000)
     ctx.set_data(d["coords"], d["X"], d["y"])
     b64, t64 = ctx.select_bandwidth(capi.BISQUARE, True, capi.CRIT_CV, capi.FP64, 20, 2000)
     bm, tm = ctx.select_bandwidth(capi.BISQUARE, True, capi.CRIT_CV, capi.MIXED, 20, 2000)
@@ -115,7 +115,7 @@ def test_certified_searches_match_fp64_on_uneven_batch(ctx):
     first 96 problems of the uneven batch workload, for CV and for AICc
     (whose value crosses zero on some of them)."""
     say("case: certified mixed vs FP64 searches, 96 uneven-batch problems x {CV, AICc}")
-    probs = synth.config4(512)[:96]
+    probs = synth.uneven_batch(512)[:96]
     differ = []
     for pi, p in enumerate(probs):
         ctx.set_data(p["coords"], p["X"], p["y"])

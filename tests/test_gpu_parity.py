@@ -52,7 +52,7 @@ def test_euclidean_distances_bitexact(ctx, oracle, londonhp100):
 
 def test_great_circle_distances(ctx, oracle):
     say("case: great-circle distances")
-    d = synth.config2(n=500)
+    d = synth.sphere_box(n=500)
     ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
     for f in [0, 3, 250, 499]:
         g = ctx.distance_row(f)
@@ -102,7 +102,7 @@ CASES = [
 @pytest.mark.parametrize("kernel,adaptive,bw", CASES)
 def test_criteria_match_oracle(ctx, oracle, kernel, adaptive, bw):
     say("case: criteria kernel=%d adaptive=%d bw=%g" % (kernel, adaptive, bw))
-    d = synth.config1(n=400)
+    d = synth.uniform_square(n=400)
     ctx.set_data(d["coords"], d["X"], d["y"])
     for crit in [capi.CRIT_CV, capi.CRIT_AIC]:
         g = ctx.criterion(kernel, adaptive, bw, crit)
@@ -113,7 +113,7 @@ def test_criteria_match_oracle(ctx, oracle, kernel, adaptive, bw):
 @pytest.mark.parametrize("kernel,adaptive,bw", CASES)
 def test_fit_matches_oracle(ctx, oracle, kernel, adaptive, bw):
     say("case: fit kernel=%d adaptive=%d bw=%g" % (kernel, adaptive, bw))
-    d = synth.config1(n=300)
+    d = synth.uniform_square(n=300)
     ctx.set_data(d["coords"], d["X"], d["y"])
     g = ctx.fit(kernel, adaptive, bw, want_S=True)
     o = oracle.fit_core(d["coords"], d["X"], d["y"], kernel, adaptive, bw, want_S=True)
@@ -186,8 +186,8 @@ def test_kat_mgwr_no_hat(ctx, londonhp100):
 # ------------------------------------------------------------------ searches vs oracle
 
 def test_config1_cv_search_matches_oracle(ctx, oracle):
-    say("case: config1 n=600 bisquare adaptive CV search")
-    d = synth.config1(n=600)
+    say("case: uniform_square n=600 bisquare adaptive CV search")
+    d = synth.uniform_square(n=600)
     ctx.set_data(d["coords"], d["X"], d["y"])
     g = ctx.gwr_run(capi.BISQUARE, True, 0, select_bw=True, crit=capi.CRIT_CV, lower=20, upper=600)
     o = oracle.gwr_run(d["coords"], d["X"], d["y"], oracle.BISQUARE, True, 0, select_bw=True, crit=oracle.CRIT_CV,
@@ -201,9 +201,9 @@ def test_config1_cv_search_matches_oracle(ctx, oracle):
 
 
 def test_config2_aicc_search_matches_oracle(ctx, oracle):
-    say("case: config2 n=800 great-circle gaussian AICc search")
-    d = synth.config2(n=800)
-    lo, hi = synth.CONFIG2_BW_KM
+    say("case: sphere_box n=800 great-circle gaussian AICc search")
+    d = synth.sphere_box(n=800)
+    lo, hi = synth.SPHERE_BOX_BW_KM
     ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
     g = ctx.gwr_run(capi.GAUSSIAN, False, 0, select_bw=True, crit=capi.CRIT_AIC, lower=lo, upper=hi)
     o = oracle.gwr_run(d["coords"], d["X"], d["y"], oracle.GAUSSIAN, False, 0, select_bw=True, crit=oracle.CRIT_AIC,

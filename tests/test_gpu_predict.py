@@ -80,10 +80,10 @@ def test_predict_at_data_points_is_fit(ctx, londonhp100):
 
 
 def test_predict_more_locations_than_data_and_single(ctx, oracle):
-    d = synth.config1(n=300)
+    d = synth.uniform_square(n=300)
     ctx.set_data(d["coords"], d["X"], d["y"])
     loc = locations_in(d["coords"], 2500, seed=22)
-    say("case: predict config1 n=300 m=2500 adaptive gaussian 60")
+    say("case: predict uniform_square n=300 m=2500 adaptive gaussian 60")
     o = oracle.predict(d["coords"], d["X"], d["y"], loc, capi.GAUSSIAN, True, 60)
     g = ctx.predict(loc, capi.GAUSSIAN, True, 60)
     assert close_cols(g, o)
@@ -93,7 +93,7 @@ def test_predict_more_locations_than_data_and_single(ctx, oracle):
 
 
 def test_predict_great_circle_vs_oracle(ctx, oracle):
-    d = synth.config2(n=3000)
+    d = synth.sphere_box(n=3000)
     ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
     loc = locations_in(d["coords"], 700, seed=23, pad=0.0)
     for kern, adaptive, bw in [(capi.GAUSSIAN, True, 80), (capi.BISQUARE, False, 400.0)]:
@@ -107,7 +107,7 @@ def test_predict_great_circle_vs_oracle(ctx, oracle):
 
 
 def test_predict_pruning_exact(ctx, oracle):
-    d = synth.config1(n=20000)
+    d = synth.uniform_square(n=20000)
     ctx.set_data(d["coords"], d["X"], d["y"])
     loc = locations_in(d["coords"], 3000, seed=24)
     try:

@@ -48,7 +48,7 @@ HARD = (209, 221, 254)
 
 @pytest.fixture(scope="module")
 def hard_problems():
-    probs = synth.config4(max(HARD) + 1)
+    probs = synth.uneven_batch(max(HARD) + 1)
     return [probs[i] for i in HARD]
 
 

@@ -114,8 +114,8 @@ def test_sliced_rounds_equal_whole_rounds(precision):
 
 
 def test_server_search_matches_oracle(oracle):
-    say("case: server CV search vs oracle (config1 n=800)")
-    d = synth.config1(n=800)
+    say("case: server CV search vs oracle (uniform_square n=800)")
+    d = synth.uniform_square(n=800)
     with capi.Server(0, capi.MIXED) as srv:
         r = srv.wait(srv.submit(d["coords"], d["X"], d["y"], capi.BISQUARE, True, 0, select_bw=True,
                                 crit=capi.CRIT_CV, lower=20, upper=800))

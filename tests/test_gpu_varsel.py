@@ -64,7 +64,7 @@ def test_singular_design_is_dbl_max(ctx, londonhp100):
 
 @pytest.mark.parametrize("has_intercept", [True, False])
 def test_select_vs_oracle_larger(ctx, oracle, has_intercept):
-    d = synth.config1(n=20000)
+    d = synth.uniform_square(n=20000)
     rng = np.random.default_rng(31)
     X = np.column_stack([d["X"], rng.normal(size=(20000, 4))])   # 4 pure-noise columns
     say("case: selection n=20000 k=%d intercept=%s" % (X.shape[1], has_intercept))
