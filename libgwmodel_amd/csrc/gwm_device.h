@@ -121,10 +121,21 @@ __device__ __forceinline__ double gcdist_f64(double lonf, double latf, double hx
 }
 
 // Kernel weights with the CPU boundary (d < bw): BandwidthWeight.h:68-121.
+//
+// Gaussian and exponential weights whose exponent is below -kExpCut
+// (e^-72 = 5.4e-32 of the weight 1 at d = 0, i.e. beyond 12 and 72
+// bandwidths) are taken as zero.  The reference keeps them until exp
+// underflows (e^-745); dropping them moves every entry of X^T W X and X^T W y
+// by at most 5.4e-32 times the unweighted sum of its |z_j|, and lets the
+// sweeps skip points beyond 12 bandwidths instead of 40 exactly.
+constexpr double kExpCut = 72.0;
+
+__device__ __forceinline__ double exp_cut(double e) { return e < -kExpCut ? 0.0 : exp(e); }
+
 __device__ __forceinline__ double kernel_f64(int kernel, double d, double bw) {
     switch (kernel) {
-    case kGaussian: return exp((d * d) / ((-2.0) * (bw * bw)));
-    case kExponential: return exp(-d / bw);
+    case kGaussian: return exp_cut((d * d) / ((-2.0) * (bw * bw)));
+    case kExponential: return exp_cut(-d / bw);
     case kBisquare: {
         double t = 1.0 - (d * d) / (bw * bw);
         return d < bw ? t * t : 0.0;
@@ -156,8 +167,8 @@ __device__ __forceinline__ KConst64 make_kconst64(double bw) {
 
 __device__ __forceinline__ double kernel_fast_f64(int kernel, double d, const KConst64& k) {
     switch (kernel) {
-    case kGaussian: return exp((d * d) * k.gauss);
-    case kExponential: return exp(-d * k.inv_bw);
+    case kGaussian: return exp_cut((d * d) * k.gauss);
+    case kExponential: return exp_cut(-d * k.inv_bw);
     case kBisquare: {
         double t = 1.0 - (d * d) * k.inv_bw2;
         return d < k.bw ? t * t : 0.0;
@@ -264,15 +275,16 @@ __device__ __forceinline__ float kernel_f32(int kernel, float d, float bw, float
 // zero contributes exactly nothing, and the sweep skips it; results are the
 // same bits as without skipping.
 
-// Distance beyond which the kernel weight is exactly zero: in FP64 (exp
-// underflows to 0 below e^-745; compact kernels vanish for d >= bw), or in the
-// mixed path, whose FP32 weight is split into FP16 hi + lo parts that are both
-// zero below 2^-36 (exp2 arguments below -37 keep a margin).  Never zero:
-// infinity.
+// Distance beyond which the kernel weight is exactly zero: in FP64 (the
+// exponent cut at -kExpCut of kernel_f64; compact kernels vanish for
+// d >= bw), or in the mixed path, whose FP32 weight is split into FP16 hi + lo
+// parts that are both zero below 2^-36 (exp2 arguments below -37 keep a
+// margin).  The FP64 distances carry a relative margin of 1e-9 over the cut
+// for the rounding of the box gaps and of the exponent.  Never zero: infinity.
 __device__ __forceinline__ double zero_weight_distance(int kernel, double bw, bool mixed) {
     switch (kernel) {
-    case kGaussian: return (mixed ? 7.25 : 40.0) * bw;      // d^2 / (2 b^2) > 26.3 (mixed) / 800 (FP64)
-    case kExponential: return (mixed ? 26.0 : 800.0) * bw;
+    case kGaussian: return (mixed ? 7.25 : 12.0 * (1.0 + 1e-9)) * bw;   // d^2 / (2 b^2) > 26.3 (mixed) / 72 (FP64)
+    case kExponential: return (mixed ? 26.0 : kExpCut * (1.0 + 1e-9)) * bw;
     default: return bw * (mixed ? 1.0 + 1e-5 : 1.0 + 1e-9);
     }
 }

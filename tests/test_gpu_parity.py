@@ -124,6 +124,30 @@ def test_fit_matches_oracle(ctx, oracle, kernel, adaptive, bw):
     assert close(g["S"], o["S"], 1e-9, 1e-12)
 
 
+@pytest.mark.parametrize("kernel,bw", [(capi.GAUSSIAN, 4.0), (capi.EXPONENTIAL, 1.0)])
+def test_exponent_cut_matches_oracle(ctx, oracle, kernel, bw):
+    """The FP64 weights take Gaussian / exponential weights below e^-72 as zero
+    (beyond 12 / 72 bandwidths), where the oracle keeps them down to exp
+    underflow.  At these bandwidths a large share of the pairs lies in that
+    band, and the results still agree at the FP64 tolerances."""
+    say("case: exponent cut kernel=%d bw=%g" % (kernel, bw))
+    d = synth.uniform_square(n=1500)
+    c = d["coords"]
+    cut = (12.0 if kernel == capi.GAUSSIAN else 72.0) * bw
+    dd = np.hypot(c[:300, None, 0] - c[None, :, 0], c[:300, None, 1] - c[None, :, 1])
+    assert np.mean(dd > cut) > 0.1   # the band the cut drops is exercised
+    ctx.set_data(c, d["X"], d["y"])
+    for crit in [capi.CRIT_CV, capi.CRIT_AIC]:
+        g = ctx.criterion(kernel, False, bw, crit)
+        o = oracle.criterion(c, d["X"], d["y"], kernel, False, bw, crit)
+        assert g == pytest.approx(o, rel=1e-11), (crit, g, o)
+    g = ctx.fit(kernel, False, bw)
+    o = oracle.fit_core(c, d["X"], d["y"], kernel, False, bw)
+    assert close(g["betas"], o["betas"], 1e-9, 1e-12)
+    assert close(g["shat"], o["shat"], 1e-10)
+    assert close(g["qDiag"], o["qDiag"], 1e-9, 1e-12)
+
+
 @pytest.mark.parametrize("k", [1, 2, 5, 8, 12, 16])
 def test_fit_all_k(ctx, oracle, k):
     say("case: fit k=%d" % k)
