@@ -73,6 +73,8 @@ void free_problem(gwm_hip_ctx* c) {
     dfree(c->vZ); dfree(c->S); dfree(c->fail);
     dfree(c->Zh); dfree(c->Zl); dfree(c->Gh); dfree(c->Gl); dfree(c->vZh); dfree(c->vZl);
     dfree(c->zscale); dfree(c->gscale); dfree(c->vscale); dfree(c->colmax);
+    dfree(c->plan_counts); dfree(c->plan_order);
+    c->plan_items = 0;
     c->part_bytes = 0;
     c->n = 0;
 }
@@ -214,6 +216,19 @@ int ensure_part(gwm_hip_ctx* c, size_t bytes) {
     c->part_bytes = 0;
     GWM_HIP_CHECK(c, hipMalloc((void**)&c->part, bytes));
     c->part_bytes = bytes;
+    return GWM_OK;
+}
+
+int ensure_plan(gwm_hip_ctx* c, size_t items) {
+    if (items <= c->plan_items) return GWM_OK;
+    hipFree(c->plan_counts);
+    hipFree(c->plan_order);
+    c->plan_counts = nullptr;
+    c->plan_order = nullptr;
+    c->plan_items = 0;
+    GWM_HIP_CHECK(c, hipMalloc((void**)&c->plan_counts, items * sizeof(int)));
+    GWM_HIP_CHECK(c, hipMalloc((void**)&c->plan_order, items * sizeof(int)));
+    c->plan_items = items;
     return GWM_OK;
 }
 
@@ -405,6 +420,20 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
         }
         L.linear_order = chosen & 1;
         L.shape_r = chosen >= 2 ? 1 : 0;
+        // Large pruned sweeps run their (tile, segment) items longest first
+        // (gwm_plan.hip): narrow bandwidths leave a few busy items per tile,
+        // and in plain order the sweep ends on a tail of them.
+        const int r = contract_r(nta, ntb, mixed ? 1 : 0);
+        const int tiles = (c->n + 64 * r - 1) / (64 * r);
+        if (c->prune && chosen == 0 && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg &&
+            seg_len <= 64 * kPlanMaxCount) {
+            if ((rc = ensure_plan(c, (size_t)tiles * nseg))) return rc;
+            GWM_HIP_CHECK(c, launch_sweep_plan(c->dist, c->points(), c->boxes, mode == kModeQDiag ? nullptr : bw_pt,
+                                               bw, j.chunk_bw, kernel, mixed ? 1 : 0, 64 * r, seg_len, nseg, tiles,
+                                               c->plan_counts, c->plan_order, c->stream));
+            L.order = c->plan_order;
+            L.norder = tiles * nseg;
+        }
     }
     timer_begin(c);
     if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(c->dist, mode, nta, ntb, L, c->stream));

@@ -62,6 +62,11 @@ struct SweepLaunch {
     // linear_order = 1 visits the tiles in order instead of by the stride
     int shape_r = 0;
     int linear_order = 0;
+    // optional work order of a single job (launch_sweep_plan): workgroup b
+    // covers item order[b] = tile * nseg + segment; norder = tiles * nseg
+    // must equal the launch's grid, or the launch fails
+    const int* order = nullptr;
+    int norder = 0;
 };
 
 // A stride coprime to `tiles`, near tiles / golden ratio: visiting tiles in
@@ -83,6 +88,19 @@ hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const Sweep
 hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s);
 // the R (focus tiles per wave) each kernel uses for a (nta, ntb) instance
 int contract_r(int nta, int ntb, int precision_mixed);
+
+// Work order of a pruned single-job sweep whose focus points are the data
+// points in order (gwm_plan.hip): counts[tile * nseg + seg] = data chunks of
+// the segment within the zero-weight distance of the tile's box, then order =
+// the items by decreasing count within each XCD group.  tile_pts = focus
+// points per tile (64 R), tiles = ceil(n / tile_pts); counts and order hold
+// tiles * nseg ints.  nseg <= kPlanMaxSeg and seg_len <= 64 kPlanMaxCount.
+constexpr int kPlanMaxSeg = 2048;
+constexpr int kPlanMaxCount = 128;
+constexpr int kPlanGroups = 8;
+hipError_t launch_sweep_plan(int dist, const Points& P, const double* boxes, const double* bw_pt, double bw,
+                             const double* chunk_bw, int kernel, int mixed, int tile_pts, int seg_len, int nseg,
+                             int tiles, int* counts, int* order, hipStream_t s);
 
 // Largest value of each 64-entry chunk of v[0..n) (chunk bandwidths for the
 // pruned qDiag sweeps).

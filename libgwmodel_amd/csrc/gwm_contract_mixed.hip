@@ -195,9 +195,15 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
         seg = it[2];
     } else {
         args = L.job0;
-        seg = blockIdx.x % L.job0.nseg;
-        const int ntiles = gridDim.x / L.job0.nseg;
-        tile = (int)(((long long)(blockIdx.x / L.job0.nseg) * L.tile_stride) % ntiles);
+        if (L.order) {
+            const int q = L.order[blockIdx.x];
+            seg = q % L.job0.nseg;
+            tile = q / L.job0.nseg;
+        } else {
+            seg = blockIdx.x % L.job0.nseg;
+            const int ntiles = gridDim.x / L.job0.nseg;
+            tile = (int)(((long long)(blockIdx.x / L.job0.nseg) * L.tile_stride) % ntiles);
+        }
     }
     // a launch groups jobs of exactly the instance's tile counts
     constexpr int nta = NTA, ntb = NTB;
@@ -431,6 +437,7 @@ hipError_t launch_mixed_rr(const SweepLaunch& a, hipStream_t s) {
         grid = tiles * a.job0.nseg;
     }
     if (grid <= 0) return hipSuccess;
+    if (a.order && (a.items || a.norder != grid)) return hipErrorInvalidValue;
     SweepLaunch b = a;
     if (!a.items) b.tile_stride = a.linear_order ? 1 : tile_stride(grid / a.job0.nseg);
     hipLaunchKernelGGL((contract_mixed_kernel<NTA, NTB, DIST, MODE, R>), dim3(grid), dim3(256), 0, s, b);

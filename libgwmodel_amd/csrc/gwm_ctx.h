@@ -43,6 +43,11 @@ struct gwm_hip_ctx {
     bool prune = true;             // skip data chunks whose weights are all exactly zero
     double* boxes = nullptr;       // bounding boxes of the 64-point chunks (sorted order)
     double* chunk_bw = nullptr;    // largest per-point bandwidth of each chunk (qDiag sweeps)
+    // work order of large pruned sweeps (launch_sweep_plan): per-item chunk
+    // counts and the item order, plan_items ints each
+    int* plan_counts = nullptr;
+    int* plan_order = nullptr;
+    size_t plan_items = 0;
     // adaptive bandwidth vectors of recently used bandwidths (least recently
     // used replaced): searches revisit bandwidths (FP64 confirmations, the
     // fixed first candidates of every multiscale search)

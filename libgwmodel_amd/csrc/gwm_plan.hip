@@ -1,0 +1,128 @@
+// Work-ordered launch of a pruned single-problem sweep.
+//
+// A sweep workgroup covers one focus tile and one data segment.  With pruning
+// and a narrow bandwidth most (tile, segment) items skip every chunk, and the
+// rest differ in work by up to the segment's chunk count; in plain launch
+// order the last busy workgroups start late and the sweep ends on a tail.  The
+// plan counts, for every item, the data chunks whose box is within the
+// zero-weight distance of the tile's box (the test the sweep applies), and
+// orders the items longest first within each XCD group (items of segments
+// s = g mod G go to workgroups b = g mod G, as the plain order does).  The
+// order only schedules: every item writes its own partial sums, so results
+// are the same bits in any order.
+#include <hip/hip_runtime.h>
+
+#include "gwm_contract.h"
+#include "gwm_device.h"
+
+namespace gwm {
+namespace {
+
+// Counts of one tile against every data chunk; one workgroup per tile.
+template <int DIST>
+__global__ __launch_bounds__(256) void plan_count_kernel(Points P, const double* boxes, const double* bw_pt, double bw,
+                                                         const double* chunk_bw, int kernel, int mixed, int tile_pts,
+                                                         int seg_len, int nseg, int* counts) {
+    constexpr int BD = DIST == kEuclidean ? 2 : 3;
+    __shared__ int cnt[kPlanMaxSeg];
+    __shared__ double red[256];
+    __shared__ double tbox[2 * BD];
+    const int tile = blockIdx.x, t = threadIdx.x;
+    const int n = P.n, nch = (n + 63) / 64;
+    GWM_GLOBAL const double* B = gptr(boxes);
+    for (int s = t; s < nseg; s += 256) cnt[s] = 0;
+    // largest focus bandwidth of the tile
+    const int f0 = tile * tile_pts, f1 = min(n, f0 + tile_pts);
+    double b = 0.0;
+    if (bw_pt)
+        for (int f = f0 + t; f < f1; f += 256) b = fmax(b, gptr(bw_pt)[f]);
+    red[t] = b;
+    // the tile's box: union of the boxes of the chunks holding its focus points
+    if (t == 0) {
+        const int c0 = f0 / 64, c1 = min(nch, (f1 + 63) / 64);
+        for (int d = 0; d < BD; ++d) {
+            double lo = INFINITY, hi = -INFINITY;
+            for (int ch = c0; ch < c1; ++ch) {
+                lo = fmin(lo, B[(size_t)ch * 2 * BD + 2 * d]);
+                hi = fmax(hi, B[(size_t)ch * 2 * BD + 2 * d + 1]);
+            }
+            tbox[2 * d] = lo;
+            tbox[2 * d + 1] = hi;
+        }
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] = fmax(red[t], red[t + w]);
+        __syncthreads();
+    }
+    const double fb = bw_pt ? red[0] : bw;
+    double lo[BD], hi[BD];
+#pragma unroll
+    for (int d = 0; d < BD; ++d) {
+        lo[d] = tbox[2 * d];
+        hi[d] = tbox[2 * d + 1];
+    }
+    const double thr = prune_threshold<DIST>(zero_weight_distance(kernel, fb, mixed != 0));
+    for (int ch = t; ch < nch; ch += 256) {
+        const double th =
+            chunk_bw ? prune_threshold<DIST>(zero_weight_distance(kernel, gptr(chunk_bw)[ch], mixed != 0)) : thr;
+        if (!(box_gap2<DIST>(lo, hi, B + (size_t)ch * 2 * BD) > th)) atomicAdd(&cnt[(ch * 64) / seg_len], 1);
+    }
+    __syncthreads();
+    for (int s = t; s < nseg; s += 256) counts[(size_t)tile * nseg + s] = cnt[s];
+}
+
+// Counting sort of the items by decreasing count within each of the G groups
+// (group of item q: (q mod nseg) mod G; every group holds nitems / G items
+// because G divides nseg): order[G i + g] = the i-th item of group g.  One
+// workgroup; the order within equal counts is that of the atomics (it does
+// not matter).
+__global__ __launch_bounds__(1024) void plan_order_kernel(const int* counts, int nitems, int nseg, int maxc, int G,
+                                                          int* order) {
+    __shared__ int hist[kPlanGroups][kPlanMaxCount + 1];
+    const int t = threadIdx.x;
+    for (int e = t; e < kPlanGroups * (kPlanMaxCount + 1); e += 1024) (&hist[0][0])[e] = 0;
+    __syncthreads();
+    for (int q = t; q < nitems; q += 1024) {
+        const int c = min(max(counts[q], 0), maxc);
+        atomicAdd(&hist[(q % nseg) % G][c], 1);
+    }
+    __syncthreads();
+    if (t < G) {
+        int run = 0;
+        for (int c = maxc; c >= 0; --c) {
+            const int h = hist[t][c];
+            hist[t][c] = run;
+            run += h;
+        }
+    }
+    __syncthreads();
+    for (int q = t; q < nitems; q += 1024) {
+        const int c = min(max(counts[q], 0), maxc);
+        const int g = (q % nseg) % G;
+        const int pos = atomicAdd(&hist[g][c], 1);
+        order[(size_t)G * pos + g] = q;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_sweep_plan(int dist, const Points& P, const double* boxes, const double* bw_pt, double bw,
+                             const double* chunk_bw, int kernel, int mixed, int tile_pts, int seg_len, int nseg,
+                             int tiles, int* counts, int* order, hipStream_t s) {
+    const int maxc = seg_len / 64;
+    if (nseg < 1 || nseg > kPlanMaxSeg || maxc > kPlanMaxCount || tiles < 1 || tile_pts % 64 != 0)
+        return hipErrorInvalidValue;
+    if ((long long)tiles * tile_pts < P.n || (long long)(tiles - 1) * tile_pts >= P.n) return hipErrorInvalidValue;
+    if (dist == kEuclidean)
+        hipLaunchKernelGGL(plan_count_kernel<kEuclidean>, dim3(tiles), dim3(256), 0, s, P, boxes, bw_pt, bw, chunk_bw,
+                           kernel, mixed, tile_pts, seg_len, nseg, counts);
+    else
+        hipLaunchKernelGGL(plan_count_kernel<kGreatCircle>, dim3(tiles), dim3(256), 0, s, P, boxes, bw_pt, bw,
+                           chunk_bw, kernel, mixed, tile_pts, seg_len, nseg, counts);
+    const int G = nseg % kPlanGroups == 0 ? kPlanGroups : 1;
+    hipLaunchKernelGGL(plan_order_kernel, dim3(1), dim3(1024), 0, s, counts, tiles * nseg, nseg, maxc, G, order);
+    return hipGetLastError();
+}
+
+}  // namespace gwm

@@ -38,3 +38,56 @@ def test_shapes_are_bit_identical(prec):
     lines = capi.autotune_report()
     say("\n".join(lines))
     assert any("euclidean cv" in l and "n<2^12" in l for l in lines)
+
+
+def _fit_arrays(fit):
+    return [fit[key] for key in ("betas", "betasSE_raw", "shat", "qDiag")]
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.MIXED])
+@pytest.mark.parametrize("geographic", [False, True])
+def test_work_order_is_bit_identical(prec, geographic):
+    """Large pruned sweeps (here 16 segments x 157 tiles) run their (tile,
+    segment) items longest first when launch shaping is on; every item
+    writes its own partial sums, so criteria and fits (hat-mode and qDiag
+    sweeps included, with the per-chunk bandwidths of an adaptive qDiag) are
+    the same bits as in plain launch order."""
+    n = 20000
+    if geographic:
+        d = synth.sphere_box(n=n)
+        dist, fixed_bw, ad_bw = capi.GREAT_CIRCLE, 90.0, 300.0
+    else:
+        d = synth.uniform_square(n=n)
+        dist, fixed_bw, ad_bw = capi.EUCLIDEAN, 2.5, 300.0
+    cases = [(capi.GAUSSIAN, False, fixed_bw), (capi.BISQUARE, True, ad_bw)]
+    out = {}
+    with capi.Context(0) as ctx:
+        ctx.set_data(d["coords"], d["X"], d["y"], dist)
+        for on in (False, True, True):
+            ctx.set_autotune(on)
+            say("case: work order %s, geographic %s, precision %d" % (on, geographic, prec))
+            res = []
+            for kern, ad, bw in cases:
+                res.append(ctx.criterion(kern, ad, bw, capi.CRIT_CV, prec))
+                res.append(ctx.criterion(kern, ad, bw, capi.CRIT_AIC, prec))
+                res.extend(_fit_arrays(ctx.fit(kern, ad, bw, prec)))
+            out.setdefault(on, []).append(res)
+    ref = out[False][0]
+    for got in out[True]:
+        assert len(got) == len(ref)
+        for a, b in zip(got, ref):
+            assert np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+
+
+def test_work_order_single_group():
+    """n = 150000: 19 data segments, not a multiple of the 8 XCD groups, so
+    the order is one longest-first list."""
+    d = synth.uniform_square(n=150000)
+    with capi.Context(0) as ctx:
+        ctx.set_data(d["coords"], d["X"], d["y"])
+        res = {}
+        for on in (False, True):
+            ctx.set_autotune(on)
+            say("case: work order %s, n = 150000" % on)
+            res[on] = [ctx.criterion(capi.GAUSSIAN, False, 1.5, capi.CRIT_AIC, prec) for prec in (capi.FP64, capi.MIXED)]
+        assert res[True] == res[False]
