@@ -204,10 +204,10 @@ int gwm_hip_set_pruning(gwm_hip_ctx* ctx, int enabled);
 int gwm_hip_last_condition(gwm_hip_ctx* ctx, double* cond, long long* refined);
 /* Absolute error bound that the certified golden section assumes for the
  * last mixed-precision criterion value of this context (0 after an FP64
- * evaluation): the largest of tol |f|, the residual rounding term
- * (20 * 2^-22 * kappa / sqrt(n), times f for CV and n for AICc, kappa the
- * indicator maximum capped at the recomputation limit) and, for AICc,
- * tol * S * tr(S) with S = n (2n - 2) / (n - 2 - tr(S))^2. */
+ * evaluation): the largest of tol |f| (3 tol |f| for CV), the rounding term
+ * r = 8 * 2^-22 * kappa * sqrt(sum_i (res_i yhat_i)^2) (CV: r; AICc: n r /
+ * RSS; kappa the indicator maximum capped at the recomputation limit) and,
+ * for AICc, tol * S * tr(S) with S = n (2n - 2) / (n - 2 - tr(S))^2. */
 int gwm_hip_last_error_bound(gwm_hip_ctx* ctx, double* bound);
 
 /* Online autotuning of the single-problem sweeps (no counterpart in the
@@ -215,8 +215,11 @@ int gwm_hip_last_error_bound(gwm_hip_ctx* ctx, double* bound);
  * column tiles, n to a power of two) the first sweeps rotate through the
  * launch shapes that leave results bit-identical -- one or two focus tiles
  * per wave, tiles visited in order or by a coprime stride -- each timed twice
- * with HIP events; the fastest is used from then on, process-wide.  Batched
- * server launches keep the default shape. */
+ * with HIP events; the fastest is used from then on, process-wide.  Larger
+ * pruned sweeps keep the default shape and run their (tile, segment)
+ * workgroups longest first (a counting pass over the chunk boxes); disabling
+ * autotuning also restores the plain order.  Batched server launches keep
+ * the default shape and order. */
 int gwm_hip_set_autotune(gwm_hip_ctx* ctx, int enabled);
 /* The tuned classes, one text line each (shape chosen, best time of every
  * shape in ms), NUL-terminated and truncated to cap bytes. */
