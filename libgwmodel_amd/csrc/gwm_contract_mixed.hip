@@ -118,15 +118,12 @@ __device__ __forceinline__ float weight_from_d2(int kernel, float d2, const BwCo
 // with sp = sin lat_i + sin lat_j, sm = sin lat_j - sin lat_i (both formed in
 // FP64 by the caller: sm^2/S cancels for nearby points); 1/C and 1/S come
 // from one reciprocal of S*C.  No square root, two transcendentals per pair
-// including the kernel's exponential.  Larger S falls back to gcdist_tail_f32.
-__device__ __forceinline__ float gc_d2_f32(float S, float sp, float sm, float slf, float slj) {
+// including the kernel's exponential.  Larger S takes gcdist_tail_f32 (the
+// producer runs it in a second, wave-uniform pass).  Branch free: coincident
+// points (S <= 1e-30) give 0.
+__device__ __forceinline__ float gc_d2_near_f32(float S, float sp, float sm) {
     const float a = 6378.137f;
     const float f8 = (float)(1.0 / 298.257223563 / 8.0);
-    if (S > 0.25f) {
-        float d = gcdist_tail_f32(S, slf, slj);
-        return d * d;
-    }
-    if (!(S > 1e-30f)) return 0.0f;
     const float p = fmaf(fmaf(fmaf(fmaf(4.2163199048e-2f, S, 2.4181311049e-2f), S, 4.5470025998e-2f), S,
                               7.4953002686e-2f), S, 1.6666752422e-1f);
     const float q = fmaf(S, p, 1.0f);
@@ -139,7 +136,8 @@ __device__ __forceinline__ float gc_d2_f32(float S, float sp, float sm, float sl
     const float rsc = __builtin_amdgcn_rcpf(S * C);
     const float corr = f8 * ((3.0f * R - 1.0f) * (S * rsc) * (sp * sp) - (3.0f * R + 1.0f) * (C * rsc) * (sm * sm));
     const float g = 1.0f + corr;
-    return (4.0f * a * a) * th2 * (g * g);
+    const float d2 = (4.0f * a * a) * th2 * (g * g);
+    return S > 1e-30f ? d2 : 0.0f;
 }
 
 // Kernel weights of 8 squared distances (the kernel switch is taken once).
@@ -337,6 +335,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                 v8h ah, al, a2h, a2l;
                 const int rb = 32 * s + 8 * kq;
                 float d2[8], w[8];
+                unsigned far = 0;   // great circle: pairs beyond 60 degrees of arc (S > 0.25)
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int rr = rb + e;
@@ -346,8 +345,21 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                     } else {
                         double ex = sh.c.hx[rr] - fc[r].a, ey = sh.c.hy[rr] - fc[r].b, ez = sh.c.hz[rr] - fc[r].c;
                         const double slj = sh.c.sl[rr];
-                        d2[e] = gc_d2_f32((float)(ex * ex + ey * ey + ez * ez), (float)(fc[r].sl + slj),
-                                          (float)(slj - fc[r].sl), (float)fc[r].sl, (float)slj);
+                        const float S = (float)(ex * ex + ey * ey + ez * ez);
+                        const float dn = gc_d2_near_f32(S, (float)(fc[r].sl + slj), (float)(slj - fc[r].sl));
+                        // a far pair keeps S for the second pass
+                        d2[e] = S > 0.25f ? S : dn;
+                        far |= (S > 0.25f ? 1u : 0u) << e;
+                    }
+                }
+                if constexpr (DIST != kEuclidean) {
+                    if (__any(far != 0u)) {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e)
+                            if (far & (1u << e)) {
+                                const float d = gcdist_tail_f32(d2[e], (float)fc[r].sl, (float)sh.c.sl[rb + e]);
+                                d2[e] = d * d;
+                            }
                     }
                 }
                 if constexpr (MODE == kModeQDiag) {
