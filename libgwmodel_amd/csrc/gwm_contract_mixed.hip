@@ -368,12 +368,14 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                 } else {
                     weights8_from_d2(args.kernel, d2, fbw[r], w);
                 }
+                // Rows past the segment end occur only in the last segment,
+                // at j >= n, where every Z table (design, G, single column)
+                // and its FP16 planes are zero padded: their products are
+                // exact zeros without masking the (finite) weights.
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    const int j = j0 + rb + e;
                     float we = w[e];
-                    if (j >= jend) we = 0.0f;
-                    if (MODE == kModeCV && j == fidx[r]) we = 0.0f;
+                    if (MODE == kModeCV && j0 + rb + e == fidx[r]) we = 0.0f;
                     const float wa = MODE == kModeQDiag ? we * we : we;
                     _Float16 h, l;
                     split16(wa, h, l);
