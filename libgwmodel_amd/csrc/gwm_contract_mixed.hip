@@ -122,8 +122,8 @@ __device__ __forceinline__ float weight_from_d2(int kernel, float d2, const BwCo
 // producer runs it in a second, wave-uniform pass).  Branch free: coincident
 // points (S <= 1e-30) give 0.
 __device__ __forceinline__ float gc_d2_near_f32(float S, float sp, float sm) {
-    const float a = 6378.137f;
-    const float f8 = (float)(1.0 / 298.257223563 / 8.0);
+    const float a2 = (float)(2.0 * 6378.137);                       // 2a
+    const float c8 = (float)(2.0 * 6378.137 / 298.257223563 / 8.0);  // 2a f / 8
     const float p = fmaf(fmaf(fmaf(fmaf(4.2163199048e-2f, S, 2.4181311049e-2f), S, 4.5470025998e-2f), S,
                               7.4953002686e-2f), S, 1.6666752422e-1f);
     const float q = fmaf(S, p, 1.0f);
@@ -134,9 +134,11 @@ __device__ __forceinline__ float gc_d2_near_f32(float S, float sp, float sm) {
                               -1.6666666666666667e-1f), u, 1.0f);
     const float C = 1.0f - S;
     const float rsc = __builtin_amdgcn_rcpf(S * C);
-    const float corr = f8 * ((3.0f * R - 1.0f) * (S * rsc) * (sp * sp) - (3.0f * R + 1.0f) * (C * rsc) * (sm * sm));
-    const float g = 1.0f + corr;
-    const float d2 = (4.0f * a * a) * th2 * (g * g);
+    // (3R - 1) sp^2 / C - (3R + 1) sm^2 / S = rsc [(3R - 1) S sp^2 - (3R + 1) C sm^2]
+    const float R3 = 3.0f * R;
+    const float inner = fmaf(fmaf(R3, S, -S), sp * sp, -(fmaf(R3, C, C) * (sm * sm)));
+    const float g2a = fmaf(c8 * rsc, inner, a2);   // 2a (1 + f/8 rsc inner)
+    const float d2 = th2 * (g2a * g2a);
     return S > 1e-30f ? d2 : 0.0f;
 }
 
