@@ -128,10 +128,10 @@ __device__ __forceinline__ float gc_d2_near_f32(float S, float sp, float sm) {
                               7.4953002686e-2f), S, 1.6666752422e-1f);
     const float q = fmaf(S, p, 1.0f);
     const float th2 = S * q * q;
-    const float u = 4.0f * th2;
-    const float R = fmaf(fmaf(fmaf(fmaf(fmaf(u, -2.5052108385441718e-8f, 2.7557319223985893e-6f), u,
-                                        -1.9841269841269841e-4f), u, 8.3333333333333333e-3f), u,
-                              -1.6666666666666667e-1f), u, 1.0f);
+    // R = sinc(2 theta) = sum_m (-1)^m 4^m theta^(2m) / (2m+1)!, in th2
+    const float R = fmaf(fmaf(fmaf(fmaf(fmaf(th2, -2.5653358986692319e-5f, 7.0546737213403880e-4f), th2,
+                                        -1.2698412698412698e-2f), th2, 1.3333333333333333e-1f), th2,
+                              -6.6666666666666667e-1f), th2, 1.0f);
     const float C = 1.0f - S;
     const float rsc = __builtin_amdgcn_rcpf(S * C);
     // (3R - 1) sp^2 / C - (3R + 1) sm^2 / S = rsc [(3R - 1) S sp^2 - (3R + 1) C sm^2]
