@@ -428,7 +428,7 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
         if (c->prune && chosen == 0 && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg &&
             seg_len <= 64 * kPlanMaxCount) {
             if ((rc = ensure_plan(c, (size_t)tiles * nseg))) return rc;
-            GWM_HIP_CHECK(c, launch_sweep_plan(c->dist, c->points(), c->boxes, mode == kModeQDiag ? nullptr : bw_pt,
+            GWM_HIP_CHECK(c, launch_sweep_plan(c->dist, c->points(), c->n, c->boxes, mode == kModeQDiag ? nullptr : bw_pt,
                                                bw, j.chunk_bw, kernel, mixed ? 1 : 0, 64 * r, seg_len, nseg, tiles,
                                                c->plan_counts, c->plan_order, c->stream));
             L.order = c->plan_order;
@@ -1096,6 +1096,21 @@ int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adapt
     L.items = nullptr;
     L.nitems = 0;
     GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
+    // large pruned sweeps: items longest first, as for the fits (the
+    // locations are in their own spatial order)
+    {
+        const int r = contract_r(c->nta, 0, mixed ? 1 : 0);
+        const int tiles = (m + 64 * r - 1) / (64 * r);
+        if (c->prune && c->autotune && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg &&
+            seg_len <= 64 * kPlanMaxCount) {
+            if ((rc = ensure_plan(c, (size_t)tiles * nseg))) return rc;
+            GWM_HIP_CHECK(c, launch_sweep_plan(c->dist, Q, c->n, c->boxes, bw_pt, bw, nullptr, kernel, mixed ? 1 : 0,
+                                               64 * r, seg_len, nseg, tiles, c->plan_counts, c->plan_order,
+                                               c->stream));
+            L.order = c->plan_order;
+            L.norder = tiles * nseg;
+        }
+    }
     timer_begin(c);
     if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(c->dist, kModePlain, c->nta, 0, L, c->stream));
     else GWM_HIP_CHECK(c, launch_contract_f64(c->dist, kModePlain, c->nta, 0, L, c->stream));

@@ -89,16 +89,17 @@ hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const Swe
 // the R (focus tiles per wave) each kernel uses for a (nta, ntb) instance
 int contract_r(int nta, int ntb, int precision_mixed);
 
-// Work order of a pruned single-job sweep whose focus points are the data
-// points in order (gwm_plan.hip): counts[tile * nseg + seg] = data chunks of
-// the segment within the zero-weight distance of the tile's box, then order =
-// the items by decreasing count within each XCD group.  tile_pts = focus
-// points per tile (64 R), tiles = ceil(n / tile_pts); counts and order hold
+// Work order of a pruned single-job sweep whose focus points are the points
+// of F in order (the data table itself, or the prediction locations) against
+// n data points (gwm_plan.hip): counts[tile * nseg + seg] = data chunks of the
+// segment within the zero-weight distance of the tile's box, then order = the
+// items by decreasing count within each XCD group.  tile_pts = focus points
+// per tile (64 R), tiles = ceil(F.n / tile_pts); counts and order hold
 // tiles * nseg ints.  nseg <= kPlanMaxSeg and seg_len <= 64 kPlanMaxCount.
 constexpr int kPlanMaxSeg = 2048;
 constexpr int kPlanMaxCount = 128;
 constexpr int kPlanGroups = 8;
-hipError_t launch_sweep_plan(int dist, const Points& P, const double* boxes, const double* bw_pt, double bw,
+hipError_t launch_sweep_plan(int dist, const Points& F, int n, const double* boxes, const double* bw_pt, double bw,
                              const double* chunk_bw, int kernel, int mixed, int tile_pts, int seg_len, int nseg,
                              int tiles, int* counts, int* order, hipStream_t s);
 

@@ -18,50 +18,37 @@
 namespace gwm {
 namespace {
 
-// Counts of one tile against every data chunk; one workgroup per tile.
+// Counts of one focus tile against every data chunk; one workgroup per tile.
+// The tile's box and largest bandwidth are those the sweep forms from the
+// same focus coordinates (workgroup_box).
 template <int DIST>
-__global__ __launch_bounds__(256) void plan_count_kernel(Points P, const double* boxes, const double* bw_pt, double bw,
-                                                         const double* chunk_bw, int kernel, int mixed, int tile_pts,
-                                                         int seg_len, int nseg, int* counts) {
+__global__ __launch_bounds__(256) void plan_count_kernel(Points F, int n, const double* boxes, const double* bw_pt,
+                                                         double bw, const double* chunk_bw, int kernel, int mixed,
+                                                         int tile_pts, int seg_len, int nseg, int* counts) {
     constexpr int BD = DIST == kEuclidean ? 2 : 3;
     __shared__ int cnt[kPlanMaxSeg];
-    __shared__ double red[256];
-    __shared__ double tbox[2 * BD];
     const int tile = blockIdx.x, t = threadIdx.x;
-    const int n = P.n, nch = (n + 63) / 64;
+    const int nch = (n + 63) / 64;
     GWM_GLOBAL const double* B = gptr(boxes);
+    const GPoints G = global_points(F);
     for (int s = t; s < nseg; s += 256) cnt[s] = 0;
-    // largest focus bandwidth of the tile
-    const int f0 = tile * tile_pts, f1 = min(n, f0 + tile_pts);
-    double b = 0.0;
-    if (bw_pt)
-        for (int f = f0 + t; f < f1; f += 256) b = fmax(b, gptr(bw_pt)[f]);
-    red[t] = b;
-    // the tile's box: union of the boxes of the chunks holding its focus points
-    if (t == 0) {
-        const int c0 = f0 / 64, c1 = min(nch, (f1 + 63) / 64);
-        for (int d = 0; d < BD; ++d) {
-            double lo = INFINITY, hi = -INFINITY;
-            for (int ch = c0; ch < c1; ++ch) {
-                lo = fmin(lo, B[(size_t)ch * 2 * BD + 2 * d]);
-                hi = fmax(hi, B[(size_t)ch * 2 * BD + 2 * d + 1]);
-            }
-            tbox[2 * d] = lo;
-            tbox[2 * d + 1] = hi;
-        }
-    }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) red[t] = fmax(red[t], red[t + w]);
-        __syncthreads();
-    }
-    const double fb = bw_pt ? red[0] : bw;
-    double lo[BD], hi[BD];
+    double lo[BD], hi[BD], b = 0.0;
 #pragma unroll
     for (int d = 0; d < BD; ++d) {
-        lo[d] = tbox[2 * d];
-        hi[d] = tbox[2 * d + 1];
+        lo[d] = INFINITY;
+        hi[d] = -INFINITY;
     }
+    const int f = tile * tile_pts + t;
+    if (t < tile_pts && f < F.n) {
+        const double x[3] = {DIST == kEuclidean ? G.px[f] : G.hx[f], DIST == kEuclidean ? G.py[f] : G.hy[f],
+                             DIST == kEuclidean ? 0.0 : G.hz[f]};
+#pragma unroll
+        for (int d = 0; d < BD; ++d) lo[d] = hi[d] = x[d];
+        b = bw_pt ? gptr(bw_pt)[f] : bw;
+    }
+    double fb;
+    workgroup_box<BD>(lo, hi, b, &fb);
+    if (!bw_pt) fb = bw;
     const double thr = prune_threshold<DIST>(zero_weight_distance(kernel, fb, mixed != 0));
     for (int ch = t; ch < nch; ch += 256) {
         const double th =
@@ -107,18 +94,19 @@ __global__ __launch_bounds__(1024) void plan_order_kernel(const int* counts, int
 
 }  // namespace
 
-hipError_t launch_sweep_plan(int dist, const Points& P, const double* boxes, const double* bw_pt, double bw,
+hipError_t launch_sweep_plan(int dist, const Points& F, int n, const double* boxes, const double* bw_pt, double bw,
                              const double* chunk_bw, int kernel, int mixed, int tile_pts, int seg_len, int nseg,
                              int tiles, int* counts, int* order, hipStream_t s) {
     const int maxc = seg_len / 64;
-    if (nseg < 1 || nseg > kPlanMaxSeg || maxc > kPlanMaxCount || tiles < 1 || tile_pts % 64 != 0)
+    if (nseg < 1 || nseg > kPlanMaxSeg || maxc > kPlanMaxCount || tiles < 1 || tile_pts % 64 != 0 ||
+        tile_pts > 256 || (long long)(nseg - 1) * seg_len >= n || (long long)nseg * seg_len < n)
         return hipErrorInvalidValue;
-    if ((long long)tiles * tile_pts < P.n || (long long)(tiles - 1) * tile_pts >= P.n) return hipErrorInvalidValue;
+    if ((long long)tiles * tile_pts < F.n || (long long)(tiles - 1) * tile_pts >= F.n) return hipErrorInvalidValue;
     if (dist == kEuclidean)
-        hipLaunchKernelGGL(plan_count_kernel<kEuclidean>, dim3(tiles), dim3(256), 0, s, P, boxes, bw_pt, bw, chunk_bw,
-                           kernel, mixed, tile_pts, seg_len, nseg, counts);
+        hipLaunchKernelGGL(plan_count_kernel<kEuclidean>, dim3(tiles), dim3(256), 0, s, F, n, boxes, bw_pt, bw,
+                           chunk_bw, kernel, mixed, tile_pts, seg_len, nseg, counts);
     else
-        hipLaunchKernelGGL(plan_count_kernel<kGreatCircle>, dim3(tiles), dim3(256), 0, s, P, boxes, bw_pt, bw,
+        hipLaunchKernelGGL(plan_count_kernel<kGreatCircle>, dim3(tiles), dim3(256), 0, s, F, n, boxes, bw_pt, bw,
                            chunk_bw, kernel, mixed, tile_pts, seg_len, nseg, counts);
     const int G = nseg % kPlanGroups == 0 ? kPlanGroups : 1;
     hipLaunchKernelGGL(plan_order_kernel, dim3(1), dim3(1024), 0, s, counts, tiles * nseg, nseg, maxc, G, order);

@@ -91,3 +91,28 @@ def test_work_order_single_group():
             say("case: work order %s, n = 150000" % on)
             res[on] = [ctx.criterion(capi.GAUSSIAN, False, 1.5, capi.CRIT_AIC, prec) for prec in (capi.FP64, capi.MIXED)]
         assert res[True] == res[False]
+
+
+@pytest.mark.parametrize("geographic", [False, True])
+def test_predict_work_order_is_bit_identical(geographic):
+    """Prediction sweeps over locations of their own (their own spatial
+    order and tile boxes) run longest first too, with the same bits."""
+    n, m = 20000, 15000
+    if geographic:
+        d = synth.sphere_box(n=n)
+        dist, cases = capi.GREAT_CIRCLE, [(capi.GAUSSIAN, False, 90.0), (capi.BISQUARE, True, 300.0)]
+    else:
+        d = synth.uniform_square(n=n)
+        dist, cases = capi.EUCLIDEAN, [(capi.GAUSSIAN, False, 2.5), (capi.BISQUARE, True, 300.0)]
+    rng = np.random.default_rng(11)
+    lo, hi = d["coords"].min(0), d["coords"].max(0)
+    loc = rng.uniform(lo, hi, (m, 2))
+    with capi.Context(0) as ctx:
+        ctx.set_data(d["coords"], d["X"], d["y"], dist)
+        res = {}
+        for on in (False, True):
+            ctx.set_autotune(on)
+            say("case: predict work order %s, geographic %s" % (on, geographic))
+            res[on] = [ctx.predict(loc, kern, ad, bw, prec) for kern, ad, bw in cases for prec in (capi.FP64, capi.MIXED)]
+        for a, b in zip(res[True], res[False]):
+            assert np.array_equal(a, b, equal_nan=True)
