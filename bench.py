@@ -115,9 +115,11 @@ def run_mgwr(capi, synth, n):
     out = {"n": n, "k": k}
     res = {}
     with capi.Context(0) as ctx:
-        ctx.set_data(d["coords"], d["X"], d["y"])
         for name, prec in (("mixed", capi.MIXED), ("fp64", capi.FP64)):
             log("mgwr %s n=%d" % (name, n))
+            # fresh problem state for each run: no adaptive-bandwidth vectors
+            # cached by the other precision's run
+            ctx.set_data(d["coords"], d["X"], d["y"])
             ctx.reset_timers()
             t0 = time.perf_counter()
             r = ctx.mgwr_run([capi.BISQUARE] * k, [1] * k, [0.0] * k, [0] * k, [capi.CRIT_CV] * k,

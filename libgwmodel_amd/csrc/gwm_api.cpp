@@ -264,7 +264,10 @@ int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw
             *bw_pt = e.buf;
             return GWM_OK;
         }
-    const size_t cap = std::min<size_t>(64, std::max<size_t>(4, (256u << 20) / ((size_t)c->n * sizeof(double))));
+    // Up to 1 GiB and 512 vectors: a multiscale backfit revisits about 15
+    // bandwidths per variable in every iteration, in the same cyclic order,
+    // which a cache smaller than that working set misses every time.
+    const size_t cap = std::min<size_t>(512, std::max<size_t>(4, (size_t(1) << 30) / ((size_t)c->n * sizeof(double))));
     gwm_hip_ctx::BwEntry* slot = nullptr;
     if (c->bw_cache.size() < cap) {
         double* buf = nullptr;
@@ -784,9 +787,12 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
     auto bound = [&](double fx, bool exact, double fb) {
         return exact || fx == DBL_MAX ? 0.0 : std::max(fb, tol_rel * std::fabs(fx));
     };
-    // upgrade both points to FP64 when a decision on (f1, f2) is not certain
+    // upgrade both points to FP64 when a decision on (f1, f2) is not certain;
+    // coincident points (rounded adaptive candidates) hold one evaluation, so
+    // f2 - f1 = 0 in either precision and every decision is the FP64 one
     auto certify_pair = [&]() -> int {
         if (e1 && e2) return GWM_OK;
+        if (x1 == x2) return GWM_OK;
         double m = bound(f1, e1, b1) + bound(f2, e2, b2);
         double diff = std::fabs(f2 - f1);
         bool sure = diff > m && std::fabs(diff - eps) > m;
@@ -807,7 +813,13 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
     };
     int rc;
     if ((rc = eval(x1, e1, &f1, &b1))) return rc;
-    if ((rc = eval(x2, e2, &f2, &b2))) return rc;
+    if (x2 == x1) {
+        f2 = f1;
+        e2 = e1;
+        b2 = b1;
+    } else if ((rc = eval(x2, e2, &f2, &b2))) {
+        return rc;
+    }
     if (f1 == DBL_MAX && f2 == DBL_MAX) return GWM_ERR_INVALID_INITIAL;
     record(x1, f1);
     record(x2, f2);
@@ -824,9 +836,15 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
             f2 = f1;
             e2 = e1;
             b2 = b1;
-            e1 = !certify || sticky;
-            if ((rc = eval(x1, e1, &f1, &b1))) return rc;
-            record(x1, f1);
+            if (x1 == x2) {   // the same candidate: its evaluation, not a second one
+                f1 = f2;
+                e1 = e2;
+                b1 = b2;
+            } else {
+                e1 = !certify || sticky;
+                if ((rc = eval(x1, e1, &f1, &b1))) return rc;
+                record(x1, f1);
+            }
         } else {
             xU = x1;
             x1 = x2;
@@ -834,9 +852,15 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
             f1 = f2;
             e1 = e2;
             b1 = b2;
-            e2 = !certify || sticky;
-            if ((rc = eval(x2, e2, &f2, &b2))) return rc;
-            record(x2, f2);
+            if (x2 == x1) {
+                f2 = f1;
+                e2 = e1;
+                b2 = b1;
+            } else {
+                e2 = !certify || sticky;
+                if ((rc = eval(x2, e2, &f2, &b2))) return rc;
+                record(x2, f2);
+            }
         }
         iter = iter + 1;
         if ((rc = certify_pair())) return rc;
