@@ -160,7 +160,9 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
             }
         }
         workgroup_box<BD>(flo, fhi, fb, &fb);
+        // qDiag: the streamed points' bandwidths (per chunk when adaptive)
         if (MODE != kModeQDiag) thr = prune_threshold<DIST>(zero_weight_distance(job.kernel, fb, false));
+        else if (!job.chunk_bw) thr = prune_threshold<DIST>(zero_weight_distance(job.kernel, job.bw, false));
     }
 
     v4d accA[R][NTA];
@@ -182,9 +184,8 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
         if (prune) {
             const int ch = j0 / kChunk;
             double t = thr;
-            if (MODE == kModeQDiag)
-                t = prune_threshold<DIST>(
-                    zero_weight_distance(job.kernel, job.chunk_bw ? gptr(job.chunk_bw)[ch] : job.bw, false));
+            if (MODE == kModeQDiag && job.chunk_bw)
+                t = prune_threshold<DIST>(zero_weight_distance(job.kernel, gptr(job.chunk_bw)[ch], false));
             const int skip = box_gap2<DIST>(flo, fhi, gptr(job.boxes) + (size_t)ch * 2 * BD) > t;
             if (__builtin_amdgcn_readfirstlane(skip)) continue;
         }
@@ -222,7 +223,8 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 double d = pair_distance<DIST, NTA, MODE>(fc[r], sh, rr);
-                double w = kernel_fast_f64(job.kernel, d, MODE == kModeQDiag ? sh.kc[rr] : fbw[r]);
+                // qDiag with a fixed bandwidth: fbw holds the same constants as sh.kc
+                double w = kernel_fast_f64(job.kernel, d, (MODE == kModeQDiag && bw_pt) ? sh.kc[rr] : fbw[r]);
                 if (j >= jend) w = 0.0;
                 if (MODE == kModeCV && j == fidx[r]) w = 0.0;
                 double wa = MODE == kModeQDiag ? w * w : w;

@@ -258,7 +258,9 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
             }
         }
         workgroup_box<BD>(flo, fhi, fb, &fb);
+        // qDiag: the streamed points' bandwidths (per chunk when adaptive)
         if (MODE != kModeQDiag) thr = prune_threshold<DIST>(zero_weight_distance(args.kernel, fb, true));
+        else if (!args.chunk_bw) thr = prune_threshold<DIST>(zero_weight_distance(args.kernel, args.bw, true));
     }
 
     v4f hh[R][NTA + NTB], xx[R][NTA + NTB], ll[R][NTA + NTB];
@@ -276,9 +278,8 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
         if (prune) {
             const int ch = j0 / kChunkM;
             double t = thr;
-            if (MODE == kModeQDiag)
-                t = prune_threshold<DIST>(
-                    zero_weight_distance(args.kernel, args.chunk_bw ? gptr(args.chunk_bw)[ch] : args.bw, true));
+            if (MODE == kModeQDiag && args.chunk_bw)
+                t = prune_threshold<DIST>(zero_weight_distance(args.kernel, gptr(args.chunk_bw)[ch], true));
             const int skip = box_gap2<DIST>(flo, fhi, gptr(args.boxes) + (size_t)ch * 2 * BD) > t;
             if (__builtin_amdgcn_readfirstlane(skip)) continue;
         }
@@ -364,7 +365,8 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                             }
                     }
                 }
-                if constexpr (MODE == kModeQDiag) {
+                // qDiag with a fixed bandwidth: fbw holds the same constants as sh.bwc
+                if (MODE == kModeQDiag && bw_pt) {
 #pragma unroll
                     for (int e = 0; e < 8; ++e) w[e] = weight_from_d2(args.kernel, d2[e], sh.bwc[rb + e]);
                 } else {
