@@ -75,6 +75,7 @@ void free_problem(gwm_hip_ctx* c) {
     dfree(c->zscale); dfree(c->gscale); dfree(c->vscale); dfree(c->colmax);
     dfree(c->plan_counts); dfree(c->plan_order);
     c->plan_items = 0;
+    c->max_dist = -1.0;
     c->part_bytes = 0;
     c->n = 0;
 }
@@ -904,6 +905,11 @@ namespace gwm {
 namespace detail {
 
 int max_distance_impl(gwm_hip_ctx* c, double* out) {
+    // depends on the coordinates only: computed once per problem
+    if (c->max_dist >= 0.0) {
+        *out = c->max_dist;
+        return GWM_OK;
+    }
     int nblocks = (c->n + 255) / 256;
     GWM_HIP_CHECK(c, launch_max_distance(c->dist, c->points(), c->blockmax, nblocks, c->stream));
     std::vector<double> bm(nblocks);
@@ -911,6 +917,7 @@ int max_distance_impl(gwm_hip_ctx* c, double* out) {
     GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     double m = 0.0;
     for (double v : bm) m = v > m ? v : m;
+    c->max_dist = m;
     *out = m;
     return GWM_OK;
 }

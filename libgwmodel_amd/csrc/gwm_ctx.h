@@ -48,6 +48,7 @@ struct gwm_hip_ctx {
     int* plan_counts = nullptr;
     int* plan_order = nullptr;
     size_t plan_items = 0;
+    double max_dist = -1.0;        // largest pairwise distance, once computed (< 0: not yet)
     // adaptive bandwidth vectors of recently used bandwidths (least recently
     // used replaced): searches revisit bandwidths (FP64 confirmations, the
     // fixed first candidates of every multiscale search)

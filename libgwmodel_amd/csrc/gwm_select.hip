@@ -590,21 +590,31 @@ __global__ __launch_bounds__(256) void adaptive_bw_batch_kernel(const SelectJob*
 
 constexpr int kFallbackGrid = 2048;
 
+// Largest distance from the focus points of workgroup column blockIdx.x to
+// the data slice blockIdx.y (slice_len points); slices combine by an atomic
+// maximum on the bit patterns, which order non-negative doubles as their
+// values, so the result does not depend on the order.  block_max[] starts
+// at zero.
 template <int DIST>
-__global__ __launch_bounds__(256) void max_distance_kernel(Points P, double* block_max) {
+__global__ __launch_bounds__(256) void max_distance_kernel(Points P, int slice_len, double* block_max) {
     __shared__ double s_red[256];
     const int n = P.n;
     const int f = blockIdx.x * 256 + threadIdx.x;
+    const int j0 = blockIdx.y * slice_len, j1 = min(n, j0 + slice_len);
     double m = 0.0;
     if (f < n)
-        for (int j = 0; j < n; ++j) m = fmax(m, distance_f64<DIST>(P, f, j));
+        for (int j = j0; j < j1; ++j) m = fmax(m, distance_f64<DIST>(P, f, j));
     s_red[threadIdx.x] = m;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) s_red[threadIdx.x] = fmax(s_red[threadIdx.x], s_red[threadIdx.x + w]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) block_max[blockIdx.x] = s_red[0];
+    if (threadIdx.x == 0) {
+        const double v = s_red[0] > 0.0 ? s_red[0] : 0.0;   // +0 for a -0 or NaN maximum
+        atomicMax(reinterpret_cast<unsigned long long*>(block_max + blockIdx.x),
+                  (unsigned long long)__double_as_longlong(v));
+    }
 }
 
 template <int DIST>
@@ -661,10 +671,17 @@ hipError_t launch_adaptive_bw_batch(int dist, const SelectJob* jobs, const int* 
 }
 
 hipError_t launch_max_distance(int dist, const Points& P, double* block_max, int nblocks, hipStream_t s) {
+    if (nblocks != (P.n + 255) / 256) return hipErrorInvalidValue;
+    // about 2048 workgroups: the data split into slices of at least 256 points
+    const int slices = std::max(1, std::min((P.n + 255) / 256, 2048 / std::max(nblocks, 1)));
+    const int slice_len = (P.n + slices - 1) / slices;
+    hipError_t e = hipMemsetAsync(block_max, 0, (size_t)nblocks * sizeof(double), s);
+    if (e != hipSuccess) return e;
+    const dim3 grid(nblocks, slices);
     if (dist == kEuclidean)
-        hipLaunchKernelGGL(max_distance_kernel<kEuclidean>, dim3(nblocks), dim3(256), 0, s, P, block_max);
+        hipLaunchKernelGGL(max_distance_kernel<kEuclidean>, grid, dim3(256), 0, s, P, slice_len, block_max);
     else
-        hipLaunchKernelGGL(max_distance_kernel<kGreatCircle>, dim3(nblocks), dim3(256), 0, s, P, block_max);
+        hipLaunchKernelGGL(max_distance_kernel<kGreatCircle>, grid, dim3(256), 0, s, P, slice_len, block_max);
     return hipGetLastError();
 }
 

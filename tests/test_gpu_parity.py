@@ -68,6 +68,21 @@ def test_max_distance_euclidean_bitexact(ctx, oracle, londonhp100):
     assert ctx.max_distance() == oracle.max_distance(londonhp100["coords"])
 
 
+def test_max_distance_sliced_and_per_problem(ctx, oracle):
+    """n = 5000: the pairs are split over 20 x 20 workgroups and combined by an
+    atomic maximum; the value is cached per problem and recomputed for new
+    coordinates."""
+    rng = np.random.default_rng(21)
+    for scale in (100.0, 37.0):
+        say("case: max distance n=5000 scale=%g" % scale)
+        coords = rng.uniform(0, scale, (5000, 2))
+        X = np.column_stack([np.ones(5000), rng.normal(size=5000)])
+        ctx.set_data(coords, X, rng.normal(size=5000))
+        ref = oracle.max_distance(coords)
+        assert ctx.max_distance() == ref
+        assert ctx.max_distance() == ref
+
+
 # ------------------------------------------------------------------ adaptive kNN
 
 @pytest.mark.parametrize("n,seed", [(100, 0), (1000, 1), (3000, 2)])
