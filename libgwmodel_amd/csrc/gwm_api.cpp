@@ -432,9 +432,11 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
         if (c->prune && chosen == 0 && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg &&
             seg_len <= 64 * kPlanMaxCount) {
             if ((rc = ensure_plan(c, (size_t)tiles * nseg))) return rc;
-            GWM_HIP_CHECK(c, launch_sweep_plan(c->dist, c->points(), c->n, c->boxes, mode == kModeQDiag ? nullptr : bw_pt,
-                                               bw, j.chunk_bw, kernel, mixed ? 1 : 0, 64 * r, seg_len, nseg, tiles,
-                                               c->plan_counts, c->plan_order, c->stream));
+            // qDiag thresholds come from the streamed points' bandwidths (chunk_bw)
+            const double* focus_bw = mode == kModeQDiag ? nullptr : bw_pt;
+            GWM_HIP_CHECK(c, launch_sweep_plan(c->dist, c->points(), c->n, c->boxes, focus_bw, bw, j.chunk_bw, kernel,
+                                               mixed ? 1 : 0, 64 * r, seg_len, nseg, tiles, c->plan_counts,
+                                               c->plan_order, c->stream));
             L.order = c->plan_order;
             L.norder = tiles * nseg;
         }
