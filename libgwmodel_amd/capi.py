@@ -192,7 +192,8 @@ class Context:
         return dict(betas=betas, betasSE_raw=se, shat=shat, qDiag=q, S=S)
 
     def set_autotune(self, on=True):
-        """Online autotuning of the sweep launch shapes (results do not change)."""
+        """Online autotuning of the sweep launch shapes and the longest-first work
+        order of large pruned sweeps (results do not change)."""
         self._check(self._lib.gwm_hip_set_autotune(self.h, int(on)))
 
     def last_error_bound(self):
