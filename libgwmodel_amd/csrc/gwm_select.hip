@@ -22,7 +22,12 @@
 // (m = ceil((floor(bw)+1)/64) full chunks hold at least floor(bw)+1 points no
 // farther than U), and every point with a key <= U lies in a chunk whose
 // lower bound is <= U.  The two order statistics are therefore the same among
-// the candidates as among all n points.
+// the candidates as among all n points.  For large bandwidths the workgroup
+// select also takes a lower bound L: the mL-th smallest chunk lower bound,
+// mL = ceil(floor(bw)/64), has fewer than floor(bw) points below it, so a
+// chunk whose upper bound is below L holds only keys ranked below both
+// wanted ones; such chunks are counted whole (their sizes lower the wanted
+// ranks) and their keys never computed.
 #include <algorithm>
 
 #include "gwm_contract.h"
@@ -260,8 +265,8 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, doub
     __shared__ unsigned long long cand[kSelCap];
     __shared__ unsigned long long s_prefix, s_above, s_next;
     __shared__ long long s_rank;
-    __shared__ unsigned int s_count, s_ncand, s_nclist;
-    __shared__ double s_U;
+    __shared__ unsigned int s_count, s_ncand, s_nclist, s_below;
+    __shared__ double s_U, s_L;
     __shared__ double s_red[256];
     __shared__ int clist[kSelChunks];
 
@@ -285,6 +290,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, doub
     const long long need = (long long)b0 + 1;   // keys up to rank floor(bw) are needed
     const bool pruned = boxes && nch <= kSelChunks && need <= 64LL * nfull;
     int nclist = 0;
+    long long below = 0;   // points of the chunks counted whole below the wanted keys
     double U = 0.0;
     if (pruned) {
         // U: upper edge of the 16-bit bin holding the m-th smallest chunk
@@ -295,15 +301,35 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, doub
         }
         __syncthreads();
         U = s_U;
-        if (tid == 0) s_nclist = 0u;
+        // L: lower edge of the 16-bit bin holding the mL-th smallest chunk
+        // lower bound over all chunks, mL = ceil(b0 / 64).  Fewer than
+        // 64 (mL - 1) < b0 points have keys below L, so both wanted keys
+        // (ranks b0 - 1 and b0) are >= L, and a chunk whose upper bound is
+        // below L holds only keys ranked below them: it is counted whole.
+        const long long mL = ((long long)b0 + 63) / 64;
+        double L = -INFINITY;
+        if (mL >= 2) {
+            if (tid < 64) {
+                const double l = wave_chunk_bound<DIST>(F, f, boxes, nch, mL - 1, false, hist);
+                if (tid == 0) s_L = l;
+            }
+            __syncthreads();
+            L = s_L;
+        }
+        if (tid == 0) {
+            s_nclist = 0u;
+            s_below = 0u;
+        }
         __syncthreads();
         for (int c = tid; c < nch; c += 256) {
             double lb, ub;
             key_bounds<DIST>(F, f, boxes + (size_t)c * BD, &lb, &ub);
-            if (lb <= U) clist[atomicAdd(&s_nclist, 1u)] = c;
+            if (ub < L) atomicAdd(&s_below, (unsigned int)min(64, n - 64 * c));
+            else if (lb <= U) clist[atomicAdd(&s_nclist, 1u)] = c;
         }
         __syncthreads();
         nclist = (int)s_nclist;
+        below = (long long)s_below;
     }
     // the selection keys of every candidate point j
     auto for_each_key = [&](auto&& fn) {
@@ -320,7 +346,8 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, doub
     int ncand = 0;
     if (pruned && nclist * 64 <= kSelCap) {
         // few candidates: hold all their keys in LDS (every key up to rank
-        // floor(bw) is among them, so nothing lies "above" the set that matters)
+        // floor(bw) is among them or counted below, so nothing lies "above" the
+        // set that matters)
         if (tid == 0) {
             s_ncand = 0u;
             s_above = ~0ull;
@@ -331,7 +358,8 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, doub
         ncand = (int)s_ncand;
         in_lds = true;
     }
-    long long rank0 = (long long)b0 - 1;   // 0-based rank of d0 (within the LDS keys, once gathered)
+    // 0-based rank of d0 among the candidate keys (within the LDS keys, once gathered)
+    long long rank0 = (long long)b0 - 1 - below;
     if (pruned && !in_lds && U > 0.0) {
         // many candidates: one pass of kSelBins linear bins over [0, U] (the
         // wanted keys are <= U; the map is monotone, larger keys share the last
