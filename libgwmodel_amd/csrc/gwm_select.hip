@@ -490,7 +490,8 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, doub
 // ---------------------------------------------------------------- one wave
 //
 // Fast path for pruned selections with few candidates (the usual adaptive
-// bandwidths, tens to hundreds of neighbours): one wave per focus point, no
+// bandwidths, tens to hundreds of neighbours, and larger ones whose annulus
+// of candidate chunks between L and U is small): one wave per focus point, no
 // workgroup barriers.  U comes from two 8-bit radix digits of the chunk upper
 // bounds (the upper edge of a 16-bit bin), the candidate chunks' keys (at most
 // kWaveKeys, padded with the largest key) are gathered into the wave's LDS,
@@ -508,11 +509,16 @@ __device__ __forceinline__ bool adaptive_bw_wave(const PTS& F, const PTS& P, dou
     const int nch = (n + 63) / 64, nfull = n / 64;
     const double b0 = floor(bw), bx = bw - b0;
     const long long need = (long long)b0 + 1;
-    if (!(bw / n < 1) || !boxes || b0 < 1 || need > kWaveKeys || need > 64LL * nfull) return false;
+    if (!(bw / n < 1) || !boxes || b0 < 1 || need > 64LL * nfull) return false;
     // U: upper edge of the 16-bit bin holding the m-th smallest chunk upper bound
     const double U = wave_chunk_bound<DIST>(F, f, boxes, nfull, (need + 63) / 64 - 1, true, hist);
+    // L: the lower cut of adaptive_bw_one (chunks wholly below both wanted
+    // keys are counted, not gathered), so that large bandwidths whose
+    // candidate annulus is small also take this path
+    const long long mL = ((long long)b0 + 63) / 64;
+    const double L = mL >= 2 ? wave_chunk_bound<DIST>(F, f, boxes, nch, mL - 1, false, hist) : -INFINITY;
     // candidate chunks, in chunk order
-    int ncl = 0;
+    int ncl = 0, below = 0;
     bool overflow = false;
     for (int c0 = 0; c0 < nch; c0 += 64) {
         const int c = c0 + lane;
@@ -520,7 +526,8 @@ __device__ __forceinline__ bool adaptive_bw_wave(const PTS& F, const PTS& P, dou
         if (c < nch) {
             double lb, ub;
             key_bounds<DIST>(F, f, boxes + (size_t)c * BD, &lb, &ub);
-            take = lb <= U;
+            if (ub < L) below += min(64, n - 64 * c);
+            else take = lb <= U;
         }
         const unsigned long long m = __ballot(take);
         const int pos = ncl + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -532,6 +539,7 @@ __device__ __forceinline__ bool adaptive_bw_wave(const PTS& F, const PTS& P, dou
         }
     }
     if (overflow) return false;
+    for (int m = 32; m >= 1; m >>= 1) below += __shfl_xor(below, m);
     wave_sync();
     // the candidates' keys; padding keys are the largest value
     const int nk = ncl * 64;
@@ -541,7 +549,7 @@ __device__ __forceinline__ bool adaptive_bw_wave(const PTS& F, const PTS& P, dou
     }
     wave_sync();
     unsigned long long key0, key1;
-    wave_rank_select(keys, nk, (long long)b0 - 1, ~0ull, hist, &key0, &key1);
+    wave_rank_select(keys, nk, (long long)b0 - 1 - below, ~0ull, hist, &key0, &key1);
     if (lane == 0) {
         const double d0 = select_to_distance<DIST>(val_of(key0));
         const double d1 = select_to_distance<DIST>(val_of(key1));
