@@ -73,7 +73,11 @@ void free_problem(gwm_hip_ctx* c) {
     dfree(c->vZ); dfree(c->S); dfree(c->fail);
     dfree(c->Zh); dfree(c->Zl); dfree(c->Gh); dfree(c->Gl); dfree(c->vZh); dfree(c->vZl);
     dfree(c->zscale); dfree(c->gscale); dfree(c->vscale); dfree(c->colmax);
-    dfree(c->plan_counts); dfree(c->plan_order);
+    // the plan buffers come from hipMalloc (ensure_plan) on every context
+    if (c->plan_counts) hipFree(c->plan_counts);
+    if (c->plan_order) hipFree(c->plan_order);
+    c->plan_counts = nullptr;
+    c->plan_order = nullptr;
     c->plan_items = 0;
     c->max_dist = -1.0;
     c->part_bytes = 0;
