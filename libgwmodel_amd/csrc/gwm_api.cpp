@@ -395,7 +395,7 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
             chosen = it == g_tune.end() ? -1 : it->second.chosen;
         }
         if (chosen < 0) {
-            const int r = contract_r(nta, ntb, mixed ? 1 : 0);
+            const int r = contract_r(nta, ntb, mixed ? 1 : 0, mode, c->dist);
             const int grid = (c->n + 64 * r - 1) / (64 * r) * nseg;
             TuneState st;
             st.ncand = grid >= kTuneGridMax ? 1 : (r == 2 ? 4 : 2);
@@ -431,7 +431,7 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
         // Large pruned sweeps run their (tile, segment) items longest first
         // (gwm_plan.hip): narrow bandwidths leave a few busy items per tile,
         // and in plain order the sweep ends on a tail of them.
-        const int r = contract_r(nta, ntb, mixed ? 1 : 0);
+        const int r = contract_r(nta, ntb, mixed ? 1 : 0, mode, c->dist);
         const int tiles = (c->n + 64 * r - 1) / (64 * r);
         if (c->prune && chosen == 0 && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg &&
             seg_len <= 64 * kPlanMaxCount) {
@@ -1136,7 +1136,7 @@ int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adapt
     // large pruned sweeps: items longest first, as for the fits (the
     // locations are in their own spatial order)
     {
-        const int r = contract_r(c->nta, 0, mixed ? 1 : 0);
+        const int r = contract_r(c->nta, 0, mixed ? 1 : 0, kModePlain, c->dist);
         const int tiles = (m + 64 * r - 1) / (64 * r);
         if (c->prune && c->autotune && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg &&
             seg_len <= 64 * kPlanMaxCount) {

@@ -479,7 +479,8 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
             }
             SweepGroup& g = group_of(sweeps, r->dist, mode, prec, nta, tb);
             g.jobs.push_back(sweep_job(r, mode, nta, tb, bw_pt, ev.bw));
-            add_items(g, (int)g.jobs.size() - 1, sl.a0, sl.a1, r->nseg, contract_r(nta, tb, prec == GWM_MIXED));
+            add_items(g, (int)g.jobs.size() - 1, sl.a0, sl.a1, r->nseg,
+                      contract_r(nta, tb, prec == GWM_MIXED, mode, r->dist));
             SolveGroup& sg = sgroup_of(solves, k, smode);
             if (sg.start.empty()) sg.start.push_back(0);
             sg.jobs.push_back(solve_job(r, PT, colB, bw_pt, sl.a0));
@@ -496,7 +497,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
             j.nfocus = r->nrefine;
             j.n_pad = npad;
             g.jobs.push_back(j);
-            add_items(g, (int)g.jobs.size() - 1, sl.e0, sl.e1, r->nseg, contract_r(nta, tb, 0));
+            add_items(g, (int)g.jobs.size() - 1, sl.e0, sl.e1, r->nseg, contract_r(nta, tb, 0, mode, r->dist));
             SolveGroup& sg = sgroup_of(solves, k, smode);
             if (sg.start.empty()) sg.start.push_back(0);
             SolveArgs sa = solve_job(r, PT, colB, bw_pt, sl.e0);
@@ -513,7 +514,8 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         if (sl.q1 > sl.q0) {
             SweepGroup& q = group_of(qsweeps, r->dist, kModeQDiag, prec, ntb, 0);
             q.jobs.push_back(sweep_job(r, kModeQDiag, ntb, 0, bw_pt, ev.bw));
-            add_items(q, (int)q.jobs.size() - 1, sl.q0, sl.q1, r->nseg, contract_r(ntb, 0, prec == GWM_MIXED));
+            add_items(q, (int)q.jobs.size() - 1, sl.q0, sl.q1, r->nseg,
+                      contract_r(ntb, 0, prec == GWM_MIXED, kModeQDiag, r->dist));
             QdGroup& qg = kgroup_of(qds, k);
             if (qg.start.empty()) qg.start.push_back(0);
             QDiagJob qj{r->part, r->nseg, ntb * 16, c->n, c->n_pad, c->X, c->sii, c->qd};

@@ -86,8 +86,9 @@ hipError_t launch_split_fp16(const double* Z, int rows, int ldz, unsigned long l
 // nitems (batch) or tiles * nseg (single job).
 hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s);
 hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s);
-// the R (focus tiles per wave) each kernel uses for a (nta, ntb) instance
-int contract_r(int nta, int ntb, int precision_mixed);
+// the R (focus tiles per wave) each kernel uses for an (nta, ntb, mode,
+// distance) instance; launchers and item tables must agree on it
+int contract_r(int nta, int ntb, int precision_mixed, int mode, int dist);
 
 // Work order of a pruned single-job sweep whose focus points are the points
 // of F in order (the data table itself, or the prediction locations) against

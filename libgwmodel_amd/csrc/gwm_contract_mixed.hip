@@ -462,10 +462,18 @@ hipError_t launch_mixed_rr(const SweepLaunch& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Default focus tiles per wave: two up to six column tiles, except the CV
+// sweep on great-circle data with three or more, whose two-tile form needs
+// 268 VGPRs (one wave per SIMD) where the one-tile form fits in 212 (two).
+template <int NTA, int NTB, int DIST, int MODE>
+constexpr int mixed_r() {
+    return (MODE == kModeCV && DIST == kGreatCircle && NTA >= 3) ? 1 : ((NTA + NTB) <= 6 ? 2 : 1);
+}
+
 // The default shape, or one focus tile per wave when asked for (single jobs).
 template <int NTA, int NTB, int DIST, int MODE>
 hipError_t launch_mixed_r(const SweepLaunch& a, hipStream_t s) {
-    constexpr int RD = (NTA + NTB) <= 6 ? 2 : 1;
+    constexpr int RD = mixed_r<NTA, NTB, DIST, MODE>();
     if constexpr (RD == 2) {
         if (a.shape_r == 1 && !a.items) return launch_mixed_rr<NTA, NTB, DIST, MODE, 1>(a, s);
     }
@@ -532,8 +540,8 @@ __global__ void split_kernel(const double* Z, int rows, int ldz, const unsigned 
 
 int contract_mixed_tile_focus(int nta, int ntb) { return kWaves * 16 * (((nta + ntb) <= 6) ? 2 : 1); }
 
-int contract_r(int nta, int ntb, int precision_mixed) {
-    if (precision_mixed) return (nta + ntb) <= 6 ? 2 : 1;
+int contract_r(int nta, int ntb, int precision_mixed, int mode, int dist) {
+    if (precision_mixed) return (mode == kModeCV && dist == kGreatCircle && nta >= 3) ? 1 : ((nta + ntb) <= 6 ? 2 : 1);
     return (nta + ntb) <= 8 ? 2 : 1;
 }
 
