@@ -76,8 +76,6 @@ struct SweepLaunch {
 // segments next to the current tiles.
 int tile_stride(int tiles);
 
-int contract_tile_focus(int nta, int ntb);
-int contract_mixed_tile_focus(int nta, int ntb);
 // FP16 hi/lo planes ([rows/32][ldz][32]) and per-column unscale factors of a
 // row-major FP64 table; colmax is ldz words of scratch.
 hipError_t launch_split_fp16(const double* Z, int rows, int ldz, unsigned long long* colmax, _Float16* Zh,

@@ -335,7 +335,6 @@ int tile_stride(int tiles) {
     return s % tiles;
 }
 
-int contract_tile_focus(int nta, int ntb) { return kWaves * 16 * (((nta + ntb) <= 8) ? 2 : 1); }
 
 hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s) {
     if (dist == kEuclidean) {

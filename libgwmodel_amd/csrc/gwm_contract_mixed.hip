@@ -462,12 +462,19 @@ hipError_t launch_mixed_rr(const SweepLaunch& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Default focus tiles per wave: two up to six column tiles, except the CV
-// sweep on great-circle data with three or more, whose two-tile form needs
-// 268 VGPRs (one wave per SIMD) where the one-tile form fits in 212 (two).
+// Default focus tiles per wave: two where the two-tile form fits in 256
+// VGPRs (two waves per SIMD), i.e. up to three column tiles except the CV and
+// qDiag sweeps on great-circle data with three; one elsewhere, where the
+// two-tile form would run one wave per SIMD and the one-tile form two (e.g.
+// Euclidean, four column tiles: 304 against 188 VGPRs; great-circle CV,
+// three: 268 against 212).  Both forms give the same bits.
+constexpr int mixed_default_r(int nta, int ntb, int mode, int dist) {
+    return (nta + ntb <= 3 && !(dist == kGreatCircle && (mode == kModeCV || mode == kModeQDiag) && nta >= 3)) ? 2
+                                                                                                              : 1;
+}
 template <int NTA, int NTB, int DIST, int MODE>
 constexpr int mixed_r() {
-    return (MODE == kModeCV && DIST == kGreatCircle && NTA >= 3) ? 1 : ((NTA + NTB) <= 6 ? 2 : 1);
+    return mixed_default_r(NTA, NTB, MODE, DIST);
 }
 
 // The default shape, or one focus tile per wave when asked for (single jobs).
@@ -538,10 +545,9 @@ __global__ void split_kernel(const double* Z, int rows, int ldz, const unsigned 
 
 }  // namespace
 
-int contract_mixed_tile_focus(int nta, int ntb) { return kWaves * 16 * (((nta + ntb) <= 6) ? 2 : 1); }
 
 int contract_r(int nta, int ntb, int precision_mixed, int mode, int dist) {
-    if (precision_mixed) return (mode == kModeCV && dist == kGreatCircle && nta >= 3) ? 1 : ((nta + ntb) <= 6 ? 2 : 1);
+    if (precision_mixed) return mixed_default_r(nta, ntb, mode, dist);
     return (nta + ntb) <= 8 ? 2 : 1;
 }
 
