@@ -467,8 +467,11 @@ hipError_t launch_mixed_rr(const SweepLaunch& a, hipStream_t s) {
 // qDiag sweeps on great-circle data with three; one elsewhere, where the
 // two-tile form would run one wave per SIMD and the one-tile form two (e.g.
 // Euclidean, four column tiles: 304 against 188 VGPRs; great-circle CV,
-// three: 268 against 212).  Both forms give the same bits.
+// three: 268 against 212).  Hat-mode sweeps keep two up to six column tiles:
+// measured faster there despite one wave per SIMD (great circle, 3 + 3
+// tiles: 5.0 against 5.6 ms).  Both forms give the same bits.
 constexpr int mixed_default_r(int nta, int ntb, int mode, int dist) {
+    if (mode == kModeHat) return nta + ntb <= 6 ? 2 : 1;
     return (nta + ntb <= 3 && !(dist == kGreatCircle && (mode == kModeCV || mode == kModeQDiag) && nta >= 3)) ? 2
                                                                                                               : 1;
 }
