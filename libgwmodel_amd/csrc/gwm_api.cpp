@@ -63,7 +63,7 @@ void free_problem(gwm_hip_ctx* c) {
     };
     dfree(c->px); dfree(c->py); dfree(c->hx); dfree(c->hy_); dfree(c->hz); dfree(c->sl);
     dfree(c->X); dfree(c->y); dfree(c->Z); dfree(c->G); dfree(c->part);
-    dfree(c->boxes); dfree(c->chunk_bw); dfree(c->sel_left); dfree(c->sel_nleft);
+    dfree(c->boxes); dfree(c->chunk_bw); dfree(c->sel_left); dfree(c->sel_nleft); dfree(c->dnn);
     for (auto& e : c->bw_cache) dfree(e.buf);
     c->bw_cache.clear();
     dfree(c->bwi); dfree(c->res2); dfree(c->cond); dfree(c->sii); dfree(c->rlist); dfree(c->rcount);
@@ -345,7 +345,7 @@ std::map<TuneKey, TuneState> g_tune;
 
 // One sweep of the fused contraction.
 int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const double* bw_pt, double bw, int kernel,
-          int precision, int* PT_out, int* nseg_out) {
+          int precision, int* PT_out, int* nseg_out, const double* dnn = nullptr) {
     const bool mixed = precision == GWM_MIXED;
     int nseg, seg_len;
     data_segments(c->n, nseg, seg_len);
@@ -369,6 +369,7 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
     j.nta = nta;
     j.ntb = ntb;
     j.out = c->part;
+    j.dnn = dnn;
     if (c->prune) {
         j.boxes = c->boxes;
         if (mode == kModeQDiag && bw_pt) {
@@ -520,7 +521,7 @@ struct PredictSet {
 // maximum of the mixed pass goes to c->last_cond.  q: the focus points are
 // prediction locations rather than the data points.
 int refine_fp64(gwm_hip_ctx* c, int mode, int smode, int ntb, int colB, const double* bw_pt, double bw, int kernel,
-                double limit, int* count, const PredictSet* q = nullptr) {
+                double limit, int* count, const PredictSet* q = nullptr, const double* dnn = nullptr) {
     const int nf = q ? q->Q.n : c->n;
     double* cond = q ? q->cond : c->cond;
     int* rlist = q ? q->rlist : c->rlist;
@@ -563,6 +564,7 @@ int refine_fp64(gwm_hip_ctx* c, int mode, int smode, int ntb, int colB, const do
     j.out = c->rpart;
     j.focus_idx = rlist;
     j.nfocus = *count;
+    j.dnn = dnn;
     if (q) j.Q = q->Q;
     if (c->prune) j.boxes = c->boxes;
     L.jobs = nullptr;
@@ -704,18 +706,27 @@ int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit
     if (stopped(c)) return GWM_ERR_CANCELLED;
     const double* bw_pt;
     if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
+    // CV zeroes the focus point's own weight: its largest weight is that of
+    // its nearest other point
+    const double* dnn = nullptr;
+    if (crit == GWM_CRIT_CV) {
+        if ((rc = ensure_dnn(c))) return rc;
+        dnn = c->dnn;
+    }
     GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
     int PT, nseg;
     const int mode = crit == GWM_CRIT_CV ? kModeCV : kModePlain;
-    if ((rc = sweep(c, mode, c->nta, 0, design_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg))) return rc;
+    if ((rc = sweep(c, mode, c->nta, 0, design_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg, dnn))) return rc;
     SolveArgs s = solve_args(c, PT, nseg, 0, bw_pt, bw, kernel);
+    if (precision == GWM_MIXED) s.dnn = dnn;
     const int smode = crit == GWM_CRIT_CV ? kSolveCV : kSolveAIC;
     timer_begin(c);
     GWM_HIP_CHECK(c, launch_solve(c->k, smode, s, c->stream));
     if ((rc = timer_end(c, "solve"))) return rc;
     if (precision == GWM_MIXED) {
         int count = 0;
-        if ((rc = refine_fp64(c, mode, smode, 0, 0, bw_pt, bw, kernel, c->refine_crit, &count))) return rc;
+        if ((rc = refine_fp64(c, mode, smode, 0, 0, bw_pt, bw, kernel, c->refine_crit, &count, nullptr, dnn)))
+            return rc;
     }
     // sums, and (mixed) the indicator maximum over the final per-point values
     const bool mixed = precision == GWM_MIXED;
@@ -928,6 +939,27 @@ int max_distance_impl(gwm_hip_ctx* c, double* out) {
     return GWM_OK;
 }
 
+// Nearest other point of every point: the adaptive threshold at 2
+// neighbours, the point itself being the first (BandwidthWeight.cpp:40-55
+// with bw = 2: the second smallest distance, 0 for coincident points).
+int ensure_dnn(gwm_hip_ctx* c) {
+    if (c->dnn) return GWM_OK;
+    double* buf = nullptr;
+    if (c->async_mem) GWM_HIP_CHECK(c, hipMallocAsync((void**)&buf, (size_t)c->n * sizeof(double), c->stream));
+    else GWM_HIP_CHECK(c, hipMalloc((void**)&buf, (size_t)c->n * sizeof(double)));
+    c->dnn = buf;
+    hipError_t e = launch_adaptive_bw(c->dist, c->points(), 2.0, buf, c->prune ? c->boxes : nullptr, c->sel_left,
+                                      c->sel_nleft, c->stream);
+    if (e != hipSuccess) {
+        c->err = std::string("nearest-point select: ") + hipGetErrorString(e);
+        if (c->async_mem) hipFreeAsync(buf, c->stream);
+        else hipFree(buf);
+        c->dnn = nullptr;
+        return GWM_ERR_HIP;
+    }
+    return GWM_OK;
+}
+
 }  // namespace detail
 }  // namespace gwm
 
@@ -984,15 +1016,31 @@ int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int
         GWM_HIP_CHECK(c, launch_split_fp16(c->vZ, c->rows_pad, 16, c->colmax, c->vZh, c->vZl, c->vscale, c->stream));
     const double* bw_pt;
     if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
-    GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
-    int PT, nseg;
     const bool cv = value && crit == GWM_CRIT_CV;
-    if ((rc = sweep(c, cv ? kModeCV : kModePlain, 1, 0, var_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg)))
-        return rc;
+    const double* dnn = nullptr;
+    if (cv) {
+        if ((rc = ensure_dnn(c))) return rc;
+        dnn = c->dnn;
+    }
+    int PT, nseg, fail = 0;
     const int smode = beta ? kSolveBeta : (cv ? kSolveCV : kSolveAIC);
-    GWM_HIP_CHECK(c, launch_solve_var(smode, c->part, nseg, PT, 0, n, c->n_pad, c->vx, c->vy, bw_pt, bw, kernel,
-                                      c->res2, c->sii, beta ? c->betas : nullptr, c->fail, c->stream));
-    int fail = 0;
+    // a mixed CV evaluation with a point far from every other (its largest
+    // weight below kMixedMinWeight: fail bit 2) is taken again in FP64
+    for (int pass = 0; pass < 2; ++pass) {
+        GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
+        if ((rc = sweep(c, cv ? kModeCV : kModePlain, 1, 0, var_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg,
+                        dnn)))
+            return rc;
+        GWM_HIP_CHECK(c, launch_solve_var(smode, c->part, nseg, PT, 0, n, c->n_pad, c->vx, c->vy, bw_pt, bw, kernel,
+                                          c->res2, c->sii, beta ? c->betas : nullptr, c->fail, c->stream,
+                                          precision == GWM_MIXED ? dnn : nullptr));
+        if (precision != GWM_MIXED || !cv) break;
+        GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        if (!(fail & 2)) break;
+        precision = GWM_FP64;
+        c->solves += n;
+    }
     if (value) {
         GWM_HIP_CHECK(c, launch_reduce(c->res2, cv ? nullptr : c->sii, nullptr, nullptr, n, c->red, c->stream));
         double red[3];
@@ -1000,7 +1048,7 @@ int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int
         GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
         c->solves += n;
-        if (fail) return GWM_ERR_NOT_SPD;
+        if (fail & 1) return GWM_ERR_NOT_SPD;
         double v = cv ? red[0] : aicc(red[0], (double)n, red[1]);
         if (!std::isfinite(v)) return GWM_ERR_NOT_SPD;
         *value = v;
@@ -1074,7 +1122,7 @@ int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adapt
     if (gc) sphere_tables(M, tab.data(), tab.data() + M, &tab[2 * M], &tab[3 * M], &tab[4 * M], &tab[5 * M]);
     // device arrays: tables, bandwidths, betas, indicator; lists
     DevBuf dbuf, ibuf;
-    const size_t nd = M * (ntab + 2 + k);
+    const size_t nd = M * (ntab + 3 + k);
     GWM_HIP_CHECK(c, hipMalloc(&dbuf.p, nd * sizeof(double)));
     GWM_HIP_CHECK(c, hipMalloc(&ibuf.p, (2 * M + 1) * sizeof(int)));
     double* d = static_cast<double*>(dbuf.p);
@@ -1094,7 +1142,8 @@ int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adapt
     Q.n = m;
     double* qbw = d + ntab * M;
     double* qcond = qbw + M;
-    double* qbetas = qcond + M;
+    double* qdnn = qcond + M;
+    double* qbetas = qdnn + M;
     const double* bw_pt = nullptr;
     if (adaptive) {
         timer_begin(c);
@@ -1103,8 +1152,18 @@ int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adapt
         if ((rc = timer_end(c, "select"))) return rc;
         bw_pt = qbw;
     }
-    // one sweep over the locations
+    // A location is not a data point, so its largest weight is that of its
+    // nearest data point (the adaptive threshold of 1 neighbour): the FP64
+    // exponent cut is relative to it, and mixed precision recomputes
+    // locations whose largest weight is tiny.
     const bool mixed = precision == GWM_MIXED;
+    const double* dnn = nullptr;
+    if (mixed || kernel == GWM_GAUSSIAN || kernel == GWM_EXPONENTIAL) {
+        GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), 1.0, qdnn, c->prune ? c->boxes : nullptr, left,
+                                            nleft, c->stream, &Q));
+        dnn = qdnn;
+    }
+    // one sweep over the locations
     int nseg, seg_len;
     data_segments(c->n, nseg, seg_len);
     const int PT = c->nta * 16;
@@ -1128,6 +1187,7 @@ int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adapt
     j.nta = c->nta;
     j.ntb = 0;
     j.out = c->part;
+    j.dnn = dnn;
     if (c->prune) j.boxes = c->boxes;
     L.jobs = nullptr;
     L.items = nullptr;
@@ -1159,11 +1219,12 @@ int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adapt
     sa.y = nullptr;
     sa.betas = qbetas;
     sa.cond = qcond;
+    if (mixed) sa.dnn = dnn;
     GWM_HIP_CHECK(c, launch_solve(k, kSolvePredict, sa, c->stream));
     if (mixed) {
         const PredictSet q{Q, qcond, rlist, qbetas};
         int count = 0;
-        if ((rc = refine_fp64(c, kModePlain, kSolvePredict, 0, 0, bw_pt, bw, kernel, c->refine_fit, &count, &q)))
+        if ((rc = refine_fp64(c, kModePlain, kSolvePredict, 0, 0, bw_pt, bw, kernel, c->refine_fit, &count, &q, dnn)))
             return rc;
     }
     int fail = 0;

@@ -126,7 +126,7 @@ size_t footprint_bytes(int n, int k, int dist, int nseg) {
     b += (size_t)n * (k + 1) * 8;                                       // X, y
     b += rows * nta * 16 * (8 + 4) + rows * ntb * 16 * (8 + 4);        // Z, G and FP16 planes
     b += rows * 16 * (8 + 4) + (size_t)n * 16;                          // single-column tables
-    b += (size_t)n * 7 * 8 + (size_t)n * k * 3 * 8;                     // per-focus vectors
+    b += (size_t)n * 8 * 8 + (size_t)n * k * 3 * 8;                     // per-focus vectors
     b += (size_t)nseg * (nta + ntb) * 16 * np * 8;                      // partial sums
     return b;
 }
@@ -237,6 +237,7 @@ SweepJob sweep_job(Request* r, int mode, int nta, int ntb, const double* bw_pt, 
     j.nta = nta;
     j.ntb = ntb;
     j.out = r->part;
+    if (mode == kModeCV) j.dnn = c->dnn;   // weights relative to the nearest other point
     if (c->prune) {
         j.boxes = c->boxes;
         if (mode == kModeQDiag && bw_pt) j.chunk_bw = c->chunk_bw;
@@ -483,7 +484,10 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
                       contract_r(nta, tb, prec == GWM_MIXED, mode, r->dist));
             SolveGroup& sg = sgroup_of(solves, k, smode);
             if (sg.start.empty()) sg.start.push_back(0);
-            sg.jobs.push_back(solve_job(r, PT, colB, bw_pt, sl.a0));
+            SolveArgs sa = solve_job(r, PT, colB, bw_pt, sl.a0);
+            // mixed CV: focus points far from every other point go to the FP64 list
+            if (prec == GWM_MIXED && mode == kModeCV) sa.dnn = c->dnn;
+            sg.jobs.push_back(sa);
             sg.start.push_back(sg.start.back() + sl.a1 - sl.a0);
         }
         if (sl.e1 > sl.e0) {
@@ -813,6 +817,7 @@ int setup(gwm_hip_server* S, Request* R) {
     if (R->select_bw && !R->adaptive && !R->has_upper) {
         if ((rc = max_distance_impl(R->ctx, &R->maxdist))) return rc;
     }
+    if (R->select_bw && R->crit == GWM_CRIT_CV && (rc = ensure_dnn(R->ctx))) return rc;
     if (hipStreamSynchronize(R->ctx->stream) != hipSuccess) return GWM_ERR_HIP;
     return GWM_OK;
 }

@@ -45,6 +45,12 @@ struct SweepJob {
     // optional count of the focus x data pairs actually swept (pruning skips
     // the rest): one atomic add per workgroup and processed chunk
     unsigned long long* pairs = nullptr;
+    // FP64 sweeps whose focus point's own weight is not the largest (CV:
+    // zeroed; predict: the focus is not a data point): per focus point the
+    // distance to its nearest other (data) point, indexed like bw_pt; the
+    // exponent cut of the Gaussian and exponential kernels is taken relative
+    // to that point's weight (make_kconst64_nn).  Null: relative to 1.
+    const double* dnn = nullptr;
 };
 
 // Launch of one sweep: either a single job (items == nullptr; workgroup b
@@ -153,6 +159,11 @@ struct SolveArgs {
     const int* focus_idx = nullptr;
     int nfocus = 0;
     double* ainv = nullptr;  // per focus: A^-1 row-major k x k (kSolveBeta), or null
+    // mixed-precision solves of CV criteria and predictions (with cond): per
+    // focus the distance to the nearest other (data) point; a focus whose
+    // largest weight, that point's, is below kMixedMinWeight gets cond =
+    // infinity, which sends it to the FP64 recomputation
+    const double* dnn = nullptr;
 };
 
 hipError_t launch_solve(int k, int mode, const SolveArgs& a, hipStream_t s);
@@ -247,8 +258,12 @@ hipError_t launch_reduce(const double* a, const double* b, const double* c, cons
 
 // Single-column local solves for the multiscale path (GWRMultiscale.cpp:420-482,
 // 560-636): part columns 0 = sum w x^2, 1 = sum w x y, colB = sum w^2 x^2.
+// *fail |= 1 on a denominator that is not positive; with dnn (mixed CV
+// evaluations, per point the distance to its nearest other point) *fail |= 2
+// when a point's largest weight is below kMixedMinWeight.
 hipError_t launch_solve_var(int mode, const double* part, int nseg, int PT, int colB, int n, int n_pad,
                             const double* x, const double* y, const double* bw_pt, double bw, int kernel,
-                            double* res2, double* sii, double* beta, int* fail, hipStream_t s);
+                            double* res2, double* sii, double* beta, int* fail, hipStream_t s,
+                            const double* dnn = nullptr);
 
 }  // namespace gwm

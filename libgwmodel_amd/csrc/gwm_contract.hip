@@ -128,6 +128,8 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
     KConst64 fbw[R];
     int fidx[R];
     double fb = 0.0;   // largest focus bandwidth of this lane
+    double fdn = 0.0;  // largest nearest-point distance of this lane's focus points (CV, predict)
+    GWM_GLOBAL const double* dnn = MODE != kModeQDiag ? gptr(job.dnn) : nullptr;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int f = fbase + 16 * r + (lane & 15);
@@ -136,7 +138,13 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
         fidx[r] = qtab ? -1 : (focus_idx ? (f < nf ? fl : -1) : f);   // self index for the CV exclusion
         fc[r] = load_focus<DIST>(FP, fl);
         const double b = (MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : job.bw;
-        fbw[r] = make_kconst64(b);
+        if (dnn) {
+            const double dn = dnn[fl];
+            fbw[r] = make_kconst64_nn(b, job.kernel, dn);
+            fdn = fmax(fdn, dn);
+        } else {
+            fbw[r] = make_kconst64(b);
+        }
         fb = fmax(fb, b);
     }
     // pruning: the workgroup's focus box and the zero-weight threshold
@@ -159,9 +167,9 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
                 fhi[d] = fmax(fhi[d], x[d]);
             }
         }
-        workgroup_box<BD>(flo, fhi, fb, &fb);
+        workgroup_box<BD>(flo, fhi, fb, &fb, fdn, &fdn);
         // qDiag: the streamed points' bandwidths (per chunk when adaptive)
-        if (MODE != kModeQDiag) thr = prune_threshold<DIST>(zero_weight_distance(job.kernel, fb, false));
+        if (MODE != kModeQDiag) thr = prune_threshold<DIST>(zero_weight_distance(job.kernel, fb, false, fdn));
         else if (!job.chunk_bw) thr = prune_threshold<DIST>(zero_weight_distance(job.kernel, job.bw, false));
     }
 

@@ -61,6 +61,10 @@ struct gwm_hip_ctx {
     unsigned long long bw_clock = 0;
     int* sel_left = nullptr;       // [n] points the one-wave select leaves to the workgroup select
     unsigned int* sel_nleft = nullptr;
+    // [n] distance from each point to its nearest other point (the adaptive
+    // threshold of 2 neighbours, self included), once computed (ensure_dnn):
+    // CV sweeps weigh relative to that point's weight
+    double* dnn = nullptr;
 
     // device buffers
     double *px = nullptr, *py = nullptr, *hx = nullptr, *hy_ = nullptr, *hz = nullptr, *sl = nullptr;
@@ -155,6 +159,8 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
            std::map<double, double>& log);
 void write_trace(const std::map<double, double>& log, double* tx, double* tf, int cap, int* len);
 int max_distance_impl(gwm_hip_ctx* c, double* out);
+// c->dnn, computed on the context stream on first use.
+int ensure_dnn(gwm_hip_ctx* c);
 
 
 }  // namespace detail

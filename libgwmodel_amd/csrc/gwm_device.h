@@ -122,15 +122,32 @@ __device__ __forceinline__ double gcdist_f64(double lonf, double latf, double hx
 
 // Kernel weights with the CPU boundary (d < bw): BandwidthWeight.h:68-121.
 //
-// Gaussian and exponential weights whose exponent is below -kExpCut
-// (e^-72 = 5.4e-32 of the weight 1 at d = 0, i.e. beyond 12 and 72
-// bandwidths) are taken as zero.  The reference keeps them until exp
-// underflows (e^-745); dropping them moves every entry of X^T W X and X^T W y
-// by at most 5.4e-32 times the unweighted sum of its |z_j|, and lets the
-// sweeps skip points beyond 12 bandwidths instead of 40 exactly.
+// Gaussian and exponential weights whose exponent lies more than kExpCut
+// below the exponent of the focus point's largest weight are taken as zero
+// (e^-72 = 5.4e-32 of that weight).  The largest weight is the focus point's
+// own (1, at d = 0) in fits and AICc sweeps; in CV sweeps, whose own weight
+// is zeroed, and at prediction locations it is the weight of the nearest
+// other point (distance dnn), so the cut sits at exponent e(dnn) - 72.  The
+// reference keeps every weight until exp underflows (e^-745); dropping them
+// moves every entry of X^T W X and X^T W y by at most 5.4e-32 times the
+// largest weight times the unweighted sum of its |z_j|, and lets the sweeps
+// skip points beyond 12 bandwidths (sqrt(dnn^2 + 144 b^2) in general) instead
+// of 40.
 constexpr double kExpCut = 72.0;
 
 __device__ __forceinline__ double exp_cut(double e) { return e < -kExpCut ? 0.0 : exp(e); }
+
+// Exponent of the kernel weight at distance d (Gaussian and exponential), the
+// reference's expression (BandwidthWeight.h:68-79).
+__device__ __forceinline__ double kernel_exponent(int kernel, double d, double bw) {
+    return kernel == kGaussian ? (d * d) / ((-2.0) * (bw * bw)) : -d / bw;
+}
+
+// Mixed precision: a focus point whose largest weight is below this lies so
+// far from every other point that its FP16-split weights lose precision
+// (FP16 hi parts turn subnormal below 2^-14); such points are recomputed in
+// FP64.
+constexpr double kMixedMinWeight = 1.0 / 4096.0;
 
 __device__ __forceinline__ double kernel_f64(int kernel, double d, double bw) {
     switch (kernel) {
@@ -153,6 +170,7 @@ __device__ __forceinline__ double kernel_f64(int kernel, double d, double bw) {
 // the boundary tests d < bw are unchanged).
 struct KConst64 {
     double bw, gauss, inv_bw, inv_bw2, inv_bw3;   // gauss = 1 / (-2 bw^2)
+    double cut;   // exponents below cut give zero weights
 };
 
 __device__ __forceinline__ KConst64 make_kconst64(double bw) {
@@ -162,13 +180,48 @@ __device__ __forceinline__ KConst64 make_kconst64(double bw) {
     k.inv_bw = 1.0 / bw;
     k.inv_bw2 = 1.0 / (bw * bw);
     k.inv_bw3 = 1.0 / (bw * bw * bw);
+    k.cut = -kExpCut;
     return k;
+}
+
+// The constants of a focus point whose largest weight is that of a point at
+// distance dnn (CV sweeps, prediction locations): the exponent cut moves down
+// by the exponent of that weight.
+__device__ __forceinline__ KConst64 make_kconst64_nn(double bw, int kernel, double dnn) {
+    KConst64 k = make_kconst64(bw);
+    if (dnn > 0.0 && (kernel == kGaussian || kernel == kExponential))
+        k.cut = kernel_exponent(kernel, dnn, bw) - kExpCut;
+    return k;
+}
+
+// Largest kernel weight of a focus point whose nearest point lies at dnn,
+// without any cut.
+__device__ __forceinline__ double nearest_weight(int kernel, double dnn, double bw) {
+    switch (kernel) {
+    case kGaussian:
+    case kExponential: return exp(kernel_exponent(kernel, dnn, bw));
+    case kBisquare: {
+        double t = 1.0 - (dnn * dnn) / (bw * bw);
+        return dnn < bw ? t * t : 0.0;
+    }
+    case kTricube: {
+        double t = 1.0 - (dnn * dnn * dnn) / (bw * bw * bw);
+        return dnn < bw ? (t * t) * t : 0.0;
+    }
+    default: return dnn < bw ? 1.0 : 0.0;
+    }
 }
 
 __device__ __forceinline__ double kernel_fast_f64(int kernel, double d, const KConst64& k) {
     switch (kernel) {
-    case kGaussian: return exp_cut((d * d) * k.gauss);
-    case kExponential: return exp_cut(-d * k.inv_bw);
+    case kGaussian: {
+        const double e = (d * d) * k.gauss;
+        return e < k.cut ? 0.0 : exp(e);
+    }
+    case kExponential: {
+        const double e = -d * k.inv_bw;
+        return e < k.cut ? 0.0 : exp(e);
+    }
     case kBisquare: {
         double t = 1.0 - (d * d) * k.inv_bw2;
         return d < k.bw ? t * t : 0.0;
@@ -281,10 +334,14 @@ __device__ __forceinline__ float kernel_f32(int kernel, float d, float bw, float
 // parts that are both zero below 2^-36 (exp2 arguments below -37 keep a
 // margin).  The FP64 distances carry a relative margin of 1e-9 over the cut
 // for the rounding of the box gaps and of the exponent.  Never zero: infinity.
-__device__ __forceinline__ double zero_weight_distance(int kernel, double bw, bool mixed) {
+// dnn: the largest distance to a focus point's nearest other point among the
+// focus points considered (FP64 CV sweeps and predictions; 0 elsewhere), which
+// moves the FP64 exponent cut outwards (make_kconst64_nn).
+__device__ __forceinline__ double zero_weight_distance(int kernel, double bw, bool mixed, double dnn = 0.0) {
     switch (kernel) {
-    case kGaussian: return (mixed ? 7.25 : 12.0 * (1.0 + 1e-9)) * bw;   // d^2 / (2 b^2) > 26.3 (mixed) / 72 (FP64)
-    case kExponential: return (mixed ? 26.0 : kExpCut * (1.0 + 1e-9)) * bw;
+    case kGaussian:   // d^2 / (2 b^2) > 26.3 (mixed) / 72 + dnn^2 / (2 b^2) (FP64)
+        return mixed ? 7.25 * bw : sqrt(dnn * dnn + 144.0 * (bw * bw)) * (1.0 + 1e-9);
+    case kExponential: return mixed ? 26.0 * bw : (dnn + kExpCut * bw) * (1.0 + 1e-9);
     default: return bw * (mixed ? 1.0 + 1e-5 : 1.0 + 1e-9);
     }
 }
@@ -318,19 +375,22 @@ __device__ __forceinline__ double box_gap2(const double* lo, const double* hi, G
     return g2;
 }
 
-// Workgroup-wide box of the focus points and their largest bandwidth: every
-// thread of the 256-thread workgroup passes the box lo/hi of its own focus
-// points and their largest bandwidth b, and receives the workgroup's.
+// Workgroup-wide box of the focus points, their largest bandwidth and their
+// largest nearest-point distance: every thread of the 256-thread workgroup
+// passes the box lo/hi of its own focus points, their largest bandwidth b and
+// distance dn, and receives the workgroup's.
 template <int D>
-__device__ __forceinline__ void workgroup_box(double* lo, double* hi, double b, double* bmax) {
-    __shared__ double s_part[4][2 * D + 1];
-    double v[2 * D + 1];
+__device__ __forceinline__ void workgroup_box(double* lo, double* hi, double b, double* bmax, double dn = 0.0,
+                                              double* dnmax = nullptr) {
+    __shared__ double s_part[4][2 * D + 2];
+    double v[2 * D + 2];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         v[2 * d] = lo[d];
         v[2 * d + 1] = hi[d];
     }
     v[2 * D] = b;
+    v[2 * D + 1] = dn;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
 #pragma unroll
@@ -339,11 +399,12 @@ __device__ __forceinline__ void workgroup_box(double* lo, double* hi, double b, 
             v[2 * d + 1] = fmax(v[2 * d + 1], __shfl_xor(v[2 * d + 1], m));
         }
         v[2 * D] = fmax(v[2 * D], __shfl_xor(v[2 * D], m));
+        v[2 * D + 1] = fmax(v[2 * D + 1], __shfl_xor(v[2 * D + 1], m));
     }
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
 #pragma unroll
-        for (int e = 0; e < 2 * D + 1; ++e) s_part[wave][e] = v[e];
+        for (int e = 0; e < 2 * D + 2; ++e) s_part[wave][e] = v[e];
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -351,6 +412,8 @@ __device__ __forceinline__ void workgroup_box(double* lo, double* hi, double b, 
         hi[d] = fmax(fmax(s_part[0][2 * d + 1], s_part[1][2 * d + 1]), fmax(s_part[2][2 * d + 1], s_part[3][2 * d + 1]));
     }
     *bmax = fmax(fmax(s_part[0][2 * D], s_part[1][2 * D]), fmax(s_part[2][2 * D], s_part[3][2 * D]));
+    if (dnmax)
+        *dnmax = fmax(fmax(s_part[0][2 * D + 1], s_part[1][2 * D + 1]), fmax(s_part[2][2 * D + 1], s_part[3][2 * D + 1]));
 }
 
 // Job of batch entry t, given the prefix sums start[0..njobs] of the jobs'

@@ -128,6 +128,10 @@ __device__ __forceinline__ void solve_one(const SolveArgs& a, int t) {
             const double l = L[Sym<K>::low(p, p)];
             c = fmax(c, A[Sym<K>::low(p, p)] / (l * l));
         }
+        // mixed precision: a focus point far from every other point has only
+        // tiny weights, which the FP16 split does not resolve
+        if (a.dnn && !(nearest_weight(a.kernel, a.dnn[i], a.bw_pt ? a.bw_pt[i] : a.bw) >= kMixedMinWeight))
+            c = INFINITY;
         a.cond[i] = c;
     }
     chol_solve<K>(L, b, beta);
@@ -368,9 +372,10 @@ __global__ __launch_bounds__(1024) void reduce_batch_kernel(const ReduceJob* job
 __global__ __launch_bounds__(256) void solve_var_kernel(int mode, const double* part, int nseg, int PT, int colB,
                                                         int n, int n_pad, const double* x, const double* y,
                                                         const double* bw_pt, double bw, int kernel, double* res2,
-                                                        double* sii, double* beta, int* fail) {
+                                                        double* sii, double* beta, int* fail, const double* dnn) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (dnn && !(nearest_weight(kernel, dnn[i], bw_pt ? bw_pt[i] : bw) >= kMixedMinWeight)) atomicOr(fail, 2);
     double a = 0.0, b = 0.0;
     for (int g = 0; g < nseg; ++g) {
         a += part[((size_t)g * PT + 0) * n_pad + i];
@@ -378,7 +383,7 @@ __global__ __launch_bounds__(256) void solve_var_kernel(int mode, const double* 
     }
     (void)colB;
     if (!(a > 0.0)) {
-        *fail = 1;
+        atomicOr(fail, 1);
         if (res2) res2[i] = NAN;
         return;
     }
@@ -524,9 +529,9 @@ hipError_t launch_reduce(const double* a, const double* b, const double* c, cons
 
 hipError_t launch_solve_var(int mode, const double* part, int nseg, int PT, int colB, int n, int n_pad,
                             const double* x, const double* y, const double* bw_pt, double bw, int kernel,
-                            double* res2, double* sii, double* beta, int* fail, hipStream_t s) {
+                            double* res2, double* sii, double* beta, int* fail, hipStream_t s, const double* dnn) {
     hipLaunchKernelGGL(solve_var_kernel, dim3((n + 255) / 256), dim3(256), 0, s, mode, part, nseg, PT, colB, n, n_pad,
-                       x, y, bw_pt, bw, kernel, res2, sii, beta, fail);
+                       x, y, bw_pt, bw, kernel, res2, sii, beta, fail, dnn);
     return hipGetLastError();
 }
 

@@ -172,6 +172,27 @@ def test_kernel_closed_forms(oracle):
         assert np.all(np.diff(w) <= 1e-15)
 
 
+def test_kernel_midpoint_closed_forms(oracle):
+    """d = bw/2 and d = bw/4 separate every kernel's exponent and power: a
+    tricube written (t*t) instead of (t*t)*t, an exponential written
+    exp(-(d/b)^2) or a bisquare without its square each fail here."""
+    bw = 2.0
+    d = np.array([bw / 2, bw / 4])
+    expect = {oracle.GAUSSIAN: [math.exp(-1 / 8), math.exp(-1 / 32)],
+              oracle.EXPONENTIAL: [math.exp(-1 / 2), math.exp(-1 / 4)],
+              oracle.BISQUARE: [(1 - 1 / 4) ** 2, (1 - 1 / 16) ** 2],
+              oracle.TRICUBE: [(1 - 1 / 8) ** 3, (1 - 1 / 64) ** 3],
+              oracle.BOXCAR: [1.0, 1.0]}
+    for kern, ex in expect.items():
+        w = oracle.weight_vec(d, kern, False, bw)
+        assert np.allclose(w, ex, rtol=2e-16, atol=0), (kern, w, ex)
+    # adaptive: the threshold replaces bw (BandwidthWeight.cpp:35-61); with
+    # the distances 0, 1, 2, ..., bw = 3.5 resolves to 2.5
+    dist = np.arange(10, dtype=float)
+    w = oracle.weight_vec(dist, oracle.TRICUBE, True, 3.5)
+    assert np.allclose(w, np.where(dist < 2.5, (1 - (dist / 2.5) ** 3) ** 3, 0.0), rtol=1e-15, atol=0)
+
+
 def test_adaptive_threshold_vs_sort(oracle):
     rng = np.random.default_rng(3)
     coords = np.round(rng.uniform(0, 20, (300, 2)), 1)   # rounding creates ties
@@ -203,8 +224,26 @@ def test_ols_limit(oracle):
         assert np.allclose(f["qDiag"], 1 - np.diag(H), rtol=1e-8, atol=1e-12)
 
 
-def dense_gwr(coords, X, y, kern, adaptive, bw):
-    """Independent dense restatement: explicit W_i, C_i = (X^T W X)^-1 X^T W_i and S."""
+def dense_kernel(kern, d, b):
+    """The five kernels from their textbook definitions (numpy, not the
+    oracle's code): Gaussian exp(-d^2 / 2b^2), exponential exp(-d/b),
+    bisquare (1 - (d/b)^2)^2, tricube (1 - (d/b)^3)^3, boxcar 1, the last
+    three for d < b only."""
+    q = d / b
+    if kern == 0:
+        return np.exp(-0.5 * q * q)
+    if kern == 1:
+        return np.exp(-q)
+    if kern == 2:
+        return np.where(d < b, (1 - q * q) ** 2, 0.0)
+    if kern == 3:
+        return np.where(d < b, (1 - q ** 3) ** 3, 0.0)
+    return np.where(d < b, 1.0, 0.0)
+
+
+def dense_gwr(coords, X, y, kern, adaptive, bw, cv=False):
+    """Independent dense restatement: explicit W_i, C_i = (X^T W X)^-1 X^T W_i and S
+    (cv: the focus point's own weight zeroed, as the CV criterion does)."""
     n, k = X.shape
     D = np.sqrt(((coords[:, None, :] - coords[None, :, :]) ** 2).sum(-1))
     betas = np.zeros((n, k))
@@ -217,10 +256,9 @@ def dense_gwr(coords, X, y, kern, adaptive, bw):
             s = np.sort(d)
             b0 = int(math.floor(bw))
             b = s[b0 - 1] + (s[b0] - s[b0 - 1]) * (bw - b0)
-        if kern == 0:
-            w = np.exp(-0.5 * (d / b) ** 2)
-        else:  # bisquare
-            w = np.where(d < b, (1 - (d / b) ** 2) ** 2, 0.0)
+        w = dense_kernel(kern, d, b)
+        if cv:
+            w[i] = 0.0
         C = np.linalg.solve(X.T @ (w[:, None] * X), (w[:, None] * X).T)
         betas[i] = C @ y
         se[i] = (C * C).sum(1)
@@ -235,7 +273,10 @@ def test_brute_force_dense(oracle):
     coords = rng.uniform(0, 10, (n, 2))
     X = np.column_stack([np.ones(n), rng.normal(size=(n, 2))])
     y = X @ np.array([0.3, 1.0, -1.0]) + rng.normal(size=n)
-    for kern, adaptive, bw in [(oracle.GAUSSIAN, False, 3.0), (oracle.BISQUARE, True, 25.5)]:
+    for kern, adaptive, bw in [(oracle.GAUSSIAN, False, 3.0), (oracle.BISQUARE, True, 25.5),
+                               (oracle.EXPONENTIAL, False, 2.0), (oracle.TRICUBE, False, 6.0),
+                               (oracle.TRICUBE, True, 30.0), (oracle.BOXCAR, False, 7.0),
+                               (oracle.EXPONENTIAL, True, 12.0)]:
         b, se, shat, q, S = dense_gwr(coords, X, y, kern, adaptive, bw)
         f = oracle.fit_core(coords, X, y, kern, adaptive, bw, want_S=True)
         assert np.allclose(f["betas"], b, rtol=1e-10, atol=1e-12)
@@ -243,6 +284,26 @@ def test_brute_force_dense(oracle):
         assert np.allclose(f["shat"], shat, rtol=1e-10)
         assert np.allclose(f["qDiag"], q, rtol=1e-10)
         assert np.allclose(f["S"], S, rtol=1e-10, atol=1e-13)
+
+
+def test_brute_force_criteria_all_kernels(oracle):
+    """CV (self weight zeroed, GWRBasic.cpp:233-268) and AICc (:270-307) of the
+    dense restatement, for all five kernels, fixed and adaptive."""
+    rng = np.random.default_rng(9)
+    n = 45
+    coords = rng.uniform(0, 10, (n, 2))
+    X = np.column_stack([np.ones(n), rng.normal(size=(n, 2))])
+    y = X @ np.array([0.3, 1.0, -1.0]) + rng.normal(size=n)
+    cases = [(0, False, 2.5), (1, False, 1.5), (2, False, 6.0), (3, False, 6.5), (4, False, 8.0),
+             (0, True, 15.0), (1, True, 12.0), (2, True, 25.5), (3, True, 28.0), (4, True, 30.0)]
+    for kern, adaptive, bw in cases:
+        b, _, shat, _, _ = dense_gwr(coords, X, y, kern, adaptive, bw)
+        rss = float(np.sum((y - np.sum(X * b, 1)) ** 2))
+        aicc = n * math.log(rss / n) + n * math.log(2 * math.pi) + n * (n + shat[0]) / (n - 2 - shat[0])
+        bcv, _, _, _, _ = dense_gwr(coords, X, y, kern, adaptive, bw, cv=True)
+        cv = float(np.sum((y - np.sum(X * bcv, 1)) ** 2))
+        assert oracle.criterion(coords, X, y, kern, adaptive, bw, oracle.CRIT_AIC) == pytest.approx(aicc, rel=1e-11)
+        assert oracle.criterion(coords, X, y, kern, adaptive, bw, oracle.CRIT_CV) == pytest.approx(cv, rel=1e-11)
 
 
 # --------------------------------------------------------------- predict (GWRBasic.cpp:167-191)
@@ -259,10 +320,7 @@ def dense_predict(coords, X, y, loc, kern, adaptive, bw):
             s = np.sort(d)
             b0 = int(math.floor(bw))
             b = s[b0 - 1] + (s[b0] - s[b0 - 1]) * (bw - b0)
-        if kern == 0:
-            w = np.exp(-0.5 * (d / b) ** 2)
-        else:  # bisquare
-            w = np.where(d < b, (1 - (d / b) ** 2) ** 2, 0.0)
+        w = dense_kernel(kern, d, b)
         betas[i] = np.linalg.solve(X.T @ (w[:, None] * X), X.T @ (w * y))
     return betas
 
@@ -311,7 +369,8 @@ def test_predict_brute_force_dense(oracle):
     X = np.column_stack([np.ones(n), rng.normal(size=(n, 2))])
     y = X @ np.array([0.3, 1.0, -1.0]) + rng.normal(size=n)
     for kern, adaptive, bw in [(oracle.GAUSSIAN, False, 3.0), (oracle.BISQUARE, True, 25.5),
-                               (oracle.GAUSSIAN, True, 7)]:
+                               (oracle.GAUSSIAN, True, 7), (oracle.EXPONENTIAL, False, 2.0),
+                               (oracle.TRICUBE, True, 20.0), (oracle.BOXCAR, True, 24.0)]:
         p = oracle.predict(coords, X, y, loc, kern, adaptive, bw)
         assert np.allclose(p, dense_predict(coords, X, y, loc, kern, adaptive, bw), rtol=1e-10, atol=1e-12)
 
