@@ -116,6 +116,23 @@ int gwm_hip_indep_vars_criterion(gwm_hip_ctx* ctx, int has_intercept, const int*
 int gwm_hip_indep_vars_select(gwm_hip_ctx* ctx, int has_intercept, double threshold, int* selected, int* nselected,
                               int* model_vars, int* model_nvars, double* model_crit, int cap_models, int* nmodels);
 
+/* Allocation-free criterion.  No counterpart in the reference: a variant of
+ * gwm_hip_criterion for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The inputs are the caller's device arrays in the caller's
+ * order: coords n x 2, X n x k column-major, y n (no gwm_hip_set_data, no
+ * spatial sort, so no pruning); FP64 arithmetic only.  Every other device
+ * buffer lives in one static array of 32 MiB per device that the library
+ * holds until the process ends; problems whose tables do not fit (about
+ * n <= 12,000 at k = 8) are rejected with GWM_ERR_INVALID and a message
+ * naming the size.  Calls on one device are serialised by the library
+ * (they share its array); the call synchronises the context stream.
+ * Results agree with gwm_hip_criterion on the same data within FP64
+ * summation-order differences.  Status and *value as gwm_hip_criterion. */
+int gwm_hip_criterion_noalloc(gwm_hip_ctx* ctx, int n, int k, const double* d_coords, const double* d_X,
+                              const double* d_y, int distance, int kernel, int adaptive, double bw, int criterion,
+                              double* value);
+
 /* GWRMultiscale::setHasHatMatrix (GWRMultiscale.h:398) for gwm_hip_mgwr_run: keep
  * S0 and the per-variable operators (GWRMultiscale.cpp:163-168, 215-225,
  * 296-301) on the device, (k + 3) n^2 doubles (GWM_ERR_OOM when the device

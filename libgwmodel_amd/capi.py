@@ -42,6 +42,7 @@ SIGNATURES = {
     "gwm_hip_criterion": (_i, [_vp, _i, _i, _d, _i, _i, _dp]),
     "gwm_hip_fit": (_i, [_vp, _i, _i, _d, _i, _dp, _dp, _dp, _dp, _dp]),
     "gwm_hip_predict": (_i, [_vp, _i, _dp, _i, _i, _d, _i, _dp]),
+    "gwm_hip_criterion_noalloc": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _i, _d, _i, _dp]),
     "gwm_hip_mgwr_set_hat_matrix": (_i, [_vp, _i]),
     "gwm_hip_last_error_bound": (_i, [_vp, _dp]),
     "gwm_hip_set_autotune": (_i, [_vp, _i]),
@@ -190,6 +191,17 @@ class Context:
         self._check(self._lib.gwm_hip_fit(self.h, int(kernel), int(adaptive), float(bw), int(precision), _p(betas),
                                           _p(se), _p(shat), _p(q), _p(S)))
         return dict(betas=betas, betasSE_raw=se, shat=shat, qDiag=q, S=S)
+
+    def criterion_noalloc(self, n, k, d_coords, d_X, d_y, kernel, adaptive, bw, crit, distance=EUCLIDEAN):
+        """gwm_hip_criterion_noalloc on device arrays (integer device
+        addresses): returns (status, value); the library allocates nothing."""
+        v = np.zeros(1)
+        rc = self._lib.gwm_hip_criterion_noalloc(self.h, int(n), int(k), d_coords, d_X, d_y, int(distance),
+                                                 int(kernel), int(adaptive), float(bw), int(crit), _p(v))
+        return rc, float(v[0])
+
+    def last_error_message(self):
+        return (self._lib.gwm_hip_last_error(self.h) or b"").decode()
 
     def set_autotune(self, on=True):
         """Online autotuning of the sweep launch shapes and the longest-first work

@@ -1399,8 +1399,9 @@ int gwm_hip_create(int device, gwm_hip_ctx** out) {
     gwm_hip_ctx* c = new gwm_hip_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        noalloc_init(device) != GWM_OK) {
+        gwm_hip_destroy(c);
         return GWM_ERR_HIP;
     }
     *out = c;

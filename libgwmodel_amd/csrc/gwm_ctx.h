@@ -161,6 +161,9 @@ void write_trace(const std::map<double, double>& log, double* tx, double* tf, in
 int max_distance_impl(gwm_hip_ctx* c, double* out);
 // c->dnn, computed on the context stream on first use.
 int ensure_dnn(gwm_hip_ctx* c);
+// Looks up the device address of the allocation-free entry point's scratch
+// array on `device` (the current device) once per process (gwm_noalloc.hip).
+int noalloc_init(int device);
 
 
 }  // namespace detail
