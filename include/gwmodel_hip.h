@@ -237,6 +237,10 @@ int gwm_hip_last_error_bound(gwm_hip_ctx* ctx, double* bound);
  * workgroups longest first (a counting pass over the chunk boxes); disabling
  * autotuning also restores the plain order.  Batched server launches keep
  * the default shape and order. */
+/* enabled: 0 off (default shape, plain order), 1 on; 2..5 force shape 0..3 on
+ * every sweep -- 0 and 1: the default focus tiles per wave, by the coprime
+ * stride / in order; 2 and 3: one focus tile per wave, likewise -- so that
+ * tests can compare every shape bit for bit. */
 int gwm_hip_set_autotune(gwm_hip_ctx* ctx, int enabled);
 /* The tuned classes, one text line each (shape chosen, best time of every
  * shape in ms), NUL-terminated and truncated to cap bytes. */

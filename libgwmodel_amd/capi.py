@@ -206,7 +206,11 @@ class Context:
     def set_autotune(self, on=True):
         """Online autotuning of the sweep launch shapes and the longest-first work
         order of large pruned sweeps (results do not change)."""
-        self._check(self._lib.gwm_hip_set_autotune(self.h, int(on)))
+        self._check(self._lib.gwm_hip_set_autotune(self.h, int(bool(on))))
+
+    def force_shape(self, shape):
+        """Every sweep in launch shape 0..3 (testing: shapes give the same bits)."""
+        self._check(self._lib.gwm_hip_set_autotune(self.h, 2 + int(shape)))
 
     def last_error_bound(self):
         """Error bound the certified search assumes for the last mixed criterion value."""

@@ -385,8 +385,11 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
         GWM_HIP_CHECK(c, hipMemsetAsync(c->pairs_dev, 0, sizeof(unsigned long long), c->stream));
         j.pairs = c->pairs_dev;
     }
-    // launch shape: from the autotuner (results are the same for every shape)
-    if (c->autotune) {
+    // launch shape: forced, or from the autotuner (results are the same for every shape)
+    if (c->force_shape >= 0) {
+        L.linear_order = c->force_shape & 1;
+        L.shape_r = c->force_shape >= 2 ? 1 : 0;
+    } else if (c->autotune) {
         TuneKey key{c->dist, mode, mixed ? 1 : 0, nta, ntb, 0};
         while ((2 << key.nlog2) <= c->n) ++key.nlog2;
         int chosen;
@@ -1991,8 +1994,9 @@ int gwm_hip_last_error_bound(gwm_hip_ctx* c, double* bound) {
 }
 
 int gwm_hip_set_autotune(gwm_hip_ctx* c, int enabled) {
-    if (!c) return GWM_ERR_INVALID;
-    c->autotune = enabled != 0;
+    if (!c || enabled < 0 || enabled > 5) return GWM_ERR_INVALID;
+    c->autotune = enabled == 1;
+    c->force_shape = enabled >= 2 ? enabled - 2 : -1;
     return GWM_OK;
 }
 

@@ -181,6 +181,20 @@ __device__ __forceinline__ void split16(float w, _Float16& hi, _Float16& lo) {
     lo = (_Float16)((w - (float)hi) * kLoScale);
 }
 
+// w * w rounded to FP32 once.  The instruction selector otherwise folds the
+// product into one of its FP16 conversions (v_fma_mixlo_f16 rounds the exact
+// product to FP16) but not into the other (v_cvt_pk_f16_f32 of the FP32
+// product), so the hi part fed to the matrix core and the hi part the lo
+// residual is taken against come from two different roundings, and hi + lo
+// no longer represents w^2 (measured: up to 1e-3 relative error in sums of
+// w^2 z).  The empty register constraint makes the FP32 product one opaque
+// value that both conversions read.
+__device__ __forceinline__ float square_rounded(float w) {
+    float w2 = w * w;
+    asm volatile("" : "+v"(w2));
+    return w2;
+}
+
 template <int NTA, int NTB, int DIST, int MODE, int R>
 __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     __shared__ MShared<NTA, DIST, MODE> sh;
@@ -380,13 +394,13 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                 for (int e = 0; e < 8; ++e) {
                     float we = w[e];
                     if (MODE == kModeCV && j0 + rb + e == fidx[r]) we = 0.0f;
-                    const float wa = MODE == kModeQDiag ? we * we : we;
+                    const float wa = MODE == kModeQDiag ? square_rounded(we) : we;
                     _Float16 h, l;
                     split16(wa, h, l);
                     ah[e] = h;
                     al[e] = l;
                     if (NTB > 0) {
-                        split16(we * we, h, l);
+                        split16(square_rounded(we), h, l);
                         a2h[e] = h;
                         a2l[e] = l;
                     }

@@ -106,6 +106,9 @@ struct gwm_hip_ctx {
     bool async_mem = false;
     // online autotuning of single-problem sweep shapes (gwm_hip_set_autotune)
     bool autotune = true;
+    // a launch shape forced for every single-problem sweep (0..3, see the
+    // autotuner), or -1
+    int force_shape = -1;
     hipEvent_t tev0 = nullptr, tev1 = nullptr;
     void* blas = nullptr;
     double mgwr_shat[2] = {0.0, 0.0};

@@ -116,3 +116,37 @@ def test_predict_work_order_is_bit_identical(geographic):
             res[on] = [ctx.predict(loc, kern, ad, bw, prec) for kern, ad, bw in cases for prec in (capi.FP64, capi.MIXED)]
         for a, b in zip(res[True], res[False]):
             assert np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.MIXED])
+@pytest.mark.parametrize("geographic", [False, True])
+def test_forced_shapes_are_bit_identical(prec, geographic):
+    """Every launch shape, forced on every sweep: one or two focus tiles per
+    wave, stride or plain order.  CV and AICc criteria and fits (hat-mode and
+    qDiag sweeps), Euclidean and great circle, k = 8 (three column tiles: the
+    great-circle CV and qDiag sweeps default to one tile per wave, the hat
+    sweep to two), equal bit for bit."""
+    n = 3000
+    if geographic:
+        d = synth.sphere_box(n=n)
+        dist, bw, ad = capi.GREAT_CIRCLE, 400.0, 150.0
+    else:
+        d = synth.sphere_box(n=n)
+        d["coords"] = d["coords"] * 10.0   # planar coordinates, same design
+        dist, bw, ad = capi.EUCLIDEAN, 40.0, 150.0
+    with capi.Context(0) as ctx:
+        ctx.set_data(d["coords"], d["X"], d["y"], dist)
+        runs = []
+        for shape in range(4):
+            say("case: forced shape %d precision %d geographic %d" % (shape, prec, geographic))
+            ctx.force_shape(shape)
+            crit = [ctx.criterion(capi.GAUSSIAN, False, bw, capi.CRIT_CV, prec),
+                    ctx.criterion(capi.BISQUARE, True, ad, capi.CRIT_AIC, prec)]
+            fits = [_fit_arrays(ctx.fit(capi.GAUSSIAN, False, bw, prec)),
+                    _fit_arrays(ctx.fit(capi.BISQUARE, True, ad, prec))]
+            runs.append((crit, fits))
+        for crit, fits in runs[1:]:
+            assert crit == runs[0][0]
+            for f, f0 in zip(fits, runs[0][1]):
+                for a, b in zip(f, f0):
+                    assert np.array_equal(a, b)

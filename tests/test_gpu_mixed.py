@@ -1,12 +1,15 @@
 """Mixed-precision path against the FP64 path (both through the C ABI).
 
-Stated bounds: beta within 1e-6 relative (column-scaled: max|db| / max|b| per
-coefficient) of the FP64 path; criteria within 1e-7 relative; tr S within
-1e-6; the raw local SE diag(A^-1 B A^-1), which applies A^-1 twice to a
-w^2-weighted sum, and qDiag, a quadratic form x^T M x whose terms cancel,
-within 1e-4 (reported, not part of the beta budget); with
-certified golden-section decisions the selected bandwidth and every iterate
-equal the FP64 search's.
+Stated bounds, mixed against FP64:
+- beta: per focus point ||db_i|| / ||b_i|| <= 1e-6 (normwise) and, per
+  coefficient, max|db| / max|b| <= 1e-6 (column-scaled); element-wise
+  |db| / max(|b|, 1e-2 max|b|) <= 1e-4 (a coefficient near zero carries the
+  absolute error of its column);
+- criteria within 1e-7 relative (AICc) and the certified bound (CV);
+- tr S within 1e-6; the raw local SE diag(A^-1 B A^-1) within 5e-6 and qDiag
+  within 1e-6 relative, element by element;
+- with certified golden-section decisions the selected bandwidth and every
+  iterate equal the FP64 search's.
 """
 import sys
 
@@ -26,6 +29,28 @@ def say(msg):
 def col_rel(b, ref):
     scale = np.maximum(np.max(np.abs(ref), axis=0), 1e-300)
     return float(np.max(np.max(np.abs(b - ref), axis=0) / scale))
+
+
+def normwise_rel(b, ref):
+    return float(np.max(np.linalg.norm(b - ref, axis=1) / np.linalg.norm(ref, axis=1)))
+
+
+def elem_rel(b, ref, floor=1e-2):
+    scale = np.maximum(np.abs(ref), floor * np.max(np.abs(ref), axis=0))
+    return float(np.max(np.abs(b - ref) / scale))
+
+
+def check_fit(gm, g64):
+    """The stated bounds on one fit; returns the measured errors."""
+    e = dict(col=col_rel(gm["betas"], g64["betas"]), norm=normwise_rel(gm["betas"], g64["betas"]),
+             elem=elem_rel(gm["betas"], g64["betas"]),
+             se=float(np.max(np.abs(gm["betasSE_raw"] - g64["betasSE_raw"]) / np.abs(g64["betasSE_raw"]))),
+             qdiag=float(np.max(np.abs(gm["qDiag"] - g64["qDiag"]) / np.abs(g64["qDiag"]))),
+             shat=float(np.max(np.abs(gm["shat"] - g64["shat"]) / np.abs(g64["shat"]))))
+    say("errors: " + "  ".join("%s %.2e" % kv for kv in e.items()))
+    assert e["col"] <= 1e-6 and e["norm"] <= 1e-6 and e["elem"] <= 1e-4
+    assert e["se"] <= 5e-6 and e["qdiag"] <= 1e-6 and e["shat"] <= 1e-6
+    return e
 
 
 @pytest.fixture(scope="module")
@@ -54,17 +79,7 @@ def test_mixed_criteria_and_fit_euclidean(ctx, kernel, adaptive, bw):
         f64 = ctx.criterion(kernel, adaptive, bw, crit, capi.FP64)
         mix = ctx.criterion(kernel, adaptive, bw, crit, capi.MIXED)
         assert mix == pytest.approx(f64, rel=1e-7), (crit, mix, f64)
-    g64 = ctx.fit(kernel, adaptive, bw, capi.FP64)
-    gm = ctx.fit(kernel, adaptive, bw, capi.MIXED)
-    eb = col_rel(gm["betas"], g64["betas"])
-    es = col_rel(gm["betasSE_raw"], g64["betasSE_raw"])
-    eq = float(np.max(np.abs(gm["qDiag"] - g64["qDiag"]) / np.abs(g64["qDiag"])))
-    et = float(np.max(np.abs(gm["shat"] - g64["shat"]) / np.abs(g64["shat"])))
-    say("errors: beta %.2e  se_raw %.2e  qdiag %.2e  shat %.2e" % (eb, es, eq, et))
-    assert eb < 1e-6
-    assert et < 1e-6
-    assert eq < 1e-4
-    assert es < 1e-4
+    check_fit(ctx.fit(kernel, adaptive, bw, capi.MIXED), ctx.fit(kernel, adaptive, bw, capi.FP64))
 
 
 def test_mixed_great_circle_sphere_box(ctx):
@@ -81,6 +96,22 @@ def test_mixed_great_circle_sphere_box(ctx):
         g64 = ctx.fit(capi.GAUSSIAN, False, bw, capi.FP64)
         gm = ctx.fit(capi.GAUSSIAN, False, bw, capi.MIXED)
         assert col_rel(gm["betas"], g64["betas"]) < 1e-6, bw
+
+
+def test_mixed_great_circle_cv_se_qdiag(ctx):
+    """Great circle at the n = 100,000 workload's selected bandwidth, on a
+    20,000-point sample of the same box: CV criteria (one focus tile per wave
+    by default for k = 8), and the fit's SE and qDiag (the w^2-weighted hat
+    and qDiag sweeps), against FP64."""
+    say("case: mixed great-circle CV / SE / qDiag sphere_box n=20000")
+    d = synth.sphere_box(n=20000)
+    ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
+    for bw in [140.0, 400.0]:
+        f64 = ctx.criterion(capi.GAUSSIAN, False, bw, capi.CRIT_CV, capi.FP64)
+        mix = ctx.criterion(capi.GAUSSIAN, False, bw, capi.CRIT_CV, capi.MIXED)
+        assert abs(mix - f64) <= ctx.last_error_bound() <= 1e-6 * f64, (bw, mix, f64)
+        check_fit(ctx.fit(capi.GAUSSIAN, False, bw, capi.MIXED), ctx.fit(capi.GAUSSIAN, False, bw, capi.FP64))
+    check_fit(ctx.fit(capi.BISQUARE, True, 200.0, capi.MIXED), ctx.fit(capi.BISQUARE, True, 200.0, capi.FP64))
 
 
 def test_certified_search_matches_fp64(ctx):
