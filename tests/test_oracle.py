@@ -415,3 +415,18 @@ def test_indep_vars_criterion_is_ols_aicc(oracle, londonhp100):
     # a zero column makes the design singular: DBL_MAX
     Xd = np.column_stack([X, np.zeros(n)])
     assert oracle.indep_vars_criterion(Xd, y, [1, 4]) == sys.float_info.max
+
+
+def test_multiscale_fixture_inputs_match_generator():
+    """tests/golden/cfg3_mgwr_n3000.npz (the oracle's GWRMultiscale result at
+    n = 3000, written by tests/golden/make_cfg3_fixture.py) belongs to the
+    inputs the seeded generator produces today."""
+    import os
+    import sys
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, golden)
+    import make_cfg3_fixture as fx
+    ref = np.load(os.path.join(golden, "cfg3_mgwr_n3000.npz"))
+    d = fx.inputs()
+    assert np.array_equal(fx.fingerprint(d), ref["fingerprint"])
+    assert ref["betas"].shape == d["X"].shape and ref["bandwidths"].shape == (d["X"].shape[1],)
