@@ -26,7 +26,7 @@ TASK_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_gpu_task.cpp")
 TASK_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_gpu_task")
 
 SOURCES = ["gwm_contract.hip", "gwm_contract_mixed.hip", "gwm_select.hip", "gwm_solve.hip", "gwm_varsel.hip",
-           "gwm_mgwr_hat.hip", "gwm_plan.hip", "gwm_noalloc.hip", "gwm_api.cpp", "gwm_batch.cpp"]
+           "gwm_mgwr_hat.hip", "gwm_gemm.hip", "gwm_plan.hip", "gwm_noalloc.hip", "gwm_api.cpp", "gwm_batch.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -59,7 +59,7 @@ def build(verbose=False):
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + [
-        "-lpthread", "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+        "-lpthread", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))

@@ -6,7 +6,6 @@
 // memory is O(n * k^2) per problem.  All HIP failures are reported as status
 // codes; the device is never reset.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cfloat>
@@ -1274,18 +1273,6 @@ int hat_alloc(gwm_hip_ctx* c, MgwrHat& h) {
     h.S = h.M + h.nn;
     h.dinv = static_cast<double*>(h.scratch.p);
     h.part = h.dinv + n;
-    if (!c->blas) {
-        rocblas_handle b = nullptr;
-        if (rocblas_create_handle(&b) != rocblas_status_success) {
-            c->err = "rocblas_create_handle failed";
-            return GWM_ERR_HIP;
-        }
-        c->blas = b;
-    }
-    if (rocblas_set_stream(static_cast<rocblas_handle>(c->blas), c->stream) != rocblas_status_success) {
-        c->err = "rocblas_set_stream failed";
-        return GWM_ERR_HIP;
-    }
     return GWM_OK;
 }
 
@@ -1303,12 +1290,7 @@ int hat_step(gwm_hip_ctx* c, MgwrHat& h, int i, int kernel, int adaptive, double
                                         h.S, c->stream));
     GWM_HIP_CHECK(c, launch_sub(Ri, h.S0, h.M, h.nn, c->stream));
     GWM_HIP_CHECK(c, hipMemcpyAsync(Ri, h.S, h.nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    const double one = 1.0;
-    if (rocblas_dgemm(static_cast<rocblas_handle>(c->blas), rocblas_operation_none, rocblas_operation_none, n, n, n,
-                      &one, h.S, n, h.M, n, &one, Ri, n) != rocblas_status_success) {
-        c->err = "rocblas_dgemm failed";
-        return GWM_ERR_HIP;
-    }
+    GWM_HIP_CHECK(c, launch_dgemm_acc(n, h.S, h.M, Ri, c->stream));   // R_i = S_i M + S_i
     GWM_HIP_CHECK(c, launch_sub(Ri, h.M, h.S0, h.nn, c->stream));
     if ((rc = timer_end(c, "mgwr_hat"))) return rc;
     int fail = 0;
@@ -1421,7 +1403,6 @@ void gwm_hip_destroy(gwm_hip_ctx* c) {
     if (c->tev0) hipEventDestroy(c->tev0);
     if (c->tev1) hipEventDestroy(c->tev1);
     if (c->pairs_dev) hipFree(c->pairs_dev);
-    if (c->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(c->blas));
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }

@@ -236,6 +236,8 @@ hipError_t launch_hat_init(int dist, const Points& P, int k, const double* X, co
 hipError_t launch_var_smatrix(int dist, const Points& P, const double* x, const double* bw_pt, double bw, int kernel,
                               double* dinv, int* fail, double* S, hipStream_t s);
 hipError_t launch_sub(const double* a, const double* b, double* out, size_t count, hipStream_t s);
+// C += A B for n x n column-major FP64 matrices on the matrix cores (gwm_gemm.hip).
+hipError_t launch_dgemm_acc(int n, const double* A, const double* B, double* C, hipStream_t s);
 hipError_t launch_trace(const double* S, int n, double* part, double* out, hipStream_t s);
 
 hipError_t launch_compact(const double* cond, int n, double limit, int* list, int* count, hipStream_t s);

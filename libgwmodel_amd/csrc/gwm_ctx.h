@@ -98,8 +98,7 @@ struct gwm_hip_ctx {
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long* pairs_dev = nullptr;   // swept-pair counter (timing only)
-    // Multiscale runs: keep the hat matrix (gwm_hip_mgwr_set_hat_matrix); the
-    // rocBLAS handle of its products (created on first use); last {tr S, tr S^T S}
+    // Multiscale runs: keep the hat matrix (gwm_hip_mgwr_set_hat_matrix); last {tr S, tr S^T S}
     bool mgwr_hat = false;
     // problem buffers from the stream-ordered allocator on this context's
     // stream (server-owned contexts: no device-wide synchronisation at free)
@@ -110,7 +109,6 @@ struct gwm_hip_ctx {
     // autotuner), or -1
     int force_shape = -1;
     hipEvent_t tev0 = nullptr, tev1 = nullptr;
-    void* blas = nullptr;
     double mgwr_shat[2] = {0.0, 0.0};
     std::map<std::string, gwm::detail::KernelStat> stats;
     long long solves = 0;
