@@ -62,6 +62,21 @@ const char* gwm_hip_last_error(const gwm_hip_ctx* ctx);
 /* Cancellation: polled between sweeps (ITelegram::stop, include/gwmodelpp/Logger.h:76-81). */
 int gwm_hip_set_stop_callback(gwm_hip_ctx* ctx, int (*stop)(void* user), void* user);
 
+/* Messages and progress: the ITelegram interface (include/gwmodelpp/Logger.h:31-83)
+ * that Algorithm::mTelegram serves (GWM_LOG_INFO / GWM_LOG_ERROR /
+ * GWM_LOG_PROGRESS / GWM_LOG_PROGRESS_PERCENT, include/gwmodelpp/Algorithm.h:21-100).
+ * print(user, level, message): level is ITelegram::LogLevel (3 error, 6 info);
+ * messages carry the reference's tags: "#stage ", "#bandwidth-criterion ",
+ * "#variable-criterion ", "#initial-bandwidth ", "#back-fitting ".
+ * progress(user, current, total): total > 0 for "current of total"
+ * (variable selection, the end of a fit: one report per fit, not one per
+ * focus point, since the fit is one sweep); total = 0 for a fraction in
+ * current (criterion evaluations: exp(-|last - value|), back-fitting
+ * iterations: exp(-|criterion - threshold|)).  Calls come from the calling
+ * thread, inside the entry point.  NULL callbacks switch reporting off. */
+int gwm_hip_set_telegram(gwm_hip_ctx* ctx, void (*print)(void* user, int level, const char* message),
+                         void (*progress)(void* user, double current, double total), void* user);
+
 /* Problem data (copied to the device).  coords n x 2 (x,y or lon,lat in
  * degrees); X n x k; y n.  Replaces the H2D uploads of
  * CRSDistance::prepareCuda (src/gwmodelpp/spatialweight/CRSDistance.cpp:158-168)
@@ -132,6 +147,17 @@ int gwm_hip_indep_vars_select(gwm_hip_ctx* ctx, int has_intercept, double thresh
 int gwm_hip_criterion_noalloc(gwm_hip_ctx* ctx, int n, int k, const double* d_coords, const double* d_X,
                               const double* d_y, int distance, int kernel, int adaptive, double bw, int criterion,
                               double* value);
+
+/* One distance per variable for gwm_hip_mgwr_run: replaces
+ * SpatialMultiscaleAlgorithm::setSpatialWeights' distance part
+ * (include/gwmodelpp/SpatialMultiscaleAlgorithm.h:66, 84-87; GWRMultiscale.cpp:1476-1483):
+ * variable i's bandwidth searches and fits use distances[i]
+ * (GWM_EUCLIDEAN or GWM_GREAT_CIRCLE on the same coordinates); the initial
+ * all-variable search and fit use distances[0], as mInitSpatialWeight is
+ * spatial weight 0.  k = 0 clears the setting: every variable then uses the
+ * distance given to gwm_hip_set_data.  The run rejects a count other than
+ * the data's k. */
+int gwm_hip_mgwr_set_distances(gwm_hip_ctx* ctx, int k, const int* distances);
 
 /* GWRMultiscale::setHasHatMatrix (GWRMultiscale.h:398) for gwm_hip_mgwr_run: keep
  * S0 and the per-variable operators (GWRMultiscale.cpp:163-168, 215-225,

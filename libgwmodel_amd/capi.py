@@ -30,6 +30,9 @@ _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _d = ctypes.c_double
 STOP_CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
+PRINT_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p)
+PROGRESS_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_double, ctypes.c_double)
+LOG_ERR, LOG_INFO = 3, 6
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -38,12 +41,14 @@ SIGNATURES = {
     "gwm_hip_destroy": (None, [_vp]),
     "gwm_hip_last_error": (ctypes.c_char_p, [_vp]),
     "gwm_hip_set_stop_callback": (_i, [_vp, STOP_CALLBACK, _vp]),
+    "gwm_hip_set_telegram": (_i, [_vp, PRINT_CALLBACK, PROGRESS_CALLBACK, _vp]),
     "gwm_hip_set_data": (_i, [_vp, _i, _i, _dp, _dp, _dp, _i]),
     "gwm_hip_criterion": (_i, [_vp, _i, _i, _d, _i, _i, _dp]),
     "gwm_hip_fit": (_i, [_vp, _i, _i, _d, _i, _dp, _dp, _dp, _dp, _dp]),
     "gwm_hip_predict": (_i, [_vp, _i, _dp, _i, _i, _d, _i, _dp]),
     "gwm_hip_criterion_noalloc": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _i, _d, _i, _dp]),
     "gwm_hip_mgwr_set_hat_matrix": (_i, [_vp, _i]),
+    "gwm_hip_mgwr_set_distances": (_i, [_vp, _i, _ip]),
     "gwm_hip_last_error_bound": (_i, [_vp, _dp]),
     "gwm_hip_set_autotune": (_i, [_vp, _i]),
     "gwm_hip_autotune_report": (_i, [ctypes.c_char_p, _i]),
@@ -133,6 +138,7 @@ class Context:
         self.n = 0
         self.k = 0
         self._stop_cb = None
+        self._tele_cb = None
 
     def close(self):
         if getattr(self, "h", None):
@@ -160,6 +166,19 @@ class Context:
             return
         self._stop_cb = STOP_CALLBACK(lambda _u: 1 if fn() else 0)
         self._check(self._lib.gwm_hip_set_stop_callback(self.h, self._stop_cb, None))
+
+    def set_telegram(self, print_fn=None, progress_fn=None):
+        """ITelegram: print_fn(level, message) and progress_fn(current, total)
+        (total 0: current is a fraction); None switches a channel off."""
+        pcb = PRINT_CALLBACK(lambda _u, lv, msg: print_fn(lv, msg.decode())) if print_fn else PRINT_CALLBACK()
+        gcb = PROGRESS_CALLBACK(lambda _u, cur, tot: progress_fn(cur, tot)) if progress_fn else PROGRESS_CALLBACK()
+        self._tele_cb = (pcb, gcb)
+        self._check(self._lib.gwm_hip_set_telegram(self.h, pcb, gcb, None))
+
+    def mgwr_set_distances(self, distances):
+        """One distance per variable for mgwr_run (None or []: the data's)."""
+        d = np.ascontiguousarray(distances if distances is not None else [], dtype=np.int32)
+        self._check(self._lib.gwm_hip_mgwr_set_distances(self.h, len(d), _ipa(d)))
 
     def set_data(self, coords, X, y, distance=EUCLIDEAN):
         c = np.asfortranarray(coords, dtype=np.float64)

@@ -18,6 +18,29 @@ namespace {
 
 int stop_adapter(void* user) { return static_cast<ITelegram*>(user)->stop() ? 1 : 0; }
 
+// The library's messages and progress reports go to the algorithm's telegram
+// as GWM_LOG_INFO / GWM_LOG_ERROR / GWM_LOG_PROGRESS(_PERCENT) would send them
+// (include/gwmodelpp/Algorithm.h:21-100); the function and file names are the
+// facade's.
+void print_adapter(void* user, int level, const char* message) {
+    static_cast<ITelegram*>(user)->print(message, static_cast<ITelegram::LogLevel>(level), "fit", __FILE__);
+}
+
+void progress_adapter(void* user, double current, double total) {
+    ITelegram* t = static_cast<ITelegram*>(user);
+    if (total > 0.0) t->progress(static_cast<std::size_t>(current), static_cast<std::size_t>(total), "fit", __FILE__);
+    else t->progress(current, "fit", __FILE__);
+}
+
+void attach(gwm_hip_ctx* ctx, ITelegram* telegram) {
+    gwm_hip_set_stop_callback(ctx, &stop_adapter, telegram);
+    gwm_hip_set_telegram(ctx, &print_adapter, &progress_adapter, telegram);
+}
+
+void log_stage(ITelegram* telegram, const char* stage) {
+    telegram->print(std::string("#stage ") + stage, ITelegram::LogLevel::LOG_INFO, "fit", __FILE__);
+}
+
 [[noreturn]] void raise(gwm_hip_ctx* ctx, int rc) {
     std::string msg = ctx ? gwm_hip_last_error(ctx) : "";
     switch (rc) {
@@ -54,8 +77,9 @@ Mat GWRBasic::fit() {
         throw std::invalid_argument("coords must be n x 2, X n x k, y n");
     const int dist = distance_id(mSpatialWeight);
     BandwidthWeight* bw = mSpatialWeight.weight<BandwidthWeight>();
+    log_stage(mTelegram, "Initializing");
     HipContext ctx(mGpuId);
-    gwm_hip_set_stop_callback(ctx.get(), &stop_adapter, mTelegram);
+    attach(ctx.get(), mTelegram);
     int rc = gwm_hip_set_data(ctx.get(), (int)n, (int)k, mCoords.memptr(), mX.memptr(), mY.memptr(), dist);
     if (rc) raise(ctx.get(), rc);
     const int prec = mPrecision == Precision::Mixed ? GWM_MIXED : GWM_FP64;
@@ -134,11 +158,13 @@ Mat GWRBasic::predict(const Mat& locations) {
         throw std::invalid_argument("coords and locations must have 2 columns, X n x k, y n");
     const int dist = distance_id(mSpatialWeight);
     BandwidthWeight* bw = mSpatialWeight.weight<BandwidthWeight>();
+    log_stage(mTelegram, "Initialization");   // GWRBasic.cpp:132-144
     HipContext ctx(mGpuId);
-    gwm_hip_set_stop_callback(ctx.get(), &stop_adapter, mTelegram);
+    attach(ctx.get(), mTelegram);
     int rc = gwm_hip_set_data(ctx.get(), (int)n, (int)k, mCoords.memptr(), mX.memptr(), mY.memptr(), dist);
     if (rc) raise(ctx.get(), rc);
     const int prec = mPrecision == Precision::Mixed ? GWM_MIXED : GWM_FP64;
+    log_stage(mTelegram, "Prediction");
     Mat betas(m, k);
     rc = gwm_hip_predict(ctx.get(), (int)m, locations.memptr(), (int)bw->kernel(), bw->adaptive() ? 1 : 0,
                          bw->bandwidth(), prec, betas.memptr());
@@ -171,11 +197,13 @@ Mat GWRMultiscale::fit() {
     if (mSpatialWeights.size() != k || mBandwidthInitilize.size() != k || mBandwidthSelectionApproach.size() != k ||
         mPreditorCentered.size() != k || mBandwidthSelectThreshold.size() != k)
         throw std::invalid_argument("per-variable settings must have one entry per independent variable");
+    // one distance per variable; the initial search and fit use variable 0's
+    // (GWRMultiscale::setSpatialWeights, GWRMultiscale.cpp:1476-1483)
     const int dist = distance_id(mSpatialWeights[0]);
-    std::vector<int> kernels(k), adaptive(k), init(k), approach(k), centered(k);
+    std::vector<int> kernels(k), adaptive(k), init(k), approach(k), centered(k), dists(k);
     std::vector<double> bws(k);
     for (std::size_t i = 0; i < k; ++i) {
-        if (distance_id(mSpatialWeights[i]) != dist) throw std::invalid_argument("all distances must be alike");
+        dists[i] = distance_id(mSpatialWeights[i]);
         const BandwidthWeight* w = mSpatialWeights[i].weight<BandwidthWeight>();
         kernels[i] = (int)w->kernel();
         adaptive[i] = w->adaptive() ? 1 : 0;
@@ -184,10 +212,12 @@ Mat GWRMultiscale::fit() {
         approach[i] = mBandwidthSelectionApproach[i] == CV ? GWM_CRIT_CV : GWM_CRIT_AIC;
         centered[i] = mPreditorCentered[i] ? 1 : 0;
     }
+    log_stage(mTelegram, "Initializing");
     HipContext ctx(mGpuId);
-    gwm_hip_set_stop_callback(ctx.get(), &stop_adapter, mTelegram);
+    attach(ctx.get(), mTelegram);
     int rc = gwm_hip_set_data(ctx.get(), (int)n, (int)k, mCoords.memptr(), mX.memptr(), mY.memptr(), dist);
     if (rc) raise(ctx.get(), rc);
+    if ((rc = gwm_hip_mgwr_set_distances(ctx.get(), (int)k, dists.data()))) raise(ctx.get(), rc);
     gwm_hip_mgwr_set_hat_matrix(ctx.get(), mHasHatMatrix ? 1 : 0);
     mBetas = Mat(n, k);
     std::vector<double> bws_out(k);

@@ -36,6 +36,12 @@ int ntb_of(int k) { return (k * (k + 1) / 2 + 15) / 16; }
 namespace {
 
 const double kPi = 3.14159265358979323846;
+// ITelegram::LogLevel (include/gwmodelpp/Logger.h:38-48)
+constexpr int kLogErr = 3;
+constexpr int kLogInfo = 6;
+const std::string kStage = "#stage ";               // GWM_LOG_TAG_STAGE (Algorithm.h:102)
+const std::string kBackfit = "#back-fitting ";      // GWM_LOG_TAG_MGWR_BACKFITTING (GWRMultiscale.h:20)
+const std::string kInitialBw = "#initial-bandwidth ";   // GWM_LOG_TAG_MGWR_INITIAL_BW (GWRMultiscale.h:19)
 
 #define GWM_HIP_CHECK(ctx, expr)                                                         \
     do {                                                                                 \
@@ -63,6 +69,11 @@ void free_problem(gwm_hip_ctx* c) {
     dfree(c->px); dfree(c->py); dfree(c->hx); dfree(c->hy_); dfree(c->hz); dfree(c->sl);
     dfree(c->X); dfree(c->y); dfree(c->Z); dfree(c->G); dfree(c->part);
     dfree(c->boxes); dfree(c->chunk_bw); dfree(c->sel_left); dfree(c->sel_nleft); dfree(c->dnn);
+    for (auto& t : c->parked) {
+        dfree(t.boxes);
+        dfree(t.dnn);
+        t = gwm_hip_ctx::DistTables{};
+    }
     for (auto& e : c->bw_cache) dfree(e.buf);
     c->bw_cache.clear();
     dfree(c->bwi); dfree(c->res2); dfree(c->cond); dfree(c->sii); dfree(c->rlist); dfree(c->rcount);
@@ -263,7 +274,7 @@ int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw
         return GWM_ERR_INVALID;
     }
     for (auto& e : c->bw_cache)
-        if (e.bw == bw) {
+        if (e.bw == bw && e.dist == c->dist) {
             e.used = ++c->bw_clock;
             *bw_pt = e.buf;
             return GWM_OK;
@@ -276,7 +287,7 @@ int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw
     if (c->bw_cache.size() < cap) {
         double* buf = nullptr;
         GWM_HIP_CHECK(c, hipMalloc((void**)&buf, (size_t)c->n * sizeof(double)));
-        c->bw_cache.push_back(gwm_hip_ctx::BwEntry{bw, buf, 0});
+        c->bw_cache.push_back(gwm_hip_ctx::BwEntry{bw, c->dist, buf, 0});
         slot = &c->bw_cache.back();
     } else {
         slot = &c->bw_cache[0];
@@ -284,6 +295,7 @@ int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw
             if (e.used < slot->used) slot = &e;
     }
     slot->bw = NAN;   // not valid until the select succeeds
+    slot->dist = c->dist;
     timer_begin(c);
     GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, slot->buf, c->prune ? c->boxes : nullptr,
                                         c->sel_left, c->sel_nleft, c->stream));
@@ -770,7 +782,7 @@ namespace gwm {
 namespace detail {
 
 int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certify, double tol_rel,
-           double* xopt_out, std::map<double, double>& log) {
+           double* xopt_out, std::map<double, double>& log, const EvalReport& report) {
     double xU = upper, xL = lower;
     const double eps = 1e-4;
     const double R = (std::sqrt(5.0) - 1) / 2;
@@ -798,6 +810,7 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
                 rc = GWM_OK;
             }
         }
+        if (rc == GWM_OK && report) report(x, *fx);
         return rc;
     };
     auto record = [&](double x, double fx) {
@@ -837,6 +850,7 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
         f2 = f1;
         e2 = e1;
         b2 = b1;
+        if (report) report(x2, f2);
     } else if ((rc = eval(x2, e2, &f2, &b2))) {
         return rc;
     }
@@ -860,6 +874,7 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
                 f1 = f2;
                 e1 = e2;
                 b1 = b2;
+                if (report) report(x1, f1);
             } else {
                 e1 = !certify || sticky;
                 if ((rc = eval(x1, e1, &f1, &b1))) return rc;
@@ -876,6 +891,7 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
                 f2 = f1;
                 e2 = e1;
                 b2 = b1;
+                if (report) report(x2, f2);
             } else {
                 e2 = !certify || sticky;
                 if ((rc = eval(x2, e2, &f2, &b2))) return rc;
@@ -960,6 +976,96 @@ int ensure_dnn(gwm_hip_ctx* c) {
         return GWM_ERR_HIP;
     }
     return GWM_OK;
+}
+
+}  // namespace detail
+}  // namespace gwm
+
+namespace {
+
+// Point tables of distance d and its 64-point chunk boxes (into *boxes):
+// great circle needs the half unit vectors and sin(lat) of every point (built
+// once) and boxes of the half unit vectors; Euclidean boxes of x / y.
+int build_tables(gwm_hip_ctx* c, int d, double** boxes) {
+    const size_t N = c->n;
+    std::vector<double> cs(2 * N);
+    to_sorted(c, c->hcoords.data(), cs.data(), 2);
+    std::vector<double> box;
+    int rc;
+    if (d == GWM_GREAT_CIRCLE) {
+        std::vector<double> h(4 * N);
+        sphere_tables(N, cs.data(), cs.data() + N, h.data(), h.data() + N, h.data() + 2 * N, h.data() + 3 * N);
+        if (!c->hx) {
+            if ((rc = dalloc(c, &c->hx, N)) || (rc = dalloc(c, &c->hy_, N)) || (rc = dalloc(c, &c->hz, N)) ||
+                (rc = dalloc(c, &c->sl, N)))
+                return rc;
+            GWM_HIP_CHECK(c, hipMemcpyAsync(c->hx, h.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            GWM_HIP_CHECK(c, hipMemcpyAsync(c->hy_, h.data() + N, N * sizeof(double), hipMemcpyHostToDevice,
+                                            c->stream));
+            GWM_HIP_CHECK(c, hipMemcpyAsync(c->hz, h.data() + 2 * N, N * sizeof(double), hipMemcpyHostToDevice,
+                                            c->stream));
+            GWM_HIP_CHECK(c, hipMemcpyAsync(c->sl, h.data() + 3 * N, N * sizeof(double), hipMemcpyHostToDevice,
+                                            c->stream));
+        }
+        const double* xs[3] = {h.data(), h.data() + N, h.data() + 2 * N};
+        box = chunk_boxes(c->n, 3, xs);
+    } else {
+        const double* xs[2] = {cs.data(), cs.data() + N};
+        box = chunk_boxes(c->n, 2, xs);
+    }
+    if ((rc = dalloc(c, boxes, box.size()))) return rc;
+    GWM_HIP_CHECK(c, hipMemcpyAsync(*boxes, box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GWM_OK;
+}
+
+}  // namespace
+
+namespace gwm {
+namespace detail {
+
+int use_distance(gwm_hip_ctx* c, int d) {
+    if (d == c->dist) return GWM_OK;
+    if (d != GWM_EUCLIDEAN && d != GWM_GREAT_CIRCLE) {
+        c->err = "unknown distance";
+        return GWM_ERR_INVALID;
+    }
+    gwm_hip_ctx::DistTables& cur = c->parked[c->dist];
+    cur.boxes = c->boxes;
+    cur.max_dist = c->max_dist;
+    cur.dnn = c->dnn;
+    cur.built = true;
+    gwm_hip_ctx::DistTables next = c->parked[d];
+    if (!next.built) {
+        next = gwm_hip_ctx::DistTables{};
+        int rc = build_tables(c, d, &next.boxes);
+        if (rc) return rc;
+    }
+    c->parked[d] = gwm_hip_ctx::DistTables{};   // the active tables live in c
+    c->boxes = next.boxes;
+    c->max_dist = next.max_dist;
+    c->dnn = next.dnn;
+    c->dist = d;
+    return GWM_OK;
+}
+
+void tele_print(gwm_hip_ctx* c, int level, const std::string& message) {
+    if (c->print) c->print(c->tele_user, level, message.c_str());
+}
+
+void tele_progress(gwm_hip_ctx* c, double current, double total) {
+    if (c->progress) c->progress(c->tele_user, current, total);
+}
+
+// std::to_string formats (int for adaptive bandwidths, "%f" otherwise), as
+// IBandwidthSelectable::infoBandwidthCriterion (IBandwidthSelectable.h:30-49).
+std::string info_bandwidth_criterion(bool adaptive, double bw, double value) {
+    return std::string("#bandwidth-criterion ") + (adaptive ? std::to_string((int)bw) : std::to_string(bw)) + "," +
+           std::to_string(value);
+}
+
+std::string info_bandwidth_criterion(bool adaptive) {
+    return std::string("#bandwidth-criterion ") + (adaptive ? "adaptive" : "fixed") + ",criterion";
 }
 
 }  // namespace detail
@@ -1350,6 +1456,26 @@ int gram(gwm_hip_ctx* c, DevBuf& g) {
     return GWM_OK;
 }
 
+// Criterion evaluations reported as the reference's criterion functions do
+// (GWRBasic.cpp:477-484, GWRMultiscale.cpp:588-594): a finite value gives the
+// "#bandwidth-criterion <bw>,<value>" message and the progress fraction
+// exp(-|last - value|); a failed evaluation (a local design that is not SPD)
+// an error message.  golden() reports every evaluation, and a candidate
+// that coincides with the previous one (rounded adaptive candidates) once
+// more, as the reference evaluates it again.
+EvalReport telegram_report(gwm_hip_ctx* c, bool adaptive) {
+    if (!c->print && !c->progress) return EvalReport();
+    return [c, adaptive](double x, double fx) {
+        if (fx < DBL_MAX) {
+            tele_print(c, kLogInfo, info_bandwidth_criterion(adaptive, x, fx));
+            tele_progress(c, std::exp(-std::fabs(c->last_crit - fx)), 0.0);
+            c->last_crit = fx;
+        } else {
+            tele_print(c, kLogErr, "inv_sympd(): matrix is singular or not positive definite");
+        }
+    };
+}
+
 int check_vars(gwm_hip_ctx* c, int has_intercept, const int* vars, int nvars) {
     const int first = has_intercept ? 1 : 0;
     if (nvars < 0 || (nvars > 0 && !vars) || nvars + first < 1 || nvars + first > c->k) {
@@ -1416,6 +1542,16 @@ int gwm_hip_set_stop_callback(gwm_hip_ctx* c, int (*stop)(void*), void* user) {
     return GWM_OK;
 }
 
+int gwm_hip_set_telegram(gwm_hip_ctx* c, void (*print)(void*, int, const char*),
+                         void (*progress)(void*, double, double), void* user) {
+    if (!c) return GWM_ERR_INVALID;
+    c->print = print;
+    c->progress = progress;
+    c->tele_user = user;
+    c->last_crit = DBL_MAX;
+    return GWM_OK;
+}
+
 int gwm_hip_set_data(gwm_hip_ctx* c, int n, int k, const double* coords, const double* X, const double* y,
                      int distance) {
     if (!c) return GWM_ERR_INVALID;
@@ -1459,51 +1595,18 @@ int gwm_hip_set_data(gwm_hip_ctx* c, int n, int k, const double* coords, const d
         return rc;
     }
     GWM_HIP_CHECK(c, hipMemsetAsync(c->G, 0, (size_t)c->rows_pad * c->ntb * 16 * sizeof(double), c->stream));
-    // great circle: the half unit vectors (hx, hy, hz) and sin(lat) of every point
-    std::vector<double> hx, hy, hz, sl;
-    if (distance == GWM_GREAT_CIRCLE) {
-        hx.resize(N);
-        hy.resize(N);
-        hz.resize(N);
-        sl.resize(N);
-        sphere_tables(N, coords, coords + N, hx.data(), hy.data(), hz.data(), sl.data());
-    }
-    // spatial order, sorted point tables and the 64-point chunk boxes
+    // spatial order, sorted coordinates, then the distance's point tables and
+    // the 64-point chunk boxes
     c->perm = spatial_order(n, coords, coords + N);
     std::vector<double> cs(2 * N);
     to_sorted(c, coords, cs.data(), 2);
     GWM_HIP_CHECK(c, hipMemcpyAsync(c->px, cs.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     GWM_HIP_CHECK(c, hipMemcpyAsync(c->py, cs.data() + N, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    std::vector<double> box;
-    if (distance == GWM_GREAT_CIRCLE) {
-        if ((rc = dalloc(c, &c->hx, N)) || (rc = dalloc(c, &c->hy_, N)) || (rc = dalloc(c, &c->hz, N)) ||
-            (rc = dalloc(c, &c->sl, N))) {
-            free_problem(c);
-            return rc;
-        }
-        std::vector<double> h(4 * N);
-        to_sorted(c, hx.data(), h.data(), 1);
-        to_sorted(c, hy.data(), h.data() + N, 1);
-        to_sorted(c, hz.data(), h.data() + 2 * N, 1);
-        to_sorted(c, sl.data(), h.data() + 3 * N, 1);
-        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hx, h.data(), N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hy_, h.data() + N, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        GWM_HIP_CHECK(c, hipMemcpyAsync(c->hz, h.data() + 2 * N, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        GWM_HIP_CHECK(c, hipMemcpyAsync(c->sl, h.data() + 3 * N, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        const double* xs[3] = {h.data(), h.data() + N, h.data() + 2 * N};
-        box = chunk_boxes(n, 3, xs);
-        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    } else {
-        const double* xs[2] = {cs.data(), cs.data() + N};
-        box = chunk_boxes(n, 2, xs);
-    }
-    if ((rc = dalloc(c, &c->boxes, box.size())) || (rc = dalloc(c, &c->chunk_bw, (N + 63) / 64))) {
+    if ((rc = dalloc(c, &c->chunk_bw, (N + 63) / 64)) || (rc = build_tables(c, distance, &c->boxes))) {
         free_problem(c);
         return rc;
     }
-    GWM_HIP_CHECK(c, hipMemcpyAsync(c->boxes, box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice,
-                                    c->stream));
-    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->dist = c->dist_data = distance;
     return upload_design(c, X, y);
 }
 
@@ -1600,7 +1703,7 @@ int gwm_hip_select_bandwidth(gwm_hip_ctx* c, int kernel, int adaptive, int crit,
                 [&](double x, bool exact, double* fx, double* fb) {
                     return criterion_impl(c, kernel, adaptive, x, crit, exact ? GWM_FP64 : precision, fx, fb);
                 },
-                precision == GWM_MIXED, c->certify_rel, &xopt, log);
+                precision == GWM_MIXED, c->certify_rel, &xopt, log, telegram_report(c, adaptive != 0));
     write_trace(log, tx, tf, cap, len);
     if (rc) return rc;
     if (bw_out) *bw_out = xopt;
@@ -1616,12 +1719,20 @@ int gwm_hip_gwr_run(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int sel
     if (!betas || !betasSE || !shat || !diag) return GWM_ERR_INVALID;
     if (len) *len = 0;
     if (select_bw) {
+        tele_print(c, kLogInfo, kStage + "Bandwidth selection");
+        tele_print(c, kLogInfo, info_bandwidth_criterion(adaptive != 0));
         if ((rc = gwm_hip_select_bandwidth(c, kernel, adaptive, crit, precision, has_lower, lower, has_upper, upper,
                                            &bw, tx, tf, cap, len)))
             return rc;
     }
     if (bw_out) *bw_out = bw;
-    if ((rc = gwm_hip_fit(c, kernel, adaptive, bw, precision, betas, betasSE, shat, qDiag, S))) return rc;
+    tele_print(c, kLogInfo, kStage + "Model fitting");
+    if ((rc = gwm_hip_fit(c, kernel, adaptive, bw, precision, betas, betasSE, shat, qDiag, S))) {
+        if (rc == GWM_ERR_NOT_SPD) tele_print(c, kLogErr, "inv_sympd(): matrix is singular or not positive definite");
+        return rc;
+    }
+    tele_progress(c, c->n, c->n);
+    tele_print(c, kLogInfo, kStage + "Model Diagnostic");
     // CalcDiagnostic (GWRBasic.cpp:21-34) and SE scaling (:100-103)
     const int n_ = c->n, k = c->k;
     calc_diagnostic(n_, k, c->hX.data(), c->hy.data(), betas, shat, diag);
@@ -1690,6 +1801,15 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         return GWM_ERR_INVALID;
     GWM_HIP_CHECK(c, hipSetDevice(c->device));
     const int n = c->n, k = c->k;
+    if (!c->mgwr_dists.empty() && (int)c->mgwr_dists.size() != k) {
+        c->err = "mgwr: the per-variable distances were set for " + std::to_string(c->mgwr_dists.size()) +
+                 " variables, the data has " + std::to_string(k);
+        return GWM_ERR_INVALID;
+    }
+    // one distance per variable (GWRMultiscale.h:578, a SpatialWeight each);
+    // the initial all-variable search and fit use variable 0's
+    // (GWRMultiscale.cpp:1476-1483)
+    auto dist_of = [&](int v) { return c->mgwr_dists.empty() ? c->dist_data : c->mgwr_dists[v]; };
     const std::vector<double> X0 = c->hX, y = c->hy;
     // GWRMultiscale.cpp:88-99: centre predictors
     std::vector<double> Xc = X0;
@@ -1701,7 +1821,10 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
             for (int j = 0; j < n; ++j) Xc[j + (size_t)i * n] -= mu;
         }
     if ((rc = upload_design(c, Xc.data(), y.data()))) return rc;
-    auto restore = [&]() { upload_design(c, X0.data(), y.data()); };
+    auto restore = [&]() {
+        upload_design(c, X0.data(), y.data());
+        use_distance(c, c->dist_data);
+    };
     std::vector<int> init(init_in, init_in + k);
     std::vector<double> bw(bws, bws + k), maxD(k, 0.0);
     auto col = [&](int v) { return Xc.data() + (size_t)v * n; };
@@ -1709,22 +1832,33 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         restore();
         return code;
     };
-    // initial per-variable bandwidths (GWRMultiscale.cpp:104-135)
-    for (int i = 0; i < k; ++i) {
-        if (init[i] != 0) continue;
-        bool ad = adaptive[i] != 0;
-        if ((rc = max_distance_impl(c, &maxD[i]))) return fail_out(rc);
-        double lo = has_lower ? lower : (ad ? (double)adaptive_lower : maxD[i] / 5000.0);
-        double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
+    // a variable's bandwidth search, reported as GWRMultiscale.cpp:588-594
+    auto search_var = [&](int i, const double* yv, double lo, double hi, double* out) {
         std::map<double, double> log;
         const double* xi = col(i);
-        rc = golden(ad, lo, hi, [&](double x, bool exact, double* fx, double* fb) {
-            return var_impl(c, xi, y.data(), kernels[i], adaptive[i], x, approach[i], exact ? GWM_FP64 : precision, fx,
-                            nullptr, fb);
-        }, precision == GWM_MIXED, c->certify_rel, &bw[i], log);
-        if (rc) return fail_out(rc);
+        return golden(adaptive[i] != 0, lo, hi,
+                      [&](double x, bool exact, double* fx, double* fb) {
+                          return var_impl(c, xi, yv, kernels[i], adaptive[i], x, approach[i],
+                                          exact ? GWM_FP64 : precision, fx, nullptr, fb);
+                      },
+                      precision == GWM_MIXED, c->certify_rel, out, log, telegram_report(c, adaptive[i] != 0));
+    };
+    // initial per-variable bandwidths (GWRMultiscale.cpp:104-135)
+    tele_print(c, kLogInfo, kStage + "Calculating initial bandwidths");
+    for (int i = 0; i < k; ++i) {
+        if (stopped(c)) return fail_out(GWM_ERR_CANCELLED);
+        if (init[i] != 0) continue;
+        bool ad = adaptive[i] != 0;
+        if ((rc = use_distance(c, dist_of(i))) || (rc = max_distance_impl(c, &maxD[i]))) return fail_out(rc);
+        double lo = has_lower ? lower : (ad ? (double)adaptive_lower : maxD[i] / 5000.0);
+        double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
+        tele_print(c, kLogInfo, kInitialBw + std::to_string(i));
+        if ((rc = search_var(i, y.data(), lo, hi, &bw[i]))) return fail_out(rc);
+        tele_print(c, kLogInfo, kInitialBw + std::to_string(i) + "," + std::to_string(bw[i]));
     }
     // initial all-variable bandwidth (GWRMultiscale.cpp:137-160)
+    tele_print(c, kLogInfo, kStage + "Calculating initial bandwidth");
+    if ((rc = use_distance(c, dist_of(0)))) return fail_out(rc);
     double maxDist;
     if ((rc = max_distance_impl(c, &maxDist))) return fail_out(rc);
     bool ad0 = adaptive[0] != 0;
@@ -1732,10 +1866,17 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
     double hi0 = has_upper ? upper : (ad0 ? (double)n : maxDist);
     std::map<double, double> log0;
     double initBw = 0.0;
-    rc = golden(ad0, lo0, hi0, [&](double x, bool exact, double* fx, double* fb) {
-        return criterion_impl(c, kernels[0], adaptive[0], x, approach[0], exact ? GWM_FP64 : precision, fx, fb);
-    }, precision == GWM_MIXED, c->certify_rel, &initBw, log0);
+    tele_print(c, kLogInfo, info_bandwidth_criterion(ad0));
+    rc = golden(ad0, lo0, hi0,
+                [&](double x, bool exact, double* fx, double* fb) {
+                    return criterion_impl(c, kernels[0], adaptive[0], x, approach[0], exact ? GWM_FP64 : precision,
+                                          fx, fb);
+                },
+                precision == GWM_MIXED, c->certify_rel, &initBw, log0, telegram_report(c, ad0));
     if (rc) return fail_out(rc);
+    if (stopped(c)) return fail_out(GWM_ERR_CANCELLED);
+    tele_print(c, kLogInfo, kStage + "Model fitting");
+    tele_print(c, kLogInfo, kBackfit + "Model fitting with inital bandwidth");
     // backfitting (GWRMultiscale.cpp:208-317)
     std::vector<double> betas((size_t)n * k), resid(n), yi(n), colv(n);
     const bool hat = c->mgwr_hat;
@@ -1766,28 +1907,41 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
     double RSS1 = DBL_MAX, crit = DBL_MAX;
     std::vector<int> bwChangeNo(k, 0);
     int iteration = 1;
+    tele_print(c, kLogInfo, kBackfit + "Selecting the optimum bandwidths for each independent variable");
     for (; iteration <= max_iter && crit > crit_threshold; ++iteration) {
         if (stopped(c)) return fail_out(GWM_ERR_CANCELLED);
+        tele_print(c, kLogInfo, kBackfit + "#iteration " + std::to_string(iteration));
         for (int i = 0; i < k; ++i) {
             const double* xi = col(i);
             for (int j = 0; j < n; ++j) yi[j] = resid[j] + betas[j + (size_t)i * n] * xi[j];
+            if ((rc = use_distance(c, dist_of(i)))) return fail_out(rc);
             if (init[i] != 2) {
+                tele_print(c, kLogInfo, kBackfit + "#variable-bandwidth-selection " + std::to_string(i));
                 bool ad = adaptive[i] != 0;
+                // maxD[i] is set for variables searched at the start; a
+                // variable given an initial bandwidth searches from 0
+                // (mMaxDistances is zero-filled there, GWRMultiscale.cpp:74)
                 double lo = has_lower ? lower : (ad ? (double)adaptive_lower : maxD[i] / 5000.0);
                 double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
-                std::map<double, double> log;
                 double bw1 = 0.0;
-                rc = golden(ad, lo, hi, [&](double x, bool exact, double* fx, double* fb) {
-                    return var_impl(c, xi, yi.data(), kernels[i], adaptive[i], x, approach[i],
-                                    exact ? GWM_FP64 : precision, fx, nullptr, fb);
-                }, precision == GWM_MIXED, c->certify_rel, &bw1, log);
-                if (rc) return fail_out(rc);
-                if (std::fabs(bw1 - bw[i]) > bw_threshold[i]) bwChangeNo[i] = 0;
-                else {
+                if ((rc = search_var(i, yi.data(), lo, hi, &bw1))) return fail_out(rc);
+                // GWRMultiscale.cpp:263-290
+                std::string vbs = std::to_string(i) + "," + std::to_string(bw[i]) + "," + std::to_string(bw1) + "," +
+                                  std::to_string(std::fabs(bw1 - bw[i]));
+                if (std::fabs(bw1 - bw[i]) > bw_threshold[i]) {
+                    bwChangeNo[i] = 0;
+                    vbs += ",false";
+                } else {
                     bwChangeNo[i] += 1;
-                    if (bwChangeNo[i] >= retry_times) init[i] = 2;
+                    if (bwChangeNo[i] >= retry_times) {
+                        init[i] = 2;
+                        vbs += ",true";
+                    } else {
+                        vbs += "," + std::to_string(bwChangeNo[i]) + "," + std::to_string(retry_times - bwChangeNo[i]);
+                    }
                 }
                 bw[i] = bw1;
+                tele_print(c, kLogInfo, kBackfit + "#variable-bandwidth-selection " + vbs);
             }
             if ((rc = var_impl(c, xi, yi.data(), kernels[i], adaptive[i], bw[i], GWM_CRIT_AIC, precision, nullptr,
                                colv.data())))
@@ -1799,8 +1953,12 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         RSS1 = 0.0;
         for (int j = 0; j < n; ++j) RSS1 += resid[j] * resid[j];
         crit = crit_type == 0 ? std::fabs(RSS1 - RSS0) : std::sqrt(std::fabs(RSS1 - RSS0) / RSS1);
+        tele_print(c, kLogInfo, kBackfit + "#backfitting-criterion " + std::to_string(crit));
+        tele_progress(c, std::exp(-std::fabs(crit - crit_threshold)), 0.0);
         RSS0 = RSS1;
     }
+    tele_print(c, kLogInfo, kBackfit + "Finished");
+    tele_print(c, kLogInfo, kStage + "Model Diagnostic");
     if (iterations_out) *iterations_out = iteration - 1;
     // CalcDiagnostic (GWRMultiscale.cpp:41-66) with shat = {tr S0, tr S0^T S0}
     // ({0, 0} without the hat matrix, :176-179)
@@ -1886,6 +2044,9 @@ int gwm_hip_indep_vars_select(gwm_hip_ctx* c, int has_intercept, double threshol
     const int m = (int)vars.size();
     std::vector<int> cur, rest(vars);
     std::vector<std::pair<std::vector<int>, double>> models;
+    double progress_cur = 0.0;
+    tele_print(c, kLogInfo, kStage + "Independent variable selection");
+    tele_print(c, kLogInfo, "#variable-criterion variables,criterion");   // IVarialbeSelectable.h:27-30
     DevBuf g;
     if (m > 0 && (rc = gram(c, g))) return rc;
     for (int step = 0; step < m; ++step) {
@@ -1898,6 +2059,18 @@ int gwm_hip_indep_vars_select(gwm_hip_ctx* c, int has_intercept, double threshol
         std::vector<double> crit;
         if ((rc = varsel_criteria(c, static_cast<double*>(g.p), has_intercept, sets, crit))) return rc;
         for (size_t j = 0; j < sets.size(); ++j) models.emplace_back(sets[j], crit[j]);
+        // GWRBasic::indepVarsSelectionCriterion (GWRBasic.cpp:338-342) reports
+        // each candidate set in evaluation order
+        for (size_t j = 0; j < sets.size(); ++j) {
+            if (crit[j] == DBL_MAX) {
+                tele_print(c, kLogErr, "inv_sympd(): matrix is singular or not positive definite");
+                continue;
+            }
+            tele_progress(c, ++progress_cur, (double)m * (m + 1) / 2);
+            std::string vars_s;
+            for (size_t q = 0; q < sets[j].size(); ++q) vars_s += (q ? "+" : "") + std::to_string(sets[j][q]);
+            tele_print(c, kLogInfo, "#variable-criterion " + vars_s + "," + std::to_string(crit[j]));
+        }
         const size_t best = std::min_element(crit.begin(), crit.end()) - crit.begin();
         cur.push_back(rest[best]);
         rest.erase(rest.begin() + best);
@@ -1926,6 +2099,17 @@ int gwm_hip_indep_vars_select(gwm_hip_ctx* c, int has_intercept, double threshol
             model_vars[(size_t)q * k + a] = a < model_nvars[q] ? models[q].first[a] : -1;
         model_crit[q] = models[q].second;
     }
+    return GWM_OK;
+}
+
+int gwm_hip_mgwr_set_distances(gwm_hip_ctx* c, int k, const int* distances) {
+    if (!c || k < 0 || (k > 0 && !distances)) return GWM_ERR_INVALID;
+    for (int i = 0; i < k; ++i)
+        if (distances[i] != GWM_EUCLIDEAN && distances[i] != GWM_GREAT_CIRCLE) {
+            c->err = "mgwr distances: variable " + std::to_string(i) + " has an unknown distance";
+            return GWM_ERR_INVALID;
+        }
+    c->mgwr_dists.assign(distances, distances + k);
     return GWM_OK;
 }
 

@@ -54,6 +54,7 @@ struct gwm_hip_ctx {
     // fixed first candidates of every multiscale search)
     struct BwEntry {
         double bw;
+        int dist;   // the distance the thresholds were selected with
         double* buf;
         unsigned long long used;
     };
@@ -65,6 +66,24 @@ struct gwm_hip_ctx {
     // threshold of 2 neighbours, self included), once computed (ensure_dnn):
     // CV sweeps weigh relative to that point's weight
     double* dnn = nullptr;
+    // Distance-dependent tables.  `dist`, `boxes`, `max_dist` and `dnn` are
+    // those of the active distance; a multiscale run whose variables use
+    // different distances (gwm_hip_mgwr_set_distances) switches between
+    // them with use_distance, which parks the inactive distance's tables here.
+    int dist_data = 0;                  // the distance of gwm_hip_set_data
+    struct DistTables {
+        double* boxes = nullptr;
+        double max_dist = -1.0;
+        double* dnn = nullptr;
+        bool built = false;
+    };
+    DistTables parked[2];
+    std::vector<int> mgwr_dists;        // per-variable distances of the next multiscale run
+    // ITelegram (include/gwmodelpp/Logger.h:31-83): messages and progress
+    void (*print)(void*, int, const char*) = nullptr;
+    void (*progress)(void*, double, double) = nullptr;
+    void* tele_user = nullptr;
+    double last_crit = 1.7976931348623157e308;   // mBandwidthLastCriterion (DBL_MAX at start)
 
     // device buffers
     double *px = nullptr, *py = nullptr, *hx = nullptr, *hy_ = nullptr, *hz = nullptr, *sl = nullptr;
@@ -156,12 +175,24 @@ typedef std::function<int(double, bool, double*, double*)> CritFn;
 // Error bound of a mixed-precision criterion value (absolute); see gwm_api.cpp.
 double crit_error_bound(int crit, double value, double rss, double trS, double q, int n, double cond_max, double limit,
                         double tol_rel);
+// report(x, value) after each evaluation golden() uses (value DBL_MAX: failed)
+typedef std::function<void(double, double)> EvalReport;
 int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certify, double tol_rel, double* xopt_out,
-           std::map<double, double>& log);
+           std::map<double, double>& log, const EvalReport& report = EvalReport());
 void write_trace(const std::map<double, double>& log, double* tx, double* tf, int cap, int* len);
 int max_distance_impl(gwm_hip_ctx* c, double* out);
 // c->dnn, computed on the context stream on first use.
 int ensure_dnn(gwm_hip_ctx* c);
+// Make distance d the active one (building its point tables and boxes on
+// first use).
+int use_distance(gwm_hip_ctx* c, int d);
+// ITelegram::print (level: ITelegram::LogLevel) and progress (total > 0:
+// current of total; total = 0: current is a fraction), when a telegram is set.
+void tele_print(gwm_hip_ctx* c, int level, const std::string& message);
+void tele_progress(gwm_hip_ctx* c, double current, double total);
+// IBandwidthSelectable::infoBandwidthCriterion (IBandwidthSelectable.h:30-49)
+std::string info_bandwidth_criterion(bool adaptive, double bw, double value);
+std::string info_bandwidth_criterion(bool adaptive);
 // Looks up the device address of the allocation-free entry point's scratch
 // array on `device` (the current device) once per process (gwm_noalloc.hip).
 int noalloc_init(int device);

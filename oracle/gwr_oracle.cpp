@@ -496,6 +496,7 @@ struct Mgwr {
     std::vector<int> kernels;
     std::vector<int> adaptive;
     std::vector<double> bws;
+    std::vector<int> geo;   // per variable: great-circle distance (GWRMultiscale.h:578, one SpatialWeight each)
     int nthreads;
     bool has_hat;
     std::vector<double> S0;       // n*n
@@ -510,12 +511,14 @@ struct Mgwr {
     double all_criterion(int crit, int kernel, bool adaptive_, double bw) {
         Problem Q = *P;
         Q.X = Xc.data();
+        Q.geo = geo[0] != 0;   // the init spatial weight is spatial weight 0 (GWRMultiscale.cpp:1476-1483)
         return criterion(Q, crit, kernel, adaptive_, bw, nthreads, 0);
     }
 
     /* bandwidthSizeCriterionVar{CV,AIC}Omp (GWRMultiscale.cpp:560-636). */
-    double var_criterion(int crit, int kernel, bool adaptive_, double bw, const double* xi, const double* yi) {
+    double var_criterion(int v, int crit, int kernel, bool adaptive_, double bw, const double* xi, const double* yi) {
         const int n = P->n;
+        const bool g = geo[v] != 0;   // mSpatialWeights[v].distance() (GWRMultiscale.cpp:560-636)
         int T = nthreads_or_default(nthreads);
         std::vector<double> acc(2 * T, 0.0);
         std::vector<double> betas(n, 0.0);
@@ -527,7 +530,7 @@ struct Mgwr {
             for (int i = 0; i < n; ++i) {
                 if (!flag) continue;
                 int t = thread_id();
-                distance_vec(n, P->coords, P->geo, i, d.data());
+                distance_vec(n, P->coords, g, i, d.data());
                 weight_vec(d.data(), n, kernel, adaptive_, bw, w.data(), scratch);
                 if (crit == ORC_CRIT_CV) w[i] = 0.0;
                 double a = 0.0, b = 0.0;
@@ -580,7 +583,7 @@ struct Mgwr {
 #pragma omp for schedule(static)
             for (int i = 0; i < n; ++i) {
                 if (!success) continue;
-                distance_vec(n, P->coords, P->geo, i, d.data());
+                distance_vec(n, P->coords, geo[0] != 0, i, d.data());
                 weight_vec(d.data(), n, kernel, adaptive_, bw, w.data(), scratch);
                 normal_equations(n, k, Xc.data(), y, w.data(), A.data(), b.data());
                 try { inv_sympd(A.data(), k); } catch (const NotSpd&) { success = false; continue; }
@@ -627,7 +630,7 @@ struct Mgwr {
 #pragma omp for schedule(static)
             for (int i = 0; i < n; ++i) {
                 if (!success) continue;
-                distance_vec(n, P->coords, P->geo, i, d.data());
+                distance_vec(n, P->coords, geo[v] != 0, i, d.data());
                 weight_vec(d.data(), n, kernel, adapt, bw, w.data(), scratch);
                 double a = 0.0, b = 0.0;
                 for (int j = 0; j < n; ++j) {
@@ -885,7 +888,7 @@ int orc_gwr_run(int n, int k, const double* coords, const double* X, const doubl
 }
 
 int orc_mgwr_run(int n, int k, const double* coords, const double* X, const double* y,
-                 int geographic, const int* kernels, const int* adaptive, const double* bws,
+                 const int* geographic, const int* kernels, const int* adaptive, const double* bws,
                  const int* init_type_in, const int* approach, const int* centered,
                  const double* bw_threshold, int has_intercept, int has_hat,
                  int crit_type, double crit_threshold, int max_iter, int retry_times,
@@ -893,9 +896,10 @@ int orc_mgwr_run(int n, int k, const double* coords, const double* X, const doub
                  int nthreads,
                  double* betas_out, double* bws_out, double* diag, double* rss0_out,
                  int* iterations_out) {
-    Problem P{n, k, coords, X, y, geographic != 0};
+    Problem P{n, k, coords, X, y, geographic[0] != 0};
     Mgwr M;
     M.P = &P;
+    M.geo.assign(geographic, geographic + k);
     M.kernels.assign(kernels, kernels + k);
     M.adaptive.assign(adaptive, adaptive + k);
     M.bws.assign(bws, bws + k);
@@ -919,17 +923,17 @@ int orc_mgwr_run(int n, int k, const double* coords, const double* X, const doub
         for (int i = 0; i < k; ++i) {
             if (init_type[i] != 0) continue;
             bool adapt = adaptive[i] != 0;
-            maxDistances[i] = orc_max_distance(n, coords, geographic, nthreads);
+            maxDistances[i] = orc_max_distance(n, coords, geographic[i], nthreads);   // GWRMultiscale.cpp:117
             double lo = has_lower ? lower : (adapt ? (double)adaptive_lower : maxDistances[i] / 5000.0);
             double hi = has_upper ? upper : (adapt ? (double)n : maxDistances[i]);
             std::map<double, double> log;
             const double* xi = M.col(i);
             M.bws[i] = golden_section(adapt, lo, hi, [&](double x) {
-                return M.var_criterion(approach[i], M.kernels[i], adapt, x, xi, y);
+                return M.var_criterion(i, approach[i], M.kernels[i], adapt, x, xi, y);
             }, log);
         }
         /* GWRMultiscale.cpp:137-160: initial all-variable bandwidth. */
-        double maxDist = orc_max_distance(n, coords, geographic, nthreads);
+        double maxDist = orc_max_distance(n, coords, geographic[0], nthreads);   // :147, spatial weight 0
         bool adapt0 = adaptive[0] != 0;
         double lo = has_lower ? lower : (adapt0 ? (double)adaptive_lower : maxDist / 5000.0);
         double hi = has_upper ? upper : (adapt0 ? (double)n : maxDist);
@@ -984,7 +988,7 @@ int orc_mgwr_run(int n, int k, const double* coords, const double* X, const doub
                     std::map<double, double> log;
                     double bw0 = M.bws[i];
                     double bw1 = golden_section(adapt, lo2, hi2, [&](double x) {
-                        return M.var_criterion(approach[i], M.kernels[i], adapt, x, xi, yi.data());
+                        return M.var_criterion(i, approach[i], M.kernels[i], adapt, x, xi, yi.data());
                     }, log);
                     if (std::fabs(bw1 - bw0) > bw_threshold[i]) bwChangeNo[i] = 0;
                     else {

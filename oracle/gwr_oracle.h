@@ -88,12 +88,14 @@ int orc_gwr_run(int n, int k, const double* coords, const double* X, const doubl
                 int* trace_len);
 
 /* GWRMultiscale::fit() (GWRMultiscale.cpp:68-317) restated.
- * Per-variable arrays have k entries. init_type: 0 Null, 1 Initial, 2 Specified.
+ * Per-variable arrays have k entries, geographic included (one SpatialWeight,
+ * hence one distance, per variable; the initial all-variable search and fitAll
+ * use variable 0's, GWRMultiscale.cpp:1476-1483). init_type: 0 Null, 1 Initial, 2 Specified.
  * approach: criterion ids as above. crit_type: 0 CVR, 1 dCVR.
  * Outputs: betas n*k, bws_out k, diag[7] (AIC left 0 as the reference leaves it unset),
  * rss0_out, iterations_out. Returns 0 ok, 1 non-SPD, 2 invalid initial values. */
 int orc_mgwr_run(int n, int k, const double* coords, const double* X, const double* y,
-                 int geographic, const int* kernels, const int* adaptive, const double* bws,
+                 const int* geographic, const int* kernels, const int* adaptive, const double* bws,
                  const int* init_type, const int* approach, const int* centered,
                  const double* bw_threshold, int has_intercept, int has_hat,
                  int crit_type, double crit_threshold, int max_iter, int retry_times,

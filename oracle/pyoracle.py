@@ -55,7 +55,7 @@ def lib():
                                   ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, _ip]
-        L.orc_mgwr_run.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
+        L.orc_mgwr_run.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip,
                                    _ip, _ip, _dp, _ip, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
@@ -210,6 +210,8 @@ def mgwr_run(coords, X, y, kernels, adaptive, bws, init_type, approach, centered
     n, k = Xc.shape
     ia = lambda v: np.ascontiguousarray(v, dtype=np.int32)
     kern, ad, it, ap, ce = ia(kernels), ia(adaptive), ia(init_type), ia(approach), ia(centered)
+    # one distance per variable (a single flag applies to all)
+    geo = ia(geographic if np.ndim(geographic) else [int(bool(geographic))] * k)
     bw = _f64(bws)
     thr = _f64(bw_threshold)
     betas = np.zeros((n, k), order="F")
@@ -217,7 +219,7 @@ def mgwr_run(coords, X, y, kernels, adaptive, bws, init_type, approach, centered
     diag = np.zeros(7)
     rss0 = np.zeros(1)
     iters = np.zeros(1, dtype=np.int32)
-    rc = lib().orc_mgwr_run(n, k, _p(c), _p(Xc), _p(yc), int(geographic), _ip_arr(kern), _ip_arr(ad), _p(bw),
+    rc = lib().orc_mgwr_run(n, k, _p(c), _p(Xc), _p(yc), _ip_arr(geo), _ip_arr(kern), _ip_arr(ad), _p(bw),
                             _ip_arr(it), _ip_arr(ap), _ip_arr(ce), _p(thr), int(has_intercept), int(has_hat),
                             int(crit_type), float(crit_threshold), int(max_iter), int(retry_times),
                             int(adaptive_lower), int(lower is not None), float(lower or 0.0),

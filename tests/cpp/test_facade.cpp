@@ -54,6 +54,24 @@ public:
     bool stop() override { return true; }
 };
 
+// Records what the algorithm reports through ITelegram.
+class Recorder : public Logger {
+public:
+    std::vector<std::string> messages;
+    std::vector<ITelegram::LogLevel> levels;
+    int percent_calls = 0, count_calls = 0;
+    void print(std::string message, LogLevel level, std::string, std::string) override {
+        messages.push_back(message);
+        levels.push_back(level);
+    }
+    void progress(std::size_t, std::size_t, std::string, std::string) override { ++count_calls; }
+    void progress(double percent, std::string, std::string) override {
+        if (percent >= 0.0 && percent <= 1.0) ++percent_calls;
+    }
+};
+
+static bool starts_with(const std::string& s, const std::string& p) { return s.compare(0, p.size(), p) == 0; }
+
 int main(int argc, char** argv) {
     std::string dir = argc > 1 ? argv[1] : "tests/golden";
     Mat coords = read_csv(dir + "/londonhp100coords.csv");
@@ -137,10 +155,41 @@ int main(int argc, char** argv) {
         algorithm.setBandwidthSelectionCriterion(GWRBasic::BandwidthSelectionCriterionType::CV);
         algorithm.setParallelType(ParallelType::HIP);
         algorithm.setPrecision(prec ? Precision::Mixed : Precision::FP64);
+        Recorder telegram;
+        algorithm.setTelegram(&telegram);
         algorithm.fit();
         std::size_t bw = (std::size_t)algorithm.spatialWeight().weight<BandwidthWeight>()->bandwidth();
         REQUIRE(bw == 67);
-        REQUIRE(!algorithm.bandwidthSelectionCriterionList().empty());
+        const BandwidthCriterionList& list = algorithm.bandwidthSelectionCriterionList();
+        REQUIRE(!list.empty());
+        // GWRBasic.cpp:38-102: the stage tags, the criterion header, one
+        // message per evaluation, then fitting and diagnostics
+        const std::vector<std::string>& m = telegram.messages;
+        REQUIRE(m.size() >= 7);
+        if (m.size() >= 7) {
+            REQUIRE(m[0] == "#stage Initializing");
+            REQUIRE(m[1] == "#stage Bandwidth selection");
+            REQUIRE(m[2] == "#bandwidth-criterion adaptive,criterion");
+            REQUIRE(m[m.size() - 2] == "#stage Model fitting");
+            REQUIRE(m[m.size() - 1] == "#stage Model Diagnostic");
+            std::size_t evals = 0;
+            bool seen67 = false;
+            for (std::size_t q = 3; q + 2 < m.size(); ++q) {
+                REQUIRE(starts_with(m[q], "#bandwidth-criterion "));
+                const std::string body = m[q].substr(21);
+                const std::string bws = body.substr(0, body.find(','));
+                bool listed = false;
+                for (const auto& e : list) listed = listed || std::to_string((int)e.first) == bws;
+                REQUIRE(listed);
+                seen67 = seen67 || bws == "67";
+                ++evals;
+            }
+            REQUIRE(seen67);
+            REQUIRE(evals >= list.size());
+            REQUIRE(telegram.percent_calls == (int)evals);
+            REQUIRE(telegram.count_calls == 1);   // the fit
+            for (auto lv : telegram.levels) REQUIRE(lv == ITelegram::LogLevel::LOG_INFO);
+        }
     }
     for (int optimize_bw = 0; optimize_bw < 2; ++optimize_bw) {
         // adaptive bandwidth | {no, CV} bandwidth optimization | AIC variable optimization
@@ -259,6 +308,44 @@ int main(int argc, char** argv) {
         REQUIRE_NEAR(d.AICc, 2437.09277417389, 1e-6);
         REQUIRE_NEAR(d.RSquare, 0.744649364494, 1e-6);
         REQUIRE_NEAR(d.RSquareAdjust, 0.712344894394, 1e-6);
+    }
+    {   // MGWR with one distance per variable: a great-circle variable among
+        // Euclidean ones (GWRMultiscale.h:578); coordinates read as degrees
+        Mat ll(n, 2);
+        for (std::size_t i = 0; i < n; ++i) {   // londonhp100 metres to a lon/lat box
+            ll(i, 0) = -0.5 + (coords(i, 0) - 500000.0) / 70000.0;
+            ll(i, 1) = 51.3 + (coords(i, 1) - 155000.0) / 111000.0;
+        }
+        std::vector<SpatialWeight> spatials;
+        for (int i = 0; i < 3; ++i) {
+            CRSDistance distance(i == 1);
+            BandwidthWeight bandwidth(0, true, BandwidthWeight::Bisquare);
+            spatials.push_back(SpatialWeight(&bandwidth, &distance));
+        }
+        GWRMultiscale algorithm;
+        Recorder telegram;
+        algorithm.setTelegram(&telegram);
+        algorithm.setCoords(ll);
+        algorithm.setDependentVariable(y);
+        algorithm.setIndependentVariables(xm);
+        algorithm.setSpatialWeights(spatials);
+        algorithm.setHasHatMatrix(false);
+        algorithm.setPreditorCentered({false, true, true});
+        algorithm.setBandwidthInitilize(std::vector<GWRMultiscale::BandwidthInitilizeType>(
+            3, GWRMultiscale::BandwidthInitilizeType::Null));
+        algorithm.setBandwidthSelectionApproach(std::vector<GWRMultiscale::BandwidthSelectionCriterionType>(
+            3, GWRMultiscale::BandwidthSelectionCriterionType::CV));
+        algorithm.setBandwidthSelectThreshold(std::vector<double>(3, 1e-5));
+        algorithm.setParallelType(ParallelType::HIP);
+        algorithm.fit();
+        REQUIRE(algorithm.status() == Status::Success);
+        for (int i = 0; i < 3; ++i) {
+            const double b = algorithm.spatialWeights()[i].weight<BandwidthWeight>()->bandwidth();
+            REQUIRE(b >= 10 && b <= (double)n);
+        }
+        const std::vector<std::string>& m = telegram.messages;
+        REQUIRE(!m.empty() && m[0] == "#stage Initializing");
+        REQUIRE(!m.empty() && m.back() == "#stage Model Diagnostic");
     }
     std::printf(failures ? "FACADE TESTS FAILED: %d\n" : "FACADE TESTS PASSED\n", failures);
     return failures ? 1 : 0;

@@ -97,6 +97,33 @@ def test_kat_mgwr_bounds(oracle, londonhp100):
     assert list(r["bandwidths"]) == [52, 99, 99]
 
 
+def test_mgwr_per_variable_distance_equator_scaling(oracle):
+    """One distance per variable (a SpatialWeight each, GWRMultiscale.h:578).
+    On the equator the great-circle distance is a * dlon (radians), so a
+    great-circle variable with fixed bandwidth b km weighs like a Euclidean
+    (degree) variable with bandwidth b / (a pi / 180): the two runs agree.
+    Variable 0 (the initial all-variable search) stays Euclidean in both.  The
+    same run with the flag ignored (b km applied to degrees) must differ."""
+    rng = np.random.default_rng(11)
+    n = 150
+    lon = np.sort(rng.uniform(0.0, 20.0, n))
+    coords = np.column_stack([lon, np.zeros(n)])
+    X = np.column_stack([np.ones(n), rng.normal(size=n), rng.normal(size=n)])
+    y = 1.0 + np.sin(lon / 3) * X[:, 1] + 0.5 * X[:, 2] + 0.1 * rng.normal(size=n)
+    c = 6378.137 * math.pi / 180
+    common = dict(init_type=[2] * 3, approach=[oracle.CRIT_CV] * 3, centered=[0, 1, 1], bw_threshold=[1e-5] * 3,
+                  has_hat=True)
+    kern = [oracle.BISQUARE] * 3
+    g = oracle.mgwr_run(coords, X, y, kern, [0] * 3, [4.0, 6.0 * c, 5.0], geographic=[0, 1, 0], **common)
+    e = oracle.mgwr_run(coords, X, y, kern, [0] * 3, [4.0, 6.0, 5.0], geographic=[0, 0, 0], **common)
+    assert g["iterations"] == e["iterations"]
+    assert np.allclose(g["betas"], e["betas"], rtol=1e-10, atol=1e-12)
+    for key in ["RSS", "AICc", "ENP", "RSquare"]:
+        assert g["diagnostic"][key] == pytest.approx(e["diagnostic"][key], rel=1e-10), key
+    wrong = oracle.mgwr_run(coords, X, y, kern, [0] * 3, [4.0, 6.0 * c, 5.0], geographic=[0, 0, 0], **common)
+    assert np.max(np.abs(wrong["betas"][:, 1] - e["betas"][:, 1])) > 1e-2
+
+
 # --------------------------------------------------------------- great circle
 
 def vincenty_km(lon1, lat1, lon2, lat2):
