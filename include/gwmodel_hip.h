@@ -78,7 +78,9 @@ int gwm_hip_set_telegram(gwm_hip_ctx* ctx, void (*print)(void* user, int level, 
                          void (*progress)(void* user, double current, double total), void* user);
 
 /* Problem data (copied to the device).  coords n x 2 (x,y or lon,lat in
- * degrees); X n x k; y n.  Replaces the H2D uploads of
+ * degrees); X n x k; y n.  1 <= k <= 16: the reference takes any k; here the
+ * per-focus k x k solve lives in registers and the product table in at most
+ * 16 MFMA column tiles, so k > 16 is rejected with GWM_ERR_INVALID.  Replaces the H2D uploads of
  * CRSDistance::prepareCuda (src/gwmodelpp/spatialweight/CRSDistance.cpp:158-168)
  * and of fitCuda / bandwidthSizeCriterion*Cuda (GWRBasic.cpp:585-600, 709-720). */
 int gwm_hip_set_data(gwm_hip_ctx* ctx, int n, int k, const double* coords, const double* X, const double* y,

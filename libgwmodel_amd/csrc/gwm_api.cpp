@@ -52,20 +52,27 @@ const std::string kInitialBw = "#initial-bandwidth ";   // GWM_LOG_TAG_MGWR_INIT
         }                                                                                \
     } while (0)
 
+// Context buffers come from dalloc and go back through cfree: stream-ordered
+// (hipMallocAsync / hipFreeAsync) on server contexts, plain otherwise.
 template <class T>
-void dfree(T*& p) {
-    if (p) hipFree(p);
+void cfree(gwm_hip_ctx* c, T*& p) {
+    if (p) {
+        if (c->async_mem) hipFreeAsync(p, c->stream);
+        else hipFree(p);
+    }
     p = nullptr;
 }
 
+template <class T>
+int dalloc(gwm_hip_ctx* c, T** p, size_t count) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (c->async_mem) GWM_HIP_CHECK(c, hipMallocAsync((void**)p, bytes, c->stream));
+    else GWM_HIP_CHECK(c, hipMalloc((void**)p, bytes));
+    return GWM_OK;
+}
+
 void free_problem(gwm_hip_ctx* c) {
-    auto dfree = [c](auto*& p) {
-        if (p) {
-            if (c->async_mem) hipFreeAsync(p, c->stream);
-            else hipFree(p);
-        }
-        p = nullptr;
-    };
+    auto dfree = [c](auto*& p) { cfree(c, p); };
     dfree(c->px); dfree(c->py); dfree(c->hx); dfree(c->hy_); dfree(c->hz); dfree(c->sl);
     dfree(c->X); dfree(c->y); dfree(c->Z); dfree(c->G); dfree(c->part);
     dfree(c->boxes); dfree(c->chunk_bw); dfree(c->sel_left); dfree(c->sel_nleft); dfree(c->dnn);
@@ -92,14 +99,6 @@ void free_problem(gwm_hip_ctx* c) {
     c->max_dist = -1.0;
     c->part_bytes = 0;
     c->n = 0;
-}
-
-template <class T>
-int dalloc(gwm_hip_ctx* c, T** p, size_t count) {
-    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    if (c->async_mem) GWM_HIP_CHECK(c, hipMallocAsync((void**)p, bytes, c->stream));
-    else GWM_HIP_CHECK(c, hipMalloc((void**)p, bytes));
-    return GWM_OK;
 }
 
 // Download a sorted-order device array (n x cols) into a caller-order buffer.
@@ -227,9 +226,10 @@ std::vector<double> chunk_boxes(int n, int dims, const double* const* xs) {
 
 int ensure_part(gwm_hip_ctx* c, size_t bytes) {
     if (bytes <= c->part_bytes) return GWM_OK;
-    dfree(c->part);
+    cfree(c, c->part);
     c->part_bytes = 0;
-    GWM_HIP_CHECK(c, hipMalloc((void**)&c->part, bytes));
+    int rc = dalloc(c, reinterpret_cast<char**>(&c->part), bytes);
+    if (rc) return rc;
     c->part_bytes = bytes;
     return GWM_OK;
 }
@@ -286,7 +286,8 @@ int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw
     gwm_hip_ctx::BwEntry* slot = nullptr;
     if (c->bw_cache.size() < cap) {
         double* buf = nullptr;
-        GWM_HIP_CHECK(c, hipMalloc((void**)&buf, (size_t)c->n * sizeof(double)));
+        int rc = dalloc(c, &buf, c->n);
+        if (rc) return rc;
         c->bw_cache.push_back(gwm_hip_ctx::BwEntry{bw, c->dist, buf, 0});
         slot = &c->bw_cache.back();
     } else {
@@ -554,9 +555,10 @@ int refine_fp64(gwm_hip_ctx* c, int mode, int smode, int ntb, int colB, const do
     const int npad = (*count + 127) / 128 * 128;
     const size_t bytes = (size_t)nseg * PT * npad * sizeof(double);
     if (bytes > c->rpart_bytes) {
-        dfree(c->rpart);
+        cfree(c, c->rpart);
         c->rpart_bytes = 0;
-        GWM_HIP_CHECK(c, hipMalloc((void**)&c->rpart, bytes));
+        int rc = dalloc(c, reinterpret_cast<char**>(&c->rpart), bytes);
+        if (rc) return rc;
         c->rpart_bytes = bytes;
     }
     SweepLaunch L;
@@ -1662,7 +1664,10 @@ int gwm_hip_fit(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int precisi
             c->err = "the S matrix is only formed for n <= 65535";
             return GWM_ERR_INVALID;
         }
-        if (!c->S) GWM_HIP_CHECK(c, hipMalloc((void**)&c->S, (size_t)c->n * c->n * sizeof(double)));
+        if (!c->S) {
+            int rc = dalloc(c, &c->S, (size_t)c->n * c->n);
+            if (rc) return rc;
+        }
         GWM_HIP_CHECK(c, launch_smatrix(c->dist, c->k, c->points(), c->X, c->gvec, bw_pt, bw, kernel, c->S, c->stream));
         std::vector<double> Ss((size_t)c->n * c->n);
         GWM_HIP_CHECK(c, hipMemcpyAsync(Ss.data(), c->S, Ss.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
