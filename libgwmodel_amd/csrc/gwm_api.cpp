@@ -449,8 +449,7 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
         // and in plain order the sweep ends on a tail of them.
         const int r = contract_r(nta, ntb, mixed ? 1 : 0, mode, c->dist);
         const int tiles = (c->n + 64 * r - 1) / (64 * r);
-        if (c->prune && chosen == 0 && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg &&
-            seg_len <= 64 * kPlanMaxCount) {
+        if (c->prune && chosen == 0 && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg) {
             if ((rc = ensure_plan(c, (size_t)tiles * nseg))) return rc;
             // qDiag thresholds come from the streamed points' bandwidths (chunk_bw)
             const double* focus_bw = mode == kModeQDiag ? nullptr : bw_pt;
@@ -1308,8 +1307,7 @@ int predict_impl(gwm_hip_ctx* c, int m, const double* loc, int kernel, int adapt
     {
         const int r = contract_r(c->nta, 0, mixed ? 1 : 0, kModePlain, c->dist);
         const int tiles = (m + 64 * r - 1) / (64 * r);
-        if (c->prune && c->autotune && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg &&
-            seg_len <= 64 * kPlanMaxCount) {
+        if (c->prune && c->autotune && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg) {
             if ((rc = ensure_plan(c, (size_t)tiles * nseg))) return rc;
             GWM_HIP_CHECK(c, launch_sweep_plan(c->dist, Q, c->n, c->boxes, bw_pt, bw, nullptr, kernel, mixed ? 1 : 0,
                                                64 * r, seg_len, nseg, tiles, c->plan_counts, c->plan_order,

@@ -100,7 +100,8 @@ int contract_r(int nta, int ntb, int precision_mixed, int mode, int dist);
 // segment within the zero-weight distance of the tile's box, then order = the
 // items by decreasing count within each XCD group.  tile_pts = focus points
 // per tile (64 R), tiles = ceil(F.n / tile_pts); counts and order hold
-// tiles * nseg ints.  nseg <= kPlanMaxSeg and seg_len <= 64 kPlanMaxCount.
+// tiles * nseg ints.  nseg <= kPlanMaxSeg; counts above kPlanMaxCount are
+// ordered by buckets of seg_len / 64 / kPlanMaxCount chunks.
 constexpr int kPlanMaxSeg = 2048;
 constexpr int kPlanMaxCount = 128;
 constexpr int kPlanGroups = 8;

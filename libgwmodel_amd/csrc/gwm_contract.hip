@@ -72,19 +72,39 @@ __device__ __forceinline__ FocusCoords<DIST> load_focus(const GPoints& P, int f)
     return fc;
 }
 
+// Kernel weight of the pair (focus fc, staged point r).  The Gaussian weighs
+// the squared distance directly (no square root; d * d of the reference
+// differs from it by an ulp at most).  Great circle: near pairs take
+// gc_d2_near_f64, the others (far, coincident, near-coincident) gcdist_f64,
+// in a branch that costs a wave only when one of its lanes needs it.
 template <int DIST, int NTA, int MODE>
-__device__ __forceinline__ double pair_distance(const FocusCoords<DIST>& fc, const SharedTile<NTA, DIST, MODE>& sh,
-                                                int r) {
+__device__ __forceinline__ double pair_weight(int kernel, const FocusCoords<DIST>& fc,
+                                              const SharedTile<NTA, DIST, MODE>& sh, int r, const KConst64& k) {
+    double d2, d = -1.0;
     if constexpr (DIST == kEuclidean) {
-        return sqrt(d2_euclid(fc.a, fc.b, sh.c.x[r], sh.c.y[r]));
+        d2 = d2_euclid(fc.a, fc.b, sh.c.x[r], sh.c.y[r]);
     } else {
-        return gcdist_f64(fc.a, fc.b, fc.hx, fc.hy, fc.hz, fc.sl, sh.c.lon[r], sh.c.lat[r], sh.c.hx[r],
-                          sh.c.hy[r], sh.c.hz[r], sh.c.sl[r]);
+        const double ex = sh.c.hx[r] - fc.hx, ey = sh.c.hy[r] - fc.hy, ez = sh.c.hz[r] - fc.hz;
+        double S = ex * ex;
+        S = S + ey * ey;
+        S = S + ez * ez;
+        d2 = gc_d2_near_f64(S, fc.sl, sh.c.sl[r]);
+        if (!(S >= kNearSMin && S <= kNearSMax)) {
+            d = gcdist_f64(fc.a, fc.b, fc.hx, fc.hy, fc.hz, fc.sl, sh.c.lon[r], sh.c.lat[r], sh.c.hx[r],
+                           sh.c.hy[r], sh.c.hz[r], sh.c.sl[r]);
+            d2 = d * d;
+        }
     }
+    if (kernel == kGaussian) {
+        const double e = d2 * k.gauss;
+        return e < k.cut ? 0.0 : exp(e);
+    }
+    return kernel_fast_f64(kernel, d >= 0.0 ? d : sqrt(d2), k);
 }
 
 template <int NTA, int NTB, int DIST, int MODE, int R>
-__global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
+// Two waves per SIMD: the register budget is 256 per lane.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void contract_f64_kernel(SweepLaunch L) {
     __shared__ SharedTile<NTA, DIST, MODE> sh;
     // The job is copied into registers once (uniform scalar loads): a
     // reference into memory would be re-read after every LDS store.
@@ -230,9 +250,9 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(SweepLaunch L) {
             for (int t = 0; t < NTA; ++t) zb[t] = t < nta ? sh.z[rr][16 * t + (lane & 15)] : 0.0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                double d = pair_distance<DIST, NTA, MODE>(fc[r], sh, rr);
                 // qDiag with a fixed bandwidth: fbw holds the same constants as sh.kc
-                double w = kernel_fast_f64(job.kernel, d, (MODE == kModeQDiag && bw_pt) ? sh.kc[rr] : fbw[r]);
+                double w = pair_weight<DIST, NTA, MODE>(job.kernel, fc[r], sh, rr,
+                                                        (MODE == kModeQDiag && bw_pt) ? sh.kc[rr] : fbw[r]);
                 if (j >= jend) w = 0.0;
                 if (MODE == kModeCV && j == fidx[r]) w = 0.0;
                 double wa = MODE == kModeQDiag ? w * w : w;

@@ -120,6 +120,52 @@ __device__ __forceinline__ double gcdist_f64(double lonf, double latf, double hx
     return D * (1 + f * H1 * (sp * sp) - f * H2 * (sm * sm));
 }
 
+// Squared great-circle distance (km^2) of a near pair, 1e-19 <= S <= 1/8
+// (central angle up to about 41 degrees), from the same identities as
+// gcdist_f64 without the arcsine and the square root: on [0, 1/8]
+//   w^2 / S = asin(sqrt(S))^2 / S     (minimax, degree 10, 3e-18 relative),
+//   R = sinc(2w)                      (minimax in S, degree 7, 3e-14 absolute;
+//                                      R only enters the flattening term,
+//                                      f (3R +- 1) / 2 < 1e-2),
+// fitted with mpmath.chebyfit; d^2 = 4 a^2 w^2 (1 + f H1 sp^2 - f H2 sm^2)^2
+// with one FP64 reciprocal.  Pairs outside the range (coincident and
+// near-coincident points, the reference's lon / lat equality tests, and far
+// pairs) take gcdist_f64.
+constexpr double kNearSMin = 1e-19, kNearSMax = 0.125;
+
+__device__ __forceinline__ double gc_d2_near_f64(double S, double slf, double slj) {
+    const double a = 6378.137;
+    const double f = 1.0 / 298.257223563;
+    double q = 0.04621192004569709;
+    q = fma(q, S, 0.021186963403752236);
+    q = fma(q, S, 0.03455657591647529);
+    q = fma(q, S, 0.039645299221225);
+    q = fma(q, S, 0.04872256259518243);
+    q = fma(q, S, 0.06156765311448536);
+    q = fma(q, S, 0.08126985231940705);
+    q = fma(q, S, 0.11428571411189105);
+    q = fma(q, S, 0.17777777777917667);
+    q = fma(q, S, 0.33333333333332893);
+    q = fma(q, S, 1.0);
+    const double w2 = S * q;
+    double R = -0.016035700859708638;
+    R = fma(R, S, -0.012188560686387616);
+    R = fma(R, S, -0.01812589061232504);
+    R = fma(R, S, -0.025953150420487386);
+    R = fma(R, S, -0.04232830192546632);
+    R = fma(R, S, -0.0888888847995917);
+    R = fma(R, S, -0.6666666666911097);
+    R = fma(R, S, 1.000000000000024);
+    const double C = 1.0 - S;
+    const double inv = 1.0 / (S * C);
+    const double H1 = (3 * R - 1) * (0.5 * S * inv);
+    const double H2 = (3 * R + 1) * (0.5 * C * inv);
+    const double sp = 0.5 * (slj + slf);
+    const double sm = 0.5 * (slj - slf);
+    const double g = 1 + f * H1 * (sp * sp) - f * H2 * (sm * sm);
+    return (4.0 * a * a) * w2 * (g * g);
+}
+
 // Kernel weights with the CPU boundary (d < bw): BandwidthWeight.h:68-121.
 //
 // Gaussian and exponential weights whose exponent lies more than kExpCut

@@ -70,14 +70,17 @@ __global__ __launch_bounds__(1024) void plan_order_kernel(const int* counts, int
     const int t = threadIdx.x;
     for (int e = t; e < kPlanGroups * (kPlanMaxCount + 1); e += 1024) (&hist[0][0])[e] = 0;
     __syncthreads();
-    for (int q = t; q < nitems; q += 1024) {
-        const int c = min(max(counts[q], 0), maxc);
-        atomicAdd(&hist[(q % nseg) % G][c], 1);
-    }
+    // counts beyond kPlanMaxCount buckets are ranked by bucket
+    const int nb = min(maxc, kPlanMaxCount);
+    auto bucket = [&](int c) {
+        c = min(max(c, 0), maxc);
+        return maxc > kPlanMaxCount ? (int)((long long)c * kPlanMaxCount / maxc) : c;
+    };
+    for (int q = t; q < nitems; q += 1024) atomicAdd(&hist[(q % nseg) % G][bucket(counts[q])], 1);
     __syncthreads();
     if (t < G) {
         int run = 0;
-        for (int c = maxc; c >= 0; --c) {
+        for (int c = nb; c >= 0; --c) {
             const int h = hist[t][c];
             hist[t][c] = run;
             run += h;
@@ -85,7 +88,7 @@ __global__ __launch_bounds__(1024) void plan_order_kernel(const int* counts, int
     }
     __syncthreads();
     for (int q = t; q < nitems; q += 1024) {
-        const int c = min(max(counts[q], 0), maxc);
+        const int c = bucket(counts[q]);
         const int g = (q % nseg) % G;
         const int pos = atomicAdd(&hist[g][c], 1);
         order[(size_t)G * pos + g] = q;
@@ -98,7 +101,7 @@ hipError_t launch_sweep_plan(int dist, const Points& F, int n, const double* box
                              const double* chunk_bw, int kernel, int mixed, int tile_pts, int seg_len, int nseg,
                              int tiles, int* counts, int* order, hipStream_t s) {
     const int maxc = seg_len / 64;
-    if (nseg < 1 || nseg > kPlanMaxSeg || maxc > kPlanMaxCount || tiles < 1 || tile_pts % 64 != 0 ||
+    if (nseg < 1 || nseg > kPlanMaxSeg || maxc < 1 || tiles < 1 || tile_pts % 64 != 0 ||
         tile_pts > 256 || (long long)(nseg - 1) * seg_len >= n || (long long)nseg * seg_len < n)
         return hipErrorInvalidValue;
     if ((long long)tiles * tile_pts < F.n || (long long)(tiles - 1) * tile_pts >= F.n) return hipErrorInvalidValue;

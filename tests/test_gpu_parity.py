@@ -163,6 +163,42 @@ def test_exponent_cut_matches_oracle(ctx, oracle, kernel, bw):
     assert close(g["qDiag"], o["qDiag"], 1e-9, 1e-12)
 
 
+@pytest.mark.parametrize("kernel,adaptive,bw", [(capi.GAUSSIAN, False, 2500.0), (capi.GAUSSIAN, False, 1500.0),
+                                                 (capi.BISQUARE, True, 60.0), (capi.EXPONENTIAL, False, 800.0)])
+def test_great_circle_world_scale(ctx, oracle, kernel, adaptive, bw):
+    """Great-circle sweeps over pairs of every separation: the FP64 sweep weighs
+    pairs up to about 41 degrees of arc through a polynomial form of the
+    distance and the others (antipodal-range pairs, coincident points, points
+    within 1e-8 degrees) through the general one.  Criteria and fits against
+    the oracle at the FP64 tolerances.  The bandwidths keep the local designs
+    well conditioned at this density (points about 900 km apart): at 300 km
+    both sides return designs with condition numbers near 1e8, whose betas
+    differ at 1e-8 from summation order alone."""
+    say("case: world-scale great circle kernel=%d adaptive=%d bw=%g" % (kernel, adaptive, bw))
+    rng = np.random.default_rng(77)
+    n = 600
+    lon = rng.uniform(-180.0, 180.0, n)
+    lat = np.degrees(np.arcsin(rng.uniform(-0.95, 0.95, n)))
+    lon[10], lat[10] = lon[11], lat[11]                       # coincident
+    lon[20], lat[20] = lon[21] + 5e-9, lat[21] - 5e-9         # within the reference's 1e-8 degree test
+    lon[30], lat[30] = 180.0, 12.5                            # the antimeridian, both ways
+    lon[31], lat[31] = -180.0, 12.5
+    coords = np.column_stack([lon, lat])
+    X = np.column_stack([np.ones(n), rng.normal(size=n), rng.normal(size=n)])
+    y = 1.0 + np.cos(np.radians(lat)) * X[:, 1] + 0.5 * X[:, 2] + 0.2 * rng.normal(size=n)
+    ctx.set_data(coords, X, y, capi.GREAT_CIRCLE)
+    for crit in [capi.CRIT_CV, capi.CRIT_AIC]:
+        g = ctx.criterion(kernel, adaptive, bw, crit)
+        o = oracle.criterion(coords, X, y, kernel, adaptive, bw, crit, geographic=True)
+        assert g == pytest.approx(o, rel=1e-11), (crit, g, o)
+    g = ctx.fit(kernel, adaptive, bw)
+    o = oracle.fit_core(coords, X, y, kernel, adaptive, bw, geographic=True)
+    assert close(g["betas"], o["betas"], 1e-9, 1e-12)
+    assert close(g["betasSE_raw"], o["betasSE_raw"], 1e-9, 1e-15)
+    assert close(g["shat"], o["shat"], 1e-10)
+    assert close(g["qDiag"], o["qDiag"], 1e-9, 1e-12)
+
+
 @pytest.mark.parametrize("k", [1, 2, 5, 8, 12, 16])
 def test_fit_all_k(ctx, oracle, k):
     say("case: fit k=%d" % k)
