@@ -5,14 +5,20 @@
 //   - coordinate differences and the squared distance (Euclidean) or the
 //     half-chord S (great circle) in FP64 -- nearby points cancel there;
 //   - the transcendental tail of the distance and the kernel weight in FP32;
-//   - w and Z split into FP16 hi + lo parts, lo scaled by 2^11 so small
-//     values stay normal: w = wh + wl * 2^-11, z = zh + zl * 2^-11 (each part
-//     carries 11 significant bits, the pair ~22);
-//   - w z = wh zh + 2^-11 (wh zl + wl zh) + 2^-22 wl zl on
-//     v_mfma_f32_16x16x32_f16 with FP32 accumulation, the three scales in
-//     separate accumulators (the wl zl term, about 2^-22 relative, roughly
-//     halves the error of the betas for 1/4 more matrix-core work, which the
-//     VALU-bound producer hides);
+//   - Z split into FP16 hi + lo parts, lo scaled by 2^11 so small values
+//     stay normal: z = zh + zl * 2^-11 (each part carries 11 significant
+//     bits, the pair ~22);
+//   - AICc and CV sweeps (w^1): the weight is formed as W = 2^11 w (the
+//     exponential kernels add 11 to the exponent) and split W = Wh + Wl with
+//     Wl the unscaled FP16 residual, so one packed conversion gives Wh and a
+//     subtraction and a second packed conversion Wl (Wh normal down to
+//     w = 2^-25, Wl's absolute error 2^-22 W or 2^-25);
+//     W z = Wh zh + Wl zh + 2^-11 (Wh zl + Wl zl), the first two in separate
+//     FP32 accumulators, the 2^-11 terms in a third, all undone at the FP64
+//     flush;
+//   - hat and qDiag sweeps (w and w^2): w = wh + wl 2^-11 with the lo part
+//     scaled, w z = wh zh + 2^-11 (wh zl + wl zh) + 2^-22 wl zl, the three
+//     scales in separate accumulators;
 //   - every 64 data points the FP32 accumulators are flushed into FP64 ones,
 //     so FP32 rounding only ever acts on 64-term partial sums;
 //   - Z columns carry a power-of-two scale (exact) that keeps them inside the
@@ -45,7 +51,7 @@ struct MCoords<kEuclidean> {
 template <>
 struct MCoords<kGreatCircle> {
     double hx[kChunkM], hy[kChunkM], hz[kChunkM];
-    double sl[kChunkM];
+    float sl[kChunkM];   // sin(lat): only enters the flattening term
 };
 
 // Kernel weight from the squared distance (Euclidean) or from the distance.
@@ -73,7 +79,8 @@ struct alignas(16) MShared {
 
 template <int DIST>
 struct MFocus {
-    double a, b, c, sl;
+    double a, b, c;
+    float sl;
 };
 
 template <int DIST>
@@ -83,12 +90,12 @@ __device__ __forceinline__ MFocus<DIST> load_mfocus(const GPoints& P, int f) {
         fc.a = P.px[f];
         fc.b = P.py[f];
         fc.c = 0.0;
-        fc.sl = 0.0;
+        fc.sl = 0.0f;
     } else {
         fc.a = P.hx[f];
         fc.b = P.hy[f];
         fc.c = P.hz[f];
-        fc.sl = P.sl[f];
+        fc.sl = (float)P.sl[f];
     }
     return fc;
 }
@@ -110,55 +117,58 @@ __device__ __forceinline__ float weight_from_d2(int kernel, float d2, const BwCo
     }
 }
 
-// Squared great-circle distance (km^2) from the half-chord S = sin^2(theta),
-// theta = half the central angle, for S <= 0.25 (central angle <= 60 deg):
-//   theta^2 = S (asin(s)/s)^2,  asin(s)/s = 1 + S p(S)   (Cephes asinf polynomial)
-//   R = sinc(2 theta) = sum_m (-1)^m (2 theta)^(2m) / (2m+1)!
-//   d = 2 a theta (1 + f/8 [(3R-1) sp^2 / C - (3R+1) sm^2 / S]),  C = 1 - S,
-// with sp = sin lat_i + sin lat_j, sm = sin lat_j - sin lat_i (both formed in
-// FP64 by the caller: sm^2/S cancels for nearby points); 1/C and 1/S come
-// from one reciprocal of S*C.  No square root, two transcendentals per pair
-// including the kernel's exponential.  Larger S takes gcdist_tail_f32 (the
-// producer runs it in a second, wave-uniform pass).  Branch free: coincident
-// points (S <= 1e-30) give 0.
+// Squared great-circle distance (km^2) from the half-chord S = sin^2(w),
+// w = half the central angle, for S <= 0.25 (central angle <= 60 deg), in the
+// identities of gcdist_f64:
+//   w^2 = S q2(S),  q2 = asin(sqrt(S))^2 / S   (minimax, degree 5, 1.4e-8),
+//   3R = 3 sinc(2w)                              (minimax in S, degree 3,
+//        4.2e-6: R only enters the flattening term, f/8 (3R +- 1) < 2e-3),
+//   d = 2 a w (1 + f/8 [(3R-1) sp^2 / C - (3R+1) sm^2 / S]),  C = 1 - S,
+// with sp = sin lat_i + sin lat_j and sm = sin lat_j - sin lat_i in FP32
+// (an error of 2^-24 sin lat in sm moves d by f 2^-24 / sqrt(S) relative,
+// and the weight's exponent by that times (d / bw)^2); 1/C and 1/S come from
+// one reciprocal of S*C, clamped so that coincident points (S = 0, sm = 0)
+// give exactly 0.  No square root, two transcendentals per pair including
+// the kernel's exponential.  Larger S takes gcdist_tail_f32 (the producer
+// runs it in a second, wave-uniform pass).  Coefficients from
+// mpmath.chebyfit on [0, 1/4].
 __device__ __forceinline__ float gc_d2_near_f32(float S, float sp, float sm) {
     const float a2 = (float)(2.0 * 6378.137);                       // 2a
     const float c8 = (float)(2.0 * 6378.137 / 298.257223563 / 8.0);  // 2a f / 8
-    const float p = fmaf(fmaf(fmaf(fmaf(4.2163199048e-2f, S, 2.4181311049e-2f), S, 4.5470025998e-2f), S,
-                              7.4953002686e-2f), S, 1.6666752422e-1f);
-    const float q = fmaf(S, p, 1.0f);
-    const float th2 = S * q * q;
-    // R = sinc(2 theta) = sum_m (-1)^m 4^m theta^(2m) / (2m+1)!, in th2
-    const float R = fmaf(fmaf(fmaf(fmaf(fmaf(th2, -2.5653358986692319e-5f, 7.0546737213403880e-4f), th2,
-                                        -1.2698412698412698e-2f), th2, 1.3333333333333333e-1f), th2,
-                              -6.6666666666666667e-1f), th2, 1.0f);
+    const float q2 = fmaf(fmaf(fmaf(fmaf(fmaf(0.1182195246219635f, S, 0.06318631768226624f), S,
+                                         0.11677248775959015f), S, 0.17762747406959534f), S,
+                               0.33333659172058105f), S, 1.0f);
+    const float R3 = fmaf(fmaf(fmaf(-0.1778002232313156f, S, -0.25808843970298767f), S, -2.0004422664642334f), S,
+                          3.0000035762786865f);
     const float C = 1.0f - S;
-    const float rsc = __builtin_amdgcn_rcpf(S * C);
+    const float rsc = __builtin_amdgcn_rcpf(fmaxf(S * C, 1e-30f));
     // (3R - 1) sp^2 / C - (3R + 1) sm^2 / S = rsc [(3R - 1) S sp^2 - (3R + 1) C sm^2]
-    const float R3 = 3.0f * R;
     const float inner = fmaf(fmaf(R3, S, -S), sp * sp, -(fmaf(R3, C, C) * (sm * sm)));
     const float g2a = fmaf(c8 * rsc, inner, a2);   // 2a (1 + f/8 rsc inner)
-    const float d2 = th2 * (g2a * g2a);
-    return S > 1e-30f ? d2 : 0.0f;
+    return S * (q2 * (g2a * g2a));
 }
 
-// Kernel weights of 8 squared distances (the kernel switch is taken once).
+// Kernel weights of 8 squared distances (the kernel switch is taken once),
+// times 2^11 when SCALED (added to the exponent of the exponential kernels).
+template <bool SCALED>
 __device__ __forceinline__ void weights8_from_d2(int kernel, const float* d2, const BwConst& c, float* w) {
+    constexpr float e0 = SCALED ? 11.0f : 0.0f;
+    constexpr float m0 = SCALED ? kLoScale : 1.0f;
     switch (kernel) {
     case kGaussian:
 #pragma unroll
-        for (int e = 0; e < 8; ++e) w[e] = __builtin_amdgcn_exp2f(d2[e] * c.gauss);
+        for (int e = 0; e < 8; ++e) w[e] = __builtin_amdgcn_exp2f(fmaf(d2[e], c.gauss, e0));
         break;
     case kExponential:
 #pragma unroll
         for (int e = 0; e < 8; ++e)
-            w[e] = __builtin_amdgcn_exp2f(-1.4426950408889634f * __builtin_sqrtf(d2[e]) * c.inv_bw);
+            w[e] = __builtin_amdgcn_exp2f(fmaf(-1.4426950408889634f * __builtin_sqrtf(d2[e]), c.inv_bw, e0));
         break;
     case kBisquare:
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             float t = 1.0f - d2[e] * c.inv_bw2;
-            w[e] = d2[e] < c.bw2 ? t * t : 0.0f;
+            w[e] = d2[e] < c.bw2 ? (t * t) * m0 : 0.0f;
         }
         break;
     case kTricube:
@@ -166,14 +176,28 @@ __device__ __forceinline__ void weights8_from_d2(int kernel, const float* d2, co
         for (int e = 0; e < 8; ++e) {
             float q = __builtin_sqrtf(d2[e]) * c.inv_bw;
             float t = 1.0f - q * q * q;
-            w[e] = d2[e] < c.bw2 ? (t * t) * t : 0.0f;
+            w[e] = d2[e] < c.bw2 ? ((t * t) * t) * m0 : 0.0f;
         }
         break;
     default:
 #pragma unroll
-        for (int e = 0; e < 8; ++e) w[e] = d2[e] < c.bw2 ? 1.0f : 0.0f;
+        for (int e = 0; e < 8; ++e) w[e] = d2[e] < c.bw2 ? m0 : 0.0f;
         break;
     }
+}
+
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+// W = Wh + Wl for two weights: one packed conversion to FP16 for the hi
+// parts, the unscaled residuals from that same packed value (the empty
+// register constraint keeps the compiler from forming the hi parts twice
+// with different roundings), a second packed conversion for the lo parts.
+__device__ __forceinline__ void split2_scaled(float w0, float w1, h2v& hi, h2v& lo) {
+    hi = __builtin_convertvector((f2v){w0, w1}, h2v);
+    asm volatile("" : "+v"(hi));
+    const f2v back = __builtin_convertvector(hi, f2v);
+    lo = __builtin_convertvector((f2v){w0 - back.x, w1 - back.y}, h2v);
 }
 
 __device__ __forceinline__ void split16(float w, _Float16& hi, _Float16& lo) {
@@ -221,6 +245,8 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     }
     // a launch groups jobs of exactly the instance's tile counts
     constexpr int nta = NTA, ntb = NTB;
+    // AICc / CV sweeps weigh with w: the scaled split of W = 2^11 w
+    constexpr bool kScaledSplit = (MODE == kModePlain || MODE == kModeCV) && NTB == 0;
     const GPoints P = global_points(args.P);
     GWM_GLOBAL const double* bw_pt = gptr(args.bw_pt);
     const int ldz = nta * 16;
@@ -323,7 +349,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                 sh.c.hx[threadIdx.x] = P.hx[jl];
                 sh.c.hy[threadIdx.x] = P.hy[jl];
                 sh.c.hz[threadIdx.x] = P.hz[jl];
-                sh.c.sl[threadIdx.x] = P.sl[jl];
+                sh.c.sl[threadIdx.x] = (float)P.sl[jl];
             }
             if constexpr (MODE == kModeQDiag) sh.bwc[threadIdx.x] = make_bw((float)(bw_pt ? bw_pt[jl] : args.bw));
         }
@@ -352,7 +378,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                 v8h ah, al, a2h, a2l;
                 const int rb = 32 * s + 8 * kq;
                 float d2[8], w[8];
-                unsigned far = 0;   // great circle: pairs beyond 60 degrees of arc (S > 0.25)
+                float smax = 0.0f;   // great circle: any pair beyond 60 degrees of arc (S > 0.25)?
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int rr = rb + e;
@@ -361,22 +387,25 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                         d2[e] = (float)(dx * dx + dy * dy);
                     } else {
                         double ex = sh.c.hx[rr] - fc[r].a, ey = sh.c.hy[rr] - fc[r].b, ez = sh.c.hz[rr] - fc[r].c;
-                        const double slj = sh.c.sl[rr];
+                        const float slj = sh.c.sl[rr];
                         const float S = (float)(ex * ex + ey * ey + ez * ez);
-                        const float dn = gc_d2_near_f32(S, (float)(fc[r].sl + slj), (float)(slj - fc[r].sl));
-                        // a far pair keeps S for the second pass
-                        d2[e] = S > 0.25f ? S : dn;
-                        far |= (S > 0.25f ? 1u : 0u) << e;
+                        d2[e] = gc_d2_near_f32(S, fc[r].sl + slj, slj - fc[r].sl);
+                        smax = fmaxf(smax, S);
                     }
                 }
                 if constexpr (DIST != kEuclidean) {
-                    if (__any(far != 0u)) {
+                    if (__any(smax > 0.25f)) {
 #pragma unroll
-                        for (int e = 0; e < 8; ++e)
-                            if (far & (1u << e)) {
-                                const float d = gcdist_tail_f32(d2[e], (float)fc[r].sl, (float)sh.c.sl[rb + e]);
+                        for (int e = 0; e < 8; ++e) {
+                            const int rr = rb + e;
+                            double ex = sh.c.hx[rr] - fc[r].a, ey = sh.c.hy[rr] - fc[r].b,
+                                   ez = sh.c.hz[rr] - fc[r].c;
+                            const float S = (float)(ex * ex + ey * ey + ez * ez);
+                            if (S > 0.25f) {
+                                const float d = gcdist_tail_f32(S, fc[r].sl, sh.c.sl[rr]);
                                 d2[e] = d * d;
                             }
+                        }
                     }
                 }
                 // qDiag with a fixed bandwidth: fbw holds the same constants as sh.bwc
@@ -384,25 +413,42 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
 #pragma unroll
                     for (int e = 0; e < 8; ++e) w[e] = weight_from_d2(args.kernel, d2[e], sh.bwc[rb + e]);
                 } else {
-                    weights8_from_d2(args.kernel, d2, fbw[r], w);
+                    weights8_from_d2<kScaledSplit>(args.kernel, d2, fbw[r], w);
                 }
                 // Rows past the segment end occur only in the last segment,
                 // at j >= n, where every Z table (design, G, single column)
                 // and its FP16 planes are zero padded: their products are
                 // exact zeros without masking the (finite) weights.
+                if constexpr (kScaledSplit) {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    float we = w[e];
-                    if (MODE == kModeCV && j0 + rb + e == fidx[r]) we = 0.0f;
-                    const float wa = MODE == kModeQDiag ? square_rounded(we) : we;
-                    _Float16 h, l;
-                    split16(wa, h, l);
-                    ah[e] = h;
-                    al[e] = l;
-                    if (NTB > 0) {
-                        split16(square_rounded(we), h, l);
-                        a2h[e] = h;
-                        a2l[e] = l;
+                    for (int e = 0; e < 8; e += 2) {
+                        float w0 = w[e], w1 = w[e + 1];
+                        if (MODE == kModeCV) {
+                            const int self = fidx[r] - (j0 + rb);
+                            if (self == e) w0 = 0.0f;
+                            if (self == e + 1) w1 = 0.0f;
+                        }
+                        h2v h, l;
+                        split2_scaled(w0, w1, h, l);
+                        ah[e] = h.x;
+                        ah[e + 1] = h.y;
+                        al[e] = l.x;
+                        al[e + 1] = l.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        float we = w[e];
+                        const float wa = MODE == kModeQDiag ? square_rounded(we) : we;
+                        _Float16 h, l;
+                        split16(wa, h, l);
+                        ah[e] = h;
+                        al[e] = l;
+                        if (NTB > 0) {
+                            split16(square_rounded(we), h, l);
+                            a2h[e] = h;
+                            a2l[e] = l;
+                        }
                     }
                 }
 #pragma unroll
@@ -410,16 +456,25 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
                     if (t >= nta) continue;
                     hh[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[t], hh[r][t], 0, 0, 0);
                     xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[t], xx[r][t], 0, 0, 0);
-                    xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], xx[r][t], 0, 0, 0);
-                    ll[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bl[t], ll[r][t], 0, 0, 0);
+                    if constexpr (kScaledSplit) {
+                        // Wl zh at scale 1 in its own accumulator, Wl zl with the 2^-11 terms
+                        ll[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], ll[r][t], 0, 0, 0);
+                        xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bl[t], xx[r][t], 0, 0, 0);
+                    } else {
+                        xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], xx[r][t], 0, 0, 0);
+                        ll[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bl[t], ll[r][t], 0, 0, 0);
+                    }
                 }
 #pragma unroll
                 for (int t = 0; t < NTB; ++t) {
                     if (t >= ntb) continue;
-                    hh[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bh[t], hh[r][NTA + t], 0, 0, 0);
-                    xx[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bl[t], xx[r][NTA + t], 0, 0, 0);
+                    hh[r][NTA + t] =
+                        __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bh[t], hh[r][NTA + t], 0, 0, 0);
+                    xx[r][NTA + t] =
+                        __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bl[t], xx[r][NTA + t], 0, 0, 0);
                     xx[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bh[t], xx[r][NTA + t], 0, 0, 0);
-                    ll[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bl[t], ll[r][NTA + t], 0, 0, 0);
+                    ll[r][NTA + t] =
+                        __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bl[t], ll[r][NTA + t], 0, 0, 0);
                 }
             }
         }
@@ -429,9 +484,16 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
 #pragma unroll
             for (int t = 0; t < NTA + NTB; ++t)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    acc[r][t][q] += (double)hh[r][t][q] + (double)xx[r][t][q] * (1.0 / 2048.0) +
-                                    (double)ll[r][t][q] * (1.0 / 4194304.0);
+                for (int q = 0; q < 4; ++q) {
+                    // hh + ll in FP32: one rounding per 64-point block, next to the
+                    // 64 of the block's accumulation
+                    if constexpr (kScaledSplit)
+                        acc[r][t][q] += ((double)(hh[r][t][q] + ll[r][t][q]) +
+                                         (double)xx[r][t][q] * (1.0 / 2048.0)) * (1.0 / 2048.0);
+                    else
+                        acc[r][t][q] += (double)hh[r][t][q] + (double)xx[r][t][q] * (1.0 / 2048.0) +
+                                        (double)ll[r][t][q] * (1.0 / 4194304.0);
+                }
     }
 
     if (args.pairs && threadIdx.x == 0 && swept)
