@@ -1100,6 +1100,26 @@ int upload_design(gwm_hip_ctx* c, const double* X_caller, const double* y_caller
 }
 
 // Single-column criterion / fit (GWRMultiscale.cpp:420-482, 560-636).
+// The column x and response y of a single-column step on the device, in
+// sorted order, with their product table (FP64 and, for mixed precision, its
+// FP16 planes).  A golden search over one variable prepares them once.
+int var_prepare(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int precision) {
+    const int n = c->n;
+    std::vector<double> xs(n), ys(n);
+    to_sorted(c, x_caller, xs.data(), 1);
+    to_sorted(c, y_caller, ys.data(), 1);
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->vx, xs.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    GWM_HIP_CHECK(c, hipMemcpyAsync(c->vy, ys.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    GWM_HIP_CHECK(c, launch_var_table(c->vx, c->vy, n, c->rows_pad, c->vZ, c->stream));
+    if (precision == GWM_MIXED)
+        GWM_HIP_CHECK(c, launch_split_fp16(c->vZ, c->rows_pad, 16, c->colmax, c->vZh, c->vZl, c->vscale, c->stream));
+    // the host copies go out of scope: finish the uploads first
+    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GWM_OK;
+}
+
+// One single-column criterion (value) or fit (beta) on the column prepared
+// by var_prepare (x_caller / y_caller non-null: prepare them first).
 int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int kernel, int adaptive, double bw,
              int crit, int precision, double* value, double* beta, double* bound = nullptr) {
     if (bound) *bound = 0.0;
@@ -1108,21 +1128,7 @@ int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int
     if (value) *value = DBL_MAX;
     if (stopped(c)) return GWM_ERR_CANCELLED;
     const int n = c->n;
-    std::vector<double> xs(n), ys(n);
-    to_sorted(c, x_caller, xs.data(), 1);
-    to_sorted(c, y_caller, ys.data(), 1);
-    const double* x = xs.data();
-    const double* y = ys.data();
-    std::vector<double> Z((size_t)c->rows_pad * 16, 0.0);
-    for (int j = 0; j < n; ++j) {
-        Z[(size_t)j * 16 + 0] = x[j] * x[j];
-        Z[(size_t)j * 16 + 1] = x[j] * y[j];
-    }
-    GWM_HIP_CHECK(c, hipMemcpyAsync(c->vZ, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    GWM_HIP_CHECK(c, hipMemcpyAsync(c->vx, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    GWM_HIP_CHECK(c, hipMemcpyAsync(c->vy, y, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    if (precision == GWM_MIXED)
-        GWM_HIP_CHECK(c, launch_split_fp16(c->vZ, c->rows_pad, 16, c->colmax, c->vZh, c->vZl, c->vscale, c->stream));
+    if (x_caller && (rc = var_prepare(c, x_caller, y_caller, precision))) return rc;
     const double* bw_pt;
     if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
     const bool cv = value && crit == GWM_CRIT_CV;
@@ -1836,12 +1842,12 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         return code;
     };
     // a variable's bandwidth search, reported as GWRMultiscale.cpp:588-594
-    auto search_var = [&](int i, const double* yv, double lo, double hi, double* out) {
+    // (on the column and response var_prepare has put on the device)
+    auto search_var = [&](int i, double lo, double hi, double* out) {
         std::map<double, double> log;
-        const double* xi = col(i);
         return golden(adaptive[i] != 0, lo, hi,
                       [&](double x, bool exact, double* fx, double* fb) {
-                          return var_impl(c, xi, yv, kernels[i], adaptive[i], x, approach[i],
+                          return var_impl(c, nullptr, nullptr, kernels[i], adaptive[i], x, approach[i],
                                           exact ? GWM_FP64 : precision, fx, nullptr, fb);
                       },
                       precision == GWM_MIXED, c->certify_rel, out, log, telegram_report(c, adaptive[i] != 0));
@@ -1856,7 +1862,8 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         double lo = has_lower ? lower : (ad ? (double)adaptive_lower : maxD[i] / 5000.0);
         double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
         tele_print(c, kLogInfo, kInitialBw + std::to_string(i));
-        if ((rc = search_var(i, y.data(), lo, hi, &bw[i]))) return fail_out(rc);
+        if ((rc = var_prepare(c, col(i), y.data(), precision)) || (rc = search_var(i, lo, hi, &bw[i])))
+            return fail_out(rc);
         tele_print(c, kLogInfo, kInitialBw + std::to_string(i) + "," + std::to_string(bw[i]));
     }
     // initial all-variable bandwidth (GWRMultiscale.cpp:137-160)
@@ -1917,7 +1924,8 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         for (int i = 0; i < k; ++i) {
             const double* xi = col(i);
             for (int j = 0; j < n; ++j) yi[j] = resid[j] + betas[j + (size_t)i * n] * xi[j];
-            if ((rc = use_distance(c, dist_of(i)))) return fail_out(rc);
+            if ((rc = use_distance(c, dist_of(i))) || (rc = var_prepare(c, xi, yi.data(), precision)))
+                return fail_out(rc);
             if (init[i] != 2) {
                 tele_print(c, kLogInfo, kBackfit + "#variable-bandwidth-selection " + std::to_string(i));
                 bool ad = adaptive[i] != 0;
@@ -1927,7 +1935,7 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
                 double lo = has_lower ? lower : (ad ? (double)adaptive_lower : maxD[i] / 5000.0);
                 double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
                 double bw1 = 0.0;
-                if ((rc = search_var(i, yi.data(), lo, hi, &bw1))) return fail_out(rc);
+                if ((rc = search_var(i, lo, hi, &bw1))) return fail_out(rc);
                 // GWRMultiscale.cpp:263-290
                 std::string vbs = std::to_string(i) + "," + std::to_string(bw[i]) + "," + std::to_string(bw1) + "," +
                                   std::to_string(std::fabs(bw1 - bw[i]));
@@ -1946,7 +1954,7 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
                 bw[i] = bw1;
                 tele_print(c, kLogInfo, kBackfit + "#variable-bandwidth-selection " + vbs);
             }
-            if ((rc = var_impl(c, xi, yi.data(), kernels[i], adaptive[i], bw[i], GWM_CRIT_AIC, precision, nullptr,
+            if ((rc = var_impl(c, nullptr, nullptr, kernels[i], adaptive[i], bw[i], GWM_CRIT_AIC, precision, nullptr,
                                colv.data())))
                 return fail_out(rc);
             std::copy(colv.begin(), colv.end(), betas.begin() + (size_t)i * n);

@@ -82,6 +82,11 @@ struct SweepLaunch {
 // segments next to the current tiles.
 int tile_stride(int tiles);
 
+// Single-column product table of a multiscale step (GWRMultiscale.cpp:
+// 420-482, 560-636): Z[j][0] = x_j^2, Z[j][1] = x_j y_j, the other 14 of its
+// 16 columns and the rows n .. rows_pad-1 zero.
+hipError_t launch_var_table(const double* x, const double* y, int n, int rows_pad, double* Z, hipStream_t s);
+
 // FP16 hi/lo planes ([rows/32][ldz][32]) and per-column unscale factors of a
 // row-major FP64 table; colmax is ldz words of scratch.
 hipError_t launch_split_fp16(const double* Z, int rows, int ldz, unsigned long long* colmax, _Float16* Zh,

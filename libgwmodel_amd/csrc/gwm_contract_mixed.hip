@@ -644,6 +644,26 @@ hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const Swe
     }
 }
 
+namespace {
+
+__global__ void var_table_kernel(const double* __restrict__ x, const double* __restrict__ y, int n, int rows_pad,
+                                 double* __restrict__ Z) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;   // one entry of the rows_pad x 16 table
+    if (e >= rows_pad * 16) return;
+    const int j = e >> 4, c = e & 15;
+    double v = 0.0;
+    if (j < n && c < 2) v = c == 0 ? x[j] * x[j] : x[j] * y[j];
+    Z[e] = v;
+}
+
+}  // namespace
+
+hipError_t launch_var_table(const double* x, const double* y, int n, int rows_pad, double* Z, hipStream_t s) {
+    const int total = rows_pad * 16;
+    hipLaunchKernelGGL(var_table_kernel, dim3((total + 255) / 256), dim3(256), 0, s, x, y, n, rows_pad, Z);
+    return hipGetLastError();
+}
+
 hipError_t launch_split_fp16(const double* Z, int rows, int ldz, unsigned long long* colmax, _Float16* Zh,
                              _Float16* Zl, double* unscale, hipStream_t s) {
     hipError_t e = hipMemsetAsync(colmax, 0, ldz * sizeof(unsigned long long), s);
