@@ -15,15 +15,30 @@ golden-section decisions); the same step is run on the FP64 path for the
 mixed/FP64 ratio and the beta error.  Inputs are resident on the device
 before the timed region starts.
 
-Three more workloads ride along in the same JSON line (keys "mgwr", "batch"
-and "serve"): GWRMultiscale backfitting at n = 50,000, k = 6 (bisquare
-adaptive bandwidths chosen by CV for every parameter, CVR threshold 1e-5) in
-mixed precision and FP64; 512 uneven problems (n log-uniform in [200, 20000], k in 3..12,
-Gaussian / bisquare / tricube, adaptive or fixed), each a CV golden-section
-search plus fit, run through the batching server against a loop of
-single-problem calls; and a Poisson stream of fit and CV-search requests
-(n log-uniform in [500, 200000], k in 2..16, clustered coordinates) replayed in
-real time against the continuous-batching server, with p50/p99 latency.
+`value` counts each (focus point, bandwidth) evaluation of the search and the
+fit once; the mixed search takes the FP64 search's iterates, so both paths
+count the same solves per step and mixed_over_fp64 is their wall-time ratio.
+
+More workloads ride along in the same JSON line:
+- "cfg1": n = 2000 uniform square, k = 4, bisquare adaptive CV search + fit,
+  GPU FP64 / mixed and the oracle's full OpenMP run (iterates compared);
+- "refbench": the reference's own benchmark shape (a fit at n = 50,000,
+  k = 3, Gaussian bandwidth 0.2), GPU and a sampled oracle fit;
+- "mgwr_cpu_parity": GWRMultiscale on the multiscale-cosines workload at
+  n = 1000, the oracle's backfit against the GPU's (bandwidths bit-exact);
+- "mgwr": GWRMultiscale backfitting at n = 50,000, k = 6 (bisquare adaptive
+  bandwidths chosen by CV for every parameter, CVR threshold 1e-5) in mixed
+  precision and FP64, with the adaptive select's rate;
+- "mgwr_hat": the same with the hat matrix at n = 4000, k = 3;
+- "batch": 512 uneven problems (n log-uniform in [200, 20000], k in 3..12,
+  Gaussian / bisquare / tricube, adaptive or fixed), each a CV golden-section
+  search plus fit, through the batching server against a loop of
+  single-problem calls;
+- "serve": a Poisson stream of 500 fit and CV-search requests (n log-uniform
+  in [500, 200000], k in 2..16, clustered coordinates) replayed in real time
+  against the continuous-batching server at two rates, with latency
+  percentiles and the beta error on requests spread over all sizes;
+- "predict", "varsel": prediction at n new locations and variable selection.
 
 Prints one JSON line (see the keys below).
 """
@@ -47,6 +62,26 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def jsonable(o):
+    """numpy scalars and arrays in the JSON line."""
+    if isinstance(o, np.generic):
+        return o.item()
+    if isinstance(o, np.ndarray):
+        return o.tolist()
+    raise TypeError(type(o).__name__)
+
+
+def leg(name, fn, *a):
+    """One side workload of the JSON line; an exception ends that leg only."""
+    try:
+        return fn(*a)
+    except Exception as e:   # noqa: BLE001 -- reported in the line, the headline stands
+        import traceback
+        traceback.print_exc()
+        log("%s failed: %r" % (name, e))
+        return {"error": repr(e)}
+
+
 def beta_rel_err(b, ref):
     """max over columns of max|b - ref| / max|ref| (column-scaled)."""
     scale = np.maximum(np.max(np.abs(ref), axis=0), 1e-300)
@@ -61,6 +96,26 @@ def run_steps(ctx, capi, prec, steps, lo, hi):
     return out
 
 
+def cpu_threads():
+    """Host threads of the CPU legs: every core of this process's affinity
+    mask, capped by OMP_NUM_THREADS when it is set (the host share one GPU
+    gets where several GPUs share a host)."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return (min(aff, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else aff), aff
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(d, bw, sample, threads):
     from oracle import pyoracle
     t0 = time.perf_counter()
@@ -68,6 +123,123 @@ def cpu_baseline(d, bw, sample, threads):
                        nthreads=threads, focus_count=sample)
     dt = time.perf_counter() - t0
     return sample / dt, dt
+
+
+def col_rel(b, ref):
+    """(column-scaled, element-wise) relative errors of b against ref; the
+    element-wise one over entries with |ref| >= 1e-3 max|ref| of their column
+    (smaller coefficients carry the column's absolute error)."""
+    scale = np.maximum(np.max(np.abs(ref), axis=0), 1e-300)
+    cs = float(np.max(np.max(np.abs(b - ref), axis=0) / scale))
+    big = np.abs(ref) >= 1e-3 * scale
+    el = float(np.max(np.abs(b - ref)[big] / np.abs(ref)[big])) if big.any() else 0.0
+    return cs, el
+
+
+def run_cfg1(capi, synth, threads):
+    """n = 2000 uniform square, k = 4, Euclidean bisquare adaptive CV search over
+    [20, 2000] neighbours + fit with hat diagnostics: GPU FP64 and mixed, and
+    the oracle's full OpenMP run on the host."""
+    from oracle import pyoracle
+    d = synth.uniform_square(n=2000)
+    n = d["X"].shape[0]
+    out = {"n": n, "k": d["X"].shape[1]}
+    res = {}
+    with capi.Context(0) as ctx:
+        ctx.set_data(d["coords"], d["X"], d["y"])
+        for name, prec in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
+            ctx.gwr_run(capi.BISQUARE, True, 0, select_bw=True, crit=capi.CRIT_CV, precision=prec, lower=20,
+                        upper=2000)   # warm (the search's selects are cached from here on: cleared below)
+            ctx.set_data(d["coords"], d["X"], d["y"])
+            ctx.reset_timers()
+            t0 = time.perf_counter()
+            res[name] = ctx.gwr_run(capi.BISQUARE, True, 0, select_bw=True, crit=capi.CRIT_CV, precision=prec,
+                                    lower=20, upper=2000)
+            dt = time.perf_counter() - t0
+            out[name + "_seconds"] = dt
+            out[name + "_solves_per_s"] = ctx.solve_count() / dt
+    log("cfg1: oracle on %d threads" % threads)
+    t0 = time.perf_counter()
+    o = pyoracle.gwr_run(d["coords"], d["X"], d["y"], pyoracle.BISQUARE, True, 0, select_bw=True,
+                         crit=pyoracle.CRIT_CV, lower=20, upper=2000, nthreads=threads)
+    dt = time.perf_counter() - t0
+    evals = len(o["trace"])
+    out["cpu_seconds"] = dt
+    out["cpu_solves_per_s"] = (evals + 1) * n / dt
+    out["cpu_threads"] = threads
+    out["bandwidth"] = float(o["bandwidth"])
+    out["iterates_bitexact_fp64"] = [x for x, _ in res["fp64"]["trace"]] == [x for x, _ in o["trace"]]
+    out["iterates_bitexact_mixed"] = [x for x, _ in res["mixed"]["trace"]] == [x for x, _ in o["trace"]]
+    out["bandwidth_identical"] = res["fp64"]["bandwidth"] == o["bandwidth"] == res["mixed"]["bandwidth"]
+    errs = {}
+    for q in ("betas", "betasSE", "qDiag"):
+        errs[q] = max(col_rel(res["fp64"][q].reshape(n, -1), o[q].reshape(n, -1)))
+    errs["diagnostic"] = max(abs(res["fp64"]["diagnostic"][key] - o["diagnostic"][key]) / abs(o["diagnostic"][key])
+                             for key in o["diagnostic"])
+    out["fp64_max_rel_err_vs_oracle"] = errs
+    out["mixed_beta_rel_err_vs_oracle"] = col_rel(res["mixed"]["betas"], o["betas"])
+    return out
+
+
+def run_refbench(capi, synth, threads, sample):
+    """The reference's own benchmark shape (test/testGWRBasic.cpp:380-419): a
+    GWRBasic fit with hat diagnostics at n = 50,000, k = 3, Euclidean Gaussian
+    fixed bandwidth 0.2; the oracle's fit on a sample of focus points."""
+    from oracle import pyoracle
+    d = synth.reference_benchmark()
+    n = d["X"].shape[0]
+    out = {"n": n, "k": 3, "bandwidth": 0.2}
+    res = {}
+    with capi.Context(0) as ctx:
+        ctx.set_data(d["coords"], d["X"], d["y"])
+        for name, prec in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
+            ctx.gwr_run(capi.GAUSSIAN, False, 0.2, precision=prec)   # warm
+            t0 = time.perf_counter()
+            res[name] = ctx.gwr_run(capi.GAUSSIAN, False, 0.2, precision=prec)
+            dt = time.perf_counter() - t0
+            out[name + "_seconds"] = dt
+            out[name + "_solves_per_s"] = n / dt
+    log("refbench: oracle fit of %d focus points on %d threads" % (sample, threads))
+    t0 = time.perf_counter()
+    o = pyoracle.fit_core(d["coords"], d["X"], d["y"], pyoracle.GAUSSIAN, False, 0.2, nthreads=threads,
+                          focus_count=sample)
+    dt = time.perf_counter() - t0
+    out["cpu_solves_per_s"] = sample / dt
+    out["cpu_sample"] = sample
+    out["cpu_threads"] = threads
+    out["fp64_beta_rel_err_vs_oracle_sample"] = col_rel(res["fp64"]["betas"][:sample], o["betas"][:sample])
+    out["mixed_beta_rel_err_vs_fp64"] = col_rel(res["mixed"]["betas"], res["fp64"]["betas"])
+    out["mixed_over_fp64"] = out["fp64_seconds"] / out["mixed_seconds"]
+    return out
+
+
+def run_mgwr_cpu_parity(capi, synth, n, threads):
+    """Multiscale cosines at reduced n (the point density of n = 50,000): the
+    oracle's backfit on the host against the GPU's FP64 and mixed runs."""
+    from oracle import pyoracle
+    d = synth.multiscale_cosines(n=n, extent=1000.0 * np.sqrt(n / 50_000))
+    k = d["X"].shape[1]
+    args = ([capi.BISQUARE] * k, [1] * k, [0.0] * k, [0] * k, [capi.CRIT_CV] * k, [0] + [1] * (k - 1), [1e-5] * k)
+    out = {"n": n, "k": k}
+    res = {}
+    with capi.Context(0) as ctx:
+        for name, prec in (("fp64", capi.FP64), ("mixed", capi.MIXED)):
+            ctx.set_data(d["coords"], d["X"], d["y"])
+            t0 = time.perf_counter()
+            res[name] = ctx.mgwr_run(*args, crit_type=0, crit_threshold=1e-5, precision=prec)
+            out[name + "_seconds"] = time.perf_counter() - t0
+    log("mgwr parity: oracle n=%d on %d threads" % (n, threads))
+    t0 = time.perf_counter()
+    o = pyoracle.mgwr_run(d["coords"], d["X"], d["y"], *args, has_hat=False, crit_type=0, crit_threshold=1e-5,
+                          nthreads=threads)
+    out["cpu_seconds"] = time.perf_counter() - t0
+    out["cpu_threads"] = threads
+    out["bandwidths"] = [float(b) for b in o["bandwidths"]]
+    out["bandwidths_bitexact_fp64"] = bool(np.array_equal(res["fp64"]["bandwidths"], o["bandwidths"]))
+    out["bandwidths_bitexact_mixed"] = bool(np.array_equal(res["mixed"]["bandwidths"], o["bandwidths"]))
+    out["fp64_beta_rel_err_vs_oracle"] = col_rel(res["fp64"]["betas"], o["betas"])
+    out["mixed_beta_rel_err_vs_oracle"] = col_rel(res["mixed"]["betas"], o["betas"])
+    return out
 
 
 def run_batch(capi, synth, num, loop):
@@ -129,9 +301,25 @@ def run_mgwr(capi, synth, n):
             out[name + "_seconds"] = dt
             out[name + "_solves_per_s"] = ctx.solve_count() / dt
             out[name + "_iterations"] = r["iterations"]
+        # a third, kernel-timed run (HIP events synchronise after every phase,
+        # so the two runs above are untimed): the adaptive select's share
+        log("mgwr mixed n=%d, kernel timers on" % n)
+        ctx.set_data(d["coords"], d["X"], d["y"])
+        ctx.reset_timers()
+        ctx.set_timing(True)
+        ctx.mgwr_run([capi.BISQUARE] * k, [1] * k, [0.0] * k, [0] * k, [capi.CRIT_CV] * k,
+                     [0] + [1] * (k - 1), [1e-5] * k, crit_type=0, crit_threshold=1e-5, precision=capi.MIXED)
+        ctx.set_timing(False)
+        sel, sel_ms = ctx.kernel_time("select")
+    # one select finds the thresholds of n focus points among n data points:
+    # n^2 element visits in the reference's sort-based algorithm
+    out["selects"] = sel
+    out["select_ms"] = sel_ms
+    out["select_elements_per_s"] = sel * float(n) * n / (sel_ms * 1e-3) if sel_ms > 0 else None
     out["bandwidths"] = [float(b) for b in res["mixed"]["bandwidths"]]
     out["bandwidths_identical"] = bool(np.array_equal(res["mixed"]["bandwidths"], res["fp64"]["bandwidths"]))
     out["beta_rel_err_vs_fp64"] = beta_rel_err(res["mixed"]["betas"], res["fp64"]["betas"])
+    out["beta_elementwise_rel_err_vs_fp64"] = col_rel(res["mixed"]["betas"], res["fp64"]["betas"])[1]
     out["aicc_rel_err_vs_fp64"] = abs(res["mixed"]["diagnostic"]["AICc"] - res["fp64"]["diagnostic"]["AICc"]) / \
         abs(res["fp64"]["diagnostic"]["AICc"])
     return out
@@ -225,44 +413,62 @@ def serve_submit(capi, srv, r):
     return srv.submit(r["coords"], r["X"], r["y"], capi.BISQUARE, True, float(min(n - 1, 100)))
 
 
-def run_serve(capi, synth, num, rate, check):
-    """Serving stream replayed in real time; fits use an adaptive bisquare
-    kernel of 100 neighbours, searches CV over [20, n] neighbours."""
-    reqs = synth.serving_stream(num, rate=rate)
-    log("serve: %d requests at %.2f/s (%d searches)" % (num, rate, sum(r["search"] for r in reqs)))
-    with capi.Server(0, capi.MIXED, max_inflight=64) as srv:
-        start = time.perf_counter()
-        ids = []
-        for r in reqs:
-            delay = r["t"] - (time.perf_counter() - start)
-            if delay > 0:
-                time.sleep(delay)
-            ids.append(serve_submit(capi, srv, r))
-        res = [srv.wait(i) for i in ids]
-        wall = time.perf_counter() - start
-        st = srv.stats()
-    lat = np.array([r["latency_ms"] for r in res])
+def run_serve(capi, synth, num, rates, check):
+    """Serving stream replayed in real time at each rate (the same requests,
+    arrival times scaled); fits use an adaptive bisquare kernel of 100
+    neighbours, searches CV over [20, n] neighbours.  The beta error is taken
+    against the FP64 server on `check` requests spread evenly over the sizes
+    of the stream (searches and fits)."""
+    reqs = synth.serving_stream(num, rate=1.0)
+    log("serve: %d requests (%d searches), rates %s/s" % (num, sum(r["search"] for r in reqs), rates))
+    out = {"requests": num, "searches": int(sum(r["search"] for r in reqs)),
+           "n_max": int(max(r["X"].shape[0] for r in reqs)), "k_max": int(max(r["X"].shape[1] for r in reqs)),
+           "rates": []}
     fits = np.array([not r["search"] for r in reqs])
-    solves = sum(r["solves"] for r in res)
-    out = {"requests": num, "rate_per_s": rate, "searches": int((~fits).sum()), "seconds": wall,
-           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-           "max_ms": float(lat.max()), "fit_p50_ms": float(np.percentile(lat[fits], 50)),
-           "fit_p99_ms": float(np.percentile(lat[fits], 99)),
-           "search_p50_ms": float(np.percentile(lat[~fits], 50)) if (~fits).any() else None,
-           "solves": solves, "solves_per_s": solves / wall, "rounds": st["rounds"],
-           "peak_inflight": st["peak_inflight"], "failed": int(sum(r["status"] != capi.GWM_OK for r in res))}
+    last = None
+    for rate in rates:
+        log("serve: rate %.2f/s" % rate)
+        with capi.Server(0, capi.MIXED, max_inflight=64) as srv:
+            start = time.perf_counter()
+            ids = []
+            for r in reqs:
+                delay = r["t"] / rate - (time.perf_counter() - start)
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(serve_submit(capi, srv, r))
+            res = [srv.wait(i) for i in ids]
+            wall = time.perf_counter() - start
+            st = srv.stats()
+        lat = np.array([r["latency_ms"] for r in res])
+        solves = sum(r["solves"] for r in res)
+        out["rates"].append({
+            "rate_per_s": rate, "seconds": wall, "p50_ms": float(np.percentile(lat, 50)),
+            "p90_ms": float(np.percentile(lat, 90)), "p99_ms": float(np.percentile(lat, 99)),
+            "max_ms": float(lat.max()), "fit_p50_ms": float(np.percentile(lat[fits], 50)),
+            "fit_p99_ms": float(np.percentile(lat[fits], 99)),
+            "search_p50_ms": float(np.percentile(lat[~fits], 50)) if (~fits).any() else None,
+            "search_p99_ms": float(np.percentile(lat[~fits], 99)) if (~fits).any() else None,
+            "solves": solves, "solves_per_s": solves / wall, "rounds": st["rounds"],
+            "device_busy_frac": st["busy_ms"] * 1e-3 / wall, "peak_inflight": st["peak_inflight"],
+            "peak_device_bytes": st["peak_bytes"],
+            "failed": int(sum(r["status"] != capi.GWM_OK for r in res))})
+        last = res
     if check:
-        # beta error vs the FP64 server on the smallest requests of each type
         idx = sorted(range(num), key=lambda i: reqs[i]["X"].shape[0])
-        pick = [i for i in idx if reqs[i]["search"]][:check] + [i for i in idx if not reqs[i]["search"]][:check]
-        errs, same_bw = [], 0
+        pick = sorted(set(idx[int(round(q))] for q in np.linspace(0, num - 1, check)))
+        cs, el, same_bw = [], [], 0
         with capi.Server(0, capi.FP64, max_inflight=16) as srv:
             ref = [srv.wait(serve_submit(capi, srv, reqs[i])) for i in pick]
         for i, g in zip(pick, ref):
-            errs.append(beta_rel_err(res[i]["betas"], g["betas"]))
-            same_bw += res[i]["bandwidth"] == g["bandwidth"]
-        out["beta_rel_err_vs_fp64"] = max(errs)
+            a, b = col_rel(last[i]["betas"], g["betas"])
+            cs.append(a)
+            el.append(b)
+            same_bw += last[i]["bandwidth"] == g["bandwidth"]
         out["checked"] = len(pick)
+        out["checked_n"] = [int(reqs[i]["X"].shape[0]) for i in pick]
+        out["checked_searches"] = int(sum(reqs[i]["search"] for i in pick))
+        out["beta_rel_err_vs_fp64"] = max(cs)
+        out["beta_elementwise_rel_err_vs_fp64"] = max(el)
         out["bandwidths_identical"] = same_bw
     return out
 
@@ -290,10 +496,17 @@ def main():
     ap.add_argument("--predict-m", type=int, default=-1, help="prediction locations (default n; 0: skip)")
     ap.add_argument("--batch-problems", type=int, default=512, help="uneven batch size (0: skip)")
     ap.add_argument("--no-loop", action="store_true", help="skip the single-problem loop of the batch")
-    ap.add_argument("--serve-requests", type=int, default=64, help="serving stream length (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="serving-stream arrival rate, requests/s")
-    ap.add_argument("--serve-check", type=int, default=3,
-                    help="requests of each type re-run in FP64 for the beta error")
+    ap.add_argument("--serve-requests", type=int, default=500, help="serving stream length (0: skip)")
+    ap.add_argument("--serve-rates", type=str, default="10,40",
+                    help="serving-stream arrival rates, requests/s (comma-separated; one replay each)")
+    ap.add_argument("--serve-check", type=int, default=12,
+                    help="requests spread over the stream's sizes re-run in FP64 for the beta error")
+    ap.add_argument("--cfg1", type=int, default=1, help="n = 2000 uniform-square leg (0: skip)")
+    ap.add_argument("--refbench-n", type=int, default=50_000,
+                    help="the reference's own benchmark shape, n = 50,000, k = 3 (0: skip)")
+    ap.add_argument("--refbench-sample", type=int, default=8192, help="focus points of its CPU sample")
+    ap.add_argument("--mgwr-parity-n", type=int, default=1000,
+                    help="multiscale CPU parity size (the oracle's full backfit; 0: skip)")
     ap.add_argument("--sweeps-only", type=int, default=0,
                     help="profiling mode: run this many mixed AICc sweeps (and as many FP64 ones) and exit")
     ap.add_argument("--sweep-bw", type=float, default=61.890021059082855,
@@ -320,14 +533,16 @@ def main():
         return
 
     from libgwmodel_amd import capi, synth
+    rates = [float(r) for r in args.serve_rates.split(",") if r]
+    threads, aff = cpu_threads()
 
     if args.skip_headline:
         extra = {"mgwr": run_mgwr(capi, synth, args.mgwr_n) if args.mgwr_n else None,
                  "mgwr_hat": run_mgwr_hat(capi, synth, args.mgwr_hat_n) if args.mgwr_hat_n else None,
                  "batch": run_batch(capi, synth, args.batch_problems, not args.no_loop) if args.batch_problems else None,
-                 "serve": run_serve(capi, synth, args.serve_requests, args.serve_rate, args.serve_check)
+                 "serve": run_serve(capi, synth, args.serve_requests, rates, args.serve_check)
                  if args.serve_requests else None}
-        print(json.dumps(extra), flush=True)
+        print(json.dumps(extra, default=jsonable), flush=True)
         return
 
     n = args.n
@@ -339,10 +554,18 @@ def main():
     ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
 
     if args.sweeps_only:
+        ctx.reset_timers()
+        ctx.set_timing(True)
         for prec in (capi.MIXED, capi.FP64):
             for _ in range(args.sweeps_only):
                 ctx.criterion(capi.GAUSSIAN, False, args.sweep_bw, capi.CRIT_AIC, prec)
-        print(json.dumps({"sweeps": args.sweeps_only, "n": n}))
+        ctx.set_timing(False)
+        rep = {"sweeps": args.sweeps_only, "n": n, "bandwidth_km": args.sweep_bw}
+        for name in ("contract_mixed_plain", "contract_fp64_plain"):
+            launches, ms = ctx.kernel_time(name)
+            rep[name] = {"launches": launches, "avg_ms": ms / max(launches, 1),
+                         "pairs_per_launch": ctx.kernel_pairs(name) / max(launches, 1)}
+        print(json.dumps(rep))
         ctx.close()
         return
 
@@ -363,9 +586,8 @@ def main():
     solves_mixed = ctx.solve_count()
     launches, ms = ctx.kernel_time("contract_mixed_plain")
     swept_pairs = ctx.kernel_pairs("contract_mixed_plain")
-    l64_in_mixed, _ = ctx.kernel_time("contract_fp64_plain")
+    l64_in_mixed, ms64_in_mixed = ctx.kernel_time("contract_fp64_plain")
     hat_l, hat_ms = ctx.kernel_time("contract_mixed_hat")
-    value = solves_mixed / t_mixed
 
     # FP64 path, same step
     log("fp64 steps x%d" % args.fp64_steps)
@@ -378,6 +600,12 @@ def main():
     solves_fp64 = ctx.solve_count()
     l64, ms64 = ctx.kernel_time("contract_fp64_plain")
     swept64 = ctx.kernel_pairs("contract_fp64_plain")
+    # Solves per step = the (focus, bandwidth) evaluations of the search and
+    # the fit, each counted once: the FP64 step's count.  The mixed search takes
+    # the same iterates; its FP64 re-evaluations of uncertain decisions repeat
+    # candidates it evaluated already and are not counted again.
+    solves_per_step = solves_fp64 // args.fp64_steps
+    value = solves_per_step * args.steps / t_mixed
     value_fp64 = solves_fp64 / t_fp64
     m = n if args.predict_m < 0 else args.predict_m
     predict = run_predict(ctx, capi, d, mixed["bandwidth"], m) if m else None
@@ -387,29 +615,44 @@ def main():
     P = k * (k + 1) // 2 + k
     flops_per_launch = 2.0 * n * n * P          # n focus x n data x P columns, multiply-add = 2
     avg_ms = ms / max(launches, 1)
-    # executed work: the pairs the sweeps processed (pruning skips chunks whose
-    # weights are exactly zero); algorithmic: all n^2 pairs
+    # algorithmic work: every (focus, data) pair of the n solves a launch
+    # performs; executed: the pairs it actually streamed (pruning skips data
+    # chunks whose weights are exactly zero)
     swept_per_launch = swept_pairs / max(launches, 1)
     executed_flops = 2.0 * swept_per_launch * P
-    achieved = executed_flops / (avg_ms * 1e-3)
-    effective = flops_per_launch / (avg_ms * 1e-3)
+    achieved = flops_per_launch / (avg_ms * 1e-3)
     avg_ms64 = ms64 / max(l64, 1)
     traffic = load_traffic("contract_mixed_plain_n%d" % n)
+    valu = load_traffic("valu_contract_mixed_plain_n%d" % n)
+    # VALU issue bound of the launch: instructions per executed pair x pairs /
+    # the chip's lane-instruction rate (1024 SIMDs x 16 lanes x 2.4 GHz)
+    lane_rate = 1024 * 16 * 2.4e9
+    valu_bound_ms = (valu["valu_insts_per_pair"] * swept_per_launch / lane_rate * 1e3) if valu else None
 
     cpu = None
+    cpu_lin = None
     if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        log("cpu baseline: oracle, %d threads, %d focus points" % (threads, args.cpu_sample))
+        log("cpu baseline: oracle, %d threads (affinity %d), %d focus points" % (threads, aff, args.cpu_sample))
+        lin = [cpu_baseline(d, mixed["bandwidth"], s, threads)[1] for s in (1024, 2048)]
+        cpu_lin = {"seconds_1024": lin[0], "seconds_2048": lin[1], "ratio": lin[1] / lin[0]}
         cv, dt = cpu_baseline(d, mixed["bandwidth"], args.cpu_sample, threads)
         cpu = {"value": cv, "unit": "local WLS solves/s", "cores": threads, "kind": "port",
                "sample": "%d focus points x n=%d data, one AICc evaluation at the selected bandwidth, "
-                         "great-circle Gaussian, OpenMP (%.1f s)" % (args.cpu_sample, n, dt)}
+                         "great-circle Gaussian, OpenMP (%.1f s); 2048 vs 1024 focus points took %.2fx the time"
+                         % (args.cpu_sample, n, dt, cpu_lin["ratio"]),
+               "affinity_cores": aff, "cpu_model": cpu_model(), "linearity": cpu_lin}
 
-    mgwr = run_mgwr(capi, synth, args.mgwr_n) if args.mgwr_n else None
-    mgwr_hat = run_mgwr_hat(capi, synth, args.mgwr_hat_n) if args.mgwr_hat_n else None
-    batch = run_batch(capi, synth, args.batch_problems, not args.no_loop) if args.batch_problems else None
-    serve = run_serve(capi, synth, args.serve_requests, args.serve_rate, args.serve_check) \
+    cfg1 = leg("cfg1", run_cfg1, capi, synth, threads) if args.cfg1 else None
+    refbench = leg("refbench", run_refbench, capi, synth, threads, args.refbench_sample) if args.refbench_n else None
+    mgwr_parity = leg("mgwr_cpu_parity", run_mgwr_cpu_parity, capi, synth, args.mgwr_parity_n, threads) \
+        if args.mgwr_parity_n else None
+    mgwr = leg("mgwr", run_mgwr, capi, synth, args.mgwr_n) if args.mgwr_n else None
+    mgwr_hat = leg("mgwr_hat", run_mgwr_hat, capi, synth, args.mgwr_hat_n) if args.mgwr_hat_n else None
+    batch = leg("batch", run_batch, capi, synth, args.batch_problems, not args.no_loop) \
+        if args.batch_problems else None
+    serve = leg("serve", run_serve, capi, synth, args.serve_requests, rates, args.serve_check) \
         if args.serve_requests else None
+    cs, el = col_rel(mixed["betas"], fp64["betas"])
 
     line = {
         "metric": METRIC,
@@ -427,43 +670,55 @@ def main():
         "config": {"workload": "GWRBasic AICc golden-section bandwidth search + fit with hat diagnostics, "
                                "great-circle, Gaussian fixed bandwidth in [55.66, 2226.39] km",
                    "n": n, "k": k, "selected_bandwidth_km": mixed["bandwidth"],
-                   "solves_per_step": solves_mixed // args.steps,
-                   "fp64_certify_sweeps_in_mixed": l64_in_mixed},
+                   "solves_per_step": solves_per_step,
+                   "solves_executed_per_step_mixed": solves_mixed // args.steps,
+                   "fp64_certify_sweeps_per_step_in_mixed": l64_in_mixed / args.steps,
+                   "fp64_certify_ms_per_step_in_mixed": ms64_in_mixed / args.steps},
         "fp64_value": value_fp64,
         "fp64_ms_per_step": 1e3 * t_fp64 / args.fp64_steps,
+        # same search and fit, same solves: the wall-time ratio per step
         "mixed_over_fp64": value / value_fp64,
-        # same search and fit, wall time per step: the mixed step's FP64
-        # re-evaluations of certified decisions count as solves above, not here
-        "mixed_step_speedup": (t_fp64 / args.fp64_steps) / (t_mixed / args.steps),
         "mixed_sweep_over_fp64_sweep": avg_ms64 / avg_ms if avg_ms > 0 else None,
         "bandwidth_identical": mixed["bandwidth"] == fp64["bandwidth"],
-        "beta_rel_err_vs_fp64": beta_rel_err(mixed["betas"], fp64["betas"]),
+        "iterates_identical": [x for x, _ in mixed["trace"]] == [x for x, _ in fp64["trace"]],
+        "beta_rel_err_vs_fp64": cs,
+        "beta_elementwise_rel_err_vs_fp64": el,
+        "beta_elementwise_floor": "entries with |beta| >= 1e-3 max|beta| of their column",
         "se_rel_err_vs_fp64": beta_rel_err(mixed["betasSE"], fp64["betasSE"]),
         "aicc_rel_err_vs_fp64": abs(mixed["diagnostic"]["AICc"] - fp64["diagnostic"]["AICc"]) /
         abs(fp64["diagnostic"]["AICc"]),
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["mixed"] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK["mixed"],
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": (traffic.get("source"), traffic.get("kernel")) if traffic else None,
                      "kernel": "contract_mixed_kernel (AICc sweep)", "avg_launch_ms": avg_ms,
-                     "flops_per_launch": executed_flops,
-                     "swept_pair_fraction": swept_per_launch / (float(n) * n),
-                     "effective_tflops_all_pairs": effective / 1e12,
+                     "launches": launches,
+                     "flops_per_launch": flops_per_launch,
+                     "executed_pair_fraction": swept_per_launch / (float(n) * n),
+                     "executed_flops_per_launch": executed_flops,
+                     "executed_tflops": executed_flops / (avg_ms * 1e-3) / 1e12,
+                     "valu_insts_per_pair": valu["valu_insts_per_pair"] if valu else None,
+                     "valu_issue_bound_ms": valu_bound_ms,
+                     "valu_source": valu.get("source") if valu else None,
                      "fp64_kernel_avg_launch_ms": avg_ms64,
-                     "fp64_frac_of_fp64_peak": 2.0 * (swept64 / max(l64, 1)) * P / (avg_ms64 * 1e-3) / PEAK["fp64"]
-                     if l64 else None,
-                     "fp64_swept_pair_fraction": swept64 / max(l64, 1) / (float(n) * n)},
+                     "fp64_frac_of_fp64_peak": flops_per_launch / (avg_ms64 * 1e-3) / PEAK["fp64"] if l64 else None,
+                     "fp64_executed_pair_fraction": swept64 / max(l64, 1) / (float(n) * n)},
         "cpu_baseline": cpu,
-        "serve_p99_ms": serve["p99_ms"] if serve else None,
-        "serve_p50_ms": serve["p50_ms"] if serve else None,
+        "serve_p99_ms": serve["rates"][0]["p99_ms"] if serve and "rates" in serve else None,
+        "serve_p50_ms": serve["rates"][0]["p50_ms"] if serve and "rates" in serve else None,
+        "serve_rate_per_s": serve["rates"][0]["rate_per_s"] if serve and "rates" in serve else None,
         "serve": serve,
         "batch": batch,
+        "cfg1": cfg1,
+        "refbench": refbench,
+        "mgwr_cpu_parity": mgwr_parity,
         "mgwr": mgwr,
         "mgwr_hat": mgwr_hat,
         "predict": predict,
         "varsel": varsel,
         "autotune": autotune_summary(capi.autotune_report()),
     }
-    print(json.dumps(line), flush=True)
+    print(json.dumps(line, default=jsonable), flush=True)
 
 
 if __name__ == "__main__":

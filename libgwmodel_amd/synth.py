@@ -158,3 +158,17 @@ def serving_stream(num=64, rate=2.0, seed=1005, n_range=(500, 200_000)):
         search = bool(s.uniform(1)[0] < 0.3)
         reqs.append(dict(t=t, coords=coords, X=X, y=y, search=search))
     return reqs
+
+
+def reference_benchmark(n=50_000, seed=1006):
+    """The reference's own GWRBasic benchmark shape (test/testGWRBasic.cpp:380-419):
+    u, v ~ U[0,1], X = three N(0,1) columns (no intercept), beta0 = u + v,
+    beta1 = u^2 + v^2, beta2 = sin(u) + cos(v), noise N(0,1); fitted with a
+    Euclidean Gaussian kernel at the fixed bandwidth 0.2."""
+    s = Stream(seed)
+    u = s.uniform(n)
+    v = s.uniform(n)
+    X = np.column_stack([s.normal(n), s.normal(n), s.normal(n)])
+    B = np.column_stack([u + v, u * u + v * v, np.sin(u) + np.cos(v)])
+    y = np.sum(X * B, axis=1) + s.normal(n)
+    return dict(coords=np.column_stack([u, v]), X=X, y=y, beta=B)
