@@ -274,9 +274,23 @@ int gwm_hip_set_autotune(gwm_hip_ctx* ctx, int enabled);
  * shape in ms), NUL-terminated and truncated to cap bytes. */
 int gwm_hip_autotune_report(char* buf, int cap);
 
+/* Distance table of great-circle problems (no counterpart in the reference,
+ * whose every criterion evaluation forms each pair's distance again,
+ * GWRBasic.cpp:440-535).  Within one gwm_hip_select_bandwidth,
+ * gwm_hip_gwr_run or gwm_hip_mgwr_run call, the first mixed-precision sweep
+ * of great-circle data stores the FP32 squared distances of all pairs in
+ * device memory (4 n^2 bytes: 40 GB at n = 100,000), and every later mixed
+ * sweep of the call reads them instead of forming them: the same values, so
+ * the same results.  The table is released when the call returns.  It is
+ * built only when it needs at most max_bytes (default 96 GiB) and the device
+ * keeps 8 GiB free beside it; max_bytes = 0 turns it off.  FP64 sweeps never
+ * use it. */
+int gwm_hip_set_distance_table(gwm_hip_ctx* ctx, double max_bytes);
+
 /* Timing of the device kernels with HIP events on the context stream
  * (enabled by gwm_hip_set_timing).  name: "contract_<fp64|mixed>_<cv|plain|
- * hat|qdiag>" for the sweeps, "select" (adaptive bandwidths), "solve". */
+ * hat|qdiag>" for the sweeps, "select" (adaptive bandwidths), "solve",
+ * "d2_table" (the distance table). */
 int gwm_hip_set_timing(gwm_hip_ctx* ctx, int enabled);
 int gwm_hip_kernel_time(gwm_hip_ctx* ctx, const char* name, long long* launches, double* total_ms);
 /* Focus x data pairs the named sweeps actually processed while timing was on

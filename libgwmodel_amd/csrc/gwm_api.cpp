@@ -90,7 +90,11 @@ void free_problem(gwm_hip_ctx* c) {
     dfree(c->vZ); dfree(c->S); dfree(c->fail);
     dfree(c->Zh); dfree(c->Zl); dfree(c->Gh); dfree(c->Gl); dfree(c->vZh); dfree(c->vZl);
     dfree(c->zscale); dfree(c->gscale); dfree(c->vscale); dfree(c->colmax);
-    // the plan buffers come from hipMalloc (ensure_plan) on every context
+    // the plan buffers and the distance table come from hipMalloc on every context
+    if (c->d2tab) hipFree(c->d2tab);
+    c->d2tab = nullptr;
+    c->d2tab_bytes = 0;
+    c->d2_valid = false;
     if (c->plan_counts) hipFree(c->plan_counts);
     if (c->plan_order) hipFree(c->plan_order);
     c->plan_counts = nullptr;
@@ -264,6 +268,60 @@ int timer_end(gwm_hip_ctx* c, const char* name) {
     return GWM_OK;
 }
 
+// The squared-distance table of a great-circle problem (launch_d2_table),
+// built once per outermost search / fit / backfit call.  It is optional:
+// when it exceeds d2_cap, or the device cannot hold it beside 8 GiB for
+// everything else, the sweeps form their distances themselves.
+int ensure_d2_table(gwm_hip_ctx* c) {
+    if (c->d2_valid || c->d2_scope <= 0 || c->async_mem || c->dist != kGreatCircle) return GWM_OK;
+    const size_t bytes = (size_t)c->n_pad * (size_t)c->rows_pad * sizeof(float);
+    if (bytes > c->d2_cap) return GWM_OK;
+    if (c->d2tab_bytes < bytes) {
+        if (c->d2tab) hipFree(c->d2tab);
+        c->d2tab = nullptr;
+        c->d2tab_bytes = 0;
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b < bytes + (size_t(8) << 30)) {
+            (void)hipGetLastError();
+            return GWM_OK;
+        }
+        if (hipMalloc((void**)&c->d2tab, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            c->d2tab = nullptr;
+            return GWM_OK;
+        }
+        c->d2tab_bytes = bytes;
+    }
+    timer_begin(c);
+    GWM_HIP_CHECK(c, launch_d2_table(c->points(), c->n, c->n_pad, c->rows_pad, c->d2tab, c->stream));
+    int rc = timer_end(c, "d2_table");
+    if (rc) return rc;
+    c->d2_valid = true;
+    return GWM_OK;
+}
+
+// Scope of an outermost call whose mixed great-circle sweeps share one
+// distance table; the table is released when that call returns.
+struct D2Scope {
+    gwm_hip_ctx* c;
+    explicit D2Scope(gwm_hip_ctx* ctx) : c(ctx) {
+        if (c->d2_scope++ == 0) c->d2_valid = false;
+    }
+    ~D2Scope() {
+        if (--c->d2_scope == 0) {
+            c->d2_valid = false;
+            if (c->d2tab) {
+                hipStreamSynchronize(c->stream);
+                hipFree(c->d2tab);
+            }
+            c->d2tab = nullptr;
+            c->d2tab_bytes = 0;
+        }
+    }
+    D2Scope(const D2Scope&) = delete;
+    D2Scope& operator=(const D2Scope&) = delete;
+};
+
 // Resolve the bandwidth: adaptive -> per-focus thresholds (a vector of the
 // bandwidth cache, filled by the select on a miss).
 int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw_pt) {
@@ -392,6 +450,17 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
     L.jobs = nullptr;
     L.items = nullptr;
     L.nitems = 0;
+    // mixed great-circle sweeps inside a search / fit / backfit call read the
+    // pairs' squared distances from the table (same values, computed once)
+    int ldist = c->dist;
+    if (mixed && c->dist == kGreatCircle) {
+        if ((rc = ensure_d2_table(c))) return rc;
+        if (c->d2_valid) {
+            j.d2tab = c->d2tab;
+            j.d2_ndb = c->rows_pad / 32;
+            ldist = kGcTable;
+        }
+    }
     if (c->timing) {
         if (!c->pairs_dev) GWM_HIP_CHECK(c, hipMalloc((void**)&c->pairs_dev, sizeof(unsigned long long)));
         GWM_HIP_CHECK(c, hipMemsetAsync(c->pairs_dev, 0, sizeof(unsigned long long), c->stream));
@@ -402,7 +471,7 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
         L.linear_order = c->force_shape & 1;
         L.shape_r = c->force_shape >= 2 ? 1 : 0;
     } else if (c->autotune) {
-        TuneKey key{c->dist, mode, mixed ? 1 : 0, nta, ntb, 0};
+        TuneKey key{ldist, mode, mixed ? 1 : 0, nta, ntb, 0};
         while ((2 << key.nlog2) <= c->n) ++key.nlog2;
         int chosen;
         {
@@ -411,7 +480,7 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
             chosen = it == g_tune.end() ? -1 : it->second.chosen;
         }
         if (chosen < 0) {
-            const int r = contract_r(nta, ntb, mixed ? 1 : 0, mode, c->dist);
+            const int r = contract_r(nta, ntb, mixed ? 1 : 0, mode, ldist);
             const int grid = (c->n + 64 * r - 1) / (64 * r) * nseg;
             TuneState st;
             st.ncand = grid >= kTuneGridMax ? 1 : (r == 2 ? 4 : 2);
@@ -426,7 +495,7 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
                         T.shape_r = q >= 2 ? 1 : 0;
                         float ms = 0.0f;
                         GWM_HIP_CHECK(c, hipEventRecord(c->tev0, c->stream));
-                        if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(c->dist, mode, nta, ntb, T, c->stream));
+                        if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(ldist, mode, nta, ntb, T, c->stream));
                         else GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, T, c->stream));
                         GWM_HIP_CHECK(c, hipEventRecord(c->tev1, c->stream));
                         GWM_HIP_CHECK(c, hipEventSynchronize(c->tev1));
@@ -447,7 +516,7 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
         // Large pruned sweeps run their (tile, segment) items longest first
         // (gwm_plan.hip): narrow bandwidths leave a few busy items per tile,
         // and in plain order the sweep ends on a tail of them.
-        const int r = contract_r(nta, ntb, mixed ? 1 : 0, mode, c->dist);
+        const int r = contract_r(nta, ntb, mixed ? 1 : 0, mode, ldist);
         const int tiles = (c->n + 64 * r - 1) / (64 * r);
         if (c->prune && chosen == 0 && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg) {
             if ((rc = ensure_plan(c, (size_t)tiles * nseg))) return rc;
@@ -461,7 +530,7 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
         }
     }
     timer_begin(c);
-    if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(c->dist, mode, nta, ntb, L, c->stream));
+    if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(ldist, mode, nta, ntb, L, c->stream));
     else GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, L, c->stream));
     if (c->timing) {
         std::string name = std::string("contract_") + (mixed ? "mixed_" : "fp64_") + mode_name(mode);
@@ -1700,6 +1769,7 @@ int gwm_hip_select_bandwidth(gwm_hip_ctx* c, int kernel, int adaptive, int crit,
     if (rc) return rc;
     if ((rc = check_kernel(c, kernel, precision))) return rc;
     GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    D2Scope scope(c);
     if (len) *len = 0;
     double lo = has_lower ? lower : (adaptive ? 20 : 0.0);
     double hi;
@@ -1726,6 +1796,7 @@ int gwm_hip_gwr_run(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int sel
     int rc = check_ready(c);
     if (rc) return rc;
     if (!betas || !betasSE || !shat || !diag) return GWM_ERR_INVALID;
+    D2Scope scope(c);
     if (len) *len = 0;
     if (select_bw) {
         tele_print(c, kLogInfo, kStage + "Bandwidth selection");
@@ -1809,6 +1880,7 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         !bws_out || !diag)
         return GWM_ERR_INVALID;
     GWM_HIP_CHECK(c, hipSetDevice(c->device));
+    D2Scope scope(c);
     const int n = c->n, k = c->k;
     if (!c->mgwr_dists.empty() && (int)c->mgwr_dists.size() != k) {
         c->err = "mgwr: the per-variable distances were set for " + std::to_string(c->mgwr_dists.size()) +
@@ -2176,6 +2248,12 @@ int gwm_hip_set_autotune(gwm_hip_ctx* c, int enabled) {
     return GWM_OK;
 }
 
+int gwm_hip_set_distance_table(gwm_hip_ctx* c, double max_bytes) {
+    if (!c || !(max_bytes >= 0.0)) return GWM_ERR_INVALID;
+    c->d2_cap = max_bytes >= 1.8e19 ? SIZE_MAX : (size_t)max_bytes;
+    return GWM_OK;
+}
+
 int gwm_hip_autotune_report(char* buf, int cap) {
     if (!buf || cap <= 0) return GWM_ERR_INVALID;
     static const char* modes[] = {"cv", "hat", "qdiag", "plain"};   // ContractMode order
@@ -2187,7 +2265,8 @@ int gwm_hip_autotune_report(char* buf, int cap) {
             const TuneState& st = kv.second;
             char line[256];
             int m = std::snprintf(line, sizeof(line), "%s %s %s nta=%d ntb=%d n<2^%d chosen=%d ms=",
-                                  k.dist == GWM_GREAT_CIRCLE ? "great_circle" : "euclidean",
+                                  k.dist == kGcTable ? "great_circle_table"
+                                                     : (k.dist == GWM_GREAT_CIRCLE ? "great_circle" : "euclidean"),
                                   k.mode >= 0 && k.mode < 4 ? modes[k.mode] : "?", k.prec ? "mixed" : "fp64", k.nta,
                                   k.ntb, k.nlog2 + 1, st.chosen);
             for (int q = 0; q < st.ncand && m < (int)sizeof(line) - 16; ++q)

@@ -51,7 +51,17 @@ struct SweepJob {
     // exponent cut of the Gaussian and exponential kernels is taken relative
     // to that point's weight (make_kconst64_nn).  Null: relative to 1.
     const double* dnn = nullptr;
+    // mixed great-circle sweeps of the data points against themselves: the
+    // FP32 squared distances of every (focus, data) pair, precomputed by
+    // launch_d2_table (layout there); the sweep is then launched with the
+    // distance id kGcTable and reads them instead of forming them
+    const float* d2tab = nullptr;
+    int d2_ndb = 0;          // 32-point data blocks per focus group (rows / 32)
 };
+
+// Distance id of a mixed sweep that reads the precomputed squared great-circle
+// distances (SweepJob::d2tab); pruning still uses the great-circle boxes.
+constexpr int kGcTable = 2;
 
 // Launch of one sweep: either a single job (items == nullptr; workgroup b
 // covers tile b / nseg and segment b % nseg) or a batch (items: triples
@@ -98,6 +108,16 @@ hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const Swe
 // the R (focus tiles per wave) each kernel uses for an (nta, ntb, mode,
 // distance) instance; launchers and item tables must agree on it
 int contract_r(int nta, int ntb, int precision_mixed, int mode, int dist);
+
+// Squared great-circle distances (km^2, FP32) of every pair of the n points of
+// P, exactly as the mixed sweep forms them: focus i < n_pad (rows past n repeat
+// point n - 1), data j < rows (a multiple of 64; columns past n repeat point
+// n - 1).  Layout: blocks of 16 focus x 32 data points, block (i / 16, j / 32)
+// at ((i / 16) * (rows / 32) + j / 32) * 512 floats; inside a block the entry
+// of focus 16 g + c and data 32 b + 8 q + e sits at (e / 4) * 256 + (16 q + c)
+// * 4 + e % 4, so a wave reads its 16 x 32 operand tile as two contiguous
+// kilobytes.  n_pad * rows * 4 bytes.
+hipError_t launch_d2_table(const Points& P, int n, int n_pad, int rows, float* tab, hipStream_t s);
 
 // Work order of a pruned single-job sweep whose focus points are the points
 // of F in order (the data table itself, or the prediction locations) against

@@ -53,6 +53,10 @@ struct MCoords<kGreatCircle> {
     double hx[kChunkM], hy[kChunkM], hz[kChunkM];
     float sl[kChunkM];   // sin(lat): only enters the flattening term
 };
+template <>
+struct MCoords<kGcTable> {   // distances come from the precomputed table
+    int unused;
+};
 
 // Kernel weight from the squared distance (Euclidean) or from the distance.
 struct BwConst {
@@ -71,10 +75,17 @@ __device__ __forceinline__ BwConst make_bw(float b) {
 
 template <int NTA, int DIST, int MODE>
 struct alignas(16) MShared {
-    _Float16 zh[2][NTA * 16][kColStride];
-    _Float16 zl[2][NTA * 16][kColStride];
+    // table sweeps stage the next chunk's planes while the current one is in
+    // use: two buffers
+    static constexpr int kBuf = DIST == kGcTable ? 2 : 1;
+    _Float16 zh[kBuf][2][NTA * 16][kColStride];
+    _Float16 zl[kBuf][2][NTA * 16][kColStride];
     MCoords<DIST> c;
-    BwConst bwc[MODE == kModeQDiag ? kChunkM : 1];   // qDiag: constants of each data point's bandwidth
+    BwConst bwc[kBuf][MODE == kModeQDiag ? kChunkM : 1];   // qDiag: constants of each data point's bandwidth
+    // table sweeps: the segment's chunks that pruning keeps (at most 128:
+    // segments hold at most 8192 points), in order
+    unsigned char live[DIST == kGcTable ? 128 : 1];
+    int nlive;
 };
 
 template <int DIST>
@@ -100,16 +111,25 @@ __device__ __forceinline__ MFocus<DIST> load_mfocus(const GPoints& P, int f) {
     return fc;
 }
 
+// sqrt(d2) inside one branch of a kernel switch.  The exponential and tricube
+// branches both take the root of the same d2; without the opaque copy the
+// compiler hoists that common root (with its denormal scaling, about ten
+// instructions) above the switch, where every kernel pays for it.
+__device__ __forceinline__ float branch_sqrt(float d2) {
+    asm volatile("" : "+v"(d2));
+    return __builtin_sqrtf(d2);
+}
+
 __device__ __forceinline__ float weight_from_d2(int kernel, float d2, const BwConst& c) {
     switch (kernel) {
     case kGaussian: return __builtin_amdgcn_exp2f(d2 * c.gauss);
-    case kExponential: return __builtin_amdgcn_exp2f(-1.4426950408889634f * __builtin_sqrtf(d2) * c.inv_bw);
+    case kExponential: return __builtin_amdgcn_exp2f(-1.4426950408889634f * branch_sqrt(d2) * c.inv_bw);
     case kBisquare: {
         float t = 1.0f - d2 * c.inv_bw2;
         return d2 < c.bw2 ? t * t : 0.0f;
     }
     case kTricube: {
-        float q = __builtin_sqrtf(d2) * c.inv_bw;
+        float q = branch_sqrt(d2) * c.inv_bw;
         float t = 1.0f - q * q * q;
         return d2 < c.bw2 ? (t * t) * t : 0.0f;
     }
@@ -148,6 +168,41 @@ __device__ __forceinline__ float gc_d2_near_f32(float S, float sp, float sm) {
     return S * (q2 * (g2a * g2a));
 }
 
+// Squared great-circle distances of one focus point to the 8 data points
+// row(0..7) (each (hx, hy, hz, sin lat)): the near form for S <= 1/4, then,
+// in a pass the wave takes only when one of its pairs needs it, the general
+// tail for pairs beyond 60 degrees of arc.  The distance table
+// (d2_table_kernel) forms its entries here; the great-circle sweep writes the
+// same expressions out in its own loop, so both give the same roundings.
+struct GcRow {
+    double hx, hy, hz;
+    float sl;
+};
+template <class ROW>
+__device__ __forceinline__ void gc_d2_8(double fa, double fb, double fcz, float fsl, ROW row, float* d2) {
+    float smax = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const GcRow q = row(e);
+        const double ex = q.hx - fa, ey = q.hy - fb, ez = q.hz - fcz;
+        const float S = (float)(ex * ex + ey * ey + ez * ez);
+        d2[e] = gc_d2_near_f32(S, fsl + q.sl, q.sl - fsl);
+        smax = fmaxf(smax, S);
+    }
+    if (__any(smax > 0.25f)) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const GcRow q = row(e);
+            const double ex = q.hx - fa, ey = q.hy - fb, ez = q.hz - fcz;
+            const float S = (float)(ex * ex + ey * ey + ez * ez);
+            if (S > 0.25f) {
+                const float d = gcdist_tail_f32(S, fsl, q.sl);
+                d2[e] = d * d;
+            }
+        }
+    }
+}
+
 // Kernel weights of 8 squared distances (the kernel switch is taken once),
 // times 2^11 when SCALED (added to the exponent of the exponential kernels).
 template <bool SCALED>
@@ -162,7 +217,7 @@ __device__ __forceinline__ void weights8_from_d2(int kernel, const float* d2, co
     case kExponential:
 #pragma unroll
         for (int e = 0; e < 8; ++e)
-            w[e] = __builtin_amdgcn_exp2f(fmaf(-1.4426950408889634f * __builtin_sqrtf(d2[e]), c.inv_bw, e0));
+            w[e] = __builtin_amdgcn_exp2f(fmaf(-1.4426950408889634f * branch_sqrt(d2[e]), c.inv_bw, e0));
         break;
     case kBisquare:
 #pragma unroll
@@ -174,7 +229,7 @@ __device__ __forceinline__ void weights8_from_d2(int kernel, const float* d2, co
     case kTricube:
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            float q = __builtin_sqrtf(d2[e]) * c.inv_bw;
+            float q = branch_sqrt(d2[e]) * c.inv_bw;
             float t = 1.0f - q * q * q;
             w[e] = d2[e] < c.bw2 ? ((t * t) * t) * m0 : 0.0f;
         }
@@ -219,8 +274,21 @@ __device__ __forceinline__ float square_rounded(float w) {
     return w2;
 }
 
+// Table sweeps stream their distances from HBM: three waves per SIMD (at most
+// 168 registers) hide more of that latency.  Asked for the AICc and CV
+// sweeps of up to three column tiles, which fit that cap without spilling
+// (164 and 167 registers at three tiles); the others keep the default.
+template <int NTA, int NTB, int DIST, int MODE>
+constexpr int mixed_min_waves() {
+    return (DIST == kGcTable && (MODE == kModePlain || MODE == kModeCV) && NTB == 0 && NTA <= 3) ? 3 : 1;
+}
+
 template <int NTA, int NTB, int DIST, int MODE, int R>
-__global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mixed_min_waves<NTA, NTB, DIST, MODE>())))
+void contract_mixed_kernel(SweepLaunch L) {
+    // the distance table sweeps great-circle data (boxes, focus coordinates)
+    constexpr bool kTab = DIST == kGcTable;
+    constexpr int GEO = kTab ? kGreatCircle : DIST;
     __shared__ MShared<NTA, DIST, MODE> sh;
     // The job is copied into registers once (uniform scalar loads): a
     // reference into memory would be re-read after every LDS store.
@@ -263,7 +331,7 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     const int kq = lane >> 4;       // which 8-row group of the 32-row step
     const int col = lane & 15;
 
-    MFocus<DIST> fc[R];
+    MFocus<GEO> fc[R];
     BwConst fbw[R];
     int fidx[R];
     double fb = 0.0;   // largest focus bandwidth of this lane
@@ -273,13 +341,13 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
         int fl = f < nf ? f : nf - 1;
         if (focus_idx) fl = focus_idx[fl];
         fidx[r] = qtab ? -1 : (focus_idx ? (f < nf ? fl : -1) : f);   // self index for the CV exclusion
-        fc[r] = load_mfocus<DIST>(FP, fl);
+        fc[r] = load_mfocus<GEO>(FP, fl);
         const double b = (MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : args.bw;
         fbw[r] = make_bw((float)b);
         fb = fmax(fb, b);
     }
     // pruning: the workgroup's focus box and the zero-weight threshold
-    constexpr int BD = DIST == kEuclidean ? 2 : 3;
+    constexpr int BD = GEO == kEuclidean ? 2 : 3;
     const bool prune = args.boxes != nullptr;
     double flo[BD], fhi[BD], thr = INFINITY;
     if (prune) {
@@ -299,8 +367,8 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
         }
         workgroup_box<BD>(flo, fhi, fb, &fb);
         // qDiag: the streamed points' bandwidths (per chunk when adaptive)
-        if (MODE != kModeQDiag) thr = prune_threshold<DIST>(zero_weight_distance(args.kernel, fb, true));
-        else if (!args.chunk_bw) thr = prune_threshold<DIST>(zero_weight_distance(args.kernel, args.bw, true));
+        if (MODE != kModeQDiag) thr = prune_threshold<GEO>(zero_weight_distance(args.kernel, fb, true));
+        else if (!args.chunk_bw) thr = prune_threshold<GEO>(zero_weight_distance(args.kernel, args.bw, true));
     }
 
     v4f hh[R][NTA + NTB], xx[R][NTA + NTB], ll[R][NTA + NTB];
@@ -314,186 +382,305 @@ __global__ __launch_bounds__(256) void contract_mixed_kernel(SweepLaunch L) {
     const int jend = min(n, jbeg + args.seg_len);
 
     int swept = 0;   // chunks processed (not pruned)
-    for (int j0 = jbeg; j0 < jend; j0 += kChunkM) {
-        if (prune) {
-            const int ch = j0 / kChunkM;
-            double t = thr;
-            if (MODE == kModeQDiag && args.chunk_bw)
-                t = prune_threshold<DIST>(zero_weight_distance(args.kernel, gptr(args.chunk_bw)[ch], true));
-            const int skip = box_gap2<DIST>(flo, fhi, gptr(args.boxes) + (size_t)ch * 2 * BD) > t;
-            if (__builtin_amdgcn_readfirstlane(skip)) continue;
+    // table sweeps: a chunk's two 16 x 32 distance blocks per focus tile (rows
+    // are padded to 64, so both exist even in the last chunk)
+    v4f tq[R][2][2], nq[R][2][2];
+    // Table sweeps run a software pipeline.  The pruning tests of the whole
+    // segment run once (wave 0, into the live list); then, per live chunk, the
+    // next live chunk's distances and FP16 planes are loaded into registers
+    // before the current chunk is computed, and stored into the other LDS
+    // buffer after it: memory latency overlaps a chunk of work, with one
+    // barrier per chunk.  (Vector-memory loads complete in issue order, so
+    // loads issued just before a wait would hold back an earlier prefetch.)
+    int nlive = 0;
+    constexpr int kZv = (2 * NTA * 16 * 4 + 255) / 256;   // 16-byte vectors per thread and plane
+    const int vecs = 2 * ldz * 4;
+    v4u zhq[kZv], zlq[kZv];
+    double bwq = 0.0;
+    auto load_next = [&](int j) {
+        GWM_GLOBAL const float* tab = gptr(args.d2tab);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const size_t fg = (size_t)((fbase + 16 * r) >> 4);
+            GWM_GLOBAL const float* b = tab + (fg * args.d2_ndb + (size_t)(j >> 5)) * 512 + 4 * lane;
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2) {
+                nq[r][s2][0] = *reinterpret_cast<GWM_GLOBAL const v4f*>(b + 512 * s2);
+                nq[r][s2][1] = *reinterpret_cast<GWM_GLOBAL const v4f*>(b + 512 * s2 + 256);
+            }
         }
-        ++swept;
-        __syncthreads();
-        // stage the two 32-row blocks of both planes: global [blk][col][32]
-        {
-            const size_t base = (size_t)(j0 / 32) * ldz * 32;
-            GWM_GLOBAL const v4u* gh = reinterpret_cast<GWM_GLOBAL const v4u*>(gptr(args.Zh) + base);
-            GWM_GLOBAL const v4u* gl = reinterpret_cast<GWM_GLOBAL const v4u*>(gptr(args.Zl) + base);
-            const int vecs = 2 * ldz * 4;   // 16-byte vectors per plane (8 halfs each)
-            for (int e = threadIdx.x; e < vecs; e += 256) {
+        const size_t base = (size_t)(j / 32) * ldz * 32;
+        GWM_GLOBAL const v4u* gh = reinterpret_cast<GWM_GLOBAL const v4u*>(gptr(args.Zh) + base);
+        GWM_GLOBAL const v4u* gl = reinterpret_cast<GWM_GLOBAL const v4u*>(gptr(args.Zl) + base);
+#pragma unroll
+        for (int v = 0; v < kZv; ++v) {
+            const int e = threadIdx.x + 256 * v;
+            if (e < vecs) {
+                zhq[v] = gh[e];
+                zlq[v] = gl[e];
+            }
+        }
+        if constexpr (MODE == kModeQDiag) {
+            if (threadIdx.x < kChunkM) {
+                const int jj = j + threadIdx.x;
+                bwq = bw_pt ? bw_pt[jj < n ? jj : n - 1] : args.bw;
+            }
+        }
+    };
+    auto store_next = [&](int zb) {
+#pragma unroll
+        for (int v = 0; v < kZv; ++v) {
+            const int e = threadIdx.x + 256 * v;
+            if (e < vecs) {
                 const int blk = e / (ldz * 4);
                 const int rem = e % (ldz * 4);
                 const int c = rem >> 2, part = rem & 3;
-                *reinterpret_cast<v4u*>(&sh.zh[blk][c][part * 8]) = gh[e];
-                *reinterpret_cast<v4u*>(&sh.zl[blk][c][part * 8]) = gl[e];
+                *reinterpret_cast<v4u*>(&sh.zh[zb][blk][c][part * 8]) = zhq[v];
+                *reinterpret_cast<v4u*>(&sh.zl[zb][blk][c][part * 8]) = zlq[v];
             }
         }
-        if (threadIdx.x < kChunkM) {
-            int j = j0 + threadIdx.x;
-            int jl = j < n ? j : n - 1;
-            if constexpr (DIST == kEuclidean) {
-                sh.c.x[threadIdx.x] = P.px[jl];
-                sh.c.y[threadIdx.x] = P.py[jl];
-            } else {
-                sh.c.hx[threadIdx.x] = P.hx[jl];
-                sh.c.hy[threadIdx.x] = P.hy[jl];
-                sh.c.hz[threadIdx.x] = P.hz[jl];
-                sh.c.sl[threadIdx.x] = (float)P.sl[jl];
+        if constexpr (MODE == kModeQDiag) {
+            if (threadIdx.x < kChunkM) sh.bwc[zb][threadIdx.x] = make_bw((float)bwq);
+        }
+    };
+    if constexpr (kTab) {
+        if (wave == 0) {
+            const int c0 = jbeg / kChunkM, nch = (jend - jbeg + kChunkM - 1) / kChunkM;   // nch <= 128
+            int cnt = 0;
+            for (int b = 0; b < nch; b += 64) {
+                const int ci = b + lane;
+                int ok = ci < nch;
+                if (ok && prune) {
+                    double t = thr;
+                    if (MODE == kModeQDiag && args.chunk_bw)
+                        t = prune_threshold<GEO>(zero_weight_distance(args.kernel, gptr(args.chunk_bw)[c0 + ci], true));
+                    ok = !(box_gap2<GEO>(flo, fhi, gptr(args.boxes) + (size_t)(c0 + ci) * 2 * BD) > t);
+                }
+                const unsigned long long m = __ballot(ok);
+                if (ok) sh.live[cnt + __popcll(m & ((1ull << lane) - 1ull))] = (unsigned char)ci;
+                cnt += __popcll(m);
             }
-            if constexpr (MODE == kModeQDiag) sh.bwc[threadIdx.x] = make_bw((float)(bw_pt ? bw_pt[jl] : args.bw));
+            if (lane == 0) sh.nlive = cnt;
         }
         __syncthreads();
+        nlive = sh.nlive;
+        if (nlive > 0) {
+            load_next(jbeg + kChunkM * sh.live[0]);
+            store_next(0);
+        }
+        __syncthreads();
+    }
+    int zb = 0;
+    for (int i = 0;; ++i) {
+        int j0;
+        bool more = false;
+        if constexpr (kTab) {
+            if (i >= nlive) break;
+            j0 = jbeg + kChunkM * sh.live[i];
 #pragma unroll
-        for (int r = 0; r < R; ++r)
+            for (int r = 0; r < R; ++r)
 #pragma unroll
-            for (int t = 0; t < NTA + NTB; ++t) {
-                hh[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
-                xx[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
-                ll[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
+                for (int s2 = 0; s2 < 2; ++s2) {
+                    tq[r][s2][0] = nq[r][s2][0];
+                    tq[r][s2][1] = nq[r][s2][1];
+                }
+            more = i + 1 < nlive;
+            if (more) load_next(jbeg + kChunkM * sh.live[i + 1]);
+        } else {
+            j0 = jbeg + kChunkM * i;
+            if (j0 >= jend) break;
+            if (prune) {
+                const int ch = j0 / kChunkM;
+                double t = thr;
+                if (MODE == kModeQDiag && args.chunk_bw)
+                    t = prune_threshold<GEO>(zero_weight_distance(args.kernel, gptr(args.chunk_bw)[ch], true));
+                const int skip = box_gap2<GEO>(flo, fhi, gptr(args.boxes) + (size_t)ch * 2 * BD) > t;
+                if (__builtin_amdgcn_readfirstlane(skip)) continue;
             }
-        const int nsteps = (jend - j0) > 32 ? 2 : 1;
-        for (int s = 0; s < nsteps; ++s) {
-            // B operands: column col of each tile, rows 8*kq .. 8*kq+7 of block s
-            v8h bh[NTA], bl[NTA];
-#pragma unroll
-            for (int t = 0; t < NTA; ++t) {
-                if (t < nta) {
-                    bh[t] = *reinterpret_cast<const v8h*>(&sh.zh[s][16 * t + col][8 * kq]);
-                    bl[t] = *reinterpret_cast<const v8h*>(&sh.zl[s][16 * t + col][8 * kq]);
+            __syncthreads();
+            // stage the two 32-row blocks of both planes: global [blk][col][32]
+            {
+                const size_t base = (size_t)(j0 / 32) * ldz * 32;
+                GWM_GLOBAL const v4u* gh = reinterpret_cast<GWM_GLOBAL const v4u*>(gptr(args.Zh) + base);
+                GWM_GLOBAL const v4u* gl = reinterpret_cast<GWM_GLOBAL const v4u*>(gptr(args.Zl) + base);
+                for (int e = threadIdx.x; e < vecs; e += 256) {
+                    const int blk = e / (ldz * 4);
+                    const int rem = e % (ldz * 4);
+                    const int c = rem >> 2, part = rem & 3;
+                    *reinterpret_cast<v4u*>(&sh.zh[0][blk][c][part * 8]) = gh[e];
+                    *reinterpret_cast<v4u*>(&sh.zl[0][blk][c][part * 8]) = gl[e];
                 }
             }
+            if (threadIdx.x < kChunkM) {
+                int j = j0 + threadIdx.x;
+                int jl = j < n ? j : n - 1;
+                if constexpr (DIST == kEuclidean) {
+                    sh.c.x[threadIdx.x] = P.px[jl];
+                    sh.c.y[threadIdx.x] = P.py[jl];
+                } else {
+                    sh.c.hx[threadIdx.x] = P.hx[jl];
+                    sh.c.hy[threadIdx.x] = P.hy[jl];
+                    sh.c.hz[threadIdx.x] = P.hz[jl];
+                    sh.c.sl[threadIdx.x] = (float)P.sl[jl];
+                }
+                if constexpr (MODE == kModeQDiag)
+                    sh.bwc[0][threadIdx.x] = make_bw((float)(bw_pt ? bw_pt[jl] : args.bw));
+            }
+            __syncthreads();
+        }
+        ++swept;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                v8h ah, al, a2h, a2l;
-                const int rb = 32 * s + 8 * kq;
-                float d2[8], w[8];
-                float smax = 0.0f;   // great circle: any pair beyond 60 degrees of arc (S > 0.25)?
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int rr = rb + e;
-                    if constexpr (DIST == kEuclidean) {
-                        double dx = sh.c.x[rr] - fc[r].a, dy = sh.c.y[rr] - fc[r].b;
-                        d2[e] = (float)(dx * dx + dy * dy);
-                    } else {
-                        double ex = sh.c.hx[rr] - fc[r].a, ey = sh.c.hy[rr] - fc[r].b, ez = sh.c.hz[rr] - fc[r].c;
-                        const float slj = sh.c.sl[rr];
-                        const float S = (float)(ex * ex + ey * ey + ez * ez);
-                        d2[e] = gc_d2_near_f32(S, fc[r].sl + slj, slj - fc[r].sl);
-                        smax = fmaxf(smax, S);
+            for (int r = 0; r < R; ++r)
+    #pragma unroll
+                for (int t = 0; t < NTA + NTB; ++t) {
+                    hh[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
+                    xx[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
+                    ll[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
+                }
+            const int nsteps = (jend - j0) > 32 ? 2 : 1;
+            for (int s = 0; s < nsteps; ++s) {
+                // B operands: column col of each tile, rows 8*kq .. 8*kq+7 of block s
+                v8h bh[NTA], bl[NTA];
+    #pragma unroll
+                for (int t = 0; t < NTA; ++t) {
+                    if (t < nta) {
+                        bh[t] = *reinterpret_cast<const v8h*>(&sh.zh[zb][s][16 * t + col][8 * kq]);
+                        bl[t] = *reinterpret_cast<const v8h*>(&sh.zl[zb][s][16 * t + col][8 * kq]);
                     }
                 }
-                if constexpr (DIST != kEuclidean) {
-                    if (__any(smax > 0.25f)) {
-#pragma unroll
+    #pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    v8h ah, al, a2h, a2l;
+                    const int rb = 32 * s + 8 * kq;
+                    float d2[8], w[8];
+                    if constexpr (DIST == kEuclidean) {
+    #pragma unroll
                         for (int e = 0; e < 8; ++e) {
                             const int rr = rb + e;
-                            double ex = sh.c.hx[rr] - fc[r].a, ey = sh.c.hy[rr] - fc[r].b,
-                                   ez = sh.c.hz[rr] - fc[r].c;
+                            double dx = sh.c.x[rr] - fc[r].a, dy = sh.c.y[rr] - fc[r].b;
+                            d2[e] = (float)(dx * dx + dy * dy);
+                        }
+                    } else if constexpr (DIST == kGreatCircle) {
+                        // the arithmetic of gc_d2_8, written out: the shared form
+                        // costs this kernel 14 VGPRs and its second wave per SIMD
+                        float smax = 0.0f;   // any pair beyond 60 degrees of arc (S > 0.25)?
+    #pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            const int rr = rb + e;
+                            double ex = sh.c.hx[rr] - fc[r].a, ey = sh.c.hy[rr] - fc[r].b, ez = sh.c.hz[rr] - fc[r].c;
+                            const float slj = sh.c.sl[rr];
                             const float S = (float)(ex * ex + ey * ey + ez * ez);
-                            if (S > 0.25f) {
-                                const float d = gcdist_tail_f32(S, fc[r].sl, sh.c.sl[rr]);
-                                d2[e] = d * d;
+                            d2[e] = gc_d2_near_f32(S, fc[r].sl + slj, slj - fc[r].sl);
+                            smax = fmaxf(smax, S);
+                        }
+                        if (__any(smax > 0.25f)) {
+    #pragma unroll
+                            for (int e = 0; e < 8; ++e) {
+                                const int rr = rb + e;
+                                double ex = sh.c.hx[rr] - fc[r].a, ey = sh.c.hy[rr] - fc[r].b,
+                                       ez = sh.c.hz[rr] - fc[r].c;
+                                const float S = (float)(ex * ex + ey * ey + ez * ez);
+                                if (S > 0.25f) {
+                                    const float d = gcdist_tail_f32(S, fc[r].sl, sh.c.sl[rr]);
+                                    d2[e] = d * d;
+                                }
+                            }
+                        }
+                    } else {
+    #pragma unroll
+                        for (int e = 0; e < 8; ++e) d2[e] = tq[r][s][e >> 2][e & 3];
+                    }
+                    // qDiag with a fixed bandwidth: fbw holds the same constants as sh.bwc
+                    if (MODE == kModeQDiag && bw_pt) {
+    #pragma unroll
+                        for (int e = 0; e < 8; ++e) w[e] = weight_from_d2(args.kernel, d2[e], sh.bwc[zb][rb + e]);
+                    } else {
+                        weights8_from_d2<kScaledSplit>(args.kernel, d2, fbw[r], w);
+                    }
+                    // Rows past the segment end occur only in the last segment,
+                    // at j >= n, where every Z table (design, G, single column)
+                    // and its FP16 planes are zero padded: their products are
+                    // exact zeros without masking the (finite) weights.
+                    if constexpr (kScaledSplit) {
+    #pragma unroll
+                        for (int e = 0; e < 8; e += 2) {
+                            float w0 = w[e], w1 = w[e + 1];
+                            if (MODE == kModeCV) {
+                                const int self = fidx[r] - (j0 + rb);
+                                if (self == e) w0 = 0.0f;
+                                if (self == e + 1) w1 = 0.0f;
+                            }
+                            h2v h, l;
+                            split2_scaled(w0, w1, h, l);
+                            ah[e] = h.x;
+                            ah[e + 1] = h.y;
+                            al[e] = l.x;
+                            al[e + 1] = l.y;
+                        }
+                    } else {
+    #pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            float we = w[e];
+                            const float wa = MODE == kModeQDiag ? square_rounded(we) : we;
+                            _Float16 h, l;
+                            split16(wa, h, l);
+                            ah[e] = h;
+                            al[e] = l;
+                            if (NTB > 0) {
+                                split16(square_rounded(we), h, l);
+                                a2h[e] = h;
+                                a2l[e] = l;
                             }
                         }
                     }
-                }
-                // qDiag with a fixed bandwidth: fbw holds the same constants as sh.bwc
-                if (MODE == kModeQDiag && bw_pt) {
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) w[e] = weight_from_d2(args.kernel, d2[e], sh.bwc[rb + e]);
-                } else {
-                    weights8_from_d2<kScaledSplit>(args.kernel, d2, fbw[r], w);
-                }
-                // Rows past the segment end occur only in the last segment,
-                // at j >= n, where every Z table (design, G, single column)
-                // and its FP16 planes are zero padded: their products are
-                // exact zeros without masking the (finite) weights.
-                if constexpr (kScaledSplit) {
-#pragma unroll
-                    for (int e = 0; e < 8; e += 2) {
-                        float w0 = w[e], w1 = w[e + 1];
-                        if (MODE == kModeCV) {
-                            const int self = fidx[r] - (j0 + rb);
-                            if (self == e) w0 = 0.0f;
-                            if (self == e + 1) w1 = 0.0f;
-                        }
-                        h2v h, l;
-                        split2_scaled(w0, w1, h, l);
-                        ah[e] = h.x;
-                        ah[e + 1] = h.y;
-                        al[e] = l.x;
-                        al[e + 1] = l.y;
-                    }
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        float we = w[e];
-                        const float wa = MODE == kModeQDiag ? square_rounded(we) : we;
-                        _Float16 h, l;
-                        split16(wa, h, l);
-                        ah[e] = h;
-                        al[e] = l;
-                        if (NTB > 0) {
-                            split16(square_rounded(we), h, l);
-                            a2h[e] = h;
-                            a2l[e] = l;
+    #pragma unroll
+                    for (int t = 0; t < NTA; ++t) {
+                        if (t >= nta) continue;
+                        hh[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[t], hh[r][t], 0, 0, 0);
+                        xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[t], xx[r][t], 0, 0, 0);
+                        if constexpr (kScaledSplit) {
+                            // Wl zh at scale 1 in its own accumulator, Wl zl with the 2^-11 terms
+                            ll[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], ll[r][t], 0, 0, 0);
+                            xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bl[t], xx[r][t], 0, 0, 0);
+                        } else {
+                            xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], xx[r][t], 0, 0, 0);
+                            ll[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bl[t], ll[r][t], 0, 0, 0);
                         }
                     }
-                }
-#pragma unroll
-                for (int t = 0; t < NTA; ++t) {
-                    if (t >= nta) continue;
-                    hh[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[t], hh[r][t], 0, 0, 0);
-                    xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[t], xx[r][t], 0, 0, 0);
-                    if constexpr (kScaledSplit) {
-                        // Wl zh at scale 1 in its own accumulator, Wl zl with the 2^-11 terms
-                        ll[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], ll[r][t], 0, 0, 0);
-                        xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bl[t], xx[r][t], 0, 0, 0);
-                    } else {
-                        xx[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[t], xx[r][t], 0, 0, 0);
-                        ll[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bl[t], ll[r][t], 0, 0, 0);
+    #pragma unroll
+                    for (int t = 0; t < NTB; ++t) {
+                        if (t >= ntb) continue;
+                        hh[r][NTA + t] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bh[t], hh[r][NTA + t], 0, 0, 0);
+                        xx[r][NTA + t] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bl[t], xx[r][NTA + t], 0, 0, 0);
+                        xx[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bh[t], xx[r][NTA + t], 0, 0, 0);
+                        ll[r][NTA + t] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bl[t], ll[r][NTA + t], 0, 0, 0);
                     }
-                }
-#pragma unroll
-                for (int t = 0; t < NTB; ++t) {
-                    if (t >= ntb) continue;
-                    hh[r][NTA + t] =
-                        __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bh[t], hh[r][NTA + t], 0, 0, 0);
-                    xx[r][NTA + t] =
-                        __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bl[t], xx[r][NTA + t], 0, 0, 0);
-                    xx[r][NTA + t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bh[t], xx[r][NTA + t], 0, 0, 0);
-                    ll[r][NTA + t] =
-                        __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bl[t], ll[r][NTA + t], 0, 0, 0);
                 }
             }
+            // flush the 64-point partial sums into FP64
+    #pragma unroll
+            for (int r = 0; r < R; ++r)
+    #pragma unroll
+                for (int t = 0; t < NTA + NTB; ++t)
+    #pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        // hh + ll in FP32: one rounding per 64-point block, next to the
+                        // 64 of the block's accumulation
+                        if constexpr (kScaledSplit)
+                            acc[r][t][q] += ((double)(hh[r][t][q] + ll[r][t][q]) +
+                                             (double)xx[r][t][q] * (1.0 / 2048.0)) * (1.0 / 2048.0);
+                        else
+                            acc[r][t][q] += (double)hh[r][t][q] + (double)xx[r][t][q] * (1.0 / 2048.0) +
+                                            (double)ll[r][t][q] * (1.0 / 4194304.0);
+                    }
+        if constexpr (kTab) {
+            if (more) store_next(zb ^ 1);
+            __syncthreads();
+            zb ^= 1;
         }
-        // flush the 64-point partial sums into FP64
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-#pragma unroll
-            for (int t = 0; t < NTA + NTB; ++t)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    // hh + ll in FP32: one rounding per 64-point block, next to the
-                    // 64 of the block's accumulation
-                    if constexpr (kScaledSplit)
-                        acc[r][t][q] += ((double)(hh[r][t][q] + ll[r][t][q]) +
-                                         (double)xx[r][t][q] * (1.0 / 2048.0)) * (1.0 / 2048.0);
-                    else
-                        acc[r][t][q] += (double)hh[r][t][q] + (double)xx[r][t][q] * (1.0 / 2048.0) +
-                                        (double)ll[r][t][q] * (1.0 / 4194304.0);
-                }
     }
 
     if (args.pairs && threadIdx.x == 0 && swept)
@@ -547,6 +734,10 @@ hipError_t launch_mixed_rr(const SweepLaunch& a, hipStream_t s) {
 // measured faster there despite one wave per SIMD (great circle, 3 + 3
 // tiles: 5.0 against 5.6 ms).  Both forms give the same bits.
 constexpr int mixed_default_r(int nta, int ntb, int mode, int dist) {
+    // table sweeps hold a chunk of distances per focus tile and prefetch the
+    // next: one tile per wave keeps 3 waves per SIMD (127 VGPRs at 3 column
+    // tiles, AICc), where two would leave one
+    if (dist == kGcTable) return 1;
     if (mode == kModeHat) return nta + ntb <= 6 ? 2 : 1;
     return (nta + ntb <= 3 && !(dist == kGreatCircle && (mode == kModeCV || mode == kModeQDiag) && nta >= 3)) ? 2
                                                                                                               : 1;
@@ -631,6 +822,13 @@ int contract_r(int nta, int ntb, int precision_mixed, int mode, int dist) {
 }
 
 hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s) {
+    if (dist == kGcTable) {
+        if (!a.job0.d2tab || a.items) return hipErrorInvalidValue;
+        if (mode == kModeCV) return dispatch_mixed<kGcTable, kModeCV>(nta, ntb, a, s);
+        if (mode == kModePlain) return dispatch_mixed<kGcTable, kModePlain>(nta, ntb, a, s);
+        if (mode == kModeHat) return dispatch_mixed<kGcTable, kModeHat>(nta, ntb, a, s);
+        return dispatch_mixed<kGcTable, kModeQDiag>(nta, ntb, a, s);
+    }
     if (dist == kEuclidean) {
         if (mode == kModeCV) return dispatch_mixed<kEuclidean, kModeCV>(nta, ntb, a, s);
         if (mode == kModePlain) return dispatch_mixed<kEuclidean, kModePlain>(nta, ntb, a, s);
@@ -646,6 +844,44 @@ hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const Swe
 
 namespace {
 
+// Distance table (layout: launch_d2_table).  A workgroup stages the
+// coordinates of kTabPts consecutive data points in LDS; each of its waves
+// takes one focus group of 16 points and writes that group's kTabPts / 32
+// blocks, 2 KB each, contiguous in the table.
+constexpr int kTabPts = 256;
+__global__ __launch_bounds__(256) void d2_table_kernel(Points P0, int n, int nfg, int ndb, float* tab) {
+    __shared__ double shx[kTabPts], shy[kTabPts], shz[kTabPts];
+    __shared__ float ssl[kTabPts];
+    const GPoints P = global_points(P0);
+    const int nsb = (ndb * 32 + kTabPts - 1) / kTabPts;   // data super-blocks
+    const int sb = (int)(blockIdx.x % (unsigned)nsb);
+    const int fq = (int)(blockIdx.x / (unsigned)nsb);
+    {
+        const int j = min(sb * kTabPts + (int)threadIdx.x, n - 1);
+        shx[threadIdx.x] = P.hx[j];
+        shy[threadIdx.x] = P.hy[j];
+        shz[threadIdx.x] = P.hz[j];
+        ssl[threadIdx.x] = (float)P.sl[j];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int fg = fq * 4 + (threadIdx.x >> 6);
+    if (fg >= nfg) return;
+    const int col = lane & 15, kq = lane >> 4;
+    const MFocus<kGreatCircle> fc = load_mfocus<kGreatCircle>(P, min(fg * 16 + col, n - 1));
+    for (int b = 0; b < kTabPts / 32; ++b) {
+        const int db = sb * (kTabPts / 32) + b;
+        if (db >= ndb) break;
+        const int rb = b * 32 + 8 * kq;
+        float d2[8];
+        gc_d2_8(fc.a, fc.b, fc.c, fc.sl,
+                [&](int e) { return GcRow{shx[rb + e], shy[rb + e], shz[rb + e], ssl[rb + e]}; }, d2);
+        GWM_GLOBAL float* out = gptr(tab) + ((size_t)fg * ndb + db) * 512 + 4 * lane;
+        *reinterpret_cast<GWM_GLOBAL v4f*>(out) = v4f{d2[0], d2[1], d2[2], d2[3]};
+        *reinterpret_cast<GWM_GLOBAL v4f*>(out + 256) = v4f{d2[4], d2[5], d2[6], d2[7]};
+    }
+}
+
 __global__ void var_table_kernel(const double* __restrict__ x, const double* __restrict__ y, int n, int rows_pad,
                                  double* __restrict__ Z) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;   // one entry of the rows_pad x 16 table
@@ -657,6 +893,15 @@ __global__ void var_table_kernel(const double* __restrict__ x, const double* __r
 }
 
 }  // namespace
+
+hipError_t launch_d2_table(const Points& P, int n, int n_pad, int rows, float* tab, hipStream_t s) {
+    if (n <= 0 || n_pad % 16 || rows % 64 || n_pad < n || rows < n) return hipErrorInvalidValue;
+    const int nfg = n_pad / 16, ndb = rows / 32;
+    const long long grid = (long long)((nfg + 3) / 4) * ((rows + kTabPts - 1) / kTabPts);
+    if (grid * 256 > 0xffffffffLL) return hipErrorInvalidValue;   // work items of one launch
+    hipLaunchKernelGGL(d2_table_kernel, dim3((unsigned)grid), dim3(256), 0, s, P, n, nfg, ndb, tab);
+    return hipGetLastError();
+}
 
 hipError_t launch_var_table(const double* x, const double* y, int n, int rows_pad, double* Z, hipStream_t s) {
     const int total = rows_pad * 16;

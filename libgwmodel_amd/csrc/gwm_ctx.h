@@ -131,6 +131,16 @@ struct gwm_hip_ctx {
     double mgwr_shat[2] = {0.0, 0.0};
     std::map<std::string, gwm::detail::KernelStat> stats;
     long long solves = 0;
+    // Squared great-circle distances of every pair (launch_d2_table), read by
+    // the mixed sweeps of one search / fit / backfit call instead of forming
+    // the distances again in every sweep.  Built at the first such sweep of a
+    // call (d2_scope > 0) when it fits d2_cap and the free device memory, and
+    // released when the outermost call returns (gwm_hip_set_distance_table).
+    float* d2tab = nullptr;
+    size_t d2tab_bytes = 0;
+    bool d2_valid = false;
+    int d2_scope = 0;
+    size_t d2_cap = size_t(96) << 30;
 
     gwm::Points points() const {
         gwm::Points P;
