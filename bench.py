@@ -497,7 +497,7 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=512, help="uneven batch size (0: skip)")
     ap.add_argument("--no-loop", action="store_true", help="skip the single-problem loop of the batch")
     ap.add_argument("--serve-requests", type=int, default=500, help="serving stream length (0: skip)")
-    ap.add_argument("--serve-rates", type=str, default="10,40",
+    ap.add_argument("--serve-rates", type=str, default="3,6",
                     help="serving-stream arrival rates, requests/s (comma-separated; one replay each)")
     ap.add_argument("--serve-check", type=int, default=12,
                     help="requests spread over the stream's sizes re-run in FP64 for the beta error")
@@ -511,6 +511,9 @@ def main():
                     help="profiling mode: run this many mixed AICc sweeps (and as many FP64 ones) and exit")
     ap.add_argument("--sweep-bw", type=float, default=61.890021059082855,
                     help="bandwidth (km) of the profiling sweeps (default: the selected one)")
+    ap.add_argument("--profile-step", action="store_true",
+                    help="profiling mode: one mixed step (search + fit) after a warm-up sweep, its kernel "
+                         "launches, times and pairs as JSON, then exit")
     ap.add_argument("--skip-headline", action="store_true",
                     help="profiling mode: run only the multiscale / batch / serve legs and print their JSON")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -553,6 +556,25 @@ def main():
     ctx = capi.Context(0)
     ctx.set_data(d["coords"], d["X"], d["y"], capi.GREAT_CIRCLE)
 
+    if args.profile_step:
+        ctx.criterion(capi.GAUSSIAN, False, 300.0, capi.CRIT_AIC, capi.MIXED)
+        ctx.reset_timers()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        r = run_steps(ctx, capi, capi.MIXED, 1, lo, hi)
+        dt = time.perf_counter() - t0
+        ctx.set_timing(False)
+        rep = {"step_ms": dt * 1e3, "bandwidth_km": r["bandwidth"], "n": n}
+        for name in ("d2_table", "contract_mixed_plain", "contract_fp64_plain", "contract_mixed_hat",
+                     "contract_mixed_qdiag", "solve"):
+            launches, ms = ctx.kernel_time(name)
+            rep[name] = {"launches": launches, "ms": ms}
+            if name.startswith("contract"):
+                rep[name]["pairs"] = ctx.kernel_pairs(name)
+        print(json.dumps(rep))
+        ctx.close()
+        return
+
     if args.sweeps_only:
         ctx.reset_timers()
         ctx.set_timing(True)
@@ -584,6 +606,7 @@ def main():
     t_mixed = time.perf_counter() - t0
     ctx.set_timing(False)
     solves_mixed = ctx.solve_count()
+    tab_l, tab_ms = ctx.kernel_time("d2_table")
     launches, ms = ctx.kernel_time("contract_mixed_plain")
     swept_pairs = ctx.kernel_pairs("contract_mixed_plain")
     l64_in_mixed, ms64_in_mixed = ctx.kernel_time("contract_fp64_plain")
@@ -653,6 +676,10 @@ def main():
     serve = leg("serve", run_serve, capi, synth, args.serve_requests, rates, args.serve_check) \
         if args.serve_requests else None
     cs, el = col_rel(mixed["betas"], fp64["betas"])
+    big = np.abs(fp64["betas"]) >= 0.1 * np.max(np.abs(fp64["betas"]), axis=0)
+    el_big = float(np.max(np.abs(mixed["betas"] - fp64["betas"])[big] / np.abs(fp64["betas"])[big]))
+    rel_all = np.abs(mixed["betas"] - fp64["betas"]) / np.maximum(np.abs(fp64["betas"]), 1e-300)
+    el_p99 = float(np.percentile(rel_all, 99))
 
     line = {
         "metric": METRIC,
@@ -684,6 +711,8 @@ def main():
         "beta_rel_err_vs_fp64": cs,
         "beta_elementwise_rel_err_vs_fp64": el,
         "beta_elementwise_floor": "entries with |beta| >= 1e-3 max|beta| of their column",
+        "beta_elementwise_rel_err_vs_fp64_floor_0.1": el_big,
+        "beta_elementwise_rel_err_vs_fp64_p99": el_p99,
         "se_rel_err_vs_fp64": beta_rel_err(mixed["betasSE"], fp64["betasSE"]),
         "aicc_rel_err_vs_fp64": abs(mixed["diagnostic"]["AICc"] - fp64["diagnostic"]["AICc"]) /
         abs(fp64["diagnostic"]["AICc"]),
@@ -700,6 +729,11 @@ def main():
                      "valu_insts_per_pair": valu["valu_insts_per_pair"] if valu else None,
                      "valu_issue_bound_ms": valu_bound_ms,
                      "valu_source": valu.get("source") if valu else None,
+                     # the sweeps read each executed pair's FP32 squared distance
+                     # from the step's distance table (4 B per pair)
+                     "table_stream_tb_per_s": 4.0 * swept_per_launch / (avg_ms * 1e-3) / 1e12 if tab_l else None,
+                     "table_stream_frac_of_hbm": 4.0 * swept_per_launch / (avg_ms * 1e-3) / HBM_PEAK if tab_l else None,
+                     "table_build_ms_per_step": tab_ms / args.steps if tab_l else None,
                      "fp64_kernel_avg_launch_ms": avg_ms64,
                      "fp64_frac_of_fp64_peak": flops_per_launch / (avg_ms64 * 1e-3) / PEAK["fp64"] if l64 else None,
                      "fp64_executed_pair_fraction": swept64 / max(l64, 1) / (float(n) * n)},
