@@ -326,6 +326,13 @@ int gwm_hip_server_set_refine_condition(gwm_hip_server* server, double criterion
 /* Round budget in ms of device time; <= 0 runs every posted evaluation whole
  * in each round. */
 int gwm_hip_server_set_round_ms(gwm_hip_server* server, double round_ms);
+/* Bytes of one request's partial sums (default 256 MiB, at least 1): a round
+ * computes at most as many of a request's focus points as their partial sums
+ * fit (whole chunks of 128, at least one chunk), and a request holds only that
+ * slice, so its device footprint stays O(n k^2) with a bounded partial-sum
+ * term.  Applies to requests submitted afterwards; results do not depend on
+ * it. */
+int gwm_hip_server_set_partial_budget(gwm_hip_server* server, double bytes);
 int gwm_hip_server_submit(gwm_hip_server* server, int n, int k, const double* coords, const double* X,
                           const double* y, int distance, int kernel, int adaptive, double bw, int select_bw,
                           int criterion, int has_lower, double lower, int has_upper, double upper, double* betas,

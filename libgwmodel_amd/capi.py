@@ -79,6 +79,7 @@ SIGNATURES = {
     "gwm_hip_server_set_certify_tolerance": (_i, [_vp, _d]),
     "gwm_hip_server_set_refine_condition": (_i, [_vp, _d, _d]),
     "gwm_hip_server_set_round_ms": (_i, [_vp, _d]),
+    "gwm_hip_server_set_partial_budget": (_i, [_vp, _d]),
     "gwm_hip_server_submit": (_i, [_vp, _i, _i, _dp, _dp, _dp, _i, _i, _i, _d, _i, _i, _i, _d, _i, _d, _dp, _dp, _dp,
                                    _dp, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "gwm_hip_server_wait": (_i, [_vp, ctypes.c_longlong, _ip, _dp, ctypes.POINTER(ctypes.c_longlong)]),
@@ -443,6 +444,12 @@ class Server:
 
     def set_refine_condition(self, criterion_limit, fit_limit):
         rc = self._lib.gwm_hip_server_set_refine_condition(self.h, float(criterion_limit), float(fit_limit))
+        if rc != GWM_OK:
+            raise GwmError(rc)
+
+    def set_partial_budget(self, nbytes):
+        """Partial-sum bytes of one request's round slice (later submissions)."""
+        rc = self._lib.gwm_hip_server_set_partial_budget(self.h, float(nbytes))
         if rc != GWM_OK:
             raise GwmError(rc)
 

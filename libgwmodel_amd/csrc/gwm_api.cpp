@@ -419,9 +419,50 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
     const bool mixed = precision == GWM_MIXED;
     int nseg, seg_len;
     data_segments(c->n, nseg, seg_len);
+    int rc;
+    // FP64 single-column sweeps (multiscale steps): the two used columns of
+    // the single-column table on the VALU, partial sums [nseg][2][n_pad]
+    if (!mixed && nta == 1 && ntb == 0 && zp.z64 == c->vZ && (mode == kModeCV || mode == kModePlain)) {
+        if ((rc = ensure_part(c, (size_t)nseg * 2 * c->n_pad * sizeof(double)))) return rc;
+        SweepJob j;
+        j.P = c->points();
+        j.Z = c->vZ;
+        j.Zh = nullptr;
+        j.Zl = nullptr;
+        j.colscale = nullptr;
+        j.bw_pt = bw_pt;
+        j.bw = bw;
+        j.kernel = kernel;
+        j.n = c->n;
+        j.n_pad = c->n_pad;
+        j.seg_len = seg_len;
+        j.nseg = nseg;
+        j.nta = 1;
+        j.ntb = 0;
+        j.out = c->part;
+        j.dnn = dnn;
+        if (c->prune) j.boxes = c->boxes;
+        if (c->timing) {
+            if (!c->pairs_dev) GWM_HIP_CHECK(c, hipMalloc((void**)&c->pairs_dev, sizeof(unsigned long long)));
+            GWM_HIP_CHECK(c, hipMemsetAsync(c->pairs_dev, 0, sizeof(unsigned long long), c->stream));
+            j.pairs = c->pairs_dev;
+        }
+        timer_begin(c);
+        GWM_HIP_CHECK(c, launch_contract_var_f64(c->dist, mode, j, c->stream));
+        if (c->timing) {
+            std::string name = std::string("contract_fp64_") + mode_name(mode);
+            if ((rc = timer_end(c, name.c_str()))) return rc;
+            unsigned long long pairs = 0;
+            GWM_HIP_CHECK(c, hipMemcpyAsync(&pairs, c->pairs_dev, sizeof(pairs), hipMemcpyDeviceToHost, c->stream));
+            GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+            c->stats[name].pairs += (double)pairs;
+        }
+        *PT_out = 2;
+        *nseg_out = nseg;
+        return GWM_OK;
+    }
     const int PT = (nta + ntb) * 16;
-    int rc = ensure_part(c, (size_t)nseg * PT * c->n_pad * sizeof(double));
-    if (rc) return rc;
+    if ((rc = ensure_part(c, (size_t)nseg * PT * c->n_pad * sizeof(double)))) return rc;
     SweepLaunch L;
     SweepJob& j = L.job0;
     j.P = c->points();
@@ -1210,6 +1251,9 @@ int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int
     const int smode = beta ? kSolveBeta : (cv ? kSolveCV : kSolveAIC);
     // a mixed CV evaluation with a point far from every other (its largest
     // weight below kMixedMinWeight: fail bit 2) is taken again in FP64
+    // one synchronisation per evaluation: the reduction is queued behind the
+    // solve and read back with the failure flags
+    double red[3] = {0.0, 0.0, 0.0};
     for (int pass = 0; pass < 2; ++pass) {
         GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
         if ((rc = sweep(c, cv ? kModeCV : kModePlain, 1, 0, var_planes(c), bw_pt, bw, kernel, precision, &PT, &nseg,
@@ -1218,19 +1262,20 @@ int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int
         GWM_HIP_CHECK(c, launch_solve_var(smode, c->part, nseg, PT, 0, n, c->n_pad, c->vx, c->vy, bw_pt, bw, kernel,
                                           c->res2, c->sii, beta ? c->betas : nullptr, c->fail, c->stream,
                                           precision == GWM_MIXED ? dnn : nullptr));
-        if (precision != GWM_MIXED || !cv) break;
-        GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-        if (!(fail & 2)) break;
+        if (value) {
+            GWM_HIP_CHECK(c, launch_reduce(c->res2, cv ? nullptr : c->sii, nullptr, nullptr, n, c->red, c->stream));
+            GWM_HIP_CHECK(c, hipMemcpyAsync(red, c->red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
+            GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        } else if (precision == GWM_MIXED && cv) {
+            GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        }
+        if (precision != GWM_MIXED || !cv || !(fail & 2)) break;
         precision = GWM_FP64;
         c->solves += n;
     }
     if (value) {
-        GWM_HIP_CHECK(c, launch_reduce(c->res2, cv ? nullptr : c->sii, nullptr, nullptr, n, c->red, c->stream));
-        double red[3];
-        GWM_HIP_CHECK(c, hipMemcpyAsync(red, c->red, sizeof(red), hipMemcpyDeviceToHost, c->stream));
-        GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
         c->solves += n;
         if (fail & 1) return GWM_ERR_NOT_SPD;
         double v = cv ? red[0] : aicc(red[0], (double)n, red[1]);

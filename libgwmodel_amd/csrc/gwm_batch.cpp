@@ -96,6 +96,9 @@ struct Request {
     double* part = nullptr;
     int* dfail = nullptr;        // non-SPD flag of the evaluation in progress
     int nseg = 1, seg_len = 64;
+    // partial sums hold one round's slice: at most s_cap focus points (or
+    // list entries), a multiple of kChunk
+    int s_cap = 128;
     size_t footprint = 0;
     double maxdist = 0.0;
     // timing and status
@@ -118,8 +121,8 @@ struct Request {
 };
 
 
-size_t footprint_bytes(int n, int k, int dist, int nseg) {
-    const size_t rows = (size_t)(n + 63) / 64 * 64, np = (size_t)(n + 127) / 128 * 128;
+size_t footprint_bytes(int n, int k, int dist, int nseg, int s_cap) {
+    const size_t rows = (size_t)(n + 63) / 64 * 64, np = (size_t)s_cap;
     const size_t nta = nta_of(k), ntb = ntb_of(k);
     size_t b = 0;
     b += (size_t)n * (dist == GWM_GREAT_CIRCLE ? 6 : 2) * 8;        // coordinates
@@ -127,8 +130,21 @@ size_t footprint_bytes(int n, int k, int dist, int nseg) {
     b += rows * nta * 16 * (8 + 4) + rows * ntb * 16 * (8 + 4);        // Z, G and FP16 planes
     b += rows * 16 * (8 + 4) + (size_t)n * 16;                          // single-column tables
     b += (size_t)n * 8 * 8 + (size_t)n * k * 3 * 8;                     // per-focus vectors
-    b += (size_t)nseg * (nta + ntb) * 16 * np * 8;                      // partial sums
+    b += (size_t)nseg * (nta + ntb) * 16 * np * 8;                      // partial sums of a slice
     return b;
+}
+
+// Focus points a round may compute for one request: all of them (padded to
+// 128) while their partial sums take at most `budget` bytes, else as many as
+// fit it (whole chunks of 128, at least one), so that a request's footprint
+// stays O(n k^2) with a bounded partial-sum term (default 256 MiB: at
+// n = 200,000, k = 16, 4,352 points of 25 segments x 304 columns, 265 MB
+// instead of 12.2 GB).
+int slice_cap(int n, int k, int nseg, size_t budget) {
+    const size_t per = (size_t)nseg * (nta_of(k) + ntb_of(k)) * 16 * sizeof(double);
+    const size_t all = (size_t)(n + 127) / 128 * 128;
+    const size_t fit = std::max<size_t>(1, budget / per / 128) * 128;
+    return (int)std::min(all, fit);
 }
 
 // Device arena for the per-round job tables.
@@ -166,6 +182,7 @@ struct gwm_hip_server {
     unsigned int* sel_nleft = nullptr;
     int sel_cap = 0;
     double round_ms = 25.0;      // work budget of a round; <= 0: whole evaluations
+    size_t part_budget = size_t(256) << 20;   // partial-sum bytes of one request's round slice
     double refine_crit = 10.0, refine_fit = 2.0;   // FP64 recomputation limits (mixed)
 };
 
@@ -212,7 +229,9 @@ struct QdGroup {
     size_t off_jobs = 0, off_start = 0;
 };
 
-SweepJob sweep_job(Request* r, int mode, int nta, int ntb, const double* bw_pt, double bw) {
+// f_off: the first focus point (or list entry) of the round's slice, the
+// first column of the request's partial-sum buffer
+SweepJob sweep_job(Request* r, int mode, int nta, int ntb, const double* bw_pt, double bw, int f_off) {
     gwm_hip_ctx* c = r->ctx;
     SweepJob j;
     j.P = c->points();
@@ -231,7 +250,8 @@ SweepJob sweep_job(Request* r, int mode, int nta, int ntb, const double* bw_pt, 
     j.bw = bw;
     j.kernel = r->kernel;
     j.n = c->n;
-    j.n_pad = c->n_pad;
+    j.n_pad = r->s_cap;
+    j.out_off = f_off;
     j.seg_len = r->seg_len;
     j.nseg = r->nseg;
     j.nta = nta;
@@ -337,10 +357,10 @@ Slice plan_slice(Request* r, double units, bool unlimited) {
     const bool fit = r->ev.type == kEvalFit;
     auto count = [&](int phase, int total) {
         const int left = total - r->next;
-        if (unlimited) return left;
+        if (unlimited) return std::min(left, r->s_cap);
         const long long want = (long long)std::ceil(std::max(units, 0.0) / unit_cost(r, phase));
         const long long chunks = std::max<long long>(1, (want + kChunk - 1) / kChunk);
-        return (int)std::min<long long>(left, chunks * kChunk);
+        return (int)std::min<long long>(std::min<long long>(left, chunks * kChunk), r->s_cap);
     };
     auto take = [&](int phase, int total, int& lo, int& hi) {
         const int c = count(phase, total);
@@ -396,7 +416,8 @@ SolveArgs solve_job(Request* r, int PT, int colB, const double* bw_pt, int f0) {
     sa.PT = PT;
     sa.colB = colB;
     sa.n = c->n;
-    sa.n_pad = c->n_pad;
+    sa.n_pad = r->s_cap;
+    sa.part_off = f0;
     sa.X = c->X;
     sa.y = c->y;
     sa.bw_pt = bw_pt;
@@ -479,7 +500,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
                 sel_start[d].push_back(sel_start[d].back() + sl.a1 - sl.a0);
             }
             SweepGroup& g = group_of(sweeps, r->dist, mode, prec, nta, tb);
-            g.jobs.push_back(sweep_job(r, mode, nta, tb, bw_pt, ev.bw));
+            g.jobs.push_back(sweep_job(r, mode, nta, tb, bw_pt, ev.bw, sl.a0));
             add_items(g, (int)g.jobs.size() - 1, sl.a0, sl.a1, r->nseg,
                       contract_r(nta, tb, prec == GWM_MIXED, mode, r->dist));
             SolveGroup& sg = sgroup_of(solves, k, smode);
@@ -493,19 +514,16 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         if (sl.e1 > sl.e0) {
             // FP64 recomputation of listed focus points: FP64 kernel, the
             // request's own segments, compact partial-sum columns in part
-            const int npad = (r->nrefine + 127) / 128 * 128;
             SweepGroup& g = group_of(sweeps, r->dist, mode, GWM_FP64, nta, tb);
-            SweepJob j = sweep_job(r, mode, nta, tb, bw_pt, ev.bw);
+            SweepJob j = sweep_job(r, mode, nta, tb, bw_pt, ev.bw, sl.e0);
             j.Z = c->Z;
             j.focus_idx = c->rlist;
             j.nfocus = r->nrefine;
-            j.n_pad = npad;
             g.jobs.push_back(j);
             add_items(g, (int)g.jobs.size() - 1, sl.e0, sl.e1, r->nseg, contract_r(nta, tb, 0, mode, r->dist));
             SolveGroup& sg = sgroup_of(solves, k, smode);
             if (sg.start.empty()) sg.start.push_back(0);
             SolveArgs sa = solve_job(r, PT, colB, bw_pt, sl.e0);
-            sa.n_pad = npad;
             sa.focus_idx = c->rlist;
             sa.nfocus = r->nrefine;
             sg.jobs.push_back(sa);
@@ -517,13 +535,14 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         if (sl.split && r->adaptive && c->prune) qstarts.push_back(r);
         if (sl.q1 > sl.q0) {
             SweepGroup& q = group_of(qsweeps, r->dist, kModeQDiag, prec, ntb, 0);
-            q.jobs.push_back(sweep_job(r, kModeQDiag, ntb, 0, bw_pt, ev.bw));
+            q.jobs.push_back(sweep_job(r, kModeQDiag, ntb, 0, bw_pt, ev.bw, sl.q0));
             add_items(q, (int)q.jobs.size() - 1, sl.q0, sl.q1, r->nseg,
                       contract_r(ntb, 0, prec == GWM_MIXED, kModeQDiag, r->dist));
             QdGroup& qg = kgroup_of(qds, k);
             if (qg.start.empty()) qg.start.push_back(0);
-            QDiagJob qj{r->part, r->nseg, ntb * 16, c->n, c->n_pad, c->X, c->sii, c->qd};
+            QDiagJob qj{r->part, r->nseg, ntb * 16, c->n, r->s_cap, c->X, c->sii, c->qd};
             qj.f0 = sl.q0;
+            qj.part_off = sl.q0;
             qg.jobs.push_back(qj);
             qg.start.push_back(qg.start.back() + sl.q1 - sl.q0);
         }
@@ -810,7 +829,7 @@ int setup(gwm_hip_server* S, Request* R) {
     rc = gwm_hip_set_data(R->ctx, R->n, R->k, R->coords.data(), R->X.data(), R->y.data(), R->dist);
     if (rc) return rc;
     R->ctx->certify_rel = S->certify_rel;
-    const size_t bytes = (size_t)R->nseg * (nta_of(R->k) + ntb_of(R->k)) * 16 * R->ctx->n_pad * sizeof(double);
+    const size_t bytes = (size_t)R->nseg * (nta_of(R->k) + ntb_of(R->k)) * 16 * R->s_cap * sizeof(double);
     if (hipMallocAsync((void**)&R->part, bytes + 256, R->ctx->stream) != hipSuccess) return GWM_ERR_OOM;
     R->dfail = reinterpret_cast<int*>(reinterpret_cast<char*>(R->part) + bytes);
     if (hipMemsetAsync(R->dfail, 0, sizeof(int), R->ctx->stream) != hipSuccess) return GWM_ERR_HIP;
@@ -952,6 +971,13 @@ int gwm_hip_server_set_round_ms(gwm_hip_server* S, double ms) {
     return GWM_OK;
 }
 
+int gwm_hip_server_set_partial_budget(gwm_hip_server* S, double bytes) {
+    if (!S || !(bytes >= 1.0)) return GWM_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->part_budget = bytes >= 1.8e19 ? SIZE_MAX : (size_t)bytes;
+    return GWM_OK;
+}
+
 int gwm_hip_server_submit(gwm_hip_server* S, int n, int k, const double* coords, const double* X, const double* y,
                           int distance, int kernel, int adaptive, double bw, int select_bw, int criterion,
                           int has_lower, double lower, int has_upper, double upper, double* betas, double* betasSE,
@@ -983,7 +1009,11 @@ int gwm_hip_server_submit(gwm_hip_server* S, int n, int k, const double* coords,
     R->o_diag = diag;
     R->o_bw = bw_out;
     data_segments(n, R->nseg, R->seg_len);
-    R->footprint = footprint_bytes(n, k, distance, R->nseg);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        R->s_cap = slice_cap(n, k, R->nseg, S->part_budget);
+    }
+    R->footprint = footprint_bytes(n, k, distance, R->nseg, R->s_cap);
     std::lock_guard<std::mutex> lk(S->mu);
     R->refine_crit = S->refine_crit;
     R->refine_fit = S->refine_fit;

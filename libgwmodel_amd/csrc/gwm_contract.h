@@ -57,6 +57,9 @@ struct SweepJob {
     // distance id kGcTable and reads them instead of forming them
     const float* d2tab = nullptr;
     int d2_ndb = 0;          // 32-point data blocks per focus group (rows / 32)
+    // out holds the focus points (or list entries) from out_off on: column t
+    // of the output is focus out_off + t (a server round's slice)
+    int out_off = 0;
 };
 
 // Distance id of a mixed sweep that reads the precomputed squared great-circle
@@ -104,6 +107,10 @@ hipError_t launch_split_fp16(const double* Z, int rows, int ldz, unsigned long l
 // nta/ntb select the compile-time tile counts (those of every job); grid size is
 // nitems (batch) or tiles * nseg (single job).
 hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s);
+// FP64 single-column sweep (nta = 1, ntb = 0, CV or plain mode, every point a
+// focus point): out = [nseg][2][n_pad], the columns x^2 and x y of the
+// single-column table, summed on the VALU (gwm_contract.hip).
+hipError_t launch_contract_var_f64(int dist, int mode, const SweepJob& j, hipStream_t s);
 hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s);
 // the R (focus tiles per wave) each kernel uses for an (nta, ntb, mode,
 // distance) instance; launchers and item tables must agree on it
@@ -185,6 +192,7 @@ struct SolveArgs {
     const int* focus_idx = nullptr;
     int nfocus = 0;
     double* ainv = nullptr;  // per focus: A^-1 row-major k x k (kSolveBeta), or null
+    int part_off = 0;        // part holds focus points (or list entries) from part_off on
     // mixed-precision solves of CV criteria and predictions (with cond): per
     // focus the distance to the nearest other (data) point; a focus whose
     // largest weight, that point's, is below kMixedMinWeight gets cond =
@@ -206,6 +214,7 @@ struct QDiagJob {
     const double* sii;
     double* qdiag;
     int f0 = 0;            // first point of a batched slice
+    int part_off = 0;      // part holds points from part_off on
 };
 hipError_t launch_qdiag_batch(int k, const QDiagJob* jobs, const int* start, int njobs, int total, hipStream_t s);
 

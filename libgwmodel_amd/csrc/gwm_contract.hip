@@ -77,9 +77,9 @@ __device__ __forceinline__ FocusCoords<DIST> load_focus(const GPoints& P, int f)
 // differs from it by an ulp at most).  Great circle: near pairs take
 // gc_d2_near_f64, the others (far, coincident, near-coincident) gcdist_f64,
 // in a branch that costs a wave only when one of its lanes needs it.
-template <int DIST, int NTA, int MODE>
-__device__ __forceinline__ double pair_weight(int kernel, const FocusCoords<DIST>& fc,
-                                              const SharedTile<NTA, DIST, MODE>& sh, int r, const KConst64& k) {
+template <int DIST, class SH>
+__device__ __forceinline__ double pair_weight(int kernel, const FocusCoords<DIST>& fc, const SH& sh, int r,
+                                              const KConst64& k) {
     double d2, d = -1.0;
     if constexpr (DIST == kEuclidean) {
         d2 = d2_euclid(fc.a, fc.b, sh.c.x[r], sh.c.y[r]);
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 // qDiag with a fixed bandwidth: fbw holds the same constants as sh.kc
-                double w = pair_weight<DIST, NTA, MODE>(job.kernel, fc[r], sh, rr,
+                double w = pair_weight<DIST>(job.kernel, fc[r], sh, rr,
                                                         (MODE == kModeQDiag && bw_pt) ? sh.kc[rr] : fbw[r]);
                 if (j >= jend) w = 0.0;
                 if (MODE == kModeCV && j == fidx[r]) w = 0.0;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
     GWM_GLOBAL double* out = gptr(job.out) + (size_t)seg * PT * np;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int f0 = fbase + 16 * r + kk;
+        const int f0 = fbase + 16 * r + kk - job.out_off;
 #pragma unroll
         for (int t = 0; t < NTA; ++t) {
             if (t >= nta) continue;
@@ -363,6 +363,161 @@ int tile_stride(int tiles) {
     return s % tiles;
 }
 
+
+namespace {
+
+// Single-column sweeps of the multiscale path (GWRMultiscale.cpp:420-482,
+// 560-636): per focus point the two sums sum_j w_ij x_j^2 and sum_j w_ij x_j
+// y_j, in FP64 on the VALU.  One lane per focus point (256 per workgroup);
+// the chunk's coordinates and its two table columns are staged in LDS and
+// read by every lane at the same address.  Same weights, pruning and data
+// segments as contract_f64_kernel, whose 16-column matrix-core form would
+// compute eight times the columns these sweeps use.
+template <int DIST>
+struct VarShared {
+    CoordStage<DIST> c;
+    double z0[kChunk], z1[kChunk];
+};
+
+template <int DIST, int MODE, int KER>
+__global__ __launch_bounds__(256) void contract_var_f64_kernel(SweepJob job) {
+    static_assert(MODE == kModeCV || MODE == kModePlain, "single-column sweeps weigh with w");
+    __shared__ VarShared<DIST> sh;
+    const int seg = blockIdx.x % job.nseg;
+    const int tile = blockIdx.x / job.nseg;
+    const GPoints P = global_points(job.P);
+    GWM_GLOBAL const double* bw_pt = gptr(job.bw_pt);
+    GWM_GLOBAL const double* dnn = gptr(job.dnn);
+    const int n = job.n;
+    const int f = tile * 256 + (int)threadIdx.x;
+    const int fl = f < n ? f : n - 1;
+    const FocusCoords<DIST> fc = load_focus<DIST>(P, fl);
+    const double b = bw_pt ? bw_pt[fl] : job.bw;
+    double fdn = 0.0;
+    KConst64 kc;
+    if (dnn) {
+        fdn = dnn[fl];
+        kc = make_kconst64_nn(b, KER, fdn);
+    } else {
+        kc = make_kconst64(b);
+    }
+    // pruning at two levels: the workgroup's box decides which chunks are
+    // staged, each wave's own (64 focus points) which of them it weighs
+    constexpr int BD = DIST == kEuclidean ? 2 : 3;
+    const bool prune = job.boxes != nullptr;
+    double flo[BD], fhi[BD], wlo[BD], whi[BD], thr = INFINITY, wthr = INFINITY;
+    if (prune) {
+        const double x[3] = {DIST == kEuclidean ? fc.a : fc.hx, DIST == kEuclidean ? fc.b : fc.hy, fc.hz};
+        double wb = b, wdn = fdn;
+#pragma unroll
+        for (int d = 0; d < BD; ++d) wlo[d] = whi[d] = x[d];
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+            for (int d = 0; d < BD; ++d) {
+                wlo[d] = fmin(wlo[d], __shfl_xor(wlo[d], m));
+                whi[d] = fmax(whi[d], __shfl_xor(whi[d], m));
+            }
+            wb = fmax(wb, __shfl_xor(wb, m));
+            wdn = fmax(wdn, __shfl_xor(wdn, m));
+        }
+        wthr = prune_threshold<DIST>(zero_weight_distance(KER, wb, false, wdn));
+        double fb = b, gdn = fdn;
+#pragma unroll
+        for (int d = 0; d < BD; ++d) {
+            flo[d] = wlo[d];
+            fhi[d] = whi[d];
+        }
+        workgroup_box<BD>(flo, fhi, wb, &fb, wdn, &gdn);
+        thr = prune_threshold<DIST>(zero_weight_distance(KER, fb, false, gdn));
+    }
+    const int jbeg = seg * job.seg_len;
+    const int jend = min(n, jbeg + job.seg_len);
+    GWM_GLOBAL const double* Z = gptr(job.Z);
+    const int ldz = job.nta * 16;
+    double a0 = 0.0, a1 = 0.0;
+    int swept = 0;
+    for (int j0 = jbeg; j0 < jend; j0 += kChunk) {
+        if (prune) {
+            const int skip = box_gap2<DIST>(flo, fhi, gptr(job.boxes) + (size_t)(j0 / kChunk) * 2 * BD) > thr;
+            if (__builtin_amdgcn_readfirstlane(skip)) continue;
+        }
+        ++swept;
+        __syncthreads();
+        if (threadIdx.x < kChunk) {
+            const int j = j0 + (int)threadIdx.x;
+            const int jl = j < n ? j : n - 1;
+            if constexpr (DIST == kEuclidean) {
+                sh.c.x[threadIdx.x] = P.px[jl];
+                sh.c.y[threadIdx.x] = P.py[jl];
+            } else {
+                sh.c.lon[threadIdx.x] = P.px[jl];
+                sh.c.lat[threadIdx.x] = P.py[jl];
+                sh.c.hx[threadIdx.x] = P.hx[jl];
+                sh.c.hy[threadIdx.x] = P.hy[jl];
+                sh.c.hz[threadIdx.x] = P.hz[jl];
+                sh.c.sl[threadIdx.x] = P.sl[jl];
+            }
+        } else if (threadIdx.x < 2 * kChunk) {
+            // rows past n are zero in the table (padded to a multiple of 64)
+            const int t = (int)threadIdx.x - kChunk;
+            const size_t row = (size_t)(j0 + t) * ldz;
+            sh.z0[t] = Z[row];
+            sh.z1[t] = Z[row + 1];
+        }
+        __syncthreads();
+        if (prune) {
+            const int wskip = box_gap2<DIST>(wlo, whi, gptr(job.boxes) + (size_t)(j0 / kChunk) * 2 * BD) > wthr;
+            if (__builtin_amdgcn_readfirstlane(wskip)) continue;   // no barrier before the next chunk's
+        }
+        const int cnt = min(kChunk, jend - j0);
+#pragma unroll 4
+        for (int e = 0; e < cnt; ++e) {
+            double w = pair_weight<DIST>(KER, fc, sh, e, kc);
+            if (MODE == kModeCV && j0 + e == f) w = 0.0;
+            a0 = fma(w, sh.z0[e], a0);
+            a1 = fma(w, sh.z1[e], a1);
+        }
+    }
+    if (job.pairs && threadIdx.x == 0 && swept)
+        atomicAdd(job.pairs, (unsigned long long)swept * 256ull * 64ull);
+    if (f < n) {
+        GWM_GLOBAL double* out = gptr(job.out) + (size_t)seg * 2 * job.n_pad;
+        out[f] = a0;
+        out[job.n_pad + f] = a1;
+    }
+}
+
+// the kernel id as a constant: the per-pair kernel switch folds away
+template <int DIST, int MODE>
+hipError_t launch_var_k(const SweepJob& j, hipStream_t s) {
+    const dim3 grid((j.n + 255) / 256 * j.nseg);
+    switch (j.kernel) {
+    case kGaussian: hipLaunchKernelGGL((contract_var_f64_kernel<DIST, MODE, kGaussian>), grid, dim3(256), 0, s, j); break;
+    case kExponential:
+        hipLaunchKernelGGL((contract_var_f64_kernel<DIST, MODE, kExponential>), grid, dim3(256), 0, s, j);
+        break;
+    case kBisquare: hipLaunchKernelGGL((contract_var_f64_kernel<DIST, MODE, kBisquare>), grid, dim3(256), 0, s, j); break;
+    case kTricube: hipLaunchKernelGGL((contract_var_f64_kernel<DIST, MODE, kTricube>), grid, dim3(256), 0, s, j); break;
+    default: hipLaunchKernelGGL((contract_var_f64_kernel<DIST, MODE, kBoxcar>), grid, dim3(256), 0, s, j); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_contract_var_f64(int dist, int mode, const SweepJob& j, hipStream_t s) {
+    if (j.nta != 1 || j.ntb != 0 || j.focus_idx || j.Q.n > 0 || j.n <= 0 || j.nseg <= 0 || j.n_pad < j.n ||
+        (mode != kModeCV && mode != kModePlain))
+        return hipErrorInvalidValue;
+    if (j.kernel < kGaussian || j.kernel > kBoxcar) return hipErrorInvalidValue;
+    if (dist == kEuclidean) {
+        if (mode == kModeCV) return launch_var_k<kEuclidean, kModeCV>(j, s);
+        return launch_var_k<kEuclidean, kModePlain>(j, s);
+    }
+    if (mode == kModeCV) return launch_var_k<kGreatCircle, kModeCV>(j, s);
+    return launch_var_k<kGreatCircle, kModePlain>(j, s);
+}
 
 hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s) {
     if (dist == kEuclidean) {

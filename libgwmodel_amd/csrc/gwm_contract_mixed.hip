@@ -700,7 +700,7 @@ void contract_mixed_kernel(SweepLaunch L) {
             if (tt >= (isB ? ntb : nta)) continue;
             const int zc = 16 * tt + col;
             const double unscale = gptr(args.colscale)[zc];
-            GWM_GLOBAL double* o = out + (size_t)((isB ? ldz : 0) + zc) * np + f0;
+            GWM_GLOBAL double* o = out + (size_t)((isB ? ldz : 0) + zc) * np + (f0 - args.out_off);
 #pragma unroll
             for (int q = 0; q < 4; ++q) o[q] = acc[r][t][q] * unscale;
         }

@@ -29,7 +29,8 @@ template <int K>
 __device__ __forceinline__ double sum_parts(const SolveArgs& a, int c, int i) {
     const size_t np = a.n_pad;
     double s = 0.0;
-    for (int g = 0; g < a.nseg; ++g) s += a.part[((size_t)g * a.PT + c) * np + i];
+    const size_t col = (size_t)(i - a.part_off);
+    for (int g = 0; g < a.nseg; ++g) s += a.part[((size_t)g * a.PT + c) * np + col];
     return s;
 }
 
@@ -250,7 +251,7 @@ __device__ __forceinline__ void qdiag_one(const QDiagJob& J, int j) {
         for (int b = a; b < K; ++b) {
             const int c = Sym<K>::zcol(a, b);
             double m = 0.0;
-            for (int g = 0; g < J.nseg; ++g) m += J.part[((size_t)g * J.PT + c) * J.n_pad + j];
+            for (int g = 0; g < J.nseg; ++g) m += J.part[((size_t)g * J.PT + c) * J.n_pad + (j - J.part_off)];
             q += (a == b ? 1.0 : 2.0) * m * x[a] * x[b];
         }
     J.qdiag[j] = 1.0 - 2.0 * J.sii[j] + q;

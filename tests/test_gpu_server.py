@@ -136,3 +136,37 @@ def test_admission_by_memory_budget():
         st = srv.stats()
     assert all(r["status"] == capi.GWM_OK for r in res)
     assert st["peak_inflight"] == 1
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+def test_partial_sum_slices_equal_single_problem(precision):
+    """A partial-sum budget of one byte holds one chunk of 128 focus points
+    per request: every phase (main sweep, FP64 recomputation, qDiag) runs as
+    many slices, each written at its own offset of a buffer of 128 columns.
+    The results must be the single-problem API's bits (mixed: the same
+    certified search)."""
+    say("case: one-chunk partial-sum slices precision=%d" % precision)
+    cases = [MIX[0], MIX[1], MIX[3], MIX[4]]
+    with capi.Server(0, precision, max_inflight=8) as srv:
+        srv.set_partial_budget(1)
+        ids = [submit(srv, c) for c in cases]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    say("rounds %d peak bytes %.3g" % (st["rounds"], st["peak_bytes"]))
+    with capi.Context(0) as ctx:
+        for case, r in zip(cases, res):
+            seed, n, k, kern, adaptive, bw, sel, crit = case
+            coords, X, y = problem(seed, n, k)
+            ctx.set_data(coords, X, y)
+            lo, hi = (20, n) if adaptive else (1.0, 150.0)
+            g = ctx.gwr_run(kern, adaptive, bw, select_bw=sel, crit=crit, precision=precision,
+                            lower=lo if sel else None, upper=hi if sel else None)
+            assert r["status"] == capi.GWM_OK, case
+            assert r["bandwidth"] == g["bandwidth"], case
+            assert np.array_equal(r["betas"], g["betas"]), case
+            assert np.array_equal(r["betasSE"], g["betasSE"]), case
+            assert np.array_equal(r["qDiag"], g["qDiag"]), case
+            assert r["diagnostic"]["AICc"] == g["diagnostic"]["AICc"], case
+    with pytest.raises(capi.GwmError):
+        with capi.Server(0, precision) as srv:
+            srv.set_partial_budget(0.5)
