@@ -69,6 +69,7 @@ constexpr int kChunk = 128;              // focus points; a multiple of every sw
 constexpr double kUnitsPerMs = 1.5e9;
 constexpr double kFp64Factor = 3.5;
 constexpr double kGatherMs = 2.0;   // longest wait for the last posts before a round starts
+constexpr double kGatherBusyMs = 0.5;   // the same while another round is on the device
 constexpr double kSelectUnits = 4.0;
 
 struct Eval {
@@ -109,6 +110,7 @@ struct Request {
     // worker handshake
     std::thread worker;
     bool posted = false;
+    bool in_round = false;       // its posted evaluation has a slice in a round not yet finished
     std::condition_variable wcv;   // the worker waits here for its posted evaluation
     Eval ev;
     // progress of the posted evaluation (Phase): focus points, list entries or
@@ -153,6 +155,28 @@ struct Arena {
     size_t cap = 0;
 };
 
+struct Request;
+
+// One round in flight.  Two slots alternate: while the device runs one
+// round, the host finishes the previous one and launches the next, so the
+// stream always holds work.  Tables go up from pinned host memory and results
+// come back after the round's end event on a stream of their own, so neither
+// copy waits behind the round queued after it.
+struct RoundSlot {
+    Arena arena;                       // job tables and compaction counts on the device
+    char* host = nullptr;              // pinned copy of the tables
+    size_t host_cap = 0;
+    double* red = nullptr;             // device reductions of the completed evaluations
+    int red_cap = 0;
+    double* red_host = nullptr;        // pinned
+    int* cnt_host = nullptr;           // pinned compaction counts
+    int cnt_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool busy = false;
+    std::vector<Request*> batch, checks, finished;
+    size_t off_cnt = 0;
+};
+
 }  // namespace
 
 struct gwm_hip_server {
@@ -175,9 +199,9 @@ struct gwm_hip_server {
     long long rounds = 0, solves = 0;
     double busy_ms = 0.0;
     std::string err;
-    Arena arena;
-    double* round_red = nullptr;
-    int round_cap = 0;
+    RoundSlot slots[2];
+    int slot_next = 0;
+    hipStream_t rb_stream = nullptr;   // read-back of finished rounds
     int* sel_left = nullptr;           // scratch of the batched adaptive-bandwidth select
     unsigned int* sel_nleft = nullptr;
     int sel_cap = 0;
@@ -439,16 +463,29 @@ SolveArgs solve_job(Request* r, int PT, int colB, const double* bw_pt, int f0) {
 
 // Executes one round: a slice of every posted evaluation.  Evaluations that
 // complete get their results and phase kDone.
-int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
-    const double t0 = now_ms();
+// Plans and launches one round into slot R (the slot is idle); its results
+// are read back by finish_round.
+int launch_round(gwm_hip_server* S, std::vector<Request*>& batch, RoundSlot& R) {
     const int ne = (int)batch.size();
-    if (ne > S->round_cap) {
-        if (S->round_red) hipFree(S->round_red);
-        S->round_red = nullptr;
-        S->round_cap = 0;
-        SRV_CHECK(S, hipMalloc((void**)&S->round_red, (size_t)ne * 2 * 5 * sizeof(double)));
-        S->round_cap = ne * 2;
+    if (ne > R.red_cap) {
+        if (R.red) hipFree(R.red);
+        if (R.red_host) hipHostFree(R.red_host);
+        R.red = nullptr;
+        R.red_host = nullptr;
+        R.red_cap = 0;
+        SRV_CHECK(S, hipMalloc((void**)&R.red, (size_t)ne * 2 * 5 * sizeof(double)));
+        SRV_CHECK(S, hipHostMalloc((void**)&R.red_host, (size_t)ne * 2 * 5 * sizeof(double), hipHostMallocDefault));
+        R.red_cap = ne * 2;
     }
+    if (ne > R.cnt_cap) {
+        if (R.cnt_host) hipHostFree(R.cnt_host);
+        R.cnt_host = nullptr;
+        R.cnt_cap = 0;
+        SRV_CHECK(S, hipHostMalloc((void**)&R.cnt_host, (size_t)ne * 2 * sizeof(int), hipHostMallocDefault));
+        R.cnt_cap = ne * 2;
+    }
+    if (!R.ev0) SRV_CHECK(S, hipEventCreate(&R.ev0));
+    if (!R.ev1) SRV_CHECK(S, hipEventCreate(&R.ev1));
     // share the round's budget, shortest remaining work first
     std::vector<int> order(ne);
     std::vector<double> rem(ne);
@@ -549,7 +586,7 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         if (sl.done) {
             ReduceJob rj;
             rj.n = c->n;
-            rj.out = S->round_red + 5 * finished.size();
+            rj.out = R.red + 5 * finished.size();
             rj.fail = r->dfail;
             if (ev.type == kEvalCrit) {
                 rj.a = c->res2;
@@ -602,22 +639,30 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
     std::vector<int*> zeros;
     for (Request* r : refine_starts) zeros.push_back(r->dfail);
     const size_t off_zero = zeros.empty() ? 0 : T.push(zeros.data(), zeros.size() * sizeof(int*));
-    if (T.host.size() > S->arena.cap) {
-        if (S->arena.base) hipFree(S->arena.base);
-        S->arena.base = nullptr;
-        S->arena.cap = 0;
+    if (T.host.size() > R.arena.cap) {
+        if (R.arena.base) hipFree(R.arena.base);
+        if (R.host) hipHostFree(R.host);
+        R.arena.base = nullptr;
+        R.host = nullptr;
+        R.arena.cap = 0;
+        R.host_cap = 0;
         size_t cap = std::max<size_t>(T.host.size() * 2, 1 << 20);
-        SRV_CHECK(S, hipMalloc((void**)&S->arena.base, cap));
-        S->arena.cap = cap;
+        SRV_CHECK(S, hipMalloc((void**)&R.arena.base, cap));
+        SRV_CHECK(S, hipHostMalloc((void**)&R.host, cap, hipHostMallocDefault));
+        R.arena.cap = cap;
+        R.host_cap = cap;
     }
-    const char* dev = S->arena.base;
+    const char* dev = R.arena.base;
     for (size_t i = 0; i < cjobs.size(); ++i) {
         CompactJob j = cjobs[i];
-        j.count = reinterpret_cast<int*>(S->arena.base + off_cnt) + i;
+        j.count = reinterpret_cast<int*>(R.arena.base + off_cnt) + i;
         std::memcpy(T.host.data() + off_cj + i * sizeof(CompactJob), &j, sizeof(CompactJob));
     }
-    if (!T.host.empty())
-        SRV_CHECK(S, hipMemcpyAsync(S->arena.base, T.host.data(), T.host.size(), hipMemcpyHostToDevice, S->stream));
+    SRV_CHECK(S, hipEventRecord(R.ev0, S->stream));
+    if (!T.host.empty()) {
+        std::memcpy(R.host, T.host.data(), T.host.size());
+        SRV_CHECK(S, hipMemcpyAsync(R.arena.base, R.host, T.host.size(), hipMemcpyHostToDevice, S->stream));
+    }
     // the failure flag of an FP64 recomputation is that of its own solves
     // (every point the mixed solves failed on is listed)
     SRV_CHECK(S, launch_zero_ints(reinterpret_cast<int* const*>(dev + off_zero), (int)zeros.size(), S->stream));
@@ -646,13 +691,9 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
                                         g.start.back(), S->stream));
     // 3b. mixed evaluations whose main sweep completed: list the focus points
     //     with an ill-conditioned local design
-    std::vector<int> ccount(checks.size(), 0);
-    if (!checks.empty()) {
+    if (!checks.empty())
         SRV_CHECK(S, launch_compact_batch(reinterpret_cast<const CompactJob*>(dev + off_cj), (int)checks.size(), cmax,
                                           S->stream));
-        SRV_CHECK(S, hipMemcpyAsync(ccount.data(), dev + off_cnt, ccount.size() * sizeof(int), hipMemcpyDeviceToHost,
-                                    S->stream));
-    }
     // 4. qDiag second sweeps of the fits (G complete: every solve of the fit precedes on the stream)
     for (Request* r : qstarts)
         SRV_CHECK(S, launch_chunk_max(r->ctx->bwi, r->n, r->ctx->chunk_bw, S->stream));
@@ -664,26 +705,46 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
         SRV_CHECK(S, launch_qdiag_batch(g.k, reinterpret_cast<const QDiagJob*>(dev + g.off_jobs),
                                         reinterpret_cast<const int*>(dev + g.off_start), (int)g.jobs.size(),
                                         g.start.back(), S->stream));
-    // 5. reductions and results of the completed evaluations
+    // 5. reductions of the completed evaluations
+    if (!finished.empty())
+        SRV_CHECK(S, launch_reduce_batch(reinterpret_cast<const ReduceJob*>(dev + off_red), (int)finished.size(),
+                                         S->stream));
+    SRV_CHECK(S, hipEventRecord(R.ev1, S->stream));
+    R.batch = batch;
+    R.checks = checks;
+    R.finished = finished;
+    R.off_cnt = off_cnt;
+    R.busy = true;
+    return GWM_OK;
+}
+
+// Waits for slot R's round, reads its results back and advances the phases of
+// its evaluations.
+int finish_round(gwm_hip_server* S, RoundSlot& R) {
+    R.busy = false;
+    SRV_CHECK(S, hipEventSynchronize(R.ev1));
+    const std::vector<Request*>& checks = R.checks;
+    const std::vector<Request*>& finished = R.finished;
     const int nd = (int)finished.size();
-    std::vector<double> red((size_t)nd * 5);
-    if (nd) {
-        SRV_CHECK(S, launch_reduce_batch(reinterpret_cast<const ReduceJob*>(dev + off_red), nd, S->stream));
-        SRV_CHECK(S, hipMemcpyAsync(red.data(), S->round_red, red.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                    S->stream));
-    }
+    if (!checks.empty())
+        SRV_CHECK(S, hipMemcpyAsync(R.cnt_host, R.arena.base + R.off_cnt, checks.size() * sizeof(int),
+                                    hipMemcpyDeviceToHost, S->rb_stream));
+    if (nd)
+        SRV_CHECK(S, hipMemcpyAsync(R.red_host, R.red, (size_t)nd * 5 * sizeof(double), hipMemcpyDeviceToHost,
+                                    S->rb_stream));
     for (Request* r : finished) {
         if (r->ev.type != kEvalFit) continue;
         const size_t nk = (size_t)r->n * r->k;
         r->hbetas.resize(nk);
         r->hse.resize(nk);
         r->hq.resize(r->n);
-        SRV_CHECK(S, hipMemcpyAsync(r->hbetas.data(), r->ctx->betas, nk * 8, hipMemcpyDeviceToHost, S->stream));
-        SRV_CHECK(S, hipMemcpyAsync(r->hse.data(), r->ctx->se, nk * 8, hipMemcpyDeviceToHost, S->stream));
-        SRV_CHECK(S, hipMemcpyAsync(r->hq.data(), r->ctx->qd, (size_t)r->n * 8, hipMemcpyDeviceToHost, S->stream));
+        SRV_CHECK(S, hipMemcpyAsync(r->hbetas.data(), r->ctx->betas, nk * 8, hipMemcpyDeviceToHost, S->rb_stream));
+        SRV_CHECK(S, hipMemcpyAsync(r->hse.data(), r->ctx->se, nk * 8, hipMemcpyDeviceToHost, S->rb_stream));
+        SRV_CHECK(S, hipMemcpyAsync(r->hq.data(), r->ctx->qd, (size_t)r->n * 8, hipMemcpyDeviceToHost, S->rb_stream));
     }
-    SRV_CHECK(S, hipStreamSynchronize(S->stream));
-    for (size_t i = 0; i < checks.size(); ++i) checks[i]->nrefine = ccount[i];
+    SRV_CHECK(S, hipStreamSynchronize(S->rb_stream));
+    const double* red = R.red_host;
+    for (size_t i = 0; i < checks.size(); ++i) checks[i]->nrefine = R.cnt_host[i];
     // checked evaluations: recompute the listed points next, or go on
     for (Request* r : checks) {
         if (r->nrefine > 0) {
@@ -724,9 +785,11 @@ int run_round(gwm_hip_server* S, std::vector<Request*>& batch) {
             ev.shat[1] = o[1];
         }
     }
+    float ms = 0.0f;
+    SRV_CHECK(S, hipEventElapsedTime(&ms, R.ev0, R.ev1));
     S->rounds += 1;
     S->solves += nsolves;
-    S->busy_ms += now_ms() - t0;
+    S->busy_ms += ms;
     return GWM_OK;
 }
 
@@ -884,23 +947,26 @@ void scheduler_main(gwm_hip_server* S) {
             S->peak_inflight = std::max<int>(S->peak_inflight, (int)S->active.size());
             R->worker = std::thread(worker_main, S, R);
         }
-        if (S->active.empty()) {
+        RoundSlot& prev = S->slots[S->slot_next ^ 1];   // the round in flight, if any
+        if (S->active.empty() && !prev.busy) {
             if (S->stopping && S->pending.empty()) break;
             S->cv.wait(lk, [&] { return S->stopping || !S->pending.empty(); });
             continue;
         }
-        // wait until every in-flight request has posted an evaluation or
-        // finished, for at most kGatherMs: a worker that is still setting up
-        // or finishing on the host joins a later round
-        S->cv.wait_for(lk, std::chrono::microseconds((long long)(kGatherMs * 1000)), [&] {
+        // Wait until every in-flight request outside the round in flight has
+        // posted an evaluation or finished, for at most kGatherMs (a worker
+        // still setting up or finishing on the host joins a later round), or
+        // kGatherBusyMs while the device works on a round in flight.
+        const double gather = prev.busy ? kGatherBusyMs : kGatherMs;
+        S->cv.wait_for(lk, std::chrono::microseconds((long long)(gather * 1000)), [&] {
             for (Request* R : S->active)
-                if (!R->posted && !R->finished) return false;
+                if (!R->in_round && !R->posted && !R->finished) return false;
             return true;
         });
         std::vector<Request*> batch;
         for (Request* R : S->active)
-            if (R->posted) batch.push_back(R);
-        if (batch.empty()) {
+            if (R->posted && !R->in_round) batch.push_back(R);
+        if (batch.empty() && !prev.busy) {
             S->cv.wait(lk, [&] {
                 for (Request* R : S->active)
                     if (R->posted || R->finished) return true;
@@ -908,19 +974,39 @@ void scheduler_main(gwm_hip_server* S) {
             });
             continue;
         }
+        RoundSlot& cur = S->slots[S->slot_next];
+        for (Request* R : batch) R->in_round = true;
         lk.unlock();
-        int rc = run_round(S, batch);
+        int rc = batch.empty() ? GWM_OK : launch_round(S, batch, cur);
+        int rc_prev = prev.busy ? finish_round(S, prev) : GWM_OK;
         lk.lock();
-        for (Request* R : batch) {
-            if (rc) {
-                R->ev.rc = rc;
-                R->ev.value = DBL_MAX;
-            } else if (R->phase != kDone) {
-                continue;   // in progress: more slices next round
+        // requests of the finished round: completed evaluations wake their
+        // workers, the others post their next slice
+        auto settle = [&](const std::vector<Request*>& rs, int code) {
+            for (Request* R : rs) {
+                R->in_round = false;
+                if (code) {
+                    R->ev.rc = code;
+                    R->ev.value = DBL_MAX;
+                } else if (R->phase != kDone) {
+                    continue;   // in progress: more slices in a later round
+                }
+                R->posted = false;
+                R->wcv.notify_one();   // wake only this request's worker
             }
-            R->posted = false;
-            R->wcv.notify_one();   // wake only this request's worker
+        };
+        if (!prev.batch.empty()) {
+            settle(prev.batch, rc_prev);
+            prev.batch.clear();
         }
+        if (rc) {
+            // a launch that failed part-way: wait for what it queued, then fail its requests
+            hipStreamSynchronize(S->stream);
+            cur.busy = false;
+            settle(batch, rc);
+            cur.batch.clear();
+        }
+        if (!batch.empty()) S->slot_next ^= 1;
     }
 }
 
@@ -937,7 +1023,9 @@ int gwm_hip_server_create(int device, int precision, double budget_bytes, int ma
     S->device = device;
     S->precision = precision;
     S->max_inflight = max_inflight;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&S->rb_stream, hipStreamNonBlocking) != hipSuccess) {
+        if (S->stream) hipStreamDestroy(S->stream);
         delete S;
         return GWM_ERR_HIP;
     }
@@ -1064,8 +1152,16 @@ void gwm_hip_server_destroy(gwm_hip_server* S) {
     }
     if (S->sched.joinable()) S->sched.join();
     for (auto& kv : S->reqs) delete kv.second;
-    if (S->arena.base) hipFree(S->arena.base);
-    if (S->round_red) hipFree(S->round_red);
+    for (RoundSlot& R : S->slots) {
+        if (R.arena.base) hipFree(R.arena.base);
+        if (R.host) hipHostFree(R.host);
+        if (R.red) hipFree(R.red);
+        if (R.red_host) hipHostFree(R.red_host);
+        if (R.cnt_host) hipHostFree(R.cnt_host);
+        if (R.ev0) hipEventDestroy(R.ev0);
+        if (R.ev1) hipEventDestroy(R.ev1);
+    }
+    if (S->rb_stream) hipStreamDestroy(S->rb_stream);
     if (S->sel_left) hipFree(S->sel_left);
     if (S->sel_nleft) hipFree(S->sel_nleft);
     if (S->stream) hipStreamDestroy(S->stream);
