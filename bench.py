@@ -254,11 +254,12 @@ def run_batch(capi, synth, num, loop):
         res = [srv.wait(i) for i in ids]
         dt = time.perf_counter() - t0
         st = srv.stats()
+        tuned = srv.autotune_report()
     bad = sum(r["status"] != capi.GWM_OK for r in res)
     solves = sum(r["solves"] for r in res)
     out = {"problems": num, "sum_n": sum_n, "solves": solves, "seconds": dt, "solves_per_s": solves / dt,
            "rounds": st["rounds"], "device_busy_frac": st["busy_ms"] * 1e-3 / dt,
-           "peak_device_bytes": st["peak_bytes"], "failed": bad}
+           "peak_device_bytes": st["peak_bytes"], "failed": bad, "server_autotune": tuned}
     if loop:
         log("batch: loop of single-problem calls over the same %d problems" % num)
         ctx = capi.Context(0)

@@ -333,6 +333,17 @@ int gwm_hip_server_set_round_ms(gwm_hip_server* server, double round_ms);
  * term.  Applies to requests submitted afterwards; results do not depend on
  * it. */
 int gwm_hip_server_set_partial_budget(gwm_hip_server* server, double bytes);
+/* The server's tuning of batched sweep launches (no counterpart in the
+ * reference): a class (distance, sweep mode, precision, column tiles) whose
+ * default is two focus tiles per wave runs its first three launches of at
+ * least 512 workgroups with two focus tiles per wave and three with one,
+ * timed with HIP events per launch and normalised by the
+ * pairs the launch swept (counted on the device), and keeps the faster
+ * shape; results do not
+ * depend on it.  One text line per tuned class (chosen shape, microseconds
+ * per million pairs of each shape, rounds timed), NUL-terminated and
+ * truncated to cap bytes. */
+int gwm_hip_server_autotune_report(gwm_hip_server* server, char* buf, int cap);
 int gwm_hip_server_submit(gwm_hip_server* server, int n, int k, const double* coords, const double* X,
                           const double* y, int distance, int kernel, int adaptive, double bw, int select_bw,
                           int criterion, int has_lower, double lower, int has_upper, double upper, double* betas,

@@ -80,6 +80,7 @@ SIGNATURES = {
     "gwm_hip_server_set_refine_condition": (_i, [_vp, _d, _d]),
     "gwm_hip_server_set_round_ms": (_i, [_vp, _d]),
     "gwm_hip_server_set_partial_budget": (_i, [_vp, _d]),
+    "gwm_hip_server_autotune_report": (_i, [_vp, ctypes.c_char_p, _i]),
     "gwm_hip_server_submit": (_i, [_vp, _i, _i, _dp, _dp, _dp, _i, _i, _i, _d, _i, _i, _i, _d, _i, _d, _dp, _dp, _dp,
                                    _dp, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "gwm_hip_server_wait": (_i, [_vp, ctypes.c_longlong, _ip, _dp, ctypes.POINTER(ctypes.c_longlong)]),
@@ -452,6 +453,12 @@ class Server:
         rc = self._lib.gwm_hip_server_set_partial_budget(self.h, float(nbytes))
         if rc != GWM_OK:
             raise GwmError(rc)
+
+    def autotune_report(self):
+        """Lines of the server's batched-sweep tuning table."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._lib.gwm_hip_server_autotune_report(self.h, buf, len(buf))
+        return [l for l in buf.value.decode().splitlines() if l]
 
     def set_round_ms(self, ms):
         """Work budget of a round in ms of device time (<= 0: whole evaluations)."""

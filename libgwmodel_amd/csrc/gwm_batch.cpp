@@ -30,9 +30,11 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstddef>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -157,6 +159,37 @@ struct Arena {
 
 struct Request;
 
+// Online tuning of batched sweep launches.  A class (distance, mode,
+// precision, column tiles) whose default is two focus tiles per wave is
+// launched alternately with two and with one for its first kTuneRounds
+// rounds each (launches of at least kTuneMinItems workgroups), timed with
+// events and normalised by the pairs the launch swept (counted on the
+// device); the faster shape is kept for the server's lifetime.  Every
+// focus point sums its data in the same order in either shape, so results do
+// not depend on the choice.
+struct TuneClass {
+    int dist, mode, prec, nta, ntb;
+    bool operator<(const TuneClass& o) const {
+        return std::tie(dist, mode, prec, nta, ntb) < std::tie(o.dist, o.mode, o.prec, o.nta, o.ntb);
+    }
+};
+struct SrvTune {
+    double ms[3] = {0.0, 0.0, 0.0}, units[3] = {0.0, 0.0, 0.0};
+    int samples[3] = {0, 0, 0};
+    int chosen = -1;   // focus tiles per wave once decided
+};
+constexpr int kTuneRounds = 3;
+// launches smaller than this (workgroups) are dominated by their fixed
+// latency, whatever the shape: they are not samples
+constexpr int kTuneMinItems = 512;
+struct TimedGroup {
+    TuneClass key{};
+    int R = 2;
+    size_t off_pairs = 0;                      // its swept-pair count in the slot's arena
+    unsigned long long* pairs_host = nullptr;  // pinned read-back of that count
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
 // One round in flight.  Two slots alternate: while the device runs one
 // round, the host finishes the previous one and launches the next, so the
 // stream always holds work.  Tables go up from pinned host memory and results
@@ -175,6 +208,8 @@ struct RoundSlot {
     bool busy = false;
     std::vector<Request*> batch, checks, finished;
     size_t off_cnt = 0;
+    std::vector<TimedGroup> timed;     // sweep launches timed for the tuner (first ntimed)
+    int ntimed = 0;
 };
 
 }  // namespace
@@ -201,6 +236,8 @@ struct gwm_hip_server {
     std::string err;
     RoundSlot slots[2];
     int slot_next = 0;
+    std::map<TuneClass, SrvTune> tune;   // the batched-sweep tuner
+    std::mutex tune_mu;
     hipStream_t rb_stream = nullptr;   // read-back of finished rounds
     int* sel_left = nullptr;           // scratch of the batched adaptive-bandwidth select
     unsigned int* sel_nleft = nullptr;
@@ -237,6 +274,9 @@ struct SweepGroup {
     std::vector<SweepJob> jobs;
     std::vector<int> items;
     size_t off_jobs = 0, off_items = 0;
+    int R = 1;            // focus tiles per wave of this launch (the tuner's choice)
+    bool timed = false;   // a launch the tuner times: its jobs count the pairs they sweep
+    size_t off_pairs = 0; // that count in the arena
 };
 
 struct SolveGroup {
@@ -304,10 +344,26 @@ void add_items(SweepGroup& g, int job, int f0, int f1, int nseg, int R) {
     }
 }
 
-SweepGroup& group_of(std::vector<SweepGroup>& gs, int dist, int mode, int prec, int nta, int ntb) {
+// Focus tiles per wave of a batched launch of class (dist, mode, prec, nta,
+// ntb): the tuner's choice, or while it is undecided the shape with fewer
+// timed rounds so far (the default first).
+int tuned_r(gwm_hip_server* S, int dist, int mode, int prec, int nta, int ntb, bool* timed) {
+    *timed = false;
+    const int rd = contract_r(nta, ntb, prec == GWM_MIXED, mode, dist);
+    if (rd != 2) return rd;
+    std::lock_guard<std::mutex> lk(S->tune_mu);
+    const SrvTune& t = S->tune[TuneClass{dist, mode, prec, nta, ntb}];
+    if (t.chosen > 0) return t.chosen;
+    *timed = true;
+    return t.samples[2] <= t.samples[1] ? 2 : 1;
+}
+
+SweepGroup& group_of(gwm_hip_server* S, std::vector<SweepGroup>& gs, int dist, int mode, int prec, int nta,
+                     int ntb) {
     for (auto& g : gs)
         if (g.dist == dist && g.mode == mode && g.precision == prec && g.nta == nta && g.ntb == ntb) return g;
     gs.push_back(SweepGroup{dist, mode, prec, nta, ntb, {}, {}, 0, 0});
+    gs.back().R = tuned_r(S, dist, mode, prec, nta, ntb, &gs.back().timed);
     return gs.back();
 }
 
@@ -327,15 +383,36 @@ SolveGroup& sgroup_of(std::vector<SolveGroup>& gs, int k, int mode) {
     return gs.back();
 }
 
-int launch_sweeps(gwm_hip_server* S, std::vector<SweepGroup>& gs, const char* dev) {
+// Launches a round's sweep groups; groups of classes still being tuned are
+// timed with events of slot R (read by finish_round).
+int launch_sweeps(gwm_hip_server* S, std::vector<SweepGroup>& gs, const char* dev, RoundSlot& R) {
     for (auto& g : gs) {
         SweepLaunch L;
         std::memset(&L.job0, 0, sizeof(L.job0));
         L.jobs = reinterpret_cast<const SweepJob*>(dev + g.off_jobs);
         L.items = reinterpret_cast<const int*>(dev + g.off_items);
         L.nitems = (int)g.items.size() / 3;
+        const int rd = contract_r(g.nta, g.ntb, g.precision == GWM_MIXED, g.mode, g.dist);
+        L.shape_r = g.R < rd ? 1 : 0;
+        const TuneClass key{g.dist, g.mode, g.precision, g.nta, g.ntb};
+        TimedGroup* tg = nullptr;
+        if (g.timed && L.nitems >= kTuneMinItems) {
+            if (R.ntimed == (int)R.timed.size()) {
+                R.timed.emplace_back();
+                SRV_CHECK(S, hipEventCreate(&R.timed.back().a));
+                SRV_CHECK(S, hipEventCreate(&R.timed.back().b));
+                SRV_CHECK(S, hipHostMalloc((void**)&R.timed.back().pairs_host, sizeof(unsigned long long),
+                                           hipHostMallocDefault));
+            }
+            tg = &R.timed[R.ntimed++];
+            tg->key = key;
+            tg->R = g.R;
+            tg->off_pairs = g.off_pairs;
+            SRV_CHECK(S, hipEventRecord(tg->a, S->stream));
+        }
         if (g.precision == GWM_MIXED) SRV_CHECK(S, launch_contract_mixed(g.dist, g.mode, g.nta, g.ntb, L, S->stream));
         else SRV_CHECK(S, launch_contract_f64(g.dist, g.mode, g.nta, g.ntb, L, S->stream));
+        if (tg) SRV_CHECK(S, hipEventRecord(tg->b, S->stream));
     }
     return GWM_OK;
 }
@@ -536,10 +613,9 @@ int launch_round(gwm_hip_server* S, std::vector<Request*>& batch, RoundSlot& R) 
                 sel[d].push_back(SelectJob{c->points(), ev.bw, c->bwi, sl.a0, c->prune ? c->boxes : nullptr});
                 sel_start[d].push_back(sel_start[d].back() + sl.a1 - sl.a0);
             }
-            SweepGroup& g = group_of(sweeps, r->dist, mode, prec, nta, tb);
+            SweepGroup& g = group_of(S, sweeps, r->dist, mode, prec, nta, tb);
             g.jobs.push_back(sweep_job(r, mode, nta, tb, bw_pt, ev.bw, sl.a0));
-            add_items(g, (int)g.jobs.size() - 1, sl.a0, sl.a1, r->nseg,
-                      contract_r(nta, tb, prec == GWM_MIXED, mode, r->dist));
+            add_items(g, (int)g.jobs.size() - 1, sl.a0, sl.a1, r->nseg, g.R);
             SolveGroup& sg = sgroup_of(solves, k, smode);
             if (sg.start.empty()) sg.start.push_back(0);
             SolveArgs sa = solve_job(r, PT, colB, bw_pt, sl.a0);
@@ -551,13 +627,13 @@ int launch_round(gwm_hip_server* S, std::vector<Request*>& batch, RoundSlot& R) 
         if (sl.e1 > sl.e0) {
             // FP64 recomputation of listed focus points: FP64 kernel, the
             // request's own segments, compact partial-sum columns in part
-            SweepGroup& g = group_of(sweeps, r->dist, mode, GWM_FP64, nta, tb);
+            SweepGroup& g = group_of(S, sweeps, r->dist, mode, GWM_FP64, nta, tb);
             SweepJob j = sweep_job(r, mode, nta, tb, bw_pt, ev.bw, sl.e0);
             j.Z = c->Z;
             j.focus_idx = c->rlist;
             j.nfocus = r->nrefine;
             g.jobs.push_back(j);
-            add_items(g, (int)g.jobs.size() - 1, sl.e0, sl.e1, r->nseg, contract_r(nta, tb, 0, mode, r->dist));
+            add_items(g, (int)g.jobs.size() - 1, sl.e0, sl.e1, r->nseg, g.R);
             SolveGroup& sg = sgroup_of(solves, k, smode);
             if (sg.start.empty()) sg.start.push_back(0);
             SolveArgs sa = solve_job(r, PT, colB, bw_pt, sl.e0);
@@ -571,10 +647,9 @@ int launch_round(gwm_hip_server* S, std::vector<Request*>& batch, RoundSlot& R) 
         if (sl.split && prec == GWM_MIXED) splits.push_back(r);
         if (sl.split && r->adaptive && c->prune) qstarts.push_back(r);
         if (sl.q1 > sl.q0) {
-            SweepGroup& q = group_of(qsweeps, r->dist, kModeQDiag, prec, ntb, 0);
+            SweepGroup& q = group_of(S, qsweeps, r->dist, kModeQDiag, prec, ntb, 0);
             q.jobs.push_back(sweep_job(r, kModeQDiag, ntb, 0, bw_pt, ev.bw, sl.q0));
-            add_items(q, (int)q.jobs.size() - 1, sl.q0, sl.q1, r->nseg,
-                      contract_r(ntb, 0, prec == GWM_MIXED, kModeQDiag, r->dist));
+            add_items(q, (int)q.jobs.size() - 1, sl.q0, sl.q1, r->nseg, q.R);
             QdGroup& qg = kgroup_of(qds, k);
             if (qg.start.empty()) qg.start.push_back(0);
             QDiagJob qj{r->part, r->nseg, ntb * 16, c->n, r->s_cap, c->X, c->sii, c->qd};
@@ -613,6 +688,10 @@ int launch_round(gwm_hip_server* S, std::vector<Request*>& batch, RoundSlot& R) 
         for (auto& g : *gv) {
             g.off_jobs = T.push(g.jobs.data(), g.jobs.size() * sizeof(SweepJob));
             g.off_items = T.push(g.items.data(), g.items.size() * sizeof(int));
+            if (g.timed) {
+                const unsigned long long zero = 0ull;
+                g.off_pairs = T.push(&zero, sizeof(zero));
+            }
         }
     for (auto& g : solves) {
         g.off_jobs = T.push(g.jobs.data(), g.jobs.size() * sizeof(SolveArgs));
@@ -653,6 +732,15 @@ int launch_round(gwm_hip_server* S, std::vector<Request*>& batch, RoundSlot& R) 
         R.host_cap = cap;
     }
     const char* dev = R.arena.base;
+    // jobs of timed launches count their swept pairs (the tuner's work measure)
+    for (auto* gv : {&sweeps, &qsweeps})
+        for (auto& g : *gv)
+            if (g.timed)
+                for (size_t i = 0; i < g.jobs.size(); ++i) {
+                    unsigned long long* cnt = reinterpret_cast<unsigned long long*>(R.arena.base + g.off_pairs);
+                    std::memcpy(T.host.data() + g.off_jobs + i * sizeof(SweepJob) + offsetof(SweepJob, pairs), &cnt,
+                                sizeof(cnt));
+                }
     for (size_t i = 0; i < cjobs.size(); ++i) {
         CompactJob j = cjobs[i];
         j.count = reinterpret_cast<int*>(R.arena.base + off_cnt) + i;
@@ -682,7 +770,8 @@ int launch_round(gwm_hip_server* S, std::vector<Request*>& batch, RoundSlot& R) 
                                                   reinterpret_cast<const int*>(dev + off_sels[d]), (int)sel[d].size(),
                                                   sel_start[d].back(), S->sel_left, S->sel_nleft + d, S->stream));
     // 2. criterion and hat sweeps
-    int rc = launch_sweeps(S, sweeps, dev);
+    R.ntimed = 0;
+    int rc = launch_sweeps(S, sweeps, dev, R);
     if (rc) return rc;
     // 3. local solves
     for (auto& g : solves)
@@ -700,7 +789,7 @@ int launch_round(gwm_hip_server* S, std::vector<Request*>& batch, RoundSlot& R) 
     for (Request* r : splits)
         SRV_CHECK(S, launch_split_fp16(r->ctx->G, r->ctx->rows_pad, ntb_of(r->k) * 16, r->ctx->colmax, r->ctx->Gh,
                                        r->ctx->Gl, r->ctx->gscale, S->stream));
-    if ((rc = launch_sweeps(S, qsweeps, dev))) return rc;
+    if ((rc = launch_sweeps(S, qsweeps, dev, R))) return rc;
     for (auto& g : qds)
         SRV_CHECK(S, launch_qdiag_batch(g.k, reinterpret_cast<const QDiagJob*>(dev + g.off_jobs),
                                         reinterpret_cast<const int*>(dev + g.off_start), (int)g.jobs.size(),
@@ -742,6 +831,9 @@ int finish_round(gwm_hip_server* S, RoundSlot& R) {
         SRV_CHECK(S, hipMemcpyAsync(r->hse.data(), r->ctx->se, nk * 8, hipMemcpyDeviceToHost, S->rb_stream));
         SRV_CHECK(S, hipMemcpyAsync(r->hq.data(), r->ctx->qd, (size_t)r->n * 8, hipMemcpyDeviceToHost, S->rb_stream));
     }
+    for (int i = 0; i < R.ntimed; ++i)
+        SRV_CHECK(S, hipMemcpyAsync(R.timed[i].pairs_host, R.arena.base + R.timed[i].off_pairs,
+                                    sizeof(unsigned long long), hipMemcpyDeviceToHost, S->rb_stream));
     SRV_CHECK(S, hipStreamSynchronize(S->rb_stream));
     const double* red = R.red_host;
     for (size_t i = 0; i < checks.size(); ++i) checks[i]->nrefine = R.cnt_host[i];
@@ -785,6 +877,21 @@ int finish_round(gwm_hip_server* S, RoundSlot& R) {
             ev.shat[1] = o[1];
         }
     }
+    for (int i = 0; i < R.ntimed; ++i) {
+        const TimedGroup& tg = R.timed[i];
+        float gms = 0.0f;
+        SRV_CHECK(S, hipEventElapsedTime(&gms, tg.a, tg.b));
+        std::lock_guard<std::mutex> lk(S->tune_mu);
+        SrvTune& t = S->tune[tg.key];
+        const double units = (double)*tg.pairs_host;
+        if (t.chosen >= 0 || !(units > 0.0)) continue;
+        t.ms[tg.R] += gms;
+        t.units[tg.R] += units;
+        t.samples[tg.R] += 1;
+        if (t.samples[1] >= kTuneRounds && t.samples[2] >= kTuneRounds)
+            t.chosen = t.ms[1] / t.units[1] < t.ms[2] / t.units[2] ? 1 : 2;
+    }
+    R.ntimed = 0;
     float ms = 0.0f;
     SRV_CHECK(S, hipEventElapsedTime(&ms, R.ev0, R.ev1));
     S->rounds += 1;
@@ -1059,6 +1166,30 @@ int gwm_hip_server_set_round_ms(gwm_hip_server* S, double ms) {
     return GWM_OK;
 }
 
+int gwm_hip_server_autotune_report(gwm_hip_server* S, char* buf, int cap) {
+    if (!S || !buf || cap <= 0) return GWM_ERR_INVALID;
+    static const char* modes[] = {"cv", "hat", "qdiag", "plain"};   // ContractMode order
+    std::string out;
+    {
+        std::lock_guard<std::mutex> lk(S->tune_mu);
+        for (const auto& kv : S->tune) {
+            const TuneClass& k = kv.first;
+            const SrvTune& t = kv.second;
+            if (t.samples[1] + t.samples[2] == 0) continue;
+            char line[256];
+            std::snprintf(line, sizeof(line), "%s %s %s nta=%d ntb=%d chosen_r=%d us_per_Mpair=%.4f,%.4f rounds=%d,%d\n",
+                          k.dist == GWM_GREAT_CIRCLE ? "great_circle" : "euclidean",
+                          k.mode >= 0 && k.mode < 4 ? modes[k.mode] : "?", k.prec == GWM_MIXED ? "mixed" : "fp64",
+                          k.nta, k.ntb, t.chosen,
+                          t.units[1] > 0 ? 1e9 * t.ms[1] / t.units[1] : -1.0,
+                          t.units[2] > 0 ? 1e9 * t.ms[2] / t.units[2] : -1.0, t.samples[1], t.samples[2]);
+            out += line;
+        }
+    }
+    std::snprintf(buf, cap, "%s", out.c_str());
+    return GWM_OK;
+}
+
 int gwm_hip_server_set_partial_budget(gwm_hip_server* S, double bytes) {
     if (!S || !(bytes >= 1.0)) return GWM_ERR_INVALID;
     std::lock_guard<std::mutex> lk(S->mu);
@@ -1160,6 +1291,11 @@ void gwm_hip_server_destroy(gwm_hip_server* S) {
         if (R.cnt_host) hipHostFree(R.cnt_host);
         if (R.ev0) hipEventDestroy(R.ev0);
         if (R.ev1) hipEventDestroy(R.ev1);
+        for (TimedGroup& tg : R.timed) {
+            if (tg.a) hipEventDestroy(tg.a);
+            if (tg.b) hipEventDestroy(tg.b);
+            if (tg.pairs_host) hipHostFree(tg.pairs_host);
+        }
     }
     if (S->rb_stream) hipStreamDestroy(S->rb_stream);
     if (S->sel_left) hipFree(S->sel_left);

@@ -325,7 +325,7 @@ template <int NTA, int NTB, int DIST, int MODE>
 hipError_t launch_f64_r(const SweepLaunch& a, hipStream_t s) {
     constexpr int RD = r_of<NTA, NTB>();
     if constexpr (RD == 2) {
-        if (a.shape_r == 1 && !a.items) return launch_f64_rr<NTA, NTB, DIST, MODE, 1>(a, s);
+        if (a.shape_r == 1) return launch_f64_rr<NTA, NTB, DIST, MODE, 1>(a, s);
     }
     return launch_f64_rr<NTA, NTB, DIST, MODE, RD>(a, s);
 }

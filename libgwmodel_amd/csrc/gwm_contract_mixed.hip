@@ -752,7 +752,7 @@ template <int NTA, int NTB, int DIST, int MODE>
 hipError_t launch_mixed_r(const SweepLaunch& a, hipStream_t s) {
     constexpr int RD = mixed_r<NTA, NTB, DIST, MODE>();
     if constexpr (RD == 2) {
-        if (a.shape_r == 1 && !a.items) return launch_mixed_rr<NTA, NTB, DIST, MODE, 1>(a, s);
+        if (a.shape_r == 1) return launch_mixed_rr<NTA, NTB, DIST, MODE, 1>(a, s);
     }
     return launch_mixed_rr<NTA, NTB, DIST, MODE, RD>(a, s);
 }

@@ -134,6 +134,11 @@ def test_uneven_batch_512_server_vs_single_and_oracle(ctx, oracle):
         ids = [srv.submit(p["coords"], p["X"], p["y"], p["kernel"], p["adaptive"], 0.0, select_bw=True,
                           crit=capi.CRIT_CV) for p in probs]
         res = [srv.wait(i) for i in ids]
+        report = srv.autotune_report()
+    # the server's shape tuner ran both shapes on classes of this batch
+    # (launches of either shape are among the bit-exact results below)
+    say("\n".join(report))
+    assert any(" rounds=0," not in l and not l.endswith(",0") for l in report)
     for pi, (p, r) in enumerate(zip(probs, res)):
         assert r["status"] == capi.GWM_OK, pi
         ctx.set_data(p["coords"], p["X"], p["y"])
