@@ -26,7 +26,8 @@ TASK_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_gpu_task.cpp")
 TASK_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_gpu_task")
 
 SOURCES = ["gwm_contract.hip", "gwm_contract_mixed.hip", "gwm_select.hip", "gwm_solve.hip", "gwm_varsel.hip",
-           "gwm_mgwr_hat.hip", "gwm_gemm.hip", "gwm_plan.hip", "gwm_noalloc.hip", "gwm_api.cpp", "gwm_batch.cpp"]
+           "gwm_mgwr_hat.hip", "gwm_gemm.hip", "gwm_plan.hip", "gwm_noalloc.hip", "gwm_problem.cpp", "gwm_sweep.cpp",
+           "gwm_api.cpp", "gwm_batch.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -19,7 +19,7 @@
 #include <tuple>
 #include <vector>
 
-#include "gwm_ctx.h"
+#include "gwm_internal.h"
 
 using namespace gwm;
 using namespace gwm::detail;
@@ -35,7 +35,6 @@ int ntb_of(int k) { return (k * (k + 1) / 2 + 15) / 16; }
 
 namespace {
 
-const double kPi = 3.14159265358979323846;
 // ITelegram::LogLevel (include/gwmodelpp/Logger.h:38-48)
 constexpr int kLogErr = 3;
 constexpr int kLogInfo = 6;
@@ -43,678 +42,6 @@ const std::string kStage = "#stage ";               // GWM_LOG_TAG_STAGE (Algori
 const std::string kBackfit = "#back-fitting ";      // GWM_LOG_TAG_MGWR_BACKFITTING (GWRMultiscale.h:20)
 const std::string kInitialBw = "#initial-bandwidth ";   // GWM_LOG_TAG_MGWR_INITIAL_BW (GWRMultiscale.h:19)
 
-#define GWM_HIP_CHECK(ctx, expr)                                                         \
-    do {                                                                                 \
-        hipError_t e__ = (expr);                                                         \
-        if (e__ != hipSuccess) {                                                         \
-            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e__);             \
-            return e__ == hipErrorOutOfMemory ? GWM_ERR_OOM : GWM_ERR_HIP;               \
-        }                                                                                \
-    } while (0)
-
-// Context buffers come from dalloc and go back through cfree: stream-ordered
-// (hipMallocAsync / hipFreeAsync) on server contexts, plain otherwise.
-template <class T>
-void cfree(gwm_hip_ctx* c, T*& p) {
-    if (p) {
-        if (c->async_mem) hipFreeAsync(p, c->stream);
-        else hipFree(p);
-    }
-    p = nullptr;
-}
-
-template <class T>
-int dalloc(gwm_hip_ctx* c, T** p, size_t count) {
-    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    if (c->async_mem) GWM_HIP_CHECK(c, hipMallocAsync((void**)p, bytes, c->stream));
-    else GWM_HIP_CHECK(c, hipMalloc((void**)p, bytes));
-    return GWM_OK;
-}
-
-void free_problem(gwm_hip_ctx* c) {
-    auto dfree = [c](auto*& p) { cfree(c, p); };
-    dfree(c->px); dfree(c->py); dfree(c->hx); dfree(c->hy_); dfree(c->hz); dfree(c->sl);
-    dfree(c->X); dfree(c->y); dfree(c->Z); dfree(c->G); dfree(c->part);
-    dfree(c->boxes); dfree(c->chunk_bw); dfree(c->sel_left); dfree(c->sel_nleft); dfree(c->dnn);
-    for (auto& t : c->parked) {
-        dfree(t.boxes);
-        dfree(t.dnn);
-        t = gwm_hip_ctx::DistTables{};
-    }
-    for (auto& e : c->bw_cache) dfree(e.buf);
-    c->bw_cache.clear();
-    dfree(c->bwi); dfree(c->res2); dfree(c->cond); dfree(c->sii); dfree(c->rlist); dfree(c->rcount);
-    dfree(c->rpart);
-    c->rpart_bytes = 0; dfree(c->ssi); dfree(c->betas); dfree(c->se);
-    dfree(c->gvec); dfree(c->qd); dfree(c->red); dfree(c->blockmax); dfree(c->vx); dfree(c->vy);
-    dfree(c->vZ); dfree(c->S); dfree(c->fail);
-    dfree(c->Zh); dfree(c->Zl); dfree(c->Gh); dfree(c->Gl); dfree(c->vZh); dfree(c->vZl);
-    dfree(c->zscale); dfree(c->gscale); dfree(c->vscale); dfree(c->colmax);
-    // the plan buffers and the distance table come from hipMalloc on every context
-    if (c->d2tab) hipFree(c->d2tab);
-    c->d2tab = nullptr;
-    c->d2tab_bytes = 0;
-    c->d2_valid = false;
-    if (c->plan_counts) hipFree(c->plan_counts);
-    if (c->plan_order) hipFree(c->plan_order);
-    c->plan_counts = nullptr;
-    c->plan_order = nullptr;
-    c->plan_items = 0;
-    c->max_dist = -1.0;
-    c->part_bytes = 0;
-    c->n = 0;
-}
-
-// Download a sorted-order device array (n x cols) into a caller-order buffer.
-int download_caller(gwm_hip_ctx* c, const double* dev, double* host, int cols) {
-    std::vector<double> tmp((size_t)c->n * cols);
-    GWM_HIP_CHECK(c, hipMemcpyAsync(tmp.data(), dev, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    to_caller(c, tmp.data(), host, cols);
-    return GWM_OK;
-}
-
-// f(i) for i in [0, n) on the host's cores (element-wise work, so the
-// result does not depend on the split).
-template <class F>
-void parallel_for(size_t n, const F& f) {
-    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
-    const size_t T = std::min<size_t>(std::min<size_t>(hw, 16), (n + 16383) / 16384);
-    if (T <= 1) {
-        for (size_t i = 0; i < n; ++i) f(i);
-        return;
-    }
-    std::vector<std::thread> th;
-    for (size_t t = 0; t < T; ++t)
-        th.emplace_back([&, t] {
-            for (size_t i = n * t / T; i < n * (t + 1) / T; ++i) f(i);
-        });
-    for (auto& x : th) x.join();
-}
-
-// Great circle: the half unit vectors (hx, hy, hz) and sin(lat) of points
-// given in degrees.
-void sphere_tables(size_t N, const double* lon, const double* lat, double* hx, double* hy, double* hz, double* sl) {
-    const double de2ra = kPi / 180;
-    parallel_for(N, [&](size_t j) {
-        const double lo = lon[j] * de2ra, la = lat[j] * de2ra;
-        hx[j] = 0.5 * std::cos(la) * std::cos(lo);
-        hy[j] = 0.5 * std::cos(la) * std::sin(lo);
-        hz[j] = 0.5 * std::sin(la);
-        sl[j] = std::sin(la);
-    });
-}
-
-// Position of the cell (x, y) along the Hilbert curve of a 2^bits x 2^bits
-// grid (the classic rotate-and-reflect walk, one quadrant per level).
-uint64_t hilbert_index(uint32_t x, uint32_t y, int bits) {
-    uint64_t d = 0;
-    for (uint32_t s = 1u << (bits - 1); s > 0; s >>= 1) {
-        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
-        d += (uint64_t)s * s * ((3u * rx) ^ ry);
-        if (ry == 0) {
-            if (rx == 1) {
-                x = s - 1 - (x & (s - 1)) + (x & ~(s - 1));
-                y = s - 1 - (y & (s - 1)) + (y & ~(s - 1));
-            }
-            const uint32_t t = x;
-            x = y;
-            y = t;
-        }
-    }
-    return d;
-}
-
-// Sorted order of the points along a Hilbert curve of their planar
-// coordinates (x, y, or lon, lat in degrees): consecutive points are close,
-// and unlike a Morton order the curve has no long jumps, so runs of 64
-// points have small bounding boxes.
-std::vector<int> spatial_order(int n, const double* xs, const double* ys) {
-    const int bits = 16;
-    const double* v[2] = {xs, ys};
-    double lo[2], hi[2];
-    for (int d = 0; d < 2; ++d) {
-        lo[d] = *std::min_element(v[d], v[d] + n);
-        hi[d] = *std::max_element(v[d], v[d] + n);
-    }
-    // 32-bit curve keys, then a stable two-pass radix sort (16-bit digits):
-    // the order of (key, index) pairs, ties in index order
-    std::vector<uint32_t> key(n);
-    const double top = (double)((1u << bits) - 1);
-    parallel_for(n, [&](size_t i) {
-        uint32_t q[2];
-        for (int d = 0; d < 2; ++d) {
-            const double span = hi[d] - lo[d];
-            const double t = span > 0 ? (v[d][i] - lo[d]) / span : 0.0;
-            q[d] = (uint32_t)std::min(top, std::max(0.0, std::floor(t * top + 0.5)));
-        }
-        key[i] = (uint32_t)hilbert_index(q[0], q[1], bits);
-    });
-    std::vector<int> a(n), b(n);
-    for (int i = 0; i < n; ++i) a[i] = i;
-    std::vector<int> count(1 << 16);
-    for (int shift = 0; shift < 32; shift += 16) {
-        std::fill(count.begin(), count.end(), 0);
-        for (int i = 0; i < n; ++i) ++count[(key[a[i]] >> shift) & 0xffff];
-        int sum = 0;
-        for (int& c : count) {
-            const int t = c;
-            c = sum;
-            sum += t;
-        }
-        for (int i = 0; i < n; ++i) b[count[(key[a[i]] >> shift) & 0xffff]++] = a[i];
-        a.swap(b);
-    }
-    return a;
-}
-
-// Bounding boxes [lo_0, hi_0, lo_1, hi_1, ...] of consecutive 64-point chunks.
-std::vector<double> chunk_boxes(int n, int dims, const double* const* xs) {
-    const int nc = (n + 63) / 64;
-    std::vector<double> b((size_t)nc * 2 * dims);
-    for (int ch = 0; ch < nc; ++ch) {
-        const int j0 = ch * 64, j1 = std::min(n, j0 + 64);
-        for (int d = 0; d < dims; ++d) {
-            double mn = xs[d][j0], mx = xs[d][j0];
-            for (int j = j0 + 1; j < j1; ++j) {
-                mn = std::min(mn, xs[d][j]);
-                mx = std::max(mx, xs[d][j]);
-            }
-            b[(size_t)ch * 2 * dims + 2 * d] = mn;
-            b[(size_t)ch * 2 * dims + 2 * d + 1] = mx;
-        }
-    }
-    return b;
-}
-
-
-int ensure_part(gwm_hip_ctx* c, size_t bytes) {
-    if (bytes <= c->part_bytes) return GWM_OK;
-    cfree(c, c->part);
-    c->part_bytes = 0;
-    int rc = dalloc(c, reinterpret_cast<char**>(&c->part), bytes);
-    if (rc) return rc;
-    c->part_bytes = bytes;
-    return GWM_OK;
-}
-
-int ensure_plan(gwm_hip_ctx* c, size_t items) {
-    if (items <= c->plan_items) return GWM_OK;
-    hipFree(c->plan_counts);
-    hipFree(c->plan_order);
-    c->plan_counts = nullptr;
-    c->plan_order = nullptr;
-    c->plan_items = 0;
-    GWM_HIP_CHECK(c, hipMalloc((void**)&c->plan_counts, items * sizeof(int)));
-    GWM_HIP_CHECK(c, hipMalloc((void**)&c->plan_order, items * sizeof(int)));
-    c->plan_items = items;
-    return GWM_OK;
-}
-
-bool stopped(gwm_hip_ctx* c) { return c->stop && c->stop(c->stop_user); }
-
-void timer_begin(gwm_hip_ctx* c) {
-    if (c->timing) hipEventRecord(c->ev0, c->stream);
-}
-int timer_end(gwm_hip_ctx* c, const char* name) {
-    if (!c->timing) return GWM_OK;
-    GWM_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
-    GWM_HIP_CHECK(c, hipEventSynchronize(c->ev1));
-    float ms = 0.f;
-    GWM_HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    auto& s = c->stats[name];
-    s.launches += 1;
-    s.ms += ms;
-    return GWM_OK;
-}
-
-// The squared-distance table of a great-circle problem (launch_d2_table),
-// built once per outermost search / fit / backfit call.  It is optional:
-// when it exceeds d2_cap, or the device cannot hold it beside 8 GiB for
-// everything else, the sweeps form their distances themselves.
-int ensure_d2_table(gwm_hip_ctx* c) {
-    if (c->d2_valid || c->d2_scope <= 0 || c->async_mem || c->dist != kGreatCircle) return GWM_OK;
-    const size_t bytes = (size_t)c->n_pad * (size_t)c->rows_pad * sizeof(float);
-    if (bytes > c->d2_cap) return GWM_OK;
-    if (c->d2tab_bytes < bytes) {
-        if (c->d2tab) hipFree(c->d2tab);
-        c->d2tab = nullptr;
-        c->d2tab_bytes = 0;
-        size_t free_b = 0, total_b = 0;
-        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b < bytes + (size_t(8) << 30)) {
-            (void)hipGetLastError();
-            return GWM_OK;
-        }
-        if (hipMalloc((void**)&c->d2tab, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            c->d2tab = nullptr;
-            return GWM_OK;
-        }
-        c->d2tab_bytes = bytes;
-    }
-    timer_begin(c);
-    GWM_HIP_CHECK(c, launch_d2_table(c->points(), c->n, c->n_pad, c->rows_pad, c->d2tab, c->stream));
-    int rc = timer_end(c, "d2_table");
-    if (rc) return rc;
-    c->d2_valid = true;
-    return GWM_OK;
-}
-
-// Scope of an outermost call whose mixed great-circle sweeps share one
-// distance table; the table is released when that call returns.
-struct D2Scope {
-    gwm_hip_ctx* c;
-    explicit D2Scope(gwm_hip_ctx* ctx) : c(ctx) {
-        if (c->d2_scope++ == 0) c->d2_valid = false;
-    }
-    ~D2Scope() {
-        if (--c->d2_scope == 0) {
-            c->d2_valid = false;
-            if (c->d2tab) {
-                hipStreamSynchronize(c->stream);
-                hipFree(c->d2tab);
-            }
-            c->d2tab = nullptr;
-            c->d2tab_bytes = 0;
-        }
-    }
-    D2Scope(const D2Scope&) = delete;
-    D2Scope& operator=(const D2Scope&) = delete;
-};
-
-// Resolve the bandwidth: adaptive -> per-focus thresholds (a vector of the
-// bandwidth cache, filled by the select on a miss).
-int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw_pt) {
-    *bw_pt = nullptr;
-    if (!adaptive) return GWM_OK;
-    if (!(bw >= 1.0)) {
-        c->err = "adaptive bandwidth must be >= 1 neighbour";
-        return GWM_ERR_INVALID;
-    }
-    for (auto& e : c->bw_cache)
-        if (e.bw == bw && e.dist == c->dist) {
-            e.used = ++c->bw_clock;
-            *bw_pt = e.buf;
-            return GWM_OK;
-        }
-    // Up to 1 GiB and 512 vectors: a multiscale backfit revisits about 15
-    // bandwidths per variable in every iteration, in the same cyclic order,
-    // which a cache smaller than that working set misses every time.
-    const size_t cap = std::min<size_t>(512, std::max<size_t>(4, (size_t(1) << 30) / ((size_t)c->n * sizeof(double))));
-    gwm_hip_ctx::BwEntry* slot = nullptr;
-    if (c->bw_cache.size() < cap) {
-        double* buf = nullptr;
-        int rc = dalloc(c, &buf, c->n);
-        if (rc) return rc;
-        c->bw_cache.push_back(gwm_hip_ctx::BwEntry{bw, c->dist, buf, 0});
-        slot = &c->bw_cache.back();
-    } else {
-        slot = &c->bw_cache[0];
-        for (auto& e : c->bw_cache)
-            if (e.used < slot->used) slot = &e;
-    }
-    slot->bw = NAN;   // not valid until the select succeeds
-    slot->dist = c->dist;
-    timer_begin(c);
-    GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, slot->buf, c->prune ? c->boxes : nullptr,
-                                        c->sel_left, c->sel_nleft, c->stream));
-    int rc = timer_end(c, "select");
-    if (rc) return rc;
-    slot->bw = bw;
-    slot->used = ++c->bw_clock;
-    *bw_pt = slot->buf;
-    return GWM_OK;
-}
-
-// A product table in both representations.
-struct ZPlanes {
-    const double* z64;
-    const _Float16* zh;
-    const _Float16* zl;
-    const double* scale;
-};
-
-const char* mode_name(int mode) {
-    switch (mode) {
-    case kModeCV: return "cv";
-    case kModeHat: return "hat";
-    case kModeQDiag: return "qdiag";
-    default: return "plain";
-    }
-}
-
-// ---------------------------------------------------------------- autotuner
-//
-// Launch shapes of a single-problem sweep that leave every result bit unchanged
-// (each focus point sums its data in the same order whatever the shape):
-//   0: default focus tiles per wave, tiles visited by the coprime stride
-//   1: default tiles per wave, tiles visited in order
-//   2: one focus tile per wave (where the default is two), stride order
-//   3: one focus tile per wave, in order
-// The shape matters where the default grid leaves the chip under-occupied:
-// fewer than kTuneGridMax workgroups (4 per CU).  The first sweep of such a
-// class (distance, mode, precision, column tiles, n to a power of two) runs
-// every candidate kTuneSamples times on the same work, timed with HIP events,
-// and the fastest is used from then on; the table is process-wide.  Larger
-// grids keep the default, whose stride spreads pruned work over the XCDs.
-struct TuneKey {
-    int dist, mode, prec, nta, ntb, nlog2;
-    bool operator<(const TuneKey& o) const {
-        return std::tie(dist, mode, prec, nta, ntb, nlog2) < std::tie(o.dist, o.mode, o.prec, o.nta, o.ntb, o.nlog2);
-    }
-};
-struct TuneState {
-    double best_ms[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-    int ncand = 1;
-    int chosen = -1;
-};
-constexpr int kTuneSamples = 2;
-constexpr int kTuneGridMax = 1024;
-std::mutex g_tune_mu;
-std::map<TuneKey, TuneState> g_tune;
-
-// One sweep of the fused contraction.
-int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const double* bw_pt, double bw, int kernel,
-          int precision, int* PT_out, int* nseg_out, const double* dnn = nullptr) {
-    const bool mixed = precision == GWM_MIXED;
-    int nseg, seg_len;
-    data_segments(c->n, nseg, seg_len);
-    int rc;
-    // FP64 single-column sweeps (multiscale steps): the two used columns of
-    // the single-column table on the VALU, partial sums [nseg][2][n_pad]
-    if (!mixed && nta == 1 && ntb == 0 && zp.z64 == c->vZ && (mode == kModeCV || mode == kModePlain)) {
-        if ((rc = ensure_part(c, (size_t)nseg * 2 * c->n_pad * sizeof(double)))) return rc;
-        SweepJob j;
-        j.P = c->points();
-        j.Z = c->vZ;
-        j.Zh = nullptr;
-        j.Zl = nullptr;
-        j.colscale = nullptr;
-        j.bw_pt = bw_pt;
-        j.bw = bw;
-        j.kernel = kernel;
-        j.n = c->n;
-        j.n_pad = c->n_pad;
-        j.seg_len = seg_len;
-        j.nseg = nseg;
-        j.nta = 1;
-        j.ntb = 0;
-        j.out = c->part;
-        j.dnn = dnn;
-        if (c->prune) j.boxes = c->boxes;
-        if (c->timing) {
-            if (!c->pairs_dev) GWM_HIP_CHECK(c, hipMalloc((void**)&c->pairs_dev, sizeof(unsigned long long)));
-            GWM_HIP_CHECK(c, hipMemsetAsync(c->pairs_dev, 0, sizeof(unsigned long long), c->stream));
-            j.pairs = c->pairs_dev;
-        }
-        timer_begin(c);
-        GWM_HIP_CHECK(c, launch_contract_var_f64(c->dist, mode, j, c->stream));
-        if (c->timing) {
-            std::string name = std::string("contract_fp64_") + mode_name(mode);
-            if ((rc = timer_end(c, name.c_str()))) return rc;
-            unsigned long long pairs = 0;
-            GWM_HIP_CHECK(c, hipMemcpyAsync(&pairs, c->pairs_dev, sizeof(pairs), hipMemcpyDeviceToHost, c->stream));
-            GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-            c->stats[name].pairs += (double)pairs;
-        }
-        *PT_out = 2;
-        *nseg_out = nseg;
-        return GWM_OK;
-    }
-    const int PT = (nta + ntb) * 16;
-    if ((rc = ensure_part(c, (size_t)nseg * PT * c->n_pad * sizeof(double)))) return rc;
-    SweepLaunch L;
-    SweepJob& j = L.job0;
-    j.P = c->points();
-    j.Z = zp.z64;
-    j.Zh = zp.zh;
-    j.Zl = zp.zl;
-    j.colscale = zp.scale;
-    j.bw_pt = bw_pt;
-    j.bw = bw;
-    j.kernel = kernel;
-    j.n = c->n;
-    j.n_pad = c->n_pad;
-    j.seg_len = seg_len;
-    j.nseg = nseg;
-    j.nta = nta;
-    j.ntb = ntb;
-    j.out = c->part;
-    j.dnn = dnn;
-    if (c->prune) {
-        j.boxes = c->boxes;
-        if (mode == kModeQDiag && bw_pt) {
-            GWM_HIP_CHECK(c, launch_chunk_max(bw_pt, c->n, c->chunk_bw, c->stream));
-            j.chunk_bw = c->chunk_bw;
-        }
-    }
-    L.jobs = nullptr;
-    L.items = nullptr;
-    L.nitems = 0;
-    // mixed great-circle sweeps inside a search / fit / backfit call read the
-    // pairs' squared distances from the table (same values, computed once)
-    int ldist = c->dist;
-    if (mixed && c->dist == kGreatCircle) {
-        if ((rc = ensure_d2_table(c))) return rc;
-        if (c->d2_valid) {
-            j.d2tab = c->d2tab;
-            j.d2_ndb = c->rows_pad / 32;
-            ldist = kGcTable;
-        }
-    }
-    if (c->timing) {
-        if (!c->pairs_dev) GWM_HIP_CHECK(c, hipMalloc((void**)&c->pairs_dev, sizeof(unsigned long long)));
-        GWM_HIP_CHECK(c, hipMemsetAsync(c->pairs_dev, 0, sizeof(unsigned long long), c->stream));
-        j.pairs = c->pairs_dev;
-    }
-    // launch shape: forced, or from the autotuner (results are the same for every shape)
-    if (c->force_shape >= 0) {
-        L.linear_order = c->force_shape & 1;
-        L.shape_r = c->force_shape >= 2 ? 1 : 0;
-    } else if (c->autotune) {
-        TuneKey key{ldist, mode, mixed ? 1 : 0, nta, ntb, 0};
-        while ((2 << key.nlog2) <= c->n) ++key.nlog2;
-        int chosen;
-        {
-            std::lock_guard<std::mutex> lk(g_tune_mu);
-            auto it = g_tune.find(key);
-            chosen = it == g_tune.end() ? -1 : it->second.chosen;
-        }
-        if (chosen < 0) {
-            const int r = contract_r(nta, ntb, mixed ? 1 : 0, mode, ldist);
-            const int grid = (c->n + 64 * r - 1) / (64 * r) * nseg;
-            TuneState st;
-            st.ncand = grid >= kTuneGridMax ? 1 : (r == 2 ? 4 : 2);
-            if (st.ncand > 1) {
-                if (!c->tev0) GWM_HIP_CHECK(c, hipEventCreate(&c->tev0));
-                if (!c->tev1) GWM_HIP_CHECK(c, hipEventCreate(&c->tev1));
-                SweepLaunch T = L;
-                T.job0.pairs = nullptr;
-                for (int q = 0; q < st.ncand; ++q)
-                    for (int rep = 0; rep < kTuneSamples; ++rep) {
-                        T.linear_order = q & 1;
-                        T.shape_r = q >= 2 ? 1 : 0;
-                        float ms = 0.0f;
-                        GWM_HIP_CHECK(c, hipEventRecord(c->tev0, c->stream));
-                        if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(ldist, mode, nta, ntb, T, c->stream));
-                        else GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, T, c->stream));
-                        GWM_HIP_CHECK(c, hipEventRecord(c->tev1, c->stream));
-                        GWM_HIP_CHECK(c, hipEventSynchronize(c->tev1));
-                        GWM_HIP_CHECK(c, hipEventElapsedTime(&ms, c->tev0, c->tev1));
-                        st.best_ms[q] = std::min(st.best_ms[q], (double)ms);
-                    }
-            }
-            int best = 0;
-            for (int q = 1; q < st.ncand; ++q)
-                if (st.best_ms[q] < st.best_ms[best]) best = q;
-            st.chosen = best;
-            std::lock_guard<std::mutex> lk(g_tune_mu);
-            g_tune.emplace(key, st);
-            chosen = g_tune[key].chosen;
-        }
-        L.linear_order = chosen & 1;
-        L.shape_r = chosen >= 2 ? 1 : 0;
-        // Large pruned sweeps run their (tile, segment) items longest first
-        // (gwm_plan.hip): narrow bandwidths leave a few busy items per tile,
-        // and in plain order the sweep ends on a tail of them.
-        const int r = contract_r(nta, ntb, mixed ? 1 : 0, mode, ldist);
-        const int tiles = (c->n + 64 * r - 1) / (64 * r);
-        if (c->prune && chosen == 0 && tiles * nseg >= kTuneGridMax && nseg <= kPlanMaxSeg) {
-            if ((rc = ensure_plan(c, (size_t)tiles * nseg))) return rc;
-            // qDiag thresholds come from the streamed points' bandwidths (chunk_bw)
-            const double* focus_bw = mode == kModeQDiag ? nullptr : bw_pt;
-            GWM_HIP_CHECK(c, launch_sweep_plan(c->dist, c->points(), c->n, c->boxes, focus_bw, bw, j.chunk_bw, kernel,
-                                               mixed ? 1 : 0, 64 * r, seg_len, nseg, tiles, c->plan_counts,
-                                               c->plan_order, c->stream));
-            L.order = c->plan_order;
-            L.norder = tiles * nseg;
-        }
-    }
-    timer_begin(c);
-    if (mixed) GWM_HIP_CHECK(c, launch_contract_mixed(ldist, mode, nta, ntb, L, c->stream));
-    else GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, L, c->stream));
-    if (c->timing) {
-        std::string name = std::string("contract_") + (mixed ? "mixed_" : "fp64_") + mode_name(mode);
-        if ((rc = timer_end(c, name.c_str()))) return rc;
-        unsigned long long pairs = 0;
-        GWM_HIP_CHECK(c, hipMemcpyAsync(&pairs, c->pairs_dev, sizeof(pairs), hipMemcpyDeviceToHost, c->stream));
-        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-        c->stats[name].pairs += (double)pairs;
-    }
-    *PT_out = PT;
-    *nseg_out = nseg;
-    return GWM_OK;
-}
-
-ZPlanes design_planes(gwm_hip_ctx* c) { return ZPlanes{c->Z, c->Zh, c->Zl, c->zscale}; }
-ZPlanes g_planes(gwm_hip_ctx* c) { return ZPlanes{c->G, c->Gh, c->Gl, c->gscale}; }
-ZPlanes var_planes(gwm_hip_ctx* c) { return ZPlanes{c->vZ, c->vZh, c->vZl, c->vscale}; }
-
-SolveArgs solve_args(gwm_hip_ctx* c, int PT, int nseg, int colB, const double* bw_pt, double bw, int kernel) {
-    SolveArgs s;
-    s.part = c->part;
-    s.nseg = nseg;
-    s.PT = PT;
-    s.colB = colB;
-    s.n = c->n;
-    s.n_pad = c->n_pad;
-    s.X = c->X;
-    s.y = c->y;
-    s.bw_pt = bw_pt;
-    s.bw = bw;
-    s.kernel = kernel;
-    s.res2 = c->res2;
-    s.sii = c->sii;
-    s.ssi = c->ssi;
-    s.betas = c->betas;
-    s.se_raw = c->se;
-    s.G = c->G;
-    s.ldg = c->ntb * 16;
-    s.gvec = c->gvec;
-    s.fail = c->fail;
-    s.cond = c->cond;
-    return s;
-}
-
-// Device memory owned by one API call.
-struct DevBuf {
-    void* p = nullptr;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() {
-        if (p) hipFree(p);
-    }
-};
-
-// Focus points of a prediction (gwm_hip_predict): their table in spatial
-// order and the device arrays the solves write.
-struct PredictSet {
-    gwm::Points Q;
-    double* cond;   // [m] conditioning indicator
-    int* rlist;     // [m] refinement list
-    double* betas;  // m x k column-major
-};
-
-// After a mixed-precision sweep and solve over all points: recompute in FP64
-// the focus points whose local design is ill conditioned (indicator above
-// limit, or not SPD), so that their results are those of the FP64 path (same
-// kernel, same data segments, so the same bits).  The failure flag is left as
-// the FP64 solves set it: every point the mixed solves failed on is in the
-// list.  *count receives the number of recomputed points; the indicator
-// maximum of the mixed pass goes to c->last_cond.  q: the focus points are
-// prediction locations rather than the data points.
-int refine_fp64(gwm_hip_ctx* c, int mode, int smode, int ntb, int colB, const double* bw_pt, double bw, int kernel,
-                double limit, int* count, const PredictSet* q = nullptr, const double* dnn = nullptr) {
-    const int nf = q ? q->Q.n : c->n;
-    double* cond = q ? q->cond : c->cond;
-    int* rlist = q ? q->rlist : c->rlist;
-    GWM_HIP_CHECK(c, launch_reduce(nullptr, nullptr, nullptr, cond, nf, c->red, c->stream));
-    GWM_HIP_CHECK(c, hipMemcpyAsync(&c->last_cond, c->red + 3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    GWM_HIP_CHECK(c, hipMemsetAsync(c->rcount, 0, sizeof(int), c->stream));
-    GWM_HIP_CHECK(c, launch_compact(cond, nf, limit, rlist, c->rcount, c->stream));
-    GWM_HIP_CHECK(c, hipMemcpyAsync(count, c->rcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    c->last_refined = *count;
-    if (*count == 0) return GWM_OK;
-    const int nta = c->nta;
-    int nseg, seg_len;
-    data_segments(c->n, nseg, seg_len);
-    const int PT = (nta + ntb) * 16;
-    const int npad = (*count + 127) / 128 * 128;
-    const size_t bytes = (size_t)nseg * PT * npad * sizeof(double);
-    if (bytes > c->rpart_bytes) {
-        cfree(c, c->rpart);
-        c->rpart_bytes = 0;
-        int rc = dalloc(c, reinterpret_cast<char**>(&c->rpart), bytes);
-        if (rc) return rc;
-        c->rpart_bytes = bytes;
-    }
-    SweepLaunch L;
-    SweepJob& j = L.job0;
-    j.P = c->points();
-    j.Z = c->Z;
-    j.Zh = nullptr;
-    j.Zl = nullptr;
-    j.colscale = nullptr;
-    j.bw_pt = bw_pt;
-    j.bw = bw;
-    j.kernel = kernel;
-    j.n = c->n;
-    j.n_pad = npad;
-    j.seg_len = seg_len;
-    j.nseg = nseg;
-    j.nta = nta;
-    j.ntb = ntb;
-    j.out = c->rpart;
-    j.focus_idx = rlist;
-    j.nfocus = *count;
-    j.dnn = dnn;
-    if (q) j.Q = q->Q;
-    if (c->prune) j.boxes = c->boxes;
-    L.jobs = nullptr;
-    L.items = nullptr;
-    L.nitems = 0;
-    timer_begin(c);
-    GWM_HIP_CHECK(c, launch_contract_f64(c->dist, mode, nta, ntb, L, c->stream));
-    int rc = timer_end(c, "refine");
-    if (rc) return rc;
-    GWM_HIP_CHECK(c, hipMemsetAsync(c->fail, 0, sizeof(int), c->stream));
-    SolveArgs s = solve_args(c, PT, nseg, colB, bw_pt, bw, kernel);
-    s.part = c->rpart;
-    s.n_pad = npad;
-    s.focus_idx = rlist;
-    s.nfocus = *count;
-    if (q) {
-        s.n = q->Q.n;
-        s.X = nullptr;
-        s.y = nullptr;
-        s.betas = q->betas;
-        s.cond = q->cond;
-    }
-    GWM_HIP_CHECK(c, launch_solve(c->k, smode, s, c->stream));
-    return GWM_OK;
-}
 
 }  // namespace
 
@@ -1021,9 +348,6 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
 }  // namespace detail
 }  // namespace gwm
 
-namespace {
-
-}  // namespace
 
 namespace gwm {
 namespace detail {
@@ -1043,9 +367,6 @@ void write_trace(const std::map<double, double>& log, double* tx, double* tf, in
 }  // namespace detail
 }  // namespace gwm
 
-namespace {
-
-}  // namespace
 
 namespace gwm {
 namespace detail {
@@ -2299,31 +1620,6 @@ int gwm_hip_set_distance_table(gwm_hip_ctx* c, double max_bytes) {
     return GWM_OK;
 }
 
-int gwm_hip_autotune_report(char* buf, int cap) {
-    if (!buf || cap <= 0) return GWM_ERR_INVALID;
-    static const char* modes[] = {"cv", "hat", "qdiag", "plain"};   // ContractMode order
-    std::string out;
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        for (const auto& kv : g_tune) {
-            const TuneKey& k = kv.first;
-            const TuneState& st = kv.second;
-            char line[256];
-            int m = std::snprintf(line, sizeof(line), "%s %s %s nta=%d ntb=%d n<2^%d chosen=%d ms=",
-                                  k.dist == kGcTable ? "great_circle_table"
-                                                     : (k.dist == GWM_GREAT_CIRCLE ? "great_circle" : "euclidean"),
-                                  k.mode >= 0 && k.mode < 4 ? modes[k.mode] : "?", k.prec ? "mixed" : "fp64", k.nta,
-                                  k.ntb, k.nlog2 + 1, st.chosen);
-            for (int q = 0; q < st.ncand && m < (int)sizeof(line) - 16; ++q)
-                m += std::snprintf(line + m, sizeof(line) - m, q ? ",%.4f" : "%.4f",
-                                   std::isfinite(st.best_ms[q]) ? st.best_ms[q] : -1.0);
-            out += line;
-            out += "\n";
-        }
-    }
-    std::snprintf(buf, cap, "%s", out.c_str());
-    return GWM_OK;
-}
 
 int gwm_hip_set_timing(gwm_hip_ctx* c, int enabled) {
     if (!c) return GWM_ERR_INVALID;
