@@ -475,8 +475,9 @@ def run_serve(capi, synth, num, rates, check):
     return out
 
 
-def load_traffic(name):
-    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % name)
+def load_profile(name):
+    """A committed counter summary, profiles/<name>.json, or None."""
+    path = os.path.join(ROOT, "profiles", "%s.json" % name)
     if os.path.exists(path):
         with open(path) as f:
             return json.load(f)
@@ -647,8 +648,8 @@ def main():
     executed_flops = 2.0 * swept_per_launch * P
     achieved = flops_per_launch / (avg_ms * 1e-3)
     avg_ms64 = ms64 / max(l64, 1)
-    traffic = load_traffic("contract_mixed_plain_n%d" % n)
-    valu = load_traffic("valu_contract_mixed_plain_n%d" % n)
+    traffic = load_profile("traffic_contract_mixed_plain_n%d" % n)
+    valu = load_profile("valu_contract_mixed_plain_n%d" % n)
     # VALU issue bound of the launch: instructions per executed pair x pairs /
     # the chip's lane-instruction rate (1024 SIMDs x 16 lanes x 2.4 GHz)
     lane_rate = 1024 * 16 * 2.4e9

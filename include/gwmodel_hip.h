@@ -240,8 +240,8 @@ int gwm_hip_set_refine_condition(gwm_hip_ctx* ctx, double criterion_limit, doubl
  * point chunks with bounding boxes, and a sweep skips every chunk whose
  * points are all farther from a workgroup's focus points than the distance
  * at which the kernel weight is exactly zero in the sweep's arithmetic
- * (compact kernels: the bandwidth; Gaussian / exponential: 12 / 72
- * bandwidths in FP64, whose weights below e^-72 = 5.4e-32 are taken as zero,
+ * (compact kernels: the bandwidth; Gaussian / exponential: 8.9 / 40
+ * bandwidths in FP64, whose weights below e^-40 = 4.2e-18 are taken as zero,
  * and FP16 underflow of the split weight in mixed precision).  Skipped
  * chunks contribute exactly zero, so results are bit-identical with pruning
  * on or off.  No counterpart in the reference, which sums over all n. */

@@ -170,16 +170,17 @@ __device__ __forceinline__ double gc_d2_near_f64(double S, double slf, double sl
 //
 // Gaussian and exponential weights whose exponent lies more than kExpCut
 // below the exponent of the focus point's largest weight are taken as zero
-// (e^-72 = 5.4e-32 of that weight).  The largest weight is the focus point's
+// (e^-40 = 4.2e-18 of that weight, below 2^-57: less than half an ulp of any
+// term the largest weight forms).  The largest weight is the focus point's
 // own (1, at d = 0) in fits and AICc sweeps; in CV sweeps, whose own weight
 // is zeroed, and at prediction locations it is the weight of the nearest
-// other point (distance dnn), so the cut sits at exponent e(dnn) - 72.  The
+// other point (distance dnn), so the cut sits at exponent e(dnn) - 40.  The
 // reference keeps every weight until exp underflows (e^-745); dropping them
-// moves every entry of X^T W X and X^T W y by at most 5.4e-32 times the
+// moves every entry of X^T W X and X^T W y by at most 4.2e-18 times the
 // largest weight times the unweighted sum of its |z_j|, and lets the sweeps
-// skip points beyond 12 bandwidths (sqrt(dnn^2 + 144 b^2) in general) instead
-// of 40.
-constexpr double kExpCut = 72.0;
+// skip points beyond 8.9 bandwidths (sqrt(dnn^2 + 80 b^2) in general) instead
+// of 38.6 (the exponent of e^-745).
+constexpr double kExpCut = 40.0;
 
 __device__ __forceinline__ double exp_cut(double e) { return e < -kExpCut ? 0.0 : exp(e); }
 
@@ -385,8 +386,8 @@ __device__ __forceinline__ float kernel_f32(int kernel, float d, float bw, float
 // moves the FP64 exponent cut outwards (make_kconst64_nn).
 __device__ __forceinline__ double zero_weight_distance(int kernel, double bw, bool mixed, double dnn = 0.0) {
     switch (kernel) {
-    case kGaussian:   // d^2 / (2 b^2) > 26.3 (mixed) / 72 + dnn^2 / (2 b^2) (FP64)
-        return mixed ? 7.25 * bw : sqrt(dnn * dnn + 144.0 * (bw * bw)) * (1.0 + 1e-9);
+    case kGaussian:   // d^2 / (2 b^2) > 26.3 (mixed) / kExpCut + dnn^2 / (2 b^2) (FP64)
+        return mixed ? 7.25 * bw : sqrt(dnn * dnn + 2.0 * kExpCut * (bw * bw)) * (1.0 + 1e-9);
     case kExponential: return mixed ? 26.0 * bw : (dnn + kExpCut * bw) * (1.0 + 1e-9);
     default: return bw * (mixed ? 1.0 + 1e-5 : 1.0 + 1e-9);
     }

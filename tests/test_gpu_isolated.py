@@ -4,7 +4,7 @@ zeroed, and prediction locations), on the GPU through the C ABI.
 Two regimes the reference handles with plain FP64 arithmetic
 (BandwidthWeight.h:68-121, GWRBasic.cpp:233-268):
 - a point 13-30 bandwidths from every other point: its Gaussian or
-  exponential weights lie below e^-72, yet they are all it has, and the
+  exponential weights lie below e^-40 (the FP64 cut), yet they are all it has, and the
   reference's criterion is finite.  The FP64 path cuts weights relative to
   the nearest point's weight, so value and status match the oracle;
 - a bandwidth below a fifth of the point spacing: every non-self weight is
@@ -64,9 +64,9 @@ def test_isolated_points_cv_match_oracle(ctx, oracle, kernel, bw):
     say("case: isolated points CV kernel=%d bw=%g" % (kernel, bw))
     coords, X, y, nn = with_outliers()
     # the outliers' nearest points lie 13-30 bandwidths away, where every
-    # weight is below e^-72 relative to a weight of 1
+    # weight is below the FP64 cut e^-40 relative to a weight of 1
     e_nn = -0.5 * (nn / bw) ** 2 if kernel == capi.GAUSSIAN else -nn / bw
-    assert np.all(e_nn < -72) and np.all(nn / bw >= 13) and np.all(nn / bw <= 100)
+    assert np.all(e_nn < -40) and np.all(nn / bw >= 13) and np.all(nn / bw <= 100)
     o = oracle.criterion(coords, X, y, kernel, False, bw, oracle.CRIT_CV)
     assert o < oracle.DBL_MAX
     ctx.set_data(coords, X, y)
@@ -151,7 +151,7 @@ def test_isolated_points_fit_and_not_spd_status(ctx, oracle):
 def test_predict_far_locations(ctx, oracle, kernel, bw):
     """Prediction locations 30-40 units from the data (GWRBasic.cpp:167-191):
     15-22 bandwidths for the Gaussian, 77-108 for the exponential, where all
-    their weights lie below e^-72.  Locations whose local design is well
+    their weights lie below e^-40, the FP64 cut.  Locations whose local design is well
     conditioned (at most 1e4): the nearer a location's few weighted points
     lie to a line, the more any two FP64 evaluation orders differ."""
     say("case: predict far locations kernel=%d" % kernel)

@@ -141,14 +141,14 @@ def test_fit_matches_oracle(ctx, oracle, kernel, adaptive, bw):
 
 @pytest.mark.parametrize("kernel,bw", [(capi.GAUSSIAN, 4.0), (capi.EXPONENTIAL, 1.0)])
 def test_exponent_cut_matches_oracle(ctx, oracle, kernel, bw):
-    """The FP64 weights take Gaussian / exponential weights below e^-72 as zero
-    (beyond 12 / 72 bandwidths), where the oracle keeps them down to exp
+    """The FP64 weights take Gaussian / exponential weights below e^-40 as zero
+    (beyond sqrt(80) = 8.9 / 40 bandwidths), where the oracle keeps them down to exp
     underflow.  At these bandwidths a large share of the pairs lies in that
     band, and the results still agree at the FP64 tolerances."""
     say("case: exponent cut kernel=%d bw=%g" % (kernel, bw))
     d = synth.uniform_square(n=1500)
     c = d["coords"]
-    cut = (12.0 if kernel == capi.GAUSSIAN else 72.0) * bw
+    cut = (np.sqrt(80.0) if kernel == capi.GAUSSIAN else 40.0) * bw
     dd = np.hypot(c[:300, None, 0] - c[None, :, 0], c[:300, None, 1] - c[None, :, 1])
     assert np.mean(dd > cut) > 0.1   # the band the cut drops is exercised
     ctx.set_data(c, d["X"], d["y"])
