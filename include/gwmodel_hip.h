@@ -287,9 +287,23 @@ int gwm_hip_autotune_report(char* buf, int cap);
  * use it. */
 int gwm_hip_set_distance_table(gwm_hip_ctx* ctx, double max_bytes);
 
+/* Neighbour table of adaptive bandwidths (no counterpart in the reference,
+ * whose BandwidthWeight::weight sorts every focus point's n distances again
+ * for each bandwidth, BandwidthWeight.cpp:40-55).  At the second adaptive
+ * select of a problem with floor(bw) < K, the context stores, per point, its
+ * K smallest selection keys in ascending order (K: the largest power of two
+ * up to 8192 below n whose n K 8 bytes fit max_bytes; 3.3 GB at n = 50,000),
+ * and resolves every later adaptive bandwidth with floor(bw) < K from it:
+ * the very order statistics the select returns, so the same thresholds.  The
+ * table stays until the data change.  It is built only when the device keeps
+ * 8 GiB free beside it; max_bytes = 0 turns it off (default 16 GiB), and a
+ * cap below a built table's size releases it. */
+int gwm_hip_set_neighbour_table(gwm_hip_ctx* ctx, double max_bytes);
+
 /* Timing of the device kernels with HIP events on the context stream
  * (enabled by gwm_hip_set_timing).  name: "contract_<fp64|mixed>_<cv|plain|
- * hat|qdiag>" for the sweeps, "select" (adaptive bandwidths), "solve",
+ * hat|qdiag>" for the sweeps, "select" (adaptive bandwidths by the select),
+ * "select_lookup" (by the neighbour table), "nbr_table" (its build), "solve",
  * "d2_table" (the distance table). */
 int gwm_hip_set_timing(gwm_hip_ctx* ctx, int enabled);
 int gwm_hip_kernel_time(gwm_hip_ctx* ctx, const char* name, long long* launches, double* total_ms);

@@ -1620,6 +1620,19 @@ int gwm_hip_set_distance_table(gwm_hip_ctx* c, double max_bytes) {
     return GWM_OK;
 }
 
+int gwm_hip_set_neighbour_table(gwm_hip_ctx* c, double max_bytes) {
+    if (!c || !(max_bytes >= 0.0)) return GWM_ERR_INVALID;
+    c->nbr_cap = max_bytes >= 1.8e19 ? SIZE_MAX : (size_t)max_bytes;
+    // tables already built stay until the data change; a lower cap drops them
+    for (auto& t : c->nbr) {
+        if (t.keys && (size_t)c->n * (size_t)t.K * sizeof(unsigned long long) > c->nbr_cap) {
+            hipFree(t.keys);
+            t = gwm_hip_ctx::NbrTable{};
+        }
+    }
+    return GWM_OK;
+}
+
 
 int gwm_hip_set_timing(gwm_hip_ctx* c, int enabled) {
     if (!c) return GWM_ERR_INVALID;

@@ -154,6 +154,16 @@ hipError_t launch_chunk_max(const double* v, int n, double* out, hipStream_t s);
 // focus (nullable): thresholds at these points instead of at P's own (predict).
 hipError_t launch_adaptive_bw(int dist, const Points& P, double bw, double* out, const double* boxes, int* left,
                               unsigned int* nleft, hipStream_t s, const Points* focus = nullptr);
+// Neighbour table: row f (K entries, K a power of two, 2 <= K <= 8192,
+// K < n) holds the K smallest selection keys of point f among all points
+// (squared distances, Euclidean; distances, great circle; as IEEE-754 bit
+// patterns) in ascending order.  The lookup resolves an adaptive bandwidth
+// with 1 <= bw and floor(bw) <= K - 1 from the table, bit-identical to
+// launch_adaptive_bw.
+hipError_t launch_neighbour_table(int dist, const Points& P, const double* boxes, int K, unsigned long long* table,
+                                  hipStream_t s);
+hipError_t launch_neighbour_lookup(int dist, const unsigned long long* table, int K, int n, double bw, double* out,
+                                   hipStream_t s);
 // Maximum pairwise distance (CRSDistance::maxDistance, CRSDistance.cpp:133-143).
 hipError_t launch_max_distance(int dist, const Points& P, double* block_max, int nblocks, hipStream_t s);
 // Distances from one focus to all points (test hook).

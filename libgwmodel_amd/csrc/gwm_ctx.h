@@ -60,6 +60,17 @@ struct gwm_hip_ctx {
     };
     std::vector<BwEntry> bw_cache;
     unsigned long long bw_clock = 0;
+    // Neighbour tables, one per distance (launch_neighbour_table): the K
+    // smallest selection keys of every point, sorted; built on the second
+    // select of a problem that they would answer, when n K 8 bytes fit
+    // nbr_cap (gwm_hip_set_neighbour_table) and the device beside 8 GiB
+    struct NbrTable {
+        unsigned long long* keys = nullptr;
+        int K = 0;
+        int misses = 0;   // selects the table would have answered before it was built
+    };
+    NbrTable nbr[2];
+    size_t nbr_cap = size_t(16) << 30;
     int* sel_left = nullptr;       // [n] points the one-wave select leaves to the workgroup select
     unsigned int* sel_nleft = nullptr;
     // [n] distance from each point to its nearest other point (the adaptive

@@ -20,6 +20,10 @@ void free_problem(gwm_hip_ctx* c) {
     }
     for (auto& e : c->bw_cache) dfree(e.buf);
     c->bw_cache.clear();
+    for (auto& t : c->nbr) {
+        if (t.keys) hipFree(t.keys);
+        t = gwm_hip_ctx::NbrTable{};
+    }
     dfree(c->bwi); dfree(c->res2); dfree(c->cond); dfree(c->sii); dfree(c->rlist); dfree(c->rcount);
     dfree(c->rpart);
     c->rpart_bytes = 0; dfree(c->ssi); dfree(c->betas); dfree(c->se);
@@ -221,6 +225,50 @@ int ensure_d2_table(gwm_hip_ctx* c) {
     return GWM_OK;
 }
 
+// Rows of the neighbour table for n points under a byte cap: the largest
+// power of two K <= 8192 with K < n and n K 8 bytes <= cap (0: none).
+static int neighbour_k(int n, size_t cap) {
+    int K = 8192;
+    while (K >= 64 && (K >= n || (size_t)n * (size_t)K * sizeof(unsigned long long) > cap)) K >>= 1;
+    return K >= 64 ? K : 0;
+}
+
+// Thresholds of an adaptive bandwidth into out: from the neighbour table of
+// the active distance when it answers bw, else by the select.  The table is
+// built at the second select it would have answered (one search is then
+// known to be under way; a single fit never pays for it).
+static int select_thresholds(gwm_hip_ctx* c, double bw, double* out) {
+    auto& t = c->nbr[c->dist];
+    const int K = t.keys ? t.K : neighbour_k(c->n, c->nbr_cap);
+    const bool fits = K > 0 && floor(bw) <= K - 1;
+    if (fits && !t.keys && ++t.misses >= 2) {
+        const size_t bytes = (size_t)c->n * (size_t)K * sizeof(unsigned long long);
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b >= bytes + (size_t(8) << 30) &&
+            hipMalloc((void**)&t.keys, bytes) == hipSuccess) {
+            t.K = K;
+            timer_begin(c);
+            GWM_HIP_CHECK(c, launch_neighbour_table(c->dist, c->points(), c->prune ? c->boxes : nullptr, K, t.keys,
+                                                    c->stream));
+            int rc = timer_end(c, "nbr_table");
+            if (rc) return rc;
+        } else {
+            (void)hipGetLastError();
+            t.keys = nullptr;
+            t.misses = -(1 << 30);   // no room: do not try again for this problem
+        }
+    }
+    if (fits && t.keys) {
+        timer_begin(c);
+        GWM_HIP_CHECK(c, launch_neighbour_lookup(c->dist, t.keys, t.K, c->n, bw, out, c->stream));
+        return timer_end(c, "select_lookup");
+    }
+    timer_begin(c);
+    GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, out, c->prune ? c->boxes : nullptr, c->sel_left,
+                                        c->sel_nleft, c->stream));
+    return timer_end(c, "select");
+}
+
 // Resolve the bandwidth: adaptive -> per-focus thresholds (a vector of the
 // bandwidth cache, filled by the select on a miss).
 int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw_pt) {
@@ -254,10 +302,7 @@ int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw
     }
     slot->bw = NAN;   // not valid until the select succeeds
     slot->dist = c->dist;
-    timer_begin(c);
-    GWM_HIP_CHECK(c, launch_adaptive_bw(c->dist, c->points(), bw, slot->buf, c->prune ? c->boxes : nullptr,
-                                        c->sel_left, c->sel_nleft, c->stream));
-    int rc = timer_end(c, "select");
+    int rc = select_thresholds(c, bw, slot->buf);
     if (rc) return rc;
     slot->bw = bw;
     slot->used = ++c->bw_clock;

@@ -255,9 +255,22 @@ __device__ __forceinline__ void resolve_digit(const unsigned int* hist, int nb, 
     __syncthreads();
 }
 
-template <int DIST, class PTS, class OUT>
-__device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, double bw, OUT out, int f,
-                                                GWM_GLOBAL const double* boxes) {
+// The interpolated threshold from the floor(bw)-th and floor(bw)+1-th
+// smallest selection keys (bx = bw - floor(bw)); every path that resolves an
+// adaptive bandwidth evaluates it through this one expression.
+template <int DIST>
+__device__ __forceinline__ double interp_threshold(unsigned long long key0, unsigned long long key1, double bx) {
+    const double d0 = select_to_distance<DIST>(val_of(key0));
+    const double d1 = select_to_distance<DIST>(val_of(key1));
+    return d0 + (d1 - d0) * bx;
+}
+
+// Workgroup select of the keys at 0-based ranks floor(bw) - 1 and floor(bw)
+// among the n selection keys of focus point f (1 <= bw < n); one thread
+// calls emit(key0, key1).  All threads return together.
+template <int DIST, class PTS, class EMIT>
+__device__ __forceinline__ void select_keys(const PTS& F, const PTS& P, double bw, int f,
+                                            GWM_GLOBAL const double* boxes, EMIT emit) {
     const int n = P.n;
     const int tid = threadIdx.x;
     constexpr int BD = DIST == kEuclidean ? 4 : 6;   // doubles per chunk box
@@ -267,24 +280,9 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, doub
     __shared__ long long s_rank;
     __shared__ unsigned int s_count, s_ncand, s_nclist, s_below;
     __shared__ double s_U, s_L;
-    __shared__ double s_red[256];
     __shared__ int clist[kSelChunks];
 
-    const double dn = bw / n;
-    if (!(dn < 1)) {
-        // (bw/n) * max(dist); distances are >= 0 and include the focus (0)
-        double m = 0.0;
-        for (int j = tid; j < n; j += 256) m = fmax(m, distance_f64<DIST>(F, f, P, j));
-        s_red[tid] = m;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (tid < w) s_red[tid] = fmax(s_red[tid], s_red[tid + w]);
-            __syncthreads();
-        }
-        if (tid == 0) out[f] = dn * s_red[0];
-        return;
-    }
-    const double b0 = floor(bw), bx = bw - b0;
+    const double b0 = floor(bw);
     // pruning: candidate chunks (see the header)
     const int nch = (n + 63) / 64, nfull = n / 64;
     const long long need = (long long)b0 + 1;   // keys up to rank floor(bw) are needed
@@ -400,11 +398,7 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, doub
         if (tid < 64) {
             unsigned long long key0, key1;
             wave_rank_select(cand, ncand, rank0, s_above, hist, &key0, &key1);
-            if (tid == 0) {
-                const double d0 = select_to_distance<DIST>(val_of(key0));
-                const double d1 = select_to_distance<DIST>(val_of(key1));
-                out[f] = d0 + (d1 - d0) * bx;
-            }
+            if (tid == 0) emit(key0, key1);
         }
         return;
     }
@@ -458,9 +452,9 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, doub
     // all 64 bits resolved: key0 = s_prefix, s_count keys equal it
     const unsigned long long key0 = s_prefix;
     const bool tie = (s_rank + 1) < (long long)s_count;
-    double d1v;
+    unsigned long long key1;
     if (tie) {
-        d1v = val_of(key0);
+        key1 = key0;
     } else {
         // second order statistic: the smallest key above key0
         if (tid == 0) s_next = in_lds ? s_above : ~0ull;
@@ -478,13 +472,36 @@ __device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, doub
         }
         atomicMin(&s_next, best);
         __syncthreads();
-        d1v = val_of(s_next);
+        key1 = s_next;
     }
-    if (tid == 0) {
-        const double d0 = select_to_distance<DIST>(val_of(key0));
-        const double d1 = select_to_distance<DIST>(d1v);
-        out[f] = d0 + (d1 - d0) * bx;
+    if (tid == 0) emit(key0, key1);
+}
+
+template <int DIST, class PTS, class OUT>
+__device__ __forceinline__ void adaptive_bw_one(const PTS& F, const PTS& P, double bw, OUT out, int f,
+                                                GWM_GLOBAL const double* boxes) {
+    const int n = P.n;
+    const int tid = threadIdx.x;
+    const double dn = bw / n;
+    if (!(dn < 1)) {
+        // (bw/n) * max(dist); distances are >= 0 and include the focus (0)
+        __shared__ double s_red[256];
+        double m = 0.0;
+        for (int j = tid; j < n; j += 256) m = fmax(m, distance_f64<DIST>(F, f, P, j));
+        s_red[tid] = m;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) s_red[tid] = fmax(s_red[tid], s_red[tid + w]);
+            __syncthreads();
+        }
+        if (tid == 0) out[f] = dn * s_red[0];
+        return;
     }
+    const double bx = bw - floor(bw);
+    select_keys<DIST>(F, P, bw, f, boxes,
+                      [&](unsigned long long key0, unsigned long long key1) {
+                          out[f] = interp_threshold<DIST>(key0, key1, bx);
+                      });
 }
 
 // ---------------------------------------------------------------- one wave
@@ -550,11 +567,7 @@ __device__ __forceinline__ bool adaptive_bw_wave(const PTS& F, const PTS& P, dou
     wave_sync();
     unsigned long long key0, key1;
     wave_rank_select(keys, nk, (long long)b0 - 1 - below, ~0ull, hist, &key0, &key1);
-    if (lane == 0) {
-        const double d0 = select_to_distance<DIST>(val_of(key0));
-        const double d1 = select_to_distance<DIST>(val_of(key1));
-        out[f] = d0 + (d1 - d0) * bx;
-    }
+    if (lane == 0) out[f] = interp_threshold<DIST>(key0, key1, bx);
     return true;
 }
 
@@ -625,6 +638,96 @@ __global__ __launch_bounds__(256) void adaptive_bw_batch_kernel(const SelectJob*
 }
 
 constexpr int kFallbackGrid = 2048;
+
+// ---------------------------------------------------------------- neighbour table
+//
+// A golden-section search over adaptive bandwidths selects 20-40 neighbour
+// counts on the same points (a multiscale backfit some hundreds), and each
+// select streams the candidate keys of every focus point again.  The
+// neighbour table keeps, per point, its K smallest selection keys in
+// ascending order (K a power of two, at most kNbrMaxK): a select whose
+// floor(bw) <= K - 1 then reads ranks floor(bw) - 1 and floor(bw) of the row.
+// Row f is built by one workgroup: T = the key at 0-based rank K - 1 (the
+// exact workgroup select), the keys below T gathered into LDS (fewer than K
+// of them; the remaining slots hold T, the value of every rank up to K - 1
+// that they do not fill), then a bitonic sort of the K slots.  The table holds
+// the very keys the select returns, so lookups are bit-identical to it.
+constexpr int kNbrMaxK = 8192;
+
+// Ascending bitonic sort of a[0..K) in LDS (K a power of two), 256 threads.
+__device__ __forceinline__ void lds_bitonic_sort(unsigned long long* a, int K) {
+    for (int size = 2; size <= K; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < K / 2; t += 256) {
+                const int lo = 2 * t - (t & (stride - 1));   // bit `stride` of lo is clear
+                const int hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const unsigned long long x = a[lo], y = a[hi];
+                if ((x > y) == up) {
+                    a[lo] = y;
+                    a[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int DIST>
+__global__ __launch_bounds__(256) void neighbour_table_kernel(Points Pt, const double* boxes_, int K,
+                                                              unsigned long long* table) {
+    __shared__ unsigned long long skeys[kNbrMaxK];
+    __shared__ unsigned long long s_T;
+    __shared__ unsigned int s_cnt;
+    constexpr int BD = DIST == kEuclidean ? 4 : 6;
+    const GPoints P = global_points(Pt);
+    GWM_GLOBAL const double* boxes = gptr(boxes_);
+    const int n = P.n, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nch = (n + 63) / 64;
+    for (int f = blockIdx.x; f < n; f += gridDim.x) {
+        select_keys<DIST>(P, P, (double)K, f, boxes, [&](unsigned long long key0, unsigned long long) { s_T = key0; });
+        if (tid == 0) s_cnt = 0u;
+        __syncthreads();
+        const unsigned long long T = s_T;
+        const double Tv = val_of(T);
+        auto take = [&](int j) {
+            const unsigned long long key = key_of(select_value<DIST>(P, f, P, j));
+            if (key < T) {
+                const unsigned int pos = atomicAdd(&s_cnt, 1u);
+                if (pos < (unsigned int)K) skeys[pos] = key;
+            }
+        };
+        if (boxes) {
+            // one chunk per wave at a time; a chunk whose lower bound exceeds T holds no key below it
+            for (int c = wave; c < nch; c += 4) {
+                double lb, ub;
+                key_bounds<DIST>(P, f, boxes + (size_t)c * BD, &lb, &ub);
+                const int j = c * 64 + lane;
+                if (lb <= Tv && j < n) take(j);
+            }
+        } else {
+            for (int j = tid; j < n; j += 256) take(j);
+        }
+        __syncthreads();
+        const int cnt = min((int)s_cnt, K);
+        for (int i = cnt + tid; i < K; i += 256) skeys[i] = T;
+        __syncthreads();
+        lds_bitonic_sort(skeys, K);
+        unsigned long long* row = table + (size_t)f * K;
+        for (int i = tid; i < K; i += 256) row[i] = skeys[i];
+        __syncthreads();
+    }
+}
+
+template <int DIST>
+__global__ __launch_bounds__(256) void neighbour_lookup_kernel(const unsigned long long* table, int K, int n,
+                                                               double bw, double* out) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    if (f >= n) return;
+    const int r = (int)floor(bw);
+    const unsigned long long* row = table + (size_t)f * K;
+    out[f] = interp_threshold<DIST>(row[r - 1], row[r], bw - floor(bw));
+}
 
 // Largest distance from the focus points of workgroup column blockIdx.x to
 // the data slice blockIdx.y (slice_len points); slices combine by an atomic
@@ -703,6 +806,28 @@ hipError_t launch_adaptive_bw_batch(int dist, const SelectJob* jobs, const int* 
         hipLaunchKernelGGL(adaptive_bw_batch_kernel<kGreatCircle>, fgrid, dim3(256), 0, s, jobs, start, njobs, left,
                            nleft);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_neighbour_table(int dist, const Points& P, const double* boxes, int K, unsigned long long* table,
+                                  hipStream_t s) {
+    if (K < 2 || K > kNbrMaxK || (K & (K - 1)) || K >= P.n) return hipErrorInvalidValue;
+    const dim3 grid(std::min(P.n, kFallbackGrid));
+    if (dist == kEuclidean)
+        hipLaunchKernelGGL(neighbour_table_kernel<kEuclidean>, grid, dim3(256), 0, s, P, boxes, K, table);
+    else
+        hipLaunchKernelGGL(neighbour_table_kernel<kGreatCircle>, grid, dim3(256), 0, s, P, boxes, K, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_neighbour_lookup(int dist, const unsigned long long* table, int K, int n, double bw, double* out,
+                                   hipStream_t s) {
+    if (!(bw >= 1.0) || !(floor(bw) <= K - 1) || K >= n) return hipErrorInvalidValue;
+    const dim3 grid((n + 255) / 256);
+    if (dist == kEuclidean)
+        hipLaunchKernelGGL(neighbour_lookup_kernel<kEuclidean>, grid, dim3(256), 0, s, table, K, n, bw, out);
+    else
+        hipLaunchKernelGGL(neighbour_lookup_kernel<kGreatCircle>, grid, dim3(256), 0, s, table, K, n, bw, out);
     return hipGetLastError();
 }
 
