@@ -313,11 +313,18 @@ def run_mgwr(capi, synth, n):
                      [0] + [1] * (k - 1), [1e-5] * k, crit_type=0, crit_threshold=1e-5, precision=capi.MIXED)
         ctx.set_timing(False)
         sel, sel_ms = ctx.kernel_time("select")
+        look, look_ms = ctx.kernel_time("select_lookup")
+        tab, tab_ms = ctx.kernel_time("nbr_table")
     # one select finds the thresholds of n focus points among n data points:
-    # n^2 element visits in the reference's sort-based algorithm
+    # n^2 element visits in the reference's sort-based algorithm.  Bandwidths
+    # below the neighbour table's K are read from it instead (one build).
     out["selects"] = sel
     out["select_ms"] = sel_ms
     out["select_elements_per_s"] = sel * float(n) * n / (sel_ms * 1e-3) if sel_ms > 0 else None
+    out["select_lookups"] = look
+    out["select_lookup_ms"] = look_ms
+    out["neighbour_table_builds"] = tab
+    out["neighbour_table_ms"] = tab_ms
     out["bandwidths"] = [float(b) for b in res["mixed"]["bandwidths"]]
     out["bandwidths_identical"] = bool(np.array_equal(res["mixed"]["bandwidths"], res["fp64"]["bandwidths"]))
     out["beta_rel_err_vs_fp64"] = beta_rel_err(res["mixed"]["betas"], res["fp64"]["betas"])

@@ -193,6 +193,11 @@ int gwm_hip_gwr_run(gwm_hip_ctx* ctx, int kernel, int adaptive, double bw, int s
 int gwm_hip_max_distance(gwm_hip_ctx* ctx, double* out);
 /* CRSDistance::distance(focus) (CRSDistance.cpp:122-131): n distances. */
 int gwm_hip_distance_row(gwm_hip_ctx* ctx, int focus, double* out);
+/* The squared-distance boundary of the FP64 bisquare and boxcar weights (test
+ * hook, no device needed): the smallest double x whose correctly rounded
+ * square root is >= bw, so that d^2 < x exactly when the reference's test
+ * d < bw holds (BandwidthWeight.h:68-121); 0 for bw <= 0. */
+int gwm_hip_sqrt_boundary(double bw, double* out);
 /* Adaptive bandwidth resolved per focus point (BandwidthWeight.cpp:40-55): n values. */
 int gwm_hip_adaptive_bandwidths(gwm_hip_ctx* ctx, double bw, double* out);
 

@@ -63,6 +63,7 @@ SIGNATURES = {
     "gwm_hip_max_distance": (_i, [_vp, _dp]),
     "gwm_hip_distance_row": (_i, [_vp, _i, _dp]),
     "gwm_hip_adaptive_bandwidths": (_i, [_vp, _d, _dp]),
+    "gwm_hip_sqrt_boundary": (_i, [_d, _dp]),
     "gwm_hip_var_criterion": (_i, [_vp, _dp, _dp, _i, _i, _d, _i, _i, _dp]),
     "gwm_hip_var_fit": (_i, [_vp, _dp, _dp, _i, _i, _d, _i, _dp]),
     "gwm_hip_mgwr_run": (_i, [_vp, _ip, _ip, _dp, _ip, _ip, _ip, _dp, _i, _i, _d, _i, _i, _i, _i, _d, _i, _d, _i,

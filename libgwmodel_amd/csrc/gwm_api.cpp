@@ -1207,6 +1207,12 @@ int gwm_hip_distance_row(gwm_hip_ctx* c, int focus, double* out) {
     return download_caller(c, c->res2, out, 1);
 }
 
+int gwm_hip_sqrt_boundary(double bw, double* out) {
+    if (!out || std::isnan(bw)) return GWM_ERR_INVALID;
+    *out = sqrt_boundary(bw);
+    return GWM_OK;
+}
+
 int gwm_hip_adaptive_bandwidths(gwm_hip_ctx* c, double bw, double* out) {
     int rc = check_ready(c);
     if (rc) return rc;

@@ -72,9 +72,11 @@ __device__ __forceinline__ FocusCoords<DIST> load_focus(const GPoints& P, int f)
     return fc;
 }
 
-// Kernel weight of the pair (focus fc, staged point r).  The Gaussian weighs
-// the squared distance directly (no square root; d * d of the reference
-// differs from it by an ulp at most).  Great circle: near pairs take
+// Kernel weight of the pair (focus fc, staged point r).  The Gaussian and
+// the bisquare weigh the squared distance directly (no square root; d * d of
+// the reference differs from it by an ulp at most), and the bisquare and
+// boxcar decide the reference's d < bw on it exactly (KConst64::d2_lt).
+// Great circle: near pairs take
 // gc_d2_near_f64, the others (far, coincident, near-coincident) gcdist_f64,
 // in a branch that costs a wave only when one of its lanes needs it.
 template <int DIST, class SH>
@@ -98,6 +100,12 @@ __device__ __forceinline__ double pair_weight(int kernel, const FocusCoords<DIST
     if (kernel == kGaussian) {
         const double e = d2 * k.gauss;
         return e < k.cut ? 0.0 : exp(e);
+    }
+    if (kernel == kBisquare || kernel == kBoxcar) {
+        if (!(d2 < k.d2_lt)) return 0.0;
+        if (kernel == kBoxcar) return 1.0;
+        const double t = 1.0 - d2 * k.inv_bw2;
+        return t * t;
     }
     return kernel_fast_f64(kernel, d >= 0.0 ? d : sqrt(d2), k);
 }

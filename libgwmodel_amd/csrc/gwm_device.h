@@ -212,17 +212,56 @@ __device__ __forceinline__ double kernel_f64(int kernel, double d, double bw) {
     }
 }
 
+// Smallest double x whose correctly rounded square root is >= bw (0 when bw
+// is not positive or NaN).  RN(sqrt(.)) is non-decreasing over the doubles,
+// so for every double d2 >= 0: d2 < x exactly when RN(sqrt(d2)) < bw -- the
+// reference's boundary test d < bw on the distance d = sqrt(d2), decided on
+// the squared distance without the square root.  bw^2 lies within a few ulps
+// of x when it is a normal number; a bandwidth below 2^-500 is scaled by
+// 2^1000 first (squares by 2^2000: exact for every x up to the boundary, and
+// RN(sqrt(x s^2)) = s RN(sqrt(x)) since both roots are normal numbers), and
+// the scaled boundary is mapped back rounding up.
+// the double `steps` representable values above a non-negative x (below for steps < 0)
+__host__ __device__ inline double ulp_step(double x, long long steps) {
+    return __builtin_bit_cast(double, __builtin_bit_cast(long long, x) + steps);
+}
+
+__host__ __device__ inline double sqrt_boundary_scaled(double bw) {
+    double x = bw * bw;   // bw >= 2^-500: normal, or inf for bw > 2^512
+    if (sqrt(x) >= bw) {
+        for (int i = 0; i < 64; ++i) {
+            const double p = ulp_step(x, -1);   // x > 0 here
+            if (!(sqrt(p) >= bw)) break;
+            x = p;
+        }
+    } else {
+        for (int i = 0; i < 64 && !(sqrt(x) >= bw); ++i) x = ulp_step(x, 1);
+    }
+    return x;
+}
+
+__host__ __device__ inline double sqrt_boundary(double bw) {
+    if (!(bw > 0.0)) return 0.0;
+    if (bw >= 0x1p-500) return sqrt_boundary_scaled(bw);
+    const double t = sqrt_boundary_scaled(bw * 0x1p1000);   // t < 2^1000
+    double q = (t * 0x1p-1000) * 0x1p-1000;                // rounds only in the last step
+    if ((q * 0x1p1000) * 0x1p1000 < t) q = ulp_step(q, 1);
+    return q;
+}
+
 // Per-bandwidth constants so that the sweep weighs pairs without divisions
 // (multiplication by the reciprocal: within an ulp of the reference's quotient;
 // the boundary tests d < bw are unchanged).
 struct KConst64 {
     double bw, gauss, inv_bw, inv_bw2, inv_bw3;   // gauss = 1 / (-2 bw^2)
-    double cut;   // exponents below cut give zero weights
+    double cut;    // exponents below cut give zero weights
+    double d2_lt;  // sqrt_boundary(bw): d < bw exactly when d^2 < d2_lt
 };
 
 __device__ __forceinline__ KConst64 make_kconst64(double bw) {
     KConst64 k;
     k.bw = bw;
+    k.d2_lt = sqrt_boundary(bw);
     k.gauss = 1.0 / ((-2.0) * (bw * bw));
     k.inv_bw = 1.0 / bw;
     k.inv_bw2 = 1.0 / (bw * bw);

@@ -8,6 +8,7 @@
 //   S_ii     = w_ii x_i^T A^-1 x_i
 //   sum_j S_ij^2 = g^T B g,    g = A^-1 x_i, B = X^T W_i^2 X
 //   SE_raw_a = (A^-1 B A^-1)_aa = rowsum(C_i .^ 2)_a
+// (both quadratic forms in B as the squared norms |L_B^T v|^2, B = L_B L_B^T)
 // A failed Cholesky pivot (not > 0, or NaN) is the inv_sympd failure: the
 // criterion becomes DBL_MAX and the fit throws (GWRBasic.cpp:222-226, 254-258).
 #include "gwm_contract.h"
@@ -85,6 +86,23 @@ __device__ __forceinline__ double quad_form(const double* Bp, const double* g) {
         s += Bp[Sym<K>::low(a, a)] * g[a] * g[a];
 #pragma unroll
         for (int b = 0; b < a; ++b) s += 2.0 * Bp[Sym<K>::low(a, b)] * g[a] * g[b];
+    }
+    return s;
+}
+
+// v^T B v from the Cholesky factor of B (packed lower, B = L L^T): the sum of
+// the squares of L^T v, never negative, as the reference's rowsum(C .^ 2) is
+// not; the direct form (quad_form) can round below zero when the local
+// design is nearly singular.
+template <int K>
+__device__ __forceinline__ double gram_form(const double* L, const double* v) {
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+        double u = 0.0;
+#pragma unroll
+        for (int b = a; b < K; ++b) u += L[Sym<K>::low(b, a)] * v[b];
+        s += u * u;
     }
     return s;
 }
@@ -195,13 +213,21 @@ __device__ __forceinline__ void solve_one(const SolveArgs& a, int t) {
     a.sii[i] = wii * xg;
     if (MODE == kSolveAIC) return;
 
-    // B = X^T W^2 X from the w^2 block; reuse A's storage
+    // B = X^T W^2 X from the w^2 block, factored in A's storage (B is positive
+    // semidefinite: the forms below are sums of squares).  A factor that fails
+    // (B singular to rounding) leaves the direct form, clamped at zero.
     double* B = A;
 #pragma unroll
     for (int r = 0; r < K; ++r)
 #pragma unroll
         for (int c = 0; c <= r; ++c) B[Sym<K>::low(r, c)] = sum_parts<K>(a, a.colB + Sym<K>::zcol(c, r), t);
-    a.ssi[i] = quad_form<K>(B, g);
+    const bool fb = cholesky<K>(B, B);
+    if (!fb)
+#pragma unroll
+        for (int r = 0; r < K; ++r)
+#pragma unroll
+            for (int c = 0; c <= r; ++c) B[Sym<K>::low(r, c)] = sum_parts<K>(a, a.colB + Sym<K>::zcol(c, r), t);
+    a.ssi[i] = fb ? gram_form<K>(B, g) : fmax(quad_form<K>(B, g), 0.0);
 #pragma unroll
     for (int p = 0; p < K; ++p) a.betas[i + (size_t)p * n] = beta[p];
 #pragma unroll
@@ -210,7 +236,7 @@ __device__ __forceinline__ void solve_one(const SolveArgs& a, int t) {
 #pragma unroll
         for (int p = 0; p < K; ++p) e[p] = p == c ? 1.0 : 0.0;
         chol_solve<K>(L, e, col);
-        a.se_raw[i + (size_t)c * n] = quad_form<K>(B, col);
+        a.se_raw[i + (size_t)c * n] = fb ? gram_form<K>(B, col) : fmax(quad_form<K>(B, col), 0.0);
     }
 #pragma unroll
     for (int p = 0; p < K; ++p) a.gvec[(size_t)i * K + p] = g[p];

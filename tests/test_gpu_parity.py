@@ -111,6 +111,7 @@ CASES = [
     (capi.TRICUBE, True, 80.5),
     (capi.EXPONENTIAL, False, 10.0),
     (capi.BOXCAR, False, 35.0),
+    (capi.BOXCAR, True, 40.0),
 ]
 
 
@@ -137,6 +138,35 @@ def test_fit_matches_oracle(ctx, oracle, kernel, adaptive, bw):
     assert close(g["shat"], o["shat"], 1e-10)
     assert close(g["qDiag"], o["qDiag"], 1e-9, 1e-12)
     assert close(g["S"], o["S"], 1e-9, 1e-12)
+
+
+@pytest.mark.parametrize("kernel,adaptive,bw", [(capi.BOXCAR, False, 5.0), (capi.BOXCAR, False, 10.0),
+                                               (capi.BOXCAR, True, 21.0), (capi.BISQUARE, False, 5.0),
+                                               (capi.BISQUARE, True, 13.0)])
+def test_boundary_ties_match_oracle(ctx, oracle, kernel, adaptive, bw):
+    """Points on an integer grid: many pairs lie exactly at the bandwidth
+    (3-4-5 and 6-8-10 triangles for fixed 5 and 10; the floor(bw)-th
+    neighbour and its ties for adaptive integer bandwidths).  The reference
+    gives them zero weight (d < bw, BandwidthWeight.h:68-121); the FP64 sweeps
+    decide the test on the squared distance, so every such pair must fall
+    out here too: a boxcar weight of 1 instead of 0 moves the criterion far
+    beyond the tolerance."""
+    say("case: boundary ties kernel=%d adaptive=%d bw=%g" % (kernel, adaptive, bw))
+    rng = np.random.default_rng(31)
+    gx, gy = np.meshgrid(np.arange(24.0), np.arange(20.0))
+    coords = np.column_stack([gx.ravel(), gy.ravel()])
+    n = coords.shape[0]
+    X = np.column_stack([np.ones(n), rng.normal(size=n), rng.normal(size=n)])
+    y = X @ np.array([1.0, 0.5, -0.25]) + np.sin(coords[:, 0] / 5.0) + rng.normal(0, 0.2, n)
+    ctx.set_data(coords, X, y)
+    for crit in [capi.CRIT_CV, capi.CRIT_AIC]:
+        g = ctx.criterion(kernel, adaptive, bw, crit)
+        o = oracle.criterion(coords, X, y, kernel, adaptive, bw, crit)
+        assert g == pytest.approx(o, rel=1e-11), (crit, g, o)
+    g = ctx.fit(kernel, adaptive, bw)
+    o = oracle.fit_core(coords, X, y, kernel, adaptive, bw)
+    assert close(g["betas"], o["betas"], 1e-9, 1e-12)
+    assert close(g["shat"], o["shat"], 1e-10)
 
 
 @pytest.mark.parametrize("kernel,bw", [(capi.GAUSSIAN, 4.0), (capi.EXPONENTIAL, 1.0)])
