@@ -201,8 +201,9 @@ int criterion_impl(gwm_hip_ctx* c, int kernel, int adaptive, double bw, int crit
 }
 
 // Golden-section search, BandwidthSelector::optimize (BandwidthSelector.cpp:6-81).
-// f(x, exact, &v) returns a status and the criterion (DBL_MAX when invalid);
-// exact = true asks for the FP64 evaluation.
+// f(x, exact, &v, &b) returns a status and the criterion (DBL_MAX when
+// invalid); exact = true asks for the FP64 evaluation.  A negative bound b
+// reports that v was evaluated in FP64 although exact was not asked for.
 //
 // With certify set (mixed-precision sweeps), every decision the search takes
 // on criterion values -- the branch f1 < f2, the stopping test |f2 - f1| > eps
@@ -239,6 +240,10 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
         if (rc == GWM_ERR_NOT_SPD) {
             *fx = DBL_MAX;
             rc = GWM_OK;
+        }
+        if (*fb < 0.0) {
+            exact = true;
+            *fb = 0.0;
         }
         if (rc == GWM_OK && !exact && *fx == DBL_MAX) {
             exact = true;
@@ -534,6 +539,15 @@ int upload_design(gwm_hip_ctx* c, const double* X_caller, const double* y_caller
 // The column x and response y of a single-column step on the device, in
 // sorted order, with their product table (FP64 and, for mixed precision, its
 // FP16 planes).  A golden search over one variable prepares them once.
+// The precision a single-column sweep of this kernel runs in.  Bisquare and
+// boxcar sweeps run in FP64 in both precisions: without a square root or an
+// exponential per pair the FP64 VALU sweep (contract_var_f64) streams pairs
+// as fast as the FP16-split MFMA sweep, whose one useful column leaves the
+// matrix cores idle, and its criteria need no certification.
+static int var_precision(int kernel, int precision) {
+    return (kernel == GWM_BISQUARE || kernel == GWM_BOXCAR) ? GWM_FP64 : precision;
+}
+
 int var_prepare(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int precision) {
     const int n = c->n;
     std::vector<double> xs(n), ys(n);
@@ -559,6 +573,8 @@ int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int
     if (value) *value = DBL_MAX;
     if (stopped(c)) return GWM_ERR_CANCELLED;
     const int n = c->n;
+    const bool promoted = precision == GWM_MIXED && var_precision(kernel, precision) == GWM_FP64;
+    precision = var_precision(kernel, precision);
     if (x_caller && (rc = var_prepare(c, x_caller, y_caller, precision))) return rc;
     const double* bw_pt;
     if ((rc = resolve_bandwidth(c, adaptive, bw, &bw_pt))) return rc;
@@ -606,7 +622,8 @@ int var_impl(gwm_hip_ctx* c, const double* x_caller, const double* y_caller, int
         // so the bound is tol_rel |f| and, for AICc, the trace term
         c->last_bound =
             precision == GWM_MIXED ? crit_error_bound(crit, v, red[0], red[1], 0.0, n, 1.0, 1.0, c->certify_rel) : 0.0;
-        if (bound) *bound = c->last_bound;
+        // an FP64 value of a mixed-precision request: reported by a negative bound
+        if (bound) *bound = promoted ? -1.0 : c->last_bound;
         return GWM_OK;
     }
     GWM_HIP_CHECK(c, hipMemcpyAsync(&fail, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1306,7 +1323,8 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         double lo = has_lower ? lower : (ad ? (double)adaptive_lower : maxD[i] / 5000.0);
         double hi = has_upper ? upper : (ad ? (double)n : maxD[i]);
         tele_print(c, kLogInfo, kInitialBw + std::to_string(i));
-        if ((rc = var_prepare(c, col(i), y.data(), precision)) || (rc = search_var(i, lo, hi, &bw[i])))
+        if ((rc = var_prepare(c, col(i), y.data(), var_precision(kernels[i], precision))) ||
+            (rc = search_var(i, lo, hi, &bw[i])))
             return fail_out(rc);
         tele_print(c, kLogInfo, kInitialBw + std::to_string(i) + "," + std::to_string(bw[i]));
     }
@@ -1368,7 +1386,8 @@ int gwm_hip_mgwr_run(gwm_hip_ctx* c, const int* kernels, const int* adaptive, co
         for (int i = 0; i < k; ++i) {
             const double* xi = col(i);
             for (int j = 0; j < n; ++j) yi[j] = resid[j] + betas[j + (size_t)i * n] * xi[j];
-            if ((rc = use_distance(c, dist_of(i))) || (rc = var_prepare(c, xi, yi.data(), precision)))
+            if ((rc = use_distance(c, dist_of(i))) ||
+                (rc = var_prepare(c, xi, yi.data(), var_precision(kernels[i], precision))))
                 return fail_out(rc);
             if (init[i] != 2) {
                 tele_print(c, kLogInfo, kBackfit + "#variable-bandwidth-selection " + std::to_string(i));

@@ -376,28 +376,52 @@ namespace {
 
 // Single-column sweeps of the multiscale path (GWRMultiscale.cpp:420-482,
 // 560-636): per focus point the two sums sum_j w_ij x_j^2 and sum_j w_ij x_j
-// y_j, in FP64 on the VALU.  One lane per focus point (256 per workgroup);
-// the chunk's coordinates and its two table columns are staged in LDS and
-// read by every lane at the same address.  Same weights, pruning and data
-// segments as contract_f64_kernel, whose 16-column matrix-core form would
-// compute eight times the columns these sweeps use.
+// y_j, in FP64 on the VALU.  Four lanes per focus point, 16 focus points per
+// wave: lane group q = lane / 16 weighs the points q, q + 4, q + 8, ... of
+// each chunk, and the four partial sums combine in a fixed order at the end.
+// One lane per focus point would leave the chip short of waves at multiscale
+// sizes (n = 50,000: 782 waves for 1,024 SIMDs, each stalled on its own FP64
+// dependences); four give 3,125.  The waves are independent: each tests its
+// own 16-point focus box against the boxes of the 16-chunk groups and of the
+// chunks, stages its live chunks in its own LDS slice (wave-level syncs, no
+// workgroup barrier), and loads the next live chunk into registers while it
+// weighs the current one.  Same weights, pruning and data segments as
+// contract_f64_kernel, whose 16-column matrix-core form would compute eight
+// times the columns these sweeps use.
 template <int DIST>
 struct VarShared {
     CoordStage<DIST> c;
     double z0[kChunk], z1[kChunk];
 };
 
+// one staged point per lane: its coordinates and its two table entries
+template <int DIST>
+struct VarPoint {
+    double c[DIST == kEuclidean ? 2 : 6];
+    double z0, z1;
+};
+
+constexpr int kVarFocus = 64;   // focus points per workgroup (16 per wave)
+
+__device__ __forceinline__ void var_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int DIST, int MODE, int KER>
 __global__ __launch_bounds__(256) void contract_var_f64_kernel(SweepJob job) {
     static_assert(MODE == kModeCV || MODE == kModePlain, "single-column sweeps weigh with w");
-    __shared__ VarShared<DIST> sh;
+    __shared__ VarShared<DIST> shw[4];
     const int seg = blockIdx.x % job.nseg;
     const int tile = blockIdx.x / job.nseg;
     const GPoints P = global_points(job.P);
     GWM_GLOBAL const double* bw_pt = gptr(job.bw_pt);
     GWM_GLOBAL const double* dnn = gptr(job.dnn);
     const int n = job.n;
-    const int f = tile * 256 + (int)threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4;
+    VarShared<DIST>& sh = shw[wave];
+    const int f = tile * kVarFocus + wave * 16 + (lane & 15);
     const int fl = f < n ? f : n - 1;
     const FocusCoords<DIST> fc = load_focus<DIST>(P, fl);
     const double b = bw_pt ? bw_pt[fl] : job.bw;
@@ -409,18 +433,18 @@ __global__ __launch_bounds__(256) void contract_var_f64_kernel(SweepJob job) {
     } else {
         kc = make_kconst64(b);
     }
-    // pruning at two levels: the workgroup's box decides which chunks are
-    // staged, each wave's own (64 focus points) which of them it weighs
+    // the wave's box: its 16 focus points sit in lanes 0-15 and again in each
+    // group of 16, so a reduction over the low four lane bits covers them
     constexpr int BD = DIST == kEuclidean ? 2 : 3;
     const bool prune = job.boxes != nullptr;
-    double flo[BD], fhi[BD], wlo[BD], whi[BD], thr = INFINITY, wthr = INFINITY;
+    double wlo[BD], whi[BD], wthr = INFINITY;
     if (prune) {
         const double x[3] = {DIST == kEuclidean ? fc.a : fc.hx, DIST == kEuclidean ? fc.b : fc.hy, fc.hz};
         double wb = b, wdn = fdn;
 #pragma unroll
         for (int d = 0; d < BD; ++d) wlo[d] = whi[d] = x[d];
 #pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) {
+        for (int m = 8; m >= 1; m >>= 1) {
 #pragma unroll
             for (int d = 0; d < BD; ++d) {
                 wlo[d] = fmin(wlo[d], __shfl_xor(wlo[d], m));
@@ -430,66 +454,102 @@ __global__ __launch_bounds__(256) void contract_var_f64_kernel(SweepJob job) {
             wdn = fmax(wdn, __shfl_xor(wdn, m));
         }
         wthr = prune_threshold<DIST>(zero_weight_distance(KER, wb, false, wdn));
-        double fb = b, gdn = fdn;
-#pragma unroll
-        for (int d = 0; d < BD; ++d) {
-            flo[d] = wlo[d];
-            fhi[d] = whi[d];
-        }
-        workgroup_box<BD>(flo, fhi, wb, &fb, wdn, &gdn);
-        thr = prune_threshold<DIST>(zero_weight_distance(KER, fb, false, gdn));
     }
     const int jbeg = seg * job.seg_len;
     const int jend = min(n, jbeg + job.seg_len);
+    const int cbeg = jbeg / kChunk, cend = (jend + kChunk - 1) / kChunk;
+    GWM_GLOBAL const double* boxes = gptr(job.boxes);
+    // the boxes of the 16-chunk groups follow the n / 64 chunk boxes
+    GWM_GLOBAL const double* gboxes = prune ? boxes + (size_t)((n + kChunk - 1) / kChunk) * 2 * BD : nullptr;
     GWM_GLOBAL const double* Z = gptr(job.Z);
     const int ldz = job.nta * 16;
+    // the first chunk at or after c that the wave weighs (cend: none); the
+    // tests are uniform over the wave
+    auto next_live = [&](int c) {
+        while (c < cend) {
+            if (!prune) return c;
+            if ((c & 15) == 0 || c == cbeg) {
+                if (box_gap2<DIST>(wlo, whi, gboxes + (size_t)(c >> 4) * 2 * BD) > wthr) {
+                    c = (c | 15) + 1;
+                    continue;
+                }
+            }
+            if (!(box_gap2<DIST>(wlo, whi, boxes + (size_t)c * 2 * BD) > wthr)) return c;
+            ++c;
+        }
+        return cend;
+    };
+    auto load_point = [&](int c) {
+        VarPoint<DIST> v;
+        const int j = c * kChunk + lane;
+        const int jl = j < n ? j : n - 1;
+        v.c[0] = P.px[jl];
+        v.c[1] = P.py[jl];
+        if constexpr (DIST == kGreatCircle) {
+            v.c[2] = P.hx[jl];
+            v.c[3] = P.hy[jl];
+            v.c[4] = P.hz[jl];
+            v.c[5] = P.sl[jl];
+        }
+        // rows past n are zero in the table (padded to a multiple of 64)
+        const size_t row = (size_t)j * ldz;
+        v.z0 = Z[row];
+        v.z1 = Z[row + 1];
+        return v;
+    };
+    auto store_point = [&](const VarPoint<DIST>& v) {
+        if constexpr (DIST == kEuclidean) {
+            sh.c.x[lane] = v.c[0];
+            sh.c.y[lane] = v.c[1];
+        } else {
+            sh.c.lon[lane] = v.c[0];
+            sh.c.lat[lane] = v.c[1];
+            sh.c.hx[lane] = v.c[2];
+            sh.c.hy[lane] = v.c[3];
+            sh.c.hz[lane] = v.c[4];
+            sh.c.sl[lane] = v.c[5];
+        }
+        sh.z0[lane] = v.z0;
+        sh.z1[lane] = v.z1;
+    };
     double a0 = 0.0, a1 = 0.0;
     int swept = 0;
-    for (int j0 = jbeg; j0 < jend; j0 += kChunk) {
-        if (prune) {
-            const int skip = box_gap2<DIST>(flo, fhi, gptr(job.boxes) + (size_t)(j0 / kChunk) * 2 * BD) > thr;
-            if (__builtin_amdgcn_readfirstlane(skip)) continue;
-        }
+    int cur = next_live(cbeg);
+    if (cur < cend) {
+        store_point(load_point(cur));
+        var_wave_sync();
+    }
+    while (cur < cend) {
+        const int nxt = next_live(cur + 1);
+        VarPoint<DIST> v;
+        if (nxt < cend) v = load_point(nxt);   // in flight while the current chunk is weighed
         ++swept;
-        __syncthreads();
-        if (threadIdx.x < kChunk) {
-            const int j = j0 + (int)threadIdx.x;
-            const int jl = j < n ? j : n - 1;
-            if constexpr (DIST == kEuclidean) {
-                sh.c.x[threadIdx.x] = P.px[jl];
-                sh.c.y[threadIdx.x] = P.py[jl];
-            } else {
-                sh.c.lon[threadIdx.x] = P.px[jl];
-                sh.c.lat[threadIdx.x] = P.py[jl];
-                sh.c.hx[threadIdx.x] = P.hx[jl];
-                sh.c.hy[threadIdx.x] = P.hy[jl];
-                sh.c.hz[threadIdx.x] = P.hz[jl];
-                sh.c.sl[threadIdx.x] = P.sl[jl];
-            }
-        } else if (threadIdx.x < 2 * kChunk) {
-            // rows past n are zero in the table (padded to a multiple of 64)
-            const int t = (int)threadIdx.x - kChunk;
-            const size_t row = (size_t)(j0 + t) * ldz;
-            sh.z0[t] = Z[row];
-            sh.z1[t] = Z[row + 1];
-        }
-        __syncthreads();
-        if (prune) {
-            const int wskip = box_gap2<DIST>(wlo, whi, gptr(job.boxes) + (size_t)(j0 / kChunk) * 2 * BD) > wthr;
-            if (__builtin_amdgcn_readfirstlane(wskip)) continue;   // no barrier before the next chunk's
-        }
-        const int cnt = min(kChunk, jend - j0);
+        const int j0 = cur * kChunk;
+        // all 64 slots, uniform over the wave: segments are whole chunks, so
+        // only the last chunk of the data is partial, and its slots past n
+        // hold zero table rows (finite weights times zero)
 #pragma unroll 4
-        for (int e = 0; e < cnt; ++e) {
+        for (int st = 0; st < kChunk / 4; ++st) {
+            const int e = 4 * st + q;
             double w = pair_weight<DIST>(KER, fc, sh, e, kc);
             if (MODE == kModeCV && j0 + e == f) w = 0.0;
             a0 = fma(w, sh.z0[e], a0);
             a1 = fma(w, sh.z1[e], a1);
         }
+        var_wave_sync();
+        if (nxt < cend) store_point(v);
+        var_wave_sync();
+        cur = nxt;
     }
-    if (job.pairs && threadIdx.x == 0 && swept)
-        atomicAdd(job.pairs, (unsigned long long)swept * 256ull * 64ull);
-    if (f < n) {
+    if (job.pairs && lane == 0 && swept)
+        atomicAdd(job.pairs, (unsigned long long)swept * 16ull * 64ull);
+    // (s_0 + s_1) + (s_2 + s_3) in every lane of the focus point (addition
+    // commutes bit for bit, so both partners of each step hold the same sum)
+    a0 += __shfl_xor(a0, 16);
+    a1 += __shfl_xor(a1, 16);
+    a0 += __shfl_xor(a0, 32);
+    a1 += __shfl_xor(a1, 32);
+    if (q == 0 && f < n) {
         GWM_GLOBAL double* out = gptr(job.out) + (size_t)seg * 2 * job.n_pad;
         out[f] = a0;
         out[job.n_pad + f] = a1;
@@ -499,7 +559,7 @@ __global__ __launch_bounds__(256) void contract_var_f64_kernel(SweepJob job) {
 // the kernel id as a constant: the per-pair kernel switch folds away
 template <int DIST, int MODE>
 hipError_t launch_var_k(const SweepJob& j, hipStream_t s) {
-    const dim3 grid((j.n + 255) / 256 * j.nseg);
+    const dim3 grid((j.n + kVarFocus - 1) / kVarFocus * j.nseg);
     switch (j.kernel) {
     case kGaussian: hipLaunchKernelGGL((contract_var_f64_kernel<DIST, MODE, kGaussian>), grid, dim3(256), 0, s, j); break;
     case kExponential:

@@ -133,10 +133,13 @@ std::vector<int> spatial_order(int n, const double* xs, const double* ys) {
     return a;
 }
 
-// Bounding boxes [lo_0, hi_0, lo_1, hi_1, ...] of consecutive 64-point chunks.
+// Bounding boxes [lo_0, hi_0, lo_1, hi_1, ...] of consecutive 64-point
+// chunks, followed by those of the groups of 16 consecutive chunks (1,024
+// points: group g covers chunks 16 g .. 16 g + 15), which let a sweep skip a
+// whole group with one test.
 std::vector<double> chunk_boxes(int n, int dims, const double* const* xs) {
-    const int nc = (n + 63) / 64;
-    std::vector<double> b((size_t)nc * 2 * dims);
+    const int nc = (n + 63) / 64, ng = (nc + 15) / 16;
+    std::vector<double> b((size_t)(nc + ng) * 2 * dims);
     for (int ch = 0; ch < nc; ++ch) {
         const int j0 = ch * 64, j1 = std::min(n, j0 + 64);
         for (int d = 0; d < dims; ++d) {
@@ -147,6 +150,19 @@ std::vector<double> chunk_boxes(int n, int dims, const double* const* xs) {
             }
             b[(size_t)ch * 2 * dims + 2 * d] = mn;
             b[(size_t)ch * 2 * dims + 2 * d + 1] = mx;
+        }
+    }
+    for (int g = 0; g < ng; ++g) {
+        double* gb = b.data() + (size_t)(nc + g) * 2 * dims;
+        const int c1 = std::min(nc, 16 * g + 16);
+        for (int d = 0; d < dims; ++d) {
+            double mn = b[(size_t)16 * g * 2 * dims + 2 * d], mx = b[(size_t)16 * g * 2 * dims + 2 * d + 1];
+            for (int ch = 16 * g + 1; ch < c1; ++ch) {
+                mn = std::min(mn, b[(size_t)ch * 2 * dims + 2 * d]);
+                mx = std::max(mx, b[(size_t)ch * 2 * dims + 2 * d + 1]);
+            }
+            gb[2 * d] = mn;
+            gb[2 * d + 1] = mx;
         }
     }
     return b;
