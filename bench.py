@@ -440,11 +440,13 @@ def run_serve(capi, synth, num, rates, check):
         with capi.Server(0, capi.MIXED, max_inflight=64) as srv:
             start = time.perf_counter()
             ids = []
-            for r in reqs:
+            for q, r in enumerate(reqs):
                 delay = r["t"] / rate - (time.perf_counter() - start)
                 if delay > 0:
                     time.sleep(delay)
                 ids.append(serve_submit(capi, srv, r))
+                if (q + 1) % 100 == 0:
+                    log("serve: rate %.2f/s, %d requests submitted" % (rate, q + 1))
             res = [srv.wait(i) for i in ids]
             wall = time.perf_counter() - start
             st = srv.stats()
