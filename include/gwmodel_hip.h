@@ -292,6 +292,17 @@ int gwm_hip_autotune_report(char* buf, int cap);
  * use it. */
 int gwm_hip_set_distance_table(gwm_hip_ctx* ctx, double max_bytes);
 
+/* FP64 distance table of certified searches (no counterpart in the
+ * reference).  In a mixed-precision golden-section search over fixed
+ * bandwidths of great-circle data, the first FP64 (certification) CV or AICc
+ * sweep stores the FP64 squared distance of every pair that the bracket's
+ * upper end reaches, exactly as the FP64 sweep forms it, and the search's
+ * later FP64 sweeps read them instead of forming them: the same values, so
+ * the same criteria and iterates.  The table is released when the search
+ * returns.  It is built only when it needs at most max_bytes (default 48 GiB)
+ * and the device keeps 8 GiB free beside it; max_bytes = 0 turns it off. */
+int gwm_hip_set_certify_table(gwm_hip_ctx* ctx, double max_bytes);
+
 /* Neighbour table of adaptive bandwidths (no counterpart in the reference,
  * whose BandwidthWeight::weight sorts every focus point's n distances again
  * for each bandwidth, BandwidthWeight.cpp:40-55).  At the second adaptive
@@ -309,7 +320,7 @@ int gwm_hip_set_neighbour_table(gwm_hip_ctx* ctx, double max_bytes);
  * (enabled by gwm_hip_set_timing).  name: "contract_<fp64|mixed>_<cv|plain|
  * hat|qdiag>" for the sweeps, "select" (adaptive bandwidths by the select),
  * "select_lookup" (by the neighbour table), "nbr_table" (its build), "solve",
- * "d2_table" (the distance table). */
+ * "d2_table" (the distance table), "f64_table" (the certification table). */
 int gwm_hip_set_timing(gwm_hip_ctx* ctx, int enabled);
 int gwm_hip_kernel_time(gwm_hip_ctx* ctx, const char* name, long long* launches, double* total_ms);
 /* Focus x data pairs the named sweeps actually processed while timing was on

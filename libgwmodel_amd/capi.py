@@ -53,6 +53,7 @@ SIGNATURES = {
     "gwm_hip_set_autotune": (_i, [_vp, _i]),
     "gwm_hip_set_distance_table": (_i, [_vp, _d]),
     "gwm_hip_set_neighbour_table": (_i, [_vp, _d]),
+    "gwm_hip_set_certify_table": (_i, [_vp, _d]),
     "gwm_hip_autotune_report": (_i, [ctypes.c_char_p, _i]),
     "gwm_hip_mgwr_last_shat": (_i, [_vp, _dp]),
     "gwm_hip_indep_vars_criterion": (_i, [_vp, _i, _ip, _i, _dp]),
@@ -384,6 +385,10 @@ class Context:
         r = ctypes.c_longlong(0)
         self._check(self._lib.gwm_hip_last_condition(self.h, _p(v), ctypes.byref(r)))
         return float(v[0]), r.value
+
+    def set_certify_table(self, max_bytes):
+        """Byte cap of the FP64 distance table of certified searches (0: off)."""
+        self._check(self._lib.gwm_hip_set_certify_table(self.h, float(max_bytes)))
 
     def set_neighbour_table(self, max_bytes):
         """Byte cap of the adaptive-bandwidth neighbour table (0: off)."""

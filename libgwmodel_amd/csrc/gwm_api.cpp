@@ -221,7 +221,7 @@ namespace gwm {
 namespace detail {
 
 int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certify, double tol_rel,
-           double* xopt_out, std::map<double, double>& log, const EvalReport& report) {
+           double* xopt_out, std::map<double, double>& log, const EvalReport& report, double* upper_out) {
     double xU = upper, xL = lower;
     const double eps = 1e-4;
     const double R = (std::sqrt(5.0) - 1) / 2;
@@ -236,6 +236,7 @@ int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certi
     // evaluate; a failed (DBL_MAX) mixed value is confirmed in FP64
     auto eval = [&](double x, bool& exact, double* fx, double* fb) -> int {
         *fb = 0.0;
+        if (upper_out) *upper_out = xU;
         int rc = f(x, exact, fx, fb);
         if (rc == GWM_ERR_NOT_SPD) {
             *fx = DBL_MAX;
@@ -1153,6 +1154,7 @@ int gwm_hip_select_bandwidth(gwm_hip_ctx* c, int kernel, int adaptive, int crit,
     if ((rc = check_kernel(c, kernel, precision))) return rc;
     GWM_HIP_CHECK(c, hipSetDevice(c->device));
     D2Scope scope(c);
+    CertScope cert(c, precision == GWM_MIXED && !adaptive);
     if (len) *len = 0;
     double lo = has_lower ? lower : (adaptive ? 20 : 0.0);
     double hi;
@@ -1165,7 +1167,8 @@ int gwm_hip_select_bandwidth(gwm_hip_ctx* c, int kernel, int adaptive, int crit,
                 [&](double x, bool exact, double* fx, double* fb) {
                     return criterion_impl(c, kernel, adaptive, x, crit, exact ? GWM_FP64 : precision, fx, fb);
                 },
-                precision == GWM_MIXED, c->certify_rel, &xopt, log, telegram_report(c, adaptive != 0));
+                precision == GWM_MIXED, c->certify_rel, &xopt, log, telegram_report(c, adaptive != 0),
+                &c->cert_hi);
     write_trace(log, tx, tf, cap, len);
     if (rc) return rc;
     if (bw_out) *bw_out = xopt;
@@ -1642,6 +1645,12 @@ int gwm_hip_set_autotune(gwm_hip_ctx* c, int enabled) {
 int gwm_hip_set_distance_table(gwm_hip_ctx* c, double max_bytes) {
     if (!c || !(max_bytes >= 0.0)) return GWM_ERR_INVALID;
     c->d2_cap = max_bytes >= 1.8e19 ? SIZE_MAX : (size_t)max_bytes;
+    return GWM_OK;
+}
+
+int gwm_hip_set_certify_table(gwm_hip_ctx* c, double max_bytes) {
+    if (!c || !(max_bytes >= 0.0)) return GWM_ERR_INVALID;
+    c->cert_cap = max_bytes >= 1.8e19 ? SIZE_MAX : (size_t)max_bytes;
     return GWM_OK;
 }
 

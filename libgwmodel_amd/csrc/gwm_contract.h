@@ -60,6 +60,17 @@ struct SweepJob {
     // out holds the focus points (or list entries) from out_off on: column t
     // of the output is focus out_off + t (a server round's slice)
     int out_off = 0;
+    // FP64 great-circle sweeps of one mixed-precision search at fixed
+    // bandwidths no wider than tab_bw (its certification evaluations): the
+    // FP64 squared distances of the pairs that bandwidth reaches, as the
+    // sweep forms them, written once (launch_contract_f64_table) and read
+    // instead of formed.  Item (tile, seg) holds tab_cnt[item] blocks of
+    // kWaves x 16 x R x 64 doubles from block tab_off[item] on, one per chunk
+    // within reach of tab_bw, in chunk order.
+    double* f64tab = nullptr;
+    const long long* tab_off = nullptr;
+    int* tab_cnt = nullptr;
+    double tab_bw = 0.0;
 };
 
 // Distance id of a mixed sweep that reads the precomputed squared great-circle
@@ -111,6 +122,14 @@ hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const Sweep
 // focus point): out = [nseg][2][n_pad], the columns x^2 and x y of the
 // single-column table, summed on the VALU (gwm_contract.hip).
 hipError_t launch_contract_var_f64(int dist, int mode, const SweepJob& j, hipStream_t s);
+// The FP64 distance table of a single-job great-circle CV / AICc sweep at a
+// fixed bandwidth (SweepJob::f64tab), in the default launch shape: stage 3
+// counts each item's chunks within reach of job0.bw into tab_cnt, stage 1
+// writes their blocks (tab_off from the counts' prefix sums), stage 2 is the
+// sweep at job0.bw <= tab_bw reading them.  Returns the doubles per block
+// in *block (stage 3).
+hipError_t launch_contract_f64_table(int stage, int mode, int nta, const SweepLaunch& a, hipStream_t s,
+                                     long long* block = nullptr);
 hipError_t launch_contract_mixed(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s);
 // the R (focus tiles per wave) each kernel uses for an (nta, ntb, mode,
 // distance) instance; launchers and item tables must agree on it

@@ -79,10 +79,12 @@ __device__ __forceinline__ FocusCoords<DIST> load_focus(const GPoints& P, int f)
 // Great circle: near pairs take
 // gc_d2_near_f64, the others (far, coincident, near-coincident) gcdist_f64,
 // in a branch that costs a wave only when one of its lanes needs it.
+// The squared distance of the pair (focus fc, staged point r); *d receives
+// the distance itself where the far form computes it (else -1).
 template <int DIST, class SH>
-__device__ __forceinline__ double pair_weight(int kernel, const FocusCoords<DIST>& fc, const SH& sh, int r,
-                                              const KConst64& k) {
-    double d2, d = -1.0;
+__device__ __forceinline__ double pair_d2(const FocusCoords<DIST>& fc, const SH& sh, int r, double* d) {
+    double d2;
+    *d = -1.0;
     if constexpr (DIST == kEuclidean) {
         d2 = d2_euclid(fc.a, fc.b, sh.c.x[r], sh.c.y[r]);
     } else {
@@ -92,11 +94,18 @@ __device__ __forceinline__ double pair_weight(int kernel, const FocusCoords<DIST
         S = S + ez * ez;
         d2 = gc_d2_near_f64(S, fc.sl, sh.c.sl[r]);
         if (!(S >= kNearSMin && S <= kNearSMax)) {
-            d = gcdist_f64(fc.a, fc.b, fc.hx, fc.hy, fc.hz, fc.sl, sh.c.lon[r], sh.c.lat[r], sh.c.hx[r],
-                           sh.c.hy[r], sh.c.hz[r], sh.c.sl[r]);
-            d2 = d * d;
+            *d = gcdist_f64(fc.a, fc.b, fc.hx, fc.hy, fc.hz, fc.sl, sh.c.lon[r], sh.c.lat[r], sh.c.hx[r],
+                            sh.c.hy[r], sh.c.hz[r], sh.c.sl[r]);
+            d2 = *d * *d;
         }
     }
+    return d2;
+}
+
+// The weight from the pair's squared distance (and its distance when known:
+// else sqrt(d2), which equals it -- the correctly rounded root of a
+// correctly rounded square is the number itself).
+__device__ __forceinline__ double weight_of_d2(int kernel, double d2, double d, const KConst64& k) {
     if (kernel == kGaussian) {
         const double e = d2 * k.gauss;
         return e < k.cut ? 0.0 : exp(e);
@@ -110,7 +119,20 @@ __device__ __forceinline__ double pair_weight(int kernel, const FocusCoords<DIST
     return kernel_fast_f64(kernel, d >= 0.0 ? d : sqrt(d2), k);
 }
 
-template <int NTA, int NTB, int DIST, int MODE, int R>
+template <int DIST, class SH>
+__device__ __forceinline__ double pair_weight(int kernel, const FocusCoords<DIST>& fc, const SH& sh, int r,
+                                              const KConst64& k) {
+    double d;
+    const double d2 = pair_d2<DIST>(fc, sh, r, &d);
+    return weight_of_d2(kernel, d2, d, k);
+}
+
+// TAB: 0 the plain sweep; for the FP64 distance table (SweepJob::f64tab) of
+// a single-job great-circle sweep at a fixed bandwidth: 3 counts each item's
+// chunks within reach of job.bw, 1 writes their squared distances, 2 sweeps
+// at job.bw <= job.tab_bw reading them (the chunks within reach of tab_bw
+// keep their block index whether or not job.bw reaches them).
+template <int NTA, int NTB, int DIST, int MODE, int R, int TAB = 0>
 // Two waves per SIMD: the register budget is 256 per lane.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void contract_f64_kernel(SweepLaunch L) {
     __shared__ SharedTile<NTA, DIST, MODE> sh;
@@ -200,6 +222,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
         if (MODE != kModeQDiag) thr = prune_threshold<DIST>(zero_weight_distance(job.kernel, fb, false, fdn));
         else if (!job.chunk_bw) thr = prune_threshold<DIST>(zero_weight_distance(job.kernel, job.bw, false));
     }
+    // the table's reach (stage 2; the same expression as the stages that built it)
+    double thr_tab = thr;
+    if constexpr (TAB == 2) thr_tab = prune_threshold<DIST>(zero_weight_distance(job.kernel, job.tab_bw, false, fdn));
+    const int item = tile * job.nseg + seg;
+    constexpr long long kBlock = (long long)kWaves * 16 * R * 64;   // doubles per table block
+    long long tb = 0;                                             // the item's next table block
+    bool poison = false;   // stage 2: a block past the item's count (cannot occur): NaN outputs
+    const long long tcnt = TAB == 1 || TAB == 2 ? (long long)job.tab_cnt[item] : 0;
+    GWM_GLOBAL double* tab = TAB == 1 || TAB == 2 ? gptr(job.f64tab) + (TAB ? job.tab_off[item] : 0) * kBlock : nullptr;
 
     v4d accA[R][NTA];
     v4d accB[R][NTB > 0 ? NTB : 1];
@@ -217,22 +248,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
 
     int swept = 0;   // chunks processed (not pruned)
     for (int j0 = jbeg; j0 < jend; j0 += kChunk) {
+        long long blk = 0;   // the chunk's table block (stages 1 and 2)
         if (prune) {
             const int ch = j0 / kChunk;
             double t = thr;
             if (MODE == kModeQDiag && job.chunk_bw)
                 t = prune_threshold<DIST>(zero_weight_distance(job.kernel, gptr(job.chunk_bw)[ch], false));
-            const int skip = box_gap2<DIST>(flo, fhi, gptr(job.boxes) + (size_t)ch * 2 * BD) > t;
+            const double gap = box_gap2<DIST>(flo, fhi, gptr(job.boxes) + (size_t)ch * 2 * BD);
+            if constexpr (TAB == 2) {
+                if (__builtin_amdgcn_readfirstlane(gap > thr_tab)) continue;
+                blk = tb++;
+            }
+            const int skip = gap > t;
             if (__builtin_amdgcn_readfirstlane(skip)) continue;
+            if constexpr (TAB == 2) {
+                if (blk >= tcnt) {   // uniform: nothing is read
+                    poison = true;
+                    continue;
+                }
+            }
+            if constexpr (TAB == 1) {
+                blk = tb++;
+                if (blk >= tcnt) continue;   // uniform; cannot occur (stage 3 counted these chunks)
+            }
         }
         ++swept;
+        if constexpr (TAB == 3) continue;
+        // stage 2: this lane's 16 x R squared distances of the chunk, loaded
+        // into registers before the Z rows are staged (the loads overlap the
+        // staging and the barrier); slots of steps past a partial last chunk
+        // are in the block but were not written, and their weights are masked
+        double tv[TAB == 2 ? 16 : 1][R];
+        if constexpr (TAB == 2) {
+            GWM_GLOBAL const double* src = tab + blk * kBlock + (size_t)wave * 16 * R * 64 + lane;
+#pragma unroll
+            for (int s = 0; s < 16; ++s)
+#pragma unroll
+                for (int r = 0; r < R; ++r) tv[s][r] = src[(s * R + r) * 64];
+        }
         __syncthreads();
         // stage Z rows [j0, j0 + kChunk) (zero padded beyond n in memory)
-        {
+        if constexpr (TAB != 1) {
             GWM_GLOBAL const double* src = gptr(job.Z) + (size_t)j0 * ldz;
             for (int e = threadIdx.x; e < kChunk * ldz; e += 256) sh.z[e / ldz][e % ldz] = src[e];
         }
-        if (threadIdx.x < kChunk) {
+        if (TAB != 2 && threadIdx.x < kChunk) {
             int j = j0 + threadIdx.x;
             int jl = j < n ? j : n - 1;
             if constexpr (DIST == kEuclidean) {
@@ -250,6 +310,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
         }
         __syncthreads();
         const int steps = min(kChunk, jend - j0 + 3) / 4;
+        if constexpr (TAB == 1) {
+            // one table block: entry ((wave * 16 + s) * R + r) * 64 + lane
+            GWM_GLOBAL double* tb_out = tab + blk * kBlock + (size_t)wave * 16 * R * 64 + lane;
+            for (int s = 0; s < steps; ++s) {
+                const int rr = 4 * s + kk;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    double d;
+                    tb_out[(s * R + r) * 64] = pair_d2<DIST>(fc[r], sh, rr, &d);
+                }
+            }
+            continue;
+        }
+        if constexpr (TAB == 2) {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                // all 16 steps: the rows past a partial last chunk have j >= jend
+                const int rr = 4 * s + kk;
+                const int j = j0 + rr;
+                double zb[NTA];
+#pragma unroll
+                for (int t = 0; t < NTA; ++t) zb[t] = t < nta ? sh.z[rr][16 * t + (lane & 15)] : 0.0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    // a slot past the chunk's end may hold anything: it is not weighed
+                    double w = j < jend ? weight_of_d2(job.kernel, tv[s][r], -1.0, fbw[r]) : 0.0;
+                    if (j >= jend) w = 0.0;
+                    if (MODE == kModeCV && j == fidx[r]) w = 0.0;
+#pragma unroll
+                    for (int t = 0; t < NTA; ++t)
+                        if (t < nta) accA[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(w, zb[t], accA[r][t], 0, 0, 0);
+                }
+            }
+            continue;
+        }
         for (int s = 0; s < steps; ++s) {
             const int rr = 4 * s + kk;    // row of the chunk this lane weighs
             const int j = j0 + rr;
@@ -260,7 +355,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
             for (int r = 0; r < R; ++r) {
                 // qDiag with a fixed bandwidth: fbw holds the same constants as sh.kc
                 double w = pair_weight<DIST>(job.kernel, fc[r], sh, rr,
-                                                        (MODE == kModeQDiag && bw_pt) ? sh.kc[rr] : fbw[r]);
+                                             (MODE == kModeQDiag && bw_pt) ? sh.kc[rr] : fbw[r]);
                 if (j >= jend) w = 0.0;
                 if (MODE == kModeCV && j == fidx[r]) w = 0.0;
                 double wa = MODE == kModeQDiag ? w * w : w;
@@ -278,9 +373,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
         }
     }
 
+    if constexpr (TAB == 3) {
+        if (threadIdx.x == 0) job.tab_cnt[item] = swept;
+        return;
+    }
+    if constexpr (TAB == 1) return;
     if (job.pairs && threadIdx.x == 0 && swept)
         atomicAdd(job.pairs, (unsigned long long)swept * (unsigned long long)(kWaves * 16 * R) * 64ull);
 
+    if (poison)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int t = 0; t < NTA; ++t) accA[r][t] = v4d{NAN, NAN, NAN, NAN};
+        }
     // D layout of v_mfma_f64_16x16x4f64 (checked on gfx950): lane holds
     // column (lane % 16), rows (lane / 16) + 4 q, q = 0..3.
     const size_t np = (size_t)job.n_pad;
@@ -336,6 +442,37 @@ hipError_t launch_f64_r(const SweepLaunch& a, hipStream_t s) {
         if (a.shape_r == 1) return launch_f64_rr<NTA, NTB, DIST, MODE, 1>(a, s);
     }
     return launch_f64_rr<NTA, NTB, DIST, MODE, RD>(a, s);
+}
+
+// Table stages of a single-job great-circle CV / AICc sweep (default shape).
+template <int NTA, int MODE, int TAB>
+hipError_t launch_f64_tab(const SweepLaunch& a, hipStream_t s) {
+    constexpr int R = r_of<NTA, 0>();
+    const int tiles = (a.job0.n + kWaves * 16 * R - 1) / (kWaves * 16 * R);
+    const int grid = tiles * a.job0.nseg;
+    if (grid <= 0) return hipSuccess;
+    if (a.order && a.norder != grid) return hipErrorInvalidValue;
+    SweepLaunch b = a;
+    b.tile_stride = a.linear_order ? 1 : tile_stride(tiles);
+    hipLaunchKernelGGL((contract_f64_kernel<NTA, 0, kGreatCircle, MODE, R, TAB>), dim3(grid), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+template <int MODE, int TAB>
+hipError_t dispatch_tab(int nta, const SweepLaunch& a, hipStream_t s) {
+    switch (nta) {
+    case 1: return launch_f64_tab<1, MODE, TAB>(a, s);
+    case 2: return launch_f64_tab<2, MODE, TAB>(a, s);
+    case 3: return launch_f64_tab<3, MODE, TAB>(a, s);
+    case 4: return launch_f64_tab<4, MODE, TAB>(a, s);
+    case 5: return launch_f64_tab<5, MODE, TAB>(a, s);
+    case 6: return launch_f64_tab<6, MODE, TAB>(a, s);
+    case 7: return launch_f64_tab<7, MODE, TAB>(a, s);
+    case 8: return launch_f64_tab<8, MODE, TAB>(a, s);
+    case 9: return launch_f64_tab<9, MODE, TAB>(a, s);
+    case 10: return launch_f64_tab<10, MODE, TAB>(a, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int DIST, int MODE>
@@ -585,6 +722,26 @@ hipError_t launch_contract_var_f64(int dist, int mode, const SweepJob& j, hipStr
     }
     if (mode == kModeCV) return launch_var_k<kGreatCircle, kModeCV>(j, s);
     return launch_var_k<kGreatCircle, kModePlain>(j, s);
+}
+
+hipError_t launch_contract_f64_table(int stage, int mode, int nta, const SweepLaunch& a, hipStream_t s,
+                                     long long* block) {
+    const SweepJob& j = a.job0;
+    if (a.items || j.focus_idx || j.Q.n > 0 || j.bw_pt || !j.boxes || j.ntb != 0 || j.nta != nta || nta < 1 ||
+        nta > 10 || (mode != kModeCV && mode != kModePlain) || stage < 1 || stage > 3)
+        return hipErrorInvalidValue;
+    if (!j.tab_cnt) return hipErrorInvalidValue;
+    if (stage != 3 && (!j.f64tab || !j.tab_off)) return hipErrorInvalidValue;
+    if (stage == 2 && !(j.bw <= j.tab_bw)) return hipErrorInvalidValue;
+    if (block) *block = (long long)kWaves * 16 * (nta <= 8 ? 2 : 1) * 64;
+    if (mode == kModeCV) {
+        if (stage == 1) return dispatch_tab<kModeCV, 1>(nta, a, s);
+        if (stage == 2) return dispatch_tab<kModeCV, 2>(nta, a, s);
+        return dispatch_tab<kModeCV, 3>(nta, a, s);
+    }
+    if (stage == 1) return dispatch_tab<kModePlain, 1>(nta, a, s);
+    if (stage == 2) return dispatch_tab<kModePlain, 2>(nta, a, s);
+    return dispatch_tab<kModePlain, 3>(nta, a, s);
 }
 
 hipError_t launch_contract_f64(int dist, int mode, int nta, int ntb, const SweepLaunch& a, hipStream_t s) {

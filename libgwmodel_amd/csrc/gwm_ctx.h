@@ -152,6 +152,26 @@ struct gwm_hip_ctx {
     bool d2_valid = false;
     int d2_scope = 0;
     size_t d2_cap = size_t(96) << 30;
+    // FP64 distance table of the certification sweeps of a mixed-precision
+    // fixed-bandwidth great-circle search (launch_contract_f64_table): built
+    // at the search's first FP64 CV / AICc sweep for the widest bandwidth the
+    // search may still evaluate (cert_hi, the bracket's upper end), released
+    // when the search returns
+    struct CertTable {
+        double* tab = nullptr;
+        size_t bytes = 0;
+        long long* off = nullptr;   // [items] first block of each (tile, segment) item
+        int* cnt = nullptr;         // [items] its blocks
+        size_t items_cap = 0;
+        double bw = 0.0;            // the table's bandwidth
+        int mode = -1, kernel = -1, nta = 0;
+        bool cv_dnn = false;
+        bool valid = false;
+        double failed_bw = 0.0;     // a table this wide did not fit: only narrower ones are tried
+    } cert;
+    int cert_scope = 0;             // > 0 inside a mixed-precision fixed-bandwidth search
+    double cert_hi = 0.0;
+    size_t cert_cap = size_t(48) << 30;
 
     gwm::Points points() const {
         gwm::Points P;
@@ -198,8 +218,10 @@ double crit_error_bound(int crit, double value, double rss, double trS, double q
                         double tol_rel);
 // report(x, value) after each evaluation golden() uses (value DBL_MAX: failed)
 typedef std::function<void(double, double)> EvalReport;
+// upper_out (optional) holds the bracket's upper end at every evaluation:
+// no later candidate lies above it.
 int golden(bool adaptBw, double lower, double upper, const CritFn& f, bool certify, double tol_rel, double* xopt_out,
-           std::map<double, double>& log, const EvalReport& report = EvalReport());
+           std::map<double, double>& log, const EvalReport& report = EvalReport(), double* upper_out = nullptr);
 void write_trace(const std::map<double, double>& log, double* tx, double* tf, int cap, int* len);
 int max_distance_impl(gwm_hip_ctx* c, double* out);
 // c->dnn, computed on the context stream on first use.

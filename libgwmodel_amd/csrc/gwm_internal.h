@@ -105,6 +105,26 @@ struct D2Scope {
     D2Scope& operator=(const D2Scope&) = delete;
 };
 
+// Scope of a mixed-precision fixed-bandwidth search whose FP64 great-circle
+// certification sweeps share one FP64 distance table (ensure_cert_table);
+// the table is released when the search returns.
+void release_cert_table(gwm_hip_ctx* c);
+struct CertScope {
+    gwm_hip_ctx* c;
+    bool on;
+    CertScope(gwm_hip_ctx* ctx, bool enable) : c(ctx), on(enable) {
+        if (on && c->cert_scope++ == 0) {
+            c->cert_hi = 0.0;
+            c->cert.failed_bw = 0.0;
+        }
+    }
+    ~CertScope() {
+        if (on && --c->cert_scope == 0) release_cert_table(c);
+    }
+    CertScope(const CertScope&) = delete;
+    CertScope& operator=(const CertScope&) = delete;
+};
+
 // Adaptive bandwidth -> per-focus thresholds (a vector of the bandwidth cache,
 // filled by the select on a miss); fixed -> *bw_pt = nullptr.
 int resolve_bandwidth(gwm_hip_ctx* c, int adaptive, double bw, const double** bw_pt);

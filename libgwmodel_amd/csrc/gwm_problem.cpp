@@ -31,7 +31,8 @@ void free_problem(gwm_hip_ctx* c) {
     dfree(c->vZ); dfree(c->S); dfree(c->fail);
     dfree(c->Zh); dfree(c->Zl); dfree(c->Gh); dfree(c->Gl); dfree(c->vZh); dfree(c->vZl);
     dfree(c->zscale); dfree(c->gscale); dfree(c->vscale); dfree(c->colmax);
-    // the plan buffers and the distance table come from hipMalloc on every context
+    // the plan buffers and the distance tables come from hipMalloc on every context
+    release_cert_table(c);
     if (c->d2tab) hipFree(c->d2tab);
     c->d2tab = nullptr;
     c->d2tab_bytes = 0;

@@ -55,6 +55,127 @@ const char* mode_name(int mode) {
     }
 }
 
+void release_cert_table(gwm_hip_ctx* c) {
+    auto& t = c->cert;
+    if (t.tab || t.off || t.cnt) hipStreamSynchronize(c->stream);
+    if (t.tab) hipFree(t.tab);
+    if (t.off) hipFree(t.off);
+    if (t.cnt) hipFree(t.cnt);
+    t = gwm_hip_ctx::CertTable{};
+    c->cert_hi = 0.0;
+}
+
+// The FP64 certification sweep of a mixed-precision fixed-bandwidth
+// great-circle search (L: the sweep's single job at bandwidth bw) through
+// the search's FP64 distance table, built here for the bracket's upper end
+// when there is none for this sweep's mode, kernel and column tiles.  *used
+// is 0 when the table does not fit (the caller then sweeps as usual).
+static int cert_sweep(gwm_hip_ctx* c, SweepLaunch L, int mode, int nta, bool cv, int* used) {
+    *used = 0;
+    auto& t = c->cert;
+    SweepJob& j = L.job0;
+    const int r = nta <= 8 ? 2 : 1;   // the FP64 sweep's default shape
+    const int tiles = (c->n + 64 * r - 1) / (64 * r);
+    const size_t items = (size_t)tiles * j.nseg;
+    const bool fits = t.valid && t.mode == mode && t.kernel == j.kernel && t.nta == nta && t.cv_dnn == cv &&
+                      j.bw <= t.bw;
+    if (!fits) {
+        const double B = c->cert_hi;
+        if (t.failed_bw > 0.0 && !(B < t.failed_bw)) return GWM_OK;
+        t.valid = false;
+        if (items > t.items_cap) {
+            if (t.off) hipFree(t.off);
+            if (t.cnt) hipFree(t.cnt);
+            t.off = nullptr;
+            t.cnt = nullptr;
+            t.items_cap = 0;
+            GWM_HIP_CHECK(c, hipMalloc((void**)&t.off, items * sizeof(long long)));
+            GWM_HIP_CHECK(c, hipMalloc((void**)&t.cnt, items * sizeof(int)));
+            t.items_cap = items;
+        }
+        SweepLaunch T = L;
+        T.job0.bw = B;
+        T.job0.pairs = nullptr;
+        T.job0.tab_cnt = t.cnt;
+        long long block = 0;
+        GWM_HIP_CHECK(c, launch_contract_f64_table(3, mode, nta, T, c->stream, &block));
+        std::vector<int> cnt(items);
+        std::vector<long long> off(items);
+        GWM_HIP_CHECK(c, hipMemcpyAsync(cnt.data(), t.cnt, items * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        long long blocks = 0;
+        for (size_t q = 0; q < items; ++q) {
+            off[q] = blocks;
+            blocks += cnt[q];
+        }
+        const size_t bytes = (size_t)blocks * (size_t)block * sizeof(double);
+        size_t free_b = 0, total_b = 0;
+        if (bytes > c->cert_cap || hipMemGetInfo(&free_b, &total_b) != hipSuccess ||
+            free_b + t.bytes < bytes + (size_t(8) << 30)) {
+            (void)hipGetLastError();
+            t.failed_bw = B;
+            return GWM_OK;
+        }
+        if (t.bytes < bytes) {
+            if (t.tab) hipFree(t.tab);
+            t.tab = nullptr;
+            t.bytes = 0;
+            if (hipMalloc((void**)&t.tab, bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                t.failed_bw = B;
+                return GWM_OK;
+            }
+            t.bytes = bytes;
+        }
+        GWM_HIP_CHECK(c, hipMemcpyAsync(t.off, off.data(), items * sizeof(long long), hipMemcpyHostToDevice,
+                                        c->stream));
+        T.job0.f64tab = t.tab;
+        T.job0.tab_off = t.off;
+        timer_begin(c);
+        GWM_HIP_CHECK(c, launch_contract_f64_table(1, mode, nta, T, c->stream));
+        int rc = timer_end(c, "f64_table");
+        if (rc) return rc;
+        // the host offsets go out of scope
+        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        t.valid = true;
+        t.bw = B;
+        t.mode = mode;
+        t.kernel = j.kernel;
+        t.nta = nta;
+        t.cv_dnn = cv;
+    }
+    j.f64tab = t.tab;
+    j.tab_off = t.off;
+    j.tab_cnt = t.cnt;
+    j.tab_bw = t.bw;
+    L.linear_order = 0;
+    L.shape_r = 0;
+    L.order = nullptr;
+    if (tiles * j.nseg >= kTuneGridMax && j.nseg <= kPlanMaxSeg) {
+        // longest first, as the other large pruned sweeps
+        int rc;
+        if ((rc = ensure_plan(c, items))) return rc;
+        GWM_HIP_CHECK(c, launch_sweep_plan(c->dist, c->points(), c->n, c->boxes, nullptr, j.bw, nullptr, j.kernel, 0,
+                                           64 * r, j.seg_len, j.nseg, tiles, c->plan_counts, c->plan_order,
+                                           c->stream));
+        L.order = c->plan_order;
+        L.norder = tiles * j.nseg;
+    }
+    timer_begin(c);
+    GWM_HIP_CHECK(c, launch_contract_f64_table(2, mode, nta, L, c->stream));
+    if (c->timing) {
+        std::string name = std::string("contract_fp64_") + mode_name(mode);
+        int rc;
+        if ((rc = timer_end(c, name.c_str()))) return rc;
+        unsigned long long pairs = 0;
+        GWM_HIP_CHECK(c, hipMemcpyAsync(&pairs, c->pairs_dev, sizeof(pairs), hipMemcpyDeviceToHost, c->stream));
+        GWM_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        c->stats[name].pairs += (double)pairs;
+    }
+    *used = 1;
+    return GWM_OK;
+}
+
 // One sweep of the fused contraction.
 int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const double* bw_pt, double bw, int kernel,
           int precision, int* PT_out, int* nseg_out, const double* dnn) {
@@ -148,6 +269,18 @@ int sweep(gwm_hip_ctx* c, int mode, int nta, int ntb, const ZPlanes& zp, const d
         if (!c->pairs_dev) GWM_HIP_CHECK(c, hipMalloc((void**)&c->pairs_dev, sizeof(unsigned long long)));
         GWM_HIP_CHECK(c, hipMemsetAsync(c->pairs_dev, 0, sizeof(unsigned long long), c->stream));
         j.pairs = c->pairs_dev;
+    }
+    // FP64 certification sweeps of a mixed-precision fixed-bandwidth
+    // great-circle search: through the search's FP64 distance table
+    if (!mixed && c->cert_scope > 0 && c->dist == kGreatCircle && !bw_pt && c->prune && ntb == 0 && nta >= 1 &&
+        nta <= 10 && (mode == kModeCV || mode == kModePlain) && bw <= c->cert_hi && c->force_shape < 0) {
+        int used = 0;
+        if ((rc = cert_sweep(c, L, mode, nta, dnn != nullptr, &used))) return rc;
+        if (used) {
+            *PT_out = PT;
+            *nseg_out = nseg;
+            return GWM_OK;
+        }
     }
     // launch shape: forced, or from the autotuner (results are the same for every shape)
     if (c->force_shape >= 0) {
