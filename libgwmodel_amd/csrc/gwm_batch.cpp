@@ -245,6 +245,14 @@ struct gwm_hip_server {
     double round_ms = 25.0;      // work budget of a round; <= 0: whole evaluations
     size_t part_budget = size_t(256) << 20;   // partial-sum bytes of one request's round slice
     double refine_crit = 10.0, refine_fit = 2.0;   // FP64 recomputation limits (mixed)
+    // Request set-ups (sort, boxes, uploads, the first select) run a few at a
+    // time, the costliest waiting one first (n^2 k): a burst of admissions
+    // then has requests ready for the device early instead of all of them
+    // finishing together, and the long searches start first.
+    std::mutex setup_mu;
+    std::condition_variable setup_cv;
+    std::vector<std::pair<double, Request*>> setup_waiting;
+    int setup_running = 0;
 };
 
 namespace {
@@ -904,9 +912,32 @@ int setup(gwm_hip_server* S, Request* R);
 void teardown(Request* R);
 
 // Worker: the host algorithm of one request.
+constexpr int kSetupSlots = 8;   // concurrent request set-ups
+
 void worker_main(gwm_hip_server* S, Request* R) {
     {
+        const double cost = (double)R->n * (double)R->n * (double)R->k;
+        {
+            std::unique_lock<std::mutex> lk(S->setup_mu);
+            S->setup_waiting.emplace_back(cost, R);
+            auto first = [&]() {
+                auto it = std::max_element(S->setup_waiting.begin(), S->setup_waiting.end(),
+                                           [](const std::pair<double, Request*>& a,
+                                              const std::pair<double, Request*>& b) { return a.first < b.first; });
+                return it->second == R;
+            };
+            S->setup_cv.wait(lk, [&] { return S->setup_running < kSetupSlots && first(); });
+            S->setup_waiting.erase(std::find_if(S->setup_waiting.begin(), S->setup_waiting.end(),
+                                                [&](const std::pair<double, Request*>& e) { return e.second == R; }));
+            ++S->setup_running;
+            S->setup_cv.notify_all();
+        }
         const int rc = setup(S, R);
+        {
+            std::lock_guard<std::mutex> lk(S->setup_mu);
+            --S->setup_running;
+            S->setup_cv.notify_all();
+        }
         if (rc) {
             teardown(R);
             std::unique_lock<std::mutex> lk(S->mu);
