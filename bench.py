@@ -509,7 +509,7 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=512, help="uneven batch size (0: skip)")
     ap.add_argument("--no-loop", action="store_true", help="skip the single-problem loop of the batch")
     ap.add_argument("--serve-requests", type=int, default=500, help="serving stream length (0: skip)")
-    ap.add_argument("--serve-rates", type=str, default="5,8",
+    ap.add_argument("--serve-rates", type=str, default="4,6",
                     help="serving-stream arrival rates, requests/s (comma-separated; one replay each)")
     ap.add_argument("--serve-check", type=int, default=12,
                     help="requests spread over the stream's sizes re-run in FP64 for the beta error")
