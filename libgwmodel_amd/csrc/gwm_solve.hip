@@ -26,13 +26,27 @@ struct Sym {
     __device__ static constexpr int low(int i, int j) { return i * (i + 1) / 2 + j; }
 };
 
+// sum_{g < nseg} p[g stride] in segment order -- the bits of the plain loop
+// -- with the loads of eight segments issued together (the plain loop waits
+// for each load before the next: 704 dependent loads per focus point at k = 8)
+__device__ __forceinline__ double seg_sum(const double* p, size_t stride, int nseg) {
+    double s = 0.0;
+    for (int g0 = 0; g0 < nseg; g0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[(size_t)min(g0 + u, nseg - 1) * stride];   // in bounds
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (g0 + u < nseg) s += v[u];
+    }
+    return s;
+}
+
 template <int K>
 __device__ __forceinline__ double sum_parts(const SolveArgs& a, int c, int i) {
     const size_t np = a.n_pad;
-    double s = 0.0;
     const size_t col = (size_t)(i - a.part_off);
-    for (int g = 0; g < a.nseg; ++g) s += a.part[((size_t)g * a.PT + c) * np + col];
-    return s;
+    return seg_sum(a.part + (size_t)c * np + col, (size_t)a.PT * np, a.nseg);
 }
 
 // Cholesky A = L L^T in packed lower storage.  Returns false on a pivot that
@@ -276,8 +290,7 @@ __device__ __forceinline__ void qdiag_one(const QDiagJob& J, int j) {
 #pragma unroll
         for (int b = a; b < K; ++b) {
             const int c = Sym<K>::zcol(a, b);
-            double m = 0.0;
-            for (int g = 0; g < J.nseg; ++g) m += J.part[((size_t)g * J.PT + c) * J.n_pad + (j - J.part_off)];
+            const double m = seg_sum(J.part + (size_t)c * J.n_pad + (j - J.part_off), (size_t)J.PT * J.n_pad, J.nseg);
             q += (a == b ? 1.0 : 2.0) * m * x[a] * x[b];
         }
     J.qdiag[j] = 1.0 - 2.0 * J.sii[j] + q;
@@ -403,11 +416,8 @@ __global__ __launch_bounds__(256) void solve_var_kernel(int mode, const double* 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (dnn && !(nearest_weight(kernel, dnn[i], bw_pt ? bw_pt[i] : bw) >= kMixedMinWeight)) atomicOr(fail, 2);
-    double a = 0.0, b = 0.0;
-    for (int g = 0; g < nseg; ++g) {
-        a += part[((size_t)g * PT + 0) * n_pad + i];
-        b += part[((size_t)g * PT + 1) * n_pad + i];
-    }
+    const double a = seg_sum(part + i, (size_t)PT * n_pad, nseg);
+    const double b = seg_sum(part + (size_t)n_pad + i, (size_t)PT * n_pad, nseg);
     (void)colB;
     if (!(a > 0.0)) {
         atomicOr(fail, 1);
