@@ -363,7 +363,28 @@ __device__ __forceinline__ void reduce_body(const double* a, const double* b, co
                                            double* out, double* mx) {
     __shared__ double sa[1024], sb[1024], sc[1024], sm[1024];
     double xa = 0.0, xb = 0.0, xc = 0.0, xm = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    // each thread's elements in order, loaded eight at a time (the plain
+    // loop waits for every load before the next)
+    int i = threadIdx.x;
+    for (; i + 7 * 1024 < n; i += 8 * 1024) {
+        double va[8], vb[8], vc[8], vm[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = i + u * 1024;
+            va[u] = a ? a[j] : 0.0;
+            vb[u] = b ? b[j] : 0.0;
+            vc[u] = c ? c[j] : 0.0;
+            vm[u] = m ? m[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (a) xa += va[u];
+            if (b) xb += vb[u];
+            if (c) xc += vc[u];
+            if (m) xm = fmax(xm, vm[u]);
+        }
+    }
+    for (; i < n; i += 1024) {
         if (a) xa += a[i];
         if (b) xb += b[i];
         if (c) xc += c[i];
