@@ -285,6 +285,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
             for (int s = 0; s < 16; ++s)
 #pragma unroll
                 for (int r = 0; r < R; ++r) tv[s][r] = src[(s * R + r) * 64];
+            // the Z rows straight from memory (cached): no staging, no barrier
+            GWM_GLOBAL const double* zrow = gptr(job.Z) + (size_t)j0 * ldz + (lane & 15);
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int rr = 4 * s + kk;
+                const int j = j0 + rr;
+                double zb[NTA];
+#pragma unroll
+                for (int t = 0; t < NTA; ++t) zb[t] = t < nta ? zrow[(size_t)rr * ldz + 16 * t] : 0.0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    // a slot past the chunk's end may hold anything: it is not weighed
+                    double w = j < jend ? weight_of_d2(job.kernel, tv[s][r], -1.0, fbw[r]) : 0.0;
+                    if (j >= jend) w = 0.0;
+                    if (MODE == kModeCV && j == fidx[r]) w = 0.0;
+#pragma unroll
+                    for (int t = 0; t < NTA; ++t)
+                        if (t < nta) accA[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(w, zb[t], accA[r][t], 0, 0, 0);
+                }
+            }
+            continue;
         }
         __syncthreads();
         // stage Z rows [j0, j0 + kChunk) (zero padded beyond n in memory)
@@ -319,28 +340,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                 for (int r = 0; r < R; ++r) {
                     double d;
                     tb_out[(s * R + r) * 64] = pair_d2<DIST>(fc[r], sh, rr, &d);
-                }
-            }
-            continue;
-        }
-        if constexpr (TAB == 2) {
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                // all 16 steps: the rows past a partial last chunk have j >= jend
-                const int rr = 4 * s + kk;
-                const int j = j0 + rr;
-                double zb[NTA];
-#pragma unroll
-                for (int t = 0; t < NTA; ++t) zb[t] = t < nta ? sh.z[rr][16 * t + (lane & 15)] : 0.0;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    // a slot past the chunk's end may hold anything: it is not weighed
-                    double w = j < jend ? weight_of_d2(job.kernel, tv[s][r], -1.0, fbw[r]) : 0.0;
-                    if (j >= jend) w = 0.0;
-                    if (MODE == kModeCV && j == fidx[r]) w = 0.0;
-#pragma unroll
-                    for (int t = 0; t < NTA; ++t)
-                        if (t < nta) accA[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(w, zb[t], accA[r][t], 0, 0, 0);
                 }
             }
             continue;
