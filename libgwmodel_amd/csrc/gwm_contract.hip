@@ -132,7 +132,8 @@ __device__ __forceinline__ double pair_weight(int kernel, const FocusCoords<DIST
 // chunks within reach of job.bw, 1 writes their squared distances, 2 sweeps
 // at job.bw <= job.tab_bw reading them (the chunks within reach of tab_bw
 // keep their block index whether or not job.bw reaches them).
-template <int NTA, int NTB, int DIST, int MODE, int R, int TAB = 0>
+// KER: the kernel id as a constant (stage 2 of Gaussian sweeps), or -1 for the job's own.
+template <int NTA, int NTB, int DIST, int MODE, int R, int TAB = 0, int KER = -1>
 // Two waves per SIMD: the register budget is 256 per lane.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void contract_f64_kernel(SweepLaunch L) {
     __shared__ SharedTile<NTA, DIST, MODE> sh;
@@ -159,6 +160,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
     }
     // a launch groups jobs of exactly the instance's tile counts
     constexpr int nta = NTA, ntb = NTB;
+    const int kernel = KER >= 0 ? KER : job.kernel;
     const GPoints P = global_points(job.P);
     GWM_GLOBAL const double* bw_pt = gptr(job.bw_pt);
     const int ldz = nta * 16;
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
         const double b = (MODE != kModeQDiag && bw_pt) ? bw_pt[fl] : job.bw;
         if (dnn) {
             const double dn = dnn[fl];
-            fbw[r] = make_kconst64_nn(b, job.kernel, dn);
+            fbw[r] = make_kconst64_nn(b, kernel, dn);
             fdn = fmax(fdn, dn);
         } else {
             fbw[r] = make_kconst64(b);
@@ -219,12 +221,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
         }
         workgroup_box<BD>(flo, fhi, fb, &fb, fdn, &fdn);
         // qDiag: the streamed points' bandwidths (per chunk when adaptive)
-        if (MODE != kModeQDiag) thr = prune_threshold<DIST>(zero_weight_distance(job.kernel, fb, false, fdn));
-        else if (!job.chunk_bw) thr = prune_threshold<DIST>(zero_weight_distance(job.kernel, job.bw, false));
+        if (MODE != kModeQDiag) thr = prune_threshold<DIST>(zero_weight_distance(kernel, fb, false, fdn));
+        else if (!job.chunk_bw) thr = prune_threshold<DIST>(zero_weight_distance(kernel, job.bw, false));
     }
     // the table's reach (stage 2; the same expression as the stages that built it)
     double thr_tab = thr;
-    if constexpr (TAB == 2) thr_tab = prune_threshold<DIST>(zero_weight_distance(job.kernel, job.tab_bw, false, fdn));
+    if constexpr (TAB == 2) thr_tab = prune_threshold<DIST>(zero_weight_distance(kernel, job.tab_bw, false, fdn));
     const int item = tile * job.nseg + seg;
     constexpr long long kBlock = (long long)kWaves * 16 * R * 64;   // doubles per table block
     long long tb = 0;                                             // the item's next table block
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
             const int ch = j0 / kChunk;
             double t = thr;
             if (MODE == kModeQDiag && job.chunk_bw)
-                t = prune_threshold<DIST>(zero_weight_distance(job.kernel, gptr(job.chunk_bw)[ch], false));
+                t = prune_threshold<DIST>(zero_weight_distance(kernel, gptr(job.chunk_bw)[ch], false));
             const double gap = box_gap2<DIST>(flo, fhi, gptr(job.boxes) + (size_t)ch * 2 * BD);
             if constexpr (TAB == 2) {
                 if (__builtin_amdgcn_readfirstlane(gap > thr_tab)) continue;
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     // a slot past the chunk's end may hold anything: it is not weighed
-                    double w = j < jend ? weight_of_d2(job.kernel, tv[s][r], -1.0, fbw[r]) : 0.0;
+                    double w = j < jend ? weight_of_d2(kernel, tv[s][r], -1.0, fbw[r]) : 0.0;
                     if (j >= jend) w = 0.0;
                     if (MODE == kModeCV && j == fidx[r]) w = 0.0;
 #pragma unroll
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 // qDiag with a fixed bandwidth: fbw holds the same constants as sh.kc
-                double w = pair_weight<DIST>(job.kernel, fc[r], sh, rr,
+                double w = pair_weight<DIST>(kernel, fc[r], sh, rr,
                                              (MODE == kModeQDiag && bw_pt) ? sh.kc[rr] : fbw[r]);
                 if (j >= jend) w = 0.0;
                 if (MODE == kModeCV && j == fidx[r]) w = 0.0;
@@ -453,7 +455,11 @@ hipError_t launch_f64_tab(const SweepLaunch& a, hipStream_t s) {
     if (a.order && a.norder != grid) return hipErrorInvalidValue;
     SweepLaunch b = a;
     b.tile_stride = a.linear_order ? 1 : tile_stride(tiles);
-    hipLaunchKernelGGL((contract_f64_kernel<NTA, 0, kGreatCircle, MODE, R, TAB>), dim3(grid), dim3(256), 0, s, b);
+    if (TAB == 2 && a.job0.kernel == kGaussian)
+        hipLaunchKernelGGL((contract_f64_kernel<NTA, 0, kGreatCircle, MODE, R, TAB, TAB == 2 ? kGaussian : -1>),
+                           dim3(grid), dim3(256), 0, s, b);
+    else
+        hipLaunchKernelGGL((contract_f64_kernel<NTA, 0, kGreatCircle, MODE, R, TAB>), dim3(grid), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
